@@ -1,0 +1,313 @@
+"""TEST INFRASTRUCTURE: ctypes access to the CPU oracle and the reference build.
+
+* :class:`OracleBody` drives ``oracle/liboracle.so`` -- the CPU restatement
+  of the reference step (``cpu_ref.hpp``), FP32 or FP64.
+* :class:`RefBody` drives ``oracle/_ref/libb3ref.so`` -- the reference's own
+  sources compiled by ``oracle/Makefile`` (present only where the reference
+  tree was available at build time).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU baseline import
+this package; the product never does.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libb3ref.so")
+
+u32p = C.POINTER(C.c_uint32)
+f32p = C.POINTER(C.c_float)
+u8p = C.POINTER(C.c_uint8)
+
+
+class B3sScene(C.Structure):
+    _fields_ = [
+        ("n_particles", C.c_uint32), ("p_type", u8p), ("p_pos", f32p), ("p_vel", f32p), ("p_damping", f32p),
+        ("n_spheres", C.c_uint32), ("s_particle", u32p), ("s_radius", f32p), ("s_friction", f32p),
+        ("n_trifix", C.c_uint32), ("tf_p", u32p), ("tf_v", f32p), ("tf_density", f32p), ("tf_radius", f32p),
+        ("tf_friction", f32p),
+        ("n_tetfix", C.c_uint32), ("ttf_p", u32p), ("ttf_v", f32p), ("ttf_density", f32p), ("ttf_radius", f32p),
+        ("ttf_friction", f32p),
+        ("n_forces", C.c_uint32), ("f_type", u32p), ("f_index", u32p),
+        ("n_tet", C.c_uint32), ("tet_p", u32p), ("tet_v", f32p), ("tet_param", f32p),
+        ("n_tri", C.c_uint32), ("tri_p", u32p), ("tri_v", f32p), ("tri_param", f32p),
+        ("n_stretch", C.c_uint32), ("st_p", u32p), ("st_param", f32p),
+        ("n_shear", C.c_uint32), ("sh_p", u32p), ("sh_param", f32p),
+        ("n_spring", C.c_uint32), ("sp_p", u32p), ("sp_param", f32p),
+        ("n_mouse", C.c_uint32), ("mo_p", u32p), ("mo_param", f32p),
+        ("n_world", C.c_uint32), ("w_type", u32p), ("w_param", f32p), ("w_friction", f32p),
+        ("w_voxels", C.POINTER(f32p)),
+        ("gravity", C.c_float * 3),
+    ]
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None and a.size else C.cast(None, t)
+
+
+def make_scene(arr: dict):
+    """Builds a ``b3s_scene`` from :meth:`SceneBuilder.arrays`; returns (struct, keepalive)."""
+    keep = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v) for k, v in arr.items()}
+    s = B3sScene()
+    s.n_particles = len(keep["p_type"])
+    s.p_type, s.p_pos, s.p_vel, s.p_damping = (_ptr(keep["p_type"], u8p), _ptr(keep["p_pos"], f32p),
+                                               _ptr(keep["p_vel"], f32p), _ptr(keep["p_damping"], f32p))
+    s.n_spheres = len(keep["s_particle"])
+    s.s_particle, s.s_radius, s.s_friction = (_ptr(keep["s_particle"], u32p), _ptr(keep["s_radius"], f32p),
+                                              _ptr(keep["s_friction"], f32p))
+    s.n_trifix = len(keep["tf_density"])
+    for k, t in (("tf_p", u32p), ("tf_v", f32p), ("tf_density", f32p), ("tf_radius", f32p), ("tf_friction", f32p)):
+        setattr(s, k, _ptr(keep[k], t))
+    s.n_tetfix = len(keep["ttf_density"])
+    for k, t in (("ttf_p", u32p), ("ttf_v", f32p), ("ttf_density", f32p), ("ttf_radius", f32p), ("ttf_friction", f32p)):
+        setattr(s, k, _ptr(keep[k], t))
+    s.n_forces = len(keep["f_type"])
+    s.f_type, s.f_index = _ptr(keep["f_type"], u32p), _ptr(keep["f_index"], u32p)
+    s.n_tet = len(keep["tet_param"]) // 3
+    s.tet_p, s.tet_v, s.tet_param = _ptr(keep["tet_p"], u32p), _ptr(keep["tet_v"], f32p), _ptr(keep["tet_param"], f32p)
+    s.n_tri = len(keep["tri_param"]) // 6
+    s.tri_p, s.tri_v, s.tri_param = _ptr(keep["tri_p"], u32p), _ptr(keep["tri_v"], f32p), _ptr(keep["tri_param"], f32p)
+    s.n_stretch = len(keep["st_param"]) // 12
+    s.st_p, s.st_param = _ptr(keep["st_p"], u32p), _ptr(keep["st_param"], f32p)
+    s.n_shear = len(keep["sh_param"]) // 8
+    s.sh_p, s.sh_param = _ptr(keep["sh_p"], u32p), _ptr(keep["sh_param"], f32p)
+    s.n_spring = len(keep["sp_param"]) // 3
+    s.sp_p, s.sp_param = _ptr(keep["sp_p"], u32p), _ptr(keep["sp_param"], f32p)
+    s.n_mouse = len(keep["mo_param"]) // 6
+    s.mo_p, s.mo_param = _ptr(keep["mo_p"], u32p), _ptr(keep["mo_param"], f32p)
+    s.n_world = len(keep["w_type"])
+    s.w_type, s.w_param, s.w_friction = (_ptr(keep["w_type"], u32p), _ptr(keep["w_param"], f32p),
+                                         _ptr(keep["w_friction"], f32p))
+    vox = keep["w_voxels"]
+    vox_arr = (f32p * max(1, len(vox)))(*[(_ptr(v, f32p) if v is not None else C.cast(None, f32p)) for v in vox])
+    keep["_vox_arr"] = vox_arr
+    s.w_voxels = C.cast(vox_arr, C.POINTER(f32p))
+    s.gravity = (C.c_float * 3)(*[float(g) for g in keep["gravity"]])
+    return s, keep
+
+
+def build(reference: bool | None = None) -> None:
+    """Builds liboracle.so (and _ref/ when the reference tree exists)."""
+    targets = ["all"]
+    if reference or (reference is None and os.path.isdir("/root/reference/src")):
+        targets.append("ref")
+    subprocess.check_call(["make", "-s", "-C", HERE, "-j8"] + targets)
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise FileNotFoundError(path)
+    return C.CDLL(path)
+
+
+_libs = {}
+
+
+def _lib(which: str):
+    if which not in _libs:
+        raw = _load(ORACLE_SO if which == "oracle" else REF_SO)
+
+        class _Named:   # CDLL.__getitem__ builds a new function object per call; attribute access caches
+            def __getitem__(self, name):
+                return getattr(raw, name)
+
+            def __getattr__(self, name):
+                return getattr(raw, name)
+
+        lib = _Named()
+        p = "b3o_" if which == "oracle" else "refh_"
+        lib[p + "create"].restype = C.c_void_p
+        lib[p + "create"].argtypes = [C.POINTER(B3sScene), C.c_int]
+        lib[p + "destroy"].argtypes = [C.c_void_p]
+        lib[p + "get_state"].argtypes = [C.c_void_p, f32p, f32p]
+        lib[p + "set_state"].argtypes = [C.c_void_p, f32p, f32p]
+        lib[p + "apply_external"].argtypes = [C.c_void_p, f32p, f32p]
+        lib[p + "get_masses"].argtypes = [C.c_void_p, f32p]
+        lib[p + "get_stats"].argtypes = [C.c_void_p, u32p]
+        lib[p + "get_energy"].argtypes = [C.c_void_p]
+        lib[p + "get_energy"].restype = C.c_double
+        lib[p + "get_contacts"].argtypes = [C.c_void_p, u32p, u32p, f32p, C.c_uint32]
+        lib[p + "get_contacts"].restype = C.c_uint32
+        lib[p + "get_tet_quats"].argtypes = [C.c_void_p, f32p]
+        lib[p + "assemble"].argtypes = [C.c_void_p, C.c_float]
+        lib[p + "assemble"].restype = C.c_uint32
+        lib[p + "get_assembled"].argtypes = [C.c_void_p, u32p, u32p, f32p, f32p, f32p]
+        lib[p + "uv_initialize"].argtypes = [f32p, f32p, f32p, f32p]
+        lib[p + "solve_cg"].argtypes = [C.c_uint32, u32p, u32p, f32p, f32p, f32p, C.c_uint32, C.c_float]
+        lib[p + "solve_cg"].restype = C.c_uint32
+        lib[p + "step_timed"].argtypes = [C.c_void_p, C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, C.c_float, C.c_float]
+        lib[p + "step_timed"].restype = C.c_double
+        if which == "oracle":
+            lib.b3o_step.argtypes = [C.c_void_p, C.c_float, C.c_uint32, C.c_uint32, C.c_float, C.c_float]
+            lib.b3o_get_state64.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+            lib.b3o_spmv.argtypes = [C.c_uint32, u32p, u32p, f32p, f32p, f32p]
+        else:
+            lib.refh_step.argtypes = [C.c_void_p, C.c_float, C.c_uint32, C.c_uint32]
+            lib.refh_step_tol.argtypes = [C.c_void_p, C.c_float, C.c_uint32, C.c_uint32, C.c_float, C.c_float]
+            lib.refh_reset_stats.argtypes = []
+        _libs[which] = lib
+    return _libs[which]
+
+
+def reference_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+class _Body:
+    prefix = ""
+    which = ""
+
+    def __init__(self, arrays: dict, **kw):
+        self.lib = _lib(self.which)
+        self.n = len(arrays["p_type"])
+        self.n_tet = len(arrays["tet_param"]) // 3
+        self._scene, self._keep = make_scene(arrays)
+        self.h = self._create(**kw)
+
+    def _f(self, name):
+        return self.lib[self.prefix + name]
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self._f("destroy")(h)
+            self.h = None
+
+    def state(self):
+        x = np.zeros(3 * self.n, np.float32)
+        v = np.zeros(3 * self.n, np.float32)
+        self._f("get_state")(self.h, _ptr(x, f32p), _ptr(v, f32p))
+        return x.reshape(-1, 3), v.reshape(-1, 3)
+
+    def set_state(self, x=None, v=None):
+        xa = None if x is None else np.ascontiguousarray(x, np.float32).reshape(-1)
+        va = None if v is None else np.ascontiguousarray(v, np.float32).reshape(-1)
+        self._f("set_state")(self.h, _ptr(xa, f32p) if xa is not None else None, _ptr(va, f32p) if va is not None else None)
+
+    def apply_external(self, force=None, translation=None):
+        fa = None if force is None else np.ascontiguousarray(force, np.float32).reshape(-1)
+        ta = None if translation is None else np.ascontiguousarray(translation, np.float32).reshape(-1)
+        self._f("apply_external")(self.h, _ptr(fa, f32p) if fa is not None else None,
+                                  _ptr(ta, f32p) if ta is not None else None)
+
+    def masses(self):
+        m = np.zeros(self.n, np.float32)
+        self._f("get_masses")(self.h, _ptr(m, f32p))
+        return m
+
+    def stats(self):
+        s = np.zeros(3, np.uint32)
+        self._f("get_stats")(self.h, _ptr(s, u32p))
+        return s
+
+    def energy(self):
+        return self._f("get_energy")(self.h)
+
+    def contacts(self):
+        cap = 1 << 20
+        p = np.zeros(cap, np.uint32)
+        w = np.zeros(cap, np.uint32)
+        f = np.zeros(cap, np.float32)
+        k = self._f("get_contacts")(self.h, _ptr(p, u32p), _ptr(w, u32p), _ptr(f, f32p), cap)
+        return p[:k].copy(), w[:k].copy(), f[:k].copy()
+
+    def tet_quats(self):
+        q = np.zeros(4 * max(1, self.n_tet), np.float32)
+        self._f("get_tet_quats")(self.h, _ptr(q, f32p))
+        return q[: 4 * self.n_tet].reshape(-1, 4)
+
+    def assemble(self, dt):
+        """pA (CSR, list order, head-to-tail rows), pb and fi of the first linearisation."""
+        nnz = self._f("assemble")(self.h, C.c_float(dt))
+        rowptr = np.zeros(self.n + 1, np.uint32)
+        cols = np.zeros(max(1, nnz), np.uint32)
+        vals = np.zeros(9 * max(1, nnz), np.float32)
+        pb = np.zeros(3 * self.n, np.float32)
+        fi = np.zeros(3 * self.n, np.float32)
+        self._f("get_assembled")(self.h, _ptr(rowptr, u32p), _ptr(cols, u32p), _ptr(vals, f32p), _ptr(pb, f32p),
+                                 _ptr(fi, f32p))
+        return rowptr, cols[:nnz], vals[: 9 * nnz].reshape(-1, 9), pb.reshape(-1, 3), fi.reshape(-1, 3)
+
+
+class OracleBody(_Body):
+    """The CPU restatement (``precision`` 32 or 64)."""
+
+    prefix = "b3o_"
+    which = "oracle"
+
+    def _create(self, precision=32):
+        self.precision = precision
+        return self.lib.b3o_create(C.byref(self._scene), precision)
+
+    def step(self, dt, iters=1, sub_iters=50, tol=0.0, sub_tol=0.0):
+        self.lib.b3o_step(self.h, C.c_float(dt), iters, sub_iters, C.c_float(tol), C.c_float(sub_tol))
+
+    def step_timed(self, steps, dt, iters=1, sub_iters=50, tol=0.0, sub_tol=0.0):
+        return self.lib.b3o_step_timed(self.h, steps, C.c_float(dt), iters, sub_iters, C.c_float(tol), C.c_float(sub_tol))
+
+    def state64(self):
+        x = np.zeros(3 * self.n, np.float64)
+        v = np.zeros(3 * self.n, np.float64)
+        self.lib.b3o_get_state64(self.h, x.ctypes.data_as(C.POINTER(C.c_double)), v.ctypes.data_as(C.POINTER(C.c_double)))
+        return x.reshape(-1, 3), v.reshape(-1, 3)
+
+
+class RefBody(_Body):
+    """The reference library itself (``fast`` = O(N) fixture set-up)."""
+
+    prefix = "refh_"
+    which = "ref"
+
+    def _create(self, fast=False):
+        self.lib.refh_reset_stats()
+        return self.lib.refh_create(C.byref(self._scene), 1 if fast else 0)
+
+    def step(self, dt, iters=1, sub_iters=50, tol=0.0, sub_tol=0.0):
+        if tol > 0 or sub_tol > 0:
+            self.lib.refh_step_tol(self.h, C.c_float(dt), iters, sub_iters, C.c_float(tol), C.c_float(sub_tol))
+        else:
+            self.lib.refh_step(self.h, C.c_float(dt), iters, sub_iters)
+
+    def step_timed(self, steps, dt, iters=1, sub_iters=50, tol=0.0, sub_tol=0.0):
+        return self.lib.refh_step_timed(self.h, steps, C.c_float(dt), iters, sub_iters, C.c_float(tol), C.c_float(sub_tol))
+
+
+def uv_initialize(which: str, A, B, C_):
+    lib = _lib(which)
+    out = np.zeros(6, np.float32)
+    a, b, c = (np.ascontiguousarray(x, np.float32) for x in (A, B, C_))
+    lib[("b3o_" if which == "oracle" else "refh_") + "uv_initialize"](_ptr(a, f32p), _ptr(b, f32p), _ptr(c, f32p),
+                                                                     _ptr(out, f32p))
+    return out
+
+
+def solve_cg(which: str, rowptr, cols, vals, b, x0, max_it, eps):
+    lib = _lib(which)
+    n = len(rowptr) - 1
+    rowptr = np.ascontiguousarray(rowptr, np.uint32)
+    cols = np.ascontiguousarray(cols, np.uint32)
+    vals = np.ascontiguousarray(vals, np.float32).reshape(-1)
+    b = np.ascontiguousarray(b, np.float32).reshape(-1)
+    x = np.ascontiguousarray(x0, np.float32).reshape(-1).copy()
+    it = lib[("b3o_" if which == "oracle" else "refh_") + "solve_cg"](
+        n, _ptr(rowptr, u32p), _ptr(cols, u32p), _ptr(vals, f32p), _ptr(b, f32p), _ptr(x, f32p), max_it, C.c_float(eps))
+    return x.reshape(-1, 3), it
+
+
+def spmv(rowptr, cols, vals, x):
+    lib = _lib("oracle")
+    n = len(rowptr) - 1
+    y = np.zeros(3 * n, np.float32)
+    rowptr = np.ascontiguousarray(rowptr, np.uint32)
+    cols = np.ascontiguousarray(cols, np.uint32)
+    vals = np.ascontiguousarray(vals, np.float32).reshape(-1)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1)
+    lib.b3o_spmv(n, _ptr(rowptr, u32p), _ptr(cols, u32p), _ptr(vals, f32p), _ptr(x, f32p), _ptr(y, f32p))
+    return y.reshape(-1, 3)

@@ -1,0 +1,272 @@
+/*
+ * TEST INFRASTRUCTURE (oracle). C entry points over the CPU restatement
+ * (cpu_ref.hpp), loaded by tests/, smoke() and bench.py's CPU baseline leg.
+ * Arrays crossing this boundary are in creation order unless stated.
+ */
+#include "cpu_ref.hpp"
+
+#include <chrono>
+#include <cstring>
+#include <memory>
+
+using namespace b3o;
+
+namespace {
+
+struct Handle {
+	int precision;
+	std::unique_ptr<Body<float>> f;
+	std::unique_ptr<Body<double>> d;
+	// last assembly capture (list order)
+	SpMat<float> capA{0};
+	DVec<float> capB, capF;
+	SpMat<double> capA64{0};
+	DVec<double> capB64, capF64;
+	// creation index of each tet / list index
+	std::vector<uint32_t> tetCreationToList;
+};
+
+template <class T, class O> void GetState(Body<T>& b, O* x, O* v) {
+	const uint32_t n = (uint32_t)b.particles.size();
+	for (uint32_t c = 0; c < n; ++c) {
+		const Particle<T>& p = b.particles[n - 1 - c];
+		if (x) { x[3 * c] = O(p.x.x); x[3 * c + 1] = O(p.x.y); x[3 * c + 2] = O(p.x.z); }
+		if (v) { v[3 * c] = O(p.v.x); v[3 * c + 1] = O(p.v.y); v[3 * c + 2] = O(p.v.z); }
+	}
+}
+
+} // namespace
+
+extern "C" {
+
+void* b3o_create(const b3s_scene* sc, int precision) {
+	Handle* h = new Handle();
+	h->precision = precision;
+	if (precision == 64) { h->d.reset(new Body<double>()); h->d->Build(*sc); }
+	else { h->f.reset(new Body<float>()); h->f->Build(*sc); }
+	// list index of each tet created k-th: forces are listed in reverse creation
+	uint32_t nt = 0;
+	for (uint32_t c = 0; c < sc->n_forces; ++c) nt += sc->f_type[c] == B3S_FORCE_TET_ELEMENT;
+	h->tetCreationToList.resize(nt);
+	for (uint32_t k = 0; k < nt; ++k) h->tetCreationToList[k] = nt - 1 - k;
+	return h;
+}
+
+void b3o_destroy(void* p) { delete (Handle*)p; }
+
+void b3o_step(void* p, float dt, uint32_t it, uint32_t sub, float tol, float subTol) {
+	Handle* h = (Handle*)p;
+	if (h->precision == 64) h->d->Step(double(dt), it, sub, double(tol), double(subTol));
+	else h->f->Step(dt, it, sub, tol, subTol);
+}
+
+/* Steps `steps` times and returns the wall time in seconds of those steps. */
+double b3o_step_timed(void* p, uint32_t steps, float dt, uint32_t it, uint32_t sub, float tol, float subTol) {
+	auto t0 = std::chrono::steady_clock::now();
+	for (uint32_t s = 0; s < steps; ++s) b3o_step(p, dt, it, sub, tol, subTol);
+	auto t1 = std::chrono::steady_clock::now();
+	return std::chrono::duration<double>(t1 - t0).count();
+}
+
+void b3o_get_state(void* p, float* x, float* v) {
+	Handle* h = (Handle*)p;
+	if (h->precision == 64) GetState(*h->d, x, v);
+	else GetState(*h->f, x, v);
+}
+
+void b3o_get_state64(void* p, double* x, double* v) {
+	Handle* h = (Handle*)p;
+	if (h->precision == 64) GetState(*h->d, x, v);
+	else GetState(*h->f, x, v);
+}
+
+/* Overwrites positions/velocities (creation order). */
+void b3o_set_state(void* p, const float* x, const float* v) {
+	Handle* h = (Handle*)p;
+	auto set = [&](auto& b) {
+		const uint32_t n = (uint32_t)b.particles.size();
+		for (uint32_t c = 0; c < n; ++c) {
+			auto& q = b.particles[n - 1 - c];
+			using T = decltype(q.mass);
+			if (x) q.x = MakeV3(T(x[3 * c]), T(x[3 * c + 1]), T(x[3 * c + 2]));
+			if (v) q.v = MakeV3(T(v[3 * c]), T(v[3 * c + 1]), T(v[3 * c + 2]));
+		}
+	};
+	if (h->precision == 64) set(*h->d);
+	else set(*h->f);
+}
+
+/* External force / translation accumulators (b3Particle::ApplyForce / ApplyTranslation), creation order. */
+void b3o_apply_external(void* p, const float* force, const float* translation) {
+	Handle* h = (Handle*)p;
+	auto app = [&](auto& b) {
+		const uint32_t n = (uint32_t)b.particles.size();
+		for (uint32_t c = 0; c < n; ++c) {
+			auto& q = b.particles[n - 1 - c];
+			using T = decltype(q.mass);
+			if (force && q.type == kDynamic) q.force += MakeV3(T(force[3 * c]), T(force[3 * c + 1]), T(force[3 * c + 2]));
+			if (translation) q.translation += MakeV3(T(translation[3 * c]), T(translation[3 * c + 1]), T(translation[3 * c + 2]));
+		}
+	};
+	if (h->precision == 64) app(*h->d);
+	else app(*h->f);
+}
+
+void b3o_get_masses(void* p, float* m) {
+	Handle* h = (Handle*)p;
+	if (h->precision == 64) {
+		const uint32_t n = (uint32_t)h->d->particles.size();
+		for (uint32_t c = 0; c < n; ++c) m[c] = float(h->d->particles[n - 1 - c].mass);
+	} else {
+		const uint32_t n = (uint32_t)h->f->particles.size();
+		for (uint32_t c = 0; c < n; ++c) m[c] = h->f->particles[n - 1 - c].mass;
+	}
+}
+
+void b3o_get_stats(void* p, uint32_t* out3) {
+	Handle* h = (Handle*)p;
+	StepStats s = h->precision == 64 ? h->d->stats : h->f->stats;
+	out3[0] = s.newtonIterations;
+	out3[1] = s.minSub;
+	out3[2] = s.maxSub;
+}
+
+double b3o_get_energy(void* p) {
+	Handle* h = (Handle*)p;
+	return h->precision == 64 ? h->d->GetEnergy() : double(h->f->GetEnergy());
+}
+
+/* Contact list in list order: creation index of the sphere fixture's particle and of the world fixture. */
+uint32_t b3o_get_contacts(void* p, uint32_t* particle, uint32_t* world, float* normal_force, uint32_t cap) {
+	Handle* h = (Handle*)p;
+	auto get = [&](auto& b) {
+		uint32_t k = 0;
+		const uint32_t n = (uint32_t)b.particles.size();
+		const uint32_t nw = (uint32_t)b.worlds.size();
+		for (const auto& c : b.contacts) {
+			if (k < cap) {
+				particle[k] = n - 1 - b.spheres[c.sphere].p;
+				world[k] = nw - 1 - c.world;
+				if (normal_force) normal_force[k] = float(c.normalForce);
+			}
+			++k;
+		}
+		return k;
+	};
+	return h->precision == 64 ? get(*h->d) : get(*h->f);
+}
+
+/* Warm-start quaternions (x y z s) of the tetrahedron elements, creation order. */
+void b3o_get_tet_quats(void* p, float* q) {
+	Handle* h = (Handle*)p;
+	auto get = [&](auto& b) {
+		const uint32_t nt = (uint32_t)b.tets.size();
+		for (uint32_t k = 0; k < nt; ++k) {
+			const auto& t = b.tets[nt - 1 - k];
+			q[4 * k] = float(t.q.v.x); q[4 * k + 1] = float(t.q.v.y); q[4 * k + 2] = float(t.q.v.z); q[4 * k + 3] = float(t.q.s);
+		}
+	};
+	if (h->precision == 64) get(*h->d);
+	else get(*h->f);
+}
+
+/*
+ * Runs the contact update and the first Newton linearisation of a step on a
+ * copy of the body and keeps pA, pb and the internal force fi (list order).
+ * Returns the number of blocks of pA.
+ */
+uint32_t b3o_assemble(void* p, float dt) {
+	Handle* h = (Handle*)p;
+	uint32_t nnz = 0;
+	if (h->precision == 64) {
+		Body<double> copy = *h->d;
+		copy.capture.want = true;
+		copy.capture.pA = &h->capA64;
+		copy.capture.pb = &h->capB64;
+		copy.capture.fi = &h->capF64;
+		copy.Step(double(dt), 1, 0);
+		for (auto& r : h->capA64.rows) nnz += (uint32_t)r.size();
+	} else {
+		Body<float> copy = *h->f;
+		copy.capture.want = true;
+		copy.capture.pA = &h->capA;
+		copy.capture.pb = &h->capB;
+		copy.capture.fi = &h->capF;
+		copy.Step(dt, 1, 0);
+		for (auto& r : h->capA.rows) nnz += (uint32_t)r.size();
+	}
+	return nnz;
+}
+
+/* pA rows in list order, each row head-to-tail (the reference SpMV order); values column-major. */
+void b3o_get_assembled(void* p, uint32_t* rowptr, uint32_t* cols, float* vals, float* pb, float* fi) {
+	Handle* h = (Handle*)p;
+	auto get = [&](auto& A, auto& b, auto& f) {
+		uint32_t k = 0;
+		rowptr[0] = 0;
+		for (uint32_t i = 0; i < A.n(); ++i) {
+			const auto& r = A.rows[i];
+			for (size_t e = r.size(); e-- > 0;) {
+				cols[k] = r[e].col;
+				const auto* m = &r[e].val.x.x;
+				for (int t = 0; t < 9; ++t) vals[9 * k + t] = float(m[t]);
+				++k;
+			}
+			rowptr[i + 1] = k;
+			for (int t = 0; t < 3; ++t) {
+				pb[3 * i + t] = float(b[i][t]);
+				fi[3 * i + t] = float(f[i][t]);
+			}
+		}
+	};
+	if (h->precision == 64) get(h->capA64, h->capB64, h->capF64);
+	else get(h->capA, h->capB, h->capF);
+}
+
+/* Restated b3StretchForceDef::Initialize (stretch_force.cpp:36-60) for test comparison. */
+void b3o_uv_initialize(const float* A, const float* B, const float* C, float* out6) {
+	UVInitialize<float>(MakeV3(A[0], A[1], A[2]), MakeV3(B[0], B[1], B[2]), MakeV3(C[0], C[1], C[2]), out6);
+}
+
+/*
+ * Standalone restatement of b3SparseSolveCG (sparse_solver.cpp:28-109) on a
+ * matrix given in list-order CSR. x is the warm start on entry and the result
+ * on exit. Returns the iteration count.
+ */
+uint32_t b3o_solve_cg(uint32_t n, const uint32_t* rowptr, const uint32_t* cols, const float* vals, const float* b,
+                      float* x, uint32_t maxIt, float eps) {
+	SpMat<float> A(n);
+	for (uint32_t i = 0; i < n; ++i) {
+		// rows arrive head-to-tail; the vector form keeps the head at back()
+		for (uint32_t e = rowptr[i + 1]; e-- > rowptr[i];) {
+			SpMat<float>::Entry en;
+			en.col = cols[e];
+			std::memcpy(&en.val.x.x, vals + 9 * e, 9 * sizeof(float));
+			A.rows[i].push_back(en);
+		}
+	}
+	DVec<float> bv(n), xv(n);
+	for (uint32_t i = 0; i < n; ++i) {
+		bv[i] = MakeV3(b[3 * i], b[3 * i + 1], b[3 * i + 2]);
+		xv[i] = MakeV3(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+	}
+	uint32_t it = b3o::SolveCGFree<float>(A, bv, maxIt, eps, xv);
+	for (uint32_t i = 0; i < n; ++i) { x[3 * i] = xv[i].x; x[3 * i + 1] = xv[i].y; x[3 * i + 2] = xv[i].z; }
+	return it;
+}
+
+/* Restated SpMV (sparse_mat33.h:494-512) on list-order CSR. */
+void b3o_spmv(uint32_t n, const uint32_t* rowptr, const uint32_t* cols, const float* vals, const float* x, float* y) {
+	for (uint32_t i = 0; i < n; ++i) {
+		V3<float> acc;
+		acc.SetZero();
+		for (uint32_t e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+			M33<float> a;
+			std::memcpy(&a.x.x, vals + 9 * e, 9 * sizeof(float));
+			acc += a * MakeV3(x[3 * cols[e]], x[3 * cols[e] + 1], x[3 * cols[e] + 2]);
+		}
+		y[3 * i] = acc.x; y[3 * i + 1] = acc.y; y[3 * i + 2] = acc.z;
+	}
+}
+
+} // extern "C"
