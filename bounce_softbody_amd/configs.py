@@ -1,0 +1,199 @@
+"""Workload definitions: the benchmark configurations and the parity scenes.
+
+Benchmark configurations (sizes as stated by the workloads):
+
+* :func:`tet_grid` -- N x N x N hex grid of 0.1 m cells split into 5 tets per
+  hex, rest positions jittered U(-0.005, 0.005) m, E = 1e5 Pa, nu = 0.3,
+  density 1000 kg/m^3, top face (y max) static, gravity -9.8 m/s^2.
+  N = 10 is the CPU-runnable case (1331 particles, 5000 tets), N = 64 the
+  single large body (274,625 particles, 1,310,720 tets).
+* :func:`cloth_on_sphere` -- (R+1)^2-vertex triangle cloth (stretch k = 1e4,
+  shear k = 1e3) with pinned corners, draped over a 0.5 m sphere.
+
+Parity scenes mirror the reference testbed scenes (examples/testbed/tests/*.h)
+at small sizes: contacts against every shape type, particle types, damping,
+springs, mouse force and triangle elements.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from .meshes import GridClothMesh, GridTetMesh, cloth_body, tet_body, splitmix64_np
+from .scene import DYNAMIC, KINEMATIC, STATIC, SceneBuilder
+
+f32 = np.float32
+DT = f32(1.0) / f32(60.0)
+
+
+def tet_grid(N: int = 10, *, seed: int = 1234, young=1e5, poisson=0.3, density=1000.0, cell=0.1,
+             jitter=0.005, young_per_element=None) -> SceneBuilder:
+    m = GridTetMesh(N, N, N).scale(cell).jitter(jitter, seed)
+    b = tet_body(m, density=density, young=young if young_per_element is None else young_per_element,
+                 poisson=poisson)
+    top = m.vertex(N, *np.meshgrid(np.arange(N + 1), np.arange(N + 1), indexing="ij")).reshape(-1)
+    b.set_particle_types(top, STATIC)
+    b.gravity[:] = (0.0, -9.8, 0.0)
+    b.mesh = m
+    return b
+
+
+def log_uniform_young(N: int, seed: int, lo=1e4, hi=1e7) -> np.ndarray:
+    """Per-element E = 10^(log10(lo) + u (log10(hi) - log10(lo))), 5 N^3 elements."""
+    u = splitmix64_np(seed ^ 0x5DEECE66D, 5 * N ** 3)
+    return (10.0 ** (np.log10(lo) + u * (np.log10(hi) - np.log10(lo)))).astype(f32)
+
+
+def cloth_on_sphere(R: int = 16, *, size=2.0, stretch=1e4, shear=1e3, density=0.1, radius=0.0, friction=0.3,
+                    sphere_radius=0.5, height=0.55, world_friction=0.5) -> SceneBuilder:
+    m = GridClothMesh(R, R).scale((size / R, 1.0, size / R)).translate((0.0, height, 0.0))
+    b = cloth_body(m, density=density, radius=radius, friction=friction, stretching_stiffness=stretch,
+                   shear_stiffness=shear)
+    corners = [m.vertex(0, 0), m.vertex(0, R), m.vertex(R, 0), m.vertex(R, R)]
+    b.set_particle_types(corners, STATIC)
+    b.add_world_sphere((0.0, 0.0, 0.0), sphere_radius, world_friction)
+    b.gravity[:] = (0.0, -9.8, 0.0)
+    b.mesh = m
+    return b
+
+
+# ---------------------------------------------------------------- parity scenes
+def scene_node_types(N: int = 3) -> SceneBuilder:
+    """examples/testbed/tests/node_types.h: tet block, side static, box ground, damping."""
+    m = GridTetMesh(N, N, N).translate((0.0, 5.0, 0.0))
+    b = tet_body(m, density=0.2, young=1000.0, poisson=0.33, radius=0.2, friction=0.6, stiffness_damping=0.02,
+                 damping=0.1)
+    side = [m.vertex(i, 0, k) for i in range(N + 1) for k in range(N + 1)]
+    b.set_particle_types(side[: len(side) // 2], STATIC)
+    b.set_particle_types(side[len(side) // 2:], KINEMATIC)
+    b.add_world_box((20.0, 1.0, 20.0), radius=0.2, friction=0.5)
+    b.gravity[:] = (0.0, -9.8, 0.0)
+    b.mesh = m
+    return b
+
+
+def scene_drop_on_box(N: int = 3) -> SceneBuilder:
+    """A free tet block falling on a rotated box: contact + friction every step."""
+    m = GridTetMesh(N, N, N).scale(0.5).translate((0.1, 1.0, -0.05))
+    b = tet_body(m, density=2.0, young=2000.0, poisson=0.3, radius=0.05, friction=0.6)
+    b.add_world_box((3.0, 0.5, 3.0), translation=(0.0, -0.2, 0.0), rotation=(0.0, 0.0, 0.0998334, 0.9950042),
+                    radius=0.05, friction=0.5)
+    b.gravity[:] = (0.0, -9.8, 0.0)
+    b.mesh = m
+    return b
+
+
+def scene_cloth_shapes(R: int = 8) -> SceneBuilder:
+    """Cloth over a capsule, a triangle and a sphere (capsule/triangle/sphere_contact.h)."""
+    m = GridClothMesh(R, R).scale((2.0 / R, 1.0, 2.0 / R)).translate((0.0, 0.4, 0.0))
+    b = cloth_body(m, density=0.2, radius=0.05, friction=0.4, stretching_stiffness=1000.0, shear_stiffness=100.0,
+                   stretch_damping=1.0, shear_damping=0.5, damping=0.05)
+    b.add_world_capsule((-0.6, 0.0, -1.0), (-0.6, 0.0, 1.0), 0.2, 0.5)
+    b.add_world_triangle((0.2, 0.05, -1.0), (1.2, 0.1, 1.0), (1.2, 0.0, -1.0), radius=0.02, friction=0.5,
+                         e=((0.0, 0.0, -2.0), None, None))
+    b.add_world_sphere((0.0, -0.3, 0.0), 0.4, 0.3)
+    b.gravity[:] = (0.0, -9.8, 0.0)
+    b.mesh = m
+    return b
+
+
+def scene_cloth_element(R: int = 6) -> SceneBuilder:
+    """examples/testbed/tests/cloth_element.h: triangle FEM cloth pinned on one edge."""
+    m = GridClothMesh(R, R)
+    b = cloth_body(m, density=0.2, create_elements=True, element_young=5000.0, element_shear=2000.0,
+                   element_poisson=0.3, element_damping=0.01)
+    b.set_particle_types([m.vertex(0, j) for j in range(R + 1)], STATIC)
+    b.gravity[:] = (0.0, -9.8, 0.0)
+    b.mesh = m
+    return b
+
+
+def scene_springs(R: int = 6) -> SceneBuilder:
+    """Spring cloth (cloth_tearing.h style): tension-only edge springs with damping, corners pinned."""
+    m = GridClothMesh(R, R).scale((0.4, 1.0, 0.4))
+    b = SceneBuilder()
+    b.add_particles(m.vertices)
+    b.add_spheres(np.arange(m.vertex_count), 0.0, 0.3)
+    b.add_triangle_fixtures(m.triangles, m.vertices[m.triangles].reshape(-1, 9), density=0.5, radius=0.0, friction=0.3)
+    edges = set()
+    for t in m.triangles:
+        for a, c in ((t[0], t[1]), (t[1], t[2]), (t[2], t[0])):
+            edges.add((min(a, c), max(a, c)))
+    e = np.array(sorted(edges), np.int64)
+    L = np.linalg.norm((m.vertices[e[:, 0]] - m.vertices[e[:, 1]]).astype(np.float64), axis=1).astype(f32)
+    b.add_springs(e, length=L, ks=500.0, kd=2.0)
+    b.set_particle_types([m.vertex(0, 0), m.vertex(0, R)], STATIC)
+    b.gravity[:] = (0.0, -9.8, 0.0)
+    b.mesh = m
+    return b
+
+
+def scene_mouse(N: int = 2) -> SceneBuilder:
+    """A tet block dragged by a b3MouseForce (body_dragger.cpp:89-101) towards a static particle."""
+    m = GridTetMesh(N, N, N)
+    b = tet_body(m, density=0.2, young=1000.0, poisson=0.3)
+    b.set_particle_types([m.vertex(0, j, k) for j in range(N + 1) for k in range(N + 1)], STATIC)
+    anchor = b.add_particles([[1.5, 2.5, 0.3]], types=[STATIC])[0]
+    t = m.triangles[len(m.triangles) // 2]
+    b.add_mouse([[anchor, t[0], t[1], t[2]]], w2=0.2, w3=0.3, w4=0.5, length=0.0, ks=100.0, kd=1.0)
+    b.gravity[:] = (0.0, -9.8, 0.0)
+    b.mesh = m
+    return b
+
+
+def sphere_union_sdf(res: int, lower, upper, seed: int, count: int = 8, invert=False):
+    """Voxel values of a seeded union of spheres and boxes (signed distance, negative inside)."""
+    lo, hi = np.asarray(lower, np.float64), np.asarray(upper, np.float64)
+    u = splitmix64_np(seed, count * 8).reshape(count, 8)
+    axes = [np.linspace(lo[a], hi[a], res) for a in range(3)]
+    Z, Y, X = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    d = np.full(X.shape, np.inf)
+    span = hi - lo
+    for s in range(count):
+        c = lo + span * (0.2 + 0.6 * u[s, :3])
+        r = 0.08 * span.min() + 0.12 * span.min() * u[s, 3]
+        if u[s, 4] < 0.5:
+            ds = np.sqrt((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) - r
+        else:
+            q = np.stack([np.abs(X - c[0]), np.abs(Y - c[1]), np.abs(Z - c[2])], -1) - r
+            ds = np.linalg.norm(np.maximum(q, 0.0), axis=-1) + np.minimum(q.max(-1), 0.0)
+        d = np.minimum(d, ds)
+    return (-d if invert else d).astype(f32)
+
+
+def scene_sdf(N: int = 3, res: int = 16, invert=False) -> SceneBuilder:
+    """Tet block dropped into an SDF world fixture (sphere_mesh_sdf_contact.h)."""
+    m = GridTetMesh(N, N, N).scale(0.25).translate((0.0, 0.6, 0.0))
+    b = tet_body(m, density=1.0, young=3000.0, poisson=0.3, radius=0.05, friction=0.5)
+    lower, upper = (-1.5, -1.5, -1.5), (1.5, 0.4, 1.5)
+    vox = sphere_union_sdf(res, lower, upper, seed=99, invert=invert)
+    b.add_world_sdf(vox, lower, upper, translation=(0.05, 0.0, -0.02), rotation=(0.0, 0.0499792, 0.0, 0.9987503),
+                    radius=0.02, invert=invert, friction=0.4)
+    b.gravity[:] = (0.0, -9.8, 0.0)
+    b.mesh = m
+    return b
+
+
+def scene_external(N: int = 2) -> SceneBuilder:
+    """Free block for b3Particle::ApplyForce / ApplyTranslation."""
+    m = GridTetMesh(N, N, N)
+    b = tet_body(m, density=0.5, young=800.0, poisson=0.25)
+    b.set_particle_types([m.vertex(0, 0, 0)], KINEMATIC)
+    b.gravity[:] = (0.0, -9.8, 0.0)
+    b.mesh = m
+    return b
+
+
+PARITY_SCENES = {
+    "tet_grid": lambda: tet_grid(4),
+    "node_types": scene_node_types,
+    "drop_on_box": scene_drop_on_box,
+    "cloth_sphere": lambda: cloth_on_sphere(8, size=1.6, height=0.5),
+    "cloth_shapes": scene_cloth_shapes,
+    "cloth_element": scene_cloth_element,
+    "springs": scene_springs,
+    "mouse": scene_mouse,
+    "sdf": scene_sdf,
+    "sdf_inverted": lambda: scene_sdf(invert=True),
+    "external": scene_external,
+}

@@ -1,0 +1,100 @@
+"""Writes the golden fixtures of tests/golden/ from the compiled reference.
+
+The reference library is compiled from its own sources by ``oracle/Makefile``
+into ``oracle/_ref/libb3ref.so`` (``make -C oracle ref``); this script drives
+it through ``oracle/ref_harness.cpp`` on the parity scenes of
+``bounce_softbody_amd.configs`` and records, per scene:
+
+* the step schedule (dt, Newton/PCG iteration caps, tolerances, external loads),
+* per-step SHA-256 digests of the float32 positions and velocities,
+* the final positions, velocities, contact list, masses, tet quaternions and
+  solver statistics,
+* the first linearisation (pA in list order, pb, fi) at the final state.
+
+Usage: python tests/golden/make_golden.py   (needs oracle/_ref built)
+"""
+
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from bounce_softbody_amd import configs  # noqa: E402
+import oracle  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def schedule(name: str, steps: int):
+    """(dt, iters, sub_iters, tol, sub_tol, external) per step."""
+    out = []
+    for s in range(steps):
+        if s % 2:
+            row = (configs.DT, 2, 20, 0.0, 0.0)          # public b3Body::Step
+        else:
+            row = (configs.DT, 1, 30, 1e-3, 1e-5)        # explicit tolerances
+        out.append(row + ((name == "external" and s % 3 == 0),))
+    return out
+
+
+def external_loads(n: int):
+    f = np.zeros((n, 3), np.float32)
+    f[:, 0] = 0.5
+    f[:, 1] = np.arange(n, dtype=np.float32) * np.float32(0.01)
+    t = np.zeros((n, 3), np.float32)
+    t[0] = (0.01, 0.0, 0.0)
+    return f, t
+
+
+def digest(x, v) -> np.ndarray:
+    h = hashlib.sha256(np.ascontiguousarray(x, np.float32).tobytes() + np.ascontiguousarray(v, np.float32).tobytes())
+    return np.frombuffer(h.digest(), np.uint8)
+
+
+def run(name: str, arrays: dict, sched, body_cls=oracle.RefBody, **kw):
+    body = body_cls(arrays, **kw)
+    n = len(arrays["p_type"])
+    digests = []
+    for dt, it, sub, tol, stol, ext in sched:
+        if ext:
+            body.apply_external(*external_loads(n))
+        body.step(dt, it, sub, tol, stol)
+        digests.append(digest(*body.state()))
+    x, v = body.state()
+    cp, cw, cf = body.contacts()
+    masses, quats, stats = body.masses(), body.tet_quats(), body.stats()
+    # last: the reference's ApplyForces writes the tet warm starts
+    rowptr, cols, vals, pb, fi = body.assemble(configs.DT)
+    return {
+        "schedule": np.array([r[:5] for r in sched], np.float64),
+        "external": np.array([r[5] for r in sched], np.bool_),
+        "digests": np.stack(digests),
+        "x": x, "v": v,
+        "contact_particle": cp, "contact_world": cw, "contact_normal_force": cf,
+        "masses": masses, "quats": quats, "stats": stats,
+        "pA_rowptr": rowptr, "pA_cols": cols, "pA_vals": vals, "pb": pb, "fi": fi,
+    }
+
+
+def main():
+    if not oracle.reference_available():
+        sys.exit("oracle/_ref/libb3ref.so is missing: run `make -C oracle ref` where the reference tree exists")
+    for name, fn in configs.PARITY_SCENES.items():
+        rec = run(name, fn().arrays(), schedule(name, 30))
+        np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **rec)
+        print(f"{name}: {len(rec['x'])} particles, {len(rec['contact_particle'])} contacts, stats {rec['stats']}")
+    # The CPU-runnable workload: 10^3 tet grid, 20 steps, PCG cap 50 at tolerance 1e-4.
+    sched = [(configs.DT, 1, 50, 0.0, 1e-4, False)] * 20
+    rec = run("tet_grid_10", configs.tet_grid(10).arrays(), sched)
+    for k in ("pA_rowptr", "pA_cols", "pA_vals", "pb", "fi"):   # keep the fixture small
+        rec.pop(k)
+    np.savez_compressed(os.path.join(OUT, "tet_grid_10.npz"), **rec)
+    print(f"tet_grid_10: stats {rec['stats']}")
+
+
+if __name__ == "__main__":
+    main()
