@@ -1,0 +1,177 @@
+"""Benchmark: simulation steps per second of the single-large-body workload.
+
+Workload (one GPU): a 64 x 64 x 64-cell tet grid (274,625 particles,
+1,310,720 corotational tetrahedra, E = 1e5 Pa, nu = 0.3, 1000 kg/m^3,
+jitter U(-0.005, 0.005) m, static top face) plus a 512 x 512-vertex cloth
+(stretch k = 1e4, shear k = 1e3, pinned corners) draping over a 0.5 m sphere;
+dt = 1/60 s, one Newton iteration, PCG tolerance 1e-5 with at most 50
+iterations. One step advances both bodies (b3Body::Step on each).
+
+Prints one JSON line: steps/s (whole job), the roofline of the dominant
+kernel (the PCG's block-sparse SpMV on the tet body, timed with HIP events on
+its stream over the timed region) and the CPU baseline (the reference library
+compiled from its own sources, oracle/_ref, timed on this host's cores on a
+bounded sample; the oracle's restatement stands in when that build is absent).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+PROFILE_PMC = os.path.join(ROOT, "profiles", "spmv_tet64_pmc.txt")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def traffic_from_profile():
+    """HBM bytes per SpMV launch measured by rocprofv3 --pmc (profiles/spmv_tet64_pmc.txt), if present."""
+    try:
+        for line in open(PROFILE_PMC):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(tet_arr, cloth_arr, dt, sub, sub_tol, sample_steps):
+    """Times `sample_steps` steps of both bodies with the reference build (or the oracle)."""
+    import oracle
+
+    kind = "reference" if oracle.reference_available() else "port"
+    total = 0.0
+    for arr in (tet_arr, cloth_arr):
+        body = oracle.RefBody(arr, fast=True) if kind == "reference" else oracle.OracleBody(arr)
+        total += body.step_timed(sample_steps, dt, 1, sub, 0.0, sub_tol)
+        del body
+    return {
+        "value": round(sample_steps / total, 6),
+        "unit": "steps/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": f"{sample_steps} step(s) of the 64^3 tet body + 512^2 cloth after set-up, single thread, "
+                  f"{total:.1f} s of CPU time",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--tet", type=int, default=64, help="cells per axis of the tet grid")
+    ap.add_argument("--cloth", type=int, default=511, help="cells per side of the cloth (vertices = cells + 1)")
+    ap.add_argument("--sub-iterations", type=int, default=50)
+    ap.add_argument("--sub-tolerance", type=float, default=1e-5)
+    ap.add_argument("--cpu-sample-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this workload is a single body per GPU and the multi-GPU path is not "
+              "part of this build (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from bounce_softbody_amd import configs
+    from bounce_softbody_amd.body import Body
+
+    dt = configs.DT
+    t0 = time.time()
+    tet_arr = configs.tet_grid(args.tet).arrays()
+    cloth_arr = configs.cloth_on_sphere(args.cloth).arrays()
+    log(f"scenes built in {time.time() - t0:.1f} s: tet {len(tet_arr['p_type'])} particles, "
+        f"cloth {len(cloth_arr['p_type'])} particles")
+    t0 = time.time()
+    tet = Body(tet_arr)
+    cloth = Body(cloth_arr)
+    log(f"device bodies built in {time.time() - t0:.1f} s; tet system {tet.system_size()}, cloth {cloth.system_size()}")
+
+    def step():
+        tet.Step(dt, 1, args.sub_iterations, 0.0, args.sub_tolerance)
+        tet.synchronize()   # keep the two bodies' kernels apart so kernel timings are clean
+        cloth.Step(dt, 1, args.sub_iterations, 0.0, args.sub_tolerance)
+
+    for _ in range(args.warmup):
+        step()
+    cloth.synchronize()
+    tet.synchronize()
+    log(f"warmup done ({args.warmup} steps)")
+
+    tet.set_kernel_timing(True)
+    tet.synchronize()
+    cloth.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    tet.synchronize()
+    cloth.synchronize()
+    elapsed = time.perf_counter() - t0
+    launches, mean_ms, bytes_per_launch = tet.spmv_timing()
+    tet.set_kernel_timing(False)
+    stats, _ = tet.stats()
+    cstats, ccontacts = cloth.stats()
+    x, v = tet.state()
+    if not (np.all(np.isfinite(x)) and np.all(np.isfinite(v))):
+        raise SystemExit("non-finite state after the timed steps")
+    value = args.steps / elapsed
+    log(f"timed {args.steps} steps in {elapsed:.3f} s; tet PCG iterations min/max {stats[1]}/{stats[2]}, "
+        f"cloth {cstats[1]}/{cstats[2]}, cloth contacts {ccontacts}; spmv launches {launches} mean {mean_ms*1e3:.1f} us")
+
+    achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9 if mean_ms > 0 else 0.0
+    baseline = None
+    if not args.no_cpu_baseline:
+        log("timing the CPU baseline ...")
+        baseline = cpu_baseline(tet_arr, cloth_arr, dt, args.sub_iterations, args.sub_tolerance, args.cpu_sample_steps)
+        log(f"CPU baseline: {baseline}")
+    out = {
+        "metric": "sim steps/s",
+        "value": round(value, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"tet grid {args.tet}^3 cells ({len(tet_arr['p_type'])} particles, "
+                        f"{len(tet_arr['tet_param']) // 3} tets) + cloth {args.cloth + 1}^2 vertices on a sphere",
+            "dt": "1/60",
+            "newton_iterations": 1,
+            "pcg_max_iterations": args.sub_iterations,
+            "pcg_tolerance": args.sub_tolerance,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "kernel": "k_cg_spmv_dq (tet body)",
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profile(),
+            "bytes_per_launch": bytes_per_launch,
+            "mean_launch_us": round(mean_ms * 1e3, 2),
+        },
+        "cpu_baseline": baseline,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

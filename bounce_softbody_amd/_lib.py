@@ -1,0 +1,205 @@
+"""ctypes binding of libb3g.so (include/b3g.h).
+
+The library is built in-tree by :mod:`bounce_softbody_amd.build`. Loading
+fails loudly when it is missing: there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libb3g.so")
+
+u32p = C.POINTER(C.c_uint32)
+f32p = C.POINTER(C.c_float)
+u8p = C.POINTER(C.c_uint8)
+
+B3G_OK = 0
+STATUS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "out of device memory", 4: "degenerate element",
+          5: "unsupported", 6: "body not built"}
+
+
+class B3gError(RuntimeError):
+    def __init__(self, status: int, where: str, detail: str):
+        super().__init__(f"{where}: {STATUS.get(status, status)}: {detail}")
+        self.status = status
+
+
+class BodyDesc(C.Structure):
+    """``b3g_body_desc`` (include/b3g.h)."""
+
+    _fields_ = [
+        ("n_particles", C.c_uint32), ("p_type", u8p), ("p_pos", f32p), ("p_vel", f32p), ("p_damping", f32p),
+        ("n_spheres", C.c_uint32), ("s_particle", u32p), ("s_radius", f32p), ("s_friction", f32p),
+        ("n_trifix", C.c_uint32), ("tf_p", u32p), ("tf_v", f32p), ("tf_density", f32p), ("tf_radius", f32p),
+        ("tf_friction", f32p),
+        ("n_tetfix", C.c_uint32), ("ttf_p", u32p), ("ttf_v", f32p), ("ttf_density", f32p), ("ttf_radius", f32p),
+        ("ttf_friction", f32p),
+        ("n_forces", C.c_uint32), ("f_type", u32p), ("f_index", u32p),
+        ("n_tet", C.c_uint32), ("tet_p", u32p), ("tet_v", f32p), ("tet_param", f32p),
+        ("n_tri", C.c_uint32), ("tri_p", u32p), ("tri_v", f32p), ("tri_param", f32p),
+        ("n_stretch", C.c_uint32), ("st_p", u32p), ("st_param", f32p),
+        ("n_shear", C.c_uint32), ("sh_p", u32p), ("sh_param", f32p),
+        ("n_spring", C.c_uint32), ("sp_p", u32p), ("sp_param", f32p),
+        ("n_mouse", C.c_uint32), ("mo_p", u32p), ("mo_param", f32p),
+        ("n_world", C.c_uint32), ("w_type", u32p), ("w_param", f32p), ("w_friction", f32p),
+        ("w_voxels", C.POINTER(f32p)),
+        ("gravity", C.c_float * 3),
+    ]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("newton_iterations", C.c_uint32), ("min_sub_iterations", C.c_uint32),
+                ("max_sub_iterations", C.c_uint32), ("contact_count", C.c_uint32)]
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None and a.size else C.cast(None, t)
+
+
+def make_desc(arr: dict):
+    """``b3g_body_desc`` over the arrays of :meth:`SceneBuilder.arrays`; returns (desc, keepalive)."""
+    keep = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v) for k, v in arr.items()}
+    d = BodyDesc()
+    d.n_particles = len(keep["p_type"])
+    d.p_type, d.p_pos, d.p_vel, d.p_damping = (_ptr(keep["p_type"], u8p), _ptr(keep["p_pos"], f32p),
+                                               _ptr(keep["p_vel"], f32p), _ptr(keep["p_damping"], f32p))
+    d.n_spheres = len(keep["s_particle"])
+    d.s_particle, d.s_radius, d.s_friction = (_ptr(keep["s_particle"], u32p), _ptr(keep["s_radius"], f32p),
+                                              _ptr(keep["s_friction"], f32p))
+    d.n_trifix = len(keep["tf_density"])
+    for k, t in (("tf_p", u32p), ("tf_v", f32p), ("tf_density", f32p), ("tf_radius", f32p), ("tf_friction", f32p)):
+        setattr(d, k, _ptr(keep[k], t))
+    d.n_tetfix = len(keep["ttf_density"])
+    for k, t in (("ttf_p", u32p), ("ttf_v", f32p), ("ttf_density", f32p), ("ttf_radius", f32p), ("ttf_friction", f32p)):
+        setattr(d, k, _ptr(keep[k], t))
+    d.n_forces = len(keep["f_type"])
+    d.f_type, d.f_index = _ptr(keep["f_type"], u32p), _ptr(keep["f_index"], u32p)
+    d.n_tet = len(keep["tet_param"]) // 3
+    d.tet_p, d.tet_v, d.tet_param = _ptr(keep["tet_p"], u32p), _ptr(keep["tet_v"], f32p), _ptr(keep["tet_param"], f32p)
+    d.n_tri = len(keep["tri_param"]) // 6
+    d.tri_p, d.tri_v, d.tri_param = _ptr(keep["tri_p"], u32p), _ptr(keep["tri_v"], f32p), _ptr(keep["tri_param"], f32p)
+    d.n_stretch = len(keep["st_param"]) // 12
+    d.st_p, d.st_param = _ptr(keep["st_p"], u32p), _ptr(keep["st_param"], f32p)
+    d.n_shear = len(keep["sh_param"]) // 8
+    d.sh_p, d.sh_param = _ptr(keep["sh_p"], u32p), _ptr(keep["sh_param"], f32p)
+    d.n_spring = len(keep["sp_param"]) // 3
+    d.sp_p, d.sp_param = _ptr(keep["sp_p"], u32p), _ptr(keep["sp_param"], f32p)
+    d.n_mouse = len(keep["mo_param"]) // 6
+    d.mo_p, d.mo_param = _ptr(keep["mo_p"], u32p), _ptr(keep["mo_param"], f32p)
+    d.n_world = len(keep["w_type"])
+    d.w_type, d.w_param, d.w_friction = (_ptr(keep["w_type"], u32p), _ptr(keep["w_param"], f32p),
+                                         _ptr(keep["w_friction"], f32p))
+    vox = keep["w_voxels"]
+    arrp = (f32p * max(1, len(vox)))(*[(_ptr(v, f32p) if v is not None else C.cast(None, f32p)) for v in vox])
+    keep["_vox"] = arrp
+    d.w_voxels = C.cast(arrp, C.POINTER(f32p))
+    d.gravity = (C.c_float * 3)(*[float(g) for g in keep["gravity"]])
+    return d, keep
+
+
+_lib = None
+
+EXPORTS = [
+    "b3g_status_string", "b3g_last_error", "b3g_device_count", "b3g_body_create", "b3g_body_destroy",
+    "b3g_body_build", "b3g_step", "b3g_get_state", "b3g_set_state", "b3g_apply_external", "b3g_get_masses",
+    "b3g_get_stats", "b3g_get_energy", "b3g_get_contacts", "b3g_get_tet_quats", "b3g_assemble", "b3g_get_system",
+    "b3g_synchronize", "b3g_set_kernel_timing", "b3g_get_spmv_timing", "b3g_get_system_size", "b3g_pattern",
+    "b3g_solve_cg", "b3g_spmv",
+]
+
+
+def lib():
+    """Loads libb3g.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is missing: build it with `python -m bounce_softbody_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "b3g_status_string": (C.c_char_p, [C.c_int]),
+        "b3g_last_error": (C.c_char_p, []),
+        "b3g_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+        "b3g_body_create": (C.c_int, [C.POINTER(vp), C.c_int]),
+        "b3g_body_destroy": (C.c_int, [vp]),
+        "b3g_body_build": (C.c_int, [vp, C.POINTER(BodyDesc)]),
+        "b3g_step": (C.c_int, [vp, C.c_float, C.c_uint32, C.c_uint32, C.c_float, C.c_float]),
+        "b3g_get_state": (C.c_int, [vp, f32p, f32p]),
+        "b3g_set_state": (C.c_int, [vp, f32p, f32p]),
+        "b3g_apply_external": (C.c_int, [vp, f32p, f32p]),
+        "b3g_get_masses": (C.c_int, [vp, f32p]),
+        "b3g_get_stats": (C.c_int, [vp, C.POINTER(StepStats)]),
+        "b3g_get_energy": (C.c_int, [vp, f32p]),
+        "b3g_get_contacts": (C.c_int, [vp, u32p, u32p, u32p, f32p, C.c_uint32]),
+        "b3g_get_tet_quats": (C.c_int, [vp, f32p]),
+        "b3g_assemble": (C.c_int, [vp, C.c_float, u32p]),
+        "b3g_get_system": (C.c_int, [vp, u32p, u32p, f32p, f32p, f32p]),
+        "b3g_synchronize": (C.c_int, [vp]),
+        "b3g_set_kernel_timing": (C.c_int, [vp, C.c_int]),
+        "b3g_get_spmv_timing": (C.c_int, [vp, u32p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "b3g_get_system_size": (C.c_int, [vp, u32p, u32p]),
+        "b3g_pattern": (C.c_int, [C.POINTER(BodyDesc), u32p, u32p, u32p, C.c_uint32]),
+        "b3g_solve_cg": (C.c_int, [C.c_int, C.c_uint32, u32p, u32p, f32p, f32p, f32p, C.c_uint32, C.c_float, u32p]),
+        "b3g_spmv": (C.c_int, [C.c_int, C.c_uint32, u32p, u32p, f32p, f32p, f32p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(status: int, where: str):
+    if status != B3G_OK:
+        raise B3gError(status, where, lib().b3g_last_error().decode())
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    st = lib().b3g_device_count(C.byref(n))
+    return n.value if st == B3G_OK else 0
+
+
+def pattern(arrays: dict):
+    """pA's block pattern (rowptr, cols) in the reference's row order, computed on the host."""
+    d, keep = make_desc(arrays)
+    n = d.n_particles
+    nnz = C.c_uint32(0)
+    rowptr = np.zeros(n + 1, np.uint32)
+    check(lib().b3g_pattern(C.byref(d), C.byref(nnz), _ptr(rowptr, u32p), None, 0), "b3g_pattern")
+    cols = np.zeros(max(1, nnz.value), np.uint32)
+    check(lib().b3g_pattern(C.byref(d), C.byref(nnz), _ptr(rowptr, u32p), _ptr(cols, u32p), nnz.value), "b3g_pattern")
+    return rowptr, cols[: nnz.value]
+
+
+def solve_cg(rowptr, cols, vals, b, x0, max_iterations, tolerance, device=0):
+    """Device PCG (b3SparseSolveCG) on a list-order CSR matrix."""
+    n = len(rowptr) - 1
+    rowptr = np.ascontiguousarray(rowptr, np.uint32)
+    cols = np.ascontiguousarray(cols, np.uint32)
+    vals = np.ascontiguousarray(vals, np.float32).reshape(-1)
+    b = np.ascontiguousarray(b, np.float32).reshape(-1)
+    x = np.ascontiguousarray(x0, np.float32).reshape(-1).copy()
+    it = C.c_uint32(0)
+    check(lib().b3g_solve_cg(device, n, _ptr(rowptr, u32p), _ptr(cols, u32p), _ptr(vals, f32p), _ptr(b, f32p),
+                             _ptr(x, f32p), max_iterations, C.c_float(tolerance), C.byref(it)), "b3g_solve_cg")
+    return x.reshape(-1, 3), it.value
+
+
+def spmv(rowptr, cols, vals, x, device=0):
+    n = len(rowptr) - 1
+    rowptr = np.ascontiguousarray(rowptr, np.uint32)
+    cols = np.ascontiguousarray(cols, np.uint32)
+    vals = np.ascontiguousarray(vals, np.float32).reshape(-1)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1)
+    y = np.zeros(3 * n, np.float32)
+    check(lib().b3g_spmv(device, n, _ptr(rowptr, u32p), _ptr(cols, u32p), _ptr(vals, f32p), _ptr(x, f32p),
+                         _ptr(y, f32p)), "b3g_spmv")
+    return y.reshape(-1, 3)

@@ -1,0 +1,157 @@
+"""Host-side mirror of the reference body API over the device library.
+
+:class:`Body` keeps the reference's method names and argument meaning
+(``b3Body``, dynamics/body.h:61-215; ``b3Particle``, particle.h:71-206) on
+top of libb3g: objects are described through a :class:`SceneBuilder`
+(creation order, exactly as the reference's ``Create*`` calls), the body is
+uploaded on the first :meth:`Step` after a change, and the step runs on the
+GPU. Errors raise :class:`B3gError` instead of asserting.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import B3gError, StepStats, _ptr, check, f32p, u32p
+from .scene import SceneBuilder
+
+f32 = np.float32
+
+
+class Body:
+    """A soft body stepped on the GPU (``b3Body``)."""
+
+    def __init__(self, scene: SceneBuilder | dict | None = None, device: int = 0):
+        self._L = _lib.lib()
+        self._h = C.c_void_p()
+        check(self._L.b3g_body_create(C.byref(self._h), device), "b3g_body_create")
+        self.device = device
+        self.scene = scene if scene is not None else SceneBuilder()
+        self._arrays = None
+        self._dirty = True
+        self.n = 0
+        self.n_tet = 0
+        if scene is not None:
+            self.commit()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._L.b3g_body_destroy(h)
+            self._h = C.c_void_p()
+
+    # ------------------------------------------------------------ building
+    def commit(self):
+        """Uploads the description (particles, fixtures, forces, world fixtures)."""
+        arr = self.scene if isinstance(self.scene, dict) else self.scene.arrays()
+        desc, keep = _lib.make_desc(arr)
+        check(self._L.b3g_body_build(self._h, C.byref(desc)), "b3g_body_build")
+        self._arrays = arr
+        self.n = len(arr["p_type"])
+        self.n_tet = len(arr["tet_param"]) // 3
+        self._dirty = False
+
+    def SetGravity(self, g):   # noqa: N802 (reference name)
+        if isinstance(self.scene, SceneBuilder):
+            self.scene.gravity[:] = np.asarray(g, f32)
+        self._dirty = True
+
+    # ------------------------------------------------------------ stepping
+    def Step(self, dt, force_iterations=1, force_sub_iterations=20, tolerance=0.0, sub_tolerance=0.0):  # noqa: N802
+        """``b3Body::Step`` (body.cpp:552-604); tolerances <= 0 use B3_EPSILON."""
+        if self._dirty:
+            self.commit()
+        check(self._L.b3g_step(self._h, C.c_float(dt), force_iterations, force_sub_iterations, C.c_float(tolerance),
+                               C.c_float(sub_tolerance)), "b3g_step")
+
+    step = Step
+
+    def synchronize(self):
+        check(self._L.b3g_synchronize(self._h), "b3g_synchronize")
+
+    # ------------------------------------------------------------ state (creation order)
+    def state(self):
+        x = np.zeros(3 * self.n, f32)
+        v = np.zeros(3 * self.n, f32)
+        check(self._L.b3g_get_state(self._h, _ptr(x, f32p), _ptr(v, f32p)), "b3g_get_state")
+        return x.reshape(-1, 3), v.reshape(-1, 3)
+
+    def set_state(self, x=None, v=None):
+        xa = None if x is None else np.ascontiguousarray(x, f32).reshape(-1)
+        va = None if v is None else np.ascontiguousarray(v, f32).reshape(-1)
+        check(self._L.b3g_set_state(self._h, _ptr(xa, f32p) if xa is not None else None,
+                                    _ptr(va, f32p) if va is not None else None), "b3g_set_state")
+
+    def apply_external(self, force=None, translation=None):
+        """``b3Particle::ApplyForce`` / ``ApplyTranslation`` for every particle."""
+        fa = None if force is None else np.ascontiguousarray(force, f32).reshape(-1)
+        ta = None if translation is None else np.ascontiguousarray(translation, f32).reshape(-1)
+        check(self._L.b3g_apply_external(self._h, _ptr(fa, f32p) if fa is not None else None,
+                                         _ptr(ta, f32p) if ta is not None else None), "b3g_apply_external")
+
+    def masses(self):
+        m = np.zeros(self.n, f32)
+        check(self._L.b3g_get_masses(self._h, _ptr(m, f32p)), "b3g_get_masses")
+        return m
+
+    def stats(self):
+        s = StepStats()
+        check(self._L.b3g_get_stats(self._h, C.byref(s)), "b3g_get_stats")
+        return np.array([s.newton_iterations, s.min_sub_iterations, s.max_sub_iterations], np.uint32), s.contact_count
+
+    def GetEnergy(self):   # noqa: N802
+        e = C.c_float(0)
+        check(self._L.b3g_get_energy(self._h, C.byref(e)), "b3g_get_energy")
+        return e.value
+
+    def contacts(self):
+        cnt = C.c_uint32(0)
+        check(self._L.b3g_get_contacts(self._h, C.byref(cnt), None, None, None, 0), "b3g_get_contacts")
+        k = cnt.value
+        p = np.zeros(max(1, k), np.uint32)
+        w = np.zeros(max(1, k), np.uint32)
+        f = np.zeros(max(1, k), f32)
+        check(self._L.b3g_get_contacts(self._h, C.byref(cnt), _ptr(p, u32p), _ptr(w, u32p), _ptr(f, f32p), k),
+              "b3g_get_contacts")
+        return p[:k].copy(), w[:k].copy(), f[:k].copy()
+
+    def tet_quats(self):
+        q = np.zeros(4 * max(1, self.n_tet), f32)
+        if self.n_tet:
+            check(self._L.b3g_get_tet_quats(self._h, _ptr(q, f32p)), "b3g_get_tet_quats")
+        return q[: 4 * self.n_tet].reshape(-1, 4)
+
+    def assemble(self, dt):
+        """pA (list-order CSR), pb and fi of the first linearisation of a step at the current state."""
+        nnz = C.c_uint32(0)
+        check(self._L.b3g_assemble(self._h, C.c_float(dt), C.byref(nnz)), "b3g_assemble")
+        k = nnz.value
+        rowptr = np.zeros(self.n + 1, np.uint32)
+        cols = np.zeros(max(1, k), np.uint32)
+        vals = np.zeros(9 * max(1, k), f32)
+        pb = np.zeros(3 * self.n, f32)
+        fi = np.zeros(3 * self.n, f32)
+        check(self._L.b3g_get_system(self._h, _ptr(rowptr, u32p), _ptr(cols, u32p), _ptr(vals, f32p), _ptr(pb, f32p),
+                                     _ptr(fi, f32p)), "b3g_get_system")
+        return rowptr, cols[:k], vals[: 9 * k].reshape(-1, 9), pb.reshape(-1, 3), fi.reshape(-1, 3)
+
+    def system_size(self):
+        r, b = C.c_uint32(0), C.c_uint32(0)
+        check(self._L.b3g_get_system_size(self._h, C.byref(r), C.byref(b)), "b3g_get_system_size")
+        return r.value, b.value
+
+    def set_kernel_timing(self, on: bool):
+        check(self._L.b3g_set_kernel_timing(self._h, 1 if on else 0), "b3g_set_kernel_timing")
+
+    def spmv_timing(self):
+        n = C.c_uint32(0)
+        ms = C.c_double(0)
+        by = C.c_double(0)
+        check(self._L.b3g_get_spmv_timing(self._h, C.byref(n), C.byref(ms), C.byref(by)), "b3g_get_spmv_timing")
+        return n.value, ms.value, by.value
+
+
+__all__ = ["Body", "B3gError"]

@@ -1,0 +1,476 @@
+// C entry points of libb3g (include/b3g.h).
+#include "b3g_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <tuple>
+
+struct b3g_body_s {
+	b3g::Body body;
+};
+
+namespace b3g {
+
+static thread_local std::string g_error;
+void set_error(const std::string& msg) { g_error = msg; }
+
+} // namespace b3g
+
+using namespace b3g;
+
+#define CHECK_BODY(bd)                                                  \
+	do {                                                                \
+		if (!(bd)) { set_error("null body handle"); return B3G_INVALID_ARGUMENT; } \
+	} while (0)
+#define CHECK_BUILT(bd)                                                 \
+	do {                                                                \
+		CHECK_BODY(bd);                                                 \
+		if (!(bd)->body.built) { set_error("body has not been built"); return B3G_NOT_COMMITTED; } \
+	} while (0)
+#define SET_DEVICE(bd) B3G_HIP(hipSetDevice((bd)->body.device))
+
+namespace {
+
+// Host copy of a list-order V3 array into creation order.
+b3g_status read_v3(Body& b, const DBuf& buf, float* out) {
+	std::vector<V3> tmp(b.n);
+	B3G_HIP(hipMemcpyAsync(tmp.data(), buf.p, b.n * sizeof(V3), hipMemcpyDeviceToHost, b.stream));
+	B3G_HIP(hipStreamSynchronize(b.stream));
+	for (uint32_t c = 0; c < b.n; ++c) {
+		const V3& v = tmp[b.n - 1 - c];
+		out[3 * c] = v.x; out[3 * c + 1] = v.y; out[3 * c + 2] = v.z;
+	}
+	return B3G_OK;
+}
+
+b3g_status write_v3(Body& b, DBuf& buf, const float* in, bool accumulate, const uint8_t* onlyType = nullptr) {
+	std::vector<V3> tmp(b.n);
+	if (accumulate) {
+		B3G_HIP(hipMemcpyAsync(tmp.data(), buf.p, b.n * sizeof(V3), hipMemcpyDeviceToHost, b.stream));
+		B3G_HIP(hipStreamSynchronize(b.stream));
+	}
+	for (uint32_t c = 0; c < b.n; ++c) {
+		uint32_t i = b.n - 1 - c;
+		V3 v = v3(in[3 * c], in[3 * c + 1], in[3 * c + 2]);
+		if (onlyType && b.hostType[i] != *onlyType) continue;
+		tmp[i] = accumulate ? tmp[i] + v : v;   // m_force += force (particle.h:249-256)
+	}
+	B3G_HIP(hipMemcpyAsync(buf.p, tmp.data(), b.n * sizeof(V3), hipMemcpyHostToDevice, b.stream));
+	B3G_HIP(hipStreamSynchronize(b.stream));
+	return B3G_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* b3g_status_string(b3g_status s) {
+	switch (s) {
+	case B3G_OK: return "ok";
+	case B3G_INVALID_ARGUMENT: return "invalid argument";
+	case B3G_HIP_ERROR: return "HIP error";
+	case B3G_OUT_OF_MEMORY: return "out of device memory";
+	case B3G_DEGENERATE_ELEMENT: return "degenerate element";
+	case B3G_UNSUPPORTED: return "unsupported";
+	case B3G_NOT_COMMITTED: return "body not built";
+	}
+	return "unknown status";
+}
+
+const char* b3g_last_error(void) { return g_error.c_str(); }
+
+b3g_status b3g_device_count(int* count) {
+	if (!count) return B3G_INVALID_ARGUMENT;
+	hipError_t e = hipGetDeviceCount(count);
+	if (e != hipSuccess) { *count = 0; set_error(hipGetErrorString(e)); return B3G_HIP_ERROR; }
+	return B3G_OK;
+}
+
+b3g_status b3g_body_create(b3g_body* out, int device) {
+	if (!out) { set_error("out is NULL"); return B3G_INVALID_ARGUMENT; }
+	*out = nullptr;
+	int count = 0;
+	B3G_HIP(hipGetDeviceCount(&count));
+	if (device < 0 || device >= count) { set_error("device index out of range"); return B3G_INVALID_ARGUMENT; }
+	B3G_HIP(hipSetDevice(device));
+	b3g_body_s* h = new (std::nothrow) b3g_body_s();
+	if (!h) return B3G_OUT_OF_MEMORY;
+	h->body.device = device;
+	hipError_t e = hipStreamCreateWithFlags(&h->body.stream, hipStreamNonBlocking);
+	if (e != hipSuccess) { set_error(hipGetErrorString(e)); delete h; return B3G_HIP_ERROR; }
+	*out = h;
+	return B3G_OK;
+}
+
+b3g_status b3g_body_destroy(b3g_body h) {
+	if (!h) return B3G_OK;
+	(void)hipSetDevice(h->body.device);
+	(void)hipStreamSynchronize(h->body.stream);
+	dfree_all(h->body);
+	for (hipEvent_t e : h->body.timing.start) (void)hipEventDestroy(e);
+	for (hipEvent_t e : h->body.timing.stop) (void)hipEventDestroy(e);
+	(void)hipStreamDestroy(h->body.stream);
+	delete h;
+	return B3G_OK;
+}
+
+b3g_status b3g_body_build(b3g_body h, const b3g_body_desc* d) {
+	CHECK_BODY(h);
+	if (!d) { set_error("desc is NULL"); return B3G_INVALID_ARGUMENT; }
+	SET_DEVICE(h);
+	return build_body(h->body, *d);
+}
+
+b3g_status b3g_step(b3g_body h, float dt, uint32_t iterations, uint32_t sub_iterations, float tolerance,
+                    float sub_tolerance) {
+	CHECK_BUILT(h);
+	if (!std::isfinite(dt)) { set_error("dt must be finite"); return B3G_INVALID_ARGUMENT; }
+	float tol = tolerance > 0.0f ? tolerance : kEps;
+	float stol = sub_tolerance > 0.0f ? sub_tolerance : kEps;
+	if (!(stol < 1.0f)) { set_error("sub_tolerance must be in (0, 1)"); return B3G_INVALID_ARGUMENT; }   // sparse_solver.cpp:36
+	SET_DEVICE(h);
+	return enqueue_step(h->body, dt, iterations, sub_iterations, tol, stol, false);
+}
+
+b3g_status b3g_get_state(b3g_body h, float* pos, float* vel) {
+	CHECK_BUILT(h);
+	SET_DEVICE(h);
+	if (pos) { b3g_status s = read_v3(h->body, h->body.x, pos); if (s != B3G_OK) return s; }
+	if (vel) { b3g_status s = read_v3(h->body, h->body.v, vel); if (s != B3G_OK) return s; }
+	return B3G_OK;
+}
+
+b3g_status b3g_set_state(b3g_body h, const float* pos, const float* vel) {
+	CHECK_BUILT(h);
+	SET_DEVICE(h);
+	Body& b = h->body;
+	if (pos) {
+		b3g_status s = write_v3(b, b.x, pos, false);
+		if (s != B3G_OK) return s;
+		// SetPosition clears the pending translation (particle.h:213-218)
+		std::vector<V3> z(b.n, zero3());
+		B3G_HIP(hipStreamSynchronize(b.stream));
+		B3G_HIP(hipMemcpy(b.translation.p, z.data(), b.n * sizeof(V3), hipMemcpyHostToDevice));
+	}
+	if (vel) {
+		// SetVelocity is ignored for static particles (particle.h:225-232)
+		std::vector<V3> cur(b.n);
+		B3G_HIP(hipStreamSynchronize(b.stream));
+		B3G_HIP(hipMemcpy(cur.data(), b.v.p, b.n * sizeof(V3), hipMemcpyDeviceToHost));
+		for (uint32_t c = 0; c < b.n; ++c) {
+			uint32_t i = b.n - 1 - c;
+			if (b.hostType[i] != kStatic) cur[i] = v3(vel[3 * c], vel[3 * c + 1], vel[3 * c + 2]);
+		}
+		B3G_HIP(hipMemcpy(b.v.p, cur.data(), b.n * sizeof(V3), hipMemcpyHostToDevice));
+	}
+	return B3G_OK;
+}
+
+b3g_status b3g_apply_external(b3g_body h, const float* force, const float* translation) {
+	CHECK_BUILT(h);
+	SET_DEVICE(h);
+	Body& b = h->body;
+	const uint8_t dyn = kDynamic;
+	if (force) { b3g_status s = write_v3(b, b.force, force, true, &dyn); if (s != B3G_OK) return s; }
+	if (translation) { b3g_status s = write_v3(b, b.translation, translation, true); if (s != B3G_OK) return s; }
+	return B3G_OK;
+}
+
+b3g_status b3g_get_masses(b3g_body h, float* mass) {
+	CHECK_BUILT(h);
+	if (!mass) return B3G_INVALID_ARGUMENT;
+	const Body& b = h->body;
+	for (uint32_t c = 0; c < b.n; ++c) mass[c] = b.hostMass[b.n - 1 - c];
+	return B3G_OK;
+}
+
+b3g_status b3g_get_stats(b3g_body h, b3g_step_stats* out) {
+	CHECK_BUILT(h);
+	if (!out) return B3G_INVALID_ARGUMENT;
+	SET_DEVICE(h);
+	Body& b = h->body;
+	SolverState st;
+	B3G_HIP(hipMemcpyAsync(&st, b.state.p, sizeof(st), hipMemcpyDeviceToHost, b.stream));
+	B3G_HIP(hipStreamSynchronize(b.stream));
+	out->newton_iterations = st.statNewton;
+	out->min_sub_iterations = st.statMinSub;
+	out->max_sub_iterations = st.statMaxSub;
+	uint32_t count = 0;
+	if (b.nWorld && b.nSpheres) {
+		std::vector<uint32_t> act((size_t)b.nSpheres * b.nWorld);
+		B3G_HIP(hipStreamSynchronize(b.stream));
+		B3G_HIP(hipMemcpy(act.data(), b.cActive.p, act.size() * 4, hipMemcpyDeviceToHost));
+		for (uint32_t a : act) count += a;
+	}
+	out->contact_count = count;
+	return B3G_OK;
+}
+
+b3g_status b3g_get_energy(b3g_body h, float* energy) {
+	CHECK_BUILT(h);
+	if (!energy) return B3G_INVALID_ARGUMENT;
+	SET_DEVICE(h);
+	Body& b = h->body;
+	std::vector<V3> v(b.n);
+	B3G_HIP(hipMemcpyAsync(v.data(), b.v.p, b.n * sizeof(V3), hipMemcpyDeviceToHost, b.stream));
+	B3G_HIP(hipStreamSynchronize(b.stream));
+	float E = 0.0f;
+	for (uint32_t i = 0; i < b.n; ++i) E += b.hostMass[i] * dot(v[i], v[i]);   // body.cpp:384-392
+	*energy = 0.5f * E;
+	return B3G_OK;
+}
+
+b3g_status b3g_get_contacts(b3g_body h, uint32_t* count, uint32_t* particle, uint32_t* world, float* normal_force,
+                            uint32_t capacity) {
+	CHECK_BUILT(h);
+	if (!count) return B3G_INVALID_ARGUMENT;
+	SET_DEVICE(h);
+	Body& b = h->body;
+	*count = 0;
+	if (!b.nWorld || !b.nSpheres) return B3G_OK;
+	const size_t pairs = (size_t)b.nSpheres * b.nWorld;
+	std::vector<uint32_t> act(pairs), stamp(pairs), sp(b.nSpheres);
+	std::vector<float> nf(pairs);
+	B3G_HIP(hipStreamSynchronize(b.stream));
+	B3G_HIP(hipMemcpy(act.data(), b.cActive.p, pairs * 4, hipMemcpyDeviceToHost));
+	B3G_HIP(hipMemcpy(stamp.data(), b.cStamp.p, pairs * 4, hipMemcpyDeviceToHost));
+	B3G_HIP(hipMemcpy(nf.data(), b.cNormalForce.p, pairs * 4, hipMemcpyDeviceToHost));
+	B3G_HIP(hipMemcpy(sp.data(), b.sphereParticle.p, b.nSpheres * 4, hipMemcpyDeviceToHost));
+	// List order (contact_manager.cpp:54-62): newest FindNewContacts call first; within a
+	// call the last created pair (sphere list order, then world list order) is the head.
+	std::vector<std::tuple<uint32_t, uint32_t, uint32_t, size_t>> keys;
+	for (size_t k = 0; k < pairs; ++k)
+		if (act[k]) {
+			uint32_t s = (uint32_t)(k / b.nWorld), w = (uint32_t)(k % b.nWorld);
+			keys.emplace_back(stamp[k] / b.nWorld, s, w, k);
+		}
+	std::sort(keys.begin(), keys.end(), [](const auto& a, const auto& c) {
+		return std::make_tuple(std::get<0>(a), std::get<1>(a), std::get<2>(a)) >
+		       std::make_tuple(std::get<0>(c), std::get<1>(c), std::get<2>(c));
+	});
+	*count = (uint32_t)keys.size();
+	for (size_t i = 0; i < keys.size() && i < capacity; ++i) {
+		size_t k = std::get<3>(keys[i]);
+		uint32_t s = std::get<1>(keys[i]), w = std::get<2>(keys[i]);
+		if (particle) particle[i] = b.n - 1 - sp[s];
+		if (world) world[i] = b.nWorld - 1 - w;
+		if (normal_force) normal_force[i] = nf[k];
+	}
+	return B3G_OK;
+}
+
+b3g_status b3g_get_tet_quats(b3g_body h, float* quats) {
+	CHECK_BUILT(h);
+	if (!quats) return B3G_INVALID_ARGUMENT;
+	SET_DEVICE(h);
+	Body& b = h->body;
+	const uint32_t m = b.tet.m;
+	if (!m) return B3G_OK;
+	std::vector<float> q(4 * (size_t)m);
+	B3G_HIP(hipStreamSynchronize(b.stream));
+	B3G_HIP(hipMemcpy(q.data(), b.tet.q.p, q.size() * 4, hipMemcpyDeviceToHost));
+	for (uint32_t e = 0; e < m; ++e)
+		for (int c = 0; c < 4; ++c) quats[4 * (size_t)e + c] = q[(size_t)c * m + e];
+	return B3G_OK;
+}
+
+b3g_status b3g_assemble(b3g_body h, float dt, uint32_t* nnz) {
+	CHECK_BUILT(h);
+	if (!(dt > 0.0f)) { set_error("dt must be positive"); return B3G_INVALID_ARGUMENT; }
+	SET_DEVICE(h);
+	b3g_status s = enqueue_step(h->body, dt, 1, 0, kEps, kEps, true);
+	if (s != B3G_OK) return s;
+	B3G_HIP(hipStreamSynchronize(h->body.stream));
+	if (nnz) *nnz = h->body.nnz;
+	return B3G_OK;
+}
+
+b3g_status b3g_get_system(b3g_body h, uint32_t* rowptr, uint32_t* cols, float* vals, float* pb, float* fi) {
+	CHECK_BUILT(h);
+	SET_DEVICE(h);
+	Body& b = h->body;
+	B3G_HIP(hipStreamSynchronize(b.stream));
+	if (rowptr) std::memcpy(rowptr, b.hostRowptr.data(), (b.n + 1) * 4);
+	if (cols) std::memcpy(cols, b.hostCols.data(), (size_t)b.nnz * 4);
+	if (vals) {
+		std::vector<float> A((size_t)b.nnz * 9);
+		B3G_HIP(hipMemcpy(A.data(), b.A.p, A.size() * 4, hipMemcpyDeviceToHost));
+		for (size_t k = 0; k < b.nnz; ++k)
+			for (int c = 0; c < 9; ++c) vals[9 * k + c] = A[(size_t)c * b.nnz + k];
+	}
+	auto rows = [&](const DBuf& buf, float* out) -> b3g_status {
+		std::vector<V3> t(b.n);
+		B3G_HIP(hipMemcpy(t.data(), buf.p, b.n * sizeof(V3), hipMemcpyDeviceToHost));
+		for (uint32_t i = 0; i < b.n; ++i) { out[3 * i] = t[i].x; out[3 * i + 1] = t[i].y; out[3 * i + 2] = t[i].z; }
+		return B3G_OK;
+	};
+	if (pb) { b3g_status s = rows(b.pb, pb); if (s != B3G_OK) return s; }
+	if (fi) { b3g_status s = rows(b.fi, fi); if (s != B3G_OK) return s; }
+	return B3G_OK;
+}
+
+b3g_status b3g_synchronize(b3g_body h) {
+	CHECK_BODY(h);
+	SET_DEVICE(h);
+	B3G_HIP(hipStreamSynchronize(h->body.stream));
+	return B3G_OK;
+}
+
+b3g_status b3g_set_kernel_timing(b3g_body h, int enabled) {
+	CHECK_BODY(h);
+	SET_DEVICE(h);
+	Timing& t = h->body.timing;
+	B3G_HIP(hipStreamSynchronize(h->body.stream));
+	t.enabled = enabled != 0;
+	t.used = 0;
+	t.totalMs = 0.0;
+	t.launches = 0;
+	return B3G_OK;
+}
+
+b3g_status b3g_get_spmv_timing(b3g_body h, uint32_t* launches, double* mean_ms, double* bytes_per_launch) {
+	CHECK_BUILT(h);
+	SET_DEVICE(h);
+	Body& b = h->body;
+	Timing& t = b.timing;
+	B3G_HIP(hipStreamSynchronize(b.stream));
+	// Launches skipped after PCG convergence still record an (empty) interval; count non-empty ones.
+	for (size_t k = 0; k < t.used; ++k) {
+		float ms = 0.0f;
+		B3G_HIP(hipEventElapsedTime(&ms, t.start[k], t.stop[k]));
+		t.totalMs += ms;
+		t.launches += 1;
+	}
+	t.used = 0;
+	if (launches) *launches = (uint32_t)t.launches;
+	if (mean_ms) *mean_ms = t.launches ? t.totalMs / (double)t.launches : 0.0;
+	if (bytes_per_launch) *bytes_per_launch = 40.0 * b.nnz + 4.0 * (b.n + 1) + 24.0 * b.n;
+	return B3G_OK;
+}
+
+b3g_status b3g_get_system_size(b3g_body h, uint32_t* rows, uint32_t* blocks) {
+	CHECK_BUILT(h);
+	if (rows) *rows = h->body.n;
+	if (blocks) *blocks = h->body.nnz;
+	return B3G_OK;
+}
+
+// Pattern of pA (list-order CSR, reference row order) from a description, host only.
+b3g_status b3g_pattern(const b3g_body_desc* d, uint32_t* nnz, uint32_t* rowptr, uint32_t* cols, uint32_t capacity) {
+	if (!d || !nnz) return B3G_INVALID_ARGUMENT;
+	std::vector<uint32_t> rp, cl;
+	b3g_status s = host_pattern(*d, rp, cl);
+	if (s != B3G_OK) return s;
+	*nnz = (uint32_t)cl.size();
+	if (rowptr) std::memcpy(rowptr, rp.data(), rp.size() * 4);
+	if (cols && capacity >= cl.size()) std::memcpy(cols, cl.data(), cl.size() * 4);
+	return B3G_OK;
+}
+
+// ----------------------------------------------------------------------------- operator-level entry points
+namespace {
+// A body that holds only a matrix (for b3g_solve_cg / b3g_spmv).
+b3g_status matrix_body(Body& b, int device, uint32_t n, const uint32_t* rowptr, const uint32_t* cols,
+                       const float* vals) {
+	B3G_HIP(hipSetDevice(device));
+	b.device = device;
+	B3G_HIP(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+	b.n = n;
+	b.nnz = rowptr[n];
+	b.hostRowptr.assign(rowptr, rowptr + n + 1);
+	b.hostCols.assign(cols, cols + b.nnz);
+	for (uint32_t i = 0; i < n; ++i) {
+		if (rowptr[i + 1] < rowptr[i]) { set_error("rowptr must be non-decreasing"); return B3G_INVALID_ARGUMENT; }
+		for (uint32_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+			if (cols[k] >= n) { set_error("column index out of range"); return B3G_INVALID_ARGUMENT; }
+	}
+	std::vector<float> planes((size_t)b.nnz * 9);
+	for (size_t k = 0; k < b.nnz; ++k)
+		for (int c = 0; c < 9; ++c) planes[(size_t)c * b.nnz + k] = vals[9 * k + c];
+	b3g_status s;
+	if ((s = dupload(b, b.A, planes.data(), planes.size() * 4)) != B3G_OK) return s;
+	if ((s = dupload(b, b.rowptr, rowptr, (n + 1) * 4)) != B3G_OK) return s;
+	if ((s = dupload(b, b.cols, cols, (size_t)b.nnz * 4)) != B3G_OK) return s;
+	for (DBuf* buf : {&b.pb, &b.invM, &b.px, &b.r, &b.s, &b.q, &b.d0, &b.d1})
+		if ((s = dalloc(b, *buf, (size_t)n * sizeof(V3))) != B3G_OK) return s;
+	if ((s = dalloc(b, b.state, sizeof(SolverState))) != B3G_OK) return s;
+	return build_tiles(b);
+}
+
+void matrix_body_free(Body& b) {
+	(void)hipStreamSynchronize(b.stream);
+	dfree_all(b);
+	if (b.stream) (void)hipStreamDestroy(b.stream);
+}
+
+__global__ void k_set_state(SolverState* st, uint32_t maxIt, float eps2) {
+	uint32_t* words = reinterpret_cast<uint32_t*>(st);
+	for (size_t k = 0; k < sizeof(SolverState) / 4; ++k) words[k] = 0;
+	st->maxIt = maxIt;
+	st->eps2_sub = eps2;
+	st->newtonActive = 1;
+}
+
+__global__ void k_invm(uint32_t n, const uint32_t* rowptr, const uint32_t* cols, const float* A, uint32_t nnz, V3* invM) {
+	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	float axx = 0.0f, ayy = 0.0f, azz = 0.0f;
+	for (uint32_t k = rowptr[i]; k < rowptr[i + 1]; ++k)   // b3SparseMat33::operator() const: first match from the head
+		if (cols[k] == i) { axx = A[k]; ayy = A[(size_t)4 * nnz + k]; azz = A[(size_t)8 * nnz + k]; break; }
+	invM[i] = v3(1.0f / axx, 1.0f / ayy, 1.0f / azz);
+}
+} // namespace
+
+b3g_status b3g_solve_cg(int device, uint32_t n, const uint32_t* rowptr, const uint32_t* cols, const float* vals,
+                        const float* b_in, float* x, uint32_t max_iterations, float tolerance, uint32_t* iterations) {
+	if (!rowptr || !cols || !vals || !b_in || !x || n == 0) { set_error("null argument"); return B3G_INVALID_ARGUMENT; }
+	if (!(tolerance > 0.0f && tolerance < 1.0f)) { set_error("tolerance must be in (0, 1)"); return B3G_INVALID_ARGUMENT; }
+	Body b;
+	b3g_status s = matrix_body(b, device, n, rowptr, cols, vals);
+	if (s == B3G_OK) {
+		auto body = [&]() -> b3g_status {
+			B3G_HIP(hipMemcpyAsync(b.pb.p, b_in, n * sizeof(V3), hipMemcpyHostToDevice, b.stream));
+			B3G_HIP(hipMemcpyAsync(b.px.p, x, n * sizeof(V3), hipMemcpyHostToDevice, b.stream));
+			hipLaunchKernelGGL(k_set_state, dim3(1), dim3(1), 0, b.stream, dptr<SolverState>(b.state), max_iterations,
+			                   tolerance * tolerance);
+			hipLaunchKernelGGL(k_invm, dim3((n + 255) / 256), dim3(256), 0, b.stream, n, dptr<uint32_t>(b.rowptr),
+			                   dptr<uint32_t>(b.cols), dptr<float>(b.A), b.nnz, dptr<V3>(b.invM));
+			B3G_HIP(hipGetLastError());
+			b3g_status s2 = enqueue_solve_cg(b, max_iterations, tolerance);
+			if (s2 != B3G_OK) return s2;
+			B3G_HIP(hipMemcpyAsync(x, b.px.p, n * sizeof(V3), hipMemcpyDeviceToHost, b.stream));
+			SolverState st;
+			B3G_HIP(hipMemcpyAsync(&st, b.state.p, sizeof(st), hipMemcpyDeviceToHost, b.stream));
+			B3G_HIP(hipStreamSynchronize(b.stream));
+			if (iterations) *iterations = st.it;
+			return B3G_OK;
+		};
+		s = body();
+	}
+	matrix_body_free(b);
+	return s;
+}
+
+b3g_status b3g_spmv(int device, uint32_t n, const uint32_t* rowptr, const uint32_t* cols, const float* vals,
+                    const float* x, float* y) {
+	if (!rowptr || !cols || !vals || !x || !y || n == 0) { set_error("null argument"); return B3G_INVALID_ARGUMENT; }
+	Body b;
+	b3g_status s = matrix_body(b, device, n, rowptr, cols, vals);
+	if (s == B3G_OK) {
+		auto body = [&]() -> b3g_status {
+			B3G_HIP(hipMemcpyAsync(b.px.p, x, n * sizeof(V3), hipMemcpyHostToDevice, b.stream));
+			b3g_status s2 = enqueue_spmv(b, b.px, b.q, false);
+			if (s2 != B3G_OK) return s2;
+			B3G_HIP(hipMemcpyAsync(y, b.q.p, n * sizeof(V3), hipMemcpyDeviceToHost, b.stream));
+			B3G_HIP(hipStreamSynchronize(b.stream));
+			return B3G_OK;
+		};
+		s = body();
+	}
+	matrix_body_free(b);
+	return s;
+}
+
+} // extern "C"

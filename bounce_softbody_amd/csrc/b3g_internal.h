@@ -1,0 +1,209 @@
+// Internal state of a b3g body (host bookkeeping + device arrays).
+#pragma once
+
+#include "b3g.h"
+#include "b3g_math.h"
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace b3g {
+
+enum : uint8_t { kStatic = 0, kKinematic = 1, kDynamic = 2 };
+
+// Thread-local error text for b3g_last_error().
+void set_error(const std::string& msg);
+
+#define B3G_HIP(expr)                                                                                  \
+	do {                                                                                               \
+		hipError_t e_ = (expr);                                                                        \
+		if (e_ != hipSuccess) {                                                                        \
+			::b3g::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                       \
+			return B3G_HIP_ERROR;                                                                      \
+		}                                                                                              \
+	} while (0)
+
+// A device allocation owned by the body.
+struct DBuf {
+	void* p = nullptr;
+	size_t bytes = 0;
+};
+
+template <class T> inline T* dptr(const DBuf& b) { return static_cast<T*>(b.p); }
+
+// Per-type element tables (device), list order within the type = reverse creation.
+struct TetTable {
+	uint32_t m = 0;
+	DBuf p;          // 4 planes of uint32 (particle list index)
+	DBuf rest;       // 12 planes: rest positions x1..x4
+	DBuf invE;       // 9 planes
+	DBuf K0;         // 144 planes: K[k] (k = i + 4j) column-major 3x3
+	DBuf q;          // 4 planes: warm-start quaternion x y z s
+	DBuf kd;         // 1 plane
+	DBuf param;      // 3 planes (E, nu, kd) for the set-up kernel
+	bool anyDamping = false;
+	uint32_t jx = 0, jv = 0, f = 0, fd = 0;   // staging bases (block k / vertex i: base + k*m + e)
+};
+
+struct TriTable {
+	uint32_t m = 0;
+	DBuf p;          // 3 planes
+	DBuf x2;         // 6 planes: rest 2D coordinates
+	DBuf invS;       // 4 planes
+	DBuf K0;         // 36 planes: 9 Mat22 blocks
+	DBuf kd;
+	bool anyDamping = false;
+	uint32_t jx = 0, jv = 0, f = 0, fd = 0;
+};
+
+struct UVTable {     // stretch (12 coefficient planes) and shear (8)
+	uint32_t m = 0;
+	DBuf p;          // 3 planes
+	DBuf c;          // coefficient planes
+	uint32_t ncoef = 0;
+	uint32_t jx = 0, jv = 0, f = 0;   // bases; per element local slots as described in b3g_setup
+	bool anyKs = false, anyKd = false;
+};
+
+struct PairTable {   // springs (2 particles, 3 coef) and mouse (4 particles, 6 coef)
+	uint32_t m = 0;
+	uint32_t np = 0;
+	DBuf p;
+	DBuf c;
+	uint32_t ncoef = 0;
+	uint32_t jx = 0, jv = 0, f = 0;
+	bool anyKs = false, anyKd = false;
+};
+
+// A world fixture (device copy is a flat array of these).
+struct WorldShape {
+	uint32_t type;
+	float radius, friction;
+	V3 a, b, c;              // sphere center / capsule centers / triangle vertices / box extents
+	uint32_t hasE[3];
+	V3 e[3];
+	Xf xf;
+	AABB grid;               // sdf voxel-grid aabb
+	uint32_t W, H, D;
+	uint32_t invert;
+	uint64_t voxOffset;      // float offset of this grid in the voxel buffer
+	AABB aabb;               // b3Shape::ComputeAABB of the fixture (static)
+};
+
+// Device-side scalar state of the Newton and PCG loops.
+struct SolverState {
+	// PCG (sparse_solver.cpp:57-103)
+	float delta_new, delta0, alpha, beta, eps2_sub;
+	uint32_t it, active, maxIt;
+	// Newton (sparse_force_solver.cpp:141-225)
+	float error, error0, eps2;
+	uint32_t newtonIt, newtonActive, newtonMax;
+	// running statistics (force_solver.cpp:30-34)
+	uint32_t statNewton, statMinSub, statMaxSub;
+	uint32_t contactCount;
+	// grid reduction bookkeeping: blocks arrived per reduction slot
+	uint32_t arrive[4];
+};
+
+struct Timing {
+	bool enabled = false;
+	std::vector<hipEvent_t> start, stop;
+	size_t used = 0;
+	double totalMs = 0.0;
+	uint64_t launches = 0;
+};
+
+struct Body {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	bool built = false;
+
+	// ---------------------------------------------------------------- sizes
+	uint32_t n = 0;          // particles
+	uint32_t nnz = 0;        // 3x3 blocks of pA
+	uint32_t nSpheres = 0, nWorld = 0;
+	uint32_t stampCounter = 0;
+	float gravity[3] = {0.0f, 0.0f, 0.0f};
+
+	// ---------------------------------------------------------------- particles (list order)
+	DBuf type, mass, invMass, damping;            // uint8, float x3
+	DBuf x, v, force, translation;                // V3
+	DBuf x0, v0, fe, y, w;                        // V3 (per-step work)
+	DBuf sphereOf;                                // int32 per particle: sphere list index or -1
+	DBuf sphereRadius, sphereFriction;            // per sphere (list order)
+	DBuf sphereParticle;                          // uint32 per sphere
+	std::vector<float> hostMass;                  // list order
+	std::vector<uint8_t> hostType;
+
+	// ---------------------------------------------------------------- elements
+	TetTable tet;
+	TriTable tri;
+	UVTable stretch, shear;
+	PairTable spring, mouse;
+	uint32_t nJStage = 0, nFStage = 0;
+	DBuf jstage;     // M33 contributions (element-major per local block)
+	DBuf fstage;     // V3 contributions
+
+	// ---------------------------------------------------------------- symbolic pattern (pA CSR, list order)
+	DBuf rowptr, cols, slotRow, diagSlot;         // uint32
+	DBuf jxPtr, jxIdx, jvPtr, jvIdx;              // per-slot contribution lists (stage indices)
+	DBuf bPtr, bSlot;                             // per-row dfdx slots in first-touch order (b's SpMV)
+	DBuf fPtr, fIdx;                              // per-row force contributions
+	DBuf tiles;                                   // uint32 row boundaries of SpMV tiles
+	uint32_t nTiles = 0;
+	std::vector<uint32_t> hostRowptr, hostCols;
+
+	// ---------------------------------------------------------------- linear system / PCG
+	DBuf A;          // 9 planes x nnz (pA blocks, column-major element planes)
+	DBuf tB;         // V3 per slot: (h dfdx_ij) w_j
+	DBuf pb, invM;   // V3 per row
+	DBuf px, r, s, q, d0, d1, fi;   // V3 per row (px = py warm start)
+	DBuf partial;    // double per block of the reduction kernels
+	uint32_t partialCount = 0;
+	DBuf state;      // SolverState
+
+	// ---------------------------------------------------------------- contacts (sphere x world pairs)
+	DBuf worlds;     // WorldShape[nWorld]
+	DBuf voxels;     // float
+	DBuf cActive, cStamp, cFriction;              // uint32 per pair
+	DBuf cNormalForce;                            // float per pair
+	DBuf cManifold;                               // 2 V3 per pair (point, normal)
+	DBuf cHit;                                    // uint32 per pair (this linearisation)
+	DBuf cF;                                      // V3 per pair
+	DBuf cK;                                      // M33 per pair
+	std::vector<WorldShape> hostWorlds;
+
+	// ---------------------------------------------------------------- diagnostics
+	bool captureFi = false;
+	Timing timing;
+
+	std::vector<DBuf*> owned;
+};
+
+// Allocation helpers (b3g_setup.hip)
+b3g_status dalloc(Body& b, DBuf& buf, size_t bytes);
+b3g_status dupload(Body& b, DBuf& buf, const void* host, size_t bytes);
+void dfree_all(Body& b);
+
+// Host build (b3g_setup.hip)
+b3g_status build_body(Body& b, const b3g_body_desc& d);
+
+// Symbolic pattern on the host only (b3g_setup.hip), for diagnostics.
+b3g_status host_pattern(const b3g_body_desc& d, std::vector<uint32_t>& rowptr, std::vector<uint32_t>& cols);
+
+// Kernel launchers (b3g_kernels.hip)
+b3g_status launch_tet_setup(Body& b);
+b3g_status enqueue_step(Body& b, float dt, uint32_t iters, uint32_t subIters, float tol, float subTol,
+                        bool assembleOnly);
+b3g_status enqueue_update_contacts(Body& b);
+b3g_status enqueue_find_contacts(Body& b);
+b3g_status enqueue_spmv(Body& b, const DBuf& xin, DBuf& yout, bool timed);
+b3g_status enqueue_solve_cg(Body& b, uint32_t maxIt, float eps);
+b3g_status build_tiles(Body& b);
+b3g_status ensure_partials(Body& b, uint32_t blocks);
+
+} // namespace b3g

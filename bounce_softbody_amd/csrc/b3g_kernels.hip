@@ -1,0 +1,1318 @@
+// Device kernels of the implicit step and their launch sequence.
+//
+// Data layout (HBM): particle vectors are arrays of 12-byte V3 in list order;
+// element tables are planar (one plane per scalar, element-contiguous) so a
+// wavefront reads 64 consecutive elements per load; pA is stored as 9 planes
+// of its blocks in CSR order (the reference's row order); element
+// contributions are staged once per linearisation and gathered per block in
+// the reference's summation order (no atomics, deterministic).
+#include "b3g_collide.h"
+
+#include <algorithm>
+
+namespace b3g {
+
+constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
+constexpr int kTileCap = 1024;     // 3x3 blocks per SpMV tile chunk (12 KB of LDS products)
+constexpr int kTileRows = 256;     // rows per SpMV tile (one phase-2 thread each)
+
+// ---------------------------------------------------------------------------
+// Helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t G) {
+	// Workgroups are dealt round-robin to the 8 XCDs; give each XCD a
+	// contiguous range of tiles so neighbouring rows share one L2.
+	uint32_t q = G / 8, r = G % 8, x = b % 8;
+	uint32_t start = x * q + (x < r ? x : r);
+	return start + b / 8;
+}
+
+__device__ __forceinline__ M33 ldA(const float* A, uint32_t nnz, uint32_t k) {
+	M33 a;
+	float* p = &a.x.x;
+#pragma unroll
+	for (int c = 0; c < 9; ++c) p[c] = A[(size_t)c * nnz + k];
+	return a;
+}
+__device__ __forceinline__ void stA(float* A, uint32_t nnz, uint32_t k, const M33& a) {
+	const float* p = &a.x.x;
+#pragma unroll
+	for (int c = 0; c < 9; ++c) A[(size_t)c * nnz + k] = p[c];
+}
+__device__ __forceinline__ M33 ldM(const float* P, uint32_t m, uint32_t plane0, uint32_t e) {
+	M33 a;
+	float* p = &a.x.x;
+#pragma unroll
+	for (int c = 0; c < 9; ++c) p[c] = P[(size_t)(plane0 + c) * m + e];
+	return a;
+}
+__device__ __forceinline__ V3 ldv(const V3* p, uint32_t i) { return p[i]; }
+
+// Wave64 + LDS block sum of doubles (fixed tree -> deterministic).
+__device__ double block_sum(double v, double* sh) {
+	for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	__syncthreads();
+	if (lane == 0) sh[wave] = v;
+	__syncthreads();
+	double t = 0.0;
+	if (threadIdx.x == 0)
+		for (int k = 0; k < kBlock / 64; ++k) t += sh[k];
+	return t;   // valid in thread 0
+}
+
+enum Finalize : int { kFinAlpha = 0, kFinDelta = 1, kFinInit = 2, kFinNewton = 3 };
+
+__device__ void finalize(SolverState* st, int kind, double total) {
+	float v = (float)total;
+	switch (kind) {
+	case kFinAlpha:   // sparse_solver.cpp:77
+		st->alpha = st->delta_new / v;
+		break;
+	case kFinInit:    // sparse_solver.cpp:57-73
+		st->delta_new = v;
+		st->delta0 = v;
+		st->it = 0;
+		st->beta = 0.0f;
+		st->active = (st->maxIt != 0 && !(v <= st->eps2_sub * v)) ? 1u : 0u;
+		break;
+	case kFinDelta: { // sparse_solver.cpp:92-102
+		float delta_old = st->delta_new;
+		st->delta_new = v;
+		st->beta = v / delta_old;
+		st->it += 1;
+		st->active = (st->it != st->maxIt && !(v <= st->eps2_sub * st->delta0)) ? 1u : 0u;
+		break;
+	}
+	case kFinNewton: { // sparse_force_solver.cpp:196-225
+		st->error = v;
+		st->statMinSub = min(st->statMinSub, st->it);
+		st->statMaxSub = max(st->statMaxSub, st->it);
+		bool conv;
+		if (st->newtonIt == 0) { st->error0 = v; conv = v <= st->eps2; }
+		else conv = v <= st->eps2 * st->error0;
+		if (conv) st->newtonActive = 0;
+		else { st->newtonIt += 1; st->newtonActive = st->newtonIt < st->newtonMax ? 1u : 0u; }
+		st->statNewton = st->newtonIt;
+		break;
+	}
+	}
+}
+
+// Last-arriving workgroup sums the per-workgroup partials in a fixed order.
+__device__ void grid_reduce(double v, double* partial, SolverState* st, int slot, int kind) {
+	__shared__ double sh[kBlock / 64];
+	__shared__ int last;
+	double bs = block_sum(v, sh);
+	if (threadIdx.x == 0) {
+		partial[blockIdx.x] = bs;
+		__threadfence();
+		uint32_t old = atomicAdd(&st->arrive[slot], 1u);
+		last = (old == gridDim.x - 1) ? 1 : 0;
+	}
+	__syncthreads();
+	if (!last) return;
+	__threadfence();
+	double t = 0.0;
+	for (uint32_t k = threadIdx.x; k < gridDim.x; k += kBlock) t += partial[k];
+	double tot = block_sum(t, sh);
+	if (threadIdx.x == 0) {
+		finalize(st, kind, tot);
+		st->arrive[slot] = 0;
+	}
+}
+
+// ---------------------------------------------------------------------------
+// Set-up: tetrahedron rest data (tetrahedron_element_force.cpp:219-265)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_tet_setup(uint32_t m, const float* rest, const float* param, float* invE,
+                                                       float* K0, uint32_t* degenerate) {
+	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+	if (e >= m) return;
+	V3 x[4];
+	for (int k = 0; k < 4; ++k)
+		x[k] = v3(rest[(size_t)(3 * k) * m + e], rest[(size_t)(3 * k + 1) * m + e], rest[(size_t)(3 * k + 2) * m + e]);
+	float E = param[e], nu = param[(size_t)m + e];
+	V3 e1 = x[1] - x[0], e2 = x[2] - x[0], e3 = x[3] - x[0];
+	M33 Em = m33(e1, e2, e3);
+	float dt = det(e1, e2, e3);
+	if (dt == 0.0f) atomicOr(degenerate, 1u);
+	float V = fabs_(dt) / 6.0f;
+	dt = 1.0f / dt;
+	M33 iE = dt * adjugate(Em);
+	const float* pi = &iE.x.x;
+	for (int c = 0; c < 9; ++c) invE[(size_t)c * m + e] = pi[c];
+	// D (b3ComputeD, :37-57)
+	float lambda = (nu * E) / ((1.0f + nu) * (1.0f - 2.0f * nu));
+	float mu = E / (2.0f * (1.0f + nu));
+	float D[36];
+	for (int i = 0; i < 36; ++i) D[i] = 0.0f;
+	D[0] = lambda + 2.0f * mu; D[1] = lambda; D[2] = lambda;
+	D[6] = lambda; D[7] = lambda + 2.0f * mu; D[8] = lambda;
+	D[12] = lambda; D[13] = lambda; D[14] = lambda + 2.0f * mu;
+	D[21] = mu; D[28] = mu; D[35] = mu;
+	// B (b3ComputeB, :121-159), 6 x 12 column-major
+	float g[4][3];
+	g[1][0] = iE.x.x; g[2][0] = iE.x.y; g[3][0] = iE.x.z;
+	g[1][1] = iE.y.x; g[2][1] = iE.y.y; g[3][1] = iE.y.z;
+	g[1][2] = iE.z.x; g[2][2] = iE.z.y; g[3][2] = iE.z.z;
+	for (int a = 0; a < 3; ++a) g[0][a] = -g[1][a] - g[2][a] - g[3][a];
+	float B[72];
+	for (int v = 0; v < 4; ++v) {
+		float dx = g[v][0], dy = g[v][1], dz = g[v][2];
+		float* c0 = B + 6 * (3 * v);
+		float* c1 = B + 6 * (3 * v + 1);
+		float* c2 = B + 6 * (3 * v + 2);
+		c0[0] = dx; c0[1] = 0; c0[2] = 0; c0[3] = dy; c0[4] = 0; c0[5] = dz;
+		c1[0] = 0; c1[1] = dy; c1[2] = 0; c1[3] = dx; c1[4] = dz; c1[5] = 0;
+		c2[0] = 0; c2[1] = 0; c2[2] = dz; c2[3] = 0; c2[4] = dy; c2[5] = dx;
+	}
+	// BT_D = B^T D (12 x 6), then K = BT_D B (12 x 12), times V
+	float BTD[72];
+	for (int i = 0; i < 12; ++i)
+		for (int j = 0; j < 6; ++j) {
+			float acc = 0.0f;
+			for (int k = 0; k < 6; ++k) acc += B[k + 6 * i] * D[k + 6 * j];
+			BTD[i + 12 * j] = acc;
+		}
+	for (int i = 0; i < 12; ++i)
+		for (int j = 0; j < 12; ++j) {
+			float acc = 0.0f;
+			for (int k = 0; k < 6; ++k) acc += BTD[i + 12 * k] * B[k + 6 * j];
+			acc *= V;
+			// b3SetK: block (i/3 + 4 (j/3)), element (i%3 + 3 (j%3)) -> plane
+			int blk = (i / 3) + 4 * (j / 3);
+			int el = (i % 3) + 3 * (j % 3);
+			K0[(size_t)(9 * blk + el) * m + e] = acc;
+		}
+}
+
+// ---------------------------------------------------------------------------
+// Per-step particle kernels
+// ---------------------------------------------------------------------------
+struct StepArgs {
+	uint32_t n;
+	float h;
+	V3 gravity;
+	const uint8_t* type;
+	const float* mass;
+	const float* damping;
+	V3 *x, *v, *force, *translation, *x0, *v0, *fe, *y, *w, *px;
+	SolverState* st;
+};
+
+// force_solver.cpp:100-130 + sparse_force_solver.cpp:135-139
+__global__ void __launch_bounds__(kBlock) k_step_begin(StepArgs a) {
+	uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= a.n) return;
+	V3 x = a.x[i], v = a.v[i];
+	a.x0[i] = x;
+	a.v0[i] = v;
+	V3 fe = a.force[i];
+	if (a.type[i] == kDynamic) fe = fe + a.mass[i] * a.gravity;
+	a.fe[i] = fe;
+	a.y[i] = a.translation[i];
+	a.px[i] = zero3();
+}
+
+__global__ void k_state_begin(SolverState* st, uint32_t iters, uint32_t subIters, float tol, float subTol) {
+	st->newtonIt = 0;
+	st->newtonMax = iters;
+	st->newtonActive = iters > 0 ? 1u : 0u;
+	st->eps2 = tol * tol;
+	st->eps2_sub = subTol * subTol;
+	st->maxIt = subIters;
+	st->active = 0;
+	st->it = 0;
+	st->arrive[0] = st->arrive[1] = st->arrive[2] = st->arrive[3] = 0;
+}
+
+// w = x0 - x + h v + y (sparse_force_solver.cpp:167)
+__global__ void __launch_bounds__(kBlock) k_newton_prep(StepArgs a) {
+	if (!a.st->newtonActive) return;
+	uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= a.n) return;
+	a.w[i] = ((a.x0[i] - a.x[i]) + a.h * a.v[i]) + a.y[i];
+}
+
+// ---------------------------------------------------------------------------
+// Element kernels: stage this linearisation's contributions.
+// ---------------------------------------------------------------------------
+struct TetArgs {
+	uint32_t m;
+	const uint32_t* p;
+	const float* rest;
+	const float* invE;
+	const float* K0;
+	float* q;
+	const float* kd;
+	const V3* x;
+	const V3* v;
+	M33* jstage;
+	V3* fstage;
+	uint32_t jx, jv, f, fd;
+	const SolverState* st;
+};
+
+// tetrahedron_element_force.cpp:273-311 (b3ExtractRotation, 32 iterations)
+__device__ Q4 extract_rotation(const M33& A, Q4 q) {
+	for (int it = 0; it < 32; ++it) {
+		M33 R = qmatrix(q);
+		float den = fabs_(dot(R.x, A.x) + dot(R.y, A.y) + dot(R.z, A.z));
+		if (den == 0.0f) break;
+		V3 torque = cross(R.x, A.x) + cross(R.y, A.y) + cross(R.z, A.z);
+		const float kTol = 1.0e-9f;
+		float inv_den = 1.0f / den + kTol;
+		V3 omega = inv_den * torque;
+		float w = length(omega);
+		if (w < kTol) break;
+		V3 axis = omega / w;
+		float theta = 0.5f * w;
+		// the reference narrows the double-precision sin/cos (quat.h:83-88)
+		double sd, cd;
+		sincos((double)theta, &sd, &cd);
+		Q4 qw;
+		qw.v = (float)sd * axis;
+		qw.s = (float)cd;
+		q = qmul(qw, q);
+		qnormalize(q);
+	}
+	return q;
+}
+
+// tetrahedron_element_force.cpp:313-464
+__global__ void __launch_bounds__(kBlock) k_tet_forces(TetArgs a) {
+	if (!a.st->newtonActive) return;
+	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+	if (e >= a.m) return;
+	const uint32_t m = a.m;
+	uint32_t id[4];
+	V3 p[4], xr[4];
+	for (int k = 0; k < 4; ++k) {
+		id[k] = a.p[(size_t)k * m + e];
+		p[k] = a.x[id[k]];
+		xr[k] = v3(a.rest[(size_t)(3 * k) * m + e], a.rest[(size_t)(3 * k + 1) * m + e], a.rest[(size_t)(3 * k + 2) * m + e]);
+	}
+	M33 Em = m33(p[1] - p[0], p[2] - p[0], p[3] - p[0]);
+	M33 F = Em * ldM(a.invE, m, 0, e);
+	Q4 q;
+	q.v = v3(a.q[e], a.q[(size_t)m + e], a.q[(size_t)2 * m + e]);
+	q.s = a.q[(size_t)3 * m + e];
+	q = extract_rotation(F, q);
+	a.q[e] = q.v.x;
+	a.q[(size_t)m + e] = q.v.y;
+	a.q[(size_t)2 * m + e] = q.v.z;
+	a.q[(size_t)3 * m + e] = q.s;
+	M33 R = qmatrix(q);
+	M33 RT = transpose(R);
+	V3 us[4];
+	for (int k = 0; k < 4; ++k) us[k] = RT * p[k] - xr[k];
+	const float kd = a.kd[e];
+	V3 vs[4];
+	if (kd > 0.0f)
+		for (int k = 0; k < 4; ++k) vs[k] = a.v[id[k]];
+	for (int i = 0; i < 4; ++i) {
+		V3 fs = zero3();
+		V3 fds = zero3();
+		for (int j = 0; j < 4; ++j) {
+			const int blk = i + 4 * j;
+			M33 K0 = ldM(a.K0, m, 9 * blk, e);
+			fs = fs + K0 * us[j];
+			M33 K = R * K0 * RT;
+			a.jstage[a.jx + (size_t)blk * m + e] = -K;     // dfdx(vi, vj) -= K
+			if (kd > 0.0f) {
+				M33 kK = kd * K;
+				fds = fds + kK * vs[j];
+				a.jstage[a.jv + (size_t)blk * m + e] = -kK;  // dfdv(vi, vj) -= kd K
+			}
+		}
+		a.fstage[a.f + (size_t)i * m + e] = -(R * fs);      // f[vi] -= R fs
+		if (kd > 0.0f) a.fstage[a.fd + (size_t)i * m + e] = -fds;
+	}
+}
+
+struct TriArgs {
+	uint32_t m;
+	const uint32_t* p;
+	const float* x2;
+	const float* invS;
+	const float* K0;
+	const float* kd;
+	const V3* x;
+	const V3* v;
+	M33* jstage;
+	V3* fstage;
+	uint32_t jx, jv, f, fd;
+	const SolverState* st;
+};
+
+__device__ M22 extract_rotation2(const M22& M) {   // triangle_element_force.cpp:131-150
+	float m11 = M.x.x, m12 = M.y.x, m21 = M.x.y, m22 = M.y.y;
+	float d = m11 * m22 - m12 * m21;
+	M22 A;
+	A.x.x = M.y.y; A.x.y = -M.y.x; A.y.x = -M.x.y; A.y.y = M.x.x;
+	M22 Q = M + sign_(d) * A;
+	normalize_div(Q.x);
+	normalize_div(Q.y);
+	return Q;
+}
+
+// triangle_element_force.cpp:285-468
+__global__ void __launch_bounds__(kBlock) k_tri_forces(TriArgs a) {
+	if (!a.st->newtonActive) return;
+	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+	if (e >= a.m) return;
+	const uint32_t m = a.m;
+	uint32_t id[3];
+	V3 p[3];
+	for (int k = 0; k < 3; ++k) { id[k] = a.p[(size_t)k * m + e]; p[k] = a.x[id[k]]; }
+	V3 n = cross(p[1] - p[0], p[2] - p[0]);
+	normalize_inplace(n);
+	V3 px, py;
+	compute_basis(n, px, py);
+	float P[6] = {px.x, py.x, px.y, py.y, px.z, py.z};
+	float PT[6] = {px.x, px.y, px.z, py.x, py.y, py.z};
+	V2 x[3];
+	for (int k = 0; k < 3; ++k) gmul<2, 3, 1>(&x[k].x, P, &p[k].x);
+	M22 T;
+	T.x = x[1] - x[0];
+	T.y = x[2] - x[0];
+	M22 iS;
+	iS.x = v2(a.invS[e], a.invS[(size_t)m + e]);
+	iS.y = v2(a.invS[(size_t)2 * m + e], a.invS[(size_t)3 * m + e]);
+	M22 R = extract_rotation2(T * iS);
+	M22 RT = transpose(R);
+	V2 us[3];
+	for (int k = 0; k < 3; ++k) us[k] = RT * x[k] - v2(a.x2[(size_t)(2 * k) * m + e], a.x2[(size_t)(2 * k + 1) * m + e]);
+	auto K0 = [&](int blk) {
+		M22 k;
+		k.x = v2(a.K0[(size_t)(4 * blk) * m + e], a.K0[(size_t)(4 * blk + 1) * m + e]);
+		k.y = v2(a.K0[(size_t)(4 * blk + 2) * m + e], a.K0[(size_t)(4 * blk + 3) * m + e]);
+		return k;
+	};
+	for (int i = 0; i < 3; ++i) {
+		V2 fs = v2(0.0f, 0.0f);
+		for (int j = 0; j < 3; ++j) fs = fs + K0(i + 3 * j) * us[j];
+		fs = R * fs;
+		V3 f3;
+		gmul<3, 2, 1>(&f3.x, PT, &fs.x);
+		a.fstage[a.f + (size_t)i * m + e] = -f3;   // f[vi] -= f
+	}
+	const float kd = a.kd[e];
+	V3 vs[3];
+	if (kd > 0.0f)
+		for (int k = 0; k < 3; ++k) vs[k] = a.v[id[k]];
+	V3 fds[3] = {zero3(), zero3(), zero3()};
+	for (int i = 0; i < 3; ++i)
+		for (int j = 0; j < 3; ++j) {
+			const int blk = i + 3 * j;
+			M22 k = R * K0(blk) * RT;
+			float PT_k[6];
+			gmul<3, 2, 2>(PT_k, PT, &k.x.x);
+			M33 Ke;
+			gmul<3, 2, 3>(&Ke.x.x, PT_k, P);
+			a.jstage[a.jx + (size_t)blk * m + e] = -Ke;
+			if (kd > 0.0f) {
+				M33 kK = kd * Ke;
+				fds[i] = fds[i] + kK * vs[j];
+				a.jstage[a.jv + (size_t)blk * m + e] = -kK;
+			}
+		}
+	if (kd > 0.0f)
+		for (int i = 0; i < 3; ++i) a.fstage[a.fd + (size_t)i * m + e] = -fds[i];
+}
+
+struct UVArgs {
+	uint32_t m;
+	const uint32_t* p;
+	const float* c;
+	const V3* x;
+	const V3* v;
+	M33* jstage;
+	V3* fstage;
+	uint32_t jx, jv, f;
+	const SolverState* st;
+};
+
+__device__ __forceinline__ float coef(const UVArgs& a, int k, uint32_t e) { return a.c[(size_t)k * a.m + e]; }
+
+// stretch_force.cpp:121-389: parts u (0) and v (1)
+__global__ void __launch_bounds__(kBlock) k_stretch_forces(UVArgs a) {
+	if (!a.st->newtonActive) return;
+	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+	if (e >= a.m) return;
+	const uint32_t m = a.m;
+	float alpha = coef(a, 0, e), du1 = coef(a, 1, e), dv1 = coef(a, 2, e), du2 = coef(a, 3, e), dv2 = coef(a, 4, e);
+	float inv_det = coef(a, 5, e);
+	V3 dwudx = v3(coef(a, 6, e), coef(a, 7, e), coef(a, 8, e));
+	V3 dwvdx = v3(coef(a, 9, e), coef(a, 10, e), coef(a, 11, e));
+	uint32_t id[3];
+	V3 xs[3], vs[3];
+	for (int k = 0; k < 3; ++k) { id[k] = a.p[(size_t)k * m + e]; xs[k] = a.x[id[k]]; vs[k] = a.v[id[k]]; }
+	M33 I = mdiag(1.0f);
+	V3 dx1 = xs[1] - xs[0], dx2 = xs[2] - xs[0];
+	V3 wu = inv_det * (dv2 * dx1 - dv1 * dx2);
+	float len_wu = length(wu);
+	V3 wv = inv_det * (-du2 * dx1 + du1 * dx2);
+	float len_wv = length(wv);
+	for (int part = 0; part < 2; ++part) {
+		V3 wvec = part == 0 ? wu : wv;
+		float len = part == 0 ? len_wu : len_wv;
+		V3 dwdx = part == 0 ? dwudx : dwvdx;
+		float ks = coef(a, 12 + 3 * part, e), kd = coef(a, 13 + 3 * part, e), bb = coef(a, 14 + 3 * part, e);
+		bool live = len > 0.0f;
+		float inv_len = live ? 1.0f / len : 0.0f;
+		V3 nw = inv_len * wvec;
+		V3 dCdx[3];
+		for (int i = 0; i < 3; ++i) dCdx[i] = alpha * comp(dwdx, i) * nw;
+		if (ks > 0.0f) {
+			float C = alpha * (len - bb);
+			for (int i = 0; i < 3; ++i)
+				a.fstage[a.f + (size_t)(6 * part + i) * m + e] = live ? -ks * C * dCdx[i] : zero3();
+			for (int i = 0; i < 3; ++i)
+				for (int j = 0; j < 3; ++j) {
+					M33 Kij = outer(dCdx[i], dCdx[j]);
+					if (len > bb) {
+						M33 d2 = (alpha * inv_len * comp(dwdx, i) * comp(dwdx, j)) * (I - outer(nw, nw));
+						Kij = Kij + C * d2;
+					}
+					a.jstage[a.jx + (size_t)(9 * part + 3 * i + j) * m + e] = live ? -ks * Kij : mzero();
+				}
+		}
+		if (kd > 0.0f) {
+			float dCdt = 0.0f;
+			for (int i = 0; i < 3; ++i) dCdt += dot(dCdx[i], vs[i]);
+			for (int i = 0; i < 3; ++i)
+				a.fstage[a.f + (size_t)(6 * part + 3 + i) * m + e] = live ? -kd * dCdt * dCdx[i] : zero3();
+			for (int i = 0; i < 3; ++i)
+				for (int j = 0; j < 3; ++j)
+					a.jstage[a.jv + (size_t)(9 * part + 3 * i + j) * m + e] = live ? -kd * outer(dCdx[i], dCdx[j]) : mzero();
+		}
+	}
+}
+
+// shear_force.cpp:106-245
+__global__ void __launch_bounds__(kBlock) k_shear_forces(UVArgs a) {
+	if (!a.st->newtonActive) return;
+	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+	if (e >= a.m) return;
+	const uint32_t m = a.m;
+	float alpha = coef(a, 0, e), du1 = coef(a, 1, e), dv1 = coef(a, 2, e), du2 = coef(a, 3, e), dv2 = coef(a, 4, e);
+	float inv_det = coef(a, 5, e);
+	V3 dwudx = v3(coef(a, 6, e), coef(a, 7, e), coef(a, 8, e));
+	V3 dwvdx = v3(coef(a, 9, e), coef(a, 10, e), coef(a, 11, e));
+	float ks = coef(a, 12, e), kd = coef(a, 13, e);
+	uint32_t id[3];
+	V3 xs[3], vs[3];
+	for (int k = 0; k < 3; ++k) { id[k] = a.p[(size_t)k * m + e]; xs[k] = a.x[id[k]]; vs[k] = a.v[id[k]]; }
+	V3 dx1 = xs[1] - xs[0], dx2 = xs[2] - xs[0];
+	V3 wu = inv_det * (dv2 * dx1 - dv1 * dx2);
+	V3 wv = inv_det * (-du2 * dx1 + du1 * dx2);
+	V3 dCdx[3];
+	for (int i = 0; i < 3; ++i) dCdx[i] = alpha * (comp(dwudx, i) * wv + comp(dwvdx, i) * wu);
+	if (ks > 0.0f) {
+		float C = alpha * dot(wu, wv);
+		for (int i = 0; i < 3; ++i) a.fstage[a.f + (size_t)i * m + e] = -ks * C * dCdx[i];
+		for (int i = 0; i < 3; ++i)
+			for (int j = 0; j < 3; ++j) a.jstage[a.jx + (size_t)(3 * i + j) * m + e] = -ks * outer(dCdx[i], dCdx[j]);
+	}
+	if (kd > 0.0f) {
+		float dCdt = 0.0f;
+		for (int i = 0; i < 3; ++i) dCdt += dot(dCdx[i], vs[i]);
+		for (int i = 0; i < 3; ++i) a.fstage[a.f + (size_t)(3 + i) * m + e] = -kd * dCdt * dCdx[i];
+		for (int i = 0; i < 3; ++i)
+			for (int j = 0; j < 3; ++j) a.jstage[a.jv + (size_t)(3 * i + j) * m + e] = -kd * outer(dCdx[i], dCdx[j]);
+	}
+}
+
+struct PairArgs {
+	uint32_t m;
+	const uint32_t* p;
+	const float* c;
+	const V3* x;
+	const V3* v;
+	M33* jstage;
+	V3* fstage;
+	uint32_t jx, jv, f;
+	const SolverState* st;
+};
+
+// spring_force.cpp:61-143
+__global__ void __launch_bounds__(kBlock) k_spring_forces(PairArgs a) {
+	if (!a.st->newtonActive) return;
+	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+	if (e >= a.m) return;
+	const uint32_t m = a.m;
+	uint32_t i1 = a.p[e], i2 = a.p[(size_t)m + e];
+	float L0 = a.c[e], ks = a.c[(size_t)m + e], kd = a.c[(size_t)2 * m + e];
+	V3 x1 = a.x[i1], v1 = a.v[i1], x2 = a.x[i2], v2_ = a.v[i2];
+	M33 I = mdiag(1.0f);
+	V3 dx = x1 - x2;
+	float L = length(dx);
+	bool live = L > 0.0f;
+	V3 n = live ? dx / L : zero3();
+	if (ks > 0.0f) {
+		bool on = live && L > L0;
+		V3 f1 = zero3(), f2 = zero3();
+		M33 K11 = mzero(), K12 = mzero();
+		if (on) {
+			float C = L - L0;
+			f1 = -ks * C * n;
+			f2 = -f1;
+			K11 = -ks * (outer(dx, dx) + (1.0f - L0 / L) * (I - outer(dx, dx)));
+			K12 = -K11;
+		}
+		a.fstage[a.f + e] = f1;
+		a.fstage[a.f + (size_t)m + e] = f2;
+		a.jstage[a.jx + e] = K11;
+		a.jstage[a.jx + (size_t)m + e] = K12;
+		a.jstage[a.jx + (size_t)2 * m + e] = K12;
+		a.jstage[a.jx + (size_t)3 * m + e] = K11;
+	}
+	if (kd > 0.0f) {
+		V3 f1 = zero3(), f2 = zero3();
+		M33 K11 = mzero(), K12 = mzero();
+		if (live) {
+			float dCdt = dot(n, v1 - v2_);
+			f1 = -kd * dCdt * n;
+			f2 = -f1;
+			K11 = -kd * outer(n, n);
+			K12 = -K11;
+		}
+		a.fstage[a.f + (size_t)2 * m + e] = f1;
+		a.fstage[a.f + (size_t)3 * m + e] = f2;
+		a.jstage[a.jv + e] = K11;
+		a.jstage[a.jv + (size_t)m + e] = K12;
+		a.jstage[a.jv + (size_t)2 * m + e] = K12;
+		a.jstage[a.jv + (size_t)3 * m + e] = K11;
+	}
+}
+
+// mouse_force.cpp:58-246
+__global__ void __launch_bounds__(kBlock) k_mouse_forces(PairArgs a) {
+	if (!a.st->newtonActive) return;
+	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+	if (e >= a.m) return;
+	const uint32_t m = a.m;
+	uint32_t id[4];
+	V3 xs[4], vs[4];
+	for (int k = 0; k < 4; ++k) { id[k] = a.p[(size_t)k * m + e]; xs[k] = a.x[id[k]]; vs[k] = a.v[id[k]]; }
+	float w2 = a.c[e], w3 = a.c[(size_t)m + e], w4 = a.c[(size_t)2 * m + e];
+	float L0 = a.c[(size_t)3 * m + e], ks = a.c[(size_t)4 * m + e], kd = a.c[(size_t)5 * m + e];
+	M33 I = mdiag(1.0f);
+	V3 c2 = w2 * xs[1] + w3 * xs[2] + w4 * xs[3];
+	V3 d = xs[0] - c2;
+	float L = length(d);
+	bool live = L > 0.0f;
+	float inv_L = live ? 1.0f / L : 0.0f;
+	V3 n = inv_L * d;
+	V3 dCdx[4] = {n, -w2 * n, -w3 * n, -w4 * n};
+	if (ks > 0.0f) {
+		bool on = live && L > L0;
+		float C = L - L0;
+		float L3 = L * L * L;
+		float inv_L3 = L3 > 0.0f ? 1.0f / L3 : 0.0f;
+		M33 A = inv_L * I - inv_L3 * outer(d, d);
+		M33 B = inv_L3 * outer(d, d) - inv_L * I;
+		const float wv[4] = {1.0f, w2, w3, w4};
+		for (int i = 0; i < 4; ++i) a.fstage[a.f + (size_t)i * m + e] = on ? -ks * C * dCdx[i] : zero3();
+		for (int i = 0; i < 4; ++i)
+			for (int j = 0; j < 4; ++j) {
+				M33 d2;
+				if (i == 0) d2 = j == 0 ? A : -wv[j] * A;
+				else if (j == 0) d2 = wv[i] * B;
+				else d2 = -wv[i] * wv[j] * B;
+				if (i > 0 && j > 0 && i > j) d2 = -wv[j] * wv[i] * B;   // d2Cdx[i][j] = -w_j w_i B below the diagonal
+				a.jstage[a.jx + (size_t)(4 * i + j) * m + e] = on ? -ks * (outer(dCdx[i], dCdx[j]) + C * d2) : mzero();
+			}
+	}
+	if (kd > 0.0f) {
+		V3 vc2 = w2 * vs[1] + w3 * vs[2] + w4 * vs[3];
+		float dCdt = dot(vs[0] - vc2, n);
+		for (int i = 0; i < 4; ++i) a.fstage[a.f + (size_t)(4 + i) * m + e] = live ? -kd * dCdt * dCdx[i] : zero3();
+		for (int i = 0; i < 4; ++i)
+			for (int j = 0; j < 4; ++j)
+				a.jstage[a.jv + (size_t)(4 * i + j) * m + e] = live ? -kd * outer(dCdx[i], dCdx[j]) : mzero();
+	}
+}
+
+// ---------------------------------------------------------------------------
+// Contacts
+// ---------------------------------------------------------------------------
+struct ContactArgs {
+	uint32_t nSpheres, nWorld;
+	const WorldShape* worlds;
+	const float* vox;
+	const uint32_t* sphereParticle;
+	const float* sphereRadius;
+	const float* sphereFriction;
+	const uint8_t* type;
+	const float* invMass;
+	V3* x;
+	V3* v;
+	uint32_t* active;
+	uint32_t* stamp;
+	uint32_t* applyFriction;
+	float* normalForce;
+	V3* manifold;   // 2 per pair: point, normal
+	uint32_t* hit;
+	V3* cF;
+	M33* cK;
+	uint32_t stampBase;
+	float dt;
+	SolverState* st;
+};
+
+// contact_manager.cpp:108-146
+__global__ void __launch_bounds__(kBlock) k_update_contacts(ContactArgs a) {
+	uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+	if (k >= a.nSpheres * a.nWorld) return;
+	if (!a.active[k]) return;
+	uint32_t s = k / a.nWorld, w = k % a.nWorld;
+	uint32_t p = a.sphereParticle[s];
+	bool keep = a.type[p] == kDynamic && overlap(aabb_cr(a.x[p], a.sphereRadius[s]), a.worlds[w].aabb);
+	if (!keep) { a.active[k] = 0; return; }
+	a.normalForce[k] = 0.0f;      // sphere_shape_contact.cpp:191-195
+	a.applyFriction[k] = 0;
+}
+
+// contact_manager.cpp:65-82 (pairs are keyed (sphere, world); stamps order the list)
+__global__ void __launch_bounds__(kBlock) k_find_contacts(ContactArgs a) {
+	uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+	if (k >= a.nSpheres * a.nWorld) return;
+	if (a.active[k]) return;
+	uint32_t s = k / a.nWorld, w = k % a.nWorld;
+	uint32_t p = a.sphereParticle[s];
+	if (!overlap(aabb_cr(a.x[p], a.sphereRadius[s]), a.worlds[w].aabb)) return;
+	if (a.type[p] != kDynamic) return;
+	a.active[k] = 1;
+	a.stamp[k] = a.stampBase + w;
+	a.normalForce[k] = 0.0f;
+	a.applyFriction[k] = 0;
+	a.manifold[2 * k] = zero3();
+	a.manifold[2 * k + 1] = zero3();
+}
+
+// sphere_shape_contact.cpp:53-143
+__global__ void __launch_bounds__(kBlock) k_contact_forces(ContactArgs a) {
+	if (!a.st->newtonActive) return;
+	uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+	if (k >= a.nSpheres * a.nWorld) return;
+	a.hit[k] = 0;
+	if (!a.active[k]) return;
+	uint32_t s = k / a.nWorld, w = k % a.nWorld;
+	uint32_t p = a.sphereParticle[s];
+	const WorldShape& ws = a.worlds[w];
+	V3 x1 = a.x[p];
+	float r1 = a.sphereRadius[s], r2 = ws.radius;
+	Manifold m2;
+	if (!collide(ws, a.vox, x1, r1, &m2)) return;
+	if (!a.applyFriction[k]) {
+		a.manifold[2 * k] = m2.point;
+		a.manifold[2 * k + 1] = m2.normal;
+		a.applyFriction[k] = 1;
+	}
+	V3 x2 = m2.point, n2 = m2.normal, n1 = -n2;
+	V3 c1 = x1 + r1 * n1;
+	V3 c2 = x2 + r2 * n2;
+	float C = length(c2 - c1);
+	C = fmin_(0.2f * C, 0.2f);                   // B3_BAUMGARTE, B3_MAX_CONTACT_LINEAR_CORRECTION
+	V3 f1 = -1000.0f * C * n1;                   // B3_CONTACT_STIFFNESS
+	M33 K11 = -1000.0f * (outer(n1, n1) + C * (mdiag(1.0f) - outer(n1, n1)));
+	a.cF[k] = f1;
+	a.cK[k] = K11;
+	a.hit[k] = 1;
+	a.normalForce[k] += length(f1);
+}
+
+// Contacts of one sphere in the reference's list order (newest first).
+__device__ int contact_order(const ContactArgs& a, uint32_t s, uint32_t* order) {
+	int cnt = 0;
+	for (uint32_t w = 0; w < a.nWorld; ++w)
+		if (a.active[s * a.nWorld + w]) order[cnt++] = w;
+	for (int i = 1; i < cnt; ++i) {   // insertion sort by stamp, descending
+		uint32_t key = order[i];
+		int j = i - 1;
+		while (j >= 0 && a.stamp[s * a.nWorld + order[j]] < a.stamp[s * a.nWorld + key]) { order[j + 1] = order[j]; --j; }
+		order[j + 1] = key;
+	}
+	return cnt;
+}
+
+// sphere_shape_contact.cpp:145-189, contacts of a particle in list order
+__global__ void __launch_bounds__(kBlock) k_friction(ContactArgs a) {
+	uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+	if (s >= a.nSpheres) return;
+	uint32_t order[16];
+	int cnt = contact_order(a, s, order);
+	if (cnt == 0) return;
+	uint32_t p = a.sphereParticle[s];
+	V3 v1 = a.v[p];
+	float im1 = a.invMass[p];
+	for (int c = 0; c < cnt; ++c) {
+		uint32_t k = s * a.nWorld + order[c];
+		if (!a.applyFriction[k]) continue;
+		V3 normal = a.manifold[2 * k + 1];
+		float normalForce = a.normalForce[k];
+		V3 t1 = perp(normal);
+		V3 t2 = cross(t1, normal);
+		float friction = sqrt_(a.sphereFriction[s] * a.worlds[order[c]].friction);   // contact.h:28-31
+		float tangentMass = im1 > 0.0f ? 1.0f / im1 : 0.0f;
+		V2 Cdot = v2(dot(v1, t1), dot(v1, t2));
+		V2 impulse = tangentMass * -Cdot;
+		float normalImpulse = a.dt * normalForce;
+		float maxImpulse = friction * normalImpulse;
+		if (dot(impulse, impulse) > maxImpulse * maxImpulse) {
+			normalize_div(impulse);
+			impulse.x *= maxImpulse;
+			impulse.y *= maxImpulse;
+		}
+		V3 P = impulse.x * t1 + impulse.y * t2;
+		v1 = v1 + im1 * P;
+	}
+	a.v[p] = v1;
+}
+
+// ---------------------------------------------------------------------------
+// Assembly of pA, pb (sparse_force_solver.cpp:165-174)
+// ---------------------------------------------------------------------------
+struct AsmArgs {
+	uint32_t n, nnz, nWorld;
+	float h;
+	const uint32_t* slotRow;
+	const uint32_t* cols;
+	const uint32_t* jxPtr;
+	const uint32_t* jxIdx;
+	const uint32_t* jvPtr;
+	const uint32_t* jvIdx;
+	const uint32_t* bPtr;
+	const uint32_t* bSlot;
+	const uint32_t* fPtr;
+	const uint32_t* fIdx;
+	const uint32_t* diagSlot;
+	const M33* jstage;
+	const V3* fstage;
+	const uint8_t* type;
+	const float* mass;
+	const float* damping;
+	const int32_t* sphereOf;
+	const V3* x0;
+	const V3* v0;
+	const V3* v;
+	const V3* fe;
+	const V3* w;
+	float* A;
+	V3* tB;
+	V3* pb;
+	V3* invM;
+	V3* fi;
+	ContactArgs c;
+	const SolverState* st;
+};
+
+__global__ void __launch_bounds__(kBlock) k_assemble_slots(AsmArgs a) {
+	if (!a.st->newtonActive) return;
+	uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+	if (s >= a.nnz) return;
+	uint32_t i = a.slotRow[s], j = a.cols[s];
+	// dfdx(i, j): staged contributions in list order, then contacts (force_solver.cpp:70-86)
+	M33 dx = mzero();
+	for (uint32_t k = a.jxPtr[s]; k < a.jxPtr[s + 1]; ++k) dx = dx + a.jstage[a.jxIdx[k]];
+	const bool dyn_i = a.type[i] == kDynamic;
+	if (i == j && a.nWorld && a.sphereOf[i] >= 0 && dyn_i) {
+		uint32_t order[16];
+		uint32_t sph = (uint32_t)a.sphereOf[i];
+		int cnt = contact_order(a.c, sph, order);
+		for (int c = 0; c < cnt; ++c) {
+			uint32_t k = sph * a.nWorld + order[c];
+			if (a.c.hit[k]) dx = dx + a.c.cK[k];
+		}
+	}
+	// dfdv(i, j): particle mass damping first, then staged contributions
+	M33 dv = mzero();
+	if (i == j && a.damping[i] > 0.0f) dv = dv + mdiag(-a.damping[i] * a.mass[i]);
+	for (uint32_t k = a.jvPtr[s]; k < a.jvPtr[s + 1]; ++k) dv = dv + a.jstage[a.jvIdx[k]];
+	// A = M - h dfdv - (h h) dfdx, block by block
+	const float h = a.h, hh = h * h;
+	M33 Mij = i == j ? mdiag(dyn_i ? a.mass[i] : 1.0f) : mzero();
+	M33 Aij = (Mij - h * dv) - hh * dx;
+	// pA = S A S^T + I - S (S = I for dynamic particles, 0 otherwise)
+	const bool dyn_j = a.type[j] == kDynamic;
+	M33 Si = dyn_i ? mdiag(1.0f) : mzero();
+	M33 Sj = dyn_j ? mdiag(1.0f) : mzero();
+	M33 P = (Si * Aij) * Sj;
+	if (i == j) P = (P + mdiag(1.0f)) - Si;
+	stA(a.A, a.nnz, s, P);
+	// (h dfdx_ij) w_j for b's product
+	a.tB[s] = (h * dx) * a.w[j];
+}
+
+__global__ void __launch_bounds__(kBlock) k_assemble_rows(AsmArgs a) {
+	if (!a.st->newtonActive) return;
+	uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= a.n) return;
+	const bool dyn = a.type[i] == kDynamic;
+	const float mass = a.mass[i];
+	V3 vi = a.v[i];
+	// fi: particle damping, forces in list order, contacts (force_solver.cpp:70-86)
+	V3 fi = zero3();
+	if (a.damping[i] > 0.0f) fi = fi + (-a.damping[i] * mass) * vi;
+	for (uint32_t k = a.fPtr[i]; k < a.fPtr[i + 1]; ++k) fi = fi + a.fstage[a.fIdx[k]];
+	if (a.nWorld && a.sphereOf[i] >= 0 && dyn) {
+		uint32_t order[16];
+		uint32_t sph = (uint32_t)a.sphereOf[i];
+		int cnt = contact_order(a.c, sph, order);
+		for (int c = 0; c < cnt; ++c) {
+			uint32_t k = sph * a.nWorld + order[c];
+			if (a.c.hit[k]) fi = fi + a.c.cF[k];
+		}
+	}
+	a.fi[i] = fi;
+	// b = M (v0 - v) + h (fe + fi) + h dfdx (x0 - x + h v + y)
+	const float h = a.h;
+	M33 M = mdiag(dyn ? mass : 1.0f);
+	V3 b = M * (a.v0[i] - vi) + h * (a.fe[i] + fi);
+	V3 hdw = zero3();
+	for (uint32_t k = a.bPtr[i]; k < a.bPtr[i + 1]; ++k) hdw = hdw + a.tB[a.bSlot[k]];
+	b = b + hdw;
+	// pb = S (b - A z), z = 0
+	M33 S = dyn ? mdiag(1.0f) : mzero();
+	a.pb[i] = S * b;
+	// Jacobi preconditioner from pA(i, i) (sparse_solver.cpp:40-55)
+	uint32_t ds = a.diagSlot[i];
+	float axx = a.A[ds], ayy = a.A[(size_t)4 * a.nnz + ds], azz = a.A[(size_t)8 * a.nnz + ds];
+	a.invM[i] = v3(1.0f / axx, 1.0f / ayy, 1.0f / azz);
+}
+
+// ---------------------------------------------------------------------------
+// PCG (sparse_solver.cpp:28-109)
+// ---------------------------------------------------------------------------
+struct CGArgs {
+	uint32_t n, nnz, nTiles;
+	const uint32_t* tiles;
+	const uint32_t* rowptr;
+	const uint32_t* cols;
+	const float* A;
+	const V3* pb;
+	const V3* invM;
+	V3* px;
+	V3* r;
+	V3* s;
+	V3* q;
+	const V3* dold;
+	V3* dnew;
+	double* partial;
+	SolverState* st;
+	const SolverState* newton;
+};
+
+__device__ __forceinline__ V3 diag_apply(V3 im, V3 r) { return mdiag3(im.x, im.y, im.z) * r; }
+
+// One SpMV tile: products of the tile's blocks with the gathered vector go to
+// LDS, then each row thread adds its products in CSR (reference) order.
+template <class Gather>
+__device__ __forceinline__ V3 tile_rows(const CGArgs& a, uint32_t t, uint32_t& row, Gather gather, float* lds) {
+	const uint32_t r0 = a.tiles[t], r1 = a.tiles[t + 1];
+	const uint32_t b0 = a.rowptr[r0], b1 = a.rowptr[r1];
+	row = r0 + threadIdx.x;
+	const bool own = row < r1;
+	uint32_t rb = own ? a.rowptr[row] : 0, re = own ? a.rowptr[row + 1] : 0;
+	V3 acc = zero3();
+	for (uint32_t cb = b0; cb < b1; cb += kTileCap) {
+		const uint32_t ce = min(cb + (uint32_t)kTileCap, b1);
+		for (uint32_t k = cb + threadIdx.x; k < ce; k += kBlock) {
+			M33 blk = ldA(a.A, a.nnz, k);
+			V3 p = blk * gather(a.cols[k]);
+			lds[3 * (k - cb)] = p.x;
+			lds[3 * (k - cb) + 1] = p.y;
+			lds[3 * (k - cb) + 2] = p.z;
+		}
+		__syncthreads();
+		if (own) {
+			const uint32_t kb = max(rb, cb), ke = min(re, ce);
+			for (uint32_t k = kb; k < ke; ++k) acc = acc + v3(lds[3 * (k - cb)], lds[3 * (k - cb) + 1], lds[3 * (k - cb) + 2]);
+		}
+		__syncthreads();
+	}
+	return acc;
+}
+
+// r = b - A x ; s = invM r ; delta = r.s   (mode kFinInit: CG start, kFinDelta: true-residual refresh)
+__global__ void __launch_bounds__(kBlock) k_cg_residual(CGArgs a, int mode) {
+	if (!a.newton->newtonActive) return;
+	if (mode == kFinDelta && !a.st->active) return;
+	__shared__ float lds[3 * kTileCap];
+	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+	uint32_t row;
+	V3 ax = tile_rows(a, t, row, [&](uint32_t c) { return a.px[c]; }, lds);
+	double part = 0.0;
+	if (row < a.tiles[t + 1]) {
+		V3 r = a.pb[row] - ax;
+		V3 s = diag_apply(a.invM[row], r);
+		a.r[row] = r;
+		a.s[row] = s;
+		part = (double)dot(r, s);
+	}
+	grid_reduce(part, a.partial, a.st, 0, mode);
+}
+
+// q = A d with d = s + beta d_old formed on the fly (d = s at the first
+// iteration), d written for the tile's rows; alpha = delta / d.q
+__global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, int first) {
+	if (!a.st->active) return;
+	__shared__ float lds[3 * kTileCap];
+	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+	const float beta = a.st->beta;
+	auto dval = [&](uint32_t c) { return first ? a.s[c] : a.s[c] + beta * a.dold[c]; };
+	uint32_t row;
+	V3 q = tile_rows(a, t, row, dval, lds);
+	double part = 0.0;
+	if (row < a.tiles[t + 1]) {
+		V3 d = dval(row);
+		a.dnew[row] = d;
+		a.q[row] = q;
+		part = (double)dot(d, q);
+	}
+	grid_reduce(part, a.partial, a.st, 1, kFinAlpha);
+}
+
+// x += alpha d ; r -= alpha q ; s = invM r ; delta = r.s
+__global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a) {
+	if (!a.st->active) return;
+	uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	const float alpha = a.st->alpha;
+	double part = 0.0;
+	if (i < a.n) {
+		V3 d = a.dnew[i];
+		a.px[i] = a.px[i] + alpha * d;
+		V3 r = a.r[i] - alpha * a.q[i];
+		V3 s = diag_apply(a.invM[i], r);
+		a.r[i] = r;
+		a.s[i] = s;
+		part = (double)dot(r, s);
+	}
+	grid_reduce(part, a.partial, a.st, 2, kFinDelta);
+}
+
+// x += alpha d (iterations that refresh the true residual)
+__global__ void __launch_bounds__(kBlock) k_cg_update_x(CGArgs a) {
+	if (!a.st->active) return;
+	uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= a.n) return;
+	a.px[i] = a.px[i] + a.st->alpha * a.dnew[i];
+}
+
+// v += dv ; x = x0 + h v + y ; error = |dv|^2 (sparse_force_solver.cpp:193-223)
+__global__ void __launch_bounds__(kBlock) k_newton_update(StepArgs s, double* partial) {
+	if (!s.st->newtonActive) return;
+	uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	double part = 0.0;
+	if (i < s.n) {
+		V3 dv = s.px[i] + zero3();
+		V3 v = s.v[i] + dv;
+		s.v[i] = v;
+		s.x[i] = (s.x0[i] + s.h * v) + s.y[i];
+		part = (double)length_sq(dv);
+	}
+	grid_reduce(part, partial, s.st, 3, kFinNewton);
+}
+
+__global__ void __launch_bounds__(kBlock) k_clear_external(uint32_t n, V3* force, V3* translation) {
+	uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= n) return;
+	force[i] = zero3();
+	translation[i] = zero3();
+}
+
+// Plain SpMV y = A x (diagnostics / operator-level entry point), reference row order.
+__global__ void __launch_bounds__(kBlock) k_spmv(CGArgs a, const V3* xin, V3* yout) {
+	__shared__ float lds[3 * kTileCap];
+	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+	uint32_t row;
+	V3 y = tile_rows(a, t, row, [&](uint32_t c) { return xin[c]; }, lds);
+	if (row < a.tiles[t + 1]) yout[row] = y;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+static inline uint32_t nblk(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+#define LAUNCH(kernel, grid, ...)                                                                    \
+	do {                                                                                             \
+		if ((grid) > 0) hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, b.stream, __VA_ARGS__); \
+		B3G_HIP(hipGetLastError());                                                                  \
+	} while (0)
+
+b3g_status build_tiles(Body& b) {
+	std::vector<uint32_t> t;
+	t.push_back(0);
+	uint32_t rows = 0, blocks = 0;
+	for (uint32_t i = 0; i < b.n; ++i) {
+		uint32_t len = b.hostRowptr[i + 1] - b.hostRowptr[i];
+		if (rows > 0 && (rows == (uint32_t)kTileRows || blocks + len > (uint32_t)kTileCap)) {
+			t.push_back(i);
+			rows = 0;
+			blocks = 0;
+		}
+		rows += 1;
+		blocks += len;
+	}
+	t.push_back(b.n);
+	b.nTiles = (uint32_t)t.size() - 1;
+	b3g_status s = dupload(b, b.tiles, t.data(), t.size() * 4);
+	if (s != B3G_OK) return s;
+	return ensure_partials(b, std::max(b.nTiles, nblk(b.n)));
+}
+
+b3g_status ensure_partials(Body& b, uint32_t blocks) {
+	if (b.partialCount >= blocks && b.partial.p) return B3G_OK;
+	b3g_status s = dalloc(b, b.partial, (size_t)blocks * sizeof(double));
+	if (s != B3G_OK) return s;
+	b.partialCount = blocks;
+	return B3G_OK;
+}
+
+b3g_status launch_tet_setup(Body& b) {
+	DBuf flag;
+	b3g_status s = dalloc(b, flag, 4);
+	if (s != B3G_OK) return s;
+	B3G_HIP(hipMemsetAsync(flag.p, 0, 4, b.stream));
+	LAUNCH(k_tet_setup, nblk(b.tet.m), b.tet.m, dptr<const float>(b.tet.rest), dptr<const float>(b.tet.param),
+	       dptr<float>(b.tet.invE), dptr<float>(b.tet.K0), dptr<uint32_t>(flag));
+	uint32_t deg = 0;
+	B3G_HIP(hipMemcpyAsync(&deg, flag.p, 4, hipMemcpyDeviceToHost, b.stream));
+	B3G_HIP(hipStreamSynchronize(b.stream));
+	(void)hipFree(flag.p);
+	b.owned.erase(std::remove(b.owned.begin(), b.owned.end(), &flag), b.owned.end());
+	if (deg) {
+		set_error("degenerate tetrahedron element (zero volume)");
+		return B3G_DEGENERATE_ELEMENT;
+	}
+	return B3G_OK;
+}
+
+static StepArgs step_args(Body& b, float h, const float g[3]) {
+	StepArgs a;
+	a.n = b.n;
+	a.h = h;
+	a.gravity = g ? v3(g[0], g[1], g[2]) : zero3();
+	a.type = dptr<uint8_t>(b.type);
+	a.mass = dptr<float>(b.mass);
+	a.damping = dptr<float>(b.damping);
+	a.x = dptr<V3>(b.x); a.v = dptr<V3>(b.v); a.force = dptr<V3>(b.force); a.translation = dptr<V3>(b.translation);
+	a.x0 = dptr<V3>(b.x0); a.v0 = dptr<V3>(b.v0); a.fe = dptr<V3>(b.fe); a.y = dptr<V3>(b.y); a.w = dptr<V3>(b.w);
+	a.px = dptr<V3>(b.px);
+	a.st = dptr<SolverState>(b.state);
+	return a;
+}
+
+static ContactArgs contact_args(Body& b, float dt) {
+	ContactArgs c;
+	std::memset(&c, 0, sizeof(c));
+	c.nSpheres = b.nSpheres;
+	c.nWorld = b.nWorld;
+	c.worlds = dptr<WorldShape>(b.worlds);
+	c.vox = dptr<float>(b.voxels);
+	c.sphereParticle = dptr<uint32_t>(b.sphereParticle);
+	c.sphereRadius = dptr<float>(b.sphereRadius);
+	c.sphereFriction = dptr<float>(b.sphereFriction);
+	c.type = dptr<uint8_t>(b.type);
+	c.invMass = dptr<float>(b.invMass);
+	c.x = dptr<V3>(b.x);
+	c.v = dptr<V3>(b.v);
+	c.active = dptr<uint32_t>(b.cActive);
+	c.stamp = dptr<uint32_t>(b.cStamp);
+	c.applyFriction = dptr<uint32_t>(b.cFriction);
+	c.normalForce = dptr<float>(b.cNormalForce);
+	c.manifold = dptr<V3>(b.cManifold);
+	c.hit = dptr<uint32_t>(b.cHit);
+	c.cF = dptr<V3>(b.cF);
+	c.cK = dptr<M33>(b.cK);
+	c.stampBase = b.stampCounter * std::max(1u, b.nWorld);
+	c.dt = dt;
+	c.st = dptr<SolverState>(b.state);
+	return c;
+}
+
+static CGArgs cg_args(Body& b) {
+	CGArgs a;
+	a.n = b.n;
+	a.nnz = b.nnz;
+	a.nTiles = b.nTiles;
+	a.tiles = dptr<uint32_t>(b.tiles);
+	a.rowptr = dptr<uint32_t>(b.rowptr);
+	a.cols = dptr<uint32_t>(b.cols);
+	a.A = dptr<float>(b.A);
+	a.pb = dptr<V3>(b.pb);
+	a.invM = dptr<V3>(b.invM);
+	a.px = dptr<V3>(b.px);
+	a.r = dptr<V3>(b.r);
+	a.s = dptr<V3>(b.s);
+	a.q = dptr<V3>(b.q);
+	a.dold = dptr<V3>(b.d0);
+	a.dnew = dptr<V3>(b.d1);
+	a.partial = dptr<double>(b.partial);
+	a.st = dptr<SolverState>(b.state);
+	a.newton = dptr<SolverState>(b.state);
+	return a;
+}
+
+b3g_status enqueue_update_contacts(Body& b) {
+	if (!b.nWorld || !b.nSpheres) return B3G_OK;
+	ContactArgs c = contact_args(b, 0.0f);
+	LAUNCH(k_update_contacts, nblk(b.nSpheres * b.nWorld), c);
+	return B3G_OK;
+}
+
+b3g_status enqueue_find_contacts(Body& b) {
+	if (!b.nWorld || !b.nSpheres) return B3G_OK;
+	ContactArgs c = contact_args(b, 0.0f);
+	LAUNCH(k_find_contacts, nblk(b.nSpheres * b.nWorld), c);
+	b.stampCounter += 1;
+	return B3G_OK;
+}
+
+// One linearisation: element kernels, contacts, pA / pb assembly.
+static b3g_status enqueue_linearise(Body& b, float h) {
+	const SolverState* st = dptr<SolverState>(b.state);
+	M33* js = dptr<M33>(b.jstage);
+	V3* fs = dptr<V3>(b.fstage);
+	StepArgs sa = step_args(b, h, nullptr);
+	LAUNCH(k_newton_prep, nblk(b.n), sa);
+	if (b.tet.m) {
+		TetArgs t{b.tet.m, dptr<uint32_t>(b.tet.p), dptr<float>(b.tet.rest), dptr<float>(b.tet.invE), dptr<float>(b.tet.K0),
+		          dptr<float>(b.tet.q), dptr<float>(b.tet.kd), dptr<V3>(b.x), dptr<V3>(b.v), js, fs,
+		          b.tet.jx, b.tet.jv, b.tet.f, b.tet.fd, st};
+		LAUNCH(k_tet_forces, nblk(b.tet.m), t);
+	}
+	if (b.tri.m) {
+		TriArgs t{b.tri.m, dptr<uint32_t>(b.tri.p), dptr<float>(b.tri.x2), dptr<float>(b.tri.invS), dptr<float>(b.tri.K0),
+		          dptr<float>(b.tri.kd), dptr<V3>(b.x), dptr<V3>(b.v), js, fs, b.tri.jx, b.tri.jv, b.tri.f, b.tri.fd, st};
+		LAUNCH(k_tri_forces, nblk(b.tri.m), t);
+	}
+	if (b.stretch.m) {
+		UVArgs u{b.stretch.m, dptr<uint32_t>(b.stretch.p), dptr<float>(b.stretch.c), dptr<V3>(b.x), dptr<V3>(b.v), js, fs,
+		         b.stretch.jx, b.stretch.jv, b.stretch.f, st};
+		LAUNCH(k_stretch_forces, nblk(b.stretch.m), u);
+	}
+	if (b.shear.m) {
+		UVArgs u{b.shear.m, dptr<uint32_t>(b.shear.p), dptr<float>(b.shear.c), dptr<V3>(b.x), dptr<V3>(b.v), js, fs,
+		         b.shear.jx, b.shear.jv, b.shear.f, st};
+		LAUNCH(k_shear_forces, nblk(b.shear.m), u);
+	}
+	if (b.spring.m) {
+		PairArgs p{b.spring.m, dptr<uint32_t>(b.spring.p), dptr<float>(b.spring.c), dptr<V3>(b.x), dptr<V3>(b.v), js, fs,
+		           b.spring.jx, b.spring.jv, b.spring.f, st};
+		LAUNCH(k_spring_forces, nblk(b.spring.m), p);
+	}
+	if (b.mouse.m) {
+		PairArgs p{b.mouse.m, dptr<uint32_t>(b.mouse.p), dptr<float>(b.mouse.c), dptr<V3>(b.x), dptr<V3>(b.v), js, fs,
+		           b.mouse.jx, b.mouse.jv, b.mouse.f, st};
+		LAUNCH(k_mouse_forces, nblk(b.mouse.m), p);
+	}
+	ContactArgs c = contact_args(b, h);
+	if (b.nWorld && b.nSpheres) LAUNCH(k_contact_forces, nblk(b.nSpheres * b.nWorld), c);
+	AsmArgs a;
+	a.n = b.n; a.nnz = b.nnz; a.nWorld = (b.nWorld && b.nSpheres) ? b.nWorld : 0; a.h = h;
+	a.slotRow = dptr<uint32_t>(b.slotRow); a.cols = dptr<uint32_t>(b.cols);
+	a.jxPtr = dptr<uint32_t>(b.jxPtr); a.jxIdx = dptr<uint32_t>(b.jxIdx);
+	a.jvPtr = dptr<uint32_t>(b.jvPtr); a.jvIdx = dptr<uint32_t>(b.jvIdx);
+	a.bPtr = dptr<uint32_t>(b.bPtr); a.bSlot = dptr<uint32_t>(b.bSlot);
+	a.fPtr = dptr<uint32_t>(b.fPtr); a.fIdx = dptr<uint32_t>(b.fIdx);
+	a.diagSlot = dptr<uint32_t>(b.diagSlot);
+	a.jstage = js; a.fstage = fs;
+	a.type = dptr<uint8_t>(b.type); a.mass = dptr<float>(b.mass); a.damping = dptr<float>(b.damping);
+	a.sphereOf = dptr<int32_t>(b.sphereOf);
+	a.x0 = dptr<V3>(b.x0); a.v0 = dptr<V3>(b.v0); a.v = dptr<V3>(b.v); a.fe = dptr<V3>(b.fe); a.w = dptr<V3>(b.w);
+	a.A = dptr<float>(b.A); a.tB = dptr<V3>(b.tB); a.pb = dptr<V3>(b.pb); a.invM = dptr<V3>(b.invM); a.fi = dptr<V3>(b.fi);
+	a.c = c;
+	a.st = st;
+	LAUNCH(k_assemble_slots, nblk(b.nnz), a);
+	LAUNCH(k_assemble_rows, nblk(b.n), a);
+	return B3G_OK;
+}
+
+static b3g_status timed_begin(Body& b) {
+	if (!b.timing.enabled) return B3G_OK;
+	if (b.timing.used == b.timing.start.size()) {
+		hipEvent_t e0, e1;
+		B3G_HIP(hipEventCreate(&e0));
+		B3G_HIP(hipEventCreate(&e1));
+		b.timing.start.push_back(e0);
+		b.timing.stop.push_back(e1);
+	}
+	B3G_HIP(hipEventRecord(b.timing.start[b.timing.used], b.stream));
+	return B3G_OK;
+}
+static b3g_status timed_end(Body& b) {
+	if (!b.timing.enabled) return B3G_OK;
+	B3G_HIP(hipEventRecord(b.timing.stop[b.timing.used], b.stream));
+	b.timing.used += 1;
+	return B3G_OK;
+}
+
+// Jacobi PCG on the assembled pA / pb with the warm start px.
+b3g_status enqueue_solve_cg(Body& b, uint32_t maxIt, float eps) {
+	(void)eps;
+	CGArgs a = cg_args(b);
+	const uint32_t rb = nblk(b.n);
+	LAUNCH(k_cg_residual, b.nTiles, a, (int)kFinInit);
+	for (uint32_t k = 0; k < maxIt; ++k) {
+		CGArgs ak = a;
+		V3* d0 = dptr<V3>(b.d0);
+		V3* d1 = dptr<V3>(b.d1);
+		ak.dold = (k & 1) ? d1 : d0;
+		ak.dnew = (k & 1) ? d0 : d1;
+		b3g_status s = timed_begin(b);
+		if (s != B3G_OK) return s;
+		LAUNCH(k_cg_spmv_dq, b.nTiles, ak, k == 0 ? 1 : 0);
+		s = timed_end(b);
+		if (s != B3G_OK) return s;
+		if (k % 50 == 0) {
+			LAUNCH(k_cg_update_x, rb, ak);
+			LAUNCH(k_cg_residual, b.nTiles, ak, (int)kFinDelta);
+		} else {
+			LAUNCH(k_cg_update, rb, ak);
+		}
+	}
+	return B3G_OK;
+}
+
+b3g_status enqueue_spmv(Body& b, const DBuf& xin, DBuf& yout, bool timed) {
+	CGArgs a = cg_args(b);
+	if (timed) { b3g_status s = timed_begin(b); if (s != B3G_OK) return s; }
+	LAUNCH(k_spmv, b.nTiles, a, dptr<V3>(xin), dptr<V3>(yout));
+	if (timed) { b3g_status s = timed_end(b); if (s != B3G_OK) return s; }
+	return B3G_OK;
+}
+
+b3g_status enqueue_step(Body& b, float dt, uint32_t iters, uint32_t subIters, float tol, float subTol,
+                        bool assembleOnly) {
+	const float* g = b.gravity;
+	SolverState* st = dptr<SolverState>(b.state);
+	b3g_status s = enqueue_update_contacts(b);
+	if (s != B3G_OK) return s;
+	if (dt > 0.0f) {
+		StepArgs sa = step_args(b, dt, g);
+		LAUNCH(k_step_begin, nblk(b.n), sa);
+		hipLaunchKernelGGL(k_state_begin, dim3(1), dim3(1), 0, b.stream, st, assembleOnly ? 1u : iters, subIters, tol, subTol);
+		B3G_HIP(hipGetLastError());
+		for (uint32_t it = 0; it < (assembleOnly ? 1u : iters); ++it) {
+			s = enqueue_linearise(b, dt);
+			if (s != B3G_OK) return s;
+			if (assembleOnly) return B3G_OK;
+			s = enqueue_solve_cg(b, subIters, subTol);
+			if (s != B3G_OK) return s;
+			LAUNCH(k_newton_update, nblk(b.n), sa, dptr<double>(b.partial));
+		}
+		if (b.nWorld && b.nSpheres) {
+			ContactArgs c = contact_args(b, dt);
+			LAUNCH(k_friction, nblk(b.nSpheres), c);
+		}
+	}
+	LAUNCH(k_clear_external, nblk(b.n), b.n, dptr<V3>(b.force), dptr<V3>(b.translation));
+	return enqueue_find_contacts(b);
+}
+
+} // namespace b3g

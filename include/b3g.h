@@ -1,0 +1,262 @@
+/*
+ * b3g -- MI355X implicit step of bounce_softbody, C ABI.
+ *
+ * One b3g_body holds a soft body resident in HBM: particles, force elements,
+ * fixtures, world fixtures and contact pairs. b3g_step() performs
+ * b3Body::Step (dynamics/body.cpp:552-604) on the GPU: contact update,
+ * backward-Euler linearisation, element force/Jacobian assembly into the
+ * block-sparse 3x3 system, S-filtered Jacobi PCG, friction and new-contact
+ * search. Results equal the reference's FP32 results operation for operation,
+ * except the PCG dot products and the Newton error norm, which are reduced
+ * in FP64 instead of sequentially in FP32.
+ *
+ * Conventions
+ *   - Arrays passed in are host arrays in *creation order* (the order in
+ *     which a program calls b3Body::Create*). The library applies the
+ *     reference's list order (reverse creation, body.cpp:67-75) internally.
+ *   - Every function returns a b3g_status; b3g_last_error() describes the
+ *     last failure of the calling thread.
+ *   - A body owns one HIP stream. b3g_step() is asynchronous; functions that
+ *     return data synchronise that stream. A body must not be used from two
+ *     threads at once.
+ *   - No torch types: plain pointers and sizes only.
+ */
+#ifndef B3G_H
+#define B3G_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum b3g_status {
+	B3G_OK = 0,
+	B3G_INVALID_ARGUMENT = 1,
+	B3G_HIP_ERROR = 2,
+	B3G_OUT_OF_MEMORY = 3,
+	B3G_DEGENERATE_ELEMENT = 4,   /* zero-volume tetrahedron / zero-area triangle (replaces B3_ASSERT(det != 0)) */
+	B3G_UNSUPPORTED = 5,
+	B3G_NOT_COMMITTED = 6
+} b3g_status;
+
+typedef struct b3g_body_s* b3g_body;
+
+/* Force type codes (b3ForceType, dynamics/forces/force.h:30-39). */
+enum { B3G_FORCE_STRETCH = 1, B3G_FORCE_SHEAR = 2, B3G_FORCE_SPRING = 3, B3G_FORCE_MOUSE = 4,
+       B3G_FORCE_TRI_ELEMENT = 5, B3G_FORCE_TET_ELEMENT = 6 };
+
+/* World shape codes (b3Shape::Type, collision/shapes/shape.h:42-51). Mesh shapes are not supported. */
+enum { B3G_SHAPE_SPHERE = 0, B3G_SHAPE_CAPSULE = 1, B3G_SHAPE_TRIANGLE = 2, B3G_SHAPE_BOX = 3,
+       B3G_SHAPE_MESH = 4, B3G_SHAPE_SDF = 5 };
+
+#define B3G_WORLD_PARAMS 32
+
+/*
+ * Body description, creation order. Mirrors the reference's definitions:
+ *   particles    b3ParticleDef (particle.h:39-68): type, position, velocity, damping
+ *   spheres      b3SphereFixtureDef (sphere_fixture.h:28-37) + b3FixtureDef radius/friction
+ *   trifix       b3TriangleFixtureDef (triangle_fixture.h:31-45): p1..p3, rest v1..v3, density, radius, friction
+ *   tetfix       b3TetrahedronFixtureDef (tetrahedron_fixture.h:28-43)
+ *   forces       b3Body::CreateForce calls in order: (type, index into the type's table)
+ *   tet          b3TetrahedronElementForceDef: p[4], rest v[12], {youngModulus, poissonRatio, stiffnessDamping}
+ *   tri          b3TriangleElementForceDef: p[3], rest v[9], {Ex, Ey, Es, nu_xy, nu_yx, stiffnessDamping}
+ *   stretch      b3StretchForceDef: p[3], {u1 v1 u2 v2 u3 v3 ks_u kd_u b_u ks_v kd_v b_v}
+ *   shear        b3ShearForceDef: p[3], {u1 v1 u2 v2 u3 v3 ks kd}
+ *   spring       b3SpringForceDef: p[2], {length ks kd}
+ *   mouse        b3MouseForceDef: p[4], {w2 w3 w4 length ks kd}
+ *   world        b3WorldFixtureDef (world_fixture.h:29-42): shape type, B3G_WORLD_PARAMS floats, friction
+ *                param[0] = shape radius;
+ *                sphere [1..3] center; capsule [1..6] centers;
+ *                triangle [1..9] vertices, [10..12] hasE flags, [13..21] edge-neighbour vertices;
+ *                box [1..3] extents, [4..6] translation, [7..10] quaternion (x y z s);
+ *                sdf [1..6] voxel-grid aabb, [7..9] voxel counts W H D, [10..12] translation,
+ *                    [13..16] quaternion, [17] invert; voxels[w] holds W*H*D values (index i + jW + kWH).
+ */
+typedef struct b3g_body_desc {
+	uint32_t n_particles;
+	const uint8_t* p_type;        /* 0 static, 1 kinematic, 2 dynamic (particle.h:31-36) */
+	const float* p_pos;
+	const float* p_vel;
+	const float* p_damping;
+
+	uint32_t n_spheres;
+	const uint32_t* s_particle;
+	const float* s_radius;
+	const float* s_friction;
+
+	uint32_t n_trifix;
+	const uint32_t* tf_p;
+	const float* tf_v;
+	const float* tf_density;
+	const float* tf_radius;
+	const float* tf_friction;
+
+	uint32_t n_tetfix;
+	const uint32_t* ttf_p;
+	const float* ttf_v;
+	const float* ttf_density;
+	const float* ttf_radius;
+	const float* ttf_friction;
+
+	uint32_t n_forces;
+	const uint32_t* f_type;
+	const uint32_t* f_index;
+
+	uint32_t n_tet;
+	const uint32_t* tet_p;
+	const float* tet_v;
+	const float* tet_param;
+
+	uint32_t n_tri;
+	const uint32_t* tri_p;
+	const float* tri_v;
+	const float* tri_param;
+
+	uint32_t n_stretch;
+	const uint32_t* st_p;
+	const float* st_param;
+
+	uint32_t n_shear;
+	const uint32_t* sh_p;
+	const float* sh_param;
+
+	uint32_t n_spring;
+	const uint32_t* sp_p;
+	const float* sp_param;
+
+	uint32_t n_mouse;
+	const uint32_t* mo_p;
+	const float* mo_param;
+
+	uint32_t n_world;
+	const uint32_t* w_type;
+	const float* w_param;
+	const float* w_friction;
+	const float* const* w_voxels;
+
+	float gravity[3];             /* b3Body::SetGravity (body.h) */
+} b3g_body_desc;
+
+typedef struct b3g_step_stats {
+	uint32_t newton_iterations;   /* b3_forceSolverIterations (force_solver.cpp:30) */
+	uint32_t min_sub_iterations;  /* b3_forceSolverMinSubIterations (running minimum) */
+	uint32_t max_sub_iterations;  /* b3_forceSolverMaxSubIterations (running maximum) */
+	uint32_t contact_count;       /* live sphere-shape contacts (b3ContactManager::m_contactCount) */
+} b3g_step_stats;
+
+const char* b3g_status_string(b3g_status s);
+const char* b3g_last_error(void);
+
+/* Number of visible devices. */
+b3g_status b3g_device_count(int* count);
+
+/* Creates an empty body on `device`. Replaces b3Body::b3Body (body.cpp:31-55). */
+b3g_status b3g_body_create(b3g_body* out, int device);
+
+/* Frees all device memory of the body. Replaces b3Body::~b3Body (body.cpp:57-60). */
+b3g_status b3g_body_destroy(b3g_body body);
+
+/*
+ * Builds the body from a description: particles, fixtures (mass lumping of
+ * b3Body::ResetMass, body.cpp:394-466), forces (element rest data of
+ * ResetElementData, tetrahedron_element_force.cpp:219-265 and
+ * triangle_element_force.cpp:208-279), world fixtures, and the symbolic
+ * block pattern of the linear system. Replaces the sequence of
+ * b3Body::CreateParticle/CreateSphere/CreateTriangle/CreateTetrahedron/
+ * CreateForce/CreateFixture/SetGravity calls (body.cpp:62-354) in O(N).
+ * Any previous contents are discarded.
+ */
+b3g_status b3g_body_build(b3g_body body, const b3g_body_desc* desc);
+
+/*
+ * One time step. Replaces b3Body::Step(dt, forceIterations, forceSubIterations)
+ * (body.h:153, body.cpp:552-604). tolerance / sub_tolerance are the Newton and
+ * PCG tolerances of b3SolveBEInput (sparse_force_solver.h:55-61); a value <= 0
+ * selects the reference default B3_EPSILON. Asynchronous.
+ */
+b3g_status b3g_step(b3g_body body, float dt, uint32_t iterations, uint32_t sub_iterations, float tolerance,
+                    float sub_tolerance);
+
+/* Positions / velocities, creation order (b3Particle::GetPosition / GetVelocity, particle.h:220-237). */
+b3g_status b3g_get_state(b3g_body body, float* pos, float* vel);
+
+/* Overwrites positions and/or velocities (NULL keeps); b3Particle::SetPosition / SetVelocity (particle.h:213-232). */
+b3g_status b3g_set_state(b3g_body body, const float* pos, const float* vel);
+
+/* Accumulates external forces / translations for the next step (b3Particle::ApplyForce / ApplyTranslation, particle.h:249-275). NULL skips. */
+b3g_status b3g_apply_external(b3g_body body, const float* force, const float* translation);
+
+/* Lumped masses (b3Particle::GetMass), creation order. */
+b3g_status b3g_get_masses(b3g_body body, float* mass);
+
+/* Solver statistics (the reference's b3_forceSolver* globals) and contact count. */
+b3g_status b3g_get_stats(b3g_body body, b3g_step_stats* stats);
+
+/* Kinetic energy 0.5 * sum m v.v in list order, FP32 sequential (b3Body::GetEnergy, body.cpp:384-392). */
+b3g_status b3g_get_energy(b3g_body body, float* energy);
+
+/*
+ * Contact list in list order (b3ContactManager::m_contactList): creation
+ * index of the sphere fixture's particle, creation index of the world
+ * fixture and the accumulated normal force. *count receives the number of
+ * contacts; at most `capacity` entries are written.
+ */
+b3g_status b3g_get_contacts(b3g_body body, uint32_t* count, uint32_t* particle, uint32_t* world, float* normal_force,
+                            uint32_t capacity);
+
+/* Warm-start quaternions (x y z s) of the tetrahedron elements, creation order (tetrahedron_element_force.h:154). */
+b3g_status b3g_get_tet_quats(b3g_body body, float* quats);
+
+/*
+ * Diagnostics: runs the contact update and the first backward-Euler
+ * linearisation of a step (sparse_force_solver.cpp:146-174) on the current
+ * state without advancing it, and keeps pA, pb and the internal force fi.
+ * b3g_get_system copies them out: rows in list order with the columns of each
+ * row in the reference's traversal order (head to tail), blocks column-major.
+ * Tet warm starts are updated as ApplyForces does in the reference.
+ */
+b3g_status b3g_assemble(b3g_body body, float dt, uint32_t* nnz);
+b3g_status b3g_get_system(b3g_body body, uint32_t* rowptr, uint32_t* cols, float* vals, float* pb, float* fi);
+
+/* Waits for all work queued on the body. */
+b3g_status b3g_synchronize(b3g_body body);
+
+/*
+ * Kernel timing for roofline reporting: while enabled, the PCG's block-sparse
+ * matrix-vector kernel is bracketed by HIP events on the body's stream.
+ * b3g_get_spmv_timing returns the number of timed launches, their mean
+ * duration in milliseconds and the algorithmic bytes one launch moves
+ * (40 per 3x3 block + 4 per row offset + 24 per row for the vector in/out).
+ */
+b3g_status b3g_set_kernel_timing(b3g_body body, int enabled);
+b3g_status b3g_get_spmv_timing(b3g_body body, uint32_t* launches, double* mean_ms, double* bytes_per_launch);
+
+/*
+ * Host-only diagnostics: the block pattern of pA for a description (list-order
+ * CSR, columns of each row in the reference's traversal order). Needs no
+ * device. *nnz receives the block count; cols is written when capacity >= nnz.
+ */
+b3g_status b3g_pattern(const b3g_body_desc* desc, uint32_t* nnz, uint32_t* rowptr, uint32_t* cols, uint32_t capacity);
+
+/* Size of the linear system: rows (particles) and 3x3 blocks of pA. */
+b3g_status b3g_get_system_size(b3g_body body, uint32_t* rows, uint32_t* blocks);
+
+/*
+ * Operator-level replacement of b3SparseSolveCG (sparse/sparse_solver.h:47,
+ * sparse_solver.cpp:28-109): Jacobi-preconditioned CG on a block-sparse
+ * matrix given as list-order CSR (rows head to tail, 3x3 blocks column-major,
+ * host arrays). x holds the initial guess on entry and the solution on exit.
+ */
+b3g_status b3g_solve_cg(int device, uint32_t n, const uint32_t* rowptr, const uint32_t* cols, const float* vals,
+                        const float* b, float* x, uint32_t max_iterations, float tolerance, uint32_t* iterations);
+
+/* y = A x for a list-order CSR matrix (b3Mul, sparse/sparse_mat33.h:494-512), host arrays. */
+b3g_status b3g_spmv(int device, uint32_t n, const uint32_t* rowptr, const uint32_t* cols, const float* vals,
+                    const float* x, float* y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
