@@ -19,6 +19,7 @@ f32p = C.POINTER(C.c_float)
 u8p = C.POINTER(C.c_uint8)
 
 B3G_OK = 0
+REDUCE_FP64, REDUCE_SEQUENTIAL = 0, 1
 STATUS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "out of device memory", 4: "degenerate element",
           5: "unsupported", 6: "body not built"}
 
@@ -109,7 +110,7 @@ EXPORTS = [
     "b3g_body_build", "b3g_step", "b3g_get_state", "b3g_set_state", "b3g_apply_external", "b3g_get_masses",
     "b3g_get_stats", "b3g_get_energy", "b3g_get_contacts", "b3g_get_tet_quats", "b3g_assemble", "b3g_get_system",
     "b3g_synchronize", "b3g_set_kernel_timing", "b3g_get_spmv_timing", "b3g_get_system_size", "b3g_pattern",
-    "b3g_solve_cg", "b3g_spmv",
+    "b3g_solve_cg", "b3g_spmv", "b3g_set_reduction", "b3g_save_state", "b3g_load_state",
 ]
 
 
@@ -147,6 +148,9 @@ def lib():
         "b3g_pattern": (C.c_int, [C.POINTER(BodyDesc), u32p, u32p, u32p, C.c_uint32]),
         "b3g_solve_cg": (C.c_int, [C.c_int, C.c_uint32, u32p, u32p, f32p, f32p, f32p, C.c_uint32, C.c_float, u32p]),
         "b3g_spmv": (C.c_int, [C.c_int, C.c_uint32, u32p, u32p, f32p, f32p, f32p]),
+        "b3g_set_reduction": (C.c_int, [vp, C.c_int]),
+        "b3g_save_state": (C.c_int, [vp, C.c_void_p, C.POINTER(C.c_uint64)]),
+        "b3g_load_state": (C.c_int, [vp, C.c_void_p, C.c_uint64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

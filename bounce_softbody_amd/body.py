@@ -143,6 +143,22 @@ class Body:
         check(self._L.b3g_get_system_size(self._h, C.byref(r), C.byref(b)), "b3g_get_system_size")
         return r.value, b.value
 
+    def set_reduction(self, mode: int):
+        """_lib.REDUCE_FP64 (default) or _lib.REDUCE_SEQUENTIAL (the reference's FP32 accumulation order)."""
+        check(self._L.b3g_set_reduction(self._h, mode), "b3g_set_reduction")
+
+    def save_state(self) -> bytes:
+        """Checkpoint of the dynamic state (positions, velocities, loads, warm starts, contacts, statistics)."""
+        n = C.c_uint64(0)
+        check(self._L.b3g_save_state(self._h, None, C.byref(n)), "b3g_save_state")
+        buf = C.create_string_buffer(n.value)
+        check(self._L.b3g_save_state(self._h, buf, C.byref(n)), "b3g_save_state")
+        return buf.raw[: n.value]
+
+    def load_state(self, blob: bytes):
+        buf = C.create_string_buffer(blob, len(blob))
+        check(self._L.b3g_load_state(self._h, buf, len(blob)), "b3g_load_state")
+
     def set_kernel_timing(self, on: bool):
         check(self._L.b3g_set_kernel_timing(self._h, 1 if on else 0), "b3g_set_kernel_timing")
 

@@ -311,6 +311,100 @@ b3g_status b3g_get_system(b3g_body h, uint32_t* rowptr, uint32_t* cols, float* v
 	return B3G_OK;
 }
 
+b3g_status b3g_set_reduction(b3g_body h, int mode) {
+	CHECK_BODY(h);
+	if (mode != B3G_REDUCE_FP64 && mode != B3G_REDUCE_SEQUENTIAL) { set_error("unknown reduction mode"); return B3G_INVALID_ARGUMENT; }
+	h->body.reduction = mode;
+	return B3G_OK;
+}
+
+namespace {
+struct StateHeader {
+	uint32_t magic, version, n, nTet, pairs, stampCounter, statNewton, statMinSub, statMaxSub, reserved;
+};
+constexpr uint32_t kStateMagic = 0x53473342u;   // "B3GS"
+
+struct StatePart {
+	DBuf* buf;
+	size_t bytes;
+};
+
+std::vector<StatePart> state_parts(Body& b) {
+	const size_t pairs = (b.nWorld && b.nSpheres) ? (size_t)b.nSpheres * b.nWorld : 0;
+	std::vector<StatePart> parts = {{&b.x, b.n * sizeof(V3)}, {&b.v, b.n * sizeof(V3)}, {&b.force, b.n * sizeof(V3)},
+	                                {&b.translation, b.n * sizeof(V3)}};
+	if (b.tet.m) parts.push_back({&b.tet.q, (size_t)b.tet.m * 16});
+	if (pairs) {
+		parts.push_back({&b.cActive, pairs * 4});
+		parts.push_back({&b.cStamp, pairs * 4});
+		parts.push_back({&b.cFriction, pairs * 4});
+		parts.push_back({&b.cNormalForce, pairs * 4});
+		parts.push_back({&b.cManifold, pairs * 2 * sizeof(V3)});
+	}
+	return parts;
+}
+} // namespace
+
+b3g_status b3g_save_state(b3g_body h, void* blob, uint64_t* bytes) {
+	CHECK_BUILT(h);
+	if (!bytes) return B3G_INVALID_ARGUMENT;
+	SET_DEVICE(h);
+	Body& b = h->body;
+	std::vector<StatePart> parts = state_parts(b);
+	size_t total = sizeof(StateHeader);
+	for (auto& p : parts) total += p.bytes;
+	if (!blob) { *bytes = total; return B3G_OK; }
+	if (*bytes < total) { set_error("state blob too small"); return B3G_INVALID_ARGUMENT; }
+	B3G_HIP(hipStreamSynchronize(b.stream));
+	SolverState st;
+	B3G_HIP(hipMemcpy(&st, b.state.p, sizeof(st), hipMemcpyDeviceToHost));
+	StateHeader hd = {kStateMagic, 1, b.n, b.tet.m, (uint32_t)((b.nWorld && b.nSpheres) ? b.nSpheres * b.nWorld : 0),
+	                  b.stampCounter, st.statNewton, st.statMinSub, st.statMaxSub, 0};
+	char* out = static_cast<char*>(blob);
+	std::memcpy(out, &hd, sizeof(hd));
+	size_t off = sizeof(hd);
+	for (auto& p : parts) {
+		B3G_HIP(hipMemcpy(out + off, p.buf->p, p.bytes, hipMemcpyDeviceToHost));
+		off += p.bytes;
+	}
+	*bytes = total;
+	return B3G_OK;
+}
+
+b3g_status b3g_load_state(b3g_body h, const void* blob, uint64_t bytes) {
+	CHECK_BUILT(h);
+	if (!blob) return B3G_INVALID_ARGUMENT;
+	SET_DEVICE(h);
+	Body& b = h->body;
+	std::vector<StatePart> parts = state_parts(b);
+	size_t total = sizeof(StateHeader);
+	for (auto& p : parts) total += p.bytes;
+	StateHeader hd;
+	if (bytes < sizeof(hd)) { set_error("state blob too small"); return B3G_INVALID_ARGUMENT; }
+	std::memcpy(&hd, blob, sizeof(hd));
+	const uint32_t pairs = (b.nWorld && b.nSpheres) ? b.nSpheres * b.nWorld : 0;
+	if (hd.magic != kStateMagic || hd.version != 1 || hd.n != b.n || hd.nTet != b.tet.m || hd.pairs != pairs ||
+	    bytes != total) {
+		set_error("state blob does not match this body");
+		return B3G_INVALID_ARGUMENT;
+	}
+	B3G_HIP(hipStreamSynchronize(b.stream));
+	const char* in = static_cast<const char*>(blob);
+	size_t off = sizeof(hd);
+	for (auto& p : parts) {
+		B3G_HIP(hipMemcpy(p.buf->p, in + off, p.bytes, hipMemcpyHostToDevice));
+		off += p.bytes;
+	}
+	b.stampCounter = hd.stampCounter;
+	SolverState st;
+	B3G_HIP(hipMemcpy(&st, b.state.p, sizeof(st), hipMemcpyDeviceToHost));
+	st.statNewton = hd.statNewton;
+	st.statMinSub = hd.statMinSub;
+	st.statMaxSub = hd.statMaxSub;
+	B3G_HIP(hipMemcpy(b.state.p, &st, sizeof(st), hipMemcpyHostToDevice));
+	return B3G_OK;
+}
+
 b3g_status b3g_synchronize(b3g_body h) {
 	CHECK_BODY(h);
 	SET_DEVICE(h);

@@ -105,6 +105,8 @@ struct SolverState {
 	// running statistics (force_solver.cpp:30-34)
 	uint32_t statNewton, statMinSub, statMaxSub;
 	uint32_t contactCount;
+	// reduction mode of the dot products: 0 FP64 tree, 1 sequential FP32 in row order
+	uint32_t seq;
 	// grid reduction bookkeeping: blocks arrived per reduction slot
 	uint32_t arrive[4];
 };
@@ -164,6 +166,8 @@ struct Body {
 	DBuf px, r, s, q, d0, d1, fi;   // V3 per row (px = py warm start)
 	DBuf partial;    // double per block of the reduction kernels
 	uint32_t partialCount = 0;
+	DBuf terms;      // float per row: per-row dot terms (sequential reduction mode)
+	int reduction = 0;
 	DBuf state;      // SolverState
 
 	// ---------------------------------------------------------------- contacts (sphere x world pairs)

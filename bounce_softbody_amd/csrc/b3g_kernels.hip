@@ -99,26 +99,61 @@ __device__ void finalize(SolverState* st, int kind, double total) {
 	}
 }
 
-// Last-arriving workgroup sums the per-workgroup partials in a fixed order.
-__device__ void grid_reduce(double v, double* partial, SolverState* st, int slot, int kind) {
+// Reduction of one scalar per row over the whole grid.
+//   FP64 mode: per-workgroup FP64 partials; the last-arriving workgroup adds
+//   them in a fixed order (deterministic, near-exact).
+//   Sequential mode: each row stores its FP32 term and the last workgroup adds
+//   the terms one by one in row order starting from 0, which is the
+//   reference's b3Dot / b3LengthSquared accumulation (dense_vec3.h:151-175).
+struct Reduce {
+	double* partial;
+	float* terms;
+	uint32_t n;
+	SolverState* st;
+};
+
+__device__ void grid_reduce(float term, bool has, uint32_t row, const Reduce& rd, int slot, int kind) {
 	__shared__ double sh[kBlock / 64];
 	__shared__ int last;
-	double bs = block_sum(v, sh);
-	if (threadIdx.x == 0) {
-		partial[blockIdx.x] = bs;
+	__shared__ float chunk[4 * kBlock];
+	const bool seq = rd.st->seq != 0;
+	if (seq && has) {
+		rd.terms[row] = term;
 		__threadfence();
-		uint32_t old = atomicAdd(&st->arrive[slot], 1u);
+	}
+	double bs = block_sum(has ? (double)term : 0.0, sh);
+	if (threadIdx.x == 0) {
+		rd.partial[blockIdx.x] = bs;
+		__threadfence();
+		uint32_t old = atomicAdd(&rd.st->arrive[slot], 1u);
 		last = (old == gridDim.x - 1) ? 1 : 0;
 	}
 	__syncthreads();
 	if (!last) return;
 	__threadfence();
+	if (seq) {
+		float acc = 0.0f;
+		for (uint32_t base = 0; base < rd.n; base += 4 * kBlock) {
+			for (uint32_t k = threadIdx.x; k < 4 * kBlock && base + k < rd.n; k += kBlock) chunk[k] = rd.terms[base + k];
+			__syncthreads();
+			if (threadIdx.x == 0) {
+				const uint32_t cnt = min((uint32_t)(4 * kBlock), rd.n - base);
+				for (uint32_t k = 0; k < cnt; ++k) acc += chunk[k];
+			}
+			__syncthreads();
+		}
+		if (threadIdx.x == 0) {
+			finalize(rd.st, kind, (double)acc);
+			rd.st->arrive[slot] = 0;
+		}
+		return;
+	}
 	double t = 0.0;
-	for (uint32_t k = threadIdx.x; k < gridDim.x; k += kBlock) t += partial[k];
+	for (uint32_t k = threadIdx.x; k < gridDim.x; k += kBlock) t += rd.partial[k];
 	double tot = block_sum(t, sh);
 	if (threadIdx.x == 0) {
-		finalize(st, kind, tot);
-		st->arrive[slot] = 0;
+		finalize(rd.st, kind, tot);
+		rd.st->arrive[slot] = 0;
 	}
 }
 
@@ -215,7 +250,8 @@ __global__ void __launch_bounds__(kBlock) k_step_begin(StepArgs a) {
 	a.px[i] = zero3();
 }
 
-__global__ void k_state_begin(SolverState* st, uint32_t iters, uint32_t subIters, float tol, float subTol) {
+__global__ void k_state_begin(SolverState* st, uint32_t iters, uint32_t subIters, float tol, float subTol, uint32_t seq) {
+	st->seq = seq;
 	st->newtonIt = 0;
 	st->newtonMax = iters;
 	st->newtonActive = iters > 0 ? 1u : 0u;
@@ -902,8 +938,10 @@ struct CGArgs {
 	const V3* dold;
 	V3* dnew;
 	double* partial;
+	float* terms;
 	SolverState* st;
 	const SolverState* newton;
+	__device__ Reduce red() const { return Reduce{partial, terms, n, st}; }
 };
 
 __device__ __forceinline__ V3 diag_apply(V3 im, V3 r) { return mdiag3(im.x, im.y, im.z) * r; }
@@ -945,15 +983,16 @@ __global__ void __launch_bounds__(kBlock) k_cg_residual(CGArgs a, int mode) {
 	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
 	uint32_t row;
 	V3 ax = tile_rows(a, t, row, [&](uint32_t c) { return a.px[c]; }, lds);
-	double part = 0.0;
-	if (row < a.tiles[t + 1]) {
+	float part = 0.0f;
+	const bool has = row < a.tiles[t + 1];
+	if (has) {
 		V3 r = a.pb[row] - ax;
 		V3 s = diag_apply(a.invM[row], r);
 		a.r[row] = r;
 		a.s[row] = s;
-		part = (double)dot(r, s);
+		part = dot(r, s);
 	}
-	grid_reduce(part, a.partial, a.st, 0, mode);
+	grid_reduce(part, has, row, a.red(), 0, mode);
 }
 
 // q = A d with d = s + beta d_old formed on the fly (d = s at the first
@@ -966,14 +1005,15 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, int first) {
 	auto dval = [&](uint32_t c) { return first ? a.s[c] : a.s[c] + beta * a.dold[c]; };
 	uint32_t row;
 	V3 q = tile_rows(a, t, row, dval, lds);
-	double part = 0.0;
-	if (row < a.tiles[t + 1]) {
+	float part = 0.0f;
+	const bool has = row < a.tiles[t + 1];
+	if (has) {
 		V3 d = dval(row);
 		a.dnew[row] = d;
 		a.q[row] = q;
-		part = (double)dot(d, q);
+		part = dot(d, q);
 	}
-	grid_reduce(part, a.partial, a.st, 1, kFinAlpha);
+	grid_reduce(part, has, row, a.red(), 1, kFinAlpha);
 }
 
 // x += alpha d ; r -= alpha q ; s = invM r ; delta = r.s
@@ -981,17 +1021,18 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a) {
 	if (!a.st->active) return;
 	uint32_t i = blockIdx.x * kBlock + threadIdx.x;
 	const float alpha = a.st->alpha;
-	double part = 0.0;
-	if (i < a.n) {
+	float part = 0.0f;
+	const bool has = i < a.n;
+	if (has) {
 		V3 d = a.dnew[i];
 		a.px[i] = a.px[i] + alpha * d;
 		V3 r = a.r[i] - alpha * a.q[i];
 		V3 s = diag_apply(a.invM[i], r);
 		a.r[i] = r;
 		a.s[i] = s;
-		part = (double)dot(r, s);
+		part = dot(r, s);
 	}
-	grid_reduce(part, a.partial, a.st, 2, kFinDelta);
+	grid_reduce(part, has, i, a.red(), 2, kFinDelta);
 }
 
 // x += alpha d (iterations that refresh the true residual)
@@ -1003,18 +1044,19 @@ __global__ void __launch_bounds__(kBlock) k_cg_update_x(CGArgs a) {
 }
 
 // v += dv ; x = x0 + h v + y ; error = |dv|^2 (sparse_force_solver.cpp:193-223)
-__global__ void __launch_bounds__(kBlock) k_newton_update(StepArgs s, double* partial) {
+__global__ void __launch_bounds__(kBlock) k_newton_update(StepArgs s, double* partial, float* terms) {
 	if (!s.st->newtonActive) return;
 	uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-	double part = 0.0;
-	if (i < s.n) {
+	float part = 0.0f;
+	const bool has = i < s.n;
+	if (has) {
 		V3 dv = s.px[i] + zero3();
 		V3 v = s.v[i] + dv;
 		s.v[i] = v;
 		s.x[i] = (s.x0[i] + s.h * v) + s.y[i];
-		part = (double)length_sq(dv);
+		part = length_sq(dv);
 	}
-	grid_reduce(part, partial, s.st, 3, kFinNewton);
+	grid_reduce(part, has, i, Reduce{partial, terms, s.n, s.st}, 3, kFinNewton);
 }
 
 __global__ void __launch_bounds__(kBlock) k_clear_external(uint32_t n, V3* force, V3* translation) {
@@ -1066,6 +1108,10 @@ b3g_status build_tiles(Body& b) {
 }
 
 b3g_status ensure_partials(Body& b, uint32_t blocks) {
+	if (!b.terms.p || b.terms.bytes < (size_t)b.n * sizeof(float)) {
+		b3g_status s = dalloc(b, b.terms, (size_t)std::max(1u, b.n) * sizeof(float));
+		if (s != B3G_OK) return s;
+	}
 	if (b.partialCount >= blocks && b.partial.p) return B3G_OK;
 	b3g_status s = dalloc(b, b.partial, (size_t)blocks * sizeof(double));
 	if (s != B3G_OK) return s;
@@ -1153,6 +1199,7 @@ static CGArgs cg_args(Body& b) {
 	a.dold = dptr<V3>(b.d0);
 	a.dnew = dptr<V3>(b.d1);
 	a.partial = dptr<double>(b.partial);
+	a.terms = dptr<float>(b.terms);
 	a.st = dptr<SolverState>(b.state);
 	a.newton = dptr<SolverState>(b.state);
 	return a;
@@ -1296,7 +1343,8 @@ b3g_status enqueue_step(Body& b, float dt, uint32_t iters, uint32_t subIters, fl
 	if (dt > 0.0f) {
 		StepArgs sa = step_args(b, dt, g);
 		LAUNCH(k_step_begin, nblk(b.n), sa);
-		hipLaunchKernelGGL(k_state_begin, dim3(1), dim3(1), 0, b.stream, st, assembleOnly ? 1u : iters, subIters, tol, subTol);
+		hipLaunchKernelGGL(k_state_begin, dim3(1), dim3(1), 0, b.stream, st, assembleOnly ? 1u : iters, subIters, tol, subTol,
+		                   (uint32_t)b.reduction);
 		B3G_HIP(hipGetLastError());
 		for (uint32_t it = 0; it < (assembleOnly ? 1u : iters); ++it) {
 			s = enqueue_linearise(b, dt);
@@ -1304,7 +1352,7 @@ b3g_status enqueue_step(Body& b, float dt, uint32_t iters, uint32_t subIters, fl
 			if (assembleOnly) return B3G_OK;
 			s = enqueue_solve_cg(b, subIters, subTol);
 			if (s != B3G_OK) return s;
-			LAUNCH(k_newton_update, nblk(b.n), sa, dptr<double>(b.partial));
+			LAUNCH(k_newton_update, nblk(b.n), sa, dptr<double>(b.partial), dptr<float>(b.terms));
 		}
 		if (b.nWorld && b.nSpheres) {
 			ContactArgs c = contact_args(b, dt);

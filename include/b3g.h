@@ -219,6 +219,28 @@ b3g_status b3g_get_tet_quats(b3g_body body, float* quats);
 b3g_status b3g_assemble(b3g_body body, float dt, uint32_t* nnz);
 b3g_status b3g_get_system(b3g_body body, uint32_t* rowptr, uint32_t* cols, float* vals, float* pb, float* fi);
 
+/*
+ * Reduction of the PCG dot products and of the Newton error norm.
+ *   B3G_REDUCE_FP64 (default): deterministic FP64 tree over the device.
+ *   B3G_REDUCE_SEQUENTIAL: FP32 terms added one by one in row order, as
+ *   b3Dot / b3LengthSquared do (dense_vec3.h:151-175). With it a step is the
+ *   reference's step operation for operation; it is O(rows) latency per
+ *   reduction and meant for validation, not throughput.
+ */
+enum { B3G_REDUCE_FP64 = 0, B3G_REDUCE_SEQUENTIAL = 1 };
+b3g_status b3g_set_reduction(b3g_body body, int mode);
+
+/*
+ * Checkpoint / resume of everything a step reads besides the description:
+ * positions, velocities, pending external forces and translations, tet
+ * warm-start quaternions, the contact pairs (with their frozen manifolds,
+ * normal forces and list order) and the solver statistics. With blob == NULL,
+ * *bytes receives the size. Loading requires a body built from the same
+ * description. No reference counterpart (the reference has no resume).
+ */
+b3g_status b3g_save_state(b3g_body body, void* blob, uint64_t* bytes);
+b3g_status b3g_load_state(b3g_body body, const void* blob, uint64_t bytes);
+
 /* Waits for all work queued on the body. */
 b3g_status b3g_synchronize(b3g_body body);
 

@@ -18,7 +18,7 @@ import pytest
 from bounce_softbody_amd import _lib, configs
 from bounce_softbody_amd.body import Body
 import oracle
-from tests.golden.make_golden import external_loads, schedule
+from tests.golden.make_golden import digest, external_loads, schedule
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
@@ -49,26 +49,74 @@ def test_first_linearisation_bit_exact(name):
     assert np.array_equal(gpb, opb), f"pb max diff {np.abs(gpb - opb).max()}"
 
 
+FIXTURES = sorted(os.path.splitext(f)[0] for f in os.listdir(GOLDEN) if f.endswith(".npz"))
+
+
+def scene_arrays(name):
+    return configs.tet_grid(10).arrays() if name == "tet_grid_10" else configs.PARITY_SCENES[name]().arrays()
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_sequential_reduction_reproduces_reference_goldens(name):
+    """With the reference's FP32 accumulation order every step is bit-identical to the reference build."""
+    gold = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+    arr = scene_arrays(name)
+    g = Body(arr)
+    g.set_reduction(_lib.REDUCE_SEQUENTIAL)
+    n = len(arr["p_type"])
+    for k, (row, ext) in enumerate(zip(gold["schedule"], gold["external"])):
+        dt, it, sub, tol, stol = row
+        if ext:
+            g.apply_external(*external_loads(n))
+        g.Step(np.float32(dt), int(it), int(sub), float(tol), float(stol))
+        assert np.array_equal(digest(*g.state()), gold["digests"][k]), f"{name}: differs from the reference after step {k}"
+    cp, cw, cf = g.contacts()
+    assert np.array_equal(cp, gold["contact_particle"]) and np.array_equal(cw, gold["contact_world"])
+    assert np.array_equal(cf, gold["contact_normal_force"])
+    assert np.array_equal(g.tet_quats(), gold["quats"])
+    assert np.array_equal(g.stats()[0], gold["stats"])
+
+
 @pytest.mark.parametrize("name", sorted(configs.PARITY_SCENES))
-def test_trajectory_within_tolerance(name):
+def test_fp64_reduction_per_step_within_tolerance(name):
+    """Default (FP64-reduced) steps taken from the reference-exact state stay within 1e-4 per step."""
     arr = configs.PARITY_SCENES[name]().arrays()
-    g, o = Body(arr), oracle.OracleBody(arr)
+    ref, fast = Body(arr), Body(arr)
+    ref.set_reduction(_lib.REDUCE_SEQUENTIAL)
     n = len(arr["p_type"])
     worst = 0.0
     for k, (dt, it, sub, tol, stol, ext) in enumerate(schedule(name, 30)):
         if ext:
-            f, t = external_loads(n)
-            g.apply_external(f, t)
-            o.apply_external(f, t)
-        g.Step(dt, it, sub, tol, stol)
-        o.step(dt, it, sub, tol, stol)
-        gx, gv = g.state()
-        ox, ov = o.state()
-        ex, ev = rel(gx, ox), rel(gv, ov)
+            ref.apply_external(*external_loads(n))
+        fast.load_state(ref.save_state())
+        ref.Step(dt, it, sub, tol, stol)
+        fast.Step(dt, it, sub, tol, stol)
+        rx, rv = ref.state()
+        fx, fv = fast.state()
+        ex, ev = rel(fx, rx), rel(fv, rv)
         worst = max(worst, ex, ev)
         assert ex <= REL_TOL and ev <= REL_TOL, f"step {k}: x {ex:.2e} v {ev:.2e}"
-    gs, _ = g.stats()
-    assert gs[0] == o.stats()[0]
+
+
+def test_checkpoint_resume_replays_bit_exactly():
+    arr = configs.PARITY_SCENES["drop_on_box"]().arrays()
+    g = Body(arr)
+    for _ in range(8):
+        g.Step(configs.DT, 1, 20, 0.0, 1e-5)
+    blob = g.save_state()
+    for _ in range(6):
+        g.Step(configs.DT, 1, 20, 0.0, 1e-5)
+    x1, v1 = g.state()
+    c1 = g.contacts()
+    g.load_state(blob)
+    for _ in range(6):
+        g.Step(configs.DT, 1, 20, 0.0, 1e-5)
+    x2, v2 = g.state()
+    assert np.array_equal(x1, x2) and np.array_equal(v1, v2)
+    assert all(np.array_equal(a, b) for a, b in zip(c1, g.contacts()))
+    other = Body(configs.PARITY_SCENES["tet_grid"]().arrays())
+    with pytest.raises(_lib.B3gError):
+        other.load_state(blob)
 
 
 def test_contact_sets_bit_exact_from_identical_state():
