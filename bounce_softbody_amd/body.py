@@ -159,15 +159,12 @@ class Body:
         buf = C.create_string_buffer(blob, len(blob))
         check(self._L.b3g_load_state(self._h, buf, len(blob)), "b3g_load_state")
 
-    def set_kernel_timing(self, on: bool):
-        check(self._L.b3g_set_kernel_timing(self._h, 1 if on else 0), "b3g_set_kernel_timing")
-
-    def spmv_timing(self):
-        n = C.c_uint32(0)
+    def time_spmv(self, launches: int = 100):
+        """Mean duration (ms) of the PCG's SpMV kernel over back-to-back launches, and its algorithmic bytes."""
         ms = C.c_double(0)
         by = C.c_double(0)
-        check(self._L.b3g_get_spmv_timing(self._h, C.byref(n), C.byref(ms), C.byref(by)), "b3g_get_spmv_timing")
-        return n.value, ms.value, by.value
+        check(self._L.b3g_time_spmv(self._h, launches, C.byref(ms), C.byref(by)), "b3g_time_spmv")
+        return ms.value, by.value
 
 
 __all__ = ["Body", "B3gError"]

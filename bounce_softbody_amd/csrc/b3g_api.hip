@@ -109,8 +109,6 @@ b3g_status b3g_body_destroy(b3g_body h) {
 	(void)hipSetDevice(h->body.device);
 	(void)hipStreamSynchronize(h->body.stream);
 	dfree_all(h->body);
-	for (hipEvent_t e : h->body.timing.start) (void)hipEventDestroy(e);
-	for (hipEvent_t e : h->body.timing.stop) (void)hipEventDestroy(e);
 	(void)hipStreamDestroy(h->body.stream);
 	delete h;
 	return B3G_OK;
@@ -268,11 +266,8 @@ b3g_status b3g_get_tet_quats(b3g_body h, float* quats) {
 	Body& b = h->body;
 	const uint32_t m = b.tet.m;
 	if (!m) return B3G_OK;
-	std::vector<float> q(4 * (size_t)m);
 	B3G_HIP(hipStreamSynchronize(b.stream));
-	B3G_HIP(hipMemcpy(q.data(), b.tet.q.p, q.size() * 4, hipMemcpyDeviceToHost));
-	for (uint32_t e = 0; e < m; ++e)
-		for (int c = 0; c < 4; ++c) quats[4 * (size_t)e + c] = q[(size_t)c * m + e];
+	B3G_HIP(hipMemcpy(quats, b.tet.q.p, 16 * (size_t)m, hipMemcpyDeviceToHost));
 	return B3G_OK;
 }
 
@@ -318,11 +313,14 @@ b3g_status b3g_set_reduction(b3g_body h, int mode) {
 	return B3G_OK;
 }
 
+} // extern "C"
+
 namespace {
 struct StateHeader {
 	uint32_t magic, version, n, nTet, pairs, stampCounter, statNewton, statMinSub, statMaxSub, reserved;
 };
 constexpr uint32_t kStateMagic = 0x53473342u;   // "B3GS"
+constexpr uint32_t kStateVersion = 2;            // 2: tetrahedron quaternions stored per element
 
 struct StatePart {
 	DBuf* buf;
@@ -345,6 +343,8 @@ std::vector<StatePart> state_parts(Body& b) {
 }
 } // namespace
 
+extern "C" {
+
 b3g_status b3g_save_state(b3g_body h, void* blob, uint64_t* bytes) {
 	CHECK_BUILT(h);
 	if (!bytes) return B3G_INVALID_ARGUMENT;
@@ -358,7 +358,7 @@ b3g_status b3g_save_state(b3g_body h, void* blob, uint64_t* bytes) {
 	B3G_HIP(hipStreamSynchronize(b.stream));
 	SolverState st;
 	B3G_HIP(hipMemcpy(&st, b.state.p, sizeof(st), hipMemcpyDeviceToHost));
-	StateHeader hd = {kStateMagic, 1, b.n, b.tet.m, (uint32_t)((b.nWorld && b.nSpheres) ? b.nSpheres * b.nWorld : 0),
+	StateHeader hd = {kStateMagic, kStateVersion, b.n, b.tet.m, (uint32_t)((b.nWorld && b.nSpheres) ? b.nSpheres * b.nWorld : 0),
 	                  b.stampCounter, st.statNewton, st.statMinSub, st.statMaxSub, 0};
 	char* out = static_cast<char*>(blob);
 	std::memcpy(out, &hd, sizeof(hd));
@@ -383,7 +383,7 @@ b3g_status b3g_load_state(b3g_body h, const void* blob, uint64_t bytes) {
 	if (bytes < sizeof(hd)) { set_error("state blob too small"); return B3G_INVALID_ARGUMENT; }
 	std::memcpy(&hd, blob, sizeof(hd));
 	const uint32_t pairs = (b.nWorld && b.nSpheres) ? b.nSpheres * b.nWorld : 0;
-	if (hd.magic != kStateMagic || hd.version != 1 || hd.n != b.n || hd.nTet != b.tet.m || hd.pairs != pairs ||
+	if (hd.magic != kStateMagic || hd.version != kStateVersion || hd.n != b.n || hd.nTet != b.tet.m || hd.pairs != pairs ||
 	    bytes != total) {
 		set_error("state blob does not match this body");
 		return B3G_INVALID_ARGUMENT;
@@ -412,34 +412,13 @@ b3g_status b3g_synchronize(b3g_body h) {
 	return B3G_OK;
 }
 
-b3g_status b3g_set_kernel_timing(b3g_body h, int enabled) {
-	CHECK_BODY(h);
-	SET_DEVICE(h);
-	Timing& t = h->body.timing;
-	B3G_HIP(hipStreamSynchronize(h->body.stream));
-	t.enabled = enabled != 0;
-	t.used = 0;
-	t.totalMs = 0.0;
-	t.launches = 0;
-	return B3G_OK;
-}
-
-b3g_status b3g_get_spmv_timing(b3g_body h, uint32_t* launches, double* mean_ms, double* bytes_per_launch) {
+b3g_status b3g_time_spmv(b3g_body h, uint32_t launches, double* mean_ms, double* bytes_per_launch) {
 	CHECK_BUILT(h);
+	if (!mean_ms || launches == 0) return B3G_INVALID_ARGUMENT;
 	SET_DEVICE(h);
 	Body& b = h->body;
-	Timing& t = b.timing;
-	B3G_HIP(hipStreamSynchronize(b.stream));
-	// Launches skipped after PCG convergence still record an (empty) interval; count non-empty ones.
-	for (size_t k = 0; k < t.used; ++k) {
-		float ms = 0.0f;
-		B3G_HIP(hipEventElapsedTime(&ms, t.start[k], t.stop[k]));
-		t.totalMs += ms;
-		t.launches += 1;
-	}
-	t.used = 0;
-	if (launches) *launches = (uint32_t)t.launches;
-	if (mean_ms) *mean_ms = t.launches ? t.totalMs / (double)t.launches : 0.0;
+	b3g_status s = time_pcg_spmv(b, launches, mean_ms);
+	if (s != B3G_OK) return s;
 	if (bytes_per_launch) *bytes_per_launch = 40.0 * b.nnz + 4.0 * (b.n + 1) + 24.0 * b.n;
 	return B3G_OK;
 }
@@ -555,7 +534,7 @@ b3g_status b3g_spmv(int device, uint32_t n, const uint32_t* rowptr, const uint32
 	if (s == B3G_OK) {
 		auto body = [&]() -> b3g_status {
 			B3G_HIP(hipMemcpyAsync(b.px.p, x, n * sizeof(V3), hipMemcpyHostToDevice, b.stream));
-			b3g_status s2 = enqueue_spmv(b, b.px, b.q, false);
+			b3g_status s2 = enqueue_spmv(b, b.px, b.q);
 			if (s2 != B3G_OK) return s2;
 			B3G_HIP(hipMemcpyAsync(y, b.q.p, n * sizeof(V3), hipMemcpyDeviceToHost, b.stream));
 			B3G_HIP(hipStreamSynchronize(b.stream));

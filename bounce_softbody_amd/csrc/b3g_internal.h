@@ -15,6 +15,10 @@ namespace b3g {
 
 enum : uint8_t { kStatic = 0, kKinematic = 1, kDynamic = 2 };
 
+// Contribution codes of the per-slot lists: a staged block index, or
+// kTetCode | e for a tetrahedron block computed during assembly.
+constexpr uint32_t kTetCode = 0x80000000u;
+
 // Thread-local error text for b3g_last_error().
 void set_error(const std::string& msg);
 
@@ -42,11 +46,11 @@ struct TetTable {
 	DBuf rest;       // 12 planes: rest positions x1..x4
 	DBuf invE;       // 9 planes
 	DBuf K0;         // 144 planes: K[k] (k = i + 4j) column-major 3x3
-	DBuf q;          // 4 planes: warm-start quaternion x y z s
+	DBuf q;          // float4 per element: warm-start quaternion (x y z s)
 	DBuf kd;         // 1 plane
 	DBuf param;      // 3 planes (E, nu, kd) for the set-up kernel
 	bool anyDamping = false;
-	uint32_t jx = 0, jv = 0, f = 0, fd = 0;   // staging bases (block k / vertex i: base + k*m + e)
+	uint32_t f = 0, fd = 0;   // force staging bases (vertex i: base + i*m + e)
 };
 
 struct TriTable {
@@ -107,17 +111,8 @@ struct SolverState {
 	uint32_t contactCount;
 	// reduction mode of the dot products: 0 FP64 tree, 1 sequential FP32 in row order
 	uint32_t seq;
-	// grid reduction bookkeeping: blocks arrived per reduction slot
-	uint32_t arrive[4];
 };
 
-struct Timing {
-	bool enabled = false;
-	std::vector<hipEvent_t> start, stop;
-	size_t used = 0;
-	double totalMs = 0.0;
-	uint64_t launches = 0;
-};
 
 struct Body {
 	int device = 0;
@@ -155,8 +150,13 @@ struct Body {
 	DBuf jxPtr, jxIdx, jvPtr, jvIdx;              // per-slot contribution lists (stage indices)
 	DBuf bPtr, bSlot;                             // per-row dfdx slots in first-touch order (b's SpMV)
 	DBuf fPtr, fIdx;                              // per-row force contributions
-	DBuf tiles;                                   // uint32 row boundaries of SpMV tiles
+	DBuf tiles;                                   // uint32 row boundaries of SpMV / row-assembly tiles
 	uint32_t nTiles = 0;
+	DBuf slotTiles;                               // uint32 slot boundaries of slot-assembly tiles
+	uint32_t nSlotTiles = 0;
+	DBuf K0x, K0v;                                // tetrahedron K0 blocks in jx / jv contribution order (9 planes)
+	uint32_t nJx = 0, nJv = 0;                    // lengths of the jx / jv lists when K0x / K0v exist
+	bool anyJv = false;                           // the body has dfdv contributions beyond mass damping
 	std::vector<uint32_t> hostRowptr, hostCols;
 
 	// ---------------------------------------------------------------- linear system / PCG
@@ -183,7 +183,6 @@ struct Body {
 
 	// ---------------------------------------------------------------- diagnostics
 	bool captureFi = false;
-	Timing timing;
 
 	std::vector<DBuf*> owned;
 };
@@ -201,11 +200,13 @@ b3g_status host_pattern(const b3g_body_desc& d, std::vector<uint32_t>& rowptr, s
 
 // Kernel launchers (b3g_kernels.hip)
 b3g_status launch_tet_setup(Body& b);
+b3g_status launch_tet_perm(Body& b, const std::vector<uint32_t>& src, DBuf& dst);
 b3g_status enqueue_step(Body& b, float dt, uint32_t iters, uint32_t subIters, float tol, float subTol,
                         bool assembleOnly);
 b3g_status enqueue_update_contacts(Body& b);
 b3g_status enqueue_find_contacts(Body& b);
-b3g_status enqueue_spmv(Body& b, const DBuf& xin, DBuf& yout, bool timed);
+b3g_status enqueue_spmv(Body& b, const DBuf& xin, DBuf& yout);
+b3g_status time_pcg_spmv(Body& b, uint32_t launches, double* mean_ms);
 b3g_status enqueue_solve_cg(Body& b, uint32_t maxIt, float eps);
 b3g_status build_tiles(Body& b);
 b3g_status ensure_partials(Body& b, uint32_t blocks);

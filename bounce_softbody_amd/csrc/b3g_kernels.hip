@@ -15,6 +15,7 @@ namespace b3g {
 constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
 constexpr int kTileCap = 1024;     // 3x3 blocks per SpMV tile chunk (12 KB of LDS products)
 constexpr int kTileRows = 256;     // rows per SpMV tile (one phase-2 thread each)
+constexpr int kRowPer = kTileCap / kBlock;   // chunk entries per thread
 
 // ---------------------------------------------------------------------------
 // Helpers
@@ -99,12 +100,16 @@ __device__ void finalize(SolverState* st, int kind, double total) {
 	}
 }
 
-// Reduction of one scalar per row over the whole grid.
-//   FP64 mode: per-workgroup FP64 partials; the last-arriving workgroup adds
-//   them in a fixed order (deterministic, near-exact).
-//   Sequential mode: each row stores its FP32 term and the last workgroup adds
-//   the terms one by one in row order starting from 0, which is the
-//   reference's b3Dot / b3LengthSquared accumulation (dense_vec3.h:151-175).
+// Reduction of one scalar per row over the whole grid, in two launches.
+// The producing kernel stores one FP64 partial per workgroup (fixed tree)
+// and, in sequential mode, each row's FP32 term; k_finalize then adds them
+// in a fixed order and updates the solver state. The kernel boundary orders
+// the stores before the loads, so no device-scope fence is needed (on a
+// multi-XCD part such a fence writes back and invalidates a whole L2).
+//   FP64 mode: deterministic, near-exact sum of the partials.
+//   Sequential mode: the terms are added one by one in row order starting
+//   from 0, which is the reference's b3Dot / b3LengthSquared accumulation
+//   (dense_vec3.h:151-175).
 struct Reduce {
 	double* partial;
 	float* terms;
@@ -112,48 +117,58 @@ struct Reduce {
 	SolverState* st;
 };
 
-__device__ void grid_reduce(float term, bool has, uint32_t row, const Reduce& rd, int slot, int kind) {
+__device__ __forceinline__ void block_partial(float term, bool has, uint32_t row, const Reduce& rd) {
 	__shared__ double sh[kBlock / 64];
-	__shared__ int last;
-	__shared__ float chunk[4 * kBlock];
-	const bool seq = rd.st->seq != 0;
-	if (seq && has) {
-		rd.terms[row] = term;
-		__threadfence();
-	}
+	if (rd.st->seq && has) rd.terms[row] = term;
 	double bs = block_sum(has ? (double)term : 0.0, sh);
-	if (threadIdx.x == 0) {
-		rd.partial[blockIdx.x] = bs;
-		__threadfence();
-		uint32_t old = atomicAdd(&rd.st->arrive[slot], 1u);
-		last = (old == gridDim.x - 1) ? 1 : 0;
-	}
-	__syncthreads();
-	if (!last) return;
-	__threadfence();
-	if (seq) {
+	if (threadIdx.x == 0) rd.partial[blockIdx.x] = bs;
+}
+
+constexpr int kFinBlock = 1024;
+
+// Gate of the finalize launch: the same condition under which its producer
+// ran (a skipped producer leaves stale partials behind).
+enum Gate : int { kGateNewton = 0, kGateCG = 1 };
+
+__global__ void __launch_bounds__(kFinBlock) k_finalize(Reduce rd, uint32_t count, int kind, int gate) {
+	if (!rd.st->newtonActive) return;
+	if (gate == kGateCG && !rd.st->active) return;
+	__shared__ double sh[kFinBlock / 64];
+	__shared__ float chunk[4 * kFinBlock];
+	if (rd.st->seq) {
 		float acc = 0.0f;
-		for (uint32_t base = 0; base < rd.n; base += 4 * kBlock) {
-			for (uint32_t k = threadIdx.x; k < 4 * kBlock && base + k < rd.n; k += kBlock) chunk[k] = rd.terms[base + k];
+		for (uint32_t base = 0; base < rd.n; base += 4 * kFinBlock) {
+			for (uint32_t k = threadIdx.x; k < 4 * kFinBlock && base + k < rd.n; k += kFinBlock) chunk[k] = rd.terms[base + k];
 			__syncthreads();
 			if (threadIdx.x == 0) {
-				const uint32_t cnt = min((uint32_t)(4 * kBlock), rd.n - base);
+				const uint32_t cnt = min((uint32_t)(4 * kFinBlock), rd.n - base);
 				for (uint32_t k = 0; k < cnt; ++k) acc += chunk[k];
 			}
 			__syncthreads();
 		}
-		if (threadIdx.x == 0) {
-			finalize(rd.st, kind, (double)acc);
-			rd.st->arrive[slot] = 0;
-		}
+		if (threadIdx.x == 0) finalize(rd.st, kind, (double)acc);
 		return;
 	}
+	// fixed order: thread t adds partials t, t + 1024, ... (four loads in flight)
 	double t = 0.0;
-	for (uint32_t k = threadIdx.x; k < gridDim.x; k += kBlock) t += rd.partial[k];
-	double tot = block_sum(t, sh);
+	for (uint32_t base = 0; base < count; base += 4 * kFinBlock) {
+		double p[4];
+#pragma unroll
+		for (int u = 0; u < 4; ++u) {
+			const uint32_t k = base + threadIdx.x + u * kFinBlock;
+			p[u] = k < count ? rd.partial[k] : 0.0;
+		}
+#pragma unroll
+		for (int u = 0; u < 4; ++u) t += p[u];
+	}
+	for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	if (lane == 0) sh[wave] = t;
+	__syncthreads();
 	if (threadIdx.x == 0) {
+		double tot = 0.0;
+		for (int k = 0; k < kFinBlock / 64; ++k) tot += sh[k];
 		finalize(rd.st, kind, tot);
-		rd.st->arrive[slot] = 0;
 	}
 }
 
@@ -260,7 +275,6 @@ __global__ void k_state_begin(SolverState* st, uint32_t iters, uint32_t subIters
 	st->maxIt = subIters;
 	st->active = 0;
 	st->it = 0;
-	st->arrive[0] = st->arrive[1] = st->arrive[2] = st->arrive[3] = 0;
 }
 
 // w = x0 - x + h v + y (sparse_force_solver.cpp:167)
@@ -280,13 +294,12 @@ struct TetArgs {
 	const float* rest;
 	const float* invE;
 	const float* K0;
-	float* q;
+	float4* q;
 	const float* kd;
 	const V3* x;
 	const V3* v;
-	M33* jstage;
 	V3* fstage;
-	uint32_t jx, jv, f, fd;
+	uint32_t f, fd;
 	const SolverState* st;
 };
 
@@ -316,7 +329,9 @@ __device__ Q4 extract_rotation(const M33& A, Q4 q) {
 	return q;
 }
 
-// tetrahedron_element_force.cpp:313-464
+// tetrahedron_element_force.cpp:313-464. The element's Jacobian blocks
+// -R K0_ij R^T are not staged: the slot assembly recomputes them from the
+// stored rotation (same operations, same values).
 __global__ void __launch_bounds__(kBlock) k_tet_forces(TetArgs a) {
 	if (!a.st->newtonActive) return;
 	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
@@ -331,14 +346,12 @@ __global__ void __launch_bounds__(kBlock) k_tet_forces(TetArgs a) {
 	}
 	M33 Em = m33(p[1] - p[0], p[2] - p[0], p[3] - p[0]);
 	M33 F = Em * ldM(a.invE, m, 0, e);
+	const float4 q4 = a.q[e];
 	Q4 q;
-	q.v = v3(a.q[e], a.q[(size_t)m + e], a.q[(size_t)2 * m + e]);
-	q.s = a.q[(size_t)3 * m + e];
+	q.v = v3(q4.x, q4.y, q4.z);
+	q.s = q4.w;
 	q = extract_rotation(F, q);
-	a.q[e] = q.v.x;
-	a.q[(size_t)m + e] = q.v.y;
-	a.q[(size_t)2 * m + e] = q.v.z;
-	a.q[(size_t)3 * m + e] = q.s;
+	a.q[e] = make_float4(q.v.x, q.v.y, q.v.z, q.s);
 	M33 R = qmatrix(q);
 	M33 RT = transpose(R);
 	V3 us[4];
@@ -354,12 +367,10 @@ __global__ void __launch_bounds__(kBlock) k_tet_forces(TetArgs a) {
 			const int blk = i + 4 * j;
 			M33 K0 = ldM(a.K0, m, 9 * blk, e);
 			fs = fs + K0 * us[j];
-			M33 K = R * K0 * RT;
-			a.jstage[a.jx + (size_t)blk * m + e] = -K;     // dfdx(vi, vj) -= K
 			if (kd > 0.0f) {
+				M33 K = R * K0 * RT;
 				M33 kK = kd * K;
 				fds = fds + kK * vs[j];
-				a.jstage[a.jv + (size_t)blk * m + e] = -kK;  // dfdv(vi, vj) -= kd K
 			}
 		}
 		a.fstage[a.f + (size_t)i * m + e] = -(R * fs);      // f[vi] -= R fs
@@ -811,10 +822,19 @@ __global__ void __launch_bounds__(kBlock) k_friction(ContactArgs a) {
 
 // ---------------------------------------------------------------------------
 // Assembly of pA, pb (sparse_force_solver.cpp:165-174)
+//
+// Both kernels work on tiles: the tile's contributions are loaded (or, for
+// tetrahedra, computed) by all threads in parallel into LDS, then one thread
+// per slot / row adds its own contributions in the reference's order. The
+// sums are the sequential ones of the reference; only the loads run wide.
 // ---------------------------------------------------------------------------
+constexpr int kAsmCap = 1024;      // contributions per LDS chunk of the slot assembly (36 KB)
+
 struct AsmArgs {
 	uint32_t n, nnz, nWorld;
 	float h;
+	const uint32_t* slotTiles;   // slot boundaries of the slot-assembly tiles
+	const uint32_t* rowTiles;    // row boundaries of the row-assembly tiles (the SpMV tiles)
 	const uint32_t* slotRow;
 	const uint32_t* cols;
 	const uint32_t* jxPtr;
@@ -828,6 +848,13 @@ struct AsmArgs {
 	const uint32_t* diagSlot;
 	const M33* jstage;
 	const V3* fstage;
+	// tetrahedra: contribution-ordered K0 blocks (9 planes) and the element rotations
+	const float* K0x;
+	const float* K0v;
+	uint32_t nJx, nJv, tetM;
+	const float4* tetQ;
+	const float* tetKd;
+	uint32_t anyJv;
 	const uint8_t* type;
 	const float* mass;
 	const float* damping;
@@ -846,54 +873,179 @@ struct AsmArgs {
 	const SolverState* st;
 };
 
+// acc += the contributions [ptr[s], ptr[s + 1]) of slot s, in order, for the
+// tile's slots [s0, s1). Each chunk of kAsmCap contributions is produced in
+// three batched phases per thread (kAsmCap / kBlock contributions each):
+// codes and K0 blocks, then the dependent gathers, then the products into
+// LDS, so a thread has all its loads in flight at once.
+constexpr int kAsmPer = kAsmCap / kBlock;
+
+struct Contrib {
+	const uint32_t* idx;    // codes
+	const float* K0;        // contribution-ordered K0 planes (tetrahedra) or null
+	uint32_t count;         // plane stride of K0
+	const float* kd;        // per-tetrahedron scale (dfdv) or null (dfdx)
+};
+
+__device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib& cb_, const uint32_t* ptr, uint32_t s0,
+                                                uint32_t s1, uint32_t s, bool own, float* lds, M33& acc) {
+	const uint32_t c0 = ptr[s0], c1 = ptr[s1];
+	const uint32_t rb = own ? ptr[s] : 0, re = own ? ptr[s + 1] : 0;
+	for (uint32_t cb = c0; cb < c1; cb += kAsmCap) {
+		const uint32_t ce = min(cb + (uint32_t)kAsmCap, c1);
+		uint32_t code[kAsmPer];
+		M33 blk[kAsmPer];
+		Q4 q[kAsmPer];
+#pragma unroll
+		for (int u = 0; u < kAsmPer; ++u) {
+			const uint32_t k = cb + threadIdx.x + u * kBlock;
+			code[u] = k < ce ? cb_.idx[k] : 0u;
+			if (k < ce && cb_.K0) blk[u] = ldA(cb_.K0, cb_.count, k);
+		}
+#pragma unroll
+		for (int u = 0; u < kAsmPer; ++u) {
+			const uint32_t k = cb + threadIdx.x + u * kBlock;
+			if (k >= ce) continue;
+			if (code[u] & kTetCode) {
+				const float4 q4 = a.tetQ[code[u] & ~kTetCode];
+				q[u].v = v3(q4.x, q4.y, q4.z);
+				q[u].s = q4.w;
+			} else {
+				blk[u] = a.jstage[code[u]];
+			}
+		}
+#pragma unroll
+		for (int u = 0; u < kAsmPer; ++u) {
+			const uint32_t k = cb + threadIdx.x + u * kBlock;
+			if (k >= ce) continue;
+			M33 v = blk[u];
+			if (code[u] & kTetCode) {
+				// R K0_ij R^T of the tetrahedron (tetrahedron_element_force.cpp:400-420)
+				M33 R = qmatrix(q[u]);
+				M33 RT = transpose(R);
+				M33 K = R * blk[u] * RT;
+				v = cb_.kd ? -(cb_.kd[code[u] & ~kTetCode] * K) : -K;
+			}
+			const float* pv = &v.x.x;
+#pragma unroll
+			for (int c = 0; c < 9; ++c) lds[c * kAsmCap + (k - cb)] = pv[c];
+		}
+		__syncthreads();
+		if (own) {
+			const uint32_t kb = max(rb, cb), ke = min(re, ce);
+			for (uint32_t k = kb; k < ke; ++k) {
+				M33 v;
+				float* pv = &v.x.x;
+#pragma unroll
+				for (int c = 0; c < 9; ++c) pv[c] = lds[c * kAsmCap + (k - cb)];
+				acc = acc + v;
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// Same over V3 contributions of rows (kTileCap per chunk): val[idx[k]].
+__device__ __forceinline__ void tile_accumulate3(const uint32_t* ptr, const uint32_t* idx, const V3* val, uint32_t r0,
+                                                 uint32_t r1, uint32_t r, bool own, float* lds, V3& acc) {
+	const uint32_t c0 = ptr[r0], c1 = ptr[r1];
+	const uint32_t rb = own ? ptr[r] : 0, re = own ? ptr[r + 1] : 0;
+	for (uint32_t cb = c0; cb < c1; cb += kTileCap) {
+		const uint32_t ce = min(cb + (uint32_t)kTileCap, c1);
+		uint32_t id[kRowPer];
+		V3 v[kRowPer];
+#pragma unroll
+		for (int u = 0; u < kRowPer; ++u) {
+			const uint32_t k = cb + threadIdx.x + u * kBlock;
+			id[u] = k < ce ? idx[k] : 0u;
+		}
+#pragma unroll
+		for (int u = 0; u < kRowPer; ++u) {
+			const uint32_t k = cb + threadIdx.x + u * kBlock;
+			if (k < ce) v[u] = val[id[u]];
+		}
+#pragma unroll
+		for (int u = 0; u < kRowPer; ++u) {
+			const uint32_t k = cb + threadIdx.x + u * kBlock;
+			if (k >= ce) continue;
+			lds[k - cb] = v[u].x;
+			lds[kTileCap + (k - cb)] = v[u].y;
+			lds[2 * kTileCap + (k - cb)] = v[u].z;
+		}
+		__syncthreads();
+		if (own) {
+			const uint32_t kb = max(rb, cb), ke = min(re, ce);
+			for (uint32_t k = kb; k < ke; ++k)
+				acc = acc + v3(lds[k - cb], lds[kTileCap + (k - cb)], lds[2 * kTileCap + (k - cb)]);
+		}
+		__syncthreads();
+	}
+}
+
 __global__ void __launch_bounds__(kBlock) k_assemble_slots(AsmArgs a) {
 	if (!a.st->newtonActive) return;
-	uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-	if (s >= a.nnz) return;
-	uint32_t i = a.slotRow[s], j = a.cols[s];
-	// dfdx(i, j): staged contributions in list order, then contacts (force_solver.cpp:70-86)
+	__shared__ float lds[9 * kAsmCap];
+	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+	const uint32_t s0 = a.slotTiles[t], s1 = a.slotTiles[t + 1];
+	const uint32_t s = s0 + threadIdx.x;
+	const bool own = s < s1;
+	const uint32_t i = own ? a.slotRow[s] : 0, j = own ? a.cols[s] : 0;
+	// per-slot data of the epilogue, loaded ahead of the contribution phase
+	const uint8_t type_i = own ? a.type[i] : 0, type_j = own ? a.type[j] : 0;
+	const float mass_i = own ? a.mass[i] : 0.0f, damp_i = own ? a.damping[i] : 0.0f;
+	const int32_t sph_i = own && a.nWorld ? a.sphereOf[i] : -1;
+	const V3 w_j = own ? a.w[j] : zero3();
+	// dfdx(i, j): contributions in list order, then contacts (force_solver.cpp:70-86)
 	M33 dx = mzero();
-	for (uint32_t k = a.jxPtr[s]; k < a.jxPtr[s + 1]; ++k) dx = dx + a.jstage[a.jxIdx[k]];
-	const bool dyn_i = a.type[i] == kDynamic;
-	if (i == j && a.nWorld && a.sphereOf[i] >= 0 && dyn_i) {
+	tile_accumulate(a, Contrib{a.jxIdx, a.K0x, a.nJx, nullptr}, a.jxPtr, s0, s1, s, own, lds, dx);
+	// dfdv(i, j): particle mass damping first, then contributions
+	M33 dv = mzero();
+	if (own && i == j && damp_i > 0.0f) dv = dv + mdiag(-damp_i * mass_i);
+	if (a.anyJv) tile_accumulate(a, Contrib{a.jvIdx, a.K0v, a.nJv, a.tetKd}, a.jvPtr, s0, s1, s, own, lds, dv);
+	if (!own) return;
+	const bool dyn_i = type_i == kDynamic;
+	if (i == j && a.nWorld && sph_i >= 0 && dyn_i) {
 		uint32_t order[16];
-		uint32_t sph = (uint32_t)a.sphereOf[i];
+		uint32_t sph = (uint32_t)sph_i;
 		int cnt = contact_order(a.c, sph, order);
 		for (int c = 0; c < cnt; ++c) {
 			uint32_t k = sph * a.nWorld + order[c];
 			if (a.c.hit[k]) dx = dx + a.c.cK[k];
 		}
 	}
-	// dfdv(i, j): particle mass damping first, then staged contributions
-	M33 dv = mzero();
-	if (i == j && a.damping[i] > 0.0f) dv = dv + mdiag(-a.damping[i] * a.mass[i]);
-	for (uint32_t k = a.jvPtr[s]; k < a.jvPtr[s + 1]; ++k) dv = dv + a.jstage[a.jvIdx[k]];
 	// A = M - h dfdv - (h h) dfdx, block by block
 	const float h = a.h, hh = h * h;
-	M33 Mij = i == j ? mdiag(dyn_i ? a.mass[i] : 1.0f) : mzero();
+	M33 Mij = i == j ? mdiag(dyn_i ? mass_i : 1.0f) : mzero();
 	M33 Aij = (Mij - h * dv) - hh * dx;
 	// pA = S A S^T + I - S (S = I for dynamic particles, 0 otherwise)
-	const bool dyn_j = a.type[j] == kDynamic;
+	const bool dyn_j = type_j == kDynamic;
 	M33 Si = dyn_i ? mdiag(1.0f) : mzero();
 	M33 Sj = dyn_j ? mdiag(1.0f) : mzero();
 	M33 P = (Si * Aij) * Sj;
 	if (i == j) P = (P + mdiag(1.0f)) - Si;
 	stA(a.A, a.nnz, s, P);
 	// (h dfdx_ij) w_j for b's product
-	a.tB[s] = (h * dx) * a.w[j];
+	a.tB[s] = (h * dx) * w_j;
 }
 
 __global__ void __launch_bounds__(kBlock) k_assemble_rows(AsmArgs a) {
 	if (!a.st->newtonActive) return;
-	uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-	if (i >= a.n) return;
-	const bool dyn = a.type[i] == kDynamic;
-	const float mass = a.mass[i];
-	V3 vi = a.v[i];
+	__shared__ float lds[3 * kTileCap];
+	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+	const uint32_t r0 = a.rowTiles[t], r1 = a.rowTiles[t + 1];
+	const uint32_t i = r0 + threadIdx.x;
+	const bool own = i < r1;
+	const float mass = own ? a.mass[i] : 0.0f;
+	const V3 vi = own ? a.v[i] : zero3();
 	// fi: particle damping, forces in list order, contacts (force_solver.cpp:70-86)
 	V3 fi = zero3();
-	if (a.damping[i] > 0.0f) fi = fi + (-a.damping[i] * mass) * vi;
-	for (uint32_t k = a.fPtr[i]; k < a.fPtr[i + 1]; ++k) fi = fi + a.fstage[a.fIdx[k]];
+	if (own && a.damping[i] > 0.0f) fi = fi + (-a.damping[i] * mass) * vi;
+	tile_accumulate3(a.fPtr, a.fIdx, a.fstage, r0, r1, i, own, lds, fi);
+	// h dfdx (x0 - x + h v + y) in the first-touch order of dfdx
+	V3 hdw = zero3();
+	tile_accumulate3(a.bPtr, a.bSlot, a.tB, r0, r1, i, own, lds, hdw);
+	if (!own) return;
+	const bool dyn = a.type[i] == kDynamic;
 	if (a.nWorld && a.sphereOf[i] >= 0 && dyn) {
 		uint32_t order[16];
 		uint32_t sph = (uint32_t)a.sphereOf[i];
@@ -908,8 +1060,6 @@ __global__ void __launch_bounds__(kBlock) k_assemble_rows(AsmArgs a) {
 	const float h = a.h;
 	M33 M = mdiag(dyn ? mass : 1.0f);
 	V3 b = M * (a.v0[i] - vi) + h * (a.fe[i] + fi);
-	V3 hdw = zero3();
-	for (uint32_t k = a.bPtr[i]; k < a.bPtr[i + 1]; ++k) hdw = hdw + a.tB[a.bSlot[k]];
 	b = b + hdw;
 	// pb = S (b - A z), z = 0
 	M33 S = dyn ? mdiag(1.0f) : mzero();
@@ -992,13 +1142,14 @@ __global__ void __launch_bounds__(kBlock) k_cg_residual(CGArgs a, int mode) {
 		a.s[row] = s;
 		part = dot(r, s);
 	}
-	grid_reduce(part, has, row, a.red(), 0, mode);
+	block_partial(part, has, row, a.red());
 }
 
 // q = A d with d = s + beta d_old formed on the fly (d = s at the first
-// iteration), d written for the tile's rows; alpha = delta / d.q
-__global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, int first) {
-	if (!a.st->active) return;
+// iteration), d written for the tile's rows; alpha = delta / d.q.
+// force = 1 runs it whatever the PCG state (timing pass on CG scratch).
+__global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, int first, int force) {
+	if (!force && !a.st->active) return;
 	__shared__ float lds[3 * kTileCap];
 	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
 	const float beta = a.st->beta;
@@ -1013,7 +1164,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, int first) {
 		a.q[row] = q;
 		part = dot(d, q);
 	}
-	grid_reduce(part, has, row, a.red(), 1, kFinAlpha);
+	block_partial(part, has, row, a.red());
 }
 
 // x += alpha d ; r -= alpha q ; s = invM r ; delta = r.s
@@ -1032,7 +1183,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a) {
 		a.s[i] = s;
 		part = dot(r, s);
 	}
-	grid_reduce(part, has, i, a.red(), 2, kFinDelta);
+	block_partial(part, has, i, a.red());
 }
 
 // x += alpha d (iterations that refresh the true residual)
@@ -1056,7 +1207,7 @@ __global__ void __launch_bounds__(kBlock) k_newton_update(StepArgs s, double* pa
 		s.x[i] = (s.x0[i] + s.h * v) + s.y[i];
 		part = length_sq(dv);
 	}
-	grid_reduce(part, has, i, Reduce{partial, terms, s.n, s.st}, 3, kFinNewton);
+	block_partial(part, has, i, Reduce{partial, terms, s.n, s.st});
 }
 
 __global__ void __launch_bounds__(kBlock) k_clear_external(uint32_t n, V3* force, V3* translation) {
@@ -1116,6 +1267,34 @@ b3g_status ensure_partials(Body& b, uint32_t blocks) {
 	b3g_status s = dalloc(b, b.partial, (size_t)blocks * sizeof(double));
 	if (s != B3G_OK) return s;
 	b.partialCount = blocks;
+	return B3G_OK;
+}
+
+// K0 blocks in contribution order: dst plane c, position k = K0 block src[k]
+// (src = block * m + e) of the element-ordered planes.
+__global__ void __launch_bounds__(kBlock) k_tet_perm(uint32_t m, uint32_t count, const uint32_t* src, const float* K0,
+                                                      float* dst) {
+	uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+	if (k >= count) return;
+	const uint32_t sk = src[k];
+	if (sk == 0xFFFFFFFFu) return;
+	const uint32_t blk = sk / m, e = sk % m;
+#pragma unroll
+	for (int c = 0; c < 9; ++c) dst[(size_t)c * count + k] = K0[(size_t)(9 * blk + c) * m + e];
+}
+
+b3g_status launch_tet_perm(Body& b, const std::vector<uint32_t>& src, DBuf& dst) {
+	const uint32_t count = (uint32_t)src.size();
+	if (!count) return B3G_OK;
+	DBuf tmp;
+	b3g_status s = dupload(b, tmp, src.data(), src.size() * 4);
+	if (s != B3G_OK) return s;
+	s = dalloc(b, dst, (size_t)count * 9 * sizeof(float));
+	if (s != B3G_OK) return s;
+	LAUNCH(k_tet_perm, nblk(count), b.tet.m, count, dptr<uint32_t>(tmp), dptr<float>(b.tet.K0), dptr<float>(dst));
+	B3G_HIP(hipStreamSynchronize(b.stream));
+	(void)hipFree(tmp.p);
+	b.owned.erase(std::remove(b.owned.begin(), b.owned.end(), &tmp), b.owned.end());
 	return B3G_OK;
 }
 
@@ -1229,8 +1408,7 @@ static b3g_status enqueue_linearise(Body& b, float h) {
 	LAUNCH(k_newton_prep, nblk(b.n), sa);
 	if (b.tet.m) {
 		TetArgs t{b.tet.m, dptr<uint32_t>(b.tet.p), dptr<float>(b.tet.rest), dptr<float>(b.tet.invE), dptr<float>(b.tet.K0),
-		          dptr<float>(b.tet.q), dptr<float>(b.tet.kd), dptr<V3>(b.x), dptr<V3>(b.v), js, fs,
-		          b.tet.jx, b.tet.jv, b.tet.f, b.tet.fd, st};
+		          dptr<float4>(b.tet.q), dptr<float>(b.tet.kd), dptr<V3>(b.x), dptr<V3>(b.v), fs, b.tet.f, b.tet.fd, st};
 		LAUNCH(k_tet_forces, nblk(b.tet.m), t);
 	}
 	if (b.tri.m) {
@@ -1262,6 +1440,9 @@ static b3g_status enqueue_linearise(Body& b, float h) {
 	if (b.nWorld && b.nSpheres) LAUNCH(k_contact_forces, nblk(b.nSpheres * b.nWorld), c);
 	AsmArgs a;
 	a.n = b.n; a.nnz = b.nnz; a.nWorld = (b.nWorld && b.nSpheres) ? b.nWorld : 0; a.h = h;
+	a.slotTiles = dptr<uint32_t>(b.slotTiles); a.rowTiles = dptr<uint32_t>(b.tiles);
+	a.K0x = dptr<float>(b.K0x); a.K0v = dptr<float>(b.K0v); a.nJx = b.nJx; a.nJv = b.nJv;
+	a.tetM = b.tet.m; a.tetQ = dptr<float4>(b.tet.q); a.tetKd = dptr<float>(b.tet.kd); a.anyJv = b.anyJv ? 1u : 0u;
 	a.slotRow = dptr<uint32_t>(b.slotRow); a.cols = dptr<uint32_t>(b.cols);
 	a.jxPtr = dptr<uint32_t>(b.jxPtr); a.jxIdx = dptr<uint32_t>(b.jxIdx);
 	a.jvPtr = dptr<uint32_t>(b.jvPtr); a.jvIdx = dptr<uint32_t>(b.jvIdx);
@@ -1275,29 +1456,23 @@ static b3g_status enqueue_linearise(Body& b, float h) {
 	a.A = dptr<float>(b.A); a.tB = dptr<V3>(b.tB); a.pb = dptr<V3>(b.pb); a.invM = dptr<V3>(b.invM); a.fi = dptr<V3>(b.fi);
 	a.c = c;
 	a.st = st;
-	LAUNCH(k_assemble_slots, nblk(b.nnz), a);
-	LAUNCH(k_assemble_rows, nblk(b.n), a);
+	LAUNCH(k_assemble_slots, b.nSlotTiles, a);
+	LAUNCH(k_assemble_rows, b.nTiles, a);
 	return B3G_OK;
 }
 
-static b3g_status timed_begin(Body& b) {
-	if (!b.timing.enabled) return B3G_OK;
-	if (b.timing.used == b.timing.start.size()) {
-		hipEvent_t e0, e1;
-		B3G_HIP(hipEventCreate(&e0));
-		B3G_HIP(hipEventCreate(&e1));
-		b.timing.start.push_back(e0);
-		b.timing.stop.push_back(e1);
-	}
-	B3G_HIP(hipEventRecord(b.timing.start[b.timing.used], b.stream));
+static b3g_status enqueue_finalize(Body& b, uint32_t count, int kind, int gate) {
+	Reduce rd{dptr<double>(b.partial), dptr<float>(b.terms), b.n, dptr<SolverState>(b.state)};
+	hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, b.stream, rd, count, kind, gate);
+	B3G_HIP(hipGetLastError());
 	return B3G_OK;
 }
-static b3g_status timed_end(Body& b) {
-	if (!b.timing.enabled) return B3G_OK;
-	B3G_HIP(hipEventRecord(b.timing.stop[b.timing.used], b.stream));
-	b.timing.used += 1;
-	return B3G_OK;
-}
+
+#define FINALIZE(count, kind, gate)                                      \
+	do {                                                                 \
+		b3g_status fs_ = enqueue_finalize(b, (count), (kind), (gate));  \
+		if (fs_ != B3G_OK) return fs_;                                   \
+	} while (0)
 
 // Jacobi PCG on the assembled pA / pb with the warm start px.
 b3g_status enqueue_solve_cg(Body& b, uint32_t maxIt, float eps) {
@@ -1305,32 +1480,63 @@ b3g_status enqueue_solve_cg(Body& b, uint32_t maxIt, float eps) {
 	CGArgs a = cg_args(b);
 	const uint32_t rb = nblk(b.n);
 	LAUNCH(k_cg_residual, b.nTiles, a, (int)kFinInit);
+	FINALIZE(b.nTiles, kFinInit, kGateNewton);
 	for (uint32_t k = 0; k < maxIt; ++k) {
 		CGArgs ak = a;
 		V3* d0 = dptr<V3>(b.d0);
 		V3* d1 = dptr<V3>(b.d1);
 		ak.dold = (k & 1) ? d1 : d0;
 		ak.dnew = (k & 1) ? d0 : d1;
-		b3g_status s = timed_begin(b);
-		if (s != B3G_OK) return s;
-		LAUNCH(k_cg_spmv_dq, b.nTiles, ak, k == 0 ? 1 : 0);
-		s = timed_end(b);
-		if (s != B3G_OK) return s;
+		LAUNCH(k_cg_spmv_dq, b.nTiles, ak, k == 0 ? 1 : 0, 0);
+		FINALIZE(b.nTiles, kFinAlpha, kGateCG);
 		if (k % 50 == 0) {
 			LAUNCH(k_cg_update_x, rb, ak);
 			LAUNCH(k_cg_residual, b.nTiles, ak, (int)kFinDelta);
+			FINALIZE(b.nTiles, kFinDelta, kGateCG);
 		} else {
 			LAUNCH(k_cg_update, rb, ak);
+			FINALIZE(rb, kFinDelta, kGateCG);
 		}
 	}
 	return B3G_OK;
 }
 
-b3g_status enqueue_spmv(Body& b, const DBuf& xin, DBuf& yout, bool timed) {
+b3g_status enqueue_spmv(Body& b, const DBuf& xin, DBuf& yout) {
 	CGArgs a = cg_args(b);
-	if (timed) { b3g_status s = timed_begin(b); if (s != B3G_OK) return s; }
 	LAUNCH(k_spmv, b.nTiles, a, dptr<V3>(xin), dptr<V3>(yout));
-	if (timed) { b3g_status s = timed_end(b); if (s != B3G_OK) return s; }
+	return B3G_OK;
+}
+
+// Back-to-back launches of the PCG's SpMV kernel (an iteration's full work,
+// on the current pA and the CG scratch vectors) between one event pair.
+b3g_status time_pcg_spmv(Body& b, uint32_t launches, double* mean_ms) {
+	CGArgs a = cg_args(b);
+	hipEvent_t e0, e1;
+	B3G_HIP(hipEventCreate(&e0));
+	B3G_HIP(hipEventCreate(&e1));
+	b3g_status st = B3G_OK;
+	float ms = 0.0f;
+	do {
+		if (hipEventRecord(e0, b.stream) != hipSuccess) { st = B3G_HIP_ERROR; break; }
+		for (uint32_t k = 0; k < launches && st == B3G_OK; ++k) {
+			CGArgs ak = a;
+			ak.dold = (k & 1) ? dptr<V3>(b.d1) : dptr<V3>(b.d0);
+			ak.dnew = (k & 1) ? dptr<V3>(b.d0) : dptr<V3>(b.d1);
+			hipLaunchKernelGGL(k_cg_spmv_dq, dim3(b.nTiles), dim3(kBlock), 0, b.stream, ak, 0, 1);
+			if (hipGetLastError() != hipSuccess) st = B3G_HIP_ERROR;
+		}
+		if (st != B3G_OK) break;
+		if (hipEventRecord(e1, b.stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+		    hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+			st = B3G_HIP_ERROR;
+	} while (0);
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	if (st != B3G_OK) {
+		set_error("SpMV timing pass failed");
+		return st;
+	}
+	*mean_ms = launches ? (double)ms / launches : 0.0;
 	return B3G_OK;
 }
 
@@ -1353,6 +1559,7 @@ b3g_status enqueue_step(Body& b, float dt, uint32_t iters, uint32_t subIters, fl
 			s = enqueue_solve_cg(b, subIters, subTol);
 			if (s != B3G_OK) return s;
 			LAUNCH(k_newton_update, nblk(b.n), sa, dptr<double>(b.partial), dptr<float>(b.terms));
+			FINALIZE(nblk(b.n), kFinNewton, kGateNewton);
 		}
 		if (b.nWorld && b.nSpheres) {
 			ContactArgs c = contact_args(b, dt);

@@ -183,7 +183,7 @@ struct SymbolicInput {
 	const std::vector<uint32_t>* spP; uint32_t spM; const std::vector<float>* spCoef;   // 3
 	const std::vector<uint32_t>* moP; uint32_t moM; const std::vector<float>* moCoef;   // 6
 	// staging bases
-	uint32_t tetJx, tetJv, tetF, tetFd, triJx, triJv, triF, triFd;
+	uint32_t tetF, tetFd, triJx, triJv, triF, triFd;
 	uint32_t stJx, stJv, stF, shJx, shJv, shF, spJx, spJv, spF, moJx, moJv, moF;
 };
 
@@ -203,12 +203,12 @@ void build_pattern(const SymbolicInput& in, Pattern& P) {
 			const uint32_t* p = in.tetP->data() + 4 * (size_t)e;
 			const uint32_t m = in.tetM;
 			for (int i = 0; i < 4; ++i)
-				for (int j = 0; j < 4; ++j) D.touch(p[i], p[j], in.tetJx + (uint32_t)(i + 4 * j) * m + e);
+				for (int j = 0; j < 4; ++j) D.touch(p[i], p[j], kTetCode | ((uint32_t)(i + 4 * j) * m + e));
 			for (int i = 0; i < 4; ++i) fTouch.push_back({p[i], in.tetF + (uint32_t)i * m + e});
 			if ((*in.tetKd)[e] > 0.0f) {
 				for (int i = 0; i < 4; ++i) fTouch.push_back({p[i], in.tetFd + (uint32_t)i * m + e});
 				for (int i = 0; i < 4; ++i)
-					for (int j = 0; j < 4; ++j) V.touch(p[i], p[j], in.tetJv + (uint32_t)(i + 4 * j) * m + e);
+					for (int j = 0; j < 4; ++j) V.touch(p[i], p[j], kTetCode | ((uint32_t)(i + 4 * j) * m + e));
 			}
 			break;
 		}
@@ -440,6 +440,11 @@ b3g_status validate(const b3g_body_desc& d) {
 	    !chk(d.sh_p, d.n_shear, 3, "shear force") || !chk(d.sp_p, d.n_spring, 2, "spring force") ||
 	    !chk(d.mo_p, d.n_mouse, 4, "mouse force"))
 		return B3G_INVALID_ARGUMENT;
+	// 16 blocks per tetrahedron must fit the 31-bit contribution codes
+	if (d.n_tet >= (1u << 27)) {
+		set_error("more than 2^27 - 1 tetrahedron elements");
+		return B3G_UNSUPPORTED;
+	}
 	const uint32_t counts[7] = {0, d.n_stretch, d.n_shear, d.n_spring, d.n_mouse, d.n_tri, d.n_tet};
 	std::vector<std::vector<char>> used(7);
 	for (int t = 1; t < 7; ++t) used[t].assign(counts[t], 0);
@@ -616,8 +621,6 @@ b3g_status build_body(Body& b, const b3g_body_desc& d) {
 	auto take = [](uint32_t& cur, uint32_t count) { uint32_t base = cur; cur += count; return base; };
 	b.tet.m = d.n_tet;
 	b.tet.anyDamping = std::any_of(tetKd.begin(), tetKd.end(), [](float k) { return k > 0.0f; });
-	b.tet.jx = take(J, 16 * d.n_tet);
-	b.tet.jv = b.tet.anyDamping ? take(J, 16 * d.n_tet) : 0;
 	b.tet.f = take(F, 4 * d.n_tet);
 	b.tet.fd = b.tet.anyDamping ? take(F, 4 * d.n_tet) : 0;
 	b.tri.m = d.n_tri;
@@ -687,7 +690,7 @@ b3g_status build_body(Body& b, const b3g_body_desc& d) {
 	si.shP = &shP; si.shM = d.n_shear; si.shCoef = &shCoef;
 	si.spP = &spP; si.spM = d.n_spring; si.spCoef = &spCoef;
 	si.moP = &moP; si.moM = d.n_mouse; si.moCoef = &moCoef;
-	si.tetJx = b.tet.jx; si.tetJv = b.tet.jv; si.tetF = b.tet.f; si.tetFd = b.tet.fd;
+	si.tetF = b.tet.f; si.tetFd = b.tet.fd;
 	si.triJx = b.tri.jx; si.triJv = b.tri.jv; si.triF = b.tri.f; si.triFd = b.tri.fd;
 	si.stJx = b.stretch.jx; si.stJv = b.stretch.jv; si.stF = b.stretch.f;
 	si.shJx = b.shear.jx; si.shJv = b.shear.jv; si.shF = b.shear.f;
@@ -698,6 +701,45 @@ b3g_status build_body(Body& b, const b3g_body_desc& d) {
 	b.nnz = P.rowptr[n];
 	b.hostRowptr = P.rowptr;
 	b.hostCols = P.cols;
+	// tetrahedron entries: code kTetCode | e, and the source block of each
+	// position for the contribution-ordered copy of K0
+	std::vector<uint32_t> permX, permV;
+	auto tet_codes = [&](std::vector<uint32_t>& idx, std::vector<uint32_t>& perm) {
+		if (!d.n_tet) return;
+		perm.assign(idx.size(), 0xFFFFFFFFu);
+		for (size_t k = 0; k < idx.size(); ++k)
+			if (idx[k] & kTetCode) {
+				const uint32_t src = idx[k] & ~kTetCode;
+				perm[k] = src;
+				idx[k] = kTetCode | (src % d.n_tet);
+			}
+	};
+	tet_codes(P.jxIdx, permX);
+	if (b.tet.anyDamping) tet_codes(P.jvIdx, permV);
+	b.nJx = (uint32_t)permX.size();
+	b.nJv = (uint32_t)permV.size();
+	b.anyJv = !P.jvIdx.empty();
+	// slot-assembly tiles: <= 256 slots, <= 1024 contributions of each list
+	// (one LDS chunk of the slot assembly)
+	{
+		const uint32_t kSlots = 256, kCap = 1024;
+		std::vector<uint32_t> t{0};
+		uint32_t slots = 0, cx = 0, cv = 0;
+		for (uint32_t s = 0; s < b.nnz; ++s) {
+			const uint32_t lx = P.jxPtr[s + 1] - P.jxPtr[s], lv = P.jvPtr[s + 1] - P.jvPtr[s];
+			if (slots > 0 && (slots == kSlots || cx + lx > kCap || cv + lv > kCap)) {
+				t.push_back(s);
+				slots = cx = cv = 0;
+			}
+			slots += 1;
+			cx += lx;
+			cv += lv;
+		}
+		t.push_back(b.nnz);
+		b.nSlotTiles = (uint32_t)t.size() - 1;
+		st = dupload(b, b.slotTiles, t.data(), t.size() * 4);
+		if (st != B3G_OK) return st;
+	}
 
 	// ------------------------------------------------------------ device uploads
 #define UP(buf, vec) do { b3g_status s_ = dupload(b, buf, (vec).data(), (vec).size() * sizeof((vec)[0])); if (s_ != B3G_OK) return s_; } while (0)
@@ -730,7 +772,7 @@ b3g_status build_body(Body& b, const b3g_body_desc& d) {
 		AL(b.tet.invE, (size_t)d.n_tet * 9 * 4);
 		AL(b.tet.K0, (size_t)d.n_tet * 144 * 4);
 		std::vector<float> q(4 * (size_t)d.n_tet, 0.0f);
-		for (uint32_t e = 0; e < d.n_tet; ++e) q[3 * (size_t)d.n_tet + e] = 1.0f;   // identity (x y z s planes)
+		for (uint32_t e = 0; e < d.n_tet; ++e) q[4 * (size_t)e + 3] = 1.0f;   // identity (x y z s per element)
 		UP(b.tet.q, q);
 	}
 	if (d.n_tri) {
@@ -796,6 +838,10 @@ b3g_status build_body(Body& b, const b3g_body_desc& d) {
 	if (d.n_tet) {
 		st = launch_tet_setup(b);
 		if (st != B3G_OK) return st;
+		st = launch_tet_perm(b, permX, b.K0x);
+		if (st != B3G_OK) return st;
+		st = launch_tet_perm(b, permV, b.K0v);
+		if (st != B3G_OK) return st;
 	}
 	B3G_HIP(hipStreamSynchronize(b.stream));
 	b.built = true;
@@ -841,7 +887,7 @@ b3g_status host_pattern(const b3g_body_desc& d, std::vector<uint32_t>& rowptr, s
 	si.shP = &shP; si.shM = d.n_shear; si.shCoef = &shCoef;
 	si.spP = &spP; si.spM = d.n_spring; si.spCoef = &spCoef;
 	si.moP = &moP; si.moM = d.n_mouse; si.moCoef = &moCoef;
-	si.tetJx = si.tetJv = si.tetF = si.tetFd = si.triJx = si.triJv = si.triF = si.triFd = 0;
+	si.tetF = si.tetFd = si.triJx = si.triJv = si.triF = si.triFd = 0;
 	si.stJx = si.stJv = si.stF = si.shJx = si.shJv = si.shF = si.spJx = si.spJv = si.spF = si.moJx = si.moJv = si.moF = 0;
 	Pattern P;
 	build_pattern(si, P);

@@ -245,14 +245,15 @@ b3g_status b3g_load_state(b3g_body body, const void* blob, uint64_t bytes);
 b3g_status b3g_synchronize(b3g_body body);
 
 /*
- * Kernel timing for roofline reporting: while enabled, the PCG's block-sparse
- * matrix-vector kernel is bracketed by HIP events on the body's stream.
- * b3g_get_spmv_timing returns the number of timed launches, their mean
- * duration in milliseconds and the algorithmic bytes one launch moves
- * (40 per 3x3 block + 4 per row offset + 24 per row for the vector in/out).
+ * Roofline pass (no counterpart in the reference): `launches` back-to-back
+ * launches of the PCG's block-sparse SpMV kernel -- one PCG iteration's full
+ * work, q = pA d with d = s + beta d_old, on the body's current pA and its PCG
+ * scratch vectors -- between one HIP event pair on the body's stream.
+ * *mean_ms = elapsed / launches; *bytes_per_launch = the algorithmic bytes of
+ * one launch (40 per 3x3 block + 4 per row offset + 24 per row for the vector
+ * in/out). Leaves positions, velocities and the solver state unchanged.
  */
-b3g_status b3g_set_kernel_timing(b3g_body body, int enabled);
-b3g_status b3g_get_spmv_timing(b3g_body body, uint32_t* launches, double* mean_ms, double* bytes_per_launch);
+b3g_status b3g_time_spmv(b3g_body body, uint32_t launches, double* mean_ms, double* bytes_per_launch);
 
 /*
  * Host-only diagnostics: the block pattern of pA for a description (list-order

@@ -179,3 +179,24 @@ def test_large_body_properties():
     static = arr["p_type"] == 0
     assert np.array_equal(x[static], x0[static]) and not np.any(v[static])
     assert np.all(np.isfinite(x)) and np.all(np.isfinite(v))
+
+
+def test_spmv_timing_pass_leaves_state_unchanged():
+    """The roofline pass runs the PCG SpMV kernel on scratch only: positions,
+    velocities and the next step are the same as without it."""
+    arr = configs.tet_grid(6).arrays()
+    a, b = Body(arr), Body(arr)
+    for body in (a, b):
+        body.Step(configs.DT, 1, 25, 0.0, 1e-6)
+    ms, bytes_per = a.time_spmv(20)
+    assert ms > 0.0
+    rows, blocks = a.system_size()
+    assert bytes_per == 40.0 * blocks + 4.0 * (rows + 1) + 24.0 * rows
+    xa, va = a.state()
+    xb, vb = b.state()
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+    for body in (a, b):
+        body.Step(configs.DT, 1, 25, 0.0, 1e-6)
+    xa, va = a.state()
+    xb, vb = b.state()
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb)
