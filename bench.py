@@ -5,13 +5,16 @@ Workload (one GPU): a 64 x 64 x 64-cell tet grid (274,625 particles,
 jitter U(-0.005, 0.005) m, static top face) plus a 512 x 512-vertex cloth
 (stretch k = 1e4, shear k = 1e3, pinned corners) draping over a 0.5 m sphere;
 dt = 1/60 s, one Newton iteration, PCG tolerance 1e-5 with at most 50
-iterations. One step advances both bodies (b3Body::Step on each).
+iterations. One step advances both bodies (b3Body::Step on each), as two
+handles on their own streams (--batch: one batch of two instances, stepped by
+one launch sequence).
 
 Prints one JSON line: steps/s (whole job), the roofline of the dominant
-kernel (the PCG's block-sparse SpMV on the tet body, timed with HIP events on
-its stream over the timed region) and the CPU baseline (the reference library
-compiled from its own sources, oracle/_ref, timed on this host's cores on a
-bounded sample; the oracle's restatement stands in when that build is absent).
+kernel (the PCG's block-sparse SpMV, timed with one HIP event pair around
+back-to-back launches on the system of the last timed step) and the CPU
+baseline (the reference library compiled from its own sources, oracle/_ref,
+timed on this host's cores on a bounded sample; the oracle's restatement
+stands in when that build is absent).
 """
 
 from __future__ import annotations
@@ -77,6 +80,8 @@ def main():
     ap.add_argument("--sub-tolerance", type=float, default=1e-5)
     ap.add_argument("--cpu-sample-steps", type=int, default=1)
     ap.add_argument("--spmv-launches", type=int, default=200, help="launches of the SpMV roofline pass")
+    ap.add_argument("--batch", action="store_true",
+                    help="step the two bodies as one batch of two instances instead of two handles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -95,35 +100,45 @@ def main():
     log(f"scenes built in {time.time() - t0:.1f} s: tet {len(tet_arr['p_type'])} particles, "
         f"cloth {len(cloth_arr['p_type'])} particles")
     t0 = time.time()
-    tet = Body(tet_arr)
-    cloth = Body(cloth_arr)
-    log(f"device bodies built in {time.time() - t0:.1f} s; tet system {tet.system_size()}, cloth {cloth.system_size()}")
+    if not args.batch:
+        tet, cloth = Body(tet_arr), Body(cloth_arr)
+        bodies = [tet, cloth]
+        roof = tet
+    else:
+        both = Body([tet_arr, cloth_arr])   # one batch: one launch sequence steps both bodies
+        bodies = [both]
+        roof = both
+    log(f"device bodies built in {time.time() - t0:.1f} s; systems {[b.system_size() for b in bodies]}")
 
     def step():
-        tet.Step(dt, 1, args.sub_iterations, 0.0, args.sub_tolerance)
-        cloth.Step(dt, 1, args.sub_iterations, 0.0, args.sub_tolerance)
+        for b in bodies:
+            b.Step(dt, 1, args.sub_iterations, 0.0, args.sub_tolerance)
+
+    def sync():
+        for b in bodies:
+            b.synchronize()
 
     for _ in range(args.warmup):
         step()
-    cloth.synchronize()
-    tet.synchronize()
+    sync()
     log(f"warmup done ({args.warmup} steps)")
 
-    tet.synchronize()
-    cloth.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
-    tet.synchronize()
-    cloth.synchronize()
+    sync()
     elapsed = time.perf_counter() - t0
-    # roofline pass: the PCG SpMV kernel back-to-back on the tet body's system of the last step
-    mean_ms, bytes_per_launch = tet.time_spmv(args.spmv_launches)
-    stats, _ = tet.stats()
-    cstats, ccontacts = cloth.stats()
-    x, v = tet.state()
-    if not (np.all(np.isfinite(x)) and np.all(np.isfinite(v))):
-        raise SystemExit("non-finite state after the timed steps")
+    # roofline pass: the PCG SpMV kernel back-to-back on the system of the last step
+    mean_ms, bytes_per_launch = roof.time_spmv(args.spmv_launches)
+    if not args.batch:
+        stats, _ = tet.stats()
+        cstats, ccontacts = cloth.stats()
+    else:
+        (stats, _), (cstats, ccontacts) = both.instance_stats()
+    for b in bodies:
+        x, v = b.state()
+        if not (np.all(np.isfinite(x)) and np.all(np.isfinite(v))):
+            raise SystemExit("non-finite state after the timed steps")
     value = args.steps / elapsed
     log(f"timed {args.steps} steps in {elapsed:.3f} s; tet PCG iterations min/max {stats[1]}/{stats[2]}, "
         f"cloth {cstats[1]}/{cstats[2]}, cloth contacts {ccontacts}; SpMV pass {args.spmv_launches} launches, "
@@ -156,15 +171,17 @@ def main():
             "pcg_max_iterations": args.sub_iterations,
             "pcg_tolerance": args.sub_tolerance,
             "parallelism": "single GPU",
+            "form": "two handles" if not args.batch else "one batch of two instances",
         },
         "roofline": {
-            "kernel": "k_cg_spmv_dq (tet body, back-to-back pass after the timed steps)",
+            "kernel": "k_cg_spmv_dq (" + ("tet body" if not args.batch else "both bodies, one batched launch")
+                      + ", back-to-back pass after the timed steps)",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profile(),
+            "traffic": traffic_from_profile() if not args.batch else None,
             "bytes_per_launch": bytes_per_launch,
             "launches": args.spmv_launches,
             "mean_launch_us": round(mean_ms * 1e3, 2),

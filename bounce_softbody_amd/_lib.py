@@ -19,7 +19,7 @@ f32p = C.POINTER(C.c_float)
 u8p = C.POINTER(C.c_uint8)
 
 B3G_OK = 0
-REDUCE_FP64, REDUCE_SEQUENTIAL = 0, 1
+REDUCE_FP64, REDUCE_SEQUENTIAL, REDUCE_FP64_FUSED = 0, 1, 2
 STATUS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "out of device memory", 4: "degenerate element",
           5: "unsupported", 6: "body not built"}
 
@@ -107,7 +107,7 @@ _lib = None
 
 EXPORTS = [
     "b3g_status_string", "b3g_last_error", "b3g_device_count", "b3g_body_create", "b3g_body_destroy",
-    "b3g_body_build", "b3g_step", "b3g_get_state", "b3g_set_state", "b3g_apply_external", "b3g_get_masses",
+    "b3g_body_build", "b3g_body_build_batch", "b3g_get_instance_count", "b3g_step", "b3g_get_state", "b3g_set_state", "b3g_apply_external", "b3g_get_masses",
     "b3g_get_stats", "b3g_get_energy", "b3g_get_contacts", "b3g_get_tet_quats", "b3g_assemble", "b3g_get_system",
     "b3g_synchronize", "b3g_time_spmv", "b3g_get_system_size", "b3g_pattern",
     "b3g_solve_cg", "b3g_spmv", "b3g_set_reduction", "b3g_save_state", "b3g_load_state",
@@ -130,6 +130,8 @@ def lib():
         "b3g_body_create": (C.c_int, [C.POINTER(vp), C.c_int]),
         "b3g_body_destroy": (C.c_int, [vp]),
         "b3g_body_build": (C.c_int, [vp, C.POINTER(BodyDesc)]),
+        "b3g_body_build_batch": (C.c_int, [vp, C.POINTER(BodyDesc), C.c_uint32]),
+        "b3g_get_instance_count": (C.c_int, [vp, u32p]),
         "b3g_step": (C.c_int, [vp, C.c_float, C.c_uint32, C.c_uint32, C.c_float, C.c_float]),
         "b3g_get_state": (C.c_int, [vp, f32p, f32p]),
         "b3g_set_state": (C.c_int, [vp, f32p, f32p]),

@@ -22,9 +22,15 @@ f32 = np.float32
 
 
 class Body:
-    """A soft body stepped on the GPU (``b3Body``)."""
+    """A soft body stepped on the GPU (``b3Body``).
 
-    def __init__(self, scene: SceneBuilder | dict | None = None, device: int = 0):
+    Given a list of scenes, the handle holds a batch: every instance is an
+    independent body with its own solver state, stepped by one launch
+    sequence. Per-particle arrays are then the instances' arrays
+    concatenated in list order of the scenes.
+    """
+
+    def __init__(self, scene: SceneBuilder | dict | list | None = None, device: int = 0):
         self._L = _lib.lib()
         self._h = C.c_void_p()
         check(self._L.b3g_body_create(C.byref(self._h), device), "b3g_body_create")
@@ -34,6 +40,7 @@ class Body:
         self._dirty = True
         self.n = 0
         self.n_tet = 0
+        self.n_inst = 1
         if scene is not None:
             self.commit()
 
@@ -46,13 +53,29 @@ class Body:
     # ------------------------------------------------------------ building
     def commit(self):
         """Uploads the description (particles, fixtures, forces, world fixtures)."""
-        arr = self.scene if isinstance(self.scene, dict) else self.scene.arrays()
-        desc, keep = _lib.make_desc(arr)
-        check(self._L.b3g_body_build(self._h, C.byref(desc)), "b3g_body_build")
-        self._arrays = arr
-        self.n = len(arr["p_type"])
-        self.n_tet = len(arr["tet_param"]) // 3
+        scenes = self.scene if isinstance(self.scene, (list, tuple)) else [self.scene]
+        arrs = [sc if isinstance(sc, dict) else sc.arrays() for sc in scenes]
+        made = [_lib.make_desc(a) for a in arrs]
+        if len(arrs) == 1:
+            check(self._L.b3g_body_build(self._h, C.byref(made[0][0])), "b3g_body_build")
+        else:
+            descs = (_lib.BodyDesc * len(made))(*[m[0] for m in made])
+            check(self._L.b3g_body_build_batch(self._h, descs, len(made)), "b3g_body_build_batch")
+        self._arrays = arrs[0] if len(arrs) == 1 else arrs
+        self.n_inst = len(arrs)
+        self.inst_particles = [len(a["p_type"]) for a in arrs]
+        self.inst_tets = [len(a["tet_param"]) // 3 for a in arrs]
+        self.n = sum(self.inst_particles)
+        self.n_tet = sum(self.inst_tets)
         self._dirty = False
+
+    def split(self, rows):
+        """Per-instance views of a concatenated per-particle array."""
+        out, o = [], 0
+        for c in self.inst_particles:
+            out.append(rows[o:o + c])
+            o += c
+        return out
 
     def SetGravity(self, g):   # noqa: N802 (reference name)
         if isinstance(self.scene, SceneBuilder):
@@ -97,15 +120,24 @@ class Body:
         check(self._L.b3g_get_masses(self._h, _ptr(m, f32p)), "b3g_get_masses")
         return m
 
+    def instance_stats(self):
+        """[(newton, min sub, max sub) array, contact count] per instance."""
+        s = (StepStats * self.n_inst)()
+        check(self._L.b3g_get_stats(self._h, s), "b3g_get_stats")
+        return [(np.array([x.newton_iterations, x.min_sub_iterations, x.max_sub_iterations], np.uint32), x.contact_count)
+                for x in s]
+
     def stats(self):
-        s = StepStats()
-        check(self._L.b3g_get_stats(self._h, C.byref(s)), "b3g_get_stats")
-        return np.array([s.newton_iterations, s.min_sub_iterations, s.max_sub_iterations], np.uint32), s.contact_count
+        """Statistics of the first (or only) instance."""
+        return self.instance_stats()[0]
+
+    def energies(self):
+        e = (C.c_float * self.n_inst)()
+        check(self._L.b3g_get_energy(self._h, e), "b3g_get_energy")
+        return [float(x) for x in e]
 
     def GetEnergy(self):   # noqa: N802
-        e = C.c_float(0)
-        check(self._L.b3g_get_energy(self._h, C.byref(e)), "b3g_get_energy")
-        return e.value
+        return self.energies()[0]
 
     def contacts(self):
         cnt = C.c_uint32(0)

@@ -118,7 +118,21 @@ b3g_status b3g_body_build(b3g_body h, const b3g_body_desc* d) {
 	CHECK_BODY(h);
 	if (!d) { set_error("desc is NULL"); return B3G_INVALID_ARGUMENT; }
 	SET_DEVICE(h);
-	return build_body(h->body, *d);
+	return build_body(h->body, d, 1);
+}
+
+b3g_status b3g_body_build_batch(b3g_body h, const b3g_body_desc* descs, uint32_t count) {
+	CHECK_BODY(h);
+	if (!descs || count == 0) { set_error("a batch needs at least one description"); return B3G_INVALID_ARGUMENT; }
+	SET_DEVICE(h);
+	return build_body(h->body, descs, count);
+}
+
+b3g_status b3g_get_instance_count(b3g_body h, uint32_t* count) {
+	CHECK_BUILT(h);
+	if (!count) return B3G_INVALID_ARGUMENT;
+	*count = h->body.nInst;
+	return B3G_OK;
 }
 
 b3g_status b3g_step(b3g_body h, float dt, uint32_t iterations, uint32_t sub_iterations, float tolerance,
@@ -189,20 +203,22 @@ b3g_status b3g_get_stats(b3g_body h, b3g_step_stats* out) {
 	if (!out) return B3G_INVALID_ARGUMENT;
 	SET_DEVICE(h);
 	Body& b = h->body;
-	SolverState st;
-	B3G_HIP(hipMemcpyAsync(&st, b.state.p, sizeof(st), hipMemcpyDeviceToHost, b.stream));
+	std::vector<SolverState> st(b.nInst);
 	B3G_HIP(hipStreamSynchronize(b.stream));
-	out->newton_iterations = st.statNewton;
-	out->min_sub_iterations = st.statMinSub;
-	out->max_sub_iterations = st.statMaxSub;
-	uint32_t count = 0;
-	if (b.nWorld && b.nSpheres) {
-		std::vector<uint32_t> act((size_t)b.nSpheres * b.nWorld);
-		B3G_HIP(hipStreamSynchronize(b.stream));
+	B3G_HIP(hipMemcpy(st.data(), b.state.p, st.size() * sizeof(SolverState), hipMemcpyDeviceToHost));
+	std::vector<uint32_t> count(b.nInst, 0);
+	if (b.nPairs) {
+		std::vector<uint32_t> act(b.nPairs);
 		B3G_HIP(hipMemcpy(act.data(), b.cActive.p, act.size() * 4, hipMemcpyDeviceToHost));
-		for (uint32_t a : act) count += a;
+		for (uint32_t k = 0; k < b.nPairs; ++k)
+			if (act[k]) count[b.hostRowInst[b.hostSphereParticle[b.hostPairSphere[k]]]] += 1;
 	}
-	out->contact_count = count;
+	for (uint32_t k = 0; k < b.nInst; ++k) {
+		out[k].newton_iterations = st[k].statNewton;
+		out[k].min_sub_iterations = st[k].statMinSub;
+		out[k].max_sub_iterations = st[k].statMaxSub;
+		out[k].contact_count = count[k];
+	}
 	return B3G_OK;
 }
 
@@ -214,9 +230,11 @@ b3g_status b3g_get_energy(b3g_body h, float* energy) {
 	std::vector<V3> v(b.n);
 	B3G_HIP(hipMemcpyAsync(v.data(), b.v.p, b.n * sizeof(V3), hipMemcpyDeviceToHost, b.stream));
 	B3G_HIP(hipStreamSynchronize(b.stream));
-	float E = 0.0f;
-	for (uint32_t i = 0; i < b.n; ++i) E += b.hostMass[i] * dot(v[i], v[i]);   // body.cpp:384-392
-	*energy = 0.5f * E;
+	for (uint32_t k = 0; k < b.nInst; ++k) {
+		float E = 0.0f;
+		for (uint32_t i = b.hostRowBegin[k]; i < b.hostRowEnd[k]; ++i) E += b.hostMass[i] * dot(v[i], v[i]);   // body.cpp:384-392
+		energy[k] = 0.5f * E;
+	}
 	return B3G_OK;
 }
 
@@ -227,34 +245,34 @@ b3g_status b3g_get_contacts(b3g_body h, uint32_t* count, uint32_t* particle, uin
 	SET_DEVICE(h);
 	Body& b = h->body;
 	*count = 0;
-	if (!b.nWorld || !b.nSpheres) return B3G_OK;
-	const size_t pairs = (size_t)b.nSpheres * b.nWorld;
-	std::vector<uint32_t> act(pairs), stamp(pairs), sp(b.nSpheres);
+	if (!b.nPairs) return B3G_OK;
+	const size_t pairs = b.nPairs;
+	std::vector<uint32_t> act(pairs), stamp(pairs);
 	std::vector<float> nf(pairs);
 	B3G_HIP(hipStreamSynchronize(b.stream));
 	B3G_HIP(hipMemcpy(act.data(), b.cActive.p, pairs * 4, hipMemcpyDeviceToHost));
 	B3G_HIP(hipMemcpy(stamp.data(), b.cStamp.p, pairs * 4, hipMemcpyDeviceToHost));
 	B3G_HIP(hipMemcpy(nf.data(), b.cNormalForce.p, pairs * 4, hipMemcpyDeviceToHost));
-	B3G_HIP(hipMemcpy(sp.data(), b.sphereParticle.p, b.nSpheres * 4, hipMemcpyDeviceToHost));
-	// List order (contact_manager.cpp:54-62): newest FindNewContacts call first; within a
-	// call the last created pair (sphere list order, then world list order) is the head.
-	std::vector<std::tuple<uint32_t, uint32_t, uint32_t, size_t>> keys;
+	// Per instance (creation order), the list order (contact_manager.cpp:54-62):
+	// newest FindNewContacts call first; within a call the last created pair
+	// (sphere list order, then world list order) is the head.
+	struct Key { uint32_t inst, call, s, w; size_t k; };
+	std::vector<Key> keys;
 	for (size_t k = 0; k < pairs; ++k)
 		if (act[k]) {
-			uint32_t s = (uint32_t)(k / b.nWorld), w = (uint32_t)(k % b.nWorld);
-			keys.emplace_back(stamp[k] / b.nWorld, s, w, k);
+			const uint32_t s = b.hostPairSphere[k], p0 = b.hostPairStart[s], nw = b.hostPairStart[s + 1] - p0;
+			keys.push_back({b.hostRowInst[b.hostSphereParticle[s]], stamp[k] / nw, s, (uint32_t)(k - p0), k});
 		}
-	std::sort(keys.begin(), keys.end(), [](const auto& a, const auto& c) {
-		return std::make_tuple(std::get<0>(a), std::get<1>(a), std::get<2>(a)) >
-		       std::make_tuple(std::get<0>(c), std::get<1>(c), std::get<2>(c));
+	std::sort(keys.begin(), keys.end(), [](const Key& a, const Key& c) {
+		if (a.inst != c.inst) return a.inst < c.inst;
+		return std::make_tuple(a.call, a.s, a.w) > std::make_tuple(c.call, c.s, c.w);
 	});
 	*count = (uint32_t)keys.size();
 	for (size_t i = 0; i < keys.size() && i < capacity; ++i) {
-		size_t k = std::get<3>(keys[i]);
-		uint32_t s = std::get<1>(keys[i]), w = std::get<2>(keys[i]);
-		if (particle) particle[i] = b.n - 1 - sp[s];
-		if (world) world[i] = b.nWorld - 1 - w;
-		if (normal_force) normal_force[i] = nf[k];
+		const Key& kk = keys[i];
+		if (particle) particle[i] = b.n - 1 - b.hostSphereParticle[kk.s];
+		if (world) world[i] = b.nWorld - 1 - b.hostPairWorld[kk.k];
+		if (normal_force) normal_force[i] = nf[kk.k];
 	}
 	return B3G_OK;
 }
@@ -308,7 +326,10 @@ b3g_status b3g_get_system(b3g_body h, uint32_t* rowptr, uint32_t* cols, float* v
 
 b3g_status b3g_set_reduction(b3g_body h, int mode) {
 	CHECK_BODY(h);
-	if (mode != B3G_REDUCE_FP64 && mode != B3G_REDUCE_SEQUENTIAL) { set_error("unknown reduction mode"); return B3G_INVALID_ARGUMENT; }
+	if (mode != B3G_REDUCE_FP64 && mode != B3G_REDUCE_SEQUENTIAL && mode != B3G_REDUCE_FP64_FUSED) {
+		set_error("unknown reduction mode");
+		return B3G_INVALID_ARGUMENT;
+	}
 	h->body.reduction = mode;
 	return B3G_OK;
 }
@@ -317,10 +338,10 @@ b3g_status b3g_set_reduction(b3g_body h, int mode) {
 
 namespace {
 struct StateHeader {
-	uint32_t magic, version, n, nTet, pairs, stampCounter, statNewton, statMinSub, statMaxSub, reserved;
+	uint32_t magic, version, n, nTet, pairs, stampCounter, nInst, reserved;
 };
 constexpr uint32_t kStateMagic = 0x53473342u;   // "B3GS"
-constexpr uint32_t kStateVersion = 2;            // 2: tetrahedron quaternions stored per element
+constexpr uint32_t kStateVersion = 3;            // 3: batches (per-instance solver statistics)
 
 struct StatePart {
 	DBuf* buf;
@@ -328,9 +349,9 @@ struct StatePart {
 };
 
 std::vector<StatePart> state_parts(Body& b) {
-	const size_t pairs = (b.nWorld && b.nSpheres) ? (size_t)b.nSpheres * b.nWorld : 0;
+	const size_t pairs = b.nPairs;
 	std::vector<StatePart> parts = {{&b.x, b.n * sizeof(V3)}, {&b.v, b.n * sizeof(V3)}, {&b.force, b.n * sizeof(V3)},
-	                                {&b.translation, b.n * sizeof(V3)}};
+	                                {&b.translation, b.n * sizeof(V3)}, {&b.state, b.nInst * sizeof(SolverState)}};
 	if (b.tet.m) parts.push_back({&b.tet.q, (size_t)b.tet.m * 16});
 	if (pairs) {
 		parts.push_back({&b.cActive, pairs * 4});
@@ -356,10 +377,7 @@ b3g_status b3g_save_state(b3g_body h, void* blob, uint64_t* bytes) {
 	if (!blob) { *bytes = total; return B3G_OK; }
 	if (*bytes < total) { set_error("state blob too small"); return B3G_INVALID_ARGUMENT; }
 	B3G_HIP(hipStreamSynchronize(b.stream));
-	SolverState st;
-	B3G_HIP(hipMemcpy(&st, b.state.p, sizeof(st), hipMemcpyDeviceToHost));
-	StateHeader hd = {kStateMagic, kStateVersion, b.n, b.tet.m, (uint32_t)((b.nWorld && b.nSpheres) ? b.nSpheres * b.nWorld : 0),
-	                  b.stampCounter, st.statNewton, st.statMinSub, st.statMaxSub, 0};
+	StateHeader hd = {kStateMagic, kStateVersion, b.n, b.tet.m, b.nPairs, b.stampCounter, b.nInst, 0};
 	char* out = static_cast<char*>(blob);
 	std::memcpy(out, &hd, sizeof(hd));
 	size_t off = sizeof(hd);
@@ -382,9 +400,8 @@ b3g_status b3g_load_state(b3g_body h, const void* blob, uint64_t bytes) {
 	StateHeader hd;
 	if (bytes < sizeof(hd)) { set_error("state blob too small"); return B3G_INVALID_ARGUMENT; }
 	std::memcpy(&hd, blob, sizeof(hd));
-	const uint32_t pairs = (b.nWorld && b.nSpheres) ? b.nSpheres * b.nWorld : 0;
-	if (hd.magic != kStateMagic || hd.version != kStateVersion || hd.n != b.n || hd.nTet != b.tet.m || hd.pairs != pairs ||
-	    bytes != total) {
+	if (hd.magic != kStateMagic || hd.version != kStateVersion || hd.n != b.n || hd.nTet != b.tet.m || hd.pairs != b.nPairs ||
+	    hd.nInst != b.nInst || bytes != total) {
 		set_error("state blob does not match this body");
 		return B3G_INVALID_ARGUMENT;
 	}
@@ -396,12 +413,6 @@ b3g_status b3g_load_state(b3g_body h, const void* blob, uint64_t bytes) {
 		off += p.bytes;
 	}
 	b.stampCounter = hd.stampCounter;
-	SolverState st;
-	B3G_HIP(hipMemcpy(&st, b.state.p, sizeof(st), hipMemcpyDeviceToHost));
-	st.statNewton = hd.statNewton;
-	st.statMinSub = hd.statMinSub;
-	st.statMaxSub = hd.statMaxSub;
-	B3G_HIP(hipMemcpy(b.state.p, &st, sizeof(st), hipMemcpyHostToDevice));
 	return B3G_OK;
 }
 

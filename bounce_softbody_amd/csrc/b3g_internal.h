@@ -98,11 +98,18 @@ struct WorldShape {
 	AABB aabb;               // b3Shape::ComputeAABB of the fixture (static)
 };
 
+// Per-instance, per-iteration scalars of one PCG solve (sparse_solver.cpp:57-103):
+// entry k holds delta_k = r_k.s_k, alpha_k, beta_k and whether iteration k runs.
+struct CGLog {
+	float delta, alpha, beta;
+	uint32_t active;
+};
+
 // Device-side scalar state of the Newton and PCG loops.
 struct SolverState {
-	// PCG (sparse_solver.cpp:57-103)
-	float delta_new, delta0, alpha, beta, eps2_sub;
-	uint32_t it, active, maxIt;
+	// PCG (sparse_solver.cpp:57-103); per-iteration scalars are in the CGLog
+	float delta0, eps2_sub;
+	uint32_t it, maxIt;
 	// Newton (sparse_force_solver.cpp:141-225)
 	float error, error0, eps2;
 	uint32_t newtonIt, newtonActive, newtonMax;
@@ -114,17 +121,30 @@ struct SolverState {
 };
 
 
+// A body holds a batch of one or more independent instances (bodies of the
+// reference). Their particles, elements, blocks and contact pairs are
+// concatenated; every instance keeps its own list orders, its own Newton /
+// PCG state and convergence, and the reductions never mix instances.
+// Instance k (creation order) occupies the list-order rows
+// [rowBegin[k], rowEnd[k]); instances appear in reverse order along the rows
+// because list order reverses creation order.
 struct Body {
 	int device = 0;
 	hipStream_t stream = nullptr;
 	bool built = false;
 
 	// ---------------------------------------------------------------- sizes
-	uint32_t n = 0;          // particles
+	uint32_t n = 0;          // particles (all instances)
 	uint32_t nnz = 0;        // 3x3 blocks of pA
-	uint32_t nSpheres = 0, nWorld = 0;
+	uint32_t nInst = 1;      // instances
+	uint32_t nSpheres = 0, nWorld = 0, nPairs = 0;
 	uint32_t stampCounter = 0;
-	float gravity[3] = {0.0f, 0.0f, 0.0f};
+	std::vector<uint32_t> hostRowInst;            // instance of each row (empty = all instance 0)
+	std::vector<uint32_t> hostRowBegin, hostRowEnd;
+	std::vector<V3> hostGravity;                  // per instance
+	DBuf gravity;                                 // V3 per instance
+	DBuf rowInst;                                 // uint32 per row
+	DBuf instRange;                               // 6 uint32 per instance: tiles, row blocks, rows (begin, end)
 
 	// ---------------------------------------------------------------- particles (list order)
 	DBuf type, mass, invMass, damping;            // uint8, float x3
@@ -133,6 +153,7 @@ struct Body {
 	DBuf sphereOf;                                // int32 per particle: sphere list index or -1
 	DBuf sphereRadius, sphereFriction;            // per sphere (list order)
 	DBuf sphereParticle;                          // uint32 per sphere
+	std::vector<uint32_t> hostSphereParticle;
 	std::vector<float> hostMass;                  // list order
 	std::vector<uint8_t> hostType;
 
@@ -152,7 +173,12 @@ struct Body {
 	DBuf fPtr, fIdx;                              // per-row force contributions
 	DBuf tiles;                                   // uint32 row boundaries of SpMV / row-assembly tiles
 	uint32_t nTiles = 0;
+	DBuf tileInst;                                // uint32 instance per tile
+	DBuf rowBlk;                                  // uint32 boundaries of <= 256-row blocks (vector kernels)
+	DBuf blkInst;
+	uint32_t nRowBlk = 0;
 	DBuf slotTiles;                               // uint32 slot boundaries of slot-assembly tiles
+	DBuf slotTileInst;
 	uint32_t nSlotTiles = 0;
 	DBuf K0x, K0v;                                // tetrahedron K0 blocks in jx / jv contribution order (9 planes)
 	uint32_t nJx = 0, nJv = 0;                    // lengths of the jx / jv lists when K0x / K0v exist
@@ -164,15 +190,21 @@ struct Body {
 	DBuf tB;         // V3 per slot: (h dfdx_ij) w_j
 	DBuf pb, invM;   // V3 per row
 	DBuf px, r, s, q, d0, d1, fi;   // V3 per row (px = py warm start)
-	DBuf partial;    // double per block of the reduction kernels
+	DBuf partial;    // double per tile / row block: residual, vector-update and Newton reductions
+	DBuf partialQ;   // double per tile: the d.q reduction of the SpMV kernel
 	uint32_t partialCount = 0;
+	DBuf cgLog;      // CGLog per instance and PCG iteration
+	uint32_t logStride = 0;   // entries per instance
 	DBuf terms;      // float per row: per-row dot terms (sequential reduction mode)
 	int reduction = 0;
-	DBuf state;      // SolverState
+	DBuf state;      // SolverState per instance
 
-	// ---------------------------------------------------------------- contacts (sphere x world pairs)
+	// ---------------------------------------------------------------- contacts (sphere x world pairs of one instance)
 	DBuf worlds;     // WorldShape[nWorld]
 	DBuf voxels;     // float
+	DBuf pairStart;                               // uint32 per sphere + 1: its pairs, one per world fixture of its instance
+	DBuf pairSphere, pairWorld;                   // uint32 per pair
+	std::vector<uint32_t> hostPairStart, hostPairSphere, hostPairWorld;
 	DBuf cActive, cStamp, cFriction;              // uint32 per pair
 	DBuf cNormalForce;                            // float per pair
 	DBuf cManifold;                               // 2 V3 per pair (point, normal)
@@ -192,8 +224,8 @@ b3g_status dalloc(Body& b, DBuf& buf, size_t bytes);
 b3g_status dupload(Body& b, DBuf& buf, const void* host, size_t bytes);
 void dfree_all(Body& b);
 
-// Host build (b3g_setup.hip)
-b3g_status build_body(Body& b, const b3g_body_desc& d);
+// Host build (b3g_setup.hip): a batch of `count` instances.
+b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count);
 
 // Symbolic pattern on the host only (b3g_setup.hip), for diagnostics.
 b3g_status host_pattern(const b3g_body_desc& d, std::vector<uint32_t>& rowptr, std::vector<uint32_t>& cols);
@@ -210,5 +242,6 @@ b3g_status time_pcg_spmv(Body& b, uint32_t launches, double* mean_ms);
 b3g_status enqueue_solve_cg(Body& b, uint32_t maxIt, float eps);
 b3g_status build_tiles(Body& b);
 b3g_status ensure_partials(Body& b, uint32_t blocks);
+b3g_status ensure_cg_log(Body& b, uint32_t iterations);
 
 } // namespace b3g

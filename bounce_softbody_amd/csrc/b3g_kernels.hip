@@ -64,25 +64,28 @@ __device__ double block_sum(double v, double* sh) {
 
 enum Finalize : int { kFinAlpha = 0, kFinDelta = 1, kFinInit = 2, kFinNewton = 3 };
 
-__device__ void finalize(SolverState* st, int kind, double total) {
+// The PCG's state machine (sparse_solver.cpp:57-103) over the per-iteration
+// log: Init fills entry 0, Alpha(k) entry k's alpha, Delta(k) entry k from
+// the reduction that follows iteration k - 1. `it` counts Delta steps, as the
+// reference's iteration counter does.
+__device__ void finalize(SolverState* st, CGLog* log, int kind, uint32_t k, double total) {
 	float v = (float)total;
 	switch (kind) {
 	case kFinAlpha:   // sparse_solver.cpp:77
-		st->alpha = st->delta_new / v;
+		log[k].alpha = log[k].delta / v;
 		break;
 	case kFinInit:    // sparse_solver.cpp:57-73
-		st->delta_new = v;
 		st->delta0 = v;
 		st->it = 0;
-		st->beta = 0.0f;
-		st->active = (st->maxIt != 0 && !(v <= st->eps2_sub * v)) ? 1u : 0u;
+		log[0].delta = v;
+		log[0].beta = 0.0f;
+		log[0].active = (st->maxIt != 0 && !(v <= st->eps2_sub * v)) ? 1u : 0u;
 		break;
 	case kFinDelta: { // sparse_solver.cpp:92-102
-		float delta_old = st->delta_new;
-		st->delta_new = v;
-		st->beta = v / delta_old;
-		st->it += 1;
-		st->active = (st->it != st->maxIt && !(v <= st->eps2_sub * st->delta0)) ? 1u : 0u;
+		log[k].delta = v;
+		log[k].beta = v / log[k - 1].delta;
+		log[k].active = (k != st->maxIt && !(v <= st->eps2_sub * st->delta0)) ? 1u : 0u;
+		st->it = k;
 		break;
 	}
 	case kFinNewton: { // sparse_force_solver.cpp:196-225
@@ -100,12 +103,14 @@ __device__ void finalize(SolverState* st, int kind, double total) {
 	}
 }
 
-// Reduction of one scalar per row over the whole grid, in two launches.
-// The producing kernel stores one FP64 partial per workgroup (fixed tree)
-// and, in sequential mode, each row's FP32 term; k_finalize then adds them
-// in a fixed order and updates the solver state. The kernel boundary orders
-// the stores before the loads, so no device-scope fence is needed (on a
-// multi-XCD part such a fence writes back and invalidates a whole L2).
+// Reductions of one scalar per row over an instance. The producing kernel
+// stores one FP64 partial per tile / row block (fixed tree) and, in
+// sequential mode, each row's FP32 term. The sum is formed either by
+// k_finalize (one workgroup per instance) or, for the PCG's alpha and delta
+// in FP64 mode, by every workgroup of the consuming kernel in the same
+// fixed order (wg_sum). The kernel boundary orders the stores before the
+// loads, so no device-scope fence is needed (on a multi-XCD part such a
+// fence writes back and invalidates a whole L2).
 //   FP64 mode: deterministic, near-exact sum of the partials.
 //   Sequential mode: the terms are added one by one in row order starting
 //   from 0, which is the reference's b3Dot / b3LengthSquared accumulation
@@ -113,63 +118,94 @@ __device__ void finalize(SolverState* st, int kind, double total) {
 struct Reduce {
 	double* partial;
 	float* terms;
-	uint32_t n;
-	SolverState* st;
+	SolverState* st;         // per instance
+	const uint32_t* range;   // per instance: tiles, row blocks, rows (begin, end)
+	CGLog* log;              // per instance, logStride entries
+	uint32_t logStride;
 };
 
-__device__ __forceinline__ void block_partial(float term, bool has, uint32_t row, const Reduce& rd) {
+// One partial per tile / row block `slot` of instance `inst`.
+__device__ __forceinline__ void block_partial(float term, bool has, uint32_t row, double* partial, float* terms,
+                                              bool seq, uint32_t slot) {
 	__shared__ double sh[kBlock / 64];
-	if (rd.st->seq && has) rd.terms[row] = term;
+	if (seq && has) terms[row] = term;
 	double bs = block_sum(has ? (double)term : 0.0, sh);
-	if (threadIdx.x == 0) rd.partial[blockIdx.x] = bs;
+	if (threadIdx.x == 0) partial[slot] = bs;
 }
 
-constexpr int kFinBlock = 1024;
+constexpr int kFinBlock = 256;
 
-// Gate of the finalize launch: the same condition under which its producer
-// ran (a skipped producer leaves stale partials behind).
-enum Gate : int { kGateNewton = 0, kGateCG = 1 };
-
-__global__ void __launch_bounds__(kFinBlock) k_finalize(Reduce rd, uint32_t count, int kind, int gate) {
-	if (!rd.st->newtonActive) return;
-	if (gate == kGateCG && !rd.st->active) return;
+// Sum of partials [p0, p1) by a kFinBlock-thread workgroup in a fixed order
+// (thread t adds p0 + t, p0 + t + 256, ..., four loads in flight; then a
+// wave tree and the four wave sums in order). Valid in every thread.
+__device__ double wg_sum(const double* partial, uint32_t p0, uint32_t p1) {
 	__shared__ double sh[kFinBlock / 64];
-	__shared__ float chunk[4 * kFinBlock];
-	if (rd.st->seq) {
-		float acc = 0.0f;
-		for (uint32_t base = 0; base < rd.n; base += 4 * kFinBlock) {
-			for (uint32_t k = threadIdx.x; k < 4 * kFinBlock && base + k < rd.n; k += kFinBlock) chunk[k] = rd.terms[base + k];
-			__syncthreads();
-			if (threadIdx.x == 0) {
-				const uint32_t cnt = min((uint32_t)(4 * kFinBlock), rd.n - base);
-				for (uint32_t k = 0; k < cnt; ++k) acc += chunk[k];
-			}
-			__syncthreads();
-		}
-		if (threadIdx.x == 0) finalize(rd.st, kind, (double)acc);
-		return;
-	}
-	// fixed order: thread t adds partials t, t + 1024, ... (four loads in flight)
+	__shared__ double total;
 	double t = 0.0;
-	for (uint32_t base = 0; base < count; base += 4 * kFinBlock) {
+	for (uint32_t base = p0; base < p1; base += 4 * kFinBlock) {
 		double p[4];
 #pragma unroll
 		for (int u = 0; u < 4; ++u) {
 			const uint32_t k = base + threadIdx.x + u * kFinBlock;
-			p[u] = k < count ? rd.partial[k] : 0.0;
+			p[u] = k < p1 ? partial[k] : 0.0;
 		}
 #pragma unroll
 		for (int u = 0; u < 4; ++u) t += p[u];
 	}
 	for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	__syncthreads();
 	if (lane == 0) sh[wave] = t;
 	__syncthreads();
 	if (threadIdx.x == 0) {
 		double tot = 0.0;
 		for (int k = 0; k < kFinBlock / 64; ++k) tot += sh[k];
-		finalize(rd.st, kind, tot);
+		total = tot;
 	}
+	__syncthreads();
+	return total;
+}
+
+// Gate of a finalize launch: the same condition under which its producer
+// ran (a skipped producer leaves stale partials behind). Partials of the
+// producer are per tile (kRangeTiles) or per row block (kRangeBlocks).
+enum Gate : int { kGateNewton = 0, kGateIter = 1 };
+enum RangeSel : int { kRangeTiles = 0, kRangeBlocks = 2 };
+
+// A new solve: no iteration beyond the first is active until its Delta step
+// says so (entries of an earlier solve must not gate this one).
+__device__ __forceinline__ void clear_log(CGLog* log, uint32_t stride) {
+	for (uint32_t j = 1 + threadIdx.x; j < stride; j += blockDim.x) log[j].active = 0u;
+}
+
+// One workgroup per instance. kGateIter: the producer belonged to PCG
+// iteration `gk` and ran only if log[gk].active.
+__global__ void __launch_bounds__(kFinBlock) k_finalize(Reduce rd, int sel, int kind, uint32_t k, int gate, uint32_t gk) {
+	const uint32_t inst = blockIdx.x;
+	SolverState* st = rd.st + inst;
+	CGLog* log = rd.log + (size_t)inst * rd.logStride;
+	if (!st->newtonActive) return;
+	if (gate == kGateIter && !log[gk].active) return;
+	if (kind == kFinInit) clear_log(log, rd.logStride);
+	const uint32_t* R = rd.range + 6 * (size_t)inst;
+	__shared__ float chunk[4 * kFinBlock];
+	if (st->seq) {
+		const uint32_t r0 = R[4], r1 = R[5];
+		float acc = 0.0f;
+		for (uint32_t base = r0; base < r1; base += 4 * kFinBlock) {
+			for (uint32_t j = threadIdx.x; j < 4 * kFinBlock && base + j < r1; j += kFinBlock) chunk[j] = rd.terms[base + j];
+			__syncthreads();
+			if (threadIdx.x == 0) {
+				const uint32_t cnt = min((uint32_t)(4 * kFinBlock), r1 - base);
+				for (uint32_t j = 0; j < cnt; ++j) acc += chunk[j];
+			}
+			__syncthreads();
+		}
+		if (threadIdx.x == 0) finalize(st, log, kind, k, (double)acc);
+		return;
+	}
+	const double tot = wg_sum(rd.partial, R[sel], R[sel + 1]);
+	if (threadIdx.x == 0) finalize(st, log, kind, k, tot);
 }
 
 // ---------------------------------------------------------------------------
@@ -243,12 +279,16 @@ __global__ void __launch_bounds__(kBlock) k_tet_setup(uint32_t m, const float* r
 struct StepArgs {
 	uint32_t n;
 	float h;
-	V3 gravity;
+	const V3* gravity;          // per instance
+	const uint32_t* rowInst;
+	const uint32_t* rowBlk;     // row block boundaries
+	const uint32_t* blkInst;
 	const uint8_t* type;
 	const float* mass;
 	const float* damping;
 	V3 *x, *v, *force, *translation, *x0, *v0, *fe, *y, *w, *px;
-	SolverState* st;
+	SolverState* st;            // per instance
+	const uint32_t* range;
 };
 
 // force_solver.cpp:100-130 + sparse_force_solver.cpp:135-139
@@ -259,13 +299,17 @@ __global__ void __launch_bounds__(kBlock) k_step_begin(StepArgs a) {
 	a.x0[i] = x;
 	a.v0[i] = v;
 	V3 fe = a.force[i];
-	if (a.type[i] == kDynamic) fe = fe + a.mass[i] * a.gravity;
+	if (a.type[i] == kDynamic) fe = fe + a.mass[i] * a.gravity[a.rowInst[i]];
 	a.fe[i] = fe;
 	a.y[i] = a.translation[i];
 	a.px[i] = zero3();
 }
 
-__global__ void k_state_begin(SolverState* st, uint32_t iters, uint32_t subIters, float tol, float subTol, uint32_t seq) {
+__global__ void __launch_bounds__(kBlock) k_state_begin(SolverState* sts, uint32_t count, uint32_t iters, uint32_t subIters,
+                                                         float tol, float subTol, uint32_t seq) {
+	uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+	if (k >= count) return;
+	SolverState* st = sts + k;
 	st->seq = seq;
 	st->newtonIt = 0;
 	st->newtonMax = iters;
@@ -273,15 +317,14 @@ __global__ void k_state_begin(SolverState* st, uint32_t iters, uint32_t subIters
 	st->eps2 = tol * tol;
 	st->eps2_sub = subTol * subTol;
 	st->maxIt = subIters;
-	st->active = 0;
 	st->it = 0;
 }
 
 // w = x0 - x + h v + y (sparse_force_solver.cpp:167)
 __global__ void __launch_bounds__(kBlock) k_newton_prep(StepArgs a) {
-	if (!a.st->newtonActive) return;
 	uint32_t i = blockIdx.x * kBlock + threadIdx.x;
 	if (i >= a.n) return;
+	if (!a.st[a.rowInst[i]].newtonActive) return;
 	a.w[i] = ((a.x0[i] - a.x[i]) + a.h * a.v[i]) + a.y[i];
 }
 
@@ -300,7 +343,8 @@ struct TetArgs {
 	const V3* v;
 	V3* fstage;
 	uint32_t f, fd;
-	const SolverState* st;
+	const SolverState* st;   // per instance
+	const uint32_t* rowInst;
 };
 
 // tetrahedron_element_force.cpp:273-311 (b3ExtractRotation, 32 iterations)
@@ -333,9 +377,9 @@ __device__ Q4 extract_rotation(const M33& A, Q4 q) {
 // -R K0_ij R^T are not staged: the slot assembly recomputes them from the
 // stored rotation (same operations, same values).
 __global__ void __launch_bounds__(kBlock) k_tet_forces(TetArgs a) {
-	if (!a.st->newtonActive) return;
 	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= a.m) return;
+	if (!a.st[a.rowInst[a.p[e]]].newtonActive) return;
 	const uint32_t m = a.m;
 	uint32_t id[4];
 	V3 p[4], xr[4];
@@ -390,7 +434,8 @@ struct TriArgs {
 	M33* jstage;
 	V3* fstage;
 	uint32_t jx, jv, f, fd;
-	const SolverState* st;
+	const SolverState* st;   // per instance
+	const uint32_t* rowInst;
 };
 
 __device__ M22 extract_rotation2(const M22& M) {   // triangle_element_force.cpp:131-150
@@ -406,9 +451,9 @@ __device__ M22 extract_rotation2(const M22& M) {   // triangle_element_force.cpp
 
 // triangle_element_force.cpp:285-468
 __global__ void __launch_bounds__(kBlock) k_tri_forces(TriArgs a) {
-	if (!a.st->newtonActive) return;
 	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= a.m) return;
+	if (!a.st[a.rowInst[a.p[e]]].newtonActive) return;
 	const uint32_t m = a.m;
 	uint32_t id[3];
 	V3 p[3];
@@ -478,16 +523,17 @@ struct UVArgs {
 	M33* jstage;
 	V3* fstage;
 	uint32_t jx, jv, f;
-	const SolverState* st;
+	const SolverState* st;   // per instance
+	const uint32_t* rowInst;
 };
 
 __device__ __forceinline__ float coef(const UVArgs& a, int k, uint32_t e) { return a.c[(size_t)k * a.m + e]; }
 
 // stretch_force.cpp:121-389: parts u (0) and v (1)
 __global__ void __launch_bounds__(kBlock) k_stretch_forces(UVArgs a) {
-	if (!a.st->newtonActive) return;
 	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= a.m) return;
+	if (!a.st[a.rowInst[a.p[e]]].newtonActive) return;
 	const uint32_t m = a.m;
 	float alpha = coef(a, 0, e), du1 = coef(a, 1, e), dv1 = coef(a, 2, e), du2 = coef(a, 3, e), dv2 = coef(a, 4, e);
 	float inv_det = coef(a, 5, e);
@@ -540,9 +586,9 @@ __global__ void __launch_bounds__(kBlock) k_stretch_forces(UVArgs a) {
 
 // shear_force.cpp:106-245
 __global__ void __launch_bounds__(kBlock) k_shear_forces(UVArgs a) {
-	if (!a.st->newtonActive) return;
 	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= a.m) return;
+	if (!a.st[a.rowInst[a.p[e]]].newtonActive) return;
 	const uint32_t m = a.m;
 	float alpha = coef(a, 0, e), du1 = coef(a, 1, e), dv1 = coef(a, 2, e), du2 = coef(a, 3, e), dv2 = coef(a, 4, e);
 	float inv_det = coef(a, 5, e);
@@ -581,14 +627,15 @@ struct PairArgs {
 	M33* jstage;
 	V3* fstage;
 	uint32_t jx, jv, f;
-	const SolverState* st;
+	const SolverState* st;   // per instance
+	const uint32_t* rowInst;
 };
 
 // spring_force.cpp:61-143
 __global__ void __launch_bounds__(kBlock) k_spring_forces(PairArgs a) {
-	if (!a.st->newtonActive) return;
 	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= a.m) return;
+	if (!a.st[a.rowInst[a.p[e]]].newtonActive) return;
 	const uint32_t m = a.m;
 	uint32_t i1 = a.p[e], i2 = a.p[(size_t)m + e];
 	float L0 = a.c[e], ks = a.c[(size_t)m + e], kd = a.c[(size_t)2 * m + e];
@@ -637,9 +684,9 @@ __global__ void __launch_bounds__(kBlock) k_spring_forces(PairArgs a) {
 
 // mouse_force.cpp:58-246
 __global__ void __launch_bounds__(kBlock) k_mouse_forces(PairArgs a) {
-	if (!a.st->newtonActive) return;
 	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= a.m) return;
+	if (!a.st[a.rowInst[a.p[e]]].newtonActive) return;
 	const uint32_t m = a.m;
 	uint32_t id[4];
 	V3 xs[4], vs[4];
@@ -687,12 +734,15 @@ __global__ void __launch_bounds__(kBlock) k_mouse_forces(PairArgs a) {
 // Contacts
 // ---------------------------------------------------------------------------
 struct ContactArgs {
-	uint32_t nSpheres, nWorld;
+	uint32_t nSpheres, nPairs;
 	const WorldShape* worlds;
 	const float* vox;
 	const uint32_t* sphereParticle;
 	const float* sphereRadius;
 	const float* sphereFriction;
+	const uint32_t* pairStart;    // per sphere + 1
+	const uint32_t* pairSphere;   // per pair
+	const uint32_t* pairWorld;    // per pair: world fixture (list order)
 	const uint8_t* type;
 	const float* invMass;
 	V3* x;
@@ -705,17 +755,18 @@ struct ContactArgs {
 	uint32_t* hit;
 	V3* cF;
 	M33* cK;
-	uint32_t stampBase;
+	uint32_t stampCounter;
 	float dt;
-	SolverState* st;
+	SolverState* st;              // per instance
+	const uint32_t* rowInst;
 };
 
 // contact_manager.cpp:108-146
 __global__ void __launch_bounds__(kBlock) k_update_contacts(ContactArgs a) {
 	uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-	if (k >= a.nSpheres * a.nWorld) return;
+	if (k >= a.nPairs) return;
 	if (!a.active[k]) return;
-	uint32_t s = k / a.nWorld, w = k % a.nWorld;
+	uint32_t s = a.pairSphere[k], w = a.pairWorld[k];
 	uint32_t p = a.sphereParticle[s];
 	bool keep = a.type[p] == kDynamic && overlap(aabb_cr(a.x[p], a.sphereRadius[s]), a.worlds[w].aabb);
 	if (!keep) { a.active[k] = 0; return; }
@@ -723,17 +774,19 @@ __global__ void __launch_bounds__(kBlock) k_update_contacts(ContactArgs a) {
 	a.applyFriction[k] = 0;
 }
 
-// contact_manager.cpp:65-82 (pairs are keyed (sphere, world); stamps order the list)
+// contact_manager.cpp:65-82. Pairs are keyed (sphere, world); a stamp
+// (FindNewContacts call, world position) orders a sphere's contact list.
 __global__ void __launch_bounds__(kBlock) k_find_contacts(ContactArgs a) {
 	uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-	if (k >= a.nSpheres * a.nWorld) return;
+	if (k >= a.nPairs) return;
 	if (a.active[k]) return;
-	uint32_t s = k / a.nWorld, w = k % a.nWorld;
+	uint32_t s = a.pairSphere[k], w = a.pairWorld[k];
 	uint32_t p = a.sphereParticle[s];
 	if (!overlap(aabb_cr(a.x[p], a.sphereRadius[s]), a.worlds[w].aabb)) return;
 	if (a.type[p] != kDynamic) return;
+	const uint32_t p0 = a.pairStart[s], nw = a.pairStart[s + 1] - p0;
 	a.active[k] = 1;
-	a.stamp[k] = a.stampBase + w;
+	a.stamp[k] = a.stampCounter * nw + (k - p0);
 	a.normalForce[k] = 0.0f;
 	a.applyFriction[k] = 0;
 	a.manifold[2 * k] = zero3();
@@ -742,13 +795,13 @@ __global__ void __launch_bounds__(kBlock) k_find_contacts(ContactArgs a) {
 
 // sphere_shape_contact.cpp:53-143
 __global__ void __launch_bounds__(kBlock) k_contact_forces(ContactArgs a) {
-	if (!a.st->newtonActive) return;
 	uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-	if (k >= a.nSpheres * a.nWorld) return;
+	if (k >= a.nPairs) return;
+	uint32_t s = a.pairSphere[k], w = a.pairWorld[k];
+	uint32_t p = a.sphereParticle[s];
+	if (!a.st[a.rowInst[p]].newtonActive) return;
 	a.hit[k] = 0;
 	if (!a.active[k]) return;
-	uint32_t s = k / a.nWorld, w = k % a.nWorld;
-	uint32_t p = a.sphereParticle[s];
 	const WorldShape& ws = a.worlds[w];
 	V3 x1 = a.x[p];
 	float r1 = a.sphereRadius[s], r2 = ws.radius;
@@ -772,15 +825,17 @@ __global__ void __launch_bounds__(kBlock) k_contact_forces(ContactArgs a) {
 	a.normalForce[k] += length(f1);
 }
 
-// Contacts of one sphere in the reference's list order (newest first).
+// Contacts of one sphere in the reference's list order (newest first), as
+// positions within the sphere's pairs.
 __device__ int contact_order(const ContactArgs& a, uint32_t s, uint32_t* order) {
+	const uint32_t p0 = a.pairStart[s], p1 = a.pairStart[s + 1];
 	int cnt = 0;
-	for (uint32_t w = 0; w < a.nWorld; ++w)
-		if (a.active[s * a.nWorld + w]) order[cnt++] = w;
+	for (uint32_t k = p0; k < p1; ++k)
+		if (a.active[k]) order[cnt++] = k - p0;
 	for (int i = 1; i < cnt; ++i) {   // insertion sort by stamp, descending
 		uint32_t key = order[i];
 		int j = i - 1;
-		while (j >= 0 && a.stamp[s * a.nWorld + order[j]] < a.stamp[s * a.nWorld + key]) { order[j + 1] = order[j]; --j; }
+		while (j >= 0 && a.stamp[p0 + order[j]] < a.stamp[p0 + key]) { order[j + 1] = order[j]; --j; }
 		order[j + 1] = key;
 	}
 	return cnt;
@@ -797,13 +852,13 @@ __global__ void __launch_bounds__(kBlock) k_friction(ContactArgs a) {
 	V3 v1 = a.v[p];
 	float im1 = a.invMass[p];
 	for (int c = 0; c < cnt; ++c) {
-		uint32_t k = s * a.nWorld + order[c];
+		uint32_t k = a.pairStart[s] + order[c];
 		if (!a.applyFriction[k]) continue;
 		V3 normal = a.manifold[2 * k + 1];
 		float normalForce = a.normalForce[k];
 		V3 t1 = perp(normal);
 		V3 t2 = cross(t1, normal);
-		float friction = sqrt_(a.sphereFriction[s] * a.worlds[order[c]].friction);   // contact.h:28-31
+		float friction = sqrt_(a.sphereFriction[s] * a.worlds[a.pairWorld[k]].friction);   // contact.h:28-31
 		float tangentMass = im1 > 0.0f ? 1.0f / im1 : 0.0f;
 		V2 Cdot = v2(dot(v1, t1), dot(v1, t2));
 		V2 impulse = tangentMass * -Cdot;
@@ -831,10 +886,12 @@ __global__ void __launch_bounds__(kBlock) k_friction(ContactArgs a) {
 constexpr int kAsmCap = 1024;      // contributions per LDS chunk of the slot assembly (36 KB)
 
 struct AsmArgs {
-	uint32_t n, nnz, nWorld;
+	uint32_t n, nnz, hasContacts;
 	float h;
 	const uint32_t* slotTiles;   // slot boundaries of the slot-assembly tiles
+	const uint32_t* slotTileInst;
 	const uint32_t* rowTiles;    // row boundaries of the row-assembly tiles (the SpMV tiles)
+	const uint32_t* tileInst;
 	const uint32_t* slotRow;
 	const uint32_t* cols;
 	const uint32_t* jxPtr;
@@ -870,14 +927,15 @@ struct AsmArgs {
 	V3* invM;
 	V3* fi;
 	ContactArgs c;
-	const SolverState* st;
+	const SolverState* st;       // per instance
 };
 
 // acc += the contributions [ptr[s], ptr[s + 1]) of slot s, in order, for the
 // tile's slots [s0, s1). Each chunk of kAsmCap contributions is produced in
 // three batched phases per thread (kAsmCap / kBlock contributions each):
-// codes and K0 blocks, then the dependent gathers, then the products into
-// LDS, so a thread has all its loads in flight at once.
+// codes, then the dependent loads (a tetrahedron's K0 block and rotation, or
+// a staged block), then the products into LDS, so a thread has all its loads
+// of a phase in flight at once.
 constexpr int kAsmPer = kAsmCap / kBlock;
 
 struct Contrib {
@@ -900,13 +958,13 @@ __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib&
 		for (int u = 0; u < kAsmPer; ++u) {
 			const uint32_t k = cb + threadIdx.x + u * kBlock;
 			code[u] = k < ce ? cb_.idx[k] : 0u;
-			if (k < ce && cb_.K0) blk[u] = ldA(cb_.K0, cb_.count, k);
 		}
 #pragma unroll
 		for (int u = 0; u < kAsmPer; ++u) {
 			const uint32_t k = cb + threadIdx.x + u * kBlock;
 			if (k >= ce) continue;
 			if (code[u] & kTetCode) {
+				blk[u] = ldA(cb_.K0, cb_.count, k);
 				const float4 q4 = a.tetQ[code[u] & ~kTetCode];
 				q[u].v = v3(q4.x, q4.y, q4.z);
 				q[u].s = q4.w;
@@ -983,9 +1041,9 @@ __device__ __forceinline__ void tile_accumulate3(const uint32_t* ptr, const uint
 }
 
 __global__ void __launch_bounds__(kBlock) k_assemble_slots(AsmArgs a) {
-	if (!a.st->newtonActive) return;
-	__shared__ float lds[9 * kAsmCap];
 	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+	if (!a.st[a.slotTileInst[t]].newtonActive) return;
+	__shared__ float lds[9 * kAsmCap];
 	const uint32_t s0 = a.slotTiles[t], s1 = a.slotTiles[t + 1];
 	const uint32_t s = s0 + threadIdx.x;
 	const bool own = s < s1;
@@ -993,7 +1051,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble_slots(AsmArgs a) {
 	// per-slot data of the epilogue, loaded ahead of the contribution phase
 	const uint8_t type_i = own ? a.type[i] : 0, type_j = own ? a.type[j] : 0;
 	const float mass_i = own ? a.mass[i] : 0.0f, damp_i = own ? a.damping[i] : 0.0f;
-	const int32_t sph_i = own && a.nWorld ? a.sphereOf[i] : -1;
+	const int32_t sph_i = own && a.hasContacts ? a.sphereOf[i] : -1;
 	const V3 w_j = own ? a.w[j] : zero3();
 	// dfdx(i, j): contributions in list order, then contacts (force_solver.cpp:70-86)
 	M33 dx = mzero();
@@ -1004,12 +1062,12 @@ __global__ void __launch_bounds__(kBlock) k_assemble_slots(AsmArgs a) {
 	if (a.anyJv) tile_accumulate(a, Contrib{a.jvIdx, a.K0v, a.nJv, a.tetKd}, a.jvPtr, s0, s1, s, own, lds, dv);
 	if (!own) return;
 	const bool dyn_i = type_i == kDynamic;
-	if (i == j && a.nWorld && sph_i >= 0 && dyn_i) {
+	if (i == j && sph_i >= 0 && dyn_i) {
 		uint32_t order[16];
 		uint32_t sph = (uint32_t)sph_i;
 		int cnt = contact_order(a.c, sph, order);
 		for (int c = 0; c < cnt; ++c) {
-			uint32_t k = sph * a.nWorld + order[c];
+			uint32_t k = a.c.pairStart[sph] + order[c];
 			if (a.c.hit[k]) dx = dx + a.c.cK[k];
 		}
 	}
@@ -1029,9 +1087,9 @@ __global__ void __launch_bounds__(kBlock) k_assemble_slots(AsmArgs a) {
 }
 
 __global__ void __launch_bounds__(kBlock) k_assemble_rows(AsmArgs a) {
-	if (!a.st->newtonActive) return;
-	__shared__ float lds[3 * kTileCap];
 	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+	if (!a.st[a.tileInst[t]].newtonActive) return;
+	__shared__ float lds[3 * kTileCap];
 	const uint32_t r0 = a.rowTiles[t], r1 = a.rowTiles[t + 1];
 	const uint32_t i = r0 + threadIdx.x;
 	const bool own = i < r1;
@@ -1046,12 +1104,12 @@ __global__ void __launch_bounds__(kBlock) k_assemble_rows(AsmArgs a) {
 	tile_accumulate3(a.bPtr, a.bSlot, a.tB, r0, r1, i, own, lds, hdw);
 	if (!own) return;
 	const bool dyn = a.type[i] == kDynamic;
-	if (a.nWorld && a.sphereOf[i] >= 0 && dyn) {
+	if (a.hasContacts && a.sphereOf[i] >= 0 && dyn) {
 		uint32_t order[16];
 		uint32_t sph = (uint32_t)a.sphereOf[i];
 		int cnt = contact_order(a.c, sph, order);
 		for (int c = 0; c < cnt; ++c) {
-			uint32_t k = sph * a.nWorld + order[c];
+			uint32_t k = a.c.pairStart[sph] + order[c];
 			if (a.c.hit[k]) fi = fi + a.c.cF[k];
 		}
 	}
@@ -1076,6 +1134,9 @@ __global__ void __launch_bounds__(kBlock) k_assemble_rows(AsmArgs a) {
 struct CGArgs {
 	uint32_t n, nnz, nTiles;
 	const uint32_t* tiles;
+	const uint32_t* tileInst;
+	const uint32_t* rowBlk;
+	const uint32_t* blkInst;
 	const uint32_t* rowptr;
 	const uint32_t* cols;
 	const float* A;
@@ -1087,11 +1148,14 @@ struct CGArgs {
 	V3* q;
 	const V3* dold;
 	V3* dnew;
-	double* partial;
+	double* partial;            // residual / vector-update partials
+	double* partialQ;           // d.q partials of the SpMV kernel
 	float* terms;
-	SolverState* st;
-	const SolverState* newton;
-	__device__ Reduce red() const { return Reduce{partial, terms, n, st}; }
+	SolverState* st;            // per instance
+	const uint32_t* range;
+	CGLog* log;
+	uint32_t logStride;
+	uint32_t fused;             // FP64 mode: alpha / delta formed in the consuming kernel
 };
 
 __device__ __forceinline__ V3 diag_apply(V3 im, V3 r) { return mdiag3(im.x, im.y, im.z) * r; }
@@ -1125,12 +1189,15 @@ __device__ __forceinline__ V3 tile_rows(const CGArgs& a, uint32_t t, uint32_t& r
 	return acc;
 }
 
-// r = b - A x ; s = invM r ; delta = r.s   (mode kFinInit: CG start, kFinDelta: true-residual refresh)
-__global__ void __launch_bounds__(kBlock) k_cg_residual(CGArgs a, int mode) {
-	if (!a.newton->newtonActive) return;
-	if (mode == kFinDelta && !a.st->active) return;
-	__shared__ float lds[3 * kTileCap];
+// r = b - A x ; s = invM r ; delta = r.s   (mode kFinInit: CG start, kFinDelta: true-residual
+// refresh after iteration k, which ran only if log[k].active)
+__global__ void __launch_bounds__(kBlock) k_cg_residual(CGArgs a, int mode, uint32_t k) {
 	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+	const uint32_t inst = a.tileInst[t];
+	const SolverState& st = a.st[inst];
+	if (!st.newtonActive) return;
+	if (mode == kFinDelta && !a.log[(size_t)inst * a.logStride + k].active) return;
+	__shared__ float lds[3 * kTileCap];
 	uint32_t row;
 	V3 ax = tile_rows(a, t, row, [&](uint32_t c) { return a.px[c]; }, lds);
 	float part = 0.0f;
@@ -1142,17 +1209,46 @@ __global__ void __launch_bounds__(kBlock) k_cg_residual(CGArgs a, int mode) {
 		a.s[row] = s;
 		part = dot(r, s);
 	}
-	block_partial(part, has, row, a.red());
+	block_partial(part, has, row, a.partial, a.terms, st.seq != 0, t);
 }
 
-// q = A d with d = s + beta d_old formed on the fly (d = s at the first
-// iteration), d written for the tile's rows; alpha = delta / d.q.
-// force = 1 runs it whatever the PCG state (timing pass on CG scratch).
-__global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, int first, int force) {
-	if (!force && !a.st->active) return;
-	__shared__ float lds[3 * kTileCap];
+// PCG iteration k: q = A d with d = s + beta_k d_old formed on the fly
+// (d = s at k = 0), d written for the tile's rows; partials of d.q.
+// Fused mode: the workgroup first forms delta_k / beta_k (Delta(k)) from the
+// partials of the reduction that ended iteration k - 1 (row blocks, or tiles
+// after a residual refresh: sel); the instance's first tile records them.
+// force = 1 runs it whatever the PCG state (timing pass on PCG scratch).
+__global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, uint32_t k, int sel, int force) {
 	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
-	const float beta = a.st->beta;
+	const uint32_t inst = a.tileInst[t];
+	SolverState& st = a.st[inst];
+	CGLog* log = a.log + (size_t)inst * a.logStride;
+	__shared__ float lds[3 * kTileCap];
+	if (!force && !st.newtonActive) return;
+	float beta = 0.0f;
+	if (k > 0) {
+		if (!force && !log[k - 1].active) return;
+		if (a.fused) {
+			const uint32_t* R = a.range + 6 * (size_t)inst;
+			const float v = (float)wg_sum(a.partial, R[sel], R[sel + 1]);
+			const float prev = log[k - 1].delta;
+			beta = v / prev;
+			const bool act = k != st.maxIt && !(v <= st.eps2_sub * st.delta0);
+			if (!force && t == R[0] && threadIdx.x == 0) {
+				log[k].delta = v;
+				log[k].beta = beta;
+				log[k].active = act ? 1u : 0u;
+				st.it = k;
+			}
+			if (!force && !act) return;
+		} else {
+			if (!force && !log[k].active) return;
+			beta = log[k].beta;
+		}
+	} else if (!force && !log[0].active) {
+		return;
+	}
+	const bool first = k == 0;
 	auto dval = [&](uint32_t c) { return first ? a.s[c] : a.s[c] + beta * a.dold[c]; };
 	uint32_t row;
 	V3 q = tile_rows(a, t, row, dval, lds);
@@ -1164,16 +1260,28 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, int first, int 
 		a.q[row] = q;
 		part = dot(d, q);
 	}
-	block_partial(part, has, row, a.red());
+	block_partial(part, has, row, a.partialQ, a.terms, st.seq != 0, t);
 }
 
-// x += alpha d ; r -= alpha q ; s = invM r ; delta = r.s
-__global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a) {
-	if (!a.st->active) return;
-	uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-	const float alpha = a.st->alpha;
+// alpha_k of a vector kernel of iteration k: fused mode sums the SpMV's d.q
+// partials of the instance (Alpha(k)); the instance's first block records it.
+__device__ __forceinline__ float cg_alpha(const CGArgs& a, uint32_t inst, uint32_t blk, uint32_t k) {
+	CGLog* log = a.log + (size_t)inst * a.logStride;
+	if (!a.fused) return log[k].alpha;
+	const uint32_t* R = a.range + 6 * (size_t)inst;
+	const float alpha = log[k].delta / (float)wg_sum(a.partialQ, R[0], R[1]);
+	if (blk == R[2] && threadIdx.x == 0) log[k].alpha = alpha;
+	return alpha;
+}
+
+// x += alpha d ; r -= alpha q ; s = invM r ; partials of r.s (iteration k)
+__global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k) {
+	const uint32_t blk = blockIdx.x, inst = a.blkInst[blk];
+	if (!a.st[inst].newtonActive || !a.log[(size_t)inst * a.logStride + k].active) return;
+	const float alpha = cg_alpha(a, inst, blk, k);
+	const uint32_t i = a.rowBlk[blk] + threadIdx.x;
 	float part = 0.0f;
-	const bool has = i < a.n;
+	const bool has = i < a.rowBlk[blk + 1];
 	if (has) {
 		V3 d = a.dnew[i];
 		a.px[i] = a.px[i] + alpha * d;
@@ -1183,23 +1291,26 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a) {
 		a.s[i] = s;
 		part = dot(r, s);
 	}
-	block_partial(part, has, i, a.red());
+	block_partial(part, has, i, a.partial, a.terms, a.st[inst].seq != 0, blk);
 }
 
 // x += alpha d (iterations that refresh the true residual)
-__global__ void __launch_bounds__(kBlock) k_cg_update_x(CGArgs a) {
-	if (!a.st->active) return;
-	uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-	if (i >= a.n) return;
-	a.px[i] = a.px[i] + a.st->alpha * a.dnew[i];
+__global__ void __launch_bounds__(kBlock) k_cg_update_x(CGArgs a, uint32_t k) {
+	const uint32_t blk = blockIdx.x, inst = a.blkInst[blk];
+	if (!a.st[inst].newtonActive || !a.log[(size_t)inst * a.logStride + k].active) return;
+	const float alpha = cg_alpha(a, inst, blk, k);
+	const uint32_t i = a.rowBlk[blk] + threadIdx.x;
+	if (i >= a.rowBlk[blk + 1]) return;
+	a.px[i] = a.px[i] + alpha * a.dnew[i];
 }
 
 // v += dv ; x = x0 + h v + y ; error = |dv|^2 (sparse_force_solver.cpp:193-223)
 __global__ void __launch_bounds__(kBlock) k_newton_update(StepArgs s, double* partial, float* terms) {
-	if (!s.st->newtonActive) return;
-	uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	const uint32_t blk = blockIdx.x, inst = s.blkInst[blk];
+	if (!s.st[inst].newtonActive) return;
+	const uint32_t i = s.rowBlk[blk] + threadIdx.x;
 	float part = 0.0f;
-	const bool has = i < s.n;
+	const bool has = i < s.rowBlk[blk + 1];
 	if (has) {
 		V3 dv = s.px[i] + zero3();
 		V3 v = s.v[i] + dv;
@@ -1207,7 +1318,7 @@ __global__ void __launch_bounds__(kBlock) k_newton_update(StepArgs s, double* pa
 		s.x[i] = (s.x0[i] + s.h * v) + s.y[i];
 		part = length_sq(dv);
 	}
-	block_partial(part, has, i, Reduce{partial, terms, s.n, s.st});
+	block_partial(part, has, i, partial, terms, s.st[inst].seq != 0, blk);
 }
 
 __global__ void __launch_bounds__(kBlock) k_clear_external(uint32_t n, V3* force, V3* translation) {
@@ -1237,25 +1348,62 @@ static inline uint32_t nblk(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 		B3G_HIP(hipGetLastError());                                                                  \
 	} while (0)
 
+// Row decompositions, both cut at instance boundaries so that every tile /
+// block belongs to one instance: SpMV tiles (<= kTileRows rows, <= kTileCap
+// blocks) and row blocks (<= kBlock rows) for the vector kernels; plus each
+// instance's ranges of tiles, blocks and rows.
 b3g_status build_tiles(Body& b) {
-	std::vector<uint32_t> t;
-	t.push_back(0);
-	uint32_t rows = 0, blocks = 0;
+	const uint32_t K = std::max(1u, b.nInst);
+	auto inst = [&](uint32_t i) { return b.hostRowInst.empty() ? 0u : b.hostRowInst[i]; };
+	std::vector<uint32_t> t{0}, ti, rb{0}, bi;
+	uint32_t rows = 0, blocks = 0, brows = 0;
 	for (uint32_t i = 0; i < b.n; ++i) {
-		uint32_t len = b.hostRowptr[i + 1] - b.hostRowptr[i];
-		if (rows > 0 && (rows == (uint32_t)kTileRows || blocks + len > (uint32_t)kTileCap)) {
+		const uint32_t len = b.hostRowptr[i + 1] - b.hostRowptr[i];
+		const bool newInst = i > 0 && inst(i) != inst(i - 1);
+		if (rows > 0 && (newInst || rows == (uint32_t)kTileRows || blocks + len > (uint32_t)kTileCap)) {
 			t.push_back(i);
+			ti.push_back(inst(i - 1));
 			rows = 0;
 			blocks = 0;
 		}
+		if (brows > 0 && (newInst || brows == (uint32_t)kBlock)) {
+			rb.push_back(i);
+			bi.push_back(inst(i - 1));
+			brows = 0;
+		}
 		rows += 1;
 		blocks += len;
+		brows += 1;
 	}
 	t.push_back(b.n);
+	ti.push_back(b.n ? inst(b.n - 1) : 0u);
+	rb.push_back(b.n);
+	bi.push_back(b.n ? inst(b.n - 1) : 0u);
 	b.nTiles = (uint32_t)t.size() - 1;
-	b3g_status s = dupload(b, b.tiles, t.data(), t.size() * 4);
-	if (s != B3G_OK) return s;
-	return ensure_partials(b, std::max(b.nTiles, nblk(b.n)));
+	b.nRowBlk = (uint32_t)rb.size() - 1;
+	// per instance: [tile begin, end), [row block begin, end), [row begin, end)
+	std::vector<uint32_t> range(6 * (size_t)K, 0);
+	for (uint32_t k = 0; k < K; ++k) {
+		range[6 * k] = range[6 * k + 2] = range[6 * k + 4] = 0xFFFFFFFFu;
+	}
+	auto extend = [&](uint32_t k, int slot, uint32_t lo, uint32_t hi) {
+		uint32_t& a = range[6 * k + slot];
+		uint32_t& z = range[6 * k + slot + 1];
+		if (a == 0xFFFFFFFFu) { a = lo; z = hi; } else { a = std::min(a, lo); z = std::max(z, hi); }
+	};
+	for (uint32_t x = 0; x < b.nTiles; ++x) extend(ti[x], 0, x, x + 1);
+	for (uint32_t x = 0; x < b.nRowBlk; ++x) extend(bi[x], 2, x, x + 1);
+	for (uint32_t i = 0; i < b.n; ++i) extend(inst(i), 4, i, i + 1);
+	for (uint32_t k = 0; k < K; ++k)
+		for (int slot = 0; slot < 6; slot += 2)
+			if (range[6 * k + slot] == 0xFFFFFFFFu) range[6 * k + slot] = range[6 * k + slot + 1] = 0;
+	b3g_status s;
+	if ((s = dupload(b, b.tiles, t.data(), t.size() * 4)) != B3G_OK) return s;
+	if ((s = dupload(b, b.tileInst, ti.data(), ti.size() * 4)) != B3G_OK) return s;
+	if ((s = dupload(b, b.rowBlk, rb.data(), rb.size() * 4)) != B3G_OK) return s;
+	if ((s = dupload(b, b.blkInst, bi.data(), bi.size() * 4)) != B3G_OK) return s;
+	if ((s = dupload(b, b.instRange, range.data(), range.size() * 4)) != B3G_OK) return s;
+	return ensure_partials(b, std::max(b.nTiles, b.nRowBlk));
 }
 
 b3g_status ensure_partials(Body& b, uint32_t blocks) {
@@ -1263,10 +1411,24 @@ b3g_status ensure_partials(Body& b, uint32_t blocks) {
 		b3g_status s = dalloc(b, b.terms, (size_t)std::max(1u, b.n) * sizeof(float));
 		if (s != B3G_OK) return s;
 	}
-	if (b.partialCount >= blocks && b.partial.p) return B3G_OK;
+	if (b.partialCount >= blocks && b.partial.p && b.partialQ.p) return B3G_OK;
 	b3g_status s = dalloc(b, b.partial, (size_t)blocks * sizeof(double));
 	if (s != B3G_OK) return s;
+	s = dalloc(b, b.partialQ, (size_t)blocks * sizeof(double));
+	if (s != B3G_OK) return s;
 	b.partialCount = blocks;
+	return B3G_OK;
+}
+
+// Room for `entries` PCG log entries per instance (a solve of maxIt
+// iterations uses maxIt + 1).
+b3g_status ensure_cg_log(Body& b, uint32_t entries) {
+	if (b.cgLog.p && b.logStride >= entries) return B3G_OK;
+	const uint32_t stride = std::max(entries, 64u);
+	B3G_HIP(hipStreamSynchronize(b.stream));   // kernels in flight may still read the old log
+	b3g_status s = dalloc(b, b.cgLog, (size_t)std::max(1u, b.nInst) * stride * sizeof(CGLog));
+	if (s != B3G_OK) return s;
+	b.logStride = stride;
 	return B3G_OK;
 }
 
@@ -1317,11 +1479,14 @@ b3g_status launch_tet_setup(Body& b) {
 	return B3G_OK;
 }
 
-static StepArgs step_args(Body& b, float h, const float g[3]) {
+static StepArgs step_args(Body& b, float h) {
 	StepArgs a;
 	a.n = b.n;
 	a.h = h;
-	a.gravity = g ? v3(g[0], g[1], g[2]) : zero3();
+	a.gravity = dptr<V3>(b.gravity);
+	a.rowInst = dptr<uint32_t>(b.rowInst);
+	a.rowBlk = dptr<uint32_t>(b.rowBlk);
+	a.blkInst = dptr<uint32_t>(b.blkInst);
 	a.type = dptr<uint8_t>(b.type);
 	a.mass = dptr<float>(b.mass);
 	a.damping = dptr<float>(b.damping);
@@ -1329,6 +1494,7 @@ static StepArgs step_args(Body& b, float h, const float g[3]) {
 	a.x0 = dptr<V3>(b.x0); a.v0 = dptr<V3>(b.v0); a.fe = dptr<V3>(b.fe); a.y = dptr<V3>(b.y); a.w = dptr<V3>(b.w);
 	a.px = dptr<V3>(b.px);
 	a.st = dptr<SolverState>(b.state);
+	a.range = dptr<uint32_t>(b.instRange);
 	return a;
 }
 
@@ -1336,12 +1502,15 @@ static ContactArgs contact_args(Body& b, float dt) {
 	ContactArgs c;
 	std::memset(&c, 0, sizeof(c));
 	c.nSpheres = b.nSpheres;
-	c.nWorld = b.nWorld;
+	c.nPairs = b.nPairs;
 	c.worlds = dptr<WorldShape>(b.worlds);
 	c.vox = dptr<float>(b.voxels);
 	c.sphereParticle = dptr<uint32_t>(b.sphereParticle);
 	c.sphereRadius = dptr<float>(b.sphereRadius);
 	c.sphereFriction = dptr<float>(b.sphereFriction);
+	c.pairStart = dptr<uint32_t>(b.pairStart);
+	c.pairSphere = dptr<uint32_t>(b.pairSphere);
+	c.pairWorld = dptr<uint32_t>(b.pairWorld);
 	c.type = dptr<uint8_t>(b.type);
 	c.invMass = dptr<float>(b.invMass);
 	c.x = dptr<V3>(b.x);
@@ -1354,9 +1523,10 @@ static ContactArgs contact_args(Body& b, float dt) {
 	c.hit = dptr<uint32_t>(b.cHit);
 	c.cF = dptr<V3>(b.cF);
 	c.cK = dptr<M33>(b.cK);
-	c.stampBase = b.stampCounter * std::max(1u, b.nWorld);
+	c.stampCounter = b.stampCounter;
 	c.dt = dt;
 	c.st = dptr<SolverState>(b.state);
+	c.rowInst = dptr<uint32_t>(b.rowInst);
 	return c;
 }
 
@@ -1366,6 +1536,9 @@ static CGArgs cg_args(Body& b) {
 	a.nnz = b.nnz;
 	a.nTiles = b.nTiles;
 	a.tiles = dptr<uint32_t>(b.tiles);
+	a.tileInst = dptr<uint32_t>(b.tileInst);
+	a.rowBlk = dptr<uint32_t>(b.rowBlk);
+	a.blkInst = dptr<uint32_t>(b.blkInst);
 	a.rowptr = dptr<uint32_t>(b.rowptr);
 	a.cols = dptr<uint32_t>(b.cols);
 	a.A = dptr<float>(b.A);
@@ -1378,23 +1551,27 @@ static CGArgs cg_args(Body& b) {
 	a.dold = dptr<V3>(b.d0);
 	a.dnew = dptr<V3>(b.d1);
 	a.partial = dptr<double>(b.partial);
+	a.partialQ = dptr<double>(b.partialQ);
 	a.terms = dptr<float>(b.terms);
 	a.st = dptr<SolverState>(b.state);
-	a.newton = dptr<SolverState>(b.state);
+	a.range = dptr<uint32_t>(b.instRange);
+	a.log = dptr<CGLog>(b.cgLog);
+	a.logStride = b.logStride;
+	a.fused = b.reduction == B3G_REDUCE_FP64_FUSED ? 1u : 0u;
 	return a;
 }
 
 b3g_status enqueue_update_contacts(Body& b) {
-	if (!b.nWorld || !b.nSpheres) return B3G_OK;
+	if (!b.nPairs) return B3G_OK;
 	ContactArgs c = contact_args(b, 0.0f);
-	LAUNCH(k_update_contacts, nblk(b.nSpheres * b.nWorld), c);
+	LAUNCH(k_update_contacts, nblk(b.nPairs), c);
 	return B3G_OK;
 }
 
 b3g_status enqueue_find_contacts(Body& b) {
-	if (!b.nWorld || !b.nSpheres) return B3G_OK;
+	if (!b.nPairs) return B3G_OK;
 	ContactArgs c = contact_args(b, 0.0f);
-	LAUNCH(k_find_contacts, nblk(b.nSpheres * b.nWorld), c);
+	LAUNCH(k_find_contacts, nblk(b.nPairs), c);
 	b.stampCounter += 1;
 	return B3G_OK;
 }
@@ -1402,45 +1579,47 @@ b3g_status enqueue_find_contacts(Body& b) {
 // One linearisation: element kernels, contacts, pA / pb assembly.
 static b3g_status enqueue_linearise(Body& b, float h) {
 	const SolverState* st = dptr<SolverState>(b.state);
+	const uint32_t* ri = dptr<uint32_t>(b.rowInst);
 	M33* js = dptr<M33>(b.jstage);
 	V3* fs = dptr<V3>(b.fstage);
-	StepArgs sa = step_args(b, h, nullptr);
+	StepArgs sa = step_args(b, h);
 	LAUNCH(k_newton_prep, nblk(b.n), sa);
 	if (b.tet.m) {
 		TetArgs t{b.tet.m, dptr<uint32_t>(b.tet.p), dptr<float>(b.tet.rest), dptr<float>(b.tet.invE), dptr<float>(b.tet.K0),
-		          dptr<float4>(b.tet.q), dptr<float>(b.tet.kd), dptr<V3>(b.x), dptr<V3>(b.v), fs, b.tet.f, b.tet.fd, st};
+		          dptr<float4>(b.tet.q), dptr<float>(b.tet.kd), dptr<V3>(b.x), dptr<V3>(b.v), fs, b.tet.f, b.tet.fd, st, ri};
 		LAUNCH(k_tet_forces, nblk(b.tet.m), t);
 	}
 	if (b.tri.m) {
 		TriArgs t{b.tri.m, dptr<uint32_t>(b.tri.p), dptr<float>(b.tri.x2), dptr<float>(b.tri.invS), dptr<float>(b.tri.K0),
-		          dptr<float>(b.tri.kd), dptr<V3>(b.x), dptr<V3>(b.v), js, fs, b.tri.jx, b.tri.jv, b.tri.f, b.tri.fd, st};
+		          dptr<float>(b.tri.kd), dptr<V3>(b.x), dptr<V3>(b.v), js, fs, b.tri.jx, b.tri.jv, b.tri.f, b.tri.fd, st, ri};
 		LAUNCH(k_tri_forces, nblk(b.tri.m), t);
 	}
 	if (b.stretch.m) {
 		UVArgs u{b.stretch.m, dptr<uint32_t>(b.stretch.p), dptr<float>(b.stretch.c), dptr<V3>(b.x), dptr<V3>(b.v), js, fs,
-		         b.stretch.jx, b.stretch.jv, b.stretch.f, st};
+		         b.stretch.jx, b.stretch.jv, b.stretch.f, st, ri};
 		LAUNCH(k_stretch_forces, nblk(b.stretch.m), u);
 	}
 	if (b.shear.m) {
 		UVArgs u{b.shear.m, dptr<uint32_t>(b.shear.p), dptr<float>(b.shear.c), dptr<V3>(b.x), dptr<V3>(b.v), js, fs,
-		         b.shear.jx, b.shear.jv, b.shear.f, st};
+		         b.shear.jx, b.shear.jv, b.shear.f, st, ri};
 		LAUNCH(k_shear_forces, nblk(b.shear.m), u);
 	}
 	if (b.spring.m) {
 		PairArgs p{b.spring.m, dptr<uint32_t>(b.spring.p), dptr<float>(b.spring.c), dptr<V3>(b.x), dptr<V3>(b.v), js, fs,
-		           b.spring.jx, b.spring.jv, b.spring.f, st};
+		           b.spring.jx, b.spring.jv, b.spring.f, st, ri};
 		LAUNCH(k_spring_forces, nblk(b.spring.m), p);
 	}
 	if (b.mouse.m) {
 		PairArgs p{b.mouse.m, dptr<uint32_t>(b.mouse.p), dptr<float>(b.mouse.c), dptr<V3>(b.x), dptr<V3>(b.v), js, fs,
-		           b.mouse.jx, b.mouse.jv, b.mouse.f, st};
+		           b.mouse.jx, b.mouse.jv, b.mouse.f, st, ri};
 		LAUNCH(k_mouse_forces, nblk(b.mouse.m), p);
 	}
 	ContactArgs c = contact_args(b, h);
-	if (b.nWorld && b.nSpheres) LAUNCH(k_contact_forces, nblk(b.nSpheres * b.nWorld), c);
+	if (b.nPairs) LAUNCH(k_contact_forces, nblk(b.nPairs), c);
 	AsmArgs a;
-	a.n = b.n; a.nnz = b.nnz; a.nWorld = (b.nWorld && b.nSpheres) ? b.nWorld : 0; a.h = h;
-	a.slotTiles = dptr<uint32_t>(b.slotTiles); a.rowTiles = dptr<uint32_t>(b.tiles);
+	a.n = b.n; a.nnz = b.nnz; a.hasContacts = b.nPairs ? 1u : 0u; a.h = h;
+	a.slotTiles = dptr<uint32_t>(b.slotTiles); a.slotTileInst = dptr<uint32_t>(b.slotTileInst);
+	a.rowTiles = dptr<uint32_t>(b.tiles); a.tileInst = dptr<uint32_t>(b.tileInst);
 	a.K0x = dptr<float>(b.K0x); a.K0v = dptr<float>(b.K0v); a.nJx = b.nJx; a.nJv = b.nJv;
 	a.tetM = b.tet.m; a.tetQ = dptr<float4>(b.tet.q); a.tetKd = dptr<float>(b.tet.kd); a.anyJv = b.anyJv ? 1u : 0u;
 	a.slotRow = dptr<uint32_t>(b.slotRow); a.cols = dptr<uint32_t>(b.cols);
@@ -1461,43 +1640,56 @@ static b3g_status enqueue_linearise(Body& b, float h) {
 	return B3G_OK;
 }
 
-static b3g_status enqueue_finalize(Body& b, uint32_t count, int kind, int gate) {
-	Reduce rd{dptr<double>(b.partial), dptr<float>(b.terms), b.n, dptr<SolverState>(b.state)};
-	hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinBlock), 0, b.stream, rd, count, kind, gate);
+static b3g_status enqueue_finalize(Body& b, const DBuf& partial, int sel, int kind, uint32_t k, int gate, uint32_t gk) {
+	Reduce rd{dptr<double>(partial), dptr<float>(b.terms), dptr<SolverState>(b.state), dptr<uint32_t>(b.instRange),
+	          dptr<CGLog>(b.cgLog), b.logStride};
+	hipLaunchKernelGGL(k_finalize, dim3(std::max(1u, b.nInst)), dim3(kFinBlock), 0, b.stream, rd, sel, kind, k, gate, gk);
 	B3G_HIP(hipGetLastError());
 	return B3G_OK;
 }
 
-#define FINALIZE(count, kind, gate)                                      \
-	do {                                                                 \
-		b3g_status fs_ = enqueue_finalize(b, (count), (kind), (gate));  \
-		if (fs_ != B3G_OK) return fs_;                                   \
+#define FINALIZE(partial, sel, kind, k, gate, gk)                                          \
+	do {                                                                                   \
+		b3g_status fs_ = enqueue_finalize(b, (partial), (sel), (kind), (k), (gate), (gk)); \
+		if (fs_ != B3G_OK) return fs_;                                                     \
 	} while (0)
 
-// Jacobi PCG on the assembled pA / pb with the warm start px.
+// Partials that end PCG iteration k: the true-residual refresh (tiles) on
+// every 50th iteration (sparse_solver.cpp:81-90), the vector update (row
+// blocks) otherwise.
+static inline int delta_range(uint32_t k) { return k % 50 == 0 ? kRangeTiles : kRangeBlocks; }
+
+// Jacobi PCG on the assembled pA / pb with the warm start px: the reference's
+// fixed launch sequence of maxIt iterations; iterations after convergence
+// return at once on their log entry.
 b3g_status enqueue_solve_cg(Body& b, uint32_t maxIt, float eps) {
 	(void)eps;
+	b3g_status s = ensure_cg_log(b, maxIt + 1);
+	if (s != B3G_OK) return s;
 	CGArgs a = cg_args(b);
-	const uint32_t rb = nblk(b.n);
-	LAUNCH(k_cg_residual, b.nTiles, a, (int)kFinInit);
-	FINALIZE(b.nTiles, kFinInit, kGateNewton);
+	const bool fused = a.fused != 0;
+	const uint32_t rb = b.nRowBlk;
+	LAUNCH(k_cg_residual, b.nTiles, a, (int)kFinInit, 0u);
+	FINALIZE(b.partial, kRangeTiles, kFinInit, 0, kGateNewton, 0);
 	for (uint32_t k = 0; k < maxIt; ++k) {
 		CGArgs ak = a;
 		V3* d0 = dptr<V3>(b.d0);
 		V3* d1 = dptr<V3>(b.d1);
 		ak.dold = (k & 1) ? d1 : d0;
 		ak.dnew = (k & 1) ? d0 : d1;
-		LAUNCH(k_cg_spmv_dq, b.nTiles, ak, k == 0 ? 1 : 0, 0);
-		FINALIZE(b.nTiles, kFinAlpha, kGateCG);
+		const int sel = k > 0 ? delta_range(k - 1) : kRangeTiles;
+		if (!fused && k > 0) FINALIZE(b.partial, sel, kFinDelta, k, kGateIter, k - 1);
+		LAUNCH(k_cg_spmv_dq, b.nTiles, ak, k, sel, 0);
+		if (!fused) FINALIZE(b.partialQ, kRangeTiles, kFinAlpha, k, kGateIter, k);
 		if (k % 50 == 0) {
-			LAUNCH(k_cg_update_x, rb, ak);
-			LAUNCH(k_cg_residual, b.nTiles, ak, (int)kFinDelta);
-			FINALIZE(b.nTiles, kFinDelta, kGateCG);
+			LAUNCH(k_cg_update_x, rb, ak, k);
+			LAUNCH(k_cg_residual, b.nTiles, ak, (int)kFinDelta, k);
 		} else {
-			LAUNCH(k_cg_update, rb, ak);
-			FINALIZE(rb, kFinDelta, kGateCG);
+			LAUNCH(k_cg_update, rb, ak, k);
 		}
 	}
+	// the reduction after the last iteration: delta, iteration count
+	if (maxIt > 0) FINALIZE(b.partial, delta_range(maxIt - 1), kFinDelta, maxIt, kGateIter, maxIt - 1);
 	return B3G_OK;
 }
 
@@ -1522,7 +1714,7 @@ b3g_status time_pcg_spmv(Body& b, uint32_t launches, double* mean_ms) {
 			CGArgs ak = a;
 			ak.dold = (k & 1) ? dptr<V3>(b.d1) : dptr<V3>(b.d0);
 			ak.dnew = (k & 1) ? dptr<V3>(b.d0) : dptr<V3>(b.d1);
-			hipLaunchKernelGGL(k_cg_spmv_dq, dim3(b.nTiles), dim3(kBlock), 0, b.stream, ak, 0, 1);
+			hipLaunchKernelGGL(k_cg_spmv_dq, dim3(b.nTiles), dim3(kBlock), 0, b.stream, ak, 1u, (int)kRangeBlocks, 1);
 			if (hipGetLastError() != hipSuccess) st = B3G_HIP_ERROR;
 		}
 		if (st != B3G_OK) break;
@@ -1542,26 +1734,24 @@ b3g_status time_pcg_spmv(Body& b, uint32_t launches, double* mean_ms) {
 
 b3g_status enqueue_step(Body& b, float dt, uint32_t iters, uint32_t subIters, float tol, float subTol,
                         bool assembleOnly) {
-	const float* g = b.gravity;
 	SolverState* st = dptr<SolverState>(b.state);
 	b3g_status s = enqueue_update_contacts(b);
 	if (s != B3G_OK) return s;
 	if (dt > 0.0f) {
-		StepArgs sa = step_args(b, dt, g);
+		StepArgs sa = step_args(b, dt);
 		LAUNCH(k_step_begin, nblk(b.n), sa);
-		hipLaunchKernelGGL(k_state_begin, dim3(1), dim3(1), 0, b.stream, st, assembleOnly ? 1u : iters, subIters, tol, subTol,
-		                   (uint32_t)b.reduction);
-		B3G_HIP(hipGetLastError());
+		LAUNCH(k_state_begin, nblk(b.nInst), st, b.nInst, assembleOnly ? 1u : iters, subIters, tol, subTol,
+		       b.reduction == B3G_REDUCE_SEQUENTIAL ? 1u : 0u);
 		for (uint32_t it = 0; it < (assembleOnly ? 1u : iters); ++it) {
 			s = enqueue_linearise(b, dt);
 			if (s != B3G_OK) return s;
 			if (assembleOnly) return B3G_OK;
 			s = enqueue_solve_cg(b, subIters, subTol);
 			if (s != B3G_OK) return s;
-			LAUNCH(k_newton_update, nblk(b.n), sa, dptr<double>(b.partial), dptr<float>(b.terms));
-			FINALIZE(nblk(b.n), kFinNewton, kGateNewton);
+			LAUNCH(k_newton_update, b.nRowBlk, sa, dptr<double>(b.partial), dptr<float>(b.terms));
+			FINALIZE(b.partial, kRangeBlocks, kFinNewton, 0, kGateNewton, 0);
 		}
-		if (b.nWorld && b.nSpheres) {
+		if (b.nPairs) {
 			ContactArgs c = contact_args(b, dt);
 			LAUNCH(k_friction, nblk(b.nSpheres), c);
 		}

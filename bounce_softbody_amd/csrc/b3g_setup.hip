@@ -172,7 +172,7 @@ struct SymbolicInput {
 	const std::vector<uint8_t>* type;
 	const std::vector<float>* damping;
 	const std::vector<int32_t>* sphereOf;
-	bool contacts;
+	const std::vector<char>* contactRow;   // per row: the row's particle can touch a world fixture (or null)
 	// forces in list order: (type, creation index within the type)
 	std::vector<std::pair<uint32_t, uint32_t>> forces;
 	// particle list indices of each element, by type (creation order)
@@ -295,9 +295,9 @@ void build_pattern(const SymbolicInput& in, Pattern& P) {
 		}
 	}
 	// contacts (sphere_shape_contact.cpp:121-122): dfdx(i,i) of particles that can touch a world fixture
-	if (in.contacts)
+	if (in.contactRow)
 		for (uint32_t i = 0; i < n; ++i)
-			if ((*in.sphereOf)[i] >= 0 && (*in.type)[i] == kDynamic) D.touch(i, i, kNoStage);
+			if ((*in.contactRow)[i] && (*in.sphereOf)[i] >= 0 && (*in.type)[i] == kDynamic) D.touch(i, i, kNoStage);
 
 	// Layout of pA rows
 	P.rowptr.assign(n + 1, 0);
@@ -472,18 +472,155 @@ b3g_status validate(const b3g_body_desc& d) {
 	return B3G_OK;
 }
 
+// Several descriptions as one, in creation order: instance 0's objects first.
+// Particle references are offset by the instance's first particle, force
+// indices by the instance's first element of the type. List order reverses
+// creation order, so each instance stays contiguous and keeps its own list
+// orders.
+struct Merged {
+	b3g_body_desc d;
+	std::vector<uint8_t> p_type;
+	std::vector<float> p_pos, p_vel, p_damping;
+	std::vector<uint32_t> s_particle;
+	std::vector<float> s_radius, s_friction;
+	std::vector<uint32_t> tf_p;
+	std::vector<float> tf_v, tf_density, tf_radius, tf_friction;
+	std::vector<uint32_t> ttf_p;
+	std::vector<float> ttf_v, ttf_density, ttf_radius, ttf_friction;
+	std::vector<uint32_t> f_type, f_index;
+	std::vector<uint32_t> tet_p, tri_p, st_p, sh_p, sp_p, mo_p;
+	std::vector<float> tet_v, tet_param, tri_v, tri_param, st_param, sh_param, sp_param, mo_param;
+	std::vector<uint32_t> w_type;
+	std::vector<float> w_param, w_friction;
+	std::vector<const float*> w_voxels;
+	std::vector<uint32_t> partBegin, worldBegin;   // count + 1, creation order
+	std::vector<V3> gravity;
+};
+
+template <class T>
+void app(std::vector<T>& v, const T* p, size_t count) {
+	if (count) v.insert(v.end(), p, p + count);
+}
+void app_p(std::vector<uint32_t>& v, const uint32_t* p, size_t count, uint32_t off) {
+	for (size_t i = 0; i < count; ++i) v.push_back(p[i] + off);
+}
+
+void merge(const b3g_body_desc* ds, uint32_t count, Merged& M) {
+	uint32_t P = 0, W = 0;
+	uint32_t typeOff[7] = {0, 0, 0, 0, 0, 0, 0};
+	for (uint32_t k = 0; k < count; ++k) {
+		const b3g_body_desc& d = ds[k];
+		M.partBegin.push_back(P);
+		M.worldBegin.push_back(W);
+		M.gravity.push_back(v3(d.gravity[0], d.gravity[1], d.gravity[2]));
+		app(M.p_type, d.p_type, d.n_particles);
+		app(M.p_pos, d.p_pos, 3 * (size_t)d.n_particles);
+		app(M.p_vel, d.p_vel, 3 * (size_t)d.n_particles);
+		app(M.p_damping, d.p_damping, d.n_particles);
+		app_p(M.s_particle, d.s_particle, d.n_spheres, P);
+		app(M.s_radius, d.s_radius, d.n_spheres);
+		app(M.s_friction, d.s_friction, d.n_spheres);
+		app_p(M.tf_p, d.tf_p, 3 * (size_t)d.n_trifix, P);
+		app(M.tf_v, d.tf_v, 9 * (size_t)d.n_trifix);
+		app(M.tf_density, d.tf_density, d.n_trifix);
+		app(M.tf_radius, d.tf_radius, d.n_trifix);
+		app(M.tf_friction, d.tf_friction, d.n_trifix);
+		app_p(M.ttf_p, d.ttf_p, 4 * (size_t)d.n_tetfix, P);
+		app(M.ttf_v, d.ttf_v, 12 * (size_t)d.n_tetfix);
+		app(M.ttf_density, d.ttf_density, d.n_tetfix);
+		app(M.ttf_radius, d.ttf_radius, d.n_tetfix);
+		app(M.ttf_friction, d.ttf_friction, d.n_tetfix);
+		for (uint32_t c = 0; c < d.n_forces; ++c) {
+			M.f_type.push_back(d.f_type[c]);
+			M.f_index.push_back(d.f_index[c] + typeOff[d.f_type[c]]);
+		}
+		app_p(M.tet_p, d.tet_p, 4 * (size_t)d.n_tet, P);
+		app(M.tet_v, d.tet_v, 12 * (size_t)d.n_tet);
+		app(M.tet_param, d.tet_param, 3 * (size_t)d.n_tet);
+		app_p(M.tri_p, d.tri_p, 3 * (size_t)d.n_tri, P);
+		app(M.tri_v, d.tri_v, 9 * (size_t)d.n_tri);
+		app(M.tri_param, d.tri_param, 6 * (size_t)d.n_tri);
+		app_p(M.st_p, d.st_p, 3 * (size_t)d.n_stretch, P);
+		app(M.st_param, d.st_param, 12 * (size_t)d.n_stretch);
+		app_p(M.sh_p, d.sh_p, 3 * (size_t)d.n_shear, P);
+		app(M.sh_param, d.sh_param, 8 * (size_t)d.n_shear);
+		app_p(M.sp_p, d.sp_p, 2 * (size_t)d.n_spring, P);
+		app(M.sp_param, d.sp_param, 3 * (size_t)d.n_spring);
+		app_p(M.mo_p, d.mo_p, 4 * (size_t)d.n_mouse, P);
+		app(M.mo_param, d.mo_param, 6 * (size_t)d.n_mouse);
+		app(M.w_type, d.w_type, d.n_world);
+		app(M.w_param, d.w_param, (size_t)B3G_WORLD_PARAMS * d.n_world);
+		app(M.w_friction, d.w_friction, d.n_world);
+		for (uint32_t w = 0; w < d.n_world; ++w) M.w_voxels.push_back(d.w_voxels ? d.w_voxels[w] : nullptr);
+		typeOff[B3G_FORCE_STRETCH] += d.n_stretch; typeOff[B3G_FORCE_SHEAR] += d.n_shear;
+		typeOff[B3G_FORCE_SPRING] += d.n_spring; typeOff[B3G_FORCE_MOUSE] += d.n_mouse;
+		typeOff[B3G_FORCE_TRI_ELEMENT] += d.n_tri; typeOff[B3G_FORCE_TET_ELEMENT] += d.n_tet;
+		P += d.n_particles;
+		W += d.n_world;
+	}
+	M.partBegin.push_back(P);
+	M.worldBegin.push_back(W);
+	b3g_body_desc& o = M.d;
+	std::memset(&o, 0, sizeof(o));
+	o.n_particles = P;
+	o.p_type = M.p_type.data(); o.p_pos = M.p_pos.data(); o.p_vel = M.p_vel.data(); o.p_damping = M.p_damping.data();
+	o.n_spheres = (uint32_t)M.s_radius.size();
+	o.s_particle = M.s_particle.data(); o.s_radius = M.s_radius.data(); o.s_friction = M.s_friction.data();
+	o.n_trifix = (uint32_t)M.tf_density.size();
+	o.tf_p = M.tf_p.data(); o.tf_v = M.tf_v.data(); o.tf_density = M.tf_density.data(); o.tf_radius = M.tf_radius.data();
+	o.tf_friction = M.tf_friction.data();
+	o.n_tetfix = (uint32_t)M.ttf_density.size();
+	o.ttf_p = M.ttf_p.data(); o.ttf_v = M.ttf_v.data(); o.ttf_density = M.ttf_density.data();
+	o.ttf_radius = M.ttf_radius.data(); o.ttf_friction = M.ttf_friction.data();
+	o.n_forces = (uint32_t)M.f_type.size(); o.f_type = M.f_type.data(); o.f_index = M.f_index.data();
+	o.n_tet = (uint32_t)(M.tet_param.size() / 3); o.tet_p = M.tet_p.data(); o.tet_v = M.tet_v.data(); o.tet_param = M.tet_param.data();
+	o.n_tri = (uint32_t)(M.tri_param.size() / 6); o.tri_p = M.tri_p.data(); o.tri_v = M.tri_v.data(); o.tri_param = M.tri_param.data();
+	o.n_stretch = (uint32_t)(M.st_param.size() / 12); o.st_p = M.st_p.data(); o.st_param = M.st_param.data();
+	o.n_shear = (uint32_t)(M.sh_param.size() / 8); o.sh_p = M.sh_p.data(); o.sh_param = M.sh_param.data();
+	o.n_spring = (uint32_t)(M.sp_param.size() / 3); o.sp_p = M.sp_p.data(); o.sp_param = M.sp_param.data();
+	o.n_mouse = (uint32_t)(M.mo_param.size() / 6); o.mo_p = M.mo_p.data(); o.mo_param = M.mo_param.data();
+	o.n_world = (uint32_t)M.w_type.size(); o.w_type = M.w_type.data(); o.w_param = M.w_param.data();
+	o.w_friction = M.w_friction.data(); o.w_voxels = M.w_voxels.data();
+	if (count) for (int c = 0; c < 3; ++c) o.gravity[c] = ds[0].gravity[c];
+}
+
 } // namespace
 
 b3g_status host_pattern(const b3g_body_desc& d, std::vector<uint32_t>& rowptr, std::vector<uint32_t>& cols);
 
-b3g_status build_body(Body& b, const b3g_body_desc& d) {
-	b3g_status st = validate(d);
-	if (st != B3G_OK) return st;
+b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
+	if (count == 0) { set_error("a batch needs at least one body description"); return B3G_INVALID_ARGUMENT; }
+	for (uint32_t k = 0; k < count; ++k) {
+		b3g_status st = validate(descs[k]);
+		if (st != B3G_OK) {
+			if (count > 1) set_error("instance " + std::to_string(k) + ": " + b3g_last_error());
+			return st;
+		}
+	}
+	Merged M;
+	merge(descs, count, M);
+	const b3g_body_desc& d = M.d;
+	if (d.n_tet >= (1u << 27)) {
+		set_error("more than 2^27 - 1 tetrahedron elements");
+		return B3G_UNSUPPORTED;
+	}
+	b3g_status st = B3G_OK;
 	dfree_all(b);
 	b.built = false;
 	const uint32_t n = d.n_particles;
 	auto L = [n](uint32_t c) { return n - 1 - c; };   // creation -> list order (body.cpp:67-75)
 	b.n = n;
+	// instances: creation range [partBegin[k], partBegin[k+1]) -> rows [n - partBegin[k+1], n - partBegin[k])
+	b.nInst = count;
+	b.hostRowInst.assign(n, 0);
+	b.hostRowBegin.resize(count);
+	b.hostRowEnd.resize(count);
+	b.hostGravity = M.gravity;
+	for (uint32_t k = 0; k < count; ++k) {
+		b.hostRowBegin[k] = n - M.partBegin[k + 1];
+		b.hostRowEnd[k] = n - M.partBegin[k];
+		for (uint32_t i = b.hostRowBegin[k]; i < b.hostRowEnd[k]; ++i) b.hostRowInst[i] = k;
+	}
 
 	// ------------------------------------------------------------ particles
 	Built h;
@@ -585,6 +722,7 @@ b3g_status build_body(Body& b, const b3g_body_desc& d) {
 	b.hostMass = h.mass;
 	b.hostType = h.type;
 	b.nSpheres = (uint32_t)h.sphereParticle.size();
+	b.hostSphereParticle = h.sphereParticle;
 
 	// ------------------------------------------------------------ forces (list order = reverse creation)
 	for (uint32_t c = d.n_forces; c-- > 0;) h.forces.push_back({d.f_type[c], d.f_index[c]});
@@ -642,7 +780,6 @@ b3g_status build_body(Body& b, const b3g_body_desc& d) {
 
 	// ------------------------------------------------------------ world fixtures (list order)
 	b.nWorld = d.n_world;
-	for (int k = 0; k < 3; ++k) b.gravity[k] = d.gravity[k];
 	b.hostWorlds.clear();
 	std::vector<float> vox;
 	for (uint32_t c = d.n_world; c-- > 0;) {
@@ -682,7 +819,26 @@ b3g_status build_body(Body& b, const b3g_body_desc& d) {
 	// ------------------------------------------------------------ symbolic pattern
 	SymbolicInput si;
 	si.n = n; si.type = &h.type; si.damping = &h.damping; si.sphereOf = &h.sphereOf;
-	si.contacts = d.n_world > 0 && b.nSpheres > 0;
+	// contact pairs: each sphere (list order) with the world fixtures of its
+	// instance (list order); rows that can touch a world fixture
+	std::vector<char> contactRow(n, 0);
+	{
+		b.hostPairStart.assign(1, 0);
+		b.hostPairSphere.clear();
+		b.hostPairWorld.clear();
+		for (uint32_t sp = 0; sp < b.nSpheres; ++sp) {
+			const uint32_t row = h.sphereParticle[sp], k = b.hostRowInst[row];
+			const uint32_t w0 = d.n_world - M.worldBegin[k + 1], w1 = d.n_world - M.worldBegin[k];
+			for (uint32_t w = w0; w < w1; ++w) {
+				b.hostPairSphere.push_back(sp);
+				b.hostPairWorld.push_back(w);
+			}
+			if (w1 > w0) contactRow[row] = 1;
+			b.hostPairStart.push_back((uint32_t)b.hostPairSphere.size());
+		}
+		b.nPairs = (uint32_t)b.hostPairSphere.size();
+	}
+	si.contactRow = b.nPairs ? &contactRow : nullptr;
 	si.forces = h.forces;
 	si.tetP = &tetP; si.tetM = d.n_tet; si.tetKd = &tetKd;
 	si.triP = &triP; si.triM = d.n_tri; si.triKd = &triKd;
@@ -725,19 +881,27 @@ b3g_status build_body(Body& b, const b3g_body_desc& d) {
 		const uint32_t kSlots = 256, kCap = 1024;
 		std::vector<uint32_t> t{0};
 		uint32_t slots = 0, cx = 0, cv = 0;
+		std::vector<uint32_t> ti;
+		uint32_t inst = b.hostRowInst[P.slotRow.empty() ? 0 : P.slotRow[0]];
 		for (uint32_t s = 0; s < b.nnz; ++s) {
 			const uint32_t lx = P.jxPtr[s + 1] - P.jxPtr[s], lv = P.jvPtr[s + 1] - P.jvPtr[s];
-			if (slots > 0 && (slots == kSlots || cx + lx > kCap || cv + lv > kCap)) {
+			const uint32_t k = b.hostRowInst[P.slotRow[s]];
+			if (slots > 0 && (slots == kSlots || cx + lx > kCap || cv + lv > kCap || k != inst)) {
 				t.push_back(s);
+				ti.push_back(inst);
 				slots = cx = cv = 0;
 			}
+			inst = k;
 			slots += 1;
 			cx += lx;
 			cv += lv;
 		}
 		t.push_back(b.nnz);
+		ti.push_back(inst);
 		b.nSlotTiles = (uint32_t)t.size() - 1;
 		st = dupload(b, b.slotTiles, t.data(), t.size() * 4);
+		if (st != B3G_OK) return st;
+		st = dupload(b, b.slotTileInst, ti.data(), ti.size() * 4);
 		if (st != B3G_OK) return st;
 	}
 
@@ -815,20 +979,30 @@ b3g_status build_body(Body& b, const b3g_body_desc& d) {
 	if (!P.fIdx.empty()) UP(b.fIdx, P.fIdx);
 	AL(b.A, (size_t)b.nnz * 9 * 4);
 	AL(b.tB, (size_t)b.nnz * sizeof(V3));
-	AL(b.state, sizeof(SolverState));
-	SolverState s0;
-	std::memset(&s0, 0, sizeof(s0));
-	s0.statMinSub = 0xFFFFFFFFu;
-	B3G_HIP(hipMemcpyAsync(b.state.p, &s0, sizeof(s0), hipMemcpyHostToDevice, b.stream));
+	{
+		SolverState s0;
+		std::memset(&s0, 0, sizeof(s0));
+		s0.statMinSub = 0xFFFFFFFFu;
+		std::vector<SolverState> states(count, s0);
+		UP(b.state, states);
+	}
+	UP(b.gravity, b.hostGravity);
+	UP(b.rowInst, b.hostRowInst);
 	// contacts
 	if (b.nWorld) {
 		UP(b.worlds, b.hostWorlds);
 		if (!vox.empty()) UP(b.voxels, vox);
-		const size_t pairs = (size_t)std::max(1u, b.nSpheres) * b.nWorld;
+	}
+	if (b.nPairs) {
+		const size_t pairs = b.nPairs;
+		UP(b.pairStart, b.hostPairStart);
+		UP(b.pairSphere, b.hostPairSphere);
+		UP(b.pairWorld, b.hostPairWorld);
 		AL(b.cActive, pairs * 4); AL(b.cStamp, pairs * 4); AL(b.cFriction, pairs * 4); AL(b.cHit, pairs * 4);
 		AL(b.cNormalForce, pairs * 4); AL(b.cManifold, pairs * 2 * sizeof(V3)); AL(b.cF, pairs * sizeof(V3));
 		AL(b.cK, pairs * sizeof(M33));
-		B3G_HIP(hipMemsetAsync(b.cActive.p, 0, pairs * 4, b.stream));
+		std::vector<uint32_t> zeros32(pairs, 0u);
+		UP(b.cActive, zeros32);
 	}
 	b.stampCounter = 0;
 #undef UP
@@ -864,7 +1038,8 @@ b3g_status host_pattern(const b3g_body_desc& d, std::vector<uint32_t>& rowptr, s
 		if (sphereOf[L(d.s_particle[c])] < 0) sphereOf[L(d.s_particle[c])] = (int32_t)nSpheres++;
 	SymbolicInput si;
 	si.n = n; si.type = &type; si.damping = &damping; si.sphereOf = &sphereOf;
-	si.contacts = d.n_world > 0 && nSpheres > 0;
+	std::vector<char> contactRow(n, d.n_world > 0 ? 1 : 0);
+	si.contactRow = d.n_world > 0 && nSpheres > 0 ? &contactRow : nullptr;
 	for (uint32_t c = d.n_forces; c-- > 0;) si.forces.push_back({d.f_type[c], d.f_index[c]});
 	std::vector<uint32_t> tetP(4 * (size_t)d.n_tet), triP(3 * (size_t)d.n_tri), stP(3 * (size_t)d.n_stretch),
 	    shP(3 * (size_t)d.n_shear), spP(2 * (size_t)d.n_spring), moP(4 * (size_t)d.n_mouse);

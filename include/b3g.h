@@ -170,6 +170,22 @@ b3g_status b3g_body_destroy(b3g_body body);
 b3g_status b3g_body_build(b3g_body body, const b3g_body_desc* desc);
 
 /*
+ * Batched form (no counterpart in the reference, which steps one b3Body at a
+ * time): builds `count` independent bodies ("instances") into one handle.
+ * b3g_step then steps all of them with one launch sequence: every kernel
+ * covers all instances, each instance keeps its own Newton / PCG state,
+ * convergence and contact list, and no reduction mixes instances, so each
+ * instance's trajectory is the one b3g_body_build + b3g_step would give it
+ * alone. Per-particle and per-element arrays of the other entry points are
+ * the instances' arrays concatenated in instance order (each in creation
+ * order); b3g_get_stats and b3g_get_energy write one entry per instance.
+ */
+b3g_status b3g_body_build_batch(b3g_body body, const b3g_body_desc* descs, uint32_t count);
+
+/* Number of instances of a built body (1 after b3g_body_build). */
+b3g_status b3g_get_instance_count(b3g_body body, uint32_t* count);
+
+/*
  * One time step. Replaces b3Body::Step(dt, forceIterations, forceSubIterations)
  * (body.h:153, body.cpp:552-604). tolerance / sub_tolerance are the Newton and
  * PCG tolerances of b3SolveBEInput (sparse_force_solver.h:55-61); a value <= 0
@@ -190,17 +206,18 @@ b3g_status b3g_apply_external(b3g_body body, const float* force, const float* tr
 /* Lumped masses (b3Particle::GetMass), creation order. */
 b3g_status b3g_get_masses(b3g_body body, float* mass);
 
-/* Solver statistics (the reference's b3_forceSolver* globals) and contact count. */
+/* Solver statistics (the reference's b3_forceSolver* globals) and contact count, one entry per instance. */
 b3g_status b3g_get_stats(b3g_body body, b3g_step_stats* stats);
 
-/* Kinetic energy 0.5 * sum m v.v in list order, FP32 sequential (b3Body::GetEnergy, body.cpp:384-392). */
+/* Kinetic energy 0.5 * sum m v.v in list order, FP32 sequential (b3Body::GetEnergy, body.cpp:384-392), one value per instance. */
 b3g_status b3g_get_energy(b3g_body body, float* energy);
 
 /*
  * Contact list in list order (b3ContactManager::m_contactList): creation
  * index of the sphere fixture's particle, creation index of the world
  * fixture and the accumulated normal force. *count receives the number of
- * contacts; at most `capacity` entries are written.
+ * contacts; at most `capacity` entries are written. A batch lists each
+ * instance's contacts in turn, instances in creation order.
  */
 b3g_status b3g_get_contacts(b3g_body body, uint32_t* count, uint32_t* particle, uint32_t* world, float* normal_force,
                             uint32_t capacity);
@@ -221,13 +238,18 @@ b3g_status b3g_get_system(b3g_body body, uint32_t* rowptr, uint32_t* cols, float
 
 /*
  * Reduction of the PCG dot products and of the Newton error norm.
- *   B3G_REDUCE_FP64 (default): deterministic FP64 tree over the device.
+ *   B3G_REDUCE_FP64 (default): deterministic FP64 tree over the device; a
+ *   small kernel per reduction forms alpha / beta / convergence.
+ *   B3G_REDUCE_FP64_FUSED: the same sums in the same order, formed instead at
+ *   the start of the kernel that consumes them (two launches fewer per PCG
+ *   iteration, a little more work per workgroup); results are identical to
+ *   B3G_REDUCE_FP64 bit for bit.
  *   B3G_REDUCE_SEQUENTIAL: FP32 terms added one by one in row order, as
  *   b3Dot / b3LengthSquared do (dense_vec3.h:151-175). With it a step is the
  *   reference's step operation for operation; it is O(rows) latency per
  *   reduction and meant for validation, not throughput.
  */
-enum { B3G_REDUCE_FP64 = 0, B3G_REDUCE_SEQUENTIAL = 1 };
+enum { B3G_REDUCE_FP64 = 0, B3G_REDUCE_SEQUENTIAL = 1, B3G_REDUCE_FP64_FUSED = 2 };
 b3g_status b3g_set_reduction(b3g_body body, int mode);
 
 /*
