@@ -20,6 +20,7 @@ u8p = C.POINTER(C.c_uint8)
 
 B3G_OK = 0
 REDUCE_FP64, REDUCE_SEQUENTIAL, REDUCE_FP64_FUSED = 0, 1, 2
+SOLVER_PCG, SOLVER_REFINED_FP32, SOLVER_REFINED_BF16 = 0, 1, 2
 STATUS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "out of device memory", 4: "degenerate element",
           5: "unsupported", 6: "body not built"}
 
@@ -55,7 +56,8 @@ class BodyDesc(C.Structure):
 
 class StepStats(C.Structure):
     _fields_ = [("newton_iterations", C.c_uint32), ("min_sub_iterations", C.c_uint32),
-                ("max_sub_iterations", C.c_uint32), ("contact_count", C.c_uint32)]
+                ("max_sub_iterations", C.c_uint32), ("contact_count", C.c_uint32),
+                ("refinement_sweeps", C.c_uint32), ("relative_residual", C.c_float)]
 
 
 def _ptr(a, t):
@@ -109,7 +111,7 @@ EXPORTS = [
     "b3g_status_string", "b3g_last_error", "b3g_device_count", "b3g_body_create", "b3g_body_destroy",
     "b3g_body_build", "b3g_body_build_batch", "b3g_get_instance_count", "b3g_step", "b3g_get_state", "b3g_set_state", "b3g_apply_external", "b3g_get_masses",
     "b3g_get_stats", "b3g_get_energy", "b3g_get_contacts", "b3g_get_tet_quats", "b3g_assemble", "b3g_get_system",
-    "b3g_synchronize", "b3g_time_spmv", "b3g_get_system_size", "b3g_pattern",
+    "b3g_synchronize", "b3g_time_spmv", "b3g_set_solver", "b3g_get_system_size", "b3g_pattern",
     "b3g_solve_cg", "b3g_spmv", "b3g_set_reduction", "b3g_save_state", "b3g_load_state",
 ]
 
@@ -145,6 +147,7 @@ def lib():
         "b3g_get_system": (C.c_int, [vp, u32p, u32p, f32p, f32p, f32p]),
         "b3g_synchronize": (C.c_int, [vp]),
         "b3g_time_spmv": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "b3g_set_solver": (C.c_int, [vp, C.c_int, C.c_uint32, C.c_float]),
         "b3g_get_system_size": (C.c_int, [vp, u32p, u32p]),
         "b3g_pattern": (C.c_int, [C.POINTER(BodyDesc), u32p, u32p, u32p, C.c_uint32]),
         "b3g_solve_cg": (C.c_int, [C.c_int, C.c_uint32, u32p, u32p, f32p, f32p, f32p, C.c_uint32, C.c_float, u32p]),

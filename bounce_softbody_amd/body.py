@@ -175,6 +175,17 @@ class Body:
         check(self._L.b3g_get_system_size(self._h, C.byref(r), C.byref(b)), "b3g_get_system_size")
         return r.value, b.value
 
+    def set_solver(self, mode: int, inner_iterations: int = 10, inner_tolerance: float = 1e-2):
+        """_lib.SOLVER_PCG (the reference's PCG) or the precision form: _lib.SOLVER_REFINED_FP32 /
+        _lib.SOLVER_REFINED_BF16 (FP64 iterative refinement around an inner PCG on FP32 or BF16 blocks)."""
+        check(self._L.b3g_set_solver(self._h, mode, inner_iterations, C.c_float(inner_tolerance)), "b3g_set_solver")
+
+    def refinement(self):
+        """(sweeps, relative FP64 residual) of the last Newton iteration, per instance."""
+        s = (StepStats * self.n_inst)()
+        check(self._L.b3g_get_stats(self._h, s), "b3g_get_stats")
+        return [(x.refinement_sweeps, x.relative_residual) for x in s]
+
     def set_reduction(self, mode: int):
         """_lib.REDUCE_FP64 (default) or _lib.REDUCE_SEQUENTIAL (the reference's FP32 accumulation order)."""
         check(self._L.b3g_set_reduction(self._h, mode), "b3g_set_reduction")

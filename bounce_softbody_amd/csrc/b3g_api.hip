@@ -218,6 +218,27 @@ b3g_status b3g_get_stats(b3g_body h, b3g_step_stats* out) {
 		out[k].min_sub_iterations = st[k].statMinSub;
 		out[k].max_sub_iterations = st[k].statMaxSub;
 		out[k].contact_count = count[k];
+		out[k].refinement_sweeps = st[k].refineOn ? st[k].statSweeps : 0u;
+		out[k].relative_residual = st[k].refineOn && st[k].res0 > 0.0 ? (float)std::sqrt(st[k].res / st[k].res0) : 0.0f;
+	}
+	return B3G_OK;
+}
+
+b3g_status b3g_set_solver(b3g_body h, int mode, uint32_t inner_iterations, float inner_tolerance) {
+	CHECK_BODY(h);
+	if (mode != B3G_SOLVER_PCG && mode != B3G_SOLVER_REFINED_FP32 && mode != B3G_SOLVER_REFINED_BF16) {
+		set_error("unknown solver mode");
+		return B3G_INVALID_ARGUMENT;
+	}
+	if (mode != B3G_SOLVER_PCG && (inner_iterations == 0 || !(inner_tolerance > 0.0f && inner_tolerance < 1.0f))) {
+		set_error("refinement needs inner_iterations >= 1 and inner_tolerance in (0, 1)");
+		return B3G_INVALID_ARGUMENT;
+	}
+	Body& b = h->body;
+	b.solver = mode;
+	if (mode != B3G_SOLVER_PCG) {
+		b.innerIt = inner_iterations;
+		b.innerTol = inner_tolerance;
 	}
 	return B3G_OK;
 }

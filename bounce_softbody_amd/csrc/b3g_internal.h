@@ -118,6 +118,11 @@ struct SolverState {
 	uint32_t contactCount;
 	// reduction mode of the dot products: 0 FP64 tree, 1 sequential FP32 in row order
 	uint32_t seq;
+	// iterative refinement (precision form): FP64 residual norms of the
+	// Newton iteration's linear system, gate and counters of the sweeps
+	double res0, res;
+	float refineTol2;
+	uint32_t refineOn, refineActive, sweeps, maxSweeps, itSum, statSweeps;
 };
 
 
@@ -195,6 +200,22 @@ struct Body {
 	uint32_t partialCount = 0;
 	DBuf cgLog;      // CGLog per instance and PCG iteration
 	uint32_t logStride = 0;   // entries per instance
+	// precision form: FP64 iterative refinement around an inner PCG on FP32
+	// or BF16-off-diagonal blocks
+	int solver = 0;           // B3G_SOLVER_*
+	uint32_t innerIt = 10;
+	float innerTol = 1e-2f;
+	DBuf x64;        // double3 per row: refined solution
+	DBuf dx, rhsF;   // V3 per row: inner correction, inner right-hand side
+	DBuf partialR;   // double per tile: FP64 residual norm
+	DBuf Ab, Ab8;    // BF16 blocks: entries 0..7 as uint4, entry 8 as uint16 (diagonal slots zero)
+	DBuf diagF;      // 9 planes x rows: FP32 diagonal blocks of the BF16 operator
+	DBuf tet64;      // 12 planes x tets: FP64 invE (9), volume, lambda, mu
+	DBuf mass64;     // double per row: lumped masses in FP64
+	std::vector<double> hostMass64;
+	DBuf fstage64;   // D3 per force staging slot (tetrahedra only): FP64 forces
+	DBuf tB64;       // double3 per block: (h dfdx_ij) w_j in FP64
+	DBuf pb64;       // double3 per row: FP64 right-hand side
 	DBuf terms;      // float per row: per-row dot terms (sequential reduction mode)
 	int reduction = 0;
 	DBuf state;      // SolverState per instance

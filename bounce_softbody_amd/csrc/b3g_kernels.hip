@@ -62,7 +62,7 @@ __device__ double block_sum(double v, double* sh) {
 	return t;   // valid in thread 0
 }
 
-enum Finalize : int { kFinAlpha = 0, kFinDelta = 1, kFinInit = 2, kFinNewton = 3 };
+enum Finalize : int { kFinAlpha = 0, kFinDelta = 1, kFinInit = 2, kFinNewton = 3, kFinRefine0 = 4, kFinRefine = 5 };
 
 // The PCG's state machine (sparse_solver.cpp:57-103) over the per-iteration
 // log: Init fills entry 0, Alpha(k) entry k's alpha, Delta(k) entry k from
@@ -74,12 +74,28 @@ __device__ void finalize(SolverState* st, CGLog* log, int kind, uint32_t k, doub
 	case kFinAlpha:   // sparse_solver.cpp:77
 		log[k].alpha = log[k].delta / v;
 		break;
-	case kFinInit:    // sparse_solver.cpp:57-73
+	case kFinInit:    // sparse_solver.cpp:57-73 (a refinement sweep solves only while the sweeps are active)
 		st->delta0 = v;
 		st->it = 0;
 		log[0].delta = v;
 		log[0].beta = 0.0f;
-		log[0].active = (st->maxIt != 0 && !(v <= st->eps2_sub * v)) ? 1u : 0u;
+		log[0].active = (st->maxIt != 0 && !(v <= st->eps2_sub * v) && (!st->refineOn || st->refineActive)) ? 1u : 0u;
+		break;
+	case kFinRefine0: // first FP64 residual of a Newton iteration: a sweep follows unless it is zero
+		st->res0 = total;
+		st->res = total;
+		st->sweeps = 0;
+		st->itSum = 0;
+		st->refineActive = (st->maxSweeps != 0 && total > 0.0) ? 1u : 0u;
+		break;
+	case kFinRefine:  // residual after a sweep: count the sweep and its inner iterations, test convergence
+		st->itSum += st->it;
+		st->it = 0;
+		if (st->refineActive) {
+			st->res = total;
+			st->sweeps += 1;
+			st->refineActive = (st->sweeps < st->maxSweeps && !(total <= (double)st->refineTol2 * st->res0)) ? 1u : 0u;
+		}
 		break;
 	case kFinDelta: { // sparse_solver.cpp:92-102
 		log[k].delta = v;
@@ -90,8 +106,10 @@ __device__ void finalize(SolverState* st, CGLog* log, int kind, uint32_t k, doub
 	}
 	case kFinNewton: { // sparse_force_solver.cpp:196-225
 		st->error = v;
-		st->statMinSub = min(st->statMinSub, st->it);
-		st->statMaxSub = max(st->statMaxSub, st->it);
+		const uint32_t its = st->refineOn ? st->itSum + st->it : st->it;
+		st->statMinSub = min(st->statMinSub, its);
+		st->statMaxSub = max(st->statMaxSub, its);
+		if (st->refineOn) st->statSweeps = st->sweeps;
 		bool conv;
 		if (st->newtonIt == 0) { st->error0 = v; conv = v <= st->eps2; }
 		else conv = v <= st->eps2 * st->error0;
@@ -189,7 +207,7 @@ __global__ void __launch_bounds__(kFinBlock) k_finalize(Reduce rd, int sel, int 
 	if (kind == kFinInit) clear_log(log, rd.logStride);
 	const uint32_t* R = rd.range + 6 * (size_t)inst;
 	__shared__ float chunk[4 * kFinBlock];
-	if (st->seq) {
+	if (st->seq && kind != kFinRefine0 && kind != kFinRefine) {
 		const uint32_t r0 = R[4], r1 = R[5];
 		float acc = 0.0f;
 		for (uint32_t base = r0; base < r1; base += 4 * kFinBlock) {
@@ -306,11 +324,16 @@ __global__ void __launch_bounds__(kBlock) k_step_begin(StepArgs a) {
 }
 
 __global__ void __launch_bounds__(kBlock) k_state_begin(SolverState* sts, uint32_t count, uint32_t iters, uint32_t subIters,
-                                                         float tol, float subTol, uint32_t seq) {
+                                                         float tol, float subTol, uint32_t seq, uint32_t refineOn,
+                                                         uint32_t maxSweeps, float refineTol) {
 	uint32_t k = blockIdx.x * kBlock + threadIdx.x;
 	if (k >= count) return;
 	SolverState* st = sts + k;
 	st->seq = seq;
+	st->refineOn = refineOn;
+	st->maxSweeps = maxSweeps;
+	st->refineTol2 = refineTol * refineTol;
+	st->refineActive = 0;
 	st->newtonIt = 0;
 	st->newtonMax = iters;
 	st->newtonActive = iters > 0 ? 1u : 0u;
@@ -419,6 +442,131 @@ __global__ void __launch_bounds__(kBlock) k_tet_forces(TetArgs a) {
 		}
 		a.fstage[a.f + (size_t)i * m + e] = -(R * fs);      // f[vi] -= R fs
 		if (kd > 0.0f) a.fstage[a.fd + (size_t)i * m + e] = -fds;
+	}
+}
+
+// ---- precision form: tetrahedron forces in FP64 ----------------------------
+// Rest data in double (tetrahedron_element_force.cpp:219-265): 12 planes per
+// tet: invE (9), volume, lambda, mu.
+__global__ void __launch_bounds__(kBlock) k_tet_setup64(uint32_t m, const float* rest, const float* param, double* tet64) {
+	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+	if (e >= m) return;
+	D3 x[4];
+	for (int k = 0; k < 4; ++k)
+		x[k] = d3(rest[(size_t)(3 * k) * m + e], rest[(size_t)(3 * k + 1) * m + e], rest[(size_t)(3 * k + 2) * m + e]);
+	D3 e1 = x[1] - x[0], e2 = x[2] - x[0], e3 = x[3] - x[0];
+	double dt = det(e1, e2, e3);
+	const double V = fabs(dt) / 6.0;
+	DM33 iE = (1.0 / dt) * adjugate(dm33(e1, e2, e3));
+	const double* pi = &iE.x.x;
+	for (int c = 0; c < 9; ++c) tet64[(size_t)c * m + e] = pi[c];
+	const double E = param[e], nu = param[(size_t)m + e];
+	tet64[(size_t)9 * m + e] = V;
+	tet64[(size_t)10 * m + e] = (nu * E) / ((1.0 + nu) * (1.0 - 2.0 * nu));   // b3ComputeD (:37-57)
+	tet64[(size_t)11 * m + e] = E / (2.0 * (1.0 + nu));
+}
+
+// K0 u = V B^T D B u for the 4 vertex vectors u (K0 = V B^T D B, :121-159 and
+// :247-261), evaluated through the strain: exact in real arithmetic, and in
+// double free of K0's rounding. g[v] = shape-function gradients.
+__device__ void k0_apply(const D3 g[4], double V, double lambda, double mu, const D3 u[4], D3 out[4]) {
+	double e0 = 0, e1 = 0, e2 = 0, e3 = 0, e4 = 0, e5 = 0;
+	for (int v = 0; v < 4; ++v) {
+		e0 += g[v].x * u[v].x;
+		e1 += g[v].y * u[v].y;
+		e2 += g[v].z * u[v].z;
+		e3 += g[v].y * u[v].x + g[v].x * u[v].y;
+		e4 += g[v].z * u[v].y + g[v].y * u[v].z;
+		e5 += g[v].z * u[v].x + g[v].x * u[v].z;
+	}
+	const double l2m = lambda + 2.0 * mu;
+	const double s0 = l2m * e0 + lambda * e1 + lambda * e2;
+	const double s1 = lambda * e0 + l2m * e1 + lambda * e2;
+	const double s2 = lambda * e0 + lambda * e1 + l2m * e2;
+	const double s3 = mu * e3, s4 = mu * e4, s5 = mu * e5;
+	for (int v = 0; v < 4; ++v)
+		out[v] = V * d3(g[v].x * s0 + g[v].y * s3 + g[v].z * s5, g[v].y * s1 + g[v].x * s3 + g[v].z * s4,
+		                g[v].z * s2 + g[v].y * s4 + g[v].x * s5);
+}
+
+// b3ExtractRotation in double (tetrahedron_element_force.cpp:273-311).
+__device__ DQ4 extract_rotation64(const DM33& A, DQ4 q) {
+	for (int it = 0; it < 32; ++it) {
+		DM33 R = qmatrix(q);
+		double den = fabs(dot(R.x, A.x) + dot(R.y, A.y) + dot(R.z, A.z));
+		if (den == 0.0) break;
+		D3 torque = cross(R.x, A.x) + cross(R.y, A.y) + cross(R.z, A.z);
+		const double kTol = 1.0e-9;
+		double inv_den = 1.0 / den + kTol;
+		D3 omega = inv_den * torque;
+		double w = length(omega);
+		if (w < kTol) break;
+		D3 axis = omega / w;
+		double sd, cd;
+		sincos(0.5 * w, &sd, &cd);
+		DQ4 qw;
+		qw.v = sd * axis;
+		qw.s = cd;
+		q = qmul(qw, q);
+		qnormalize(q);
+	}
+	return q;
+}
+
+// k_tet_forces in FP64: polar decomposition, u = R^T x - x_rest and the forces
+// in double, written to the FP64 force arena (and, rounded, to the FP32 one).
+// The rotation is kept (rounded) as the warm start; the Jacobian blocks the
+// assembly recomputes from it stay FP32.
+__global__ void __launch_bounds__(kBlock) k_tet_forces64(TetArgs a, const double* tet64, D3* fstage64) {
+	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+	if (e >= a.m) return;
+	if (!a.st[a.rowInst[a.p[e]]].newtonActive) return;
+	const uint32_t m = a.m;
+	uint32_t id[4];
+	D3 p[4], xr[4];
+	for (int k = 0; k < 4; ++k) {
+		id[k] = a.p[(size_t)k * m + e];
+		p[k] = d3(a.x[id[k]]);
+		xr[k] = d3(a.rest[(size_t)(3 * k) * m + e], a.rest[(size_t)(3 * k + 1) * m + e], a.rest[(size_t)(3 * k + 2) * m + e]);
+	}
+	DM33 iE;
+	double* pi = &iE.x.x;
+	for (int c = 0; c < 9; ++c) pi[c] = tet64[(size_t)c * m + e];
+	const double V = tet64[(size_t)9 * m + e], lambda = tet64[(size_t)10 * m + e], mu = tet64[(size_t)11 * m + e];
+	DM33 F = dm33(p[1] - p[0], p[2] - p[0], p[3] - p[0]) * iE;
+	const float4 q4 = a.q[e];
+	DQ4 q;
+	q.v = d3(q4.x, q4.y, q4.z);
+	q.s = q4.w;
+	q = extract_rotation64(F, q);
+	a.q[e] = make_float4((float)q.v.x, (float)q.v.y, (float)q.v.z, (float)q.s);
+	DM33 R = qmatrix(q);
+	DM33 RT = transpose(R);
+	// shape-function gradients (rows of invE; vertex 0 = minus their sum)
+	D3 g[4];
+	g[1] = d3(iE.x.x, iE.y.x, iE.z.x);
+	g[2] = d3(iE.x.y, iE.y.y, iE.z.y);
+	g[3] = d3(iE.x.z, iE.y.z, iE.z.z);
+	g[0] = d3(-g[1].x - g[2].x - g[3].x, -g[1].y - g[2].y - g[3].y, -g[1].z - g[2].z - g[3].z);
+	D3 us[4], fs[4];
+	for (int k = 0; k < 4; ++k) us[k] = RT * p[k] - xr[k];
+	k0_apply(g, V, lambda, mu, us, fs);
+	const float kd = a.kd[e];
+	D3 fds[4];
+	if (kd > 0.0f) {   // f_d = -kd R K0 R^T v
+		D3 uv[4];
+		for (int k = 0; k < 4; ++k) uv[k] = RT * d3(a.v[id[k]]);
+		k0_apply(g, V, lambda, mu, uv, fds);
+	}
+	for (int i = 0; i < 4; ++i) {
+		const D3 f = -(R * fs[i]);
+		fstage64[a.f + (size_t)i * m + e] = f;
+		a.fstage[a.f + (size_t)i * m + e] = v3((float)f.x, (float)f.y, (float)f.z);
+		if (kd > 0.0f) {
+			const D3 fd = -((double)kd * (R * fds[i]));
+			fstage64[a.fd + (size_t)i * m + e] = fd;
+			a.fstage[a.fd + (size_t)i * m + e] = v3((float)fd.x, (float)fd.y, (float)fd.z);
+		}
 	}
 }
 
@@ -921,6 +1069,7 @@ struct AsmArgs {
 	const V3* v;
 	const V3* fe;
 	const V3* w;
+	const V3* y;
 	float* A;
 	V3* tB;
 	V3* pb;
@@ -928,6 +1077,15 @@ struct AsmArgs {
 	V3* fi;
 	ContactArgs c;
 	const SolverState* st;       // per instance
+	// precision form: FP64 right-hand side
+	double* tB64;                // double3 per block, or null
+	const D3* fstage64;
+	uint32_t tetF0, tetF1, tetFd0, tetFd1;   // staging ranges of the FP64 tetrahedron forces
+	double* pb64;
+	const double* mass64;
+	const V3* force;
+	const V3* gravity;
+	const uint32_t* rowInst;
 };
 
 // acc += the contributions [ptr[s], ptr[s + 1]) of slot s, in order, for the
@@ -1084,6 +1242,82 @@ __global__ void __launch_bounds__(kBlock) k_assemble_slots(AsmArgs a) {
 	stA(a.A, a.nnz, s, P);
 	// (h dfdx_ij) w_j for b's product
 	a.tB[s] = (h * dx) * w_j;
+	if (a.tB64) {   // precision form: the same product in FP64 with w in FP64
+		const D3 w64 = ((d3(a.x0[j]) - d3(a.c.x[j])) + (double)h * d3(a.v[j])) + d3(a.y[j]);
+		const D3 t = ((double)h * dm33(dx)) * w64;
+		a.tB64[3 * (size_t)s] = t.x;
+		a.tB64[3 * (size_t)s + 1] = t.y;
+		a.tB64[3 * (size_t)s + 2] = t.z;
+	}
+}
+
+// Precision form: fi, b and pb in FP64 (sparse_force_solver.cpp:165-174),
+// from the FP64 tetrahedron forces and the FP32 contributions of the other
+// forces, in the reference's order per row.
+__global__ void __launch_bounds__(kBlock) k_assemble_rows64(AsmArgs a) {
+	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+	if (!a.st[a.tileInst[t]].newtonActive) return;
+	__shared__ double lds[3 * (kTileCap / 2)];
+	constexpr uint32_t cap = kTileCap / 2;
+	const uint32_t r0 = a.rowTiles[t], r1 = a.rowTiles[t + 1];
+	const uint32_t i = r0 + threadIdx.x;
+	const bool own = i < r1;
+	const double mass = own ? a.mass64[i] : 0.0;
+	const D3 vi = own ? d3(a.v[i]) : d3(0.0, 0.0, 0.0);
+	D3 fi = d3(0.0, 0.0, 0.0);
+	if (own && a.damping[i] > 0.0f) fi = fi + (-(double)a.damping[i] * mass) * vi;
+	D3 hdw = d3(0.0, 0.0, 0.0);
+	for (int pass = 0; pass < 2; ++pass) {
+		const uint32_t* ptr = pass == 0 ? a.fPtr : a.bPtr;
+		const uint32_t c0 = ptr[r0], c1 = ptr[r1];
+		const uint32_t rb = own ? ptr[i] : 0, re = own ? ptr[i + 1] : 0;
+		D3& acc = pass == 0 ? fi : hdw;
+		for (uint32_t cb = c0; cb < c1; cb += cap) {
+			const uint32_t ce = min(cb + cap, c1);
+			for (uint32_t k = cb + threadIdx.x; k < ce; k += kBlock) {
+				D3 v;
+				if (pass == 0) {
+					const uint32_t idx = a.fIdx[k];
+					const bool tet = (idx >= a.tetF0 && idx < a.tetF1) || (idx >= a.tetFd0 && idx < a.tetFd1);
+					v = tet ? a.fstage64[idx] : d3(a.fstage[idx]);
+				} else {
+					const double* tb = a.tB64 + 3 * (size_t)a.bSlot[k];
+					v = d3(tb[0], tb[1], tb[2]);
+				}
+				lds[3 * (k - cb)] = v.x;
+				lds[3 * (k - cb) + 1] = v.y;
+				lds[3 * (k - cb) + 2] = v.z;
+			}
+			__syncthreads();
+			if (own) {
+				const uint32_t kb = max(rb, cb), ke = min(re, ce);
+				for (uint32_t k = kb; k < ke; ++k) acc = acc + d3(lds[3 * (k - cb)], lds[3 * (k - cb) + 1], lds[3 * (k - cb) + 2]);
+			}
+			__syncthreads();
+		}
+	}
+	if (!own) return;
+	const bool dyn = a.type[i] == kDynamic;
+	if (a.hasContacts && a.sphereOf[i] >= 0 && dyn) {
+		uint32_t order[16];
+		uint32_t sph = (uint32_t)a.sphereOf[i];
+		int cnt = contact_order(a.c, sph, order);
+		for (int c = 0; c < cnt; ++c) {
+			uint32_t k = a.c.pairStart[sph] + order[c];
+			if (a.c.hit[k]) fi = fi + d3(a.c.cF[k]);
+		}
+	}
+	const double h = a.h;
+	const double mm = dyn ? mass : 1.0;
+	// fe = f_ext + m g (force_solver.cpp:100-130) with the FP64 mass
+	D3 fe = d3(a.force[i]);
+	if (dyn) fe = fe + mass * d3(a.gravity[a.rowInst[i]]);
+	D3 b = mm * (d3(a.v0[i]) - vi) + h * (fe + fi);
+	b = b + hdw;
+	if (!dyn) b = d3(0.0, 0.0, 0.0);   // pb = S b
+	a.pb64[3 * (size_t)i] = b.x;
+	a.pb64[3 * (size_t)i + 1] = b.y;
+	a.pb64[3 * (size_t)i + 2] = b.z;
 }
 
 __global__ void __launch_bounds__(kBlock) k_assemble_rows(AsmArgs a) {
@@ -1156,12 +1390,33 @@ struct CGArgs {
 	CGLog* log;
 	uint32_t logStride;
 	uint32_t fused;             // FP64 mode: alpha / delta formed in the consuming kernel
+	// BF16 operator (precision form): off-diagonal blocks in BF16, diagonal blocks FP32 per row
+	uint32_t bf16;
+	const uint4* Ab;
+	const uint16_t* Ab8;
+	const float* diagF;
 };
 
 __device__ __forceinline__ V3 diag_apply(V3 im, V3 r) { return mdiag3(im.x, im.y, im.z) * r; }
 
 // One SpMV tile: products of the tile's blocks with the gathered vector go to
 // LDS, then each row thread adds its products in CSR (reference) order.
+__device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
+
+// Block k of the BF16 operator (entries in the column-major order of M33).
+__device__ __forceinline__ M33 ldAb(const CGArgs& a, uint32_t k) {
+	const uint4 w = a.Ab[k];
+	M33 m;
+	float* p = &m.x.x;
+	p[0] = bf16_lo(w.x); p[1] = bf16_hi(w.x);
+	p[2] = bf16_lo(w.y); p[3] = bf16_hi(w.y);
+	p[4] = bf16_lo(w.z); p[5] = bf16_hi(w.z);
+	p[6] = bf16_lo(w.w); p[7] = bf16_hi(w.w);
+	p[8] = __uint_as_float((uint32_t)a.Ab8[k] << 16);
+	return m;
+}
+
 template <class Gather>
 __device__ __forceinline__ V3 tile_rows(const CGArgs& a, uint32_t t, uint32_t& row, Gather gather, float* lds) {
 	const uint32_t r0 = a.tiles[t], r1 = a.tiles[t + 1];
@@ -1173,7 +1428,7 @@ __device__ __forceinline__ V3 tile_rows(const CGArgs& a, uint32_t t, uint32_t& r
 	for (uint32_t cb = b0; cb < b1; cb += kTileCap) {
 		const uint32_t ce = min(cb + (uint32_t)kTileCap, b1);
 		for (uint32_t k = cb + threadIdx.x; k < ce; k += kBlock) {
-			M33 blk = ldA(a.A, a.nnz, k);
+			M33 blk = a.bf16 ? ldAb(a, k) : ldA(a.A, a.nnz, k);
 			V3 p = blk * gather(a.cols[k]);
 			lds[3 * (k - cb)] = p.x;
 			lds[3 * (k - cb) + 1] = p.y;
@@ -1186,6 +1441,7 @@ __device__ __forceinline__ V3 tile_rows(const CGArgs& a, uint32_t t, uint32_t& r
 		}
 		__syncthreads();
 	}
+	if (a.bf16 && own) acc = acc + ldM(a.diagF, a.n, 0, row) * gather(row);   // FP32 diagonal block (zero in the BF16 planes)
 	return acc;
 }
 
@@ -1335,6 +1591,152 @@ __global__ void __launch_bounds__(kBlock) k_spmv(CGArgs a, const V3* xin, V3* yo
 	uint32_t row;
 	V3 y = tile_rows(a, t, row, [&](uint32_t c) { return xin[c]; }, lds);
 	if (row < a.tiles[t + 1]) yout[row] = y;
+}
+
+// ---------------------------------------------------------------------------
+// Precision form: FP64 iterative refinement around the PCG
+// ---------------------------------------------------------------------------
+struct RefineArgs {
+	uint32_t n, nnz;
+	const uint32_t* tiles;
+	const uint32_t* tileInst;
+	const uint32_t* rowptr;
+	const uint32_t* cols;
+	const uint32_t* slotRow;
+	const uint32_t* rowInst;
+	const float* A;
+	const V3* pb;
+	const V3* invM;
+	double* x64;     // 3 per row
+	const double* pb64;   // 3 per row: the FP64 right-hand side
+	V3* px;
+	V3* dx;
+	V3* rhs;
+	V3* r;
+	V3* s;
+	double* partialR;
+	double* partial;
+	SolverState* st;
+	uint4* Ab;
+	uint16_t* Ab8;
+	float* diagF;
+};
+
+// Round-to-nearest-even FP32 -> BF16 (NaN stays NaN).
+__device__ __forceinline__ uint32_t bf16_bits(float f) {
+	uint32_t u = __float_as_uint(f);
+	if ((u & 0x7F800000u) == 0x7F800000u) return (u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u);
+	u += 0x7FFFu + ((u >> 16) & 1u);
+	return u >> 16;
+}
+
+// BF16 copy of pA's off-diagonal blocks (diagonal slots zero) and the FP32
+// diagonal blocks per row, once per linearisation.
+__global__ void __launch_bounds__(kBlock) k_split_bf16(RefineArgs a) {
+	uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+	if (k >= a.nnz) return;
+	const uint32_t row = a.slotRow[k];
+	if (!a.st[a.rowInst[row]].newtonActive) return;
+	M33 m = ldA(a.A, a.nnz, k);
+	const float* p = &m.x.x;
+	if (a.cols[k] == row) {
+		for (int c = 0; c < 9; ++c) a.diagF[(size_t)c * a.n + row] = p[c];
+		a.Ab[k] = make_uint4(0u, 0u, 0u, 0u);
+		a.Ab8[k] = 0;
+		return;
+	}
+	uint32_t h[9];
+	for (int c = 0; c < 9; ++c) h[c] = bf16_bits(p[c]);
+	a.Ab[k] = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
+	a.Ab8[k] = (uint16_t)h[8];
+}
+
+// r = pb - pA x in FP64 (FP32 blocks promoted, FP64 x and sums, the
+// reference's row order), partials of |r|^2; then the sweep's inner system:
+// rhs = (float) r, dx = 0, and the PCG start r_in = rhs, s_in = invM rhs with
+// partials of r_in.s_in (the inner solve starts from dx = 0, so its first
+// residual needs no product).
+__global__ void __launch_bounds__(kBlock) k_resid64(RefineArgs a, int first) {
+	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+	const uint32_t inst = a.tileInst[t];
+	const SolverState& st = a.st[inst];
+	if (!st.newtonActive || (!first && !st.refineActive)) return;
+	__shared__ double lds[3 * (kTileCap / 2)];
+	const uint32_t r0 = a.tiles[t], r1 = a.tiles[t + 1];
+	const uint32_t b0 = a.rowptr[r0], b1 = a.rowptr[r1];
+	const uint32_t row = r0 + threadIdx.x;
+	const bool own = row < r1;
+	const uint32_t rb = own ? a.rowptr[row] : 0, re = own ? a.rowptr[row + 1] : 0;
+	double ax = 0.0, ay = 0.0, az = 0.0;
+	constexpr uint32_t cap = kTileCap / 2;
+	for (uint32_t cb = b0; cb < b1; cb += cap) {
+		const uint32_t ce = min(cb + cap, b1);
+		for (uint32_t k = cb + threadIdx.x; k < ce; k += kBlock) {
+			M33 m = ldA(a.A, a.nnz, k);
+			const double* xv = a.x64 + 3 * (size_t)a.cols[k];
+			const double x0 = xv[0], x1 = xv[1], x2 = xv[2];
+			lds[3 * (k - cb)] = (double)m.x.x * x0 + (double)m.y.x * x1 + (double)m.z.x * x2;
+			lds[3 * (k - cb) + 1] = (double)m.x.y * x0 + (double)m.y.y * x1 + (double)m.z.y * x2;
+			lds[3 * (k - cb) + 2] = (double)m.x.z * x0 + (double)m.y.z * x1 + (double)m.z.z * x2;
+		}
+		__syncthreads();
+		if (own) {
+			const uint32_t kb = max(rb, cb), ke = min(re, ce);
+			for (uint32_t k = kb; k < ke; ++k) {
+				ax += lds[3 * (k - cb)];
+				ay += lds[3 * (k - cb) + 1];
+				az += lds[3 * (k - cb) + 2];
+			}
+		}
+		__syncthreads();
+	}
+	__shared__ double sh[kBlock / 64];
+	double rr = 0.0, rs = 0.0;
+	if (own) {
+		const double* b = a.pb64 + 3 * (size_t)row;
+		const double rx = b[0] - ax, ry = b[1] - ay, rz = b[2] - az;
+		rr = rx * rx + ry * ry + rz * rz;
+		const V3 rf = v3((float)rx, (float)ry, (float)rz);
+		const V3 sf = diag_apply(a.invM[row], rf);
+		a.rhs[row] = rf;
+		a.r[row] = rf;
+		a.s[row] = sf;
+		a.dx[row] = zero3();
+		rs = (double)dot(rf, sf);
+	}
+	const double brr = block_sum(rr, sh);
+	if (threadIdx.x == 0) a.partialR[t] = brr;
+	const double brs = block_sum(rs, sh);
+	if (threadIdx.x == 0) a.partial[t] = brs;
+}
+
+// x64 = px (the warm start) at the start of a Newton iteration's solve.
+__global__ void __launch_bounds__(kBlock) k_x64_init(RefineArgs a) {
+	uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= a.n || !a.st[a.rowInst[i]].newtonActive) return;
+	const V3 x = a.px[i];
+	a.x64[3 * (size_t)i] = x.x;
+	a.x64[3 * (size_t)i + 1] = x.y;
+	a.x64[3 * (size_t)i + 2] = x.z;
+}
+
+// x64 += dx after a sweep that solved.
+__global__ void __launch_bounds__(kBlock) k_x64_add(RefineArgs a) {
+	uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= a.n) return;
+	const SolverState& st = a.st[a.rowInst[i]];
+	if (!st.newtonActive || !st.refineActive) return;
+	const V3 d = a.dx[i];
+	a.x64[3 * (size_t)i] += (double)d.x;
+	a.x64[3 * (size_t)i + 1] += (double)d.y;
+	a.x64[3 * (size_t)i + 2] += (double)d.z;
+}
+
+// px = (float) x64: the Newton update's dv.
+__global__ void __launch_bounds__(kBlock) k_x64_finish(RefineArgs a) {
+	uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= a.n || !a.st[a.rowInst[i]].newtonActive) return;
+	a.px[i] = v3((float)a.x64[3 * (size_t)i], (float)a.x64[3 * (size_t)i + 1], (float)a.x64[3 * (size_t)i + 2]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1558,6 +1960,10 @@ static CGArgs cg_args(Body& b) {
 	a.log = dptr<CGLog>(b.cgLog);
 	a.logStride = b.logStride;
 	a.fused = b.reduction == B3G_REDUCE_FP64_FUSED ? 1u : 0u;
+	a.bf16 = b.solver == B3G_SOLVER_REFINED_BF16 && b.Ab.p ? 1u : 0u;
+	a.Ab = dptr<uint4>(b.Ab);
+	a.Ab8 = dptr<uint16_t>(b.Ab8);
+	a.diagF = dptr<float>(b.diagF);
 	return a;
 }
 
@@ -1584,10 +1990,12 @@ static b3g_status enqueue_linearise(Body& b, float h) {
 	V3* fs = dptr<V3>(b.fstage);
 	StepArgs sa = step_args(b, h);
 	LAUNCH(k_newton_prep, nblk(b.n), sa);
+	const bool precise = b.solver != B3G_SOLVER_PCG && b.pb64.p;
 	if (b.tet.m) {
 		TetArgs t{b.tet.m, dptr<uint32_t>(b.tet.p), dptr<float>(b.tet.rest), dptr<float>(b.tet.invE), dptr<float>(b.tet.K0),
 		          dptr<float4>(b.tet.q), dptr<float>(b.tet.kd), dptr<V3>(b.x), dptr<V3>(b.v), fs, b.tet.f, b.tet.fd, st, ri};
-		LAUNCH(k_tet_forces, nblk(b.tet.m), t);
+		if (precise) LAUNCH(k_tet_forces64, nblk(b.tet.m), t, dptr<double>(b.tet64), dptr<D3>(b.fstage64));
+		else LAUNCH(k_tet_forces, nblk(b.tet.m), t);
 	}
 	if (b.tri.m) {
 		TriArgs t{b.tri.m, dptr<uint32_t>(b.tri.p), dptr<float>(b.tri.x2), dptr<float>(b.tri.invS), dptr<float>(b.tri.K0),
@@ -1635,8 +2043,19 @@ static b3g_status enqueue_linearise(Body& b, float h) {
 	a.A = dptr<float>(b.A); a.tB = dptr<V3>(b.tB); a.pb = dptr<V3>(b.pb); a.invM = dptr<V3>(b.invM); a.fi = dptr<V3>(b.fi);
 	a.c = c;
 	a.st = st;
+	a.y = dptr<V3>(b.y);
+	a.tB64 = precise ? dptr<double>(b.tB64) : nullptr;
+	a.fstage64 = dptr<D3>(b.fstage64);
+	a.tetF0 = b.tet.f; a.tetF1 = b.tet.f + 4 * b.tet.m;
+	a.tetFd0 = b.tet.anyDamping ? b.tet.fd : 0; a.tetFd1 = b.tet.anyDamping ? b.tet.fd + 4 * b.tet.m : 0;
+	a.pb64 = dptr<double>(b.pb64);
+	a.mass64 = dptr<double>(b.mass64);
+	a.force = dptr<V3>(b.force);
+	a.gravity = dptr<V3>(b.gravity);
+	a.rowInst = dptr<uint32_t>(b.rowInst);
 	LAUNCH(k_assemble_slots, b.nSlotTiles, a);
 	LAUNCH(k_assemble_rows, b.nTiles, a);
+	if (precise) LAUNCH(k_assemble_rows64, b.nTiles, a);
 	return B3G_OK;
 }
 
@@ -1659,18 +2078,13 @@ static b3g_status enqueue_finalize(Body& b, const DBuf& partial, int sel, int ki
 // blocks) otherwise.
 static inline int delta_range(uint32_t k) { return k % 50 == 0 ? kRangeTiles : kRangeBlocks; }
 
-// Jacobi PCG on the assembled pA / pb with the warm start px: the reference's
-// fixed launch sequence of maxIt iterations; iterations after convergence
-// return at once on their log entry.
-b3g_status enqueue_solve_cg(Body& b, uint32_t maxIt, float eps) {
-	(void)eps;
-	b3g_status s = ensure_cg_log(b, maxIt + 1);
-	if (s != B3G_OK) return s;
-	CGArgs a = cg_args(b);
+// Iterations 0 .. maxIt - 1 of the PCG (sparse_solver.cpp:75-103) after its
+// Init reduction, then the reduction that ends the last one: the reference's
+// fixed launch sequence; iterations after convergence return at once on their
+// log entry.
+static b3g_status enqueue_pcg_iterations(Body& b, const CGArgs& a, uint32_t maxIt) {
 	const bool fused = a.fused != 0;
 	const uint32_t rb = b.nRowBlk;
-	LAUNCH(k_cg_residual, b.nTiles, a, (int)kFinInit, 0u);
-	FINALIZE(b.partial, kRangeTiles, kFinInit, 0, kGateNewton, 0);
 	for (uint32_t k = 0; k < maxIt; ++k) {
 		CGArgs ak = a;
 		V3* d0 = dptr<V3>(b.d0);
@@ -1688,8 +2102,104 @@ b3g_status enqueue_solve_cg(Body& b, uint32_t maxIt, float eps) {
 			LAUNCH(k_cg_update, rb, ak, k);
 		}
 	}
-	// the reduction after the last iteration: delta, iteration count
 	if (maxIt > 0) FINALIZE(b.partial, delta_range(maxIt - 1), kFinDelta, maxIt, kGateIter, maxIt - 1);
+	return B3G_OK;
+}
+
+// Jacobi PCG on the assembled pA / pb with the warm start px.
+b3g_status enqueue_solve_cg(Body& b, uint32_t maxIt, float eps) {
+	(void)eps;
+	b3g_status s = ensure_cg_log(b, maxIt + 1);
+	if (s != B3G_OK) return s;
+	CGArgs a = cg_args(b);
+	LAUNCH(k_cg_residual, b.nTiles, a, (int)kFinInit, 0u);
+	FINALIZE(b.partial, kRangeTiles, kFinInit, 0, kGateNewton, 0);
+	return enqueue_pcg_iterations(b, a, maxIt);
+}
+
+static RefineArgs refine_args(Body& b) {
+	RefineArgs r;
+	r.n = b.n;
+	r.nnz = b.nnz;
+	r.tiles = dptr<uint32_t>(b.tiles);
+	r.tileInst = dptr<uint32_t>(b.tileInst);
+	r.rowptr = dptr<uint32_t>(b.rowptr);
+	r.cols = dptr<uint32_t>(b.cols);
+	r.slotRow = dptr<uint32_t>(b.slotRow);
+	r.rowInst = dptr<uint32_t>(b.rowInst);
+	r.A = dptr<float>(b.A);
+	r.pb = dptr<V3>(b.pb);
+	r.invM = dptr<V3>(b.invM);
+	r.x64 = dptr<double>(b.x64);
+	r.pb64 = dptr<double>(b.pb64);
+	r.px = dptr<V3>(b.px);
+	r.dx = dptr<V3>(b.dx);
+	r.rhs = dptr<V3>(b.rhsF);
+	r.r = dptr<V3>(b.r);
+	r.s = dptr<V3>(b.s);
+	r.partialR = dptr<double>(b.partialR);
+	r.partial = dptr<double>(b.partial);
+	r.st = dptr<SolverState>(b.state);
+	r.Ab = dptr<uint4>(b.Ab);
+	r.Ab8 = dptr<uint16_t>(b.Ab8);
+	r.diagF = dptr<float>(b.diagF);
+	return r;
+}
+
+// Buffers of the precision form (allocated on first use).
+static b3g_status ensure_refine(Body& b) {
+	b3g_status s;
+	auto need = [](const DBuf& d, size_t bytes) { return !d.p || d.bytes < bytes; };
+	if (need(b.x64, (size_t)b.n * 24) && (s = dalloc(b, b.x64, (size_t)b.n * 24)) != B3G_OK) return s;
+	if (need(b.dx, (size_t)b.n * sizeof(V3)) && (s = dalloc(b, b.dx, (size_t)b.n * sizeof(V3))) != B3G_OK) return s;
+	if (need(b.rhsF, (size_t)b.n * sizeof(V3)) && (s = dalloc(b, b.rhsF, (size_t)b.n * sizeof(V3))) != B3G_OK) return s;
+	if (need(b.partialR, (size_t)b.partialCount * 8) && (s = dalloc(b, b.partialR, (size_t)b.partialCount * 8)) != B3G_OK)
+		return s;
+	if (b.solver == B3G_SOLVER_REFINED_BF16) {
+		if (need(b.Ab, (size_t)b.nnz * 16) && (s = dalloc(b, b.Ab, (size_t)b.nnz * 16)) != B3G_OK) return s;
+		if (need(b.Ab8, (size_t)b.nnz * 2) && (s = dalloc(b, b.Ab8, (size_t)b.nnz * 2)) != B3G_OK) return s;
+		if (need(b.diagF, (size_t)b.n * 36) && (s = dalloc(b, b.diagF, (size_t)b.n * 36)) != B3G_OK) return s;
+	}
+	// FP64 right-hand side: tetrahedron forces and (h dfdx) w in double
+	if (need(b.pb64, (size_t)b.n * 24) && (s = dalloc(b, b.pb64, (size_t)b.n * 24)) != B3G_OK) return s;
+	if (need(b.tB64, (size_t)b.nnz * 24) && (s = dalloc(b, b.tB64, (size_t)b.nnz * 24)) != B3G_OK) return s;
+	if (need(b.fstage64, (size_t)std::max(1u, b.nFStage) * sizeof(D3)) &&
+	    (s = dalloc(b, b.fstage64, (size_t)std::max(1u, b.nFStage) * sizeof(D3))) != B3G_OK)
+		return s;
+	if (b.tet.m && !b.tet64.p) {
+		if ((s = dalloc(b, b.tet64, (size_t)b.tet.m * 12 * sizeof(double))) != B3G_OK) return s;
+		LAUNCH(k_tet_setup64, nblk(b.tet.m), b.tet.m, dptr<float>(b.tet.rest), dptr<float>(b.tet.param),
+		       dptr<double>(b.tet64));
+	}
+	if (!b.mass64.p) {
+		if ((s = dupload(b, b.mass64, b.hostMass64.data(), b.hostMass64.size() * sizeof(double))) != B3G_OK) return s;
+	}
+	return B3G_OK;
+}
+
+// Precision form of one Newton iteration's solve: sweeps of an FP64 residual
+// and an inner PCG of at most innerIt iterations on pA dx = residual (FP32 or
+// BF16-off-diagonal blocks), x += dx in FP64; then px = x.
+static b3g_status enqueue_solve_refined(Body& b, uint32_t sweeps) {
+	b3g_status s = ensure_cg_log(b, b.innerIt + 1);
+	if (s != B3G_OK) return s;
+	RefineArgs ra = refine_args(b);
+	CGArgs a = cg_args(b);
+	a.pb = dptr<V3>(b.rhsF);
+	a.px = dptr<V3>(b.dx);
+	if (b.solver == B3G_SOLVER_REFINED_BF16) LAUNCH(k_split_bf16, nblk(b.nnz), ra);
+	LAUNCH(k_x64_init, nblk(b.n), ra);
+	for (uint32_t sw = 0; sw < sweeps; ++sw) {
+		LAUNCH(k_resid64, b.nTiles, ra, sw == 0 ? 1 : 0);
+		FINALIZE(b.partialR, kRangeTiles, sw == 0 ? kFinRefine0 : kFinRefine, 0, kGateNewton, 0);
+		FINALIZE(b.partial, kRangeTiles, kFinInit, 0, kGateNewton, 0);
+		s = enqueue_pcg_iterations(b, a, b.innerIt);
+		if (s != B3G_OK) return s;
+		LAUNCH(k_x64_add, nblk(b.n), ra);
+	}
+	LAUNCH(k_resid64, b.nTiles, ra, 0);   // the residual after the last sweep (skipped once converged)
+	FINALIZE(b.partialR, kRangeTiles, kFinRefine, 0, kGateNewton, 0);
+	LAUNCH(k_x64_finish, nblk(b.n), ra);
 	return B3G_OK;
 }
 
@@ -1740,13 +2250,21 @@ b3g_status enqueue_step(Body& b, float dt, uint32_t iters, uint32_t subIters, fl
 	if (dt > 0.0f) {
 		StepArgs sa = step_args(b, dt);
 		LAUNCH(k_step_begin, nblk(b.n), sa);
-		LAUNCH(k_state_begin, nblk(b.nInst), st, b.nInst, assembleOnly ? 1u : iters, subIters, tol, subTol,
-		       b.reduction == B3G_REDUCE_SEQUENTIAL ? 1u : 0u);
+		const bool refined = b.solver != B3G_SOLVER_PCG && !assembleOnly;
+		const uint32_t sweeps = refined ? std::max(1u, (subIters + b.innerIt - 1) / std::max(1u, b.innerIt)) : 0u;
+		if (refined) {
+			s = ensure_refine(b);
+			if (s != B3G_OK) return s;
+		}
+		// refined: the PCG runs innerIt iterations at innerTol per sweep, subTol is the sweeps' target
+		LAUNCH(k_state_begin, nblk(b.nInst), st, b.nInst, assembleOnly ? 1u : iters, refined ? b.innerIt : subIters, tol,
+		       refined ? b.innerTol : subTol, b.reduction == B3G_REDUCE_SEQUENTIAL ? 1u : 0u, refined ? 1u : 0u, sweeps,
+		       subTol);
 		for (uint32_t it = 0; it < (assembleOnly ? 1u : iters); ++it) {
 			s = enqueue_linearise(b, dt);
 			if (s != B3G_OK) return s;
 			if (assembleOnly) return B3G_OK;
-			s = enqueue_solve_cg(b, subIters, subTol);
+			s = refined ? enqueue_solve_refined(b, sweeps) : enqueue_solve_cg(b, subIters, subTol);
 			if (s != B3G_OK) return s;
 			LAUNCH(k_newton_update, b.nRowBlk, sa, dptr<double>(b.partial), dptr<float>(b.terms));
 			FINALIZE(b.partial, kRangeBlocks, kFinNewton, 0, kGateNewton, 0);

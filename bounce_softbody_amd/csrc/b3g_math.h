@@ -157,6 +157,57 @@ struct Xf { V3 p; Q4 q; };
 B3G_HD V3 xmul(const Xf& X, V3 v) { return qrotate(X.q, v) + X.p; }
 B3G_HD V3 xmulT(const Xf& X, V3 v) { return qrotate(qconj(X.q), v - X.p); }
 
+// FP64 counterparts for the precision form's tetrahedron forces: the same
+// expression trees in double (the reference with its scalar type set to
+// double; normalisation threshold DBL_EPSILON).
+constexpr double kEps64 = 2.220446049250313e-16;
+struct D3 { double x, y, z; };
+B3G_HD D3 d3(double x, double y, double z) { D3 r; r.x = x; r.y = y; r.z = z; return r; }
+B3G_HD D3 d3(V3 v) { return d3(v.x, v.y, v.z); }
+B3G_HD D3 operator+(D3 a, D3 b) { return d3(a.x + b.x, a.y + b.y, a.z + b.z); }
+B3G_HD D3 operator-(D3 a, D3 b) { return d3(a.x - b.x, a.y - b.y, a.z - b.z); }
+B3G_HD D3 operator-(D3 a) { return d3(-a.x, -a.y, -a.z); }
+B3G_HD D3 operator*(double s, D3 v) { return d3(s * v.x, s * v.y, s * v.z); }
+B3G_HD D3 operator/(D3 v, double s) { return d3(v.x / s, v.y / s, v.z / s); }
+B3G_HD double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+B3G_HD D3 cross(D3 a, D3 b) { return d3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+B3G_HD double det(D3 a, D3 b, D3 c) { return dot(a, cross(b, c)); }
+B3G_HD double length(D3 v) { return sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
+struct DM33 { D3 x, y, z; };
+B3G_HD DM33 dm33(D3 a, D3 b, D3 c) { DM33 r; r.x = a; r.y = b; r.z = c; return r; }
+B3G_HD DM33 dm33(const M33& A) { return dm33(d3(A.x), d3(A.y), d3(A.z)); }
+B3G_HD D3 operator*(const DM33& A, D3 v) { return v.x * A.x + v.y * A.y + v.z * A.z; }
+B3G_HD DM33 operator*(const DM33& A, const DM33& B) { return dm33(A * B.x, A * B.y, A * B.z); }
+B3G_HD DM33 operator*(double s, const DM33& A) { return dm33(s * A.x, s * A.y, s * A.z); }
+B3G_HD DM33 transpose(const DM33& A) {
+	return dm33(d3(A.x.x, A.y.x, A.z.x), d3(A.x.y, A.y.y, A.z.y), d3(A.x.z, A.y.z, A.z.z));
+}
+B3G_HD DM33 adjugate(const DM33& A) { return transpose(dm33(cross(A.y, A.z), cross(A.z, A.x), cross(A.x, A.y))); }
+struct DQ4 { D3 v; double s; };
+B3G_HD void qnormalize(DQ4& q) {
+	double len = sqrt(dot(q.v, q.v) + q.s * q.s);
+	if (len > kEps64) {
+		double s = 1.0 / len;
+		q.v.x *= s; q.v.y *= s; q.v.z *= s;
+		q.s /= len;
+	}
+}
+B3G_HD DQ4 qmul(const DQ4& a, const DQ4& b) {
+	DQ4 r;
+	r.v = cross(a.v, b.v) + a.s * b.v + b.s * a.v;
+	r.s = a.s * b.s - dot(a.v, b.v);
+	return r;
+}
+B3G_HD DM33 qmatrix(const DQ4& q) {
+	double x = q.v.x, y = q.v.y, z = q.v.z, w = q.s;
+	double x2 = x + x, y2 = y + y, z2 = z + z;
+	double xx = x * x2, xy = x * y2, xz = x * z2;
+	double yy = y * y2, yz = y * z2, zz = z * z2;
+	double wx = w * x2, wy = w * y2, wz = w * z2;
+	return dm33(d3(1.0 - (yy + zz), xy + wz, xz - wy), d3(xy - wz, 1.0 - (xx + zz), yz + wx),
+	            d3(xz + wy, yz - wx, 1.0 - (xx + yy)));
+}
+
 struct AABB { V3 lo, hi; };
 B3G_HD AABB aabb_cr(V3 c, float r) { V3 rv = v3(r, r, r); AABB a; a.lo = c - rv; a.hi = c + rv; return a; }
 B3G_HD AABB aabb_cr(V3 c, V3 r) { AABB a; a.lo = c - r; a.hi = c + r; return a; }

@@ -655,8 +655,8 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 	}
 	// mass lumping: body.cpp:394-466 over the deduplicated fixtures in list order
 	{
-		struct TriF { uint32_t p[3]; float area, density; };
-		struct TetF { uint32_t p[4]; float volume, density; };
+		struct TriF { uint32_t p[3]; float area, density; double area64; };
+		struct TetF { uint32_t p[4]; float volume, density; double volume64; };
 		std::vector<TriF> tris;
 		std::vector<TetF> tets;
 		std::vector<std::array<uint32_t, 4>> seen;
@@ -679,6 +679,7 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 				const float* v = d.tf_v + 9 * (size_t)c;
 				V3 A = load3(v), B = load3(v + 3), C = load3(v + 6);
 				t.area = 0.5f * length(cross(B - A, C - A));   // triangle_fixture.cpp:35-37
+				t.area64 = 0.5 * length(cross(d3(B) - d3(A), d3(C) - d3(A)));
 				t.density = d.tf_density[c];
 				tris.push_back(t);
 			}
@@ -704,6 +705,7 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 				float dt = det(B - A, C - A, D - A);
 				float sg = sign_(dt);
 				t.volume = inv6 * sg * dt;                    // tetrahedron_fixture.cpp:41-45
+				t.volume64 = fabs(det(d3(B) - d3(A), d3(C) - d3(A), d3(D) - d3(A))) / 6.0;
 				t.density = d.ttf_density[c];
 				tets.push_back(t);
 			}
@@ -713,11 +715,20 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 		const float inv3 = 1.0f / 3.0f, inv4 = 1.0f / 4.0f;
 		for (auto& t : tris) { float m = t.density * t.area; for (int k = 0; k < 3; ++k) h.mass[t.p[k]] += inv3 * m; }
 		for (auto& t : tets) { float m = t.density * t.volume; for (int k = 0; k < 4; ++k) h.mass[t.p[k]] += inv4 * m; }
+		// the same lumping in double (the precision form's right-hand side)
+		std::vector<double> m64(n);
+		for (uint32_t i = 0; i < n; ++i) m64[i] = h.type[i] == kDynamic ? 1.0 : 0.0;
+		for (auto& t : tris) for (int k = 0; k < 3; ++k) m64[t.p[k]] = 0.0;
+		for (auto& t : tets) for (int k = 0; k < 4; ++k) m64[t.p[k]] = 0.0;
+		for (auto& t : tris) { double m = (double)t.density * t.area64; for (int k = 0; k < 3; ++k) m64[t.p[k]] += m / 3.0; }
+		for (auto& t : tets) { double m = (double)t.density * t.volume64; for (int k = 0; k < 4; ++k) m64[t.p[k]] += m / 4.0; }
 		for (uint32_t i = 0; i < n; ++i) {
-			if (h.type[i] != kDynamic) { h.mass[i] = 0.0f; h.invMass[i] = 0.0f; continue; }
+			if (h.type[i] != kDynamic) { h.mass[i] = 0.0f; h.invMass[i] = 0.0f; m64[i] = 0.0; continue; }
 			if (h.mass[i] > 0.0f) h.invMass[i] = 1.0f / h.mass[i];
 			else { h.mass[i] = 1.0f; h.invMass[i] = 1.0f; }
+			if (!(m64[i] > 0.0)) m64[i] = 1.0;
 		}
+		b.hostMass64 = std::move(m64);
 	}
 	b.hostMass = h.mass;
 	b.hostType = h.type;

@@ -143,6 +143,8 @@ typedef struct b3g_step_stats {
 	uint32_t min_sub_iterations;  /* b3_forceSolverMinSubIterations (running minimum) */
 	uint32_t max_sub_iterations;  /* b3_forceSolverMaxSubIterations (running maximum) */
 	uint32_t contact_count;       /* live sphere-shape contacts (b3ContactManager::m_contactCount) */
+	uint32_t refinement_sweeps;   /* precision form: FP64 residual sweeps of the last Newton iteration */
+	float relative_residual;      /* precision form: last FP64 ||pb - pA x|| / ||pb - pA x0|| */
 } b3g_step_stats;
 
 const char* b3g_status_string(b3g_status s);
@@ -262,6 +264,23 @@ b3g_status b3g_set_reduction(b3g_body body, int mode);
  */
 b3g_status b3g_save_state(b3g_body body, void* blob, uint64_t* bytes);
 b3g_status b3g_load_state(b3g_body body, const void* blob, uint64_t bytes);
+
+/*
+ * Precision form of the linear solve (no counterpart in the reference).
+ *   B3G_SOLVER_PCG (default): the reference's Jacobi PCG on the FP32 pA
+ *   (sparse_solver.cpp:28-109).
+ *   B3G_SOLVER_REFINED_FP32 / B3G_SOLVER_REFINED_BF16: iterative refinement.
+ *   Each sweep forms the residual pb - pA x in FP64 (FP32 blocks, FP64 x and
+ *   sums), then runs the reference's PCG for at most `inner_iterations` at
+ *   relative tolerance `inner_tolerance` on pA dx = residual, with pA's
+ *   off-diagonal blocks in FP32 or in BF16 (FP32 accumulation; diagonal
+ *   blocks FP32), and adds dx to x in FP64. In these modes b3g_step's
+ *   sub_tolerance is the target of ||residual|| relative to the first sweep's
+ *   and sub_iterations caps the inner iterations of a Newton iteration
+ *   (sweeps = ceil(sub_iterations / inner_iterations)).
+ */
+enum { B3G_SOLVER_PCG = 0, B3G_SOLVER_REFINED_FP32 = 1, B3G_SOLVER_REFINED_BF16 = 2 };
+b3g_status b3g_set_solver(b3g_body body, int mode, uint32_t inner_iterations, float inner_tolerance);
 
 /* Waits for all work queued on the body. */
 b3g_status b3g_synchronize(b3g_body body);

@@ -139,6 +139,9 @@ def _lib(which: str):
         lib[p + "assemble"].argtypes = [C.c_void_p, C.c_float]
         lib[p + "assemble"].restype = C.c_uint32
         lib[p + "get_assembled"].argtypes = [C.c_void_p, u32p, u32p, f32p, f32p, f32p]
+        if which == "oracle":
+            f64p = C.POINTER(C.c_double)
+            lib.b3o_get_assembled64.argtypes = [C.c_void_p, u32p, u32p, f64p, f64p, f64p]
         lib[p + "uv_initialize"].argtypes = [f32p, f32p, f32p, f32p]
         lib[p + "solve_cg"].argtypes = [C.c_uint32, u32p, u32p, f32p, f32p, f32p, C.c_uint32, C.c_float]
         lib[p + "solve_cg"].restype = C.c_uint32
@@ -238,6 +241,19 @@ class _Body:
 
 class OracleBody(_Body):
     """The CPU restatement (``precision`` 32 or 64)."""
+
+    def assemble64(self, dt):
+        """Like :meth:`assemble` (list-order CSR, column-major blocks) without rounding to float."""
+        nnz = self._f("assemble")(self.h, C.c_float(dt))
+        rowptr = np.zeros(self.n + 1, np.uint32)
+        cols = np.zeros(max(1, nnz), np.uint32)
+        vals = np.zeros(9 * max(1, nnz), np.float64)
+        pb = np.zeros(3 * self.n, np.float64)
+        fi = np.zeros(3 * self.n, np.float64)
+        d = C.POINTER(C.c_double)
+        self.lib.b3o_get_assembled64(self.h, _ptr(rowptr, u32p), _ptr(cols, u32p), vals.ctypes.data_as(d),
+                                     pb.ctypes.data_as(d), fi.ctypes.data_as(d))
+        return rowptr, cols[:nnz], vals[: 9 * nnz].reshape(-1, 9), pb.reshape(-1, 3), fi.reshape(-1, 3)
 
     prefix = "b3o_"
     which = "oracle"

@@ -223,6 +223,31 @@ void b3o_get_assembled(void* p, uint32_t* rowptr, uint32_t* cols, float* vals, f
 	else get(h->capA, h->capB, h->capF);
 }
 
+/* The same in double (FP64 oracle: the assembled system without rounding to float). */
+void b3o_get_assembled64(void* p, uint32_t* rowptr, uint32_t* cols, double* vals, double* pb, double* fi) {
+	Handle* h = (Handle*)p;
+	auto get = [&](auto& A, auto& b, auto& f) {
+		uint32_t k = 0;
+		rowptr[0] = 0;
+		for (uint32_t i = 0; i < A.n(); ++i) {
+			const auto& r = A.rows[i];
+			for (size_t e = r.size(); e-- > 0;) {
+				cols[k] = r[e].col;
+				const auto* m = &r[e].val.x.x;
+				for (int t = 0; t < 9; ++t) vals[9 * k + t] = double(m[t]);
+				++k;
+			}
+			rowptr[i + 1] = k;
+			for (int t = 0; t < 3; ++t) {
+				pb[3 * i + t] = double(b[i][t]);
+				fi[3 * i + t] = double(f[i][t]);
+			}
+		}
+	};
+	if (h->precision == 64) get(h->capA64, h->capB64, h->capF64);
+	else get(h->capA, h->capB, h->capF);
+}
+
 /* Restated b3StretchForceDef::Initialize (stretch_force.cpp:36-60) for test comparison. */
 void b3o_uv_initialize(const float* A, const float* B, const float* C, float* out6) {
 	UVInitialize<float>(MakeV3(A[0], A[1], A[2]), MakeV3(B[0], B[1], B[2]), MakeV3(C[0], C[1], C[2]), out6);
