@@ -69,6 +69,97 @@ def cpu_baseline(tet_arr, cloth_arr, dt, sub, sub_tol, sample_steps):
     }
 
 
+def run_precision(args):
+    """Precision form on the stiffness-contrast workload: a tet grid (args.tet cells per axis) whose elements
+    have Young's moduli log-uniform in [1e4, 1e7] Pa. Every form steps from the same deformed, moving state
+    (after args.pre_steps plain steps); its error is the relative max-norm of one step's velocities against
+    the FP64 oracle (the reference restated in double, PCG tolerance 1e-10) from that state."""
+    from bounce_softbody_amd import _lib, configs
+    from bounce_softbody_amd.body import Body
+
+    dt = configs.DT
+    n = args.tet
+    arr = configs.tet_grid(n, young_per_element=configs.log_uniform_young(n, seed=7)).arrays()
+    body = Body(arr)
+    for _ in range(args.pre_steps):
+        body.Step(dt, 1, 200, 0.0, 1e-6)
+    body.synchronize()
+    start = body.save_state()
+    x0, v0 = body.state()
+    v64 = None
+    if not args.no_cpu_baseline:
+        import oracle
+        t0 = time.time()
+        o = oracle.OracleBody(arr, precision=64)
+        o.set_state(x0, v0)
+        o.step(dt, 1, 20000, 0.0, 1e-10)
+        v64 = o.state64()[1]
+        del o
+        log(f"FP64 oracle step in {time.time() - t0:.1f} s")
+    forms = [
+        ("plain_fp32", _lib.SOLVER_PCG, (), args.sub_iterations, args.sub_tolerance),
+        ("plain_fp32_tight", _lib.SOLVER_PCG, (), 1000, 1e-7),
+        ("refined_fp32", _lib.SOLVER_REFINED_FP32, (args.inner_iterations, args.inner_tolerance),
+         args.max_sweeps * args.inner_iterations, args.refine_tolerance),
+        ("refined_bf16", _lib.SOLVER_REFINED_BF16, (args.inner_iterations, args.inner_tolerance),
+         args.max_sweeps * args.inner_iterations, args.refine_tolerance),
+    ]
+    out = {}
+    for name, mode, inner, sub, tol in forms:
+        body.set_solver(mode, *inner)
+        body.load_state(start)
+        body.Step(dt, 1, sub, 0.0, tol)
+        _, v1 = body.state()
+        err = float(np.abs(v1 - v64).max() / np.abs(v64).max()) if v64 is not None else None
+        st = body.instance_stats()[0][0]
+        sweeps, res = body.refinement()[0]
+        body.load_state(start)
+        for _ in range(args.warmup):
+            body.Step(dt, 1, sub, 0.0, tol)
+        body.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            body.Step(dt, 1, sub, 0.0, tol)
+        body.synchronize()
+        el = time.perf_counter() - t0
+        mean_ms, by = body.time_spmv(args.spmv_launches)
+        out[name] = {"steps_per_s": round(args.steps / el, 3), "rel_err_vs_fp64": err,
+                     "inner_iterations_first_step": int(st[2]), "sweeps": int(sweeps),
+                     "fp64_relative_residual": float(res) if mode != _lib.SOLVER_PCG else None,
+                     "pcg": {"max_iterations": sub, "tolerance": tol, "inner": list(inner)},
+                     "spmv": {"mean_us": round(mean_ms * 1e3, 2), "bytes_per_launch": by,
+                              "gbs": round(by / (mean_ms * 1e-3) / 1e9, 1)}}
+        log(f"{name}: {out[name]}")
+    best_name = max(("refined_fp32", "refined_bf16"), key=lambda k: out[k]["steps_per_s"])
+    best = out[best_name]
+    line = {
+        "metric": "sim steps/s",
+        "value": best["steps_per_s"],
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 / best["steps_per_s"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner / f64 residual",
+        "data": "synthetic",
+        "config": {"workload": f"precision form: tet grid {n}^3 cells, E log-uniform in [1e4, 1e7] Pa",
+                   "dt": "1/60", "newton_iterations": 1, "form": best_name + " (value)"},
+        "forms": out,
+        "speedup_vs_plain_fp32_tight": round(best["steps_per_s"] / out["plain_fp32_tight"]["steps_per_s"], 3),
+        "note": "plain FP32 cannot reach 1e-5 here: its right-hand side loses ~1e-5 to cancellation in FP32",
+        "roofline": {"kernel": "k_cg_spmv_dq, " + ("BF16 off-diagonal" if best_name == "refined_bf16" else "FP32")
+                               + " operator (back-to-back pass)", "bound": "hbm",
+                     "achieved": best["spmv"]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(best["spmv"]["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": best["spmv"]["bytes_per_launch"], "mean_launch_us": best["spmv"]["mean_us"]},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,12 +174,23 @@ def main():
     ap.add_argument("--batch", action="store_true",
                     help="step the two bodies as one batch of two instances instead of two handles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["two_body", "precision"], default="two_body")
+    ap.add_argument("--pre-steps", type=int, default=10, help="precision workload: plain steps before the forms")
+    ap.add_argument("--inner-iterations", type=int, default=50, help="precision workload: PCG iterations per sweep")
+    ap.add_argument("--inner-tolerance", type=float, default=1e-3, help="precision workload: PCG tolerance per sweep")
+    ap.add_argument("--max-sweeps", type=int, default=4, help="precision workload: refinement sweeps at most")
+    ap.add_argument("--refine-tolerance", type=float, default=1e-4,
+                    help="precision workload: FP64 residual target relative to the first sweep's")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this workload is a single body per GPU and the multi-GPU path is not "
               "part of this build (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "precision":
+        run_precision(args)
+        return
 
     from bounce_softbody_amd import configs
     from bounce_softbody_amd.body import Body

@@ -451,7 +451,10 @@ b3g_status b3g_time_spmv(b3g_body h, uint32_t launches, double* mean_ms, double*
 	Body& b = h->body;
 	b3g_status s = time_pcg_spmv(b, launches, mean_ms);
 	if (s != B3G_OK) return s;
-	if (bytes_per_launch) *bytes_per_launch = 40.0 * b.nnz + 4.0 * (b.n + 1) + 24.0 * b.n;
+	// FP32 blocks: 36 + 4 (column) per block; BF16 operator: 18 + 4 per block and a 36-byte FP32 diagonal per row
+	const bool bf16 = b.solver == B3G_SOLVER_REFINED_BF16 && b.Ab.p;
+	if (bytes_per_launch)
+		*bytes_per_launch = (bf16 ? 22.0 * b.nnz + 36.0 * b.n : 40.0 * b.nnz) + 4.0 * (b.n + 1) + 24.0 * b.n;
 	return B3G_OK;
 }
 

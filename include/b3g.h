@@ -292,7 +292,9 @@ b3g_status b3g_synchronize(b3g_body body);
  * scratch vectors -- between one HIP event pair on the body's stream.
  * *mean_ms = elapsed / launches; *bytes_per_launch = the algorithmic bytes of
  * one launch (40 per 3x3 block + 4 per row offset + 24 per row for the vector
- * in/out). Leaves positions, velocities and the solver state unchanged.
+ * in/out; with the BF16 operator of B3G_SOLVER_REFINED_BF16, 22 per block +
+ * 36 per row for the FP32 diagonal instead of 40 per block). Leaves
+ * positions, velocities and the solver state unchanged.
  */
 b3g_status b3g_time_spmv(b3g_body body, uint32_t launches, double* mean_ms, double* bytes_per_launch);
 
