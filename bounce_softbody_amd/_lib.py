@@ -111,7 +111,7 @@ EXPORTS = [
     "b3g_status_string", "b3g_last_error", "b3g_device_count", "b3g_body_create", "b3g_body_destroy",
     "b3g_body_build", "b3g_body_build_batch", "b3g_get_instance_count", "b3g_step", "b3g_get_state", "b3g_set_state", "b3g_apply_external", "b3g_get_masses",
     "b3g_get_stats", "b3g_get_energy", "b3g_get_contacts", "b3g_get_tet_quats", "b3g_assemble", "b3g_get_system",
-    "b3g_synchronize", "b3g_time_spmv", "b3g_set_solver", "b3g_get_system_size", "b3g_pattern",
+    "b3g_synchronize", "b3g_time_spmv", "b3g_set_solver", "b3g_set_graph", "b3g_get_system_size", "b3g_pattern",
     "b3g_solve_cg", "b3g_spmv", "b3g_set_reduction", "b3g_save_state", "b3g_load_state",
 ]
 
@@ -148,6 +148,7 @@ def lib():
         "b3g_synchronize": (C.c_int, [vp]),
         "b3g_time_spmv": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "b3g_set_solver": (C.c_int, [vp, C.c_int, C.c_uint32, C.c_float]),
+        "b3g_set_graph": (C.c_int, [vp, C.c_int]),
         "b3g_get_system_size": (C.c_int, [vp, u32p, u32p]),
         "b3g_pattern": (C.c_int, [C.POINTER(BodyDesc), u32p, u32p, u32p, C.c_uint32]),
         "b3g_solve_cg": (C.c_int, [C.c_int, C.c_uint32, u32p, u32p, f32p, f32p, f32p, C.c_uint32, C.c_float, u32p]),

@@ -186,6 +186,10 @@ class Body:
         check(self._L.b3g_get_stats(self._h, s), "b3g_get_stats")
         return [(x.refinement_sweeps, x.relative_residual) for x in s]
 
+    def set_graph(self, enabled: bool):
+        """Capture the step's launch sequence in a HIP graph and replay it (identical results)."""
+        check(self._L.b3g_set_graph(self._h, 1 if enabled else 0), "b3g_set_graph")
+
     def set_reduction(self, mode: int):
         """_lib.REDUCE_FP64 (default) or _lib.REDUCE_SEQUENTIAL (the reference's FP32 accumulation order)."""
         check(self._L.b3g_set_reduction(self._h, mode), "b3g_set_reduction")

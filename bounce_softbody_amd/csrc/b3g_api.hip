@@ -108,6 +108,7 @@ b3g_status b3g_body_destroy(b3g_body h) {
 	if (!h) return B3G_OK;
 	(void)hipSetDevice(h->body.device);
 	(void)hipStreamSynchronize(h->body.stream);
+	drop_graph(h->body);
 	dfree_all(h->body);
 	(void)hipStreamDestroy(h->body.stream);
 	delete h;
@@ -143,7 +144,42 @@ b3g_status b3g_step(b3g_body h, float dt, uint32_t iterations, uint32_t sub_iter
 	float stol = sub_tolerance > 0.0f ? sub_tolerance : kEps;
 	if (!(stol < 1.0f)) { set_error("sub_tolerance must be in (0, 1)"); return B3G_INVALID_ARGUMENT; }   // sparse_solver.cpp:36
 	SET_DEVICE(h);
-	return enqueue_step(h->body, dt, iterations, sub_iterations, tol, stol, false);
+	Body& b = h->body;
+	b3g_status s = prepare_step(b, sub_iterations);
+	if (s != B3G_OK) return s;
+	if (!b.useGraph) return enqueue_step(b, dt, iterations, sub_iterations, tol, stol, false);
+	// Captured step: one graph per set of step parameters and modes, replayed.
+	const std::vector<double> key = {(double)dt, (double)iterations, (double)sub_iterations, (double)tol, (double)stol,
+	                                  (double)b.reduction, (double)b.solver, (double)b.innerIt, (double)b.innerTol,
+	                                  (double)b.logStride};
+	if (!b.graphExec || key != b.graphKey) {
+		drop_graph(b);
+		hipGraph_t graph = nullptr;
+		B3G_HIP(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
+		s = enqueue_step(b, dt, iterations, sub_iterations, tol, stol, false);
+		hipError_t e = hipStreamEndCapture(b.stream, &graph);
+		if (s != B3G_OK) { if (graph) (void)hipGraphDestroy(graph); return s; }
+		if (e != hipSuccess) { set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e)); return B3G_HIP_ERROR; }
+		e = hipGraphInstantiate(&b.graphExec, graph, nullptr, nullptr, 0);
+		(void)hipGraphDestroy(graph);
+		if (e != hipSuccess) {
+			b.graphExec = nullptr;
+			set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+			return B3G_HIP_ERROR;
+		}
+		b.graphKey = key;
+	}
+	B3G_HIP(hipGraphLaunch(b.graphExec, b.stream));
+	return B3G_OK;
+}
+
+b3g_status b3g_set_graph(b3g_body h, int enabled) {
+	CHECK_BODY(h);
+	SET_DEVICE(h);
+	B3G_HIP(hipStreamSynchronize(h->body.stream));
+	h->body.useGraph = enabled != 0;
+	if (!h->body.useGraph) drop_graph(h->body);
+	return B3G_OK;
 }
 
 b3g_status b3g_get_state(b3g_body h, float* pos, float* vel) {
@@ -398,7 +434,9 @@ b3g_status b3g_save_state(b3g_body h, void* blob, uint64_t* bytes) {
 	if (!blob) { *bytes = total; return B3G_OK; }
 	if (*bytes < total) { set_error("state blob too small"); return B3G_INVALID_ARGUMENT; }
 	B3G_HIP(hipStreamSynchronize(b.stream));
-	StateHeader hd = {kStateMagic, kStateVersion, b.n, b.tet.m, b.nPairs, b.stampCounter, b.nInst, 0};
+	uint32_t stamps = 0;
+	B3G_HIP(hipMemcpy(&stamps, b.stampCounter.p, 4, hipMemcpyDeviceToHost));
+	StateHeader hd = {kStateMagic, kStateVersion, b.n, b.tet.m, b.nPairs, stamps, b.nInst, 0};
 	char* out = static_cast<char*>(blob);
 	std::memcpy(out, &hd, sizeof(hd));
 	size_t off = sizeof(hd);
@@ -433,7 +471,7 @@ b3g_status b3g_load_state(b3g_body h, const void* blob, uint64_t bytes) {
 		B3G_HIP(hipMemcpy(p.buf->p, in + off, p.bytes, hipMemcpyHostToDevice));
 		off += p.bytes;
 	}
-	b.stampCounter = hd.stampCounter;
+	B3G_HIP(hipMemcpy(b.stampCounter.p, &hd.stampCounter, 4, hipMemcpyHostToDevice));
 	return B3G_OK;
 }
 

@@ -143,7 +143,7 @@ struct Body {
 	uint32_t nnz = 0;        // 3x3 blocks of pA
 	uint32_t nInst = 1;      // instances
 	uint32_t nSpheres = 0, nWorld = 0, nPairs = 0;
-	uint32_t stampCounter = 0;
+	DBuf stampCounter;                            // uint32 on the device: FindNewContacts calls so far
 	std::vector<uint32_t> hostRowInst;            // instance of each row (empty = all instance 0)
 	std::vector<uint32_t> hostRowBegin, hostRowEnd;
 	std::vector<V3> hostGravity;                  // per instance
@@ -234,6 +234,11 @@ struct Body {
 	DBuf cK;                                      // M33 per pair
 	std::vector<WorldShape> hostWorlds;
 
+	// ---------------------------------------------------------------- captured step (hipGraph)
+	bool useGraph = false;
+	hipGraphExec_t graphExec = nullptr;
+	std::vector<double> graphKey;                 // the step parameters and modes the graph was captured with
+
 	// ---------------------------------------------------------------- diagnostics
 	bool captureFi = false;
 
@@ -256,6 +261,8 @@ b3g_status launch_tet_setup(Body& b);
 b3g_status launch_tet_perm(Body& b, const std::vector<uint32_t>& src, DBuf& dst);
 b3g_status enqueue_step(Body& b, float dt, uint32_t iters, uint32_t subIters, float tol, float subTol,
                         bool assembleOnly);
+b3g_status prepare_step(Body& b, uint32_t subIters);
+void drop_graph(Body& b);
 b3g_status enqueue_update_contacts(Body& b);
 b3g_status enqueue_find_contacts(Body& b);
 b3g_status enqueue_spmv(Body& b, const DBuf& xin, DBuf& yout);

@@ -903,7 +903,7 @@ struct ContactArgs {
 	uint32_t* hit;
 	V3* cF;
 	M33* cK;
-	uint32_t stampCounter;
+	uint32_t* stampCounter;       // device counter of FindNewContacts calls
 	float dt;
 	SolverState* st;              // per instance
 	const uint32_t* rowInst;
@@ -934,12 +934,15 @@ __global__ void __launch_bounds__(kBlock) k_find_contacts(ContactArgs a) {
 	if (a.type[p] != kDynamic) return;
 	const uint32_t p0 = a.pairStart[s], nw = a.pairStart[s + 1] - p0;
 	a.active[k] = 1;
-	a.stamp[k] = a.stampCounter * nw + (k - p0);
+	a.stamp[k] = *a.stampCounter * nw + (k - p0);
 	a.normalForce[k] = 0.0f;
 	a.applyFriction[k] = 0;
 	a.manifold[2 * k] = zero3();
 	a.manifold[2 * k + 1] = zero3();
 }
+
+// One FindNewContacts call done.
+__global__ void k_stamp_advance(uint32_t* counter) { *counter += 1u; }
 
 // sphere_shape_contact.cpp:53-143
 __global__ void __launch_bounds__(kBlock) k_contact_forces(ContactArgs a) {
@@ -1925,7 +1928,7 @@ static ContactArgs contact_args(Body& b, float dt) {
 	c.hit = dptr<uint32_t>(b.cHit);
 	c.cF = dptr<V3>(b.cF);
 	c.cK = dptr<M33>(b.cK);
-	c.stampCounter = b.stampCounter;
+	c.stampCounter = dptr<uint32_t>(b.stampCounter);
 	c.dt = dt;
 	c.st = dptr<SolverState>(b.state);
 	c.rowInst = dptr<uint32_t>(b.rowInst);
@@ -1978,7 +1981,8 @@ b3g_status enqueue_find_contacts(Body& b) {
 	if (!b.nPairs) return B3G_OK;
 	ContactArgs c = contact_args(b, 0.0f);
 	LAUNCH(k_find_contacts, nblk(b.nPairs), c);
-	b.stampCounter += 1;
+	hipLaunchKernelGGL(k_stamp_advance, dim3(1), dim3(1), 0, b.stream, c.stampCounter);
+	B3G_HIP(hipGetLastError());
 	return B3G_OK;
 }
 
@@ -2240,6 +2244,23 @@ b3g_status time_pcg_spmv(Body& b, uint32_t launches, double* mean_ms) {
 	}
 	*mean_ms = launches ? (double)ms / launches : 0.0;
 	return B3G_OK;
+}
+
+// Everything a step allocates or sets up once, so that the step itself only
+// enqueues kernels (and can be captured into a graph).
+b3g_status prepare_step(Body& b, uint32_t subIters) {
+	if (b.solver != B3G_SOLVER_PCG) {
+		b3g_status s = ensure_refine(b);
+		if (s != B3G_OK) return s;
+		return ensure_cg_log(b, b.innerIt + 1);
+	}
+	return ensure_cg_log(b, subIters + 1);
+}
+
+void drop_graph(Body& b) {
+	if (b.graphExec) (void)hipGraphExecDestroy(b.graphExec);
+	b.graphExec = nullptr;
+	b.graphKey.clear();
 }
 
 b3g_status enqueue_step(Body& b, float dt, uint32_t iters, uint32_t subIters, float tol, float subTol,

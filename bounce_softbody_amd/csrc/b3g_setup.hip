@@ -605,6 +605,7 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 		return B3G_UNSUPPORTED;
 	}
 	b3g_status st = B3G_OK;
+	drop_graph(b);
 	dfree_all(b);
 	b.built = false;
 	const uint32_t n = d.n_particles;
@@ -1015,7 +1016,11 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 		std::vector<uint32_t> zeros32(pairs, 0u);
 		UP(b.cActive, zeros32);
 	}
-	b.stampCounter = 0;
+	{
+		const uint32_t zero = 0;
+		st = dupload(b, b.stampCounter, &zero, 4);
+		if (st != B3G_OK) return st;
+	}
 #undef UP
 #undef AL
 	st = build_tiles(b);

@@ -282,6 +282,14 @@ b3g_status b3g_load_state(b3g_body body, const void* blob, uint64_t bytes);
 enum { B3G_SOLVER_PCG = 0, B3G_SOLVER_REFINED_FP32 = 1, B3G_SOLVER_REFINED_BF16 = 2 };
 b3g_status b3g_set_solver(b3g_body body, int mode, uint32_t inner_iterations, float inner_tolerance);
 
+/*
+ * Captured steps (no counterpart in the reference): with enabled != 0,
+ * b3g_step records its launch sequence into a HIP graph once per set of step
+ * parameters and solver modes and replays it afterwards (one launch per
+ * step). Results are identical to uncaptured steps.
+ */
+b3g_status b3g_set_graph(b3g_body body, int enabled);
+
 /* Waits for all work queued on the body. */
 b3g_status b3g_synchronize(b3g_body body);
 
