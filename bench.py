@@ -160,6 +160,103 @@ def run_precision(args):
     print(json.dumps(line), flush=True)
 
 
+def run_batch(args):
+    """Many-body batch: args.bodies uneven tet boxes in one handle (one launch sequence per step, captured as a
+    HIP graph). Also times the same first args.handles_subset bodies as separate handles, and the reference
+    CPU path on a sample of bodies scaled to the whole batch by tetrahedron count."""
+    from bounce_softbody_amd import configs
+    from bounce_softbody_amd.body import Body
+
+    dt = configs.DT
+    t0 = time.time()
+    arrs = configs.batch_boxes(args.bodies)
+    nv = sum(len(a["p_type"]) for a in arrs)
+    nt = sum(len(a["tet_param"]) // 3 for a in arrs)
+    log(f"{args.bodies} bodies built in {time.time() - t0:.1f} s: {nv} particles, {nt} tets")
+    t0 = time.time()
+    batch = Body(arrs)
+    batch.set_graph(not args.no_graph)
+    rows, blocks = batch.system_size()
+    log(f"batch built on the device in {time.time() - t0:.1f} s: {rows} rows, {blocks} blocks")
+
+    def steps(bodies, k):
+        for _ in range(k):
+            for b in bodies:
+                b.Step(dt, 1, args.sub_iterations, 0.0, args.sub_tolerance)
+        for b in bodies:
+            b.synchronize()
+
+    steps([batch], args.warmup)
+    t0 = time.perf_counter()
+    steps([batch], args.steps)
+    el = time.perf_counter() - t0
+    value = args.steps / el
+    its = [int(s[0][2]) for s in batch.instance_stats()]
+    mean_ms, by = batch.time_spmv(args.spmv_launches)
+    achieved = by / (mean_ms * 1e-3) / 1e9
+    log(f"batch: {value:.2f} steps/s ({value * args.bodies:.0f} body-steps/s), PCG iterations min/max "
+        f"{min(its)}/{max(its)}; SpMV {mean_ms * 1e3:.1f} us, {achieved:.0f} GB/s")
+    # one handle per body on a subset, against the batch of the same subset
+    sub = arrs[: args.handles_subset]
+    handles = [Body(a) for a in sub]
+    small = Body(sub)
+    small.set_graph(not args.no_graph)
+    steps(handles, 2)
+    steps([small], 2)
+    t0 = time.perf_counter()
+    steps(handles, 5)
+    t_handles = (time.perf_counter() - t0) / 5
+    t0 = time.perf_counter()
+    steps([small], 5)
+    t_small = (time.perf_counter() - t0) / 5
+    log(f"{len(sub)} bodies: {1e3 * t_handles:.2f} ms/step as handles, {1e3 * t_small:.2f} ms/step as one batch")
+    del handles, small
+    baseline = None
+    if not args.no_cpu_baseline:
+        import oracle
+        kind = "reference" if oracle.reference_available() else "port"
+        sample = arrs[: args.cpu_sample_bodies]
+        st = sum(len(a["tet_param"]) // 3 for a in sample)
+        total = 0.0
+        for a in sample:
+            body = oracle.RefBody(a, fast=True) if kind == "reference" else oracle.OracleBody(a)
+            total += body.step_timed(1, dt, 1, args.sub_iterations, 0.0, args.sub_tolerance)
+            del body
+        per_batch_step = total * nt / max(1, st)
+        baseline = {"value": round(1.0 / per_batch_step, 6), "unit": "steps/s", "cores": 1, "kind": kind,
+                    "sample": f"one step of each of the first {len(sample)} bodies ({st} tets, {total:.1f} s of "
+                              f"CPU time), scaled to the batch's {nt} tets; single thread"}
+        log(f"CPU baseline: {baseline}")
+    line = {
+        "metric": "sim steps/s",
+        "value": round(value, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 / value, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"batch of {args.bodies} tet boxes (a, b, c ~ U{{3..24}} cells), {nv} particles, "
+                               f"{nt} tets, {blocks} blocks",
+                   "dt": "1/60", "newton_iterations": 1, "pcg_max_iterations": args.sub_iterations,
+                   "pcg_tolerance": args.sub_tolerance, "form": "one batch" + ("" if args.no_graph else ", captured steps"),
+                   "body_steps_per_s": round(value * args.bodies, 1),
+                   "pcg_iterations_min_max": [min(its), max(its)],
+                   "subset_ms_per_step": {"bodies": len(sub), "handles": round(1e3 * t_handles, 3),
+                                          "batch": round(1e3 * t_small, 3)}},
+        "roofline": {"kernel": "k_cg_spmv_dq (all instances, back-to-back pass)", "bound": "hbm",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": by,
+                     "mean_launch_us": round(mean_ms * 1e3, 2)},
+        "cpu_baseline": baseline,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,8 +270,13 @@ def main():
     ap.add_argument("--spmv-launches", type=int, default=200, help="launches of the SpMV roofline pass")
     ap.add_argument("--batch", action="store_true",
                     help="step the two bodies as one batch of two instances instead of two handles")
+    ap.add_argument("--graph", action="store_true", help="replay each body's step as a captured HIP graph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["two_body", "precision"], default="two_body")
+    ap.add_argument("--workload", choices=["two_body", "precision", "batch"], default="two_body")
+    ap.add_argument("--bodies", type=int, default=4096, help="batch workload: number of bodies")
+    ap.add_argument("--handles-subset", type=int, default=64, help="batch workload: bodies also timed as handles")
+    ap.add_argument("--cpu-sample-bodies", type=int, default=32, help="batch workload: bodies in the CPU sample")
+    ap.add_argument("--no-graph", action="store_true", help="batch workload: uncaptured steps")
     ap.add_argument("--pre-steps", type=int, default=10, help="precision workload: plain steps before the forms")
     ap.add_argument("--inner-iterations", type=int, default=50, help="precision workload: PCG iterations per sweep")
     ap.add_argument("--inner-tolerance", type=float, default=1e-3, help="precision workload: PCG tolerance per sweep")
@@ -190,6 +292,9 @@ def main():
 
     if args.workload == "precision":
         run_precision(args)
+        return
+    if args.workload == "batch":
+        run_batch(args)
         return
 
     from bounce_softbody_amd import configs
@@ -211,6 +316,9 @@ def main():
         bodies = [both]
         roof = both
     log(f"device bodies built in {time.time() - t0:.1f} s; systems {[b.system_size() for b in bodies]}")
+    if args.graph:
+        for b in bodies:
+            b.set_graph(True)
 
     def step():
         for b in bodies:
@@ -273,7 +381,8 @@ def main():
             "pcg_max_iterations": args.sub_iterations,
             "pcg_tolerance": args.sub_tolerance,
             "parallelism": "single GPU",
-            "form": "two handles" if not args.batch else "one batch of two instances",
+            "form": ("two handles" if not args.batch else "one batch of two instances")
+                    + (", captured steps" if args.graph else ""),
         },
         "roofline": {
             "kernel": "k_cg_spmv_dq (" + ("tet body" if not args.batch else "both bodies, one batched launch")

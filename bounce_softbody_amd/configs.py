@@ -38,6 +38,28 @@ def tet_grid(N: int = 10, *, seed: int = 1234, young=1e5, poisson=0.3, density=1
     return b
 
 
+def batch_boxes(count: int, seed: int = 2024, cell=0.1, jitter=0.005):
+    """The many-body batch: `count` tet boxes of a x b x c cells, a, b, c ~ U{3..24},
+    E ~ U[1e4, 1e6] Pa, nu ~ U[0.2, 0.45], 1000 kg/m^3, a fraction U[0.05, 0.2] of
+    the vertices static (chosen at random), gravity (0, -9.8, 0). Returns arrays
+    in creation order, one dict per body."""
+    u = splitmix64_np(seed, 6 * count).reshape(count, 6)
+    out = []
+    for k in range(count):
+        a, b, c = (3 + np.minimum((u[k, :3] * 22).astype(int), 21)).tolist()
+        young = f32(1e4 + u[k, 3] * (1e6 - 1e4))
+        poisson = f32(0.2 + u[k, 4] * 0.25)
+        m = GridTetMesh(a, b, c).scale(cell).jitter(jitter, seed + 7919 * (k + 1))
+        body = tet_body(m, density=1000.0, young=young, poisson=poisson)
+        nv = m.vertex_count
+        frac = 0.05 + 0.15 * u[k, 5]
+        pick = splitmix64_np(seed ^ (0x5bd1e995 + k), nv) < frac
+        body.set_particle_types(np.nonzero(pick)[0], STATIC)
+        body.gravity[:] = (0.0, -9.8, 0.0)
+        out.append(body.arrays())
+    return out
+
+
 def log_uniform_young(N: int, seed: int, lo=1e4, hi=1e7) -> np.ndarray:
     """Per-element E = 10^(log10(lo) + u (log10(hi) - log10(lo))), 5 N^3 elements."""
     u = splitmix64_np(seed ^ 0x5DEECE66D, 5 * N ** 3)
