@@ -257,6 +257,91 @@ def run_batch(args):
     print(json.dumps(line), flush=True)
 
 
+def run_contact(args):
+    """Contact workload: args.bodies bodies of [cloth (args.contact_cloth+1)^2 vertices + tet args.contact_tet^3
+    cells] over one 128^3 SDF, every vertex a sphere fixture, in one handle with captured steps. The bodies start
+    clear of the terrain; args.pre_steps untimed steps let them settle into contact first."""
+    from bounce_softbody_amd import configs
+    from bounce_softbody_amd.body import Body
+
+    dt = configs.DT
+    t0 = time.time()
+    arrs = configs.contact_batch(args.bodies, cloth_cells=args.contact_cloth, tet_cells=args.contact_tet,
+                                 sdf_res=args.sdf_res)
+    nv = sum(len(a["p_type"]) for a in arrs)
+    nt = sum(len(a["tet_param"]) // 3 for a in arrs)
+    ntri = sum(len(a["st_param"]) // 12 for a in arrs)
+    log(f"{args.bodies} bodies built in {time.time() - t0:.1f} s: {nv} particles, {nt} tets, {ntri} cloth triangles")
+    t0 = time.time()
+    batch = Body(arrs)
+    batch.set_graph(not args.no_graph)
+    rows, blocks = batch.system_size()
+    log(f"batch built on the device in {time.time() - t0:.1f} s: {rows} rows, {blocks} blocks")
+
+    def steps(k):
+        for _ in range(k):
+            batch.Step(dt, 1, args.sub_iterations, 0.0, args.sub_tolerance)
+        batch.synchronize()
+
+    steps(args.pre_steps)
+    contacts_start = len(batch.contacts()[0])
+    steps(args.warmup)
+    t0 = time.perf_counter()
+    steps(args.steps)
+    el = time.perf_counter() - t0
+    value = args.steps / el
+    contacts_end = len(batch.contacts()[0])
+    its = [int(s[0][2]) for s in batch.instance_stats()]
+    mean_ms, by = batch.time_spmv(args.spmv_launches)
+    achieved = by / (mean_ms * 1e-3) / 1e9
+    log(f"contact batch: {value:.2f} steps/s ({value * args.bodies:.0f} body-steps/s), contacts {contacts_start} -> "
+        f"{contacts_end}, PCG iterations min/max {min(its)}/{max(its)}; SpMV {mean_ms * 1e3:.1f} us, {achieved:.0f} GB/s")
+    baseline = None
+    if not args.no_cpu_baseline:
+        import oracle
+        kind = "reference" if oracle.reference_available() else "port"
+        sample = arrs[: args.cpu_sample_bodies]
+        total = 0.0
+        for a in sample:
+            body = oracle.RefBody(a, fast=True) if kind == "reference" else oracle.OracleBody(a)
+            total += body.step_timed(1, dt, 1, args.sub_iterations, 0.0, args.sub_tolerance)
+            del body
+        per_batch_step = total * len(arrs) / max(1, len(sample))
+        baseline = {"value": round(1.0 / per_batch_step, 6), "unit": "steps/s", "cores": 1, "kind": kind,
+                    "sample": f"the first step of each of the first {len(sample)} bodies ({total:.1f} s of CPU "
+                              f"time), scaled to the batch's {len(arrs)} equal-sized bodies; single thread"}
+        log(f"CPU baseline: {baseline}")
+    line = {
+        "metric": "sim steps/s",
+        "value": round(value, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 / value, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"contact: {args.bodies} x [cloth {args.contact_cloth + 1}^2 vertices + tet "
+                               f"{args.contact_tet}^3 cells] over one {args.sdf_res}^3 SDF, {nv} particles "
+                               f"(all sphere fixtures), {nt} tets, {ntri} cloth triangles, {blocks} blocks",
+                   "dt": "1/60", "newton_iterations": 1, "pcg_max_iterations": args.sub_iterations,
+                   "pcg_tolerance": args.sub_tolerance, "pre_steps": args.pre_steps,
+                   "form": "one batch" + ("" if args.no_graph else ", captured steps"),
+                   "body_steps_per_s": round(value * args.bodies, 1),
+                   "contacts_before_after": [contacts_start, contacts_end],
+                   "pcg_iterations_min_max": [min(its), max(its)]},
+        "roofline": {"kernel": "k_cg_spmv_dq (all instances, back-to-back pass)", "bound": "hbm",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": by,
+                     "mean_launch_us": round(mean_ms * 1e3, 2)},
+        "cpu_baseline": baseline,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,12 +357,18 @@ def main():
                     help="step the two bodies as one batch of two instances instead of two handles")
     ap.add_argument("--graph", action="store_true", help="replay each body's step as a captured HIP graph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["two_body", "precision", "batch"], default="two_body")
-    ap.add_argument("--bodies", type=int, default=4096, help="batch workload: number of bodies")
+    ap.add_argument("--workload", choices=["two_body", "precision", "batch", "contact"], default="two_body")
+    ap.add_argument("--bodies", type=int, default=None, help="batch / contact workloads: number of bodies "
+                                                               "(default 4096 / 256)")
     ap.add_argument("--handles-subset", type=int, default=64, help="batch workload: bodies also timed as handles")
-    ap.add_argument("--cpu-sample-bodies", type=int, default=32, help="batch workload: bodies in the CPU sample")
-    ap.add_argument("--no-graph", action="store_true", help="batch workload: uncaptured steps")
-    ap.add_argument("--pre-steps", type=int, default=10, help="precision workload: plain steps before the forms")
+    ap.add_argument("--cpu-sample-bodies", type=int, default=None,
+                    help="batch / contact workloads: bodies in the CPU sample (default 32 / 2)")
+    ap.add_argument("--no-graph", action="store_true", help="batch / contact workloads: uncaptured steps")
+    ap.add_argument("--contact-cloth", type=int, default=127, help="contact workload: cloth cells per side")
+    ap.add_argument("--contact-tet", type=int, default=16, help="contact workload: tet block cells per axis")
+    ap.add_argument("--sdf-res", type=int, default=128, help="contact workload: SDF voxels per axis")
+    ap.add_argument("--pre-steps", type=int, default=None,
+                    help="precision / contact workloads: untimed steps first (default 10 / 40)")
     ap.add_argument("--inner-iterations", type=int, default=50, help="precision workload: PCG iterations per sweep")
     ap.add_argument("--inner-tolerance", type=float, default=1e-3, help="precision workload: PCG tolerance per sweep")
     ap.add_argument("--max-sweeps", type=int, default=4, help="precision workload: refinement sweeps at most")
@@ -290,8 +381,18 @@ def main():
               "part of this build (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    contact = args.workload == "contact"
+    if args.bodies is None:
+        args.bodies = 256 if contact else 4096
+    if args.cpu_sample_bodies is None:
+        args.cpu_sample_bodies = 2 if contact else 32
+    if args.pre_steps is None:
+        args.pre_steps = 40 if contact else 10
     if args.workload == "precision":
         run_precision(args)
+        return
+    if contact:
+        run_contact(args)
         return
     if args.workload == "batch":
         run_batch(args)

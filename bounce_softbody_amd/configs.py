@@ -60,6 +60,40 @@ def batch_boxes(count: int, seed: int = 2024, cell=0.1, jitter=0.005):
     return out
 
 
+def contact_batch(count: int = 256, *, cloth_cells=127, tet_cells=16, sdf_res=128, seed=4242, radius=0.02,
+                  friction=0.3):
+    """The contact workload: `count` bodies, each a (cloth_cells+1)^2-vertex cloth (2 m square, stretch
+    k = 1e4, shear k = 1e3, 0.1 kg/m^2, corners static) and a tet_cells^3 block of 0.05 m cells (E = 1e5 Pa,
+    nu = 0.3, 1000 kg/m^3, top face static), every vertex a sphere fixture of `radius`, over one sdf_res^3
+    SDF world fixture filled from a seeded union of 8 spheres / boxes on [-2, 2] x [-1.5, 1.5] x [-2, 2] m.
+    Friction `friction` on both sides (mixed as sqrt(mu1 mu2)). Each body is offset by U(-0.3, 0.3) m in x
+    and z and jittered with its own seed. All bodies pass the same voxel array, so the grid is uploaded once.
+    Returns arrays in creation order, one dict per body."""
+    lower, upper = (-2.0, -1.5, -2.0), (2.0, 1.5, 2.0)
+    vox = sphere_union_sdf(sdf_res, lower, upper, seed=seed, count=8)
+    u = splitmix64_np(seed ^ 0x9E3779B9, 2 * count).reshape(count, 2)
+    R, N = cloth_cells, tet_cells
+    out = []
+    for k in range(count):
+        ox, oz = (-0.3 + 0.6 * u[k]).tolist()
+        cm = (GridClothMesh(R, R).scale((2.0 / R, 1.0, 2.0 / R)).translate((-0.7 + ox, 0.9, oz))
+              .jitter(1e-3, seed + 31 * (k + 1)))
+        b = cloth_body(cm, density=0.1, radius=radius, friction=friction, stretching_stiffness=1e4,
+                       shear_stiffness=1e3)
+        b.set_particle_types([cm.vertex(0, 0), cm.vertex(0, R), cm.vertex(R, 0), cm.vertex(R, R)], STATIC)
+        tm = GridTetMesh(N, N, N).scale(0.05).jitter(0.0025, seed + 37 * (k + 1))
+        lo = tm.vertices.min(0)
+        tm.translate((0.6 + ox - lo[0], 0.4 - lo[1], -0.4 + oz - lo[2]))
+        base = b.n_particles
+        tet_body(tm, density=1000.0, young=1e5, poisson=0.3, radius=radius, friction=friction, builder=b)
+        top = tm.vertex(N, *np.meshgrid(np.arange(N + 1), np.arange(N + 1), indexing="ij")).reshape(-1)
+        b.set_particle_types(base + top, STATIC)
+        b.add_world_sdf(vox, lower, upper, radius=0.0, friction=friction)
+        b.gravity[:] = (0.0, -9.8, 0.0)
+        out.append(b.arrays())
+    return out
+
+
 def log_uniform_young(N: int, seed: int, lo=1e4, hi=1e7) -> np.ndarray:
     """Per-element E = 10^(log10(lo) + u (log10(hi) - log10(lo))), 5 N^3 elements."""
     u = splitmix64_np(seed ^ 0x5DEECE66D, 5 * N ** 3)

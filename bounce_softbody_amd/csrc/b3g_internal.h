@@ -15,9 +15,16 @@ namespace b3g {
 
 enum : uint8_t { kStatic = 0, kKinematic = 1, kDynamic = 2 };
 
-// Contribution codes of the per-slot lists: a staged block index, or
-// kTetCode | e for a tetrahedron block computed during assembly.
+// Contribution codes of the per-slot lists: a staged block index (< 2^30),
+// kTetCode | e for a tetrahedron block computed during assembly, or
+// kUVCode | variant << kUVShift | e for a stretch (variants 0..17: 9 part +
+// 3 i + j) or shear (18 + 3 i + j) block computed during assembly from the
+// element's record.
 constexpr uint32_t kTetCode = 0x80000000u;
+constexpr uint32_t kUVCode = 0x40000000u;
+constexpr uint32_t kUVShift = 25;
+constexpr uint32_t kUVMaxElements = 1u << kUVShift;   // stretch / shear forces per body or batch
+constexpr uint32_t kShearVariant = 18;
 
 // Thread-local error text for b3g_last_error().
 void set_error(const std::string& msg);
@@ -64,12 +71,18 @@ struct TriTable {
 	uint32_t jx = 0, jv = 0, f = 0, fd = 0;
 };
 
-struct UVTable {     // stretch (12 coefficient planes) and shear (8)
+struct UVTable {     // stretch (20 coefficient planes) and shear (16)
 	uint32_t m = 0;
 	DBuf p;          // 3 planes
 	DBuf c;          // coefficient planes
 	uint32_t ncoef = 0;
-	uint32_t jx = 0, jv = 0, f = 0;   // bases; per element local slots as described in b3g_setup
+	uint32_t f = 0;  // force staging base; per element local slots as described in b3g_setup
+	// float4 records the assembly rebuilds the Jacobian blocks from.
+	// stretch, 3 per part (u, v): (n_w, |w|) written per linearisation,
+	//   (dw/dx, alpha), (ks, kd, b, 0) from set-up;
+	// shear, 4: (w_u, alpha), (w_v, ks) written per linearisation,
+	//   (dw_u/dx, kd), (dw_v/dx, 0) from set-up.
+	DBuf rec;
 	bool anyKs = false, anyKd = false;
 };
 

@@ -668,16 +668,18 @@ struct UVArgs {
 	const float* c;
 	const V3* x;
 	const V3* v;
-	M33* jstage;
+	float4* rec;             // per-element records the assembly rebuilds the blocks from
 	V3* fstage;
-	uint32_t jx, jv, f;
+	uint32_t f;
 	const SolverState* st;   // per instance
 	const uint32_t* rowInst;
 };
 
 __device__ __forceinline__ float coef(const UVArgs& a, int k, uint32_t e) { return a.c[(size_t)k * a.m + e]; }
 
-// stretch_force.cpp:121-389: parts u (0) and v (1)
+// stretch_force.cpp:121-389: parts u (0) and v (1). Forces are staged; the
+// Jacobian blocks of each part are rebuilt in the assembly (uv_block) from
+// (n_w, |w|) written here and the part's constants.
 __global__ void __launch_bounds__(kBlock) k_stretch_forces(UVArgs a) {
 	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= a.m) return;
@@ -690,7 +692,6 @@ __global__ void __launch_bounds__(kBlock) k_stretch_forces(UVArgs a) {
 	uint32_t id[3];
 	V3 xs[3], vs[3];
 	for (int k = 0; k < 3; ++k) { id[k] = a.p[(size_t)k * m + e]; xs[k] = a.x[id[k]]; vs[k] = a.v[id[k]]; }
-	M33 I = mdiag(1.0f);
 	V3 dx1 = xs[1] - xs[0], dx2 = xs[2] - xs[0];
 	V3 wu = inv_det * (dv2 * dx1 - dv1 * dx2);
 	float len_wu = length(wu);
@@ -710,25 +711,14 @@ __global__ void __launch_bounds__(kBlock) k_stretch_forces(UVArgs a) {
 			float C = alpha * (len - bb);
 			for (int i = 0; i < 3; ++i)
 				a.fstage[a.f + (size_t)(6 * part + i) * m + e] = live ? -ks * C * dCdx[i] : zero3();
-			for (int i = 0; i < 3; ++i)
-				for (int j = 0; j < 3; ++j) {
-					M33 Kij = outer(dCdx[i], dCdx[j]);
-					if (len > bb) {
-						M33 d2 = (alpha * inv_len * comp(dwdx, i) * comp(dwdx, j)) * (I - outer(nw, nw));
-						Kij = Kij + C * d2;
-					}
-					a.jstage[a.jx + (size_t)(9 * part + 3 * i + j) * m + e] = live ? -ks * Kij : mzero();
-				}
 		}
 		if (kd > 0.0f) {
 			float dCdt = 0.0f;
 			for (int i = 0; i < 3; ++i) dCdt += dot(dCdx[i], vs[i]);
 			for (int i = 0; i < 3; ++i)
 				a.fstage[a.f + (size_t)(6 * part + 3 + i) * m + e] = live ? -kd * dCdt * dCdx[i] : zero3();
-			for (int i = 0; i < 3; ++i)
-				for (int j = 0; j < 3; ++j)
-					a.jstage[a.jv + (size_t)(9 * part + 3 * i + j) * m + e] = live ? -kd * outer(dCdx[i], dCdx[j]) : mzero();
 		}
+		a.rec[(size_t)(2 * e + part) * 3] = make_float4(nw.x, nw.y, nw.z, len);
 	}
 }
 
@@ -754,16 +744,14 @@ __global__ void __launch_bounds__(kBlock) k_shear_forces(UVArgs a) {
 	if (ks > 0.0f) {
 		float C = alpha * dot(wu, wv);
 		for (int i = 0; i < 3; ++i) a.fstage[a.f + (size_t)i * m + e] = -ks * C * dCdx[i];
-		for (int i = 0; i < 3; ++i)
-			for (int j = 0; j < 3; ++j) a.jstage[a.jx + (size_t)(3 * i + j) * m + e] = -ks * outer(dCdx[i], dCdx[j]);
 	}
 	if (kd > 0.0f) {
 		float dCdt = 0.0f;
 		for (int i = 0; i < 3; ++i) dCdt += dot(dCdx[i], vs[i]);
 		for (int i = 0; i < 3; ++i) a.fstage[a.f + (size_t)(3 + i) * m + e] = -kd * dCdt * dCdx[i];
-		for (int i = 0; i < 3; ++i)
-			for (int j = 0; j < 3; ++j) a.jstage[a.jv + (size_t)(3 * i + j) * m + e] = -kd * outer(dCdx[i], dCdx[j]);
 	}
+	a.rec[4 * (size_t)e] = make_float4(wu.x, wu.y, wu.z, alpha);
+	a.rec[4 * (size_t)e + 1] = make_float4(wv.x, wv.y, wv.z, ks);
 }
 
 struct PairArgs {
@@ -1062,6 +1050,9 @@ struct AsmArgs {
 	uint32_t nJx, nJv, tetM;
 	const float4* tetQ;
 	const float* tetKd;
+	// stretch / shear records (UVTable::rec)
+	const float4* stRec;
+	const float4* shRec;
 	uint32_t anyJv;
 	const uint8_t* type;
 	const float* mass;
@@ -1104,7 +1095,41 @@ struct Contrib {
 	const float* K0;        // contribution-ordered K0 planes (tetrahedra) or null
 	uint32_t count;         // plane stride of K0
 	const float* kd;        // per-tetrahedron scale (dfdv) or null (dfdx)
+	bool dv;                // the dfdv list
 };
+
+// A stretch or shear Jacobian block from the element's record r (see
+// UVTable), with the element kernel's operations in the reference's order.
+__device__ __forceinline__ M33 uv_block(uint32_t variant, const float4* r, bool dv) {
+	if (variant < kShearVariant) {   // stretch_force.cpp:163-207 (dfdx), :231-248 (dfdv)
+		const int i = (int)(variant % 9) / 3, j = (int)(variant % 3);
+		const float len = r[0].w;
+		if (!(len > 0.0f)) return mzero();
+		const V3 nw = v3(r[0].x, r[0].y, r[0].z);
+		const V3 dwdx = v3(r[1].x, r[1].y, r[1].z);
+		const float alpha = r[1].w;
+		const V3 dCi = alpha * comp(dwdx, i) * nw;
+		const V3 dCj = alpha * comp(dwdx, j) * nw;
+		if (dv) return -r[2].y * outer(dCi, dCj);
+		const float ks = r[2].x, bb = r[2].z;
+		const float inv_len = 1.0f / len;
+		const float C = alpha * (len - bb);
+		M33 K = outer(dCi, dCj);
+		if (len > bb) {
+			const M33 d2 = (alpha * inv_len * comp(dwdx, i) * comp(dwdx, j)) * (mdiag(1.0f) - outer(nw, nw));
+			K = K + C * d2;
+		}
+		return -ks * K;
+	}
+	// shear_force.cpp:161-244
+	const int k = (int)(variant - kShearVariant), i = k / 3, j = k % 3;
+	const V3 wu = v3(r[0].x, r[0].y, r[0].z), wv = v3(r[1].x, r[1].y, r[1].z);
+	const V3 dwudx = v3(r[2].x, r[2].y, r[2].z), dwvdx = v3(r[3].x, r[3].y, r[3].z);
+	const float alpha = r[0].w;
+	const V3 dCi = alpha * (comp(dwudx, i) * wv + comp(dwvdx, i) * wu);
+	const V3 dCj = alpha * (comp(dwudx, j) * wv + comp(dwvdx, j) * wu);
+	return (dv ? -r[2].w : -r[1].w) * outer(dCi, dCj);
+}
 
 __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib& cb_, const uint32_t* ptr, uint32_t s0,
                                                 uint32_t s1, uint32_t s, bool own, float* lds, M33& acc) {
@@ -1113,8 +1138,9 @@ __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib&
 	for (uint32_t cb = c0; cb < c1; cb += kAsmCap) {
 		const uint32_t ce = min(cb + (uint32_t)kAsmCap, c1);
 		uint32_t code[kAsmPer];
-		M33 blk[kAsmPer];
-		Q4 q[kAsmPer];
+		// per contribution: a K0 or staged block in r[0..2] (+ the rotation in
+		// r[3]) or a stretch / shear record in r[0..3]
+		float4 r[kAsmPer][4];
 #pragma unroll
 		for (int u = 0; u < kAsmPer; ++u) {
 			const uint32_t k = cb + threadIdx.x + u * kBlock;
@@ -1124,26 +1150,56 @@ __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib&
 		for (int u = 0; u < kAsmPer; ++u) {
 			const uint32_t k = cb + threadIdx.x + u * kBlock;
 			if (k >= ce) continue;
-			if (code[u] & kTetCode) {
-				blk[u] = ldA(cb_.K0, cb_.count, k);
-				const float4 q4 = a.tetQ[code[u] & ~kTetCode];
-				q[u].v = v3(q4.x, q4.y, q4.z);
-				q[u].s = q4.w;
+			const uint32_t c = code[u];
+			if (c & kTetCode) {
+				const M33 m = ldA(cb_.K0, cb_.count, k);
+				r[u][0] = make_float4(m.x.x, m.x.y, m.x.z, m.y.x);
+				r[u][1] = make_float4(m.y.y, m.y.z, m.z.x, m.z.y);
+				r[u][2] = make_float4(m.z.z, 0.0f, 0.0f, 0.0f);
+				r[u][3] = a.tetQ[c & ~kTetCode];
+			} else if (c & kUVCode) {
+				const uint32_t variant = (c & ~kUVCode) >> kUVShift, e = c & (kUVMaxElements - 1);
+				if (variant < kShearVariant) {
+					const float4* rec = a.stRec + (size_t)(2 * e + variant / 9) * 3;
+					r[u][0] = rec[0];
+					r[u][1] = rec[1];
+					r[u][2] = rec[2];
+				} else {
+					const float4* rec = a.shRec + 4 * (size_t)e;
+					r[u][0] = rec[0];
+					r[u][1] = rec[1];
+					r[u][2] = rec[2];
+					r[u][3] = rec[3];
+				}
 			} else {
-				blk[u] = a.jstage[code[u]];
+				const M33 m = a.jstage[c];
+				r[u][0] = make_float4(m.x.x, m.x.y, m.x.z, m.y.x);
+				r[u][1] = make_float4(m.y.y, m.y.z, m.z.x, m.z.y);
+				r[u][2] = make_float4(m.z.z, 0.0f, 0.0f, 0.0f);
 			}
 		}
 #pragma unroll
 		for (int u = 0; u < kAsmPer; ++u) {
 			const uint32_t k = cb + threadIdx.x + u * kBlock;
 			if (k >= ce) continue;
-			M33 v = blk[u];
-			if (code[u] & kTetCode) {
+			const uint32_t c = code[u];
+			M33 v;
+			if (c & kUVCode && !(c & kTetCode)) {
+				v = uv_block((c & ~kUVCode) >> kUVShift, r[u], cb_.dv);
+			} else {
+				v.x = v3(r[u][0].x, r[u][0].y, r[u][0].z);
+				v.y = v3(r[u][0].w, r[u][1].x, r[u][1].y);
+				v.z = v3(r[u][1].z, r[u][1].w, r[u][2].x);
+			}
+			if (c & kTetCode) {
 				// R K0_ij R^T of the tetrahedron (tetrahedron_element_force.cpp:400-420)
-				M33 R = qmatrix(q[u]);
+				Q4 q;
+				q.v = v3(r[u][3].x, r[u][3].y, r[u][3].z);
+				q.s = r[u][3].w;
+				M33 R = qmatrix(q);
 				M33 RT = transpose(R);
-				M33 K = R * blk[u] * RT;
-				v = cb_.kd ? -(cb_.kd[code[u] & ~kTetCode] * K) : -K;
+				M33 K = R * v * RT;
+				v = cb_.kd ? -(cb_.kd[c & ~kTetCode] * K) : -K;
 			}
 			const float* pv = &v.x.x;
 #pragma unroll
@@ -1216,11 +1272,11 @@ __global__ void __launch_bounds__(kBlock) k_assemble_slots(AsmArgs a) {
 	const V3 w_j = own ? a.w[j] : zero3();
 	// dfdx(i, j): contributions in list order, then contacts (force_solver.cpp:70-86)
 	M33 dx = mzero();
-	tile_accumulate(a, Contrib{a.jxIdx, a.K0x, a.nJx, nullptr}, a.jxPtr, s0, s1, s, own, lds, dx);
+	tile_accumulate(a, Contrib{a.jxIdx, a.K0x, a.nJx, nullptr, false}, a.jxPtr, s0, s1, s, own, lds, dx);
 	// dfdv(i, j): particle mass damping first, then contributions
 	M33 dv = mzero();
 	if (own && i == j && damp_i > 0.0f) dv = dv + mdiag(-damp_i * mass_i);
-	if (a.anyJv) tile_accumulate(a, Contrib{a.jvIdx, a.K0v, a.nJv, a.tetKd}, a.jvPtr, s0, s1, s, own, lds, dv);
+	if (a.anyJv) tile_accumulate(a, Contrib{a.jvIdx, a.K0v, a.nJv, a.tetKd, true}, a.jvPtr, s0, s1, s, own, lds, dv);
 	if (!own) return;
 	const bool dyn_i = type_i == kDynamic;
 	if (i == j && sph_i >= 0 && dyn_i) {
@@ -2007,13 +2063,13 @@ static b3g_status enqueue_linearise(Body& b, float h) {
 		LAUNCH(k_tri_forces, nblk(b.tri.m), t);
 	}
 	if (b.stretch.m) {
-		UVArgs u{b.stretch.m, dptr<uint32_t>(b.stretch.p), dptr<float>(b.stretch.c), dptr<V3>(b.x), dptr<V3>(b.v), js, fs,
-		         b.stretch.jx, b.stretch.jv, b.stretch.f, st, ri};
+		UVArgs u{b.stretch.m, dptr<uint32_t>(b.stretch.p), dptr<float>(b.stretch.c), dptr<V3>(b.x), dptr<V3>(b.v),
+		         dptr<float4>(b.stretch.rec), fs, b.stretch.f, st, ri};
 		LAUNCH(k_stretch_forces, nblk(b.stretch.m), u);
 	}
 	if (b.shear.m) {
-		UVArgs u{b.shear.m, dptr<uint32_t>(b.shear.p), dptr<float>(b.shear.c), dptr<V3>(b.x), dptr<V3>(b.v), js, fs,
-		         b.shear.jx, b.shear.jv, b.shear.f, st, ri};
+		UVArgs u{b.shear.m, dptr<uint32_t>(b.shear.p), dptr<float>(b.shear.c), dptr<V3>(b.x), dptr<V3>(b.v),
+		         dptr<float4>(b.shear.rec), fs, b.shear.f, st, ri};
 		LAUNCH(k_shear_forces, nblk(b.shear.m), u);
 	}
 	if (b.spring.m) {
@@ -2034,6 +2090,7 @@ static b3g_status enqueue_linearise(Body& b, float h) {
 	a.rowTiles = dptr<uint32_t>(b.tiles); a.tileInst = dptr<uint32_t>(b.tileInst);
 	a.K0x = dptr<float>(b.K0x); a.K0v = dptr<float>(b.K0v); a.nJx = b.nJx; a.nJv = b.nJv;
 	a.tetM = b.tet.m; a.tetQ = dptr<float4>(b.tet.q); a.tetKd = dptr<float>(b.tet.kd); a.anyJv = b.anyJv ? 1u : 0u;
+	a.stRec = dptr<float4>(b.stretch.rec); a.shRec = dptr<float4>(b.shear.rec);
 	a.slotRow = dptr<uint32_t>(b.slotRow); a.cols = dptr<uint32_t>(b.cols);
 	a.jxPtr = dptr<uint32_t>(b.jxPtr); a.jxIdx = dptr<uint32_t>(b.jxIdx);
 	a.jvPtr = dptr<uint32_t>(b.jvPtr); a.jvIdx = dptr<uint32_t>(b.jvIdx);

@@ -184,8 +184,31 @@ struct SymbolicInput {
 	const std::vector<uint32_t>* moP; uint32_t moM; const std::vector<float>* moCoef;   // 6
 	// staging bases
 	uint32_t tetF, tetFd, triJx, triJv, triF, triFd;
-	uint32_t stJx, stJv, stF, shJx, shJv, shF, spJx, spJv, spF, moJx, moJv, moF;
+	uint32_t stF, shF, spJx, spJv, spF, moJx, moJv, moF;
 };
+
+// A stretch / shear block rebuilt in the assembly (variant: see kUVCode).
+inline uint32_t uv_code(uint32_t variant, uint32_t e) { return kUVCode | (variant << kUVShift) | e; }
+
+// Element counts the 32-bit contribution codes can address: 16 blocks per
+// tetrahedron in 31 bits, stretch / shear indices in kUVShift bits, staged
+// blocks (triangle elements, springs, mouse) below kUVCode.
+bool code_limits(const b3g_body_desc& d) {
+	if (d.n_tet >= (1u << 27)) {
+		set_error("more than 2^27 - 1 tetrahedron elements");
+		return false;
+	}
+	if (d.n_stretch >= kUVMaxElements || d.n_shear >= kUVMaxElements) {
+		set_error("more than 2^25 - 1 stretch or shear forces");
+		return false;
+	}
+	const uint64_t staged = 18ull * d.n_tri + 8ull * d.n_spring + 32ull * d.n_mouse;
+	if (staged >= kUVCode) {
+		set_error("more than 2^30 staged Jacobian blocks (triangle elements, springs, mouse forces)");
+		return false;
+	}
+	return true;
+}
 
 void build_pattern(const SymbolicInput& in, Pattern& P) {
 	const uint32_t n = in.n;
@@ -234,12 +257,12 @@ void build_pattern(const SymbolicInput& in, Pattern& P) {
 				if (ks > 0.0f) {
 					for (int i = 0; i < 3; ++i) fTouch.push_back({p[i], in.stF + (uint32_t)(6 * part + i) * m + e});
 					for (int i = 0; i < 3; ++i)
-						for (int j = 0; j < 3; ++j) D.touch(p[i], p[j], in.stJx + (uint32_t)(9 * part + 3 * i + j) * m + e);
+						for (int j = 0; j < 3; ++j) D.touch(p[i], p[j], uv_code(9 * part + 3 * i + j, e));
 				}
 				if (kd > 0.0f) {
 					for (int i = 0; i < 3; ++i) fTouch.push_back({p[i], in.stF + (uint32_t)(6 * part + 3 + i) * m + e});
 					for (int i = 0; i < 3; ++i)
-						for (int j = 0; j < 3; ++j) V.touch(p[i], p[j], in.stJv + (uint32_t)(9 * part + 3 * i + j) * m + e);
+						for (int j = 0; j < 3; ++j) V.touch(p[i], p[j], uv_code(9 * part + 3 * i + j, e));
 				}
 			}
 			break;
@@ -251,12 +274,12 @@ void build_pattern(const SymbolicInput& in, Pattern& P) {
 			if (c[12] > 0.0f) {
 				for (int i = 0; i < 3; ++i) fTouch.push_back({p[i], in.shF + (uint32_t)i * m + e});
 				for (int i = 0; i < 3; ++i)
-					for (int j = 0; j < 3; ++j) D.touch(p[i], p[j], in.shJx + (uint32_t)(3 * i + j) * m + e);
+					for (int j = 0; j < 3; ++j) D.touch(p[i], p[j], uv_code(kShearVariant + 3 * i + j, e));
 			}
 			if (c[13] > 0.0f) {
 				for (int i = 0; i < 3; ++i) fTouch.push_back({p[i], in.shF + (uint32_t)(3 + i) * m + e});
 				for (int i = 0; i < 3; ++i)
-					for (int j = 0; j < 3; ++j) V.touch(p[i], p[j], in.shJv + (uint32_t)(3 * i + j) * m + e);
+					for (int j = 0; j < 3; ++j) V.touch(p[i], p[j], uv_code(kShearVariant + 3 * i + j, e));
 			}
 			break;
 		}
@@ -440,11 +463,7 @@ b3g_status validate(const b3g_body_desc& d) {
 	    !chk(d.sh_p, d.n_shear, 3, "shear force") || !chk(d.sp_p, d.n_spring, 2, "spring force") ||
 	    !chk(d.mo_p, d.n_mouse, 4, "mouse force"))
 		return B3G_INVALID_ARGUMENT;
-	// 16 blocks per tetrahedron must fit the 31-bit contribution codes
-	if (d.n_tet >= (1u << 27)) {
-		set_error("more than 2^27 - 1 tetrahedron elements");
-		return B3G_UNSUPPORTED;
-	}
+	if (!code_limits(d)) return B3G_UNSUPPORTED;
 	const uint32_t counts[7] = {0, d.n_stretch, d.n_shear, d.n_spring, d.n_mouse, d.n_tri, d.n_tet};
 	std::vector<std::vector<char>> used(7);
 	for (int t = 1; t < 7; ++t) used[t].assign(counts[t], 0);
@@ -600,10 +619,7 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 	Merged M;
 	merge(descs, count, M);
 	const b3g_body_desc& d = M.d;
-	if (d.n_tet >= (1u << 27)) {
-		set_error("more than 2^27 - 1 tetrahedron elements");
-		return B3G_UNSUPPORTED;
-	}
+	if (!code_limits(d)) return B3G_UNSUPPORTED;
 	b3g_status st = B3G_OK;
 	drop_graph(b);
 	dfree_all(b);
@@ -780,9 +796,9 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 	b.tri.f = take(F, 3 * d.n_tri);
 	b.tri.fd = b.tri.anyDamping ? take(F, 3 * d.n_tri) : 0;
 	b.stretch.m = d.n_stretch; b.stretch.ncoef = 20;
-	b.stretch.jx = take(J, 18 * d.n_stretch); b.stretch.jv = take(J, 18 * d.n_stretch); b.stretch.f = take(F, 12 * d.n_stretch);
+	b.stretch.f = take(F, 12 * d.n_stretch);
 	b.shear.m = d.n_shear; b.shear.ncoef = 16;
-	b.shear.jx = take(J, 9 * d.n_shear); b.shear.jv = take(J, 9 * d.n_shear); b.shear.f = take(F, 6 * d.n_shear);
+	b.shear.f = take(F, 6 * d.n_shear);
 	b.spring.m = d.n_spring; b.spring.np = 2; b.spring.ncoef = 3;
 	b.spring.jx = take(J, 4 * d.n_spring); b.spring.jv = take(J, 4 * d.n_spring); b.spring.f = take(F, 4 * d.n_spring);
 	b.mouse.m = d.n_mouse; b.mouse.np = 4; b.mouse.ncoef = 6;
@@ -794,6 +810,10 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 	b.nWorld = d.n_world;
 	b.hostWorlds.clear();
 	std::vector<float> vox;
+	// A grid passed by several fixtures (e.g. every instance of a batch over
+	// one terrain) is uploaded once: fixtures with the same voxel pointer and
+	// dimensions share one offset.
+	std::vector<std::pair<const float*, std::pair<size_t, uint64_t>>> voxSeen;
 	for (uint32_t c = d.n_world; c-- > 0;) {
 		const float* q = d.w_param + (size_t)B3G_WORLD_PARAMS * c;
 		WorldShape w;
@@ -817,9 +837,17 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 			w.W = (uint32_t)q[7]; w.H = (uint32_t)q[8]; w.D = (uint32_t)q[9];
 			w.xf.p = load3(q + 10); w.xf.q.v = load3(q + 13); w.xf.q.s = q[16];
 			w.invert = q[17] != 0.0f;
-			w.voxOffset = vox.size();
 			size_t nv = (size_t)w.W * w.H * w.D;
-			vox.insert(vox.end(), d.w_voxels[c], d.w_voxels[c] + nv);
+			auto seen = std::find_if(voxSeen.begin(), voxSeen.end(), [&](const auto& e) {
+				return e.first == d.w_voxels[c] && e.second.first == nv;
+			});
+			if (seen != voxSeen.end()) {
+				w.voxOffset = seen->second.second;
+			} else {
+				w.voxOffset = vox.size();
+				voxSeen.push_back({d.w_voxels[c], {nv, w.voxOffset}});
+				vox.insert(vox.end(), d.w_voxels[c], d.w_voxels[c] + nv);
+			}
 			break;
 		}
 		default: break;
@@ -860,8 +888,8 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 	si.moP = &moP; si.moM = d.n_mouse; si.moCoef = &moCoef;
 	si.tetF = b.tet.f; si.tetFd = b.tet.fd;
 	si.triJx = b.tri.jx; si.triJv = b.tri.jv; si.triF = b.tri.f; si.triFd = b.tri.fd;
-	si.stJx = b.stretch.jx; si.stJv = b.stretch.jv; si.stF = b.stretch.f;
-	si.shJx = b.shear.jx; si.shJv = b.shear.jv; si.shF = b.shear.f;
+	si.stF = b.stretch.f;
+	si.shF = b.shear.f;
 	si.spJx = b.spring.jx; si.spJv = b.spring.jv; si.spF = b.spring.f;
 	si.moJx = b.mouse.jx; si.moJv = b.mouse.jv; si.moF = b.mouse.f;
 	Pattern P;
@@ -970,8 +998,33 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 		UP(b.tri.K0, planes(K0, d.n_tri, 36));
 		UP(b.tri.kd, triKd);
 	}
-	if (d.n_stretch) { UP(b.stretch.p, planes(stP, d.n_stretch, 3)); UP(b.stretch.c, planes(stCoef, d.n_stretch, 20)); }
-	if (d.n_shear) { UP(b.shear.p, planes(shP, d.n_shear, 3)); UP(b.shear.c, planes(shCoef, d.n_shear, 16)); }
+	if (d.n_stretch) {
+		UP(b.stretch.p, planes(stP, d.n_stretch, 3));
+		UP(b.stretch.c, planes(stCoef, d.n_stretch, 20));
+		// records per part: (n_w, |w|) | (dw/dx, alpha) | (ks, kd, b, 0)
+		std::vector<float4> rec(6 * (size_t)d.n_stretch, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+		for (size_t e = 0; e < d.n_stretch; ++e) {
+			const float* c = &stCoef[20 * e];
+			for (int part = 0; part < 2; ++part) {
+				const float* dw = c + 6 + 3 * part;
+				rec[6 * e + 3 * part + 1] = make_float4(dw[0], dw[1], dw[2], c[0]);
+				rec[6 * e + 3 * part + 2] = make_float4(c[12 + 3 * part], c[13 + 3 * part], c[14 + 3 * part], 0.0f);
+			}
+		}
+		UP(b.stretch.rec, rec);
+	}
+	if (d.n_shear) {
+		UP(b.shear.p, planes(shP, d.n_shear, 3));
+		UP(b.shear.c, planes(shCoef, d.n_shear, 16));
+		// records: (w_u, alpha) | (w_v, ks) | (dw_u/dx, kd) | (dw_v/dx, 0)
+		std::vector<float4> rec(4 * (size_t)d.n_shear, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+		for (size_t e = 0; e < d.n_shear; ++e) {
+			const float* c = &shCoef[16 * e];
+			rec[4 * e + 2] = make_float4(c[6], c[7], c[8], c[13]);
+			rec[4 * e + 3] = make_float4(c[9], c[10], c[11], 0.0f);
+		}
+		UP(b.shear.rec, rec);
+	}
 	if (d.n_spring) { UP(b.spring.p, planes(spP, d.n_spring, 2)); UP(b.spring.c, planes(spCoef, d.n_spring, 3)); }
 	if (d.n_mouse) { UP(b.mouse.p, planes(moP, d.n_mouse, 4)); UP(b.mouse.c, planes(moCoef, d.n_mouse, 6)); }
 	AL(b.jstage, (size_t)std::max(1u, b.nJStage) * sizeof(M33));
@@ -1079,7 +1132,7 @@ b3g_status host_pattern(const b3g_body_desc& d, std::vector<uint32_t>& rowptr, s
 	si.spP = &spP; si.spM = d.n_spring; si.spCoef = &spCoef;
 	si.moP = &moP; si.moM = d.n_mouse; si.moCoef = &moCoef;
 	si.tetF = si.tetFd = si.triJx = si.triJv = si.triF = si.triFd = 0;
-	si.stJx = si.stJv = si.stF = si.shJx = si.shJv = si.shF = si.spJx = si.spJv = si.spF = si.moJx = si.moJv = si.moF = 0;
+	si.stF = si.shF = si.spJx = si.spJv = si.spF = si.moJx = si.moJv = si.moF = 0;
 	Pattern P;
 	build_pattern(si, P);
 	rowptr = P.rowptr;

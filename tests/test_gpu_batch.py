@@ -138,3 +138,39 @@ def test_fused_reductions_equal_separate_finalize():
         xb, vb = b.state()
         assert np.array_equal(xa, xb) and np.array_equal(va, vb)
     assert [s[0].tolist() for s in a.instance_stats()] == [s[0].tolist() for s in b.instance_stats()]
+
+
+def test_contact_workload_batch_equals_oracle():
+    """The contact workload at small size (3 bodies of cloth + tet block over one shared SDF grid): in
+    sequential mode every instance reproduces the oracle bit for bit, contacts and normal forces included,
+    and the FP64-reduced batch equals standalone bodies bit for bit."""
+    import oracle
+    arrs = configs.contact_batch(3, cloth_cells=11, tet_cells=3, sdf_res=24)
+    assert len({a["w_voxels"][0].ctypes.data for a in arrs}) == 1   # one grid, uploaded once
+    batch = Body(arrs)
+    batch.set_reduction(_lib.REDUCE_SEQUENTIAL)
+    refs = [oracle.OracleBody(a) for a in arrs]
+    touching = 0
+    for step in range(15):
+        batch.Step(configs.DT, 1, 30, 0.0, 1e-5)
+        for o in refs:
+            o.step(configs.DT, 1, 30, 0.0, 1e-5)
+        xb, vb = batch.state()
+        for k, (xs, vs) in enumerate(zip(batch.split(xb), batch.split(vb))):
+            xo, vo = refs[k].state()
+            assert np.array_equal(xs, xo) and np.array_equal(vs, vo), f"instance {k} differs at step {step}"
+    bp, bw, bf = batch.contacts()
+    p_off = w_off = 0
+    exp = [[], [], []]
+    for o, a in zip(refs, arrs):
+        p, w, f = o.contacts()
+        exp[0].append(p + p_off)
+        exp[1].append(w + w_off)
+        exp[2].append(f)
+        touching += int(np.count_nonzero(f))
+        p_off += len(a["p_type"])
+        w_off += len(a["w_type"])
+    assert np.array_equal(bp, np.concatenate(exp[0])) and np.array_equal(bw, np.concatenate(exp[1]))
+    assert np.array_equal(bf, np.concatenate(exp[2]))
+    assert touching > 0
+    run_both(arrs, _lib.REDUCE_FP64, steps=6, it=1, sub=30, stol=1e-5)
