@@ -112,7 +112,7 @@ EXPORTS = [
     "b3g_body_build", "b3g_body_build_batch", "b3g_get_instance_count", "b3g_step", "b3g_get_state", "b3g_set_state", "b3g_apply_external", "b3g_get_masses",
     "b3g_get_stats", "b3g_get_energy", "b3g_get_contacts", "b3g_get_tet_quats", "b3g_assemble", "b3g_get_system",
     "b3g_synchronize", "b3g_time_spmv", "b3g_set_solver", "b3g_set_graph", "b3g_get_system_size", "b3g_pattern",
-    "b3g_solve_cg", "b3g_spmv", "b3g_set_reduction", "b3g_save_state", "b3g_load_state",
+    "b3g_solve_cg", "b3g_spmv", "b3g_set_reduction", "b3g_save_state", "b3g_load_state", "b3g_ray_cast",
 ]
 
 
@@ -134,6 +134,7 @@ def lib():
         "b3g_body_build": (C.c_int, [vp, C.POINTER(BodyDesc)]),
         "b3g_body_build_batch": (C.c_int, [vp, C.POINTER(BodyDesc), C.c_uint32]),
         "b3g_get_instance_count": (C.c_int, [vp, u32p]),
+        "b3g_ray_cast": (C.c_int, [vp, C.c_uint32, C.c_uint32, f32p, f32p, C.POINTER(C.c_int64), f32p, f32p]),
         "b3g_step": (C.c_int, [vp, C.c_float, C.c_uint32, C.c_uint32, C.c_float, C.c_float]),
         "b3g_get_state": (C.c_int, [vp, f32p, f32p]),
         "b3g_set_state": (C.c_int, [vp, f32p, f32p]),

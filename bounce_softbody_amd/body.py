@@ -206,6 +206,20 @@ class Body:
         buf = C.create_string_buffer(blob, len(blob))
         check(self._L.b3g_load_state(self._h, buf, len(blob)), "b3g_load_state")
 
+    def RayCastSingle(self, p1, p2, instance: int = 0):  # noqa: N802 (reference name)
+        """``b3Body::RayCastSingle`` for each ray p1[i] -> p2[i] (arrays of shape (n, 3)) against the triangle
+        fixtures of ``instance``: (creation index of the fixture hit or -1, fraction, normal)."""
+        a = np.ascontiguousarray(p1, f32).reshape(-1, 3)
+        b = np.ascontiguousarray(p2, f32).reshape(-1, 3)
+        n = len(a)
+        idx = np.zeros(max(1, n), np.int64)
+        frac = np.zeros(max(1, n), f32)
+        nrm = np.zeros((max(1, n), 3), f32)
+        check(self._L.b3g_ray_cast(self._h, instance, n, _ptr(a, f32p), _ptr(b, f32p),
+                                   idx.ctypes.data_as(C.POINTER(C.c_int64)), _ptr(frac, f32p), _ptr(nrm, f32p)),
+              "b3g_ray_cast")
+        return idx[:n], frac[:n], nrm[:n]
+
     def time_spmv(self, launches: int = 100):
         """Mean duration (ms) of the PCG's SpMV kernel over back-to-back launches, and its algorithmic bytes."""
         ms = C.c_double(0)

@@ -136,6 +136,15 @@ b3g_status b3g_get_instance_count(b3g_body h, uint32_t* count) {
 	return B3G_OK;
 }
 
+b3g_status b3g_ray_cast(b3g_body h, uint32_t instance, uint32_t count, const float* p1, const float* p2, int64_t* index,
+                        float* fraction, float* normal) {
+	CHECK_BUILT(h);
+	if (count && (!p1 || !p2 || !index || !fraction)) { set_error("null ray or result array"); return B3G_INVALID_ARGUMENT; }
+	if (instance >= h->body.nInst) { set_error("instance out of range"); return B3G_INVALID_ARGUMENT; }
+	SET_DEVICE(h);
+	return ray_cast(h->body, instance, count, p1, p2, index, fraction, normal);
+}
+
 b3g_status b3g_step(b3g_body h, float dt, uint32_t iterations, uint32_t sub_iterations, float tolerance,
                     float sub_tolerance) {
 	CHECK_BUILT(h);
@@ -147,6 +156,7 @@ b3g_status b3g_step(b3g_body h, float dt, uint32_t iterations, uint32_t sub_iter
 	Body& b = h->body;
 	b3g_status s = prepare_step(b, sub_iterations);
 	if (s != B3G_OK) return s;
+	++b.xVersion;
 	if (!b.useGraph) return enqueue_step(b, dt, iterations, sub_iterations, tol, stol, false);
 	// Captured step: one graph per set of step parameters and modes, replayed.
 	const std::vector<double> key = {(double)dt, (double)iterations, (double)sub_iterations, (double)tol, (double)stol,
@@ -195,6 +205,7 @@ b3g_status b3g_set_state(b3g_body h, const float* pos, const float* vel) {
 	SET_DEVICE(h);
 	Body& b = h->body;
 	if (pos) {
+		++b.xVersion;
 		b3g_status s = write_v3(b, b.x, pos, false);
 		if (s != B3G_OK) return s;
 		// SetPosition clears the pending translation (particle.h:213-218)
@@ -365,10 +376,10 @@ b3g_status b3g_get_system(b3g_body h, uint32_t* rowptr, uint32_t* cols, float* v
 	if (rowptr) std::memcpy(rowptr, b.hostRowptr.data(), (b.n + 1) * 4);
 	if (cols) std::memcpy(cols, b.hostCols.data(), (size_t)b.nnz * 4);
 	if (vals) {
-		std::vector<float> A((size_t)b.nnz * 9);
+		std::vector<float> A((size_t)b.aStride * 9);
 		B3G_HIP(hipMemcpy(A.data(), b.A.p, A.size() * 4, hipMemcpyDeviceToHost));
 		for (size_t k = 0; k < b.nnz; ++k)
-			for (int c = 0; c < 9; ++c) vals[9 * k + c] = A[(size_t)c * b.nnz + k];
+			for (int c = 0; c < 9; ++c) vals[9 * k + c] = A[(size_t)c * b.aStride + k];
 	}
 	auto rows = [&](const DBuf& buf, float* out) -> b3g_status {
 		std::vector<V3> t(b.n);
@@ -465,6 +476,7 @@ b3g_status b3g_load_state(b3g_body h, const void* blob, uint64_t bytes) {
 		return B3G_INVALID_ARGUMENT;
 	}
 	B3G_HIP(hipStreamSynchronize(b.stream));
+	++b.xVersion;
 	const char* in = static_cast<const char*>(blob);
 	size_t off = sizeof(hd);
 	for (auto& p : parts) {
@@ -532,13 +544,16 @@ b3g_status matrix_body(Body& b, int device, uint32_t n, const uint32_t* rowptr, 
 		for (uint32_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
 			if (cols[k] >= n) { set_error("column index out of range"); return B3G_INVALID_ARGUMENT; }
 	}
-	std::vector<float> planes((size_t)b.nnz * 9);
+	b.aStride = plane_stride(b.nnz);
+	std::vector<float> planes((size_t)b.aStride * 9, 0.0f);
 	for (size_t k = 0; k < b.nnz; ++k)
-		for (int c = 0; c < 9; ++c) planes[(size_t)c * b.nnz + k] = vals[9 * k + c];
+		for (int c = 0; c < 9; ++c) planes[(size_t)c * b.aStride + k] = vals[9 * k + c];
+	std::vector<uint32_t> colsPad(b.aStride, 0u);
+	std::copy(cols, cols + b.nnz, colsPad.begin());
 	b3g_status s;
 	if ((s = dupload(b, b.A, planes.data(), planes.size() * 4)) != B3G_OK) return s;
 	if ((s = dupload(b, b.rowptr, rowptr, (n + 1) * 4)) != B3G_OK) return s;
-	if ((s = dupload(b, b.cols, cols, (size_t)b.nnz * 4)) != B3G_OK) return s;
+	if ((s = dupload(b, b.cols, colsPad.data(), colsPad.size() * 4)) != B3G_OK) return s;
 	for (DBuf* buf : {&b.pb, &b.invM, &b.px, &b.r, &b.s, &b.q, &b.d0, &b.d1})
 		if ((s = dalloc(b, *buf, (size_t)n * sizeof(V3))) != B3G_OK) return s;
 	if ((s = dalloc(b, b.state, sizeof(SolverState))) != B3G_OK) return s;
@@ -559,12 +574,12 @@ __global__ void k_set_state(SolverState* st, uint32_t maxIt, float eps2) {
 	st->newtonActive = 1;
 }
 
-__global__ void k_invm(uint32_t n, const uint32_t* rowptr, const uint32_t* cols, const float* A, uint32_t nnz, V3* invM) {
+__global__ void k_invm(uint32_t n, const uint32_t* rowptr, const uint32_t* cols, const float* A, uint32_t stride, V3* invM) {
 	uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	float axx = 0.0f, ayy = 0.0f, azz = 0.0f;
 	for (uint32_t k = rowptr[i]; k < rowptr[i + 1]; ++k)   // b3SparseMat33::operator() const: first match from the head
-		if (cols[k] == i) { axx = A[k]; ayy = A[(size_t)4 * nnz + k]; azz = A[(size_t)8 * nnz + k]; break; }
+		if (cols[k] == i) { axx = A[k]; ayy = A[(size_t)4 * stride + k]; azz = A[(size_t)8 * stride + k]; break; }
 	invM[i] = v3(1.0f / axx, 1.0f / ayy, 1.0f / azz);
 }
 } // namespace
@@ -582,7 +597,7 @@ b3g_status b3g_solve_cg(int device, uint32_t n, const uint32_t* rowptr, const ui
 			hipLaunchKernelGGL(k_set_state, dim3(1), dim3(1), 0, b.stream, dptr<SolverState>(b.state), max_iterations,
 			                   tolerance * tolerance);
 			hipLaunchKernelGGL(k_invm, dim3((n + 255) / 256), dim3(256), 0, b.stream, n, dptr<uint32_t>(b.rowptr),
-			                   dptr<uint32_t>(b.cols), dptr<float>(b.A), b.nnz, dptr<V3>(b.invM));
+			                   dptr<uint32_t>(b.cols), dptr<float>(b.A), b.aStride, dptr<V3>(b.invM));
 			B3G_HIP(hipGetLastError());
 			b3g_status s2 = enqueue_solve_cg(b, max_iterations, tolerance);
 			if (s2 != B3G_OK) return s2;

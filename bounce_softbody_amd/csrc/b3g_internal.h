@@ -38,6 +38,8 @@ void set_error(const std::string& msg);
 		}                                                                                              \
 	} while (0)
 
+inline uint32_t plane_stride(uint32_t nnz) { return (nnz + 3u) & ~3u; }
+
 // A device allocation owned by the body.
 struct DBuf {
 	void* p = nullptr;
@@ -154,6 +156,7 @@ struct Body {
 	// ---------------------------------------------------------------- sizes
 	uint32_t n = 0;          // particles (all instances)
 	uint32_t nnz = 0;        // 3x3 blocks of pA
+	uint32_t aStride = 0;    // plane stride of pA's nine value planes and of cols: nnz rounded up to 4
 	uint32_t nInst = 1;      // instances
 	uint32_t nSpheres = 0, nWorld = 0, nPairs = 0;
 	DBuf stampCounter;                            // uint32 on the device: FindNewContacts calls so far
@@ -252,6 +255,18 @@ struct Body {
 	hipGraphExec_t graphExec = nullptr;
 	std::vector<double> graphKey;                 // the step parameters and modes the graph was captured with
 
+	// ---------------------------------------------------------------- ray casts (triangle fixtures)
+	uint32_t nRayTri = 0;
+	std::vector<uint32_t> hostRayTri;             // per fixture: 3 particle list indices, creation index;
+	                                              // ordered by instance, then creation index
+	std::vector<uint32_t> hostRayInstBegin;       // nInst + 1 offsets into hostRayTri
+	uint64_t xVersion = 0;                        // advanced by everything that moves particles
+	uint64_t rtVersion = ~0ull;                   // xVersion the device tree was built at
+	size_t rtSortBytes = 0;
+	DBuf rtTri, rtInstBegin, rtBox, rtKey, rtKey2, rtVal, rtVal2, rtSortTemp, rtLeafTri, rtLeafBox, rtChild, rtNodeBox,
+	    rtChunk1, rtChunk2, rtChunk3, rtInstBox, rtRays, rtHits;
+	size_t rtRayCap = 0;
+
 	// ---------------------------------------------------------------- diagnostics
 	bool captureFi = false;
 
@@ -260,6 +275,9 @@ struct Body {
 
 // Allocation helpers (b3g_setup.hip)
 b3g_status dalloc(Body& b, DBuf& buf, size_t bytes);
+// b3Body::RayCastSingle for count rays against instance inst (b3g_raycast.hip).
+b3g_status ray_cast(Body& b, uint32_t inst, uint32_t count, const float* p1, const float* p2, int64_t* index,
+                    float* fraction, float* normal);
 b3g_status dupload(Body& b, DBuf& buf, const void* host, size_t bytes);
 void dfree_all(Body& b);
 

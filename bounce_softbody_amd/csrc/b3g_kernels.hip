@@ -28,17 +28,17 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t G) {
 	return start + b / 8;
 }
 
-__device__ __forceinline__ M33 ldA(const float* A, uint32_t nnz, uint32_t k) {
+__device__ __forceinline__ M33 ldA(const float* A, uint32_t stride, uint32_t k) {
 	M33 a;
 	float* p = &a.x.x;
 #pragma unroll
-	for (int c = 0; c < 9; ++c) p[c] = A[(size_t)c * nnz + k];
+	for (int c = 0; c < 9; ++c) p[c] = A[(size_t)c * stride + k];
 	return a;
 }
-__device__ __forceinline__ void stA(float* A, uint32_t nnz, uint32_t k, const M33& a) {
+__device__ __forceinline__ void stA(float* A, uint32_t stride, uint32_t k, const M33& a) {
 	const float* p = &a.x.x;
 #pragma unroll
-	for (int c = 0; c < 9; ++c) A[(size_t)c * nnz + k] = p[c];
+	for (int c = 0; c < 9; ++c) A[(size_t)c * stride + k] = p[c];
 }
 __device__ __forceinline__ M33 ldM(const float* P, uint32_t m, uint32_t plane0, uint32_t e) {
 	M33 a;
@@ -1026,6 +1026,7 @@ constexpr int kAsmCap = 1024;      // contributions per LDS chunk of the slot as
 
 struct AsmArgs {
 	uint32_t n, nnz, hasContacts;
+	uint32_t stride;             // plane stride of A
 	float h;
 	const uint32_t* slotTiles;   // slot boundaries of the slot-assembly tiles
 	const uint32_t* slotTileInst;
@@ -1298,7 +1299,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble_slots(AsmArgs a) {
 	M33 Sj = dyn_j ? mdiag(1.0f) : mzero();
 	M33 P = (Si * Aij) * Sj;
 	if (i == j) P = (P + mdiag(1.0f)) - Si;
-	stA(a.A, a.nnz, s, P);
+	stA(a.A, a.stride, s, P);
 	// (h dfdx_ij) w_j for b's product
 	a.tB[s] = (h * dx) * w_j;
 	if (a.tB64) {   // precision form: the same product in FP64 with w in FP64
@@ -1417,7 +1418,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble_rows(AsmArgs a) {
 	a.pb[i] = S * b;
 	// Jacobi preconditioner from pA(i, i) (sparse_solver.cpp:40-55)
 	uint32_t ds = a.diagSlot[i];
-	float axx = a.A[ds], ayy = a.A[(size_t)4 * a.nnz + ds], azz = a.A[(size_t)8 * a.nnz + ds];
+	float axx = a.A[ds], ayy = a.A[(size_t)4 * a.stride + ds], azz = a.A[(size_t)8 * a.stride + ds];
 	a.invM[i] = v3(1.0f / axx, 1.0f / ayy, 1.0f / azz);
 }
 
@@ -1426,6 +1427,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble_rows(AsmArgs a) {
 // ---------------------------------------------------------------------------
 struct CGArgs {
 	uint32_t n, nnz, nTiles;
+	uint32_t stride;            // plane stride of A and length of cols (nnz rounded up to 4)
 	const uint32_t* tiles;
 	const uint32_t* tileInst;
 	const uint32_t* rowBlk;
@@ -1476,6 +1478,9 @@ __device__ __forceinline__ M33 ldAb(const CGArgs& a, uint32_t k) {
 	return m;
 }
 
+// FP32 tiles read four blocks per thread with 16-byte loads.
+constexpr bool kSpmvQuad = true;
+
 template <class Gather>
 __device__ __forceinline__ V3 tile_rows(const CGArgs& a, uint32_t t, uint32_t& row, Gather gather, float* lds) {
 	const uint32_t r0 = a.tiles[t], r1 = a.tiles[t + 1];
@@ -1486,12 +1491,37 @@ __device__ __forceinline__ V3 tile_rows(const CGArgs& a, uint32_t t, uint32_t& r
 	V3 acc = zero3();
 	for (uint32_t cb = b0; cb < b1; cb += kTileCap) {
 		const uint32_t ce = min(cb + (uint32_t)kTileCap, b1);
-		for (uint32_t k = cb + threadIdx.x; k < ce; k += kBlock) {
-			M33 blk = a.bf16 ? ldAb(a, k) : ldA(a.A, a.nnz, k);
-			V3 p = blk * gather(a.cols[k]);
-			lds[3 * (k - cb)] = p.x;
-			lds[3 * (k - cb) + 1] = p.y;
-			lds[3 * (k - cb) + 2] = p.z;
+		if (a.bf16 || !kSpmvQuad) {
+			for (uint32_t k = cb + threadIdx.x; k < ce; k += kBlock) {
+				M33 blk = a.bf16 ? ldAb(a, k) : ldA(a.A, a.stride, k);
+				V3 p = blk * gather(a.cols[k]);
+				lds[3 * (k - cb)] = p.x;
+				lds[3 * (k - cb) + 1] = p.y;
+				lds[3 * (k - cb) + 2] = p.z;
+			}
+		} else {
+			// four consecutive blocks per thread: one 16-byte load per plane and
+			// one for the columns (planes and cols are padded to a multiple of 4)
+			for (uint32_t g = (cb & ~3u) + 4 * threadIdx.x; g < ce; g += 4 * kBlock) {
+				float4 pl[9];
+#pragma unroll
+				for (int c = 0; c < 9; ++c) pl[c] = *reinterpret_cast<const float4*>(a.A + (size_t)c * a.stride + g);
+				const uint4 col = *reinterpret_cast<const uint4*>(a.cols + g);
+#pragma unroll
+				for (int u = 0; u < 4; ++u) {
+					const uint32_t k = g + u;
+					if (k < cb || k >= ce) continue;
+					M33 blk;
+					float* pb = &blk.x.x;
+#pragma unroll
+					for (int c = 0; c < 9; ++c) pb[c] = u == 0 ? pl[c].x : u == 1 ? pl[c].y : u == 2 ? pl[c].z : pl[c].w;
+					const uint32_t cj = u == 0 ? col.x : u == 1 ? col.y : u == 2 ? col.z : col.w;
+					V3 p = blk * gather(cj);
+					lds[3 * (k - cb)] = p.x;
+					lds[3 * (k - cb) + 1] = p.y;
+					lds[3 * (k - cb) + 2] = p.z;
+				}
+			}
 		}
 		__syncthreads();
 		if (own) {
@@ -1657,6 +1687,7 @@ __global__ void __launch_bounds__(kBlock) k_spmv(CGArgs a, const V3* xin, V3* yo
 // ---------------------------------------------------------------------------
 struct RefineArgs {
 	uint32_t n, nnz;
+	uint32_t stride;          // plane stride of A
 	const uint32_t* tiles;
 	const uint32_t* tileInst;
 	const uint32_t* rowptr;
@@ -1696,7 +1727,7 @@ __global__ void __launch_bounds__(kBlock) k_split_bf16(RefineArgs a) {
 	if (k >= a.nnz) return;
 	const uint32_t row = a.slotRow[k];
 	if (!a.st[a.rowInst[row]].newtonActive) return;
-	M33 m = ldA(a.A, a.nnz, k);
+	M33 m = ldA(a.A, a.stride, k);
 	const float* p = &m.x.x;
 	if (a.cols[k] == row) {
 		for (int c = 0; c < 9; ++c) a.diagF[(size_t)c * a.n + row] = p[c];
@@ -1731,7 +1762,7 @@ __global__ void __launch_bounds__(kBlock) k_resid64(RefineArgs a, int first) {
 	for (uint32_t cb = b0; cb < b1; cb += cap) {
 		const uint32_t ce = min(cb + cap, b1);
 		for (uint32_t k = cb + threadIdx.x; k < ce; k += kBlock) {
-			M33 m = ldA(a.A, a.nnz, k);
+			M33 m = ldA(a.A, a.stride, k);
 			const double* xv = a.x64 + 3 * (size_t)a.cols[k];
 			const double x0 = xv[0], x1 = xv[1], x2 = xv[2];
 			lds[3 * (k - cb)] = (double)m.x.x * x0 + (double)m.y.x * x1 + (double)m.z.x * x2;
@@ -1883,13 +1914,25 @@ b3g_status ensure_partials(Body& b, uint32_t blocks) {
 
 // Room for `entries` PCG log entries per instance (a solve of maxIt
 // iterations uses maxIt + 1).
+__global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t* p, size_t count, uint32_t value) {
+	const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+	if (i < count) p[i] = value;
+}
+
+// The per-instance PCG log (zeroed when it is (re)allocated). Steps allocate
+// it before any capture, so a captured step never reaches the allocation.
 b3g_status ensure_cg_log(Body& b, uint32_t entries) {
 	if (b.cgLog.p && b.logStride >= entries) return B3G_OK;
 	const uint32_t stride = std::max(entries, 64u);
 	B3G_HIP(hipStreamSynchronize(b.stream));   // kernels in flight may still read the old log
-	b3g_status s = dalloc(b, b.cgLog, (size_t)std::max(1u, b.nInst) * stride * sizeof(CGLog));
+	const size_t bytes = (size_t)std::max(1u, b.nInst) * stride * sizeof(CGLog);
+	b3g_status s = dalloc(b, b.cgLog, bytes);
 	if (s != B3G_OK) return s;
 	b.logStride = stride;
+	const size_t words = bytes / 4;
+	hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((words + kBlock - 1) / kBlock)), dim3(kBlock), 0, b.stream,
+	                   dptr<uint32_t>(b.cgLog), words, 0u);
+	B3G_HIP(hipGetLastError());
 	return B3G_OK;
 }
 
@@ -1995,6 +2038,7 @@ static CGArgs cg_args(Body& b) {
 	CGArgs a;
 	a.n = b.n;
 	a.nnz = b.nnz;
+	a.stride = b.aStride;
 	a.nTiles = b.nTiles;
 	a.tiles = dptr<uint32_t>(b.tiles);
 	a.tileInst = dptr<uint32_t>(b.tileInst);
@@ -2085,7 +2129,7 @@ static b3g_status enqueue_linearise(Body& b, float h) {
 	ContactArgs c = contact_args(b, h);
 	if (b.nPairs) LAUNCH(k_contact_forces, nblk(b.nPairs), c);
 	AsmArgs a;
-	a.n = b.n; a.nnz = b.nnz; a.hasContacts = b.nPairs ? 1u : 0u; a.h = h;
+	a.n = b.n; a.nnz = b.nnz; a.stride = b.aStride; a.hasContacts = b.nPairs ? 1u : 0u; a.h = h;
 	a.slotTiles = dptr<uint32_t>(b.slotTiles); a.slotTileInst = dptr<uint32_t>(b.slotTileInst);
 	a.rowTiles = dptr<uint32_t>(b.tiles); a.tileInst = dptr<uint32_t>(b.tileInst);
 	a.K0x = dptr<float>(b.K0x); a.K0v = dptr<float>(b.K0v); a.nJx = b.nJx; a.nJv = b.nJv;
@@ -2182,6 +2226,7 @@ static RefineArgs refine_args(Body& b) {
 	RefineArgs r;
 	r.n = b.n;
 	r.nnz = b.nnz;
+	r.stride = b.aStride;
 	r.tiles = dptr<uint32_t>(b.tiles);
 	r.tileInst = dptr<uint32_t>(b.tileInst);
 	r.rowptr = dptr<uint32_t>(b.rowptr);
@@ -2273,6 +2318,8 @@ b3g_status enqueue_spmv(Body& b, const DBuf& xin, DBuf& yout) {
 // Back-to-back launches of the PCG's SpMV kernel (an iteration's full work,
 // on the current pA and the CG scratch vectors) between one event pair.
 b3g_status time_pcg_spmv(Body& b, uint32_t launches, double* mean_ms) {
+	b3g_status s0 = ensure_cg_log(b, 2);   // the kernel reads entry 1's beta (the log exists only after a step)
+	if (s0 != B3G_OK) return s0;
 	CGArgs a = cg_args(b);
 	hipEvent_t e0, e1;
 	B3G_HIP(hipEventCreate(&e0));

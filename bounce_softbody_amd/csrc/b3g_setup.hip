@@ -624,6 +624,9 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 	drop_graph(b);
 	dfree_all(b);
 	b.built = false;
+	b.rtVersion = ~0ull;   // the ray-cast tree is rebuilt at the next query
+	b.rtSortBytes = 0;
+	b.rtRayCap = 0;
 	const uint32_t n = d.n_particles;
 	auto L = [n](uint32_t c) { return n - 1 - c; };   // creation -> list order (body.cpp:67-75)
 	b.n = n;
@@ -689,10 +692,13 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 			std::stable_sort(srt.begin(), srt.end(), [](auto& a, auto& b2) { return a.first < b2.first; });
 			for (size_t k = 1; k < srt.size(); ++k)
 				if (srt[k].first == srt[k - 1].first) dup[srt[k].second] = 1;
+			// ray-cast table: (instance, creation index, particles), ordered by instance then creation index
+			std::vector<std::array<uint32_t, 5>> ray;
 			for (uint32_t c = d.n_trifix; c-- > 0;) {
 				if (dup[c]) continue;
 				TriF t;
 				for (int k = 0; k < 3; ++k) t.p[k] = L(d.tf_p[3 * c + k]);
+				ray.push_back({b.hostRowInst[t.p[0]], c, t.p[0], t.p[1], t.p[2]});
 				const float* v = d.tf_v + 9 * (size_t)c;
 				V3 A = load3(v), B = load3(v + 3), C = load3(v + 6);
 				t.area = 0.5f * length(cross(B - A, C - A));   // triangle_fixture.cpp:35-37
@@ -700,6 +706,16 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 				t.density = d.tf_density[c];
 				tris.push_back(t);
 			}
+			std::sort(ray.begin(), ray.end());
+			b.nRayTri = (uint32_t)ray.size();
+			b.hostRayTri.resize(4 * ray.size());
+			b.hostRayInstBegin.assign(count + 1, 0);
+			for (size_t k = 0; k < ray.size(); ++k) {
+				for (int j = 0; j < 3; ++j) b.hostRayTri[4 * k + j] = ray[k][2 + j];
+				b.hostRayTri[4 * k + 3] = ray[k][1];
+				b.hostRayInstBegin[ray[k][0] + 1] += 1;
+			}
+			for (uint32_t k = 0; k < count; ++k) b.hostRayInstBegin[k + 1] += b.hostRayInstBegin[k];
 		}
 		{
 			std::vector<char> dup(d.n_tetfix, 0);
@@ -895,6 +911,7 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 	Pattern P;
 	build_pattern(si, P);
 	b.nnz = P.rowptr[n];
+	b.aStride = plane_stride(b.nnz);
 	b.hostRowptr = P.rowptr;
 	b.hostCols = P.cols;
 	// tetrahedron entries: code kTetCode | e, and the source block of each
@@ -1031,7 +1048,11 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 	AL(b.fstage, (size_t)std::max(1u, b.nFStage) * sizeof(V3));
 	// pattern
 	UP(b.rowptr, P.rowptr);
-	UP(b.cols, P.cols);
+	{
+		std::vector<uint32_t> colsPad(P.cols);   // padded to the plane stride for 16-byte loads
+		colsPad.resize(b.aStride, 0u);
+		UP(b.cols, colsPad);
+	}
 	UP(b.slotRow, P.slotRow);
 	UP(b.diagSlot, P.diagSlot);
 	UP(b.jxPtr, P.jxPtr);
@@ -1042,7 +1063,7 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 	if (!P.bSlot.empty()) UP(b.bSlot, P.bSlot);
 	UP(b.fPtr, P.fPtr);
 	if (!P.fIdx.empty()) UP(b.fIdx, P.fIdx);
-	AL(b.A, (size_t)b.nnz * 9 * 4);
+	AL(b.A, (size_t)b.aStride * 9 * 4);
 	AL(b.tB, (size_t)b.nnz * sizeof(V3));
 	{
 		SolverState s0;

@@ -188,6 +188,23 @@ b3g_status b3g_body_build_batch(b3g_body body, const b3g_body_desc* descs, uint3
 b3g_status b3g_get_instance_count(b3g_body body, uint32_t* count);
 
 /*
+ * Replaces b3Body::RayCastSingle(output, p1, p2) (dynamics/body.cpp:497-521,
+ * declared body.h:156) for `count` rays p1[i] -> p2[i] (3 floats each) against
+ * the triangle fixtures of instance `instance` (0 for a single body), at the
+ * current positions. Per ray: index = creation index of the triangle fixture
+ * hit (the fixture b3BodyRayCastSingleOutput::triangle points to; in a batch,
+ * an index into the instances' concatenated triangle-fixture arrays) or -1,
+ * fraction and normal as b3TriangleShape::RayCast computes them
+ * (triangle_shape.cpp:314-382); `normal` may be NULL. Among equal fractions
+ * the lowest creation index wins (the reference takes whichever its tree
+ * reports first). The reference keeps its triangle tree in step with the
+ * particles after every b3Body::Step (body.cpp:584-600); here the step does no
+ * tree work and the query builds the tree when particles have moved.
+ */
+b3g_status b3g_ray_cast(b3g_body body, uint32_t instance, uint32_t count, const float* p1, const float* p2,
+                        int64_t* index, float* fraction, float* normal);
+
+/*
  * One time step. Replaces b3Body::Step(dt, forceIterations, forceSubIterations)
  * (body.h:153, body.cpp:552-604). tolerance / sub_tolerance are the Newton and
  * PCG tolerances of b3SolveBEInput (sparse_force_solver.h:55-61); a value <= 0

@@ -147,6 +147,8 @@ def _lib(which: str):
         lib[p + "solve_cg"].restype = C.c_uint32
         lib[p + "step_timed"].argtypes = [C.c_void_p, C.c_uint32, C.c_float, C.c_uint32, C.c_uint32, C.c_float, C.c_float]
         lib[p + "step_timed"].restype = C.c_double
+        i64p = C.POINTER(C.c_int64)
+        lib[p + "ray_cast"].argtypes = [C.c_void_p, C.c_uint32, f32p, f32p, i64p, f32p, f32p]
         if which == "oracle":
             lib.b3o_step.argtypes = [C.c_void_p, C.c_float, C.c_uint32, C.c_uint32, C.c_float, C.c_float]
             lib.b3o_get_state64.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -225,6 +227,19 @@ class _Body:
         q = np.zeros(4 * max(1, self.n_tet), np.float32)
         self._f("get_tet_quats")(self.h, _ptr(q, f32p))
         return q[: 4 * self.n_tet].reshape(-1, 4)
+
+    def ray_cast(self, p1, p2):
+        """``b3Body::RayCastSingle`` for each ray p1[i] -> p2[i]: (creation index of the triangle fixture hit or
+        -1, fraction, normal)."""
+        a = np.ascontiguousarray(p1, np.float32).reshape(-1, 3)
+        b = np.ascontiguousarray(p2, np.float32).reshape(-1, 3)
+        n = len(a)
+        idx = np.zeros(max(1, n), np.int64)
+        frac = np.zeros(max(1, n), np.float32)
+        nrm = np.zeros((max(1, n), 3), np.float32)
+        self._f("ray_cast")(self.h, n, _ptr(a, f32p), _ptr(b, f32p), idx.ctypes.data_as(C.POINTER(C.c_int64)),
+                            _ptr(frac, f32p), _ptr(nrm, f32p))
+        return idx[:n], frac[:n], nrm[:n]
 
     def assemble(self, dt):
         """pA (CSR, list order, head-to-tail rows), pb and fi of the first linearisation."""

@@ -71,7 +71,7 @@ template <class T> struct MouseForce {        // mouse_force.h
 };
 
 template <class T> struct SphereFixture { uint32_t p; T radius, friction; };
-template <class T> struct TriFixture { uint32_t p[3]; T area, density; };
+template <class T> struct TriFixture { uint32_t p[3]; T area, density; uint32_t index; };   // index: creation order
 template <class T> struct TetFixture { uint32_t p[4]; T volume, density; };
 
 template <class T> struct WorldFixture {
@@ -150,6 +150,10 @@ public:
 	void ApplyForces(T h, const DVec<T>& x, const DVec<T>& v, DVec<T>& f, SpMat<T>& dfdx, SpMat<T>& dfdv);
 
 	bool Collide(const WorldFixture<T>& w, V3<T> center, T radius, Manifold<T>* m) const;
+	// b3Body::RayCastSingle (body.cpp:497-521) by brute force over the
+	// triangle fixtures: the smallest hit fraction; among equal fractions the
+	// lowest creation index. Returns the fixture's creation index or -1.
+	int64_t RayCastSingle(V3<T> p1, V3<T> p2, T* fraction, V3<T>* normal) const;
 	AABB<T> WorldAABB(const WorldFixture<T>& w) const;
 
 private:
@@ -362,6 +366,7 @@ template <class T> void Body<T>::Build(const b3s_scene& sc) {
 			V3<T> N = Cross(B - A, C - A);
 			t.area = T(0.5) * Length(N);
 			t.density = T(sc.tf_density[c]);
+			t.index = c;
 			created.push_back(t);
 		}
 		triFixtures.assign(created.rbegin(), created.rend());
@@ -1371,6 +1376,57 @@ template <class T> void Body<T>::Solve(T dt, uint32_t iters, uint32_t subIters, 
 		particles[i].v = v[i];
 	}
 	for (auto& c : contacts) ApplyFriction(c, dt);
+}
+
+// b3TriangleShape::RayCast (triangle_shape.cpp:314-382); the shape's radius
+// does not enter.
+template <class T> bool RayCastTriangle(V3<T> v1, V3<T> v2, V3<T> v3, V3<T> p1, V3<T> p2, T maxFraction, T* fraction,
+                                        V3<T>* normal) {
+	V3<T> d = p2 - p1;
+	if (LengthSquared(d) < Eps<T>::value() * Eps<T>::value()) return false;
+	V3<T> n = Cross(v2 - v1, v3 - v1);
+	n.Normalize();
+	T num = Dot(n, v1 - p1);
+	T den = Dot(n, d);
+	if (den == T(0)) return false;
+	T t = num / den;
+	if (t < T(0) || maxFraction < t) return false;
+	V3<T> Q = p1 + t * d;
+	V3<T> A = v1, B = v2, C = v3;
+	V3<T> AB = B - A, AC = C - A;
+	V3<T> QA = A - Q, QB = B - Q, QC = C - Q;
+	V3<T> QB_x_QC = Cross(QB, QC), QC_x_QA = Cross(QC, QA), QA_x_QB = Cross(QA, QB);
+	V3<T> AB_x_AC = Cross(AB, AC);
+	T u = Dot(QB_x_QC, AB_x_AC), v = Dot(QC_x_QA, AB_x_AC), w = Dot(QA_x_QB, AB_x_AC);
+	if (u >= T(0) && v >= T(0) && w >= T(0)) {
+		*fraction = t;
+		*normal = num > T(0) ? -n : n;
+		return true;
+	}
+	return false;
+}
+
+// body.cpp:497-521: the body ray cast reports the triangle fixture whose
+// RayCast (triangle_fixture.cpp:58-66) gives the smallest fraction in [0, 1].
+template <class T> int64_t Body<T>::RayCastSingle(V3<T> p1, V3<T> p2, T* fraction, V3<T>* normal) const {
+	int64_t best = -1;
+	T bestT = T(0);
+	V3<T> bestN = MakeV3(T(0), T(0), T(0));
+	for (const auto& t : triFixtures) {
+		T f;
+		V3<T> n;
+		if (!RayCastTriangle(particles[t.p[0]].x, particles[t.p[1]].x, particles[t.p[2]].x, p1, p2, T(1), &f, &n)) continue;
+		if (best < 0 || f < bestT || (f == bestT && (int64_t)t.index < best)) {
+			best = t.index;
+			bestT = f;
+			bestN = n;
+		}
+	}
+	if (best >= 0) {
+		*fraction = bestT;
+		*normal = bestN;
+	}
+	return best;
 }
 
 // body.cpp:552-604

@@ -294,4 +294,21 @@ void b3o_spmv(uint32_t n, const uint32_t* rowptr, const uint32_t* cols, const fl
 	}
 }
 
+// n rays p1[i] -> p2[i] against the FP32 body's triangle fixtures:
+// index[i] = creation index of the fixture hit (-1: none), fraction, normal.
+void b3o_ray_cast(void* p, uint32_t n, const float* p1, const float* p2, int64_t* index, float* fraction, float* normal) {
+	Handle* h = (Handle*)p;
+	for (uint32_t i = 0; i < n; ++i) {
+		float f = 0.0f;
+		V3<float> nn = MakeV3(0.0f, 0.0f, 0.0f);
+		const V3<float> a = MakeV3(p1[3 * i], p1[3 * i + 1], p1[3 * i + 2]);
+		const V3<float> b = MakeV3(p2[3 * i], p2[3 * i + 1], p2[3 * i + 2]);
+		index[i] = h->f->RayCastSingle(a, b, &f, &nn);
+		fraction[i] = index[i] >= 0 ? f : 0.0f;
+		normal[3 * i] = index[i] >= 0 ? nn.x : 0.0f;
+		normal[3 * i + 1] = index[i] >= 0 ? nn.y : 0.0f;
+		normal[3 * i + 2] = index[i] >= 0 ? nn.z : 0.0f;
+	}
+}
+
 } // extern "C"

@@ -52,6 +52,7 @@ struct RefBody {
 	std::vector<b3SDF*> sdfs;
 	uint32_t nworld = 0;
 	std::vector<b3WorldFixture*> worlds;  // creation order
+	std::vector<b3TriangleFixture*> triangles;   // creation order (a duplicate maps to the fixture it repeats)
 	// captured linear system (list order)
 	std::vector<uint32_t> capRowptr, capCols;
 	std::vector<float> capVals, capB, capF;
@@ -125,9 +126,10 @@ void* refh_create(const b3s_scene* sc, int fast) {
 		td.v1 = V(sc->tf_v + 9 * c); td.v2 = V(sc->tf_v + 9 * c + 3); td.v3 = V(sc->tf_v + 9 * c + 6);
 		td.density = sc->tf_density[c]; td.radius = sc->tf_radius[c]; td.friction = sc->tf_friction[c];
 		td.userIndex = c;
-		if (!fast) { body.CreateTriangle(td); continue; }
+		if (!fast) { rb->triangles.push_back(body.CreateTriangle(td)); continue; }
 		void* mem = body.m_blockAllocator.Allocate(sizeof(b3TriangleFixture));
 		b3TriangleFixture* t = new (mem) b3TriangleFixture(td, &body);
+		rb->triangles.push_back(t);
 		t->m_proxyId = body.m_tree.CreateProxy(t->ComputeAABB(), t);
 		t->m_prev = nullptr; t->m_next = body.m_triangleList;
 		if (body.m_triangleList) body.m_triangleList->m_prev = t;
@@ -519,6 +521,24 @@ uint32_t refh_solve_cg(uint32_t n, const uint32_t* rowptr, const uint32_t* cols,
 	b3SparseSolveCG(&out, &in);
 	for (uint32_t i = 0; i < n; ++i) { x[3 * i] = xv[i].x; x[3 * i + 1] = xv[i].y; x[3 * i + 2] = xv[i].z; }
 	return out.iterations;
+}
+
+// b3Body::RayCastSingle (body.cpp:497-521) for n rays: index = creation
+// index of the triangle fixture hit (-1: none), fraction, normal.
+void refh_ray_cast(void* p, uint32_t n, const float* p1, const float* p2, int64_t* index, float* fraction, float* normal) {
+	RefBody* rb = (RefBody*)p;
+	for (uint32_t i = 0; i < n; ++i) {
+		b3BodyRayCastSingleOutput out;
+		const bool hit = rb->body.RayCastSingle(&out, V(p1 + 3 * i), V(p2 + 3 * i));
+		index[i] = -1;
+		fraction[i] = 0.0f;
+		normal[3 * i] = normal[3 * i + 1] = normal[3 * i + 2] = 0.0f;
+		if (!hit) continue;
+		for (size_t c = 0; c < rb->triangles.size(); ++c)
+			if (rb->triangles[c] == out.triangle) { index[i] = (int64_t)c; break; }
+		fraction[i] = out.fraction;
+		normal[3 * i] = out.normal.x; normal[3 * i + 1] = out.normal.y; normal[3 * i + 2] = out.normal.z;
+	}
 }
 
 } // extern "C"

@@ -11,7 +11,11 @@ it through ``oracle/ref_harness.cpp`` on the parity scenes of
   solver statistics,
 * the first linearisation (pA in list order, pb, fi) at the final state.
 
-Usage: python tests/golden/make_golden.py   (needs oracle/_ref built)
+``ray_cast.npz`` holds ``b3Body::RayCastSingle`` results of the reference
+(creation index of the triangle fixture hit or -1, fraction, normal) for
+seeded rays at the state reached after 12 steps of four scenes.
+
+Usage: python tests/golden/make_golden.py [rays]   (needs oracle/_ref built; ``rays``: ray_cast.npz only)
 """
 
 import hashlib
@@ -80,9 +84,49 @@ def run(name: str, arrays: dict, sched, body_cls=oracle.RefBody, **kw):
     }
 
 
+RAY_SCENES = ("cloth_sphere", "cloth_shapes", "tet_grid", "drop_on_box")
+
+
+def rays_for(x: np.ndarray, seed: int, n: int = 400):
+    """Seeded rays over the bounding box of x: n/2 vertical rays through the body, n/2 - 2 between random points,
+    one degenerate ray (p1 == p2) and one pointing away from the body."""
+    rng = np.random.default_rng(seed)
+    lo, hi = x.min(0) - 0.1, x.max(0) + 0.1
+    h = n // 2
+    p1 = (lo + (hi - lo) * rng.random((n, 3))).astype(np.float32)
+    p2 = (lo + (hi - lo) * rng.random((n, 3))).astype(np.float32)
+    p1[:h, 1] = hi[1] + 0.5
+    p2[:h] = p1[:h]
+    p2[:h, 1] = lo[1] - 0.5
+    p2[n - 2] = p1[n - 2]
+    p1[n - 1] = hi + 0.2
+    p2[n - 1] = hi + 1.0
+    return p1, p2
+
+
+def ray_goldens():
+    rec = {}
+    for k, name in enumerate(RAY_SCENES):
+        arrays = configs.PARITY_SCENES[name]().arrays()
+        body = oracle.RefBody(arrays)
+        for _ in range(12):
+            body.step(configs.DT, 1, 20)
+        x, v = body.state()
+        p1, p2 = rays_for(x, seed=100 + k)
+        idx, frac, nrm = body.ray_cast(p1, p2)
+        rec.update({f"{name}/x": x, f"{name}/v": v, f"{name}/p1": p1, f"{name}/p2": p2, f"{name}/index": idx,
+                    f"{name}/fraction": frac, f"{name}/normal": nrm})
+        print(f"ray_cast {name}: {int((idx >= 0).sum())} of {len(idx)} rays hit")
+    np.savez_compressed(os.path.join(OUT, "ray_cast.npz"), **rec)
+
+
 def main():
     if not oracle.reference_available():
         sys.exit("oracle/_ref/libb3ref.so is missing: run `make -C oracle ref` where the reference tree exists")
+    if sys.argv[1:] == ["rays"]:
+        ray_goldens()
+        return
+    ray_goldens()
     for name, fn in configs.PARITY_SCENES.items():
         rec = run(name, fn().arrays(), schedule(name, 30))
         np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **rec)

@@ -48,3 +48,22 @@ def test_cg_restatement_bit_exact():
     xr, itr = oracle.solve_cg("ref", rowptr, cols, vals, pb, x0, 40, 1e-6)
     xo, ito = oracle.solve_cg("oracle", rowptr, cols, vals, pb, x0, 40, 1e-6)
     assert itr == ito and np.array_equal(xr, xo)
+
+
+@pytest.mark.parametrize("name", ["cloth_sphere", "tet_grid", "drop_on_box", "cloth_element"])
+def test_ray_cast_live(name):
+    """Random rays after a few steps: the oracle's brute-force RayCastSingle equals the reference's tree query."""
+    arr = configs.PARITY_SCENES[name]().arrays()
+    ref, orc = oracle.RefBody(arr), oracle.OracleBody(arr)
+    for _ in range(8):
+        ref.step(configs.DT, 1, 20)
+        orc.step(configs.DT, 1, 20)
+    x, _ = orc.state()
+    rng = np.random.default_rng(17)
+    lo, hi = x.min(0) - 0.1, x.max(0) + 0.1
+    p1 = (lo + (hi - lo) * rng.random((1500, 3))).astype(np.float32)
+    p2 = (lo + (hi - lo) * rng.random((1500, 3))).astype(np.float32)
+    a, b = ref.ray_cast(p1, p2), orc.ray_cast(p1, p2)
+    assert (a[0] >= 0).sum() > 300
+    for u, w in zip(a, b):
+        assert np.array_equal(u, w)
