@@ -1478,9 +1478,6 @@ __device__ __forceinline__ M33 ldAb(const CGArgs& a, uint32_t k) {
 	return m;
 }
 
-// FP32 tiles read four blocks per thread with 16-byte loads.
-constexpr bool kSpmvQuad = true;
-
 template <class Gather>
 __device__ __forceinline__ V3 tile_rows(const CGArgs& a, uint32_t t, uint32_t& row, Gather gather, float* lds) {
 	const uint32_t r0 = a.tiles[t], r1 = a.tiles[t + 1];
@@ -1491,37 +1488,12 @@ __device__ __forceinline__ V3 tile_rows(const CGArgs& a, uint32_t t, uint32_t& r
 	V3 acc = zero3();
 	for (uint32_t cb = b0; cb < b1; cb += kTileCap) {
 		const uint32_t ce = min(cb + (uint32_t)kTileCap, b1);
-		if (a.bf16 || !kSpmvQuad) {
-			for (uint32_t k = cb + threadIdx.x; k < ce; k += kBlock) {
-				M33 blk = a.bf16 ? ldAb(a, k) : ldA(a.A, a.stride, k);
-				V3 p = blk * gather(a.cols[k]);
-				lds[3 * (k - cb)] = p.x;
-				lds[3 * (k - cb) + 1] = p.y;
-				lds[3 * (k - cb) + 2] = p.z;
-			}
-		} else {
-			// four consecutive blocks per thread: one 16-byte load per plane and
-			// one for the columns (planes and cols are padded to a multiple of 4)
-			for (uint32_t g = (cb & ~3u) + 4 * threadIdx.x; g < ce; g += 4 * kBlock) {
-				float4 pl[9];
-#pragma unroll
-				for (int c = 0; c < 9; ++c) pl[c] = *reinterpret_cast<const float4*>(a.A + (size_t)c * a.stride + g);
-				const uint4 col = *reinterpret_cast<const uint4*>(a.cols + g);
-#pragma unroll
-				for (int u = 0; u < 4; ++u) {
-					const uint32_t k = g + u;
-					if (k < cb || k >= ce) continue;
-					M33 blk;
-					float* pb = &blk.x.x;
-#pragma unroll
-					for (int c = 0; c < 9; ++c) pb[c] = u == 0 ? pl[c].x : u == 1 ? pl[c].y : u == 2 ? pl[c].z : pl[c].w;
-					const uint32_t cj = u == 0 ? col.x : u == 1 ? col.y : u == 2 ? col.z : col.w;
-					V3 p = blk * gather(cj);
-					lds[3 * (k - cb)] = p.x;
-					lds[3 * (k - cb) + 1] = p.y;
-					lds[3 * (k - cb) + 2] = p.z;
-				}
-			}
+		for (uint32_t k = cb + threadIdx.x; k < ce; k += kBlock) {
+			M33 blk = a.bf16 ? ldAb(a, k) : ldA(a.A, a.stride, k);
+			V3 p = blk * gather(a.cols[k]);
+			lds[3 * (k - cb)] = p.x;
+			lds[3 * (k - cb) + 1] = p.y;
+			lds[3 * (k - cb) + 2] = p.z;
 		}
 		__syncthreads();
 		if (own) {
