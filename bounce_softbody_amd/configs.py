@@ -230,6 +230,56 @@ def scene_sdf(N: int = 3, res: int = 16, invert=False) -> SceneBuilder:
     return b
 
 
+def grid_mesh(H: int, W: int):
+    """``b3GridMesh<H, W>`` (collision/geometry/grid_mesh.h:25-98): (H+1) x (W+1) vertices v(i, j) = i (W+1) + j at
+    (j - W/2, 0, i - H/2); two triangles per cell (v1 v2 v3), (v3 v4 v1)."""
+    v = np.zeros(((H + 1) * (W + 1), 3), f32)
+    for j in range(W + 1):
+        for i in range(H + 1):
+            v[i * (W + 1) + j] = (f32(j) + f32(-0.5) * f32(W), 0.0, f32(i) + f32(-0.5) * f32(H))
+    t = []
+    for i in range(H):
+        for j in range(W):
+            v1, v2, v3, v4 = i * (W + 1) + j, (i + 1) * (W + 1) + j, (i + 1) * (W + 1) + j + 1, i * (W + 1) + j + 1
+            t += [(v1, v2, v3), (v3, v4, v1)]
+    return v, np.array(t, np.uint32)
+
+
+def scene_grid_contact(R: int = 8) -> SceneBuilder:
+    """Cloth over a scaled grid mesh (grid_contact.h): b3GridMesh<5, 5> with adjacency, scale (5, 1, 5), mesh
+    radius 0.05, friction 0.5; cloth radius 0.1, friction 0.8. The mesh is tilted and shifted, and the cloth starts
+    0.25 m above it."""
+    m = GridClothMesh(R, R).scale((2.0 / R, 1.0, 2.0 / R)).translate((0.3, 0.25, -0.2))
+    b = cloth_body(m, density=0.1, radius=0.1, friction=0.8, stretching_stiffness=1000.0, shear_stiffness=100.0)
+    gv, gt = grid_mesh(5, 5)
+    b.add_world_mesh(gv, gt, scale=(5.0, 1.0, 5.0), translation=(0.1, -0.05, 0.0), rotation=(0.0, 0.0, 0.0499792, 0.9987503),
+                     radius=0.05, friction=0.5)
+    b.gravity[:] = (0.0, -9.8, 0.0)
+    b.mesh = m
+    return b
+
+
+def terrain_mesh(n: int, seed: int, size=2.0, amp=0.12):
+    """An n x n-cell height-field mesh of the given size: grid_mesh topology, heights from seeded sine bumps."""
+    v, t = grid_mesh(n, n)
+    u = splitmix64_np(seed, 6)
+    x, z = v[:, 0] * f32(size / n), v[:, 2] * f32(size / n)
+    y = amp * (np.sin(3.0 * x + 6.0 * u[0]) * np.cos(2.0 * z + 6.0 * u[1]) + 0.5 * np.sin(5.0 * (x + z) + 6.0 * u[2]))
+    return np.stack([x, y, z], 1).astype(f32), t
+
+
+def scene_mesh_terrain(N: int = 3) -> SceneBuilder:
+    """Tet block dropped on a bumpy terrain mesh (unit scale, adjacency, rotated about y)."""
+    m = GridTetMesh(N, N, N).scale(0.2).translate((-0.3, 0.2, -0.25))
+    b = tet_body(m, density=1.0, young=3000.0, poisson=0.3, radius=0.03, friction=0.6)
+    tv, tt = terrain_mesh(8, seed=31)
+    b.add_world_mesh(tv, tt, translation=(0.0, -0.05, 0.0), rotation=(0.0, 0.0998334, 0.0, 0.9950042), radius=0.01,
+                     friction=0.4)
+    b.gravity[:] = (0.0, -9.8, 0.0)
+    b.mesh = m
+    return b
+
+
 def scene_external(N: int = 2) -> SceneBuilder:
     """Free block for b3Particle::ApplyForce / ApplyTranslation."""
     m = GridTetMesh(N, N, N)
@@ -252,4 +302,6 @@ PARITY_SCENES = {
     "sdf": scene_sdf,
     "sdf_inverted": lambda: scene_sdf(invert=True),
     "external": scene_external,
+    "grid_contact": scene_grid_contact,
+    "mesh_terrain": scene_mesh_terrain,
 }

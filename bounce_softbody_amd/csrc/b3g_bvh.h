@@ -8,13 +8,18 @@
 // HPG 2012) over the current positions when they moved since the last query,
 // one sub-tree per instance of a batch, and traverses it one ray per thread.
 //
-// The functions are __host__ __device__ so that a CPU harness can run the same
-// construction and traversal.
+// The same header holds the static trees of mesh world fixtures
+// (b3MeshShape), built once on the host. The functions are __host__ __device__
+// so that a CPU harness can run the same construction and traversal.
 #pragma once
 
 #include "b3g_math.h"
 
+#include <algorithm>
 #include <cstdint>
+#include <functional>
+#include <utility>
+#include <vector>
 
 namespace b3g {
 
@@ -158,6 +163,81 @@ B3G_HD bool rt_slab(const RtBox& b, V3 p1, V3 d, float tmax) {
 		if (!(t0 <= t1)) return false;
 	}
 	return true;
+}
+
+// Host only from here.
+
+// Static tree of a mesh world fixture (the device's counterpart of
+// b3Mesh::BuildTree, mesh.cpp:26-39): leaf boxes as b3Mesh::GetTriangleAABB
+// (mesh.h:107-121: the triangle's box in mesh space grown by B3_LINEAR_SLOP),
+// leaves in Morton order of their centres, internal nodes by Karras'
+// construction, node boxes the unions of their leaves. The candidate set of a
+// query (leaves whose box overlaps the query box) is the reference tree's.
+// Appends one segment at position `base` of the tree arrays.
+struct MeshTrees {
+	std::vector<uint32_t> leaf;   // local triangle index per position
+	std::vector<RtBox> leafBox, nodeBox;
+	std::vector<uint2> child;
+};
+
+inline void build_mesh_tree(const float* verts, const uint32_t* tris, uint32_t nt, MeshTrees& out) {
+	const uint32_t base = (uint32_t)out.leaf.size();
+	std::vector<RtBox> box(nt);
+	RtBox cb = rt_empty();
+	for (uint32_t t = 0; t < nt; ++t) {
+		auto vert = [&](int j) {
+			const float* p = verts + 3 * (size_t)tris[6 * (size_t)t + j];
+			return v3(p[0], p[1], p[2]);
+		};
+		const V3 v1 = vert(0), v2 = vert(1), v3_ = vert(2);
+		const V3 lo = vmin(v1, vmin(v2, v3_)) - v3(kLinearSlop, kLinearSlop, kLinearSlop);
+		const V3 hi = vmax(v1, vmax(v2, v3_)) + v3(kLinearSlop, kLinearSlop, kLinearSlop);
+		box[t].lo = make_float4(lo.x, lo.y, lo.z, 0.0f);
+		box[t].hi = make_float4(hi.x, hi.y, hi.z, 0.0f);
+		RtBox c;
+		c.lo = c.hi = make_float4(0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z), 0.0f);
+		rt_unite(cb, c);
+	}
+	std::vector<std::pair<uint32_t, uint32_t>> order(nt);
+	for (uint32_t t = 0; t < nt; ++t) {
+		const float ex = cb.hi.x - cb.lo.x, ey = cb.hi.y - cb.lo.y, ez = cb.hi.z - cb.lo.z;
+		const float cx = 0.5f * (box[t].lo.x + box[t].hi.x), cy = 0.5f * (box[t].lo.y + box[t].hi.y),
+		            cz = 0.5f * (box[t].lo.z + box[t].hi.z);
+		order[t] = {rt_morton(ex > 0.0f ? (cx - cb.lo.x) / ex : 0.0f, ey > 0.0f ? (cy - cb.lo.y) / ey : 0.0f,
+		                      ez > 0.0f ? (cz - cb.lo.z) / ez : 0.0f), t};
+	}
+	std::stable_sort(order.begin(), order.end(), [](const auto& a, const auto& b2) { return a.first < b2.first; });
+	std::vector<uint32_t> code(nt);
+	for (uint32_t i = 0; i < nt; ++i) {
+		code[i] = order[i].first;
+		out.leaf.push_back(order[i].second);
+		out.leafBox.push_back(box[order[i].second]);
+	}
+	std::vector<uint2> child(nt, make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu));
+	for (uint32_t i = 0; i + 1 < nt; ++i) {
+		uint32_t l, r, first, last;
+		rt_karras(code.data(), i, 0, nt, l, r, first, last);
+		child[i] = make_uint2(l, r);
+	}
+	// node boxes: unions of the children, from the root down (depth <= 62)
+	std::vector<RtBox> nodeBox(nt, rt_empty());
+	std::function<RtBox(uint32_t)> boxOf = [&](uint32_t c) -> RtBox {
+		if (c & kRtLeaf) return box[order[c & ~kRtLeaf].second];
+		RtBox r = boxOf(child[c].x);
+		rt_unite(r, boxOf(child[c].y));
+		nodeBox[c] = r;
+		return r;
+	};
+	if (nt > 1) boxOf(0u);
+	for (uint32_t i = 0; i < nt; ++i) {
+		uint2 c = child[i];
+		if (c.x != 0xFFFFFFFFu) {   // positions are global: offset by the segment's base
+			c.x = (c.x & kRtLeaf) | ((c.x & ~kRtLeaf) + base);
+			c.y = (c.y & kRtLeaf) | ((c.y & ~kRtLeaf) + base);
+		}
+		out.child.push_back(c);
+		out.nodeBox.push_back(nodeBox[i]);
+	}
 }
 
 } // namespace b3g

@@ -5,6 +5,7 @@
 // dynamics/contacts/sphere_shape_contact.cpp.
 #pragma once
 
+#include "b3g_bvh.h"
 #include "b3g_internal.h"
 
 namespace b3g {
@@ -56,8 +57,118 @@ B3G_HD bool edge_or_vertex(V3 Q, V3 P, float radius, V3 triN, Manifold* m) {
 	return true;
 }
 
+// b3TriangleShape::Collide (triangle_shape.cpp:65-312) for triangle A B C of
+// radius `tradius` with optional edge-neighbour (wing) vertices.
+B3G_HD bool collide_triangle(V3 A, V3 B, V3 C, float tradius, const uint32_t hasE[3], const V3 e[3], V3 Q, float sradius,
+                             Manifold* m) {
+	float radius = tradius + sradius;
+	V3 triN = cross(B - A, C - A);
+	normalize_inplace(triN);
+	float wAB[3], wBC[3], wCA[3];
+	bary2(wAB, A, B, Q);
+	bary2(wBC, B, C, Q);
+	bary2(wCA, C, A, Q);
+	if (wAB[1] <= 0.0f && wCA[0] <= 0.0f) return edge_or_vertex(Q, A, radius, triN, m);
+	if (wAB[0] <= 0.0f && wBC[1] <= 0.0f) return edge_or_vertex(Q, B, radius, triN, m);
+	if (wBC[0] <= 0.0f && wCA[1] <= 0.0f) return edge_or_vertex(Q, C, radius, triN, m);
+	float wABC[4];
+	bary3(wABC, A, B, C, Q);
+	int edge = -1;
+	V3 P, B1, C1;
+	if (wAB[0] > 0.0f && wAB[1] > 0.0f && wABC[3] * wABC[2] <= 0.0f) {
+		edge = 0; P = (wAB[0] * A + wAB[1] * B) / wAB[2]; B1 = B; C1 = A;
+	} else if (wBC[0] > 0.0f && wBC[1] > 0.0f && wABC[3] * wABC[0] <= 0.0f) {
+		edge = 1; P = (wBC[0] * B + wBC[1] * C) / wBC[2]; B1 = C; C1 = B;
+	} else if (wCA[0] > 0.0f && wCA[1] > 0.0f && wABC[3] * wABC[1] <= 0.0f) {
+		edge = 2; P = (wCA[0] * C + wCA[1] * A) / wCA[2]; B1 = A; C1 = C;
+	}
+	if (edge >= 0) {
+		V3 d = Q - P;
+		float dd = dot(d, d);
+		if (dd > radius * radius) return false;
+		if (hasE[edge]) {   // adjacent face veto (triangle_shape.cpp:148-164)
+			float w1[4];
+			bary3(w1, e[edge], B1, C1, Q);
+			if (w1[0] > 0.0f && w1[1] > 0.0f && w1[2] > 0.0f) return false;
+		}
+		return edge_or_vertex(Q, P, radius, triN, m);
+	}
+	return edge_or_vertex(Q, (wABC[0] * A + wABC[1] * B + wABC[2] * C) / wABC[3], radius, triN, m);
+}
+
+// Static data the shapes refer to: SDF voxels and the mesh arrays.
+struct ShapeData {
+	const float* vox;
+	const V3* meshV;
+	const uint32_t* meshT;       // 6 per triangle: v1 v2 v3 u1 u2 u3
+	const uint32_t* meshLeaf;    // tree position -> local triangle index
+	const RtBox* meshLeafBox;
+	const uint2* meshChild;
+	const RtBox* meshNodeBox;
+};
+
+B3G_HD bool rt_overlap(const RtBox& a, const AABB& b) {   // b3TestOverlap (aabb.h:371-391)
+	if (a.hi.x < b.lo.x || a.lo.x > b.hi.x) return false;
+	if (a.hi.y < b.lo.y || a.lo.y > b.hi.y) return false;
+	if (a.hi.z < b.lo.z || a.lo.z > b.hi.z) return false;
+	return true;
+}
+
+// b3MeshShape::Collide (mesh_shape.cpp:87-121): the candidates are the
+// triangles whose tree box overlaps the query box around the sphere centre in
+// the unscaled mesh frame (radius in world units, as the reference has it);
+// the one whose manifold point is closest wins, ties to the lowest index.
+B3G_HD bool collide_mesh(const WorldShape& w, const ShapeData& sd, V3 Q, float sradius, Manifold* m) {
+	const V3 scale = w.a;
+	const V3 invScale = v3(1.0f / scale.x, 1.0f / scale.y, 1.0f / scale.z);
+	const V3 c0 = xmulT(w.xf, Q);
+	const V3 center = v3(invScale.x * c0.x, invScale.y * c0.y, invScale.z * c0.z);
+	const AABB query = aabb_cr(center, sradius + w.radius);
+	auto world = [&](uint32_t vi) {   // GetChildTriangle (mesh_shape.cpp:123-155)
+		const V3 v = sd.meshV[w.meshVert + vi];
+		return xmul(w.xf, v3(scale.x * v.x, scale.y * v.y, scale.z * v.z));
+	};
+	float dd0 = 3.40282347e+38f;   // B3_MAX_SCALAR
+	uint32_t best = 0xFFFFFFFFu;
+	const uint32_t s0 = w.meshNode, e0 = w.meshNode + w.meshCount;
+	uint32_t stack[kRtStack];
+	int sp = 0;
+	stack[sp++] = w.meshCount == 1 ? (kRtLeaf | s0) : s0;
+	while (sp > 0) {
+		const uint32_t c = stack[--sp];
+		if (c & kRtLeaf) {
+			const uint32_t k = c & ~kRtLeaf;
+			if (k < s0 || k >= e0 || !rt_overlap(sd.meshLeafBox[k], query)) continue;
+			const uint32_t ti = sd.meshLeaf[k];
+			const uint32_t* t = sd.meshT + 6 * (size_t)(w.meshTri + ti);
+			uint32_t hasE[3];
+			V3 e[3];
+			for (int j = 0; j < 3; ++j) {
+				hasE[j] = t[3 + j] != 0xFFFFFFFFu ? 1u : 0u;
+				e[j] = hasE[j] ? world(t[3 + j]) : zero3();
+			}
+			Manifold mm;
+			if (!collide_triangle(world(t[0]), world(t[1]), world(t[2]), w.radius, hasE, e, Q, sradius, &mm)) continue;
+			const float dd = length_sq(Q - mm.point);
+			if (dd < dd0 || (best != 0xFFFFFFFFu && dd == dd0 && ti < best)) {
+				dd0 = dd;
+				best = ti;
+				*m = mm;
+			}
+			continue;
+		}
+		if (c < s0 || c + 1 >= e0 || !rt_overlap(sd.meshNodeBox[c], query)) continue;
+		if (sp + 2 > kRtStack) break;   // unreachable: the tree depth is at most 62
+		const uint2 ch = sd.meshChild[c];
+		stack[sp++] = ch.y;
+		stack[sp++] = ch.x;
+	}
+	return best != 0xFFFFFFFFu;
+}
+
 // b3Shape::Collide for each supported shape (shape.h:67).
-B3G_HD bool collide(const WorldShape& w, const float* vox, V3 Q, float sradius, Manifold* m) {
+B3G_HD bool collide(const WorldShape& w, const ShapeData& sd, V3 Q, float sradius, Manifold* m) {
+	const float* vox = sd.vox;
 	switch (w.type) {
 	case B3G_SHAPE_SPHERE: {                       // sphere_shape.cpp:50-77
 		V3 center = w.a;
@@ -102,42 +213,10 @@ B3G_HD bool collide(const WorldShape& w, const float* vox, V3 Q, float sradius, 
 		m->normal = n;
 		return true;
 	}
-	case B3G_SHAPE_TRIANGLE: {                     // triangle_shape.cpp:65-312
-		V3 A = w.a, B = w.b, C = w.c;
-		float radius = w.radius + sradius;
-		V3 triN = cross(B - A, C - A);
-		normalize_inplace(triN);
-		float wAB[3], wBC[3], wCA[3];
-		bary2(wAB, A, B, Q);
-		bary2(wBC, B, C, Q);
-		bary2(wCA, C, A, Q);
-		if (wAB[1] <= 0.0f && wCA[0] <= 0.0f) return edge_or_vertex(Q, A, radius, triN, m);
-		if (wAB[0] <= 0.0f && wBC[1] <= 0.0f) return edge_or_vertex(Q, B, radius, triN, m);
-		if (wBC[0] <= 0.0f && wCA[1] <= 0.0f) return edge_or_vertex(Q, C, radius, triN, m);
-		float wABC[4];
-		bary3(wABC, A, B, C, Q);
-		int edge = -1;
-		V3 P, B1, C1;
-		if (wAB[0] > 0.0f && wAB[1] > 0.0f && wABC[3] * wABC[2] <= 0.0f) {
-			edge = 0; P = (wAB[0] * A + wAB[1] * B) / wAB[2]; B1 = B; C1 = A;
-		} else if (wBC[0] > 0.0f && wBC[1] > 0.0f && wABC[3] * wABC[0] <= 0.0f) {
-			edge = 1; P = (wBC[0] * B + wBC[1] * C) / wBC[2]; B1 = C; C1 = B;
-		} else if (wCA[0] > 0.0f && wCA[1] > 0.0f && wABC[3] * wABC[1] <= 0.0f) {
-			edge = 2; P = (wCA[0] * C + wCA[1] * A) / wCA[2]; B1 = A; C1 = C;
-		}
-		if (edge >= 0) {
-			V3 d = Q - P;
-			float dd = dot(d, d);
-			if (dd > radius * radius) return false;
-			if (w.hasE[edge]) {   // adjacent face veto (triangle_shape.cpp:148-164)
-				float w1[4];
-				bary3(w1, w.e[edge], B1, C1, Q);
-				if (w1[0] > 0.0f && w1[1] > 0.0f && w1[2] > 0.0f) return false;
-			}
-			return edge_or_vertex(Q, P, radius, triN, m);
-		}
-		return edge_or_vertex(Q, (wABC[0] * A + wABC[1] * B + wABC[2] * C) / wABC[3], radius, triN, m);
-	}
+	case B3G_SHAPE_TRIANGLE:                       // triangle_shape.cpp:65-312
+		return collide_triangle(w.a, w.b, w.c, w.radius, w.hasE, w.e, Q, sradius, m);
+	case B3G_SHAPE_MESH:                           // mesh_shape.cpp:87-121
+		return collide_mesh(w, sd, Q, sradius, m);
 	case B3G_SHAPE_BOX: {                          // box_shape.cpp:63-142
 		V3 e = w.a;
 		float radius = w.radius + sradius;

@@ -111,6 +111,10 @@ struct WorldShape {
 	uint32_t invert;
 	uint64_t voxOffset;      // float offset of this grid in the voxel buffer
 	AABB aabb;               // b3Shape::ComputeAABB of the fixture (static)
+	// mesh (scale in a): the mesh's vertices start at meshVert, its triangles
+	// (6 words each) at meshTri; its tree's leaves and internal nodes are
+	// positions [meshNode, meshNode + meshCount) of the mesh-tree arrays
+	uint32_t meshVert, meshTri, meshNode, meshCount;
 };
 
 // Per-instance, per-iteration scalars of one PCG solve (sparse_solver.cpp:57-103):
@@ -239,6 +243,10 @@ struct Body {
 	// ---------------------------------------------------------------- contacts (sphere x world pairs of one instance)
 	DBuf worlds;     // WorldShape[nWorld]
 	DBuf voxels;     // float
+	// mesh world fixtures: vertices (V3), triangles (6 words), and per tree
+	// position: leaf triangle (local index), leaf box, internal node children
+	// and box (b3g_bvh.h layout, one segment per mesh)
+	DBuf meshV, meshT, meshLeaf, meshLeafBox, meshChild, meshNodeBox;
 	DBuf pairStart;                               // uint32 per sphere + 1: its pairs, one per world fixture of its instance
 	DBuf pairSphere, pairWorld;                   // uint32 per pair
 	std::vector<uint32_t> hostPairStart, hostPairSphere, hostPairWorld;

@@ -872,7 +872,7 @@ __global__ void __launch_bounds__(kBlock) k_mouse_forces(PairArgs a) {
 struct ContactArgs {
 	uint32_t nSpheres, nPairs;
 	const WorldShape* worlds;
-	const float* vox;
+	ShapeData shapes;        // SDF voxels, mesh arrays
 	const uint32_t* sphereParticle;
 	const float* sphereRadius;
 	const float* sphereFriction;
@@ -945,7 +945,7 @@ __global__ void __launch_bounds__(kBlock) k_contact_forces(ContactArgs a) {
 	V3 x1 = a.x[p];
 	float r1 = a.sphereRadius[s], r2 = ws.radius;
 	Manifold m2;
-	if (!collide(ws, a.vox, x1, r1, &m2)) return;
+	if (!collide(ws, a.shapes, x1, r1, &m2)) return;
 	if (!a.applyFriction[k]) {
 		a.manifold[2 * k] = m2.point;
 		a.manifold[2 * k + 1] = m2.normal;
@@ -1980,7 +1980,13 @@ static ContactArgs contact_args(Body& b, float dt) {
 	c.nSpheres = b.nSpheres;
 	c.nPairs = b.nPairs;
 	c.worlds = dptr<WorldShape>(b.worlds);
-	c.vox = dptr<float>(b.voxels);
+	c.shapes.vox = dptr<float>(b.voxels);
+	c.shapes.meshV = dptr<V3>(b.meshV);
+	c.shapes.meshT = dptr<uint32_t>(b.meshT);
+	c.shapes.meshLeaf = dptr<uint32_t>(b.meshLeaf);
+	c.shapes.meshLeafBox = dptr<RtBox>(b.meshLeafBox);
+	c.shapes.meshChild = dptr<uint2>(b.meshChild);
+	c.shapes.meshNodeBox = dptr<RtBox>(b.meshNodeBox);
 	c.sphereParticle = dptr<uint32_t>(b.sphereParticle);
 	c.sphereRadius = dptr<float>(b.sphereRadius);
 	c.sphereFriction = dptr<float>(b.sphereFriction);

@@ -14,6 +14,7 @@
 namespace b3g {
 
 constexpr float kEps = 1.1920928955078125e-7f;   // FLT_EPSILON (settings.h:54)
+constexpr float kLinearSlop = 0.005f;             // B3_LINEAR_SLOP (settings.h:70)
 
 B3G_HD float fabs_(float x) { return x < 0.0f ? -x : (x == 0.0f ? 0.0f : x); }
 B3G_HD float fmin_(float a, float b) { return a < b ? a : b; }   // math.h:53-56

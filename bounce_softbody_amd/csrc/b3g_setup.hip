@@ -16,9 +16,11 @@
 //     first-touch order of dfdx.
 // Each pA block keeps the list of staged element contributions that the
 // reference adds into it, in the reference's order.
+#include "b3g_bvh.h"
 #include "b3g_internal.h"
 
 #include <algorithm>
+#include <functional>
 #include <cstring>
 #include <numeric>
 
@@ -479,8 +481,21 @@ b3g_status validate(const b3g_body_desc& d) {
 			if (!u) return bad("every force table row must appear once in the force order");
 	for (uint32_t w = 0; w < d.n_world; ++w) {
 		uint32_t t = d.w_type[w];
-		if (t == B3G_SHAPE_MESH) { set_error("mesh world shapes are not supported"); return B3G_UNSUPPORTED; }
 		if (t > B3G_SHAPE_SDF) return bad("unknown world shape type");
+		if (t == B3G_SHAPE_MESH) {
+			const float* q = d.w_param + (size_t)B3G_WORLD_PARAMS * w;
+			if (!d.w_mesh_vertices || !d.w_mesh_triangles || !d.w_mesh_counts || !d.w_mesh_vertices[w] ||
+			    !d.w_mesh_triangles[w])
+				return bad("mesh world fixture without vertices or triangles");
+			const uint32_t nv = d.w_mesh_counts[2 * w], nt = d.w_mesh_counts[2 * w + 1];
+			if (nv == 0 || nt == 0) return bad("mesh world fixture with no vertices or no triangles");
+			if (q[1] == 0.0f || q[2] == 0.0f || q[3] == 0.0f) return bad("mesh scale has a zero component");   // mesh_shape.cpp:94-96
+			const uint32_t* tr = d.w_mesh_triangles[w];
+			for (size_t k = 0; k < 6 * (size_t)nt; ++k) {
+				const bool wing = k % 6 >= 3;
+				if (tr[k] >= nv && !(wing && tr[k] == 0xFFFFFFFFu)) return bad("mesh triangle vertex index out of range");
+			}
+		}
 		if (t == B3G_SHAPE_SDF) {
 			const float* q = d.w_param + (size_t)B3G_WORLD_PARAMS * w;
 			if (!d.w_voxels || !d.w_voxels[w]) return bad("sdf world fixture without voxels");
@@ -512,6 +527,9 @@ struct Merged {
 	std::vector<uint32_t> w_type;
 	std::vector<float> w_param, w_friction;
 	std::vector<const float*> w_voxels;
+	std::vector<const float*> w_mesh_vertices;
+	std::vector<const uint32_t*> w_mesh_triangles;
+	std::vector<uint32_t> w_mesh_counts;
 	std::vector<uint32_t> partBegin, worldBegin;   // count + 1, creation order
 	std::vector<V3> gravity;
 };
@@ -570,7 +588,14 @@ void merge(const b3g_body_desc* ds, uint32_t count, Merged& M) {
 		app(M.w_type, d.w_type, d.n_world);
 		app(M.w_param, d.w_param, (size_t)B3G_WORLD_PARAMS * d.n_world);
 		app(M.w_friction, d.w_friction, d.n_world);
-		for (uint32_t w = 0; w < d.n_world; ++w) M.w_voxels.push_back(d.w_voxels ? d.w_voxels[w] : nullptr);
+		for (uint32_t w = 0; w < d.n_world; ++w) {
+			M.w_voxels.push_back(d.w_voxels ? d.w_voxels[w] : nullptr);
+			const bool mesh = d.w_type[w] == B3G_SHAPE_MESH;
+			M.w_mesh_vertices.push_back(mesh ? d.w_mesh_vertices[w] : nullptr);
+			M.w_mesh_triangles.push_back(mesh ? d.w_mesh_triangles[w] : nullptr);
+			M.w_mesh_counts.push_back(mesh ? d.w_mesh_counts[2 * w] : 0u);
+			M.w_mesh_counts.push_back(mesh ? d.w_mesh_counts[2 * w + 1] : 0u);
+		}
 		typeOff[B3G_FORCE_STRETCH] += d.n_stretch; typeOff[B3G_FORCE_SHEAR] += d.n_shear;
 		typeOff[B3G_FORCE_SPRING] += d.n_spring; typeOff[B3G_FORCE_MOUSE] += d.n_mouse;
 		typeOff[B3G_FORCE_TRI_ELEMENT] += d.n_tri; typeOff[B3G_FORCE_TET_ELEMENT] += d.n_tet;
@@ -600,6 +625,8 @@ void merge(const b3g_body_desc* ds, uint32_t count, Merged& M) {
 	o.n_mouse = (uint32_t)(M.mo_param.size() / 6); o.mo_p = M.mo_p.data(); o.mo_param = M.mo_param.data();
 	o.n_world = (uint32_t)M.w_type.size(); o.w_type = M.w_type.data(); o.w_param = M.w_param.data();
 	o.w_friction = M.w_friction.data(); o.w_voxels = M.w_voxels.data();
+	o.w_mesh_vertices = M.w_mesh_vertices.data(); o.w_mesh_triangles = M.w_mesh_triangles.data();
+	o.w_mesh_counts = M.w_mesh_counts.data();
 	if (count) for (int c = 0; c < 3; ++c) o.gravity[c] = ds[0].gravity[c];
 }
 
@@ -830,6 +857,12 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 	// one terrain) is uploaded once: fixtures with the same voxel pointer and
 	// dimensions share one offset.
 	std::vector<std::pair<const float*, std::pair<size_t, uint64_t>>> voxSeen;
+	// meshes: one copy (vertices, triangles, tree) per distinct (vertex array, triangle array)
+	std::vector<float> meshV;
+	std::vector<uint32_t> meshT;
+	MeshTrees trees;
+	struct MeshSeen { const float* v; const uint32_t* t; uint32_t nv, nt, vOff, tOff, node; };
+	std::vector<MeshSeen> meshSeen;
 	for (uint32_t c = d.n_world; c-- > 0;) {
 		const float* q = d.w_param + (size_t)B3G_WORLD_PARAMS * c;
 		WorldShape w;
@@ -866,9 +899,44 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 			}
 			break;
 		}
+		case B3G_SHAPE_MESH: {
+			w.a = load3(q + 1);   // scale
+			w.xf.p = load3(q + 4); w.xf.q.v = load3(q + 7); w.xf.q.s = q[10];
+			const float* mv = d.w_mesh_vertices[c];
+			const uint32_t* mt = d.w_mesh_triangles[c];
+			const uint32_t nv = d.w_mesh_counts[2 * c], nt = d.w_mesh_counts[2 * c + 1];
+			auto seen = std::find_if(meshSeen.begin(), meshSeen.end(), [&](const MeshSeen& e) {
+				return e.v == mv && e.t == mt && e.nv == nv && e.nt == nt;
+			});
+			if (seen == meshSeen.end()) {
+				MeshSeen e{mv, mt, nv, nt, (uint32_t)(meshV.size() / 3), (uint32_t)(meshT.size() / 6), (uint32_t)trees.leaf.size()};
+				meshV.insert(meshV.end(), mv, mv + 3 * (size_t)nv);
+				meshT.insert(meshT.end(), mt, mt + 6 * (size_t)nt);
+				build_mesh_tree(mv, mt, nt, trees);
+				meshSeen.push_back(e);
+				seen = meshSeen.end() - 1;
+			}
+			w.meshVert = seen->vOff;
+			w.meshTri = seen->tOff;
+			w.meshNode = seen->node;
+			w.meshCount = nt;
+			// b3MeshShape::ComputeAABB (mesh_shape.cpp:44-50, aabb.h:90-99)
+			auto world = [&](uint32_t i) {
+				const V3 v = load3(mv + 3 * (size_t)i);
+				return xmul(w.xf, v3(w.a.x * v.x, w.a.y * v.y, w.a.z * v.z));
+			};
+			w.aabb.lo = w.aabb.hi = world(0);
+			for (uint32_t i = 1; i < nv; ++i) {
+				const V3 p = world(i);
+				w.aabb.lo = vmin(w.aabb.lo, p);
+				w.aabb.hi = vmax(w.aabb.hi, p);
+			}
+			aabb_extend(w.aabb, w.radius);
+			break;
+		}
 		default: break;
 		}
-		w.aabb = shape_aabb(w);
+		if (w.type != B3G_SHAPE_MESH) w.aabb = shape_aabb(w);
 		b.hostWorlds.push_back(w);
 	}
 
@@ -1078,6 +1146,14 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 	if (b.nWorld) {
 		UP(b.worlds, b.hostWorlds);
 		if (!vox.empty()) UP(b.voxels, vox);
+		if (!meshT.empty()) {
+			UP(b.meshV, meshV);
+			UP(b.meshT, meshT);
+			UP(b.meshLeaf, trees.leaf);
+			UP(b.meshLeafBox, trees.leafBox);
+			UP(b.meshChild, trees.child);
+			UP(b.meshNodeBox, trees.nodeBox);
+		}
 	}
 	if (b.nPairs) {
 		const size_t pairs = b.nPairs;

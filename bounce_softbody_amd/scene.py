@@ -198,10 +198,11 @@ class SceneBuilder:
         self._forces.append((types, idx))
 
     # ------------------------------------------------------------ world
-    def _add_world(self, stype, params, friction, voxels=None):
+    def _add_world(self, stype, params, friction, voxels=None, mesh=None):
         q = np.zeros(WORLD_PARAMS, f32)
         q[: len(params)] = np.asarray(params, f32)
-        self._world.append((stype, q, f32(friction), None if voxels is None else np.ascontiguousarray(voxels, f32).reshape(-1)))
+        self._world.append((stype, q, f32(friction), None if voxels is None else np.ascontiguousarray(voxels, f32).reshape(-1),
+                            mesh))
         return len(self._world) - 1
 
     def add_world_sphere(self, center, radius, friction=0.5):
@@ -226,6 +227,20 @@ class SceneBuilder:
         D, H, W = vox.shape
         prm = [radius, *lower, *upper, W, H, D, *translation, *rotation, 1.0 if invert else 0.0]
         return self._add_world(SHAPE_SDF, prm, friction, vox)
+
+    def add_world_mesh(self, vertices, triangles, scale=(1, 1, 1), translation=(0, 0, 0), rotation=(0, 0, 0, 1),
+                       radius=0.0, friction=0.5, adjacency=True):
+        """``b3MeshShape`` over a ``b3Mesh`` (mesh.h:52-87): vertices (n, 3); triangles (m, 3) vertex indices, or
+        (m, 6) with the edge wing vertices u1 u2 u3 (``NULL_VERTEX``: open edge). With (m, 3) triangles the wings
+        come from :func:`mesh_adjacency` when ``adjacency`` (``b3Mesh::BuildAdjacency``), else all edges are open.
+        The arrays are passed by reference, so fixtures given the same arrays share one device copy."""
+        v = np.ascontiguousarray(vertices, f32).reshape(-1, 3)
+        t = np.asarray(triangles, np.uint32)
+        if t.shape[1] == 3:
+            wings = mesh_adjacency(t) if adjacency else np.full(t.shape, NULL_VERTEX, np.uint32)
+            t = np.concatenate([t, wings], axis=1)
+        t = np.ascontiguousarray(t, np.uint32)
+        return self._add_world(SHAPE_MESH, [radius, *scale, *translation, *rotation], friction, mesh=(v, t))
 
     # ------------------------------------------------------------ output
     def arrays(self) -> dict:
@@ -265,4 +280,31 @@ class SceneBuilder:
         out["w_param"] = (np.stack([w[1] for w in self._world]) if self._world else np.zeros((0, WORLD_PARAMS), f32)).reshape(-1)
         out["w_friction"] = np.array([w[2] for w in self._world], f32)
         out["w_voxels"] = [w[3] for w in self._world]
+        out["w_mesh_vertices"] = [None if w[4] is None else w[4][0].reshape(-1) for w in self._world]
+        out["w_mesh_triangles"] = [None if w[4] is None else w[4][1].reshape(-1) for w in self._world]
+        out["w_mesh_counts"] = np.array([[0, 0] if w[4] is None else [len(w[4][0]), len(w[4][1])] for w in self._world],
+                                        np.uint32).reshape(-1)
         return out
+
+
+NULL_VERTEX = 0xFFFFFFFF
+
+
+def mesh_adjacency(triangles) -> np.ndarray:
+    """Wing vertex of each triangle edge (mesh.cpp:41-98, b3Mesh::BuildAdjacency): for edge j (vertex j -> j + 1)
+    of triangle i1, the third vertex of the first triangle i2 != i1 (in index order, then edge order) that has
+    the reversed edge; NULL_VERTEX when there is none."""
+    t = np.asarray(triangles, np.int64).reshape(-1, 3)
+    first = {}
+    for i2 in range(len(t)):
+        for j2 in range(3):
+            key = (int(t[i2, j2]), int(t[i2, (j2 + 1) % 3]))
+            first.setdefault(key, []).append((i2, int(t[i2, (j2 + 2) % 3])))
+    out = np.full(t.shape, NULL_VERTEX, np.uint32)
+    for i1 in range(len(t)):
+        for j1 in range(3):
+            for i2, n2 in first.get((int(t[i1, (j1 + 1) % 3]), int(t[i1, j1])), []):
+                if i2 != i1:
+                    out[i1, j1] = n2
+                    break
+    return out

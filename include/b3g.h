@@ -71,7 +71,11 @@ enum { B3G_SHAPE_SPHERE = 0, B3G_SHAPE_CAPSULE = 1, B3G_SHAPE_TRIANGLE = 2, B3G_
  *                triangle [1..9] vertices, [10..12] hasE flags, [13..21] edge-neighbour vertices;
  *                box [1..3] extents, [4..6] translation, [7..10] quaternion (x y z s);
  *                sdf [1..6] voxel-grid aabb, [7..9] voxel counts W H D, [10..12] translation,
- *                    [13..16] quaternion, [17] invert; voxels[w] holds W*H*D values (index i + jW + kWH).
+ *                    [13..16] quaternion, [17] invert; voxels[w] holds W*H*D values (index i + jW + kWH);
+ *                mesh (b3MeshShape, mesh_shape.h:30-50) [1..3] scale, [4..6] translation, [7..10] quaternion;
+ *                    the b3Mesh (mesh.h:52-87) in w_mesh_vertices / w_mesh_triangles / w_mesh_counts.
+ *                    The library builds the mesh's tree (b3Mesh::BuildTree); wing vertices
+ *                    (b3Mesh::BuildAdjacency) are part of the triangles as given.
  */
 typedef struct b3g_body_desc {
 	uint32_t n_particles;
@@ -136,6 +140,14 @@ typedef struct b3g_body_desc {
 	const float* const* w_voxels;
 
 	float gravity[3];             /* b3Body::SetGravity (body.h) */
+
+	/* Mesh world fixtures, one entry per world fixture (NULL / 0 for other shapes):
+	 * w_mesh_vertices[w]: 3 floats per vertex; w_mesh_triangles[w]: 6 per triangle, b3Triangle's
+	 * v1 v2 v3 u1 u2 u3 (u = 0xFFFFFFFF, B3_NULL_VERTEX: open edge); w_mesh_counts[2 w], [2 w + 1]:
+	 * vertex and triangle counts. Fixtures passing the same arrays share one device copy. */
+	const float* const* w_mesh_vertices;
+	const uint32_t* const* w_mesh_triangles;
+	const uint32_t* w_mesh_counts;
 } b3g_body_desc;
 
 typedef struct b3g_step_stats {

@@ -45,6 +45,7 @@ class B3sScene(C.Structure):
         ("n_world", C.c_uint32), ("w_type", u32p), ("w_param", f32p), ("w_friction", f32p),
         ("w_voxels", C.POINTER(f32p)),
         ("gravity", C.c_float * 3),
+        ("w_mesh_vertices", C.POINTER(f32p)), ("w_mesh_triangles", C.POINTER(u32p)), ("w_mesh_counts", u32p),
     ]
 
 
@@ -90,6 +91,15 @@ def make_scene(arr: dict):
     keep["_vox_arr"] = vox_arr
     s.w_voxels = C.cast(vox_arr, C.POINTER(f32p))
     s.gravity = (C.c_float * 3)(*[float(g) for g in keep["gravity"]])
+    mv = keep.get("w_mesh_vertices", [None] * len(vox))
+    mt = keep.get("w_mesh_triangles", [None] * len(vox))
+    mv_arr = (f32p * max(1, len(mv)))(*[(_ptr(v, f32p) if v is not None else C.cast(None, f32p)) for v in mv])
+    mt_arr = (u32p * max(1, len(mt)))(*[(_ptr(t, u32p) if t is not None else C.cast(None, u32p)) for t in mt])
+    counts = np.ascontiguousarray(keep.get("w_mesh_counts", np.zeros(2 * len(vox), np.uint32)), np.uint32)
+    keep["_mv_arr"], keep["_mt_arr"], keep["_mesh_counts"] = mv_arr, mt_arr, counts
+    s.w_mesh_vertices = C.cast(mv_arr, C.POINTER(f32p))
+    s.w_mesh_triangles = C.cast(mt_arr, C.POINTER(u32p))
+    s.w_mesh_counts = _ptr(counts, u32p)
     return s, keep
 
 

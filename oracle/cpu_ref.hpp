@@ -85,6 +85,10 @@ template <class T> struct WorldFixture {
 	uint32_t W, H, D;             // sdf voxel counts
 	bool invert;
 	std::vector<T> vox;
+	// mesh: b3MeshShape (mesh_shape.h:30-50) over a b3Mesh (mesh.h:52-87)
+	V3<T> scale;
+	std::vector<V3<T>> meshV;
+	std::vector<uint32_t> meshT;  // 6 per triangle: v1 v2 v3 u1 u2 u3
 };
 
 template <class T> struct Manifold { V3<T> point, normal; };
@@ -492,6 +496,15 @@ template <class T> void Body<T>::Build(const b3s_scene& sc) {
 			size_t nv = (size_t)w.W * w.H * w.D;
 			w.vox.resize(nv);
 			for (size_t i = 0; i < nv; ++i) w.vox[i] = T(sc.w_voxels[c][i]);
+			break;
+		}
+		case B3S_SHAPE_MESH: {
+			w.scale = LoadV3<T>(q + 1);
+			w.xf.p = LoadV3<T>(q + 4);
+			w.xf.q.v = LoadV3<T>(q + 7); w.xf.q.s = T(q[10]);
+			const uint32_t nv = sc.w_mesh_counts[2 * c], nt = sc.w_mesh_counts[2 * c + 1];
+			for (uint32_t i = 0; i < nv; ++i) w.meshV.push_back(LoadV3<T>(sc.w_mesh_vertices[c] + 3 * i));
+			w.meshT.assign(sc.w_mesh_triangles[c], sc.w_mesh_triangles[c] + 6 * (size_t)nt);
 			break;
 		}
 		default: break;
@@ -1044,6 +1057,51 @@ template <class T> bool Body<T>::Collide(const WorldFixture<T>& w, V3<T> Q, T sr
 			return edge((wCA[0] * C + wCA[1] * A) / wCA[2], 2, A, C);
 		return finish((wABC[0] * A + wABC[1] * B + wABC[2] * C) / wABC[3]);
 	}
+	case B3S_SHAPE_MESH: {  // mesh_shape.cpp:87-121
+		// The sphere centre in the unscaled mesh frame; the query box keeps the
+		// world radius there (the tree is built over unscaled vertices).
+		V3<T> invScale = MakeV3(T(1) / w.scale.x, T(1) / w.scale.y, T(1) / w.scale.z);
+		V3<T> c0 = XMulT(w.xf, Q);
+		V3<T> center = MakeV3(invScale.x * c0.x, invScale.y * c0.y, invScale.z * c0.z);
+		AABB<T> box = AABB<T>::FromCenterRadius(center, sradius + w.radius);
+		const uint32_t nt = (uint32_t)(w.meshT.size() / 6);
+		T dd0 = std::numeric_limits<T>::max();
+		bool found = false;
+		auto world = [&](uint32_t vi) {   // GetChildTriangle (mesh_shape.cpp:123-155): xf * (scale . v)
+			V3<T> v = w.meshV[vi];
+			return XMul(w.xf, MakeV3(w.scale.x * v.x, w.scale.y * v.y, w.scale.z * v.z));
+		};
+		// Brute force over the triangles whose tree box (mesh.h:107-121: the
+		// triangle's box grown by B3_LINEAR_SLOP) overlaps the query box: the
+		// same candidates the static tree reports (static_tree.h:100-139), in
+		// index order, so among equal distances the lowest index wins.
+		for (uint32_t ti = 0; ti < nt; ++ti) {
+			const uint32_t* t = &w.meshT[6 * (size_t)ti];
+			V3<T> v1 = w.meshV[t[0]], v2 = w.meshV[t[1]], v3 = w.meshV[t[2]];
+			AABB<T> leaf = {Min(v1, Min(v2, v3)), Max(v1, Max(v2, v3))};
+			leaf.Extend(T(0.005));
+			if (leaf.hi.x < box.lo.x || leaf.lo.x > box.hi.x) continue;   // aabb.h:371-391
+			if (leaf.hi.y < box.lo.y || leaf.lo.y > box.hi.y) continue;
+			if (leaf.hi.z < box.lo.z || leaf.lo.z > box.hi.z) continue;
+			WorldFixture<T> tri;
+			tri.type = B3S_SHAPE_TRIANGLE;
+			tri.radius = w.radius;
+			tri.a = world(t[0]); tri.b = world(t[1]); tri.c = world(t[2]);
+			for (int k = 0; k < 3; ++k) {
+				tri.hasE[k] = t[3 + k] != 0xFFFFFFFFu;
+				tri.e[k] = tri.hasE[k] ? world(t[3 + k]) : MakeV3(T(0), T(0), T(0));
+			}
+			Manifold<T> mm;
+			if (!Collide(tri, Q, sradius, &mm)) continue;
+			T dd = LengthSquared(Q - mm.point);
+			if (dd < dd0) {
+				dd0 = dd;
+				*m = mm;
+				found = true;
+			}
+		}
+		return found;
+	}
 	case B3S_SHAPE_BOX: {  // box_shape.cpp:63-142
 		V3<T> e = w.a;
 		T radius = w.radius + sradius;
@@ -1134,6 +1192,18 @@ template <class T> AABB<T> Body<T>::WorldAABB(const WorldFixture<T>& w) const {
 	case B3S_SHAPE_SDF: {                                                     // sdf_shape.cpp:42-48
 		AABB<T> b = w.gridBox;
 		b.Transform(w.xf);
+		b.Extend(w.radius);
+		return b;
+	}
+	case B3S_SHAPE_MESH: {                                                    // mesh_shape.cpp:44-50, aabb.h:90-99
+		auto world = [&](V3<T> v) { return XMul(w.xf, MakeV3(w.scale.x * v.x, w.scale.y * v.y, w.scale.z * v.z)); };
+		AABB<T> b;
+		b.lo = b.hi = world(w.meshV[0]);
+		for (size_t i = 1; i < w.meshV.size(); ++i) {
+			V3<T> v = world(w.meshV[i]);
+			b.lo = Min(b.lo, v);
+			b.hi = Max(b.hi, v);
+		}
 		b.Extend(w.radius);
 		return b;
 	}

@@ -45,8 +45,16 @@ extern uint32 b3_forceSolverMaxSubIterations;
 
 namespace {
 
+// A b3Mesh with its own vertex and triangle storage (the mesh holds raw pointers).
+struct HarnessMesh {
+	std::vector<b3Vec3> v;
+	std::vector<b3Triangle> t;
+	b3Mesh mesh;
+};
+
 struct RefBody {
 	b3Body body;
+	std::vector<HarnessMesh*> meshes;   // deleted after the body (its shapes point at them)
 	std::vector<b3Particle*> particles;   // creation order
 	std::vector<b3Force*> tets;           // tet element forces, creation order
 	std::vector<b3SDF*> sdfs;
@@ -231,6 +239,7 @@ void* refh_create(const b3s_scene* sc, int fast) {
 		b3WorldFixtureDef wd;
 		wd.friction = sc->w_friction[c];
 		b3SphereShape sphere; b3CapsuleShape capsule; b3TriangleShape tri; b3BoxShape box; b3SDFShape sdf;
+		b3MeshShape meshShape;
 		switch (sc->w_type[c]) {
 		case B3S_SHAPE_SPHERE: sphere.m_radius = q[0]; sphere.m_center = V(q + 1); wd.shape = &sphere; break;
 		case B3S_SHAPE_CAPSULE: capsule.m_radius = q[0]; capsule.m_center1 = V(q + 1); capsule.m_center2 = V(q + 4); wd.shape = &capsule; break;
@@ -262,6 +271,30 @@ void* refh_create(const b3s_scene* sc, int fast) {
 			wd.shape = &sdf;
 			break;
 		}
+		case B3S_SHAPE_MESH: {   // the scene carries the wing vertices; the tree is the reference's own
+			HarnessMesh* hm = new HarnessMesh();
+			const uint32_t nv = sc->w_mesh_counts[2 * c], nt = sc->w_mesh_counts[2 * c + 1];
+			for (uint32_t i = 0; i < nv; ++i) hm->v.push_back(V(sc->w_mesh_vertices[c] + 3 * i));
+			for (uint32_t i = 0; i < nt; ++i) {
+				const uint32_t* t = sc->w_mesh_triangles[c] + 6 * (size_t)i;
+				b3Triangle tr;
+				tr.v1 = t[0]; tr.v2 = t[1]; tr.v3 = t[2]; tr.u1 = t[3]; tr.u2 = t[4]; tr.u3 = t[5];
+				hm->t.push_back(tr);
+			}
+			hm->mesh.vertexCount = nv;
+			hm->mesh.vertices = hm->v.data();
+			hm->mesh.triangleCount = nt;
+			hm->mesh.triangles = hm->t.data();
+			hm->mesh.BuildTree();
+			rb->meshes.push_back(hm);
+			meshShape.m_radius = q[0];
+			meshShape.m_mesh = &hm->mesh;
+			meshShape.m_scale = V(q + 1);
+			meshShape.m_xf.translation = V(q + 4);
+			meshShape.m_xf.rotation.v = V(q + 7); meshShape.m_xf.rotation.s = q[10];
+			wd.shape = &meshShape;
+			break;
+		}
 		default: break;
 		}
 		rb->worlds.push_back(body.CreateFixture(wd));
@@ -274,8 +307,10 @@ void* refh_create(const b3s_scene* sc, int fast) {
 void refh_destroy(void* p) {
 	RefBody* rb = (RefBody*)p;
 	std::vector<b3SDF*> sdfs = rb->sdfs;
+	std::vector<HarnessMesh*> meshes = rb->meshes;
 	delete rb;
 	for (b3SDF* s : sdfs) delete s;
+	for (HarnessMesh* m : meshes) delete m;
 }
 
 /* The public entry point: b3Body::Step (body.cpp:552-604). */

@@ -52,6 +52,8 @@ enum {
  *   box      [1..3] extents, [4..6] translation, [7..10] quaternion (x y z s)
  *   sdf      [1..3] aabb lower, [4..6] aabb upper, [7..9] voxel counts W H D,
  *            [10..12] translation, [13..16] quaternion (x y z s), [17] invert
+ *   mesh     [1..3] scale, [4..6] translation, [7..10] quaternion (x y z s);
+ *            vertices / triangles (v1 v2 v3 u1 u2 u3) in w_mesh_*
  */
 
 typedef struct b3s_scene {
@@ -117,6 +119,11 @@ typedef struct b3s_scene {
 	const float* const* w_voxels; /* per fixture: W*H*D voxel values for SDF shapes, else NULL */
 
 	float gravity[3];
+
+	/* mesh world fixtures, per fixture (NULL / 0 for other shapes) */
+	const float* const* w_mesh_vertices;      /* 3 per vertex */
+	const uint32_t* const* w_mesh_triangles;  /* 6 per triangle: v1 v2 v3 u1 u2 u3 (0xFFFFFFFF: open edge) */
+	const uint32_t* w_mesh_counts;            /* 2 per fixture: vertices, triangles */
 } b3s_scene;
 
 #ifdef __cplusplus

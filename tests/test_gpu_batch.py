@@ -18,7 +18,7 @@ from tests.golden.make_golden import external_loads
 pytestmark = pytest.mark.gpu
 
 SCENES = ["tet_grid", "node_types", "drop_on_box", "cloth_sphere", "cloth_shapes", "cloth_element", "springs",
-          "mouse", "sdf", "sdf_inverted", "external"]
+          "mouse", "sdf", "sdf_inverted", "external", "grid_contact", "mesh_terrain"]
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -174,3 +174,13 @@ def test_contact_workload_batch_equals_oracle():
     assert np.array_equal(bf, np.concatenate(exp[2]))
     assert touching > 0
     run_both(arrs, _lib.REDUCE_FP64, steps=6, it=1, sub=30, stol=1e-5)
+
+
+def test_batch_sharing_one_mesh():
+    """Instances passing the same mesh arrays share one device copy of the mesh and its tree; every instance still
+    equals its standalone body bit for bit (sequential mode: the reference's own trajectory)."""
+    a = configs.PARITY_SCENES["grid_contact"]().arrays()
+    b = configs.PARITY_SCENES["mesh_terrain"]().arrays()
+    arrs = [a, b, a, b, a]
+    assert arrs[0]["w_mesh_vertices"][0] is arrs[2]["w_mesh_vertices"][0]
+    run_both(arrs, _lib.REDUCE_SEQUENTIAL, steps=8, it=2, sub=20)

@@ -121,7 +121,7 @@ def test_checkpoint_resume_replays_bit_exactly():
 
 def test_contact_sets_bit_exact_from_identical_state():
     """Contact pairs created and culled from the same state are identical (sphere, box, capsule, triangle, SDF)."""
-    for name in ("drop_on_box", "cloth_sphere", "cloth_shapes", "sdf", "sdf_inverted"):
+    for name in ("drop_on_box", "cloth_sphere", "cloth_shapes", "sdf", "sdf_inverted", "grid_contact", "mesh_terrain"):
         arr = configs.PARITY_SCENES[name]().arrays()
         o = oracle.OracleBody(arr)
         for _ in range(12):

@@ -57,3 +57,23 @@ def test_invalid_descriptions_are_rejected():
     bad["p_type"][0] = 7
     with pytest.raises(_lib.B3gError):
         _lib.pattern(bad)
+
+
+def test_invalid_meshes_are_rejected():
+    """Mesh world fixtures: vertex indices (wings may be 0xFFFFFFFF) must be in range, the scale non-zero
+    (mesh_shape.cpp:94-96), the mesh non-empty."""
+    from bounce_softbody_amd.scene import SceneBuilder, STATIC
+    def scene(tris, scale=(1.0, 1.0, 1.0)):
+        b = SceneBuilder()
+        b.add_particles(np.zeros((1, 3), np.float32), types=np.full(1, STATIC, np.uint8))
+        b.add_world_mesh(np.array([[0, 0, 0], [1, 0, 0], [0, 0, 1]], np.float32), tris, scale=scale)
+        return b.arrays()
+    _lib.pattern(scene(np.array([[0, 1, 2, 0xFFFFFFFF, 0xFFFFFFFF, 0xFFFFFFFF]], np.uint32)))
+    with pytest.raises(_lib.B3gError, match="out of range"):
+        _lib.pattern(scene(np.array([[0, 1, 3]], np.uint32)))
+    with pytest.raises(_lib.B3gError, match="out of range"):
+        _lib.pattern(scene(np.array([[0, 1, 2, 5, 0xFFFFFFFF, 0xFFFFFFFF]], np.uint32)))
+    with pytest.raises(_lib.B3gError, match="scale"):
+        _lib.pattern(scene(np.array([[0, 1, 2]], np.uint32), scale=(1.0, 0.0, 1.0)))
+    with pytest.raises(_lib.B3gError, match="triangles"):
+        _lib.pattern(scene(np.zeros((0, 3), np.uint32)))
