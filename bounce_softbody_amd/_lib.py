@@ -123,6 +123,7 @@ EXPORTS = [
     "b3g_get_stats", "b3g_get_energy", "b3g_get_contacts", "b3g_get_tet_quats", "b3g_assemble", "b3g_get_system",
     "b3g_synchronize", "b3g_time_spmv", "b3g_set_solver", "b3g_set_graph", "b3g_get_system_size", "b3g_pattern",
     "b3g_solve_cg", "b3g_spmv", "b3g_set_reduction", "b3g_save_state", "b3g_load_state", "b3g_ray_cast",
+    "b3g_sdf_grid", "b3g_build_sdf",
 ]
 
 
@@ -145,6 +146,9 @@ def lib():
         "b3g_body_build_batch": (C.c_int, [vp, C.POINTER(BodyDesc), C.c_uint32]),
         "b3g_get_instance_count": (C.c_int, [vp, u32p]),
         "b3g_ray_cast": (C.c_int, [vp, C.c_uint32, C.c_uint32, f32p, f32p, C.POINTER(C.c_int64), f32p, f32p]),
+        "b3g_sdf_grid": (C.c_int, [f32p, C.c_uint32, f32p, C.c_float, f32p, u32p]),
+        "b3g_build_sdf": (C.c_int, [C.c_int, f32p, C.c_uint32, u32p, C.c_uint32, C.c_uint32, f32p, C.c_float, f32p,
+                                    C.c_uint64]),
         "b3g_step": (C.c_int, [vp, C.c_float, C.c_uint32, C.c_uint32, C.c_float, C.c_float]),
         "b3g_get_state": (C.c_int, [vp, f32p, f32p]),
         "b3g_set_state": (C.c_int, [vp, f32p, f32p]),
@@ -223,3 +227,22 @@ def spmv(rowptr, cols, vals, x, device=0):
     check(lib().b3g_spmv(device, n, _ptr(rowptr, u32p), _ptr(cols, u32p), _ptr(vals, f32p), _ptr(x, f32p),
                          _ptr(y, f32p)), "b3g_spmv")
     return y.reshape(-1, 3)
+
+
+def build_sdf(vertices, triangles, cell, extension: float = 1.0, device: int = 0):
+    """``b3BuildSDF(sdf, mesh, cellSize, aabbVolumeExtension)`` on the device: (box lower, box upper, (W, H, D),
+    voxels as an array of shape (D, H, W)). ``triangles`` is (m, 3) or (m, 6) (b3Triangle with wings)."""
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1)
+    t = np.ascontiguousarray(triangles, np.uint32)
+    stride = t.shape[1]
+    c = np.ascontiguousarray(cell, np.float32).reshape(3)
+    box = np.zeros(6, np.float32)
+    dims = np.zeros(3, np.uint32)
+    check(lib().b3g_sdf_grid(_ptr(v, f32p), len(v) // 3, _ptr(c, f32p), C.c_float(extension), _ptr(box, f32p),
+                             _ptr(dims, u32p)), "b3g_sdf_grid")
+    W, H, D = (int(x) for x in dims)
+    vox = np.zeros(W * H * D, np.float32)
+    tf = t.reshape(-1)
+    check(lib().b3g_build_sdf(device, _ptr(v, f32p), len(v) // 3, _ptr(tf, u32p), len(tf) // stride, stride,
+                              _ptr(c, f32p), C.c_float(extension), _ptr(vox, f32p), vox.size), "b3g_build_sdf")
+    return box[:3].copy(), box[3:].copy(), (W, H, D), vox.reshape(D, H, W)

@@ -280,6 +280,47 @@ def scene_mesh_terrain(N: int = 3) -> SceneBuilder:
     return b
 
 
+def icosphere(subdivisions: int = 1, radius=0.5):
+    """Closed triangle mesh of a sphere: an icosahedron whose faces are split in four `subdivisions` times,
+    vertices projected to the sphere. Returns (vertices (n, 3) float32, triangles (m, 3) uint32)."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    v = [np.array(p, np.float64) / np.linalg.norm(p) for p in v]
+    for _ in range(subdivisions):
+        mid = {}
+        def m(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in mid:
+                p = v[a] + v[b]
+                v.append(p / np.linalg.norm(p))
+                mid[k] = len(v) - 1
+            return mid[k]
+        f = [g for a, b, c in f for g in ((a, m(a, b), m(c, a)), (b, m(b, c), m(a, b)), (c, m(c, a), m(b, c)),
+                                          (m(a, b), m(b, c), m(c, a)))]
+    return (np.array(v) * radius).astype(f32), np.array(f, np.uint32)
+
+
+def torus_mesh(R=0.5, r=0.2, nu: int = 24, nv: int = 12):
+    """Closed triangle mesh of a torus in the x-z plane: nu x nv quads, two triangles each."""
+    v = []
+    for i in range(nu):
+        a = 2.0 * np.pi * i / nu
+        for j in range(nv):
+            b = 2.0 * np.pi * j / nv
+            v.append(((R + r * np.cos(b)) * np.cos(a), r * np.sin(b), (R + r * np.cos(b)) * np.sin(a)))
+    t = []
+    for i in range(nu):
+        for j in range(nv):
+            p, q = i * nv + j, ((i + 1) % nu) * nv + j
+            p2, q2 = i * nv + (j + 1) % nv, ((i + 1) % nu) * nv + (j + 1) % nv
+            t += [(p, q, q2), (q2, p2, p)]
+    return np.array(v, f32), np.array(t, np.uint32)
+
+
 def scene_external(N: int = 2) -> SceneBuilder:
     """Free block for b3Particle::ApplyForce / ApplyTranslation."""
     m = GridTetMesh(N, N, N)

@@ -136,6 +136,24 @@ b3g_status b3g_get_instance_count(b3g_body h, uint32_t* count) {
 	return B3G_OK;
 }
 
+b3g_status b3g_sdf_grid(const float* vertices, uint32_t vertex_count, const float cell_size[3], float aabb_extension,
+                        float box[6], uint32_t dims[3]) {
+	if (!box || !dims) { set_error("null output"); return B3G_INVALID_ARGUMENT; }
+	AABB b;
+	b3g_status s = sdf_grid(vertices, vertex_count, cell_size, aabb_extension, b, dims);
+	if (s != B3G_OK) return s;
+	box[0] = b.lo.x; box[1] = b.lo.y; box[2] = b.lo.z;
+	box[3] = b.hi.x; box[4] = b.hi.y; box[5] = b.hi.z;
+	return B3G_OK;
+}
+
+b3g_status b3g_build_sdf(int device, const float* vertices, uint32_t vertex_count, const uint32_t* triangles,
+                         uint32_t triangle_count, uint32_t triangle_stride, const float cell_size[3],
+                         float aabb_extension, float* voxels, uint64_t capacity) {
+	return build_sdf(device, vertices, vertex_count, triangles, triangle_count, triangle_stride, cell_size,
+	                 aabb_extension, voxels, capacity);
+}
+
 b3g_status b3g_ray_cast(b3g_body h, uint32_t instance, uint32_t count, const float* p1, const float* p2, int64_t* index,
                         float* fraction, float* normal) {
 	CHECK_BUILT(h);

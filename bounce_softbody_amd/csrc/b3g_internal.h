@@ -283,6 +283,10 @@ struct Body {
 
 // Allocation helpers (b3g_setup.hip)
 b3g_status dalloc(Body& b, DBuf& buf, size_t bytes);
+// b3BuildSDF on the device (b3g_sdf.hip): the grid, then the voxels.
+b3g_status sdf_grid(const float* vertices, uint32_t nv, const float cell[3], float extension, AABB& box, uint32_t dims[3]);
+b3g_status build_sdf(int device, const float* vertices, uint32_t nv, const uint32_t* triangles, uint32_t nt,
+                     uint32_t stride, const float cell[3], float extension, float* voxels, uint64_t capacity);
 // b3Body::RayCastSingle for count rays against instance inst (b3g_raycast.hip).
 b3g_status ray_cast(Body& b, uint32_t inst, uint32_t count, const float* p1, const float* p2, int64_t* index,
                     float* fraction, float* normal);

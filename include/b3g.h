@@ -217,6 +217,28 @@ b3g_status b3g_ray_cast(b3g_body body, uint32_t instance, uint32_t count, const 
                         int64_t* index, float* fraction, float* normal);
 
 /*
+ * The voxel grid b3BuildSDF(sdf, mesh, cellSize, aabbVolumeExtension) creates
+ * (collision/geometry/sdf_builder.cpp:155-174, declared sdf.h:53): the mesh's
+ * box grown by `aabb_extension`, box[0..2] lower and box[3..5] upper bound,
+ * dims = ceil(size / cell_size) + 1 voxels per axis. Host only.
+ */
+b3g_status b3g_sdf_grid(const float* vertices, uint32_t vertex_count, const float cell_size[3], float aabb_extension,
+                        float box[6], uint32_t dims[3]);
+
+/*
+ * Replaces b3BuildSDF: the signed distance of every voxel of that grid to the
+ * mesh (vertices: 3 floats each; triangles: `triangle_stride` >= 3 words each,
+ * the vertex indices first, so b3Triangle arrays pass with stride 6), negative
+ * inside, into `voxels` (W * H * D floats at i + j W + k W H, the layout of
+ * b3VoxelGrid and of the descriptor's SDF voxels). The values are the
+ * reference's bit for bit, including its inside test (5 parity rays,
+ * sdf_builder.cpp:26-78). Computed on `device`; blocks until done.
+ */
+b3g_status b3g_build_sdf(int device, const float* vertices, uint32_t vertex_count, const uint32_t* triangles,
+                         uint32_t triangle_count, uint32_t triangle_stride, const float cell_size[3],
+                         float aabb_extension, float* voxels, uint64_t capacity);
+
+/*
  * One time step. Replaces b3Body::Step(dt, forceIterations, forceSubIterations)
  * (body.h:153, body.cpp:552-604). tolerance / sub_tolerance are the Newton and
  * PCG tolerances of b3SolveBEInput (sparse_force_solver.h:55-61); a value <= 0

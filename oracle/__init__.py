@@ -159,6 +159,9 @@ def _lib(which: str):
         lib[p + "step_timed"].restype = C.c_double
         i64p = C.POINTER(C.c_int64)
         lib[p + "ray_cast"].argtypes = [C.c_void_p, C.c_uint32, f32p, f32p, i64p, f32p, f32p]
+        lib[p + "build_sdf"].argtypes = [f32p, C.c_uint32, u32p, C.c_uint32, C.c_uint32, f32p, C.c_float, f32p, u32p,
+                                         f32p, C.c_uint64]
+        lib[p + "build_sdf"].restype = C.c_uint64
         if which == "oracle":
             lib.b3o_step.argtypes = [C.c_void_p, C.c_float, C.c_uint32, C.c_uint32, C.c_float, C.c_float]
             lib.b3o_get_state64.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -352,3 +355,24 @@ def spmv(rowptr, cols, vals, x):
     x = np.ascontiguousarray(x, np.float32).reshape(-1)
     lib.b3o_spmv(n, _ptr(rowptr, u32p), _ptr(cols, u32p), _ptr(vals, f32p), _ptr(x, f32p), _ptr(y, f32p))
     return y.reshape(-1, 3)
+
+
+def build_sdf(vertices, triangles, cell, extension=1.0, which: str = "oracle"):
+    """``b3BuildSDF(sdf, mesh, cellSize, aabbVolumeExtension)`` (sdf_builder.cpp:155-174) by the oracle or the
+    reference build: (box lower (3), box upper (3), voxel counts (W, H, D), voxels as (D, H, W))."""
+    lib = _lib(which)
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1)
+    t = np.ascontiguousarray(triangles, np.uint32)
+    stride = t.shape[1]
+    t = t.reshape(-1)
+    c = np.ascontiguousarray(cell, np.float32).reshape(3)
+    box = np.zeros(6, np.float32)
+    dims = np.zeros(3, np.uint32)
+    p = "b3o_" if which == "oracle" else "refh_"
+    n = lib[p + "build_sdf"](_ptr(v, f32p), len(v) // 3, _ptr(t, u32p), len(t) // stride, stride, _ptr(c, f32p),
+                             C.c_float(extension), _ptr(box, f32p), _ptr(dims, u32p), None, 0)
+    vox = np.zeros(n, np.float32)
+    lib[p + "build_sdf"](_ptr(v, f32p), len(v) // 3, _ptr(t, u32p), len(t) // stride, stride, _ptr(c, f32p),
+                         C.c_float(extension), _ptr(box, f32p), _ptr(dims, u32p), _ptr(vox, f32p), n)
+    W, H, D = (int(x) for x in dims)
+    return box[:3].copy(), box[3:].copy(), (W, H, D), vox.reshape(D, H, W)

@@ -18,6 +18,7 @@
 #include "scene.h"
 
 #include <algorithm>
+#include <cmath>
 #include <list>
 #include <set>
 #include <utility>
@@ -1497,6 +1498,85 @@ template <class T> int64_t Body<T>::RayCastSingle(V3<T> p1, V3<T> p2, T* fractio
 		*normal = bestN;
 	}
 	return best;
+}
+
+// b3ClosestPointOnTriangle (collision/geometry/geometry.cpp:45-104).
+template <class T> V3<T> ClosestPointOnTriangle(V3<T> A, V3<T> B, V3<T> C, V3<T> Q) {
+	T wAB[3], wBC[3], wCA[3];
+	Barycentric2(wAB, A, B, Q);
+	Barycentric2(wBC, B, C, Q);
+	Barycentric2(wCA, C, A, Q);
+	if (wAB[1] <= T(0) && wCA[0] <= T(0)) return A;
+	if (wAB[0] <= T(0) && wBC[1] <= T(0)) return B;
+	if (wBC[0] <= T(0) && wCA[1] <= T(0)) return C;
+	T wABC[4];
+	Barycentric3(wABC, A, B, C, Q);
+	if (wAB[0] > T(0) && wAB[1] > T(0) && wABC[3] * wABC[2] <= T(0)) return (wAB[0] * A + wAB[1] * B) / wAB[2];
+	if (wBC[0] > T(0) && wBC[1] > T(0) && wABC[3] * wABC[0] <= T(0)) return (wBC[0] * B + wBC[1] * C) / wBC[2];
+	if (wCA[0] > T(0) && wCA[1] > T(0) && wABC[3] * wABC[1] <= T(0)) return (wCA[0] * C + wCA[1] * A) / wCA[2];
+	return (wABC[0] * A + wABC[1] * B + wABC[2] * C) / wABC[3];
+}
+
+// The voxel grid b3BuildSDF creates (sdf_builder.cpp:155-174): the mesh's box
+// (mesh.h:123-128, aabb.h:67-75) grown by `extension`, ceil(size / cell) + 1
+// voxels per axis.
+inline void SdfGrid(const float* v, uint32_t nv, const float cell[3], float extension, AABB<float>* box, uint32_t dims[3]) {
+	AABB<float> b;
+	b.lo = b.hi = LoadV3<float>(v);
+	for (uint32_t i = 1; i < nv; ++i) {
+		V3<float> p = LoadV3<float>(v + 3 * i);
+		b.lo = Min(b.lo, p);
+		b.hi = Max(b.hi, p);
+	}
+	b.Extend(extension);
+	V3<float> size = b.Dimensions();
+	dims[0] = uint32_t(std::ceil(size.x / cell[0])) + 1;
+	dims[1] = uint32_t(std::ceil(size.y / cell[1])) + 1;
+	dims[2] = uint32_t(std::ceil(size.z / cell[2])) + 1;
+	*box = b;
+}
+
+// b3BuildSDF's distances (sdf_builder.cpp:26-148): per voxel (the lower
+// corner of its cell, regular_grid_indexer.h:86-104) the distance to the
+// closest triangle, negative when at least 3 of 5 rays to fixed far points
+// (|coordinate| = the grid box's volume) cross the mesh an odd number of
+// times. Voxel (i, j, k) is stored at i + j W + k W H.
+inline void BuildSdf(const float* v, const uint32_t* tri, uint32_t nt, uint32_t stride, const AABB<float>& box,
+                     const uint32_t dims[3], float* out) {
+	const uint32_t W = dims[0], H = dims[1], D = dims[2];
+	V3<float> gd = box.Dimensions();
+	V3<float> cs = MakeV3(gd.x / float(W - 1), gd.y / float(H - 1), gd.z / float(D - 1));
+	V3<float> d = box.hi - box.lo;
+	const float far = d.x * d.y * d.z;
+	const V3<float> farPoints[5] = {MakeV3(far, 0.0f, 0.0f), MakeV3(0.0f, far, 0.0f), MakeV3(0.0f, 0.0f, far),
+	                                MakeV3(far, 0.0f, far), MakeV3(-far, far, 0.0f)};
+	auto vert = [&](uint32_t t, int k) { return LoadV3<float>(v + 3 * (size_t)tri[(size_t)stride * t + k]); };
+	for (uint32_t x = 0; x < W; ++x)
+		for (uint32_t y = 0; y < H; ++y)
+			for (uint32_t z = 0; z < D; ++z) {
+				V3<float> local = MakeV3(float(x) * cs.x, float(y) * cs.y, float(z) * cs.z);
+				V3<float> center = box.lo + local + cs / 2.0f;
+				V3<float> P = center - cs / 2.0f;
+				float best = std::numeric_limits<float>::max();
+				for (uint32_t t = 0; t < nt; ++t) {
+					V3<float> q = ClosestPointOnTriangle(vert(t, 0), vert(t, 1), vert(t, 2), P);
+					float dd = LengthSquared(q - P);
+					if (dd < best) best = dd;
+				}
+				float dist = Sqrt(best);
+				uint32_t inCount = 0;
+				for (int r = 0; r < 5 && inCount < 3; ++r) {
+					uint32_t hits = 0;
+					for (uint32_t t = 0; t < nt; ++t) {
+						float f;
+						V3<float> n;
+						if (RayCastTriangle(vert(t, 0), vert(t, 1), vert(t, 2), P, farPoints[r], 1.0f, &f, &n)) ++hits;
+					}
+					if (hits % 2 != 0) ++inCount;
+				}
+				if (inCount >= 3) dist *= -1.0f;
+				out[x + (size_t)y * W + (size_t)z * W * H] = dist;
+			}
 }
 
 // body.cpp:552-604

@@ -311,4 +311,18 @@ void b3o_ray_cast(void* p, uint32_t n, const float* p1, const float* p2, int64_t
 	}
 }
 
+// b3BuildSDF (sdf_builder.cpp:155-174): the grid (box lower/upper, voxel
+// counts) and, when `voxels` is non-null and holds `capacity` floats, the
+// voxel values. Returns the voxel count.
+uint64_t b3o_build_sdf(const float* vertices, uint32_t nv, const uint32_t* triangles, uint32_t nt, uint32_t stride,
+                       const float* cell, float extension, float* box6, uint32_t* dims, float* voxels, uint64_t capacity) {
+	AABB<float> box;
+	SdfGrid(vertices, nv, cell, extension, &box, dims);
+	box6[0] = box.lo.x; box6[1] = box.lo.y; box6[2] = box.lo.z;
+	box6[3] = box.hi.x; box6[4] = box.hi.y; box6[5] = box.hi.z;
+	const uint64_t count = (uint64_t)dims[0] * dims[1] * dims[2];
+	if (voxels && capacity >= count) BuildSdf(vertices, triangles, nt, stride, box, dims, voxels);
+	return count;
+}
+
 } // extern "C"

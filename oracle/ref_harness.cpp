@@ -23,6 +23,9 @@
 #include <new>
 #include <vector>
 #include <algorithm>
+#include <cstdio>
+#include <fcntl.h>
+#include <unistd.h>
 
 #define private public
 #define protected public
@@ -574,6 +577,42 @@ void refh_ray_cast(void* p, uint32_t n, const float* p1, const float* p2, int64_
 		fraction[i] = out.fraction;
 		normal[3 * i] = out.normal.x; normal[3 * i + 1] = out.normal.y; normal[3 * i + 2] = out.normal.z;
 	}
+}
+
+// b3BuildSDF (sdf_builder.cpp:155-174) on a b3Mesh of the given arrays
+// (triangles: `stride` words each, vertex indices first): box lower/upper,
+// voxel counts and, when `voxels` holds `capacity` floats, the voxel values
+// in the grid's order. Returns the voxel count.
+uint64_t refh_build_sdf(const float* vertices, uint32_t nv, const uint32_t* triangles, uint32_t nt, uint32_t stride,
+                        const float* cell, float extension, float* box6, uint32_t* dims, float* voxels, uint64_t capacity) {
+	HarnessMesh hm;
+	for (uint32_t i = 0; i < nv; ++i) hm.v.push_back(V(vertices + 3 * i));
+	for (uint32_t i = 0; i < nt; ++i) {
+		b3Triangle tr;
+		tr.v1 = triangles[(size_t)stride * i]; tr.v2 = triangles[(size_t)stride * i + 1]; tr.v3 = triangles[(size_t)stride * i + 2];
+		tr.u1 = tr.u2 = tr.u3 = B3_NULL_VERTEX;
+		hm.t.push_back(tr);
+	}
+	hm.mesh.vertexCount = nv; hm.mesh.vertices = hm.v.data();
+	hm.mesh.triangleCount = nt; hm.mesh.triangles = hm.t.data();
+	b3SDF sdf;
+	// b3BuildSDF logs its progress on stdout (b3Log): keep it quiet
+	fflush(stdout);
+	const int saved = dup(1), devnull = open("/dev/null", O_WRONLY);
+	if (saved >= 0 && devnull >= 0) dup2(devnull, 1);
+	b3BuildSDF(&sdf, &hm.mesh, V(cell), extension);
+	fflush(stdout);
+	if (saved >= 0 && devnull >= 0) dup2(saved, 1);
+	if (devnull >= 0) close(devnull);
+	if (saved >= 0) close(saved);
+	const b3AABB& box = sdf.GetAABB();
+	box6[0] = box.lowerBound.x; box6[1] = box.lowerBound.y; box6[2] = box.lowerBound.z;
+	box6[3] = box.upperBound.x; box6[4] = box.upperBound.y; box6[5] = box.upperBound.z;
+	dims[0] = sdf.voxelGrid.GetWidth(); dims[1] = sdf.voxelGrid.GetHeight(); dims[2] = sdf.voxelGrid.GetDepth();
+	const uint64_t count = (uint64_t)dims[0] * dims[1] * dims[2];
+	if (voxels && capacity >= count)
+		for (uint64_t i = 0; i < count; ++i) voxels[i] = sdf.voxelGrid.GetVoxel((uint32)i);
+	return count;
 }
 
 } // extern "C"

@@ -11,11 +11,14 @@ it through ``oracle/ref_harness.cpp`` on the parity scenes of
   solver statistics,
 * the first linearisation (pA in list order, pb, fi) at the final state.
 
+``sdf_build.npz`` holds ``b3BuildSDF`` grids of the reference for three
+closed meshes (with the meshes, cell sizes and box extensions).
+
 ``ray_cast.npz`` holds ``b3Body::RayCastSingle`` results of the reference
 (creation index of the triangle fixture hit or -1, fraction, normal) for
 seeded rays at the state reached after 12 steps of four scenes.
 
-Usage: python tests/golden/make_golden.py [rays]   (needs oracle/_ref built; ``rays``: ray_cast.npz only)
+Usage: python tests/golden/make_golden.py [rays | sdf]   (needs oracle/_ref built; ``rays`` / ``sdf``: that fixture only)
 """
 
 import hashlib
@@ -120,13 +123,38 @@ def ray_goldens():
     np.savez_compressed(os.path.join(OUT, "ray_cast.npz"), **rec)
 
 
+def sdf_cases():
+    """Meshes, cell sizes and box extensions of the SDF-build fixtures."""
+    rng = np.random.default_rng(3)
+    v, t = configs.torus_mesh(0.4, 0.15, 16, 9)
+    q = np.linalg.qr(rng.normal(size=(3, 3)))[0].astype(np.float32)
+    return {"icosphere": (configs.icosphere(1, 0.5), (0.1, 0.1, 0.1), 0.2),
+            "torus": (configs.torus_mesh(), (0.05, 0.05, 0.05), 0.1),
+            "torus_rotated": ((v @ q.T + np.float32([0.1, -0.2, 0.05]), t), (0.04, 0.05, 0.03), 0.1)}
+
+
+def sdf_goldens():
+    rec = {}
+    for name, ((v, t), cell, ext) in sdf_cases().items():
+        lo, hi, dims, vox = oracle.build_sdf(v, t, cell, ext, which="reference")
+        rec.update({f"{name}/vertices": np.asarray(v, np.float32), f"{name}/triangles": np.asarray(t, np.uint32),
+                    f"{name}/cell": np.float32(cell), f"{name}/extension": np.float32(ext), f"{name}/lower": lo,
+                    f"{name}/upper": hi, f"{name}/voxels": vox})
+        print(f"sdf {name}: grid {dims}, {int((vox < 0).sum())} inside")
+    np.savez_compressed(os.path.join(OUT, "sdf_build.npz"), **rec)
+
+
 def main():
     if not oracle.reference_available():
         sys.exit("oracle/_ref/libb3ref.so is missing: run `make -C oracle ref` where the reference tree exists")
     if sys.argv[1:] == ["rays"]:
         ray_goldens()
         return
+    if sys.argv[1:] == ["sdf"]:
+        sdf_goldens()
+        return
     ray_goldens()
+    sdf_goldens()
     for name, fn in configs.PARITY_SCENES.items():
         rec = run(name, fn().arrays(), schedule(name, 30))
         np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **rec)

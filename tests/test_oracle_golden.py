@@ -19,7 +19,7 @@ from tests.golden.make_golden import digest, external_loads
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 FIXTURES = sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
-                  if os.path.basename(p) != "ray_cast.npz")
+                  if os.path.basename(p) not in ("ray_cast.npz", "sdf_build.npz"))
 
 
 def scene_arrays(name):
@@ -101,3 +101,30 @@ def test_ray_cast_closed_form():
     assert nrm[0].tolist() == [0.0, 1.0, 0.0]
     assert nrm[1].tolist() == [0.0, -1.0, 0.0]
     assert float(np.dot(nrm[0], p1[0] - p2[0])) > 0.0   # the normal points back toward the ray's start
+
+
+def test_sdf_build_matches_reference_golden():
+    """b3BuildSDF: the oracle's grid and voxel values equal the reference's bit for bit."""
+    from tests.golden.make_golden import sdf_cases
+    g = np.load(os.path.join(GOLDEN, "sdf_build.npz"))
+    for name in sdf_cases():
+        lo, hi, dims, vox = oracle.build_sdf(g[f"{name}/vertices"], g[f"{name}/triangles"], g[f"{name}/cell"],
+                                             float(g[f"{name}/extension"]))
+        assert np.array_equal(lo, g[f"{name}/lower"]) and np.array_equal(hi, g[f"{name}/upper"]), name
+        assert np.array_equal(vox, g[f"{name}/voxels"]), name
+
+
+def test_sdf_build_approaches_the_sphere_distance():
+    """A fine icosphere (1280 triangles, radius 0.5): outside, the unsigned distance is within the chord error of
+    |p| - r; voxels near the centre are inside (negative). The sign comes from the reference's five parity rays,
+    which count a ray through a shared edge once per triangle, so a few voxels get the wrong sign."""
+    v, t = configs.icosphere(3, 0.5)
+    lo, hi, (W, H, D), vox = oracle.build_sdf(v, t, (0.1, 0.1, 0.1), 0.2)
+    z, y, x = np.meshgrid(*[np.arange(n) for n in (D, H, W)], indexing="ij")
+    cs = (hi - lo) / np.array([W - 1, H - 1, D - 1], np.float32)
+    p = np.stack([lo[0] + x * cs[0], lo[1] + y * cs[1], lo[2] + z * cs[2]], -1)
+    r = np.linalg.norm(p, axis=-1)
+    out = r > 0.55
+    assert np.abs(np.abs(vox[out]) - (r[out] - 0.5)).max() < 0.01
+    assert (vox[out] > 0).mean() > 0.95
+    assert (vox[r < 0.3] < 0).mean() > 0.95
