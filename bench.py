@@ -342,6 +342,87 @@ def run_contact(args):
     print(json.dumps(line), flush=True)
 
 
+def run_sdf_build(args):
+    """b3BuildSDF on the device: a closed icosphere (args.sdf_subdivisions subdivisions, radius 0.5) on an
+    args.sdf_res^3 grid (box extension 0.2); the reference's CPU path (oracle/_ref where built, else the oracle)
+    on a 24^3 grid of the same mesh, scaled by voxel count."""
+    from bounce_softbody_amd import _lib, configs
+
+    v, t = configs.icosphere(args.sdf_subdivisions, 0.5)
+    cell = 1.4 / (args.sdf_res - 1)
+    for _ in range(args.warmup):
+        _lib.build_sdf(v, t, (cell, cell, cell), 0.2)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        lo, hi, dims, vox = _lib.build_sdf(v, t, (cell, cell, cell), 0.2)
+    el = (time.perf_counter() - t0) / args.steps
+    nvox = dims[0] * dims[1] * dims[2]
+    pairs = nvox * len(t)
+    log(f"SDF build: grid {dims}, {len(t)} triangles, {el * 1e3:.1f} ms per build, {pairs / el / 1e9:.1f} G pairs/s")
+    baseline = None
+    if not args.no_cpu_baseline:
+        import oracle
+        kind = "reference" if oracle.reference_available() else "port"
+        c = 1.4 / 23
+        t0 = time.perf_counter()
+        _, _, d2, _ = oracle.build_sdf(v, t, (c, c, c), 0.2, which=kind if kind == "reference" else "oracle")
+        cel = time.perf_counter() - t0
+        n2 = d2[0] * d2[1] * d2[2]
+        baseline = {"value": round(1.0 / (cel * nvox / n2), 6), "unit": "builds/s", "cores": 1, "kind": kind,
+                    "sample": f"a {d2[0]}x{d2[1]}x{d2[2]} grid of the same mesh ({cel:.1f} s), scaled by voxel count"}
+        log(f"CPU baseline: {baseline}")
+    line = {"metric": "SDF builds/s", "value": round(1.0 / el, 4), "unit": "builds/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(el * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"b3BuildSDF of an icosphere ({len(t)} triangles) on a {dims[0]}x{dims[1]}x{dims[2]} grid",
+                       "voxel_triangle_pairs_per_s": round(pairs / el, 1)},
+            "roofline": None, "cpu_baseline": baseline}
+    print(json.dumps(line), flush=True)
+
+
+def run_ray_cast(args):
+    """b3Body::RayCastSingle on the device: the 512^2 cloth after 20 steps, args.rays rays per query; each query
+    after the cloth moved (tree rebuilt) and again on the same state (tree reused)."""
+    from bounce_softbody_amd import configs
+    from bounce_softbody_amd.body import Body
+
+    arr = configs.cloth_on_sphere(args.cloth).arrays()
+    b = Body(arr)
+    for _ in range(20):
+        b.Step(configs.DT, 1, 50, 0.0, 1e-5)
+    x, _ = b.state()
+    rng = np.random.default_rng(9)
+    lo, hi = x.min(0) - 0.1, x.max(0) + 0.1
+    p1 = (lo + (hi - lo) * rng.random((args.rays, 3))).astype(np.float32)
+    p1[:, 1] = hi[1] + 0.5
+    p2 = p1.copy()
+    p2[:, 1] = lo[1] - 0.5
+    for _ in range(args.warmup):
+        b.set_state(x)
+        b.RayCastSingle(p1, p2)
+    rebuild = reuse = 0.0
+    for _ in range(args.steps):
+        b.set_state(x)   # moves the particles' version: the next query rebuilds the tree
+        t0 = time.perf_counter()
+        idx, _, _ = b.RayCastSingle(p1, p2)
+        t1 = time.perf_counter()
+        b.RayCastSingle(p1, p2)
+        rebuild += t1 - t0
+        reuse += time.perf_counter() - t1
+    rebuild /= args.steps
+    reuse /= args.steps
+    log(f"ray cast: {args.rays} rays, {1e3 * rebuild:.2f} ms with tree rebuild, {1e3 * reuse:.2f} ms reused; "
+        f"{int((idx >= 0).sum())} hits")
+    line = {"metric": "rays/s", "value": round(args.rays / rebuild, 1), "unit": "rays/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(rebuild * 1e3, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"RayCastSingle x {args.rays} against a {args.cloth + 1}^2 cloth "
+                                   f"({len(arr['tf_density'])} triangle fixtures), tree rebuilt per query",
+                       "ms_tree_reused": round(reuse * 1e3, 3)},
+            "roofline": None, "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -357,7 +438,11 @@ def main():
                     help="step the two bodies as one batch of two instances instead of two handles")
     ap.add_argument("--graph", action="store_true", help="replay each body's step as a captured HIP graph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["two_body", "precision", "batch", "contact"], default="two_body")
+    ap.add_argument("--workload", choices=["two_body", "precision", "batch", "contact", "sdf_build", "ray_cast"],
+                    default="two_body")
+    ap.add_argument("--sdf-subdivisions", type=int, default=4, help="sdf_build workload: icosphere subdivisions")
+    ap.add_argument("--sdf-res", type=int, default=128, help="sdf_build workload: voxels per axis")
+    ap.add_argument("--rays", type=int, default=1000000, help="ray_cast workload: rays per query")
     ap.add_argument("--bodies", type=int, default=None, help="batch / contact workloads: number of bodies "
                                                                "(default 4096 / 256)")
     ap.add_argument("--handles-subset", type=int, default=64, help="batch workload: bodies also timed as handles")
@@ -390,6 +475,12 @@ def main():
         args.pre_steps = 40 if contact else 10
     if args.workload == "precision":
         run_precision(args)
+        return
+    if args.workload == "sdf_build":
+        run_sdf_build(args)
+        return
+    if args.workload == "ray_cast":
+        run_ray_cast(args)
         return
     if contact:
         run_contact(args)
