@@ -123,8 +123,15 @@ EXPORTS = [
     "b3g_get_stats", "b3g_get_energy", "b3g_get_contacts", "b3g_get_tet_quats", "b3g_assemble", "b3g_get_system",
     "b3g_synchronize", "b3g_time_spmv", "b3g_set_solver", "b3g_set_graph", "b3g_get_system_size", "b3g_pattern",
     "b3g_solve_cg", "b3g_spmv", "b3g_set_reduction", "b3g_save_state", "b3g_load_state", "b3g_ray_cast",
-    "b3g_sdf_grid", "b3g_build_sdf",
+    "b3g_sdf_grid", "b3g_build_sdf", "b3g_body_rebuild", "b3g_set_action_forces", "b3g_get_action_forces",
 ]
+
+
+class Carry(C.Structure):
+    """``b3g_carry`` (include/b3g.h)."""
+
+    _fields_ = [("particle_prev", C.POINTER(C.c_int64)), ("tet_prev", C.POINTER(C.c_int64)),
+                ("sphere_prev", C.POINTER(C.c_int64)), ("world_prev", C.POINTER(C.c_int64))]
 
 
 def lib():
@@ -147,6 +154,9 @@ def lib():
         "b3g_get_instance_count": (C.c_int, [vp, u32p]),
         "b3g_ray_cast": (C.c_int, [vp, C.c_uint32, C.c_uint32, f32p, f32p, C.POINTER(C.c_int64), f32p, f32p]),
         "b3g_sdf_grid": (C.c_int, [f32p, C.c_uint32, f32p, C.c_float, f32p, u32p]),
+        "b3g_body_rebuild": (C.c_int, [vp, C.c_void_p, C.c_uint32, C.POINTER(Carry)]),
+        "b3g_set_action_forces": (C.c_int, [vp, C.c_int]),
+        "b3g_get_action_forces": (C.c_int, [vp, C.c_uint32, f32p, C.c_uint64]),
         "b3g_build_sdf": (C.c_int, [C.c_int, f32p, C.c_uint32, u32p, C.c_uint32, C.c_uint32, f32p, C.c_float, f32p,
                                     C.c_uint64]),
         "b3g_step": (C.c_int, [vp, C.c_float, C.c_uint32, C.c_uint32, C.c_float, C.c_float]),

@@ -69,6 +69,47 @@ class Body:
         self.n_tet = sum(self.inst_tets)
         self._dirty = False
 
+    def rebuild(self, scene, carry: dict | None = None):
+        """Topology edits (b3g_body_rebuild): the edited description(s) and, per new object, the creation index of
+        the object it continues (bounce_softbody_amd.scene.edit_arrays builds both)."""
+        scenes = scene if isinstance(scene, (list, tuple)) else [scene]
+        arrs = [sc if isinstance(sc, dict) else sc.arrays() for sc in scenes]
+        made = [_lib.make_desc(a) for a in arrs]
+        descs = (_lib.BodyDesc * len(made))(*[m[0] for m in made])
+        keep = []
+        cp = None
+        if carry is not None:
+            c = _lib.Carry()
+            for k in ("particle_prev", "tet_prev", "sphere_prev", "world_prev"):
+                a = carry.get(k)
+                if a is not None:
+                    a = np.ascontiguousarray(a, np.int64)
+                    keep.append(a)
+                    setattr(c, k, a.ctypes.data_as(C.POINTER(C.c_int64)))
+            cp = C.byref(c)
+        check(self._L.b3g_body_rebuild(self._h, descs, len(made), cp), "b3g_body_rebuild")
+        self.scene = scene
+        self._arrays = arrs[0] if len(arrs) == 1 else arrs
+        self.n_inst = len(arrs)
+        self.inst_particles = [len(a["p_type"]) for a in arrs]
+        self.inst_tets = [len(a["tet_param"]) // 3 for a in arrs]
+        self.n = sum(self.inst_particles)
+        self.n_tet = sum(self.inst_tets)
+
+    def set_action_forces(self, enabled: bool):
+        check(self._L.b3g_set_action_forces(self._h, 1 if enabled else 0), "b3g_set_action_forces")
+
+    def action_forces(self, force_type: int):
+        """Action forces of the forces of one type in creation order: (count, 2 | 3 | 4, 3)."""
+        per = {3: 2, 1: 3, 2: 3, 4: 4}[force_type]
+        arrs = self._arrays if isinstance(self._arrays, list) else [self._arrays]
+        key = {3: "sp_param", 1: "st_param", 2: "sh_param", 4: "mo_param"}[force_type]
+        width = {3: 3, 1: 12, 2: 8, 4: 6}[force_type]
+        m = sum(len(a[key]) // width for a in arrs)
+        out = np.zeros(max(1, 3 * per * m), f32)
+        check(self._L.b3g_get_action_forces(self._h, force_type, _ptr(out, f32p), out.size), "b3g_get_action_forces")
+        return out[: 3 * per * m].reshape(m, per, 3)
+
     def split(self, rows):
         """Per-instance views of a concatenated per-particle array."""
         out, o = [], 0

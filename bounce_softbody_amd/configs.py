@@ -184,6 +184,61 @@ def scene_springs(R: int = 6) -> SceneBuilder:
     return b
 
 
+def scene_tearing(R: int = 6) -> SceneBuilder:
+    """Spring cloth pinned at two corners over a world sphere (cloth_tearing.h): tension springs with damping,
+    every particle a 0.02 m sphere fixture."""
+    m = GridClothMesh(R, R).scale((0.4, 1.0, 0.4)).translate((0.0, 0.3, 0.0))
+    b = SceneBuilder()
+    b.add_particles(m.vertices)
+    b.add_spheres(np.arange(m.vertex_count), 0.02, 0.3)
+    b.add_triangle_fixtures(m.triangles, m.vertices[m.triangles].reshape(-1, 9), density=0.5, radius=0.0, friction=0.3)
+    edges = set()
+    for t in m.triangles:
+        for a, c in ((t[0], t[1]), (t[1], t[2]), (t[2], t[0])):
+            edges.add((min(a, c), max(a, c)))
+    e = np.array(sorted(edges), np.int64)
+    L = np.linalg.norm((m.vertices[e[:, 0]] - m.vertices[e[:, 1]]).astype(np.float64), axis=1).astype(f32)
+    b.add_springs(e, length=L, ks=500.0, kd=2.0)
+    b.set_particle_types([m.vertex(0, 0), m.vertex(0, R)], STATIC)
+    b.add_world_sphere((0.1, -0.1, 0.15), 0.3, 0.4)
+    b.gravity[:] = (0.0, -9.8, 0.0)
+    b.mesh = m
+    return b
+
+
+def tear_edits(arrays: dict, x: np.ndarray, v: np.ndarray, action: np.ndarray, count: int = 2):
+    """Topology edits splitting cloth at its most tensioned springs (the gist of cloth_tearing.h:425-450): for
+    each of the `count` springs with the largest |action force| (creation order breaks ties), a new particle at
+    the position and velocity of the spring's first dynamic particle, a sphere fixture for it, a spring from it to
+    the other end with the same parameters, and the old spring destroyed. Returns the edit records
+    (bounce_softbody_amd.scene.edit_arrays); x, v and action are current values in creation order."""
+    from .scene import FORCE_SPRING
+    sp = np.asarray(arrays["sp_p"]).reshape(-1, 2)
+    prm = np.asarray(arrays["sp_param"]).reshape(-1, 3)
+    mag = np.einsum("ij,ij->i", action[:, 0].astype(np.float64), action[:, 0].astype(np.float64))
+    order = sorted(range(len(sp)), key=lambda k: (-mag[k], k))
+    edits, n, done, destroyed = [], len(arrays["p_type"]), 0, []
+    sph = {int(pp): (float(r), float(f)) for pp, r, f in zip(arrays["s_particle"], arrays["s_radius"], arrays["s_friction"])}
+    for k in order:
+        p1, p2 = int(sp[k, 0]), int(sp[k, 1])
+        a, other = (p1, p2) if arrays["p_type"][p1] == DYNAMIC else (p2, p1)
+        if arrays["p_type"][a] != DYNAMIC:
+            continue
+        edits.append(("create_particle", DYNAMIC, x[a].copy(), v[a].copy(), float(arrays["p_damping"][a])))
+        if a in sph:
+            edits.append(("create_sphere", n, *sph[a]))
+        edits.append(("create_spring", n, other, *[float(q) for q in prm[k]]))
+        destroyed.append(k)
+        n += 1
+        done += 1
+        if done == count:
+            break
+    # destroy from the highest index down so earlier indices stay valid
+    for k in sorted(destroyed, reverse=True):
+        edits.append(("destroy_force", FORCE_SPRING, k))
+    return edits
+
+
 def scene_mouse(N: int = 2) -> SceneBuilder:
     """A tet block dragged by a b3MouseForce (body_dragger.cpp:89-101) towards a static particle."""
     m = GridTetMesh(N, N, N)

@@ -2,6 +2,7 @@
 #include "b3g_internal.h"
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -129,6 +130,161 @@ b3g_status b3g_body_build_batch(b3g_body h, const b3g_body_desc* descs, uint32_t
 	return build_body(h->body, descs, count);
 }
 
+b3g_status b3g_body_rebuild(b3g_body h, const b3g_body_desc* descs, uint32_t count, const b3g_carry* carry) {
+	CHECK_BUILT(h);
+	if (!descs || count == 0) { set_error("a rebuild needs at least one description"); return B3G_INVALID_ARGUMENT; }
+	SET_DEVICE(h);
+	Body& b = h->body;
+	B3G_HIP(hipStreamSynchronize(b.stream));
+	// the previous body's state, on the host
+	const uint32_t n0 = b.n, m0 = b.tet.m, pairs0 = b.nPairs, nw0 = b.nWorld;
+	std::vector<V3> x0(n0), v0(n0);
+	std::vector<float4> q0(m0);
+	B3G_HIP(hipMemcpy(x0.data(), b.x.p, n0 * sizeof(V3), hipMemcpyDeviceToHost));
+	B3G_HIP(hipMemcpy(v0.data(), b.v.p, n0 * sizeof(V3), hipMemcpyDeviceToHost));
+	if (m0) B3G_HIP(hipMemcpy(q0.data(), b.tet.q.p, m0 * sizeof(float4), hipMemcpyDeviceToHost));
+	std::vector<uint32_t> act0(pairs0), stamp0(pairs0), fric0(pairs0);
+	std::vector<float> nf0(pairs0);
+	std::vector<V3> man0(2 * (size_t)pairs0);
+	uint32_t counter0 = 0;
+	if (pairs0) {
+		B3G_HIP(hipMemcpy(act0.data(), b.cActive.p, pairs0 * 4, hipMemcpyDeviceToHost));
+		B3G_HIP(hipMemcpy(stamp0.data(), b.cStamp.p, pairs0 * 4, hipMemcpyDeviceToHost));
+		B3G_HIP(hipMemcpy(fric0.data(), b.cFriction.p, pairs0 * 4, hipMemcpyDeviceToHost));
+		B3G_HIP(hipMemcpy(nf0.data(), b.cNormalForce.p, pairs0 * 4, hipMemcpyDeviceToHost));
+		B3G_HIP(hipMemcpy(man0.data(), b.cManifold.p, pairs0 * 2 * sizeof(V3), hipMemcpyDeviceToHost));
+		B3G_HIP(hipMemcpy(&counter0, b.stampCounter.p, 4, hipMemcpyDeviceToHost));
+	}
+	std::map<std::pair<uint32_t, uint32_t>, uint32_t> pair0;   // (sphere, world) creation indices -> pair
+	std::vector<uint32_t> call0(pairs0);                          // the FindNewContacts call that created it
+	for (uint32_t k = 0; k < pairs0; ++k) {
+		const uint32_t sp = b.hostPairSphere[k];
+		pair0[{b.hostSphereCreation[sp], nw0 - 1 - b.hostPairWorld[k]}] = k;
+		call0[k] = stamp0[k] / (b.hostPairStart[sp + 1] - b.hostPairStart[sp]);
+	}
+	uint32_t sphereCount0 = 0;
+	for (uint32_t c : b.hostSphereCreation) sphereCount0 = std::max(sphereCount0, c + 1);
+
+	// merged counts of the new description; every map entry is checked before
+	// anything is rebuilt, so a rejected call leaves the body as it was
+	uint64_t np = 0, nt = 0, nsph = 0, nw = 0;
+	for (uint32_t k = 0; k < count; ++k) {
+		np += descs[k].n_particles; nt += descs[k].n_tet; nsph += descs[k].n_spheres; nw += descs[k].n_world;
+	}
+	if (carry) {
+		auto bad = [](const char* what) { set_error(std::string("carry map out of range: ") + what); return B3G_INVALID_ARGUMENT; };
+		auto check = [](const int64_t* map, uint64_t len, uint64_t prev) {
+			for (uint64_t c = 0; map && c < len; ++c)
+				if (map[c] >= (int64_t)prev) return false;
+			return true;
+		};
+		if (!check(carry->particle_prev, np, n0)) return bad("particle");
+		if (!check(carry->tet_prev, nt, m0)) return bad("tetrahedron");
+		if (!check(carry->sphere_prev, nsph, sphereCount0)) return bad("sphere");
+		if (!check(carry->world_prev, nw, nw0)) return bad("world");
+	}
+
+	b3g_status s = build_body(b, descs, count);
+	if (s != B3G_OK) return s;
+	if (!carry) return B3G_OK;
+	if (carry->particle_prev) {
+		std::vector<V3> x(b.n), v(b.n);
+		B3G_HIP(hipMemcpy(x.data(), b.x.p, b.n * sizeof(V3), hipMemcpyDeviceToHost));
+		B3G_HIP(hipMemcpy(v.data(), b.v.p, b.n * sizeof(V3), hipMemcpyDeviceToHost));
+		for (uint64_t c = 0; c < np; ++c) {
+			const int64_t q = carry->particle_prev[c];
+			if (q < 0) continue;
+			x[b.n - 1 - c] = x0[n0 - 1 - q];
+			v[b.n - 1 - c] = v0[n0 - 1 - q];
+		}
+		B3G_HIP(hipMemcpy(b.x.p, x.data(), b.n * sizeof(V3), hipMemcpyHostToDevice));
+		B3G_HIP(hipMemcpy(b.v.p, v.data(), b.n * sizeof(V3), hipMemcpyHostToDevice));
+	}
+	if (carry->tet_prev && b.tet.m) {
+		std::vector<float4> q(b.tet.m);
+		B3G_HIP(hipMemcpy(q.data(), b.tet.q.p, b.tet.m * sizeof(float4), hipMemcpyDeviceToHost));
+		for (uint64_t c = 0; c < nt; ++c) {
+			const int64_t p = carry->tet_prev[c];
+			if (p >= 0) q[c] = q0[p];
+		}
+		B3G_HIP(hipMemcpy(b.tet.q.p, q.data(), b.tet.m * sizeof(float4), hipMemcpyHostToDevice));
+	}
+	if (carry->sphere_prev && carry->world_prev && b.nPairs && pairs0) {
+		const uint32_t P = b.nPairs;
+		std::vector<uint32_t> act(P, 0), stamp(P, 0), fric(P, 0);
+		std::vector<float> nf(P, 0.0f);
+		std::vector<V3> man(2 * (size_t)P, zero3());
+		// a carried contact keeps the FindNewContacts call that created it; its
+		// stamp is re-encoded with the sphere's new pair count (call * pairs +
+		// world), so the list order (newest call first, then sphere, then
+		// world) is unchanged and later calls still sort after it
+		for (uint32_t sp = 0; sp < b.nSpheres; ++sp) {
+			const uint32_t p0 = b.hostPairStart[sp], nwp = b.hostPairStart[sp + 1] - p0;
+			for (uint32_t k = p0; k < b.hostPairStart[sp + 1]; ++k) {
+				const int64_t ps = carry->sphere_prev[b.hostSphereCreation[sp]];
+				const int64_t pw = carry->world_prev[b.nWorld - 1 - b.hostPairWorld[k]];
+				if (ps < 0 || pw < 0) continue;
+				auto it = pair0.find({(uint32_t)ps, (uint32_t)pw});
+				if (it == pair0.end() || !act0[it->second]) continue;
+				const uint32_t o = it->second;
+				act[k] = 1;
+				fric[k] = fric0[o];
+				nf[k] = nf0[o];
+				man[2 * (size_t)k] = man0[2 * (size_t)o];
+				man[2 * (size_t)k + 1] = man0[2 * (size_t)o + 1];
+				stamp[k] = call0[o] * nwp + (k - p0);
+			}
+		}
+		const uint32_t counter = counter0;
+		B3G_HIP(hipMemcpy(b.cActive.p, act.data(), P * 4, hipMemcpyHostToDevice));
+		B3G_HIP(hipMemcpy(b.cStamp.p, stamp.data(), P * 4, hipMemcpyHostToDevice));
+		B3G_HIP(hipMemcpy(b.cFriction.p, fric.data(), P * 4, hipMemcpyHostToDevice));
+		B3G_HIP(hipMemcpy(b.cNormalForce.p, nf.data(), P * 4, hipMemcpyHostToDevice));
+		B3G_HIP(hipMemcpy(b.cManifold.p, man.data(), P * 2 * sizeof(V3), hipMemcpyHostToDevice));
+		B3G_HIP(hipMemcpy(b.stampCounter.p, &counter, 4, hipMemcpyHostToDevice));
+	}
+	++b.xVersion;
+	return B3G_OK;
+}
+
+b3g_status b3g_set_action_forces(b3g_body h, int enabled) {
+	CHECK_BODY(h);
+	SET_DEVICE(h);
+	Body& b = h->body;
+	B3G_HIP(hipStreamSynchronize(b.stream));
+	b.trackActions = enabled != 0;
+	drop_graph(b);
+	if (b.built) return ensure_actions(b);
+	return B3G_OK;
+}
+
+b3g_status b3g_get_action_forces(b3g_body h, uint32_t force_type, float* out, uint64_t capacity) {
+	CHECK_BUILT(h);
+	SET_DEVICE(h);
+	Body& b = h->body;
+	if (!b.trackActions) { set_error("action forces are not tracked (b3g_set_action_forces)"); return B3G_INVALID_ARGUMENT; }
+	uint32_t m, per, base;
+	switch (force_type) {
+	case B3G_FORCE_SPRING: m = b.spring.m; per = 2; base = b.actSpring; break;
+	case B3G_FORCE_STRETCH: m = b.stretch.m; per = 3; base = b.actStretch; break;
+	case B3G_FORCE_SHEAR: m = b.shear.m; per = 3; base = b.actShear; break;
+	case B3G_FORCE_MOUSE: m = b.mouse.m; per = 4; base = b.actMouse; break;
+	default: set_error("no action forces for this force type"); return B3G_INVALID_ARGUMENT;
+	}
+	if (!out || capacity < 3ull * per * m) { set_error("output holds fewer than 3 * per * count floats"); return B3G_INVALID_ARGUMENT; }
+	if (!m) return B3G_OK;
+	std::vector<V3> planes((size_t)per * m);
+	B3G_HIP(hipStreamSynchronize(b.stream));
+	B3G_HIP(hipMemcpy(planes.data(), dptr<V3>(b.act) + base, planes.size() * sizeof(V3), hipMemcpyDeviceToHost));
+	for (uint32_t e = 0; e < m; ++e)
+		for (uint32_t i = 0; i < per; ++i) {
+			const V3 f = planes[(size_t)i * m + e];
+			float* o = out + 3 * ((size_t)per * e + i);
+			o[0] = f.x; o[1] = f.y; o[2] = f.z;
+		}
+	return B3G_OK;
+}
+
 b3g_status b3g_get_instance_count(b3g_body h, uint32_t* count) {
 	CHECK_BUILT(h);
 	if (!count) return B3G_INVALID_ARGUMENT;
@@ -179,7 +335,7 @@ b3g_status b3g_step(b3g_body h, float dt, uint32_t iterations, uint32_t sub_iter
 	// Captured step: one graph per set of step parameters and modes, replayed.
 	const std::vector<double> key = {(double)dt, (double)iterations, (double)sub_iterations, (double)tol, (double)stol,
 	                                  (double)b.reduction, (double)b.solver, (double)b.innerIt, (double)b.innerTol,
-	                                  (double)b.logStride};
+	                                  (double)b.logStride, (double)b.trackActions};
 	if (!b.graphExec || key != b.graphKey) {
 		drop_graph(b);
 		hipGraph_t graph = nullptr;

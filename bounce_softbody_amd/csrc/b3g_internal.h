@@ -179,6 +179,7 @@ struct Body {
 	DBuf sphereRadius, sphereFriction;            // per sphere (list order)
 	DBuf sphereParticle;                          // uint32 per sphere
 	std::vector<uint32_t> hostSphereParticle;
+	std::vector<uint32_t> hostSphereCreation;     // creation index of each sphere fixture (list order)
 	std::vector<float> hostMass;                  // list order
 	std::vector<uint8_t> hostType;
 
@@ -275,6 +276,14 @@ struct Body {
 	    rtChunk1, rtChunk2, rtChunk3, rtInstBox, rtRays, rtHits;
 	size_t rtRayCap = 0;
 
+	// ---------------------------------------------------------------- action forces
+	// m_f1.. of spring (2), stretch (3), shear (3) and mouse (4) forces:
+	// accumulated over a step's Newton iterations when enabled; V3 planes
+	// [base + i m + e] per type, element e in creation order
+	bool trackActions = false;
+	DBuf act;
+	uint32_t actSpring = 0, actStretch = 0, actShear = 0, actMouse = 0, actCount = 0;
+
 	// ---------------------------------------------------------------- diagnostics
 	bool captureFi = false;
 
@@ -283,6 +292,8 @@ struct Body {
 
 // Allocation helpers (b3g_setup.hip)
 b3g_status dalloc(Body& b, DBuf& buf, size_t bytes);
+// Allocates the action-force planes when they are tracked (after a build).
+b3g_status ensure_actions(Body& b);
 // b3BuildSDF on the device (b3g_sdf.hip): the grid, then the voxels.
 b3g_status sdf_grid(const float* vertices, uint32_t nv, const float cell[3], float extension, AABB& box, uint32_t dims[3]);
 b3g_status build_sdf(int device, const float* vertices, uint32_t nv, const uint32_t* triangles, uint32_t nt,

@@ -754,6 +754,44 @@ __global__ void __launch_bounds__(kBlock) k_shear_forces(UVArgs a) {
 	a.rec[4 * (size_t)e + 1] = make_float4(wv.x, wv.y, wv.z, ks);
 }
 
+// Action forces (e.g. spring_force.cpp:101-102, 128-129): per Newton
+// iteration each vertex's accumulator takes the force's terms in the
+// reference's order, stiffness then damping (stretch: u then v). A term
+// whose coefficient is not positive is never added; a stiffness term the
+// element switched off is staged as +0, and adding +0 to an accumulator that
+// starts at +0 leaves it exactly as skipping the term would.
+struct ActArgs {
+	uint32_t m, np, nterm;
+	uint32_t f;                  // staging base of the force type
+	const V3* fstage;
+	const float* c;              // coefficient planes
+	uint32_t coef[4];            // per term: coefficient plane that must be positive
+	uint32_t slot[4];            // per term: staging plane of vertex 0 (vertex i: slot + i)
+	V3* act;
+	uint32_t base;
+	const uint32_t* p;           // particle planes (first vertex: instance)
+	const SolverState* st;
+	const uint32_t* rowInst;
+};
+
+__global__ void __launch_bounds__(kBlock) k_action_forces(ActArgs a) {
+	const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
+	if (e >= a.m) return;
+	if (!a.st[a.rowInst[a.p[e]]].newtonActive) return;
+	const uint32_t m = a.m;
+	for (uint32_t i = 0; i < a.np; ++i) {
+		V3 s = a.act[a.base + (size_t)i * m + e];
+		for (uint32_t t = 0; t < a.nterm; ++t)
+			if (a.c[(size_t)a.coef[t] * m + e] > 0.0f) s = s + a.fstage[a.f + (size_t)(a.slot[t] + i) * m + e];
+		a.act[a.base + (size_t)i * m + e] = s;
+	}
+}
+
+__global__ void __launch_bounds__(kBlock) k_clear_v3(size_t count, V3* p) {
+	const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+	if (i < count) p[i] = zero3();
+}
+
 struct PairArgs {
 	uint32_t m;
 	const uint32_t* p;
@@ -2104,6 +2142,26 @@ static b3g_status enqueue_linearise(Body& b, float h) {
 		           b.mouse.jx, b.mouse.jv, b.mouse.f, st, ri};
 		LAUNCH(k_mouse_forces, nblk(b.mouse.m), p);
 	}
+	if (b.trackActions && b.act.p) {
+		auto acc = [&](uint32_t m, uint32_t np, const DBuf& p, const DBuf& cf, uint32_t f, uint32_t base,
+		               std::initializer_list<std::pair<uint32_t, uint32_t>> terms) -> b3g_status {
+			if (!m) return B3G_OK;
+			ActArgs aa{};
+			aa.m = m; aa.np = np; aa.nterm = (uint32_t)terms.size(); aa.f = f; aa.fstage = fs;
+			aa.c = dptr<float>(cf);
+			uint32_t t = 0;
+			for (const auto& tc : terms) { aa.coef[t] = tc.first; aa.slot[t] = tc.second; ++t; }
+			aa.act = dptr<V3>(b.act); aa.base = base; aa.p = dptr<uint32_t>(p); aa.st = st; aa.rowInst = ri;
+			LAUNCH(k_action_forces, nblk(m), aa);
+			return B3G_OK;
+		};
+		b3g_status s;
+		// (coefficient plane, staging plane of vertex 0) per term, in the order the reference adds them
+		if ((s = acc(b.stretch.m, 3, b.stretch.p, b.stretch.c, b.stretch.f, b.actStretch, {{12, 0}, {13, 3}, {15, 6}, {16, 9}})) != B3G_OK) return s;
+		if ((s = acc(b.shear.m, 3, b.shear.p, b.shear.c, b.shear.f, b.actShear, {{12, 0}, {13, 3}})) != B3G_OK) return s;
+		if ((s = acc(b.spring.m, 2, b.spring.p, b.spring.c, b.spring.f, b.actSpring, {{1, 0}, {2, 2}})) != B3G_OK) return s;
+		if ((s = acc(b.mouse.m, 4, b.mouse.p, b.mouse.c, b.mouse.f, b.actMouse, {{4, 0}, {5, 4}})) != B3G_OK) return s;
+	}
 	ContactArgs c = contact_args(b, h);
 	if (b.nPairs) LAUNCH(k_contact_forces, nblk(b.nPairs), c);
 	AsmArgs a;
@@ -2339,6 +2397,21 @@ b3g_status prepare_step(Body& b, uint32_t subIters) {
 	return ensure_cg_log(b, subIters + 1);
 }
 
+b3g_status ensure_actions(Body& b) {
+	if (!b.trackActions) return B3G_OK;
+	b.actStretch = 0;
+	b.actShear = b.actStretch + 3 * b.stretch.m;
+	b.actSpring = b.actShear + 3 * b.shear.m;
+	b.actMouse = b.actSpring + 2 * b.spring.m;
+	b.actCount = b.actMouse + 4 * b.mouse.m;
+	if (b.act.p && b.act.bytes >= (size_t)std::max(1u, b.actCount) * sizeof(V3)) return B3G_OK;
+	b3g_status s = dalloc(b, b.act, (size_t)std::max(1u, b.actCount) * sizeof(V3));
+	if (s != B3G_OK) return s;
+	LAUNCH(k_clear_v3, (uint32_t)(((size_t)std::max(1u, b.actCount) + kBlock - 1) / kBlock), (size_t)std::max(1u, b.actCount),
+	       dptr<V3>(b.act));
+	return B3G_OK;
+}
+
 void drop_graph(Body& b) {
 	if (b.graphExec) (void)hipGraphExecDestroy(b.graphExec);
 	b.graphExec = nullptr;
@@ -2353,6 +2426,8 @@ b3g_status enqueue_step(Body& b, float dt, uint32_t iters, uint32_t subIters, fl
 	if (dt > 0.0f) {
 		StepArgs sa = step_args(b, dt);
 		LAUNCH(k_step_begin, nblk(b.n), sa);
+		if (b.trackActions && b.act.p && !assembleOnly)   // b3Force::ClearForces (body.cpp:566-570)
+			LAUNCH(k_clear_v3, (uint32_t)(((size_t)b.actCount + kBlock - 1) / kBlock), (size_t)b.actCount, dptr<V3>(b.act));
 		const bool refined = b.solver != B3G_SOLVER_PCG && !assembleOnly;
 		const uint32_t sweeps = refined ? std::max(1u, (subIters + b.innerIt - 1) / std::max(1u, b.innerIt)) : 0u;
 		if (refined) {

@@ -683,17 +683,19 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 	// sphere fixtures: the first fixture of a particle wins (body.cpp:151-176)
 	h.sphereOf.assign(n, -1);
 	{
-		std::vector<uint32_t> created;
+		std::vector<uint32_t> created, createdIndex;
 		std::vector<float> rad, fr;
 		std::vector<char> has(n, 0);
 		for (uint32_t c = 0; c < d.n_spheres; ++c) {
 			uint32_t p = L(d.s_particle[c]);
 			if (has[p]) continue;
 			has[p] = 1;
-			created.push_back(p); rad.push_back(d.s_radius[c]); fr.push_back(d.s_friction[c]);
+			created.push_back(p); createdIndex.push_back(c); rad.push_back(d.s_radius[c]); fr.push_back(d.s_friction[c]);
 		}
 		const uint32_t ns = (uint32_t)created.size();
+		b.hostSphereCreation.clear();
 		for (uint32_t k = 0; k < ns; ++k) {   // list order = reverse
+			b.hostSphereCreation.push_back(createdIndex[ns - 1 - k]);
 			h.sphereParticle.push_back(created[ns - 1 - k]);
 			h.sphereRadius.push_back(rad[ns - 1 - k]);
 			h.sphereFriction.push_back(fr[ns - 1 - k]);
@@ -1183,6 +1185,8 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 		st = launch_tet_perm(b, permV, b.K0v);
 		if (st != B3G_OK) return st;
 	}
+	st = ensure_actions(b);
+	if (st != B3G_OK) return st;
 	B3G_HIP(hipStreamSynchronize(b.stream));
 	b.built = true;
 	return B3G_OK;

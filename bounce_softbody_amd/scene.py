@@ -308,3 +308,59 @@ def mesh_adjacency(triangles) -> np.ndarray:
                     out[i1, j1] = n2
                     break
     return out
+
+
+def edit_arrays(arrays: dict, edits):
+    """A description after topology edits, as the reference's body would hold it after the same calls
+    (body.cpp:62-354), and the maps from the new objects to the ones they continue.
+
+    Edit records, applied in order (creation indices as in ``arrays``; a new object takes the next index):
+      ("destroy_force", force_type, index_in_type)     b3Body::DestroyForce
+      ("create_particle", type, position, velocity, damping)
+      ("create_sphere", particle, radius, friction)
+      ("create_spring", p1, p2, length, ks, kd)
+    Returns (new arrays, carry) with carry = {"particle_prev", "tet_prev", "sphere_prev", "world_prev"}: per new
+    object its creation index in ``arrays``, or -1 for an object the edits created."""
+    a = {k: (v.copy() if isinstance(v, np.ndarray) else list(v) if isinstance(v, list) else v) for k, v in arrays.items()}
+    n0, t0, s0, w0 = len(a["p_type"]), len(a["tet_param"]) // 3, len(a["s_particle"]), len(a["w_type"])
+    particle_prev = list(range(n0))
+    tet_prev = list(range(t0))
+    widths = {FORCE_TET_ELEMENT: ("tet", {"_p": 4, "_v": 12, "_param": 3}), FORCE_TRI_ELEMENT: ("tri", {"_p": 3, "_v": 9, "_param": 6}),
+              FORCE_STRETCH: ("st", {"_p": 3, "_param": 12}), FORCE_SHEAR: ("sh", {"_p": 3, "_param": 8}),
+              FORCE_SPRING: ("sp", {"_p": 2, "_param": 3}), FORCE_MOUSE: ("mo", {"_p": 4, "_param": 6})}
+    for e in edits:
+        kind = e[0]
+        if kind == "destroy_force":
+            ftype, k = e[1], e[2]
+            name, cols = widths[ftype]
+            for suffix, w in cols.items():
+                arr = a[name + suffix].reshape(-1, w)
+                a[name + suffix] = np.ascontiguousarray(np.delete(arr, k, axis=0).reshape(-1))
+            keep = ~((a["f_type"] == ftype) & (a["f_index"] == k))
+            a["f_type"], a["f_index"] = a["f_type"][keep], a["f_index"][keep].copy()
+            later = (a["f_type"] == ftype) & (a["f_index"] > k)
+            a["f_index"][later] -= 1
+            if ftype == FORCE_TET_ELEMENT:
+                del tet_prev[k]
+        elif kind == "create_particle":
+            a["p_type"] = np.append(a["p_type"], np.uint8(e[1]))
+            a["p_pos"] = np.append(a["p_pos"], np.asarray(e[2], f32))
+            a["p_vel"] = np.append(a["p_vel"], np.asarray(e[3], f32))
+            a["p_damping"] = np.append(a["p_damping"], f32(e[4]))
+            particle_prev.append(-1)
+        elif kind == "create_sphere":
+            a["s_particle"] = np.append(a["s_particle"], np.uint32(e[1]))
+            a["s_radius"] = np.append(a["s_radius"], f32(e[2]))
+            a["s_friction"] = np.append(a["s_friction"], f32(e[3]))
+        elif kind == "create_spring":
+            a["sp_p"] = np.append(a["sp_p"], np.uint32([e[1], e[2]]))
+            a["sp_param"] = np.append(a["sp_param"], f32([e[3], e[4], e[5]]))
+            a["f_type"] = np.append(a["f_type"], np.uint32(FORCE_SPRING))
+            a["f_index"] = np.append(a["f_index"], np.uint32(len(a["sp_param"]) // 3 - 1))
+        else:
+            raise ValueError(kind)
+    ns = len(a["s_particle"])
+    carry = {"particle_prev": np.array(particle_prev, np.int64), "tet_prev": np.array(tet_prev, np.int64),
+             "sphere_prev": np.array(list(range(s0)) + [-1] * (ns - s0), np.int64),
+             "world_prev": np.arange(w0, dtype=np.int64)}
+    return a, carry

@@ -200,6 +200,44 @@ b3g_status b3g_body_build_batch(b3g_body body, const b3g_body_desc* descs, uint3
 b3g_status b3g_get_instance_count(b3g_body body, uint32_t* count);
 
 /*
+ * Topology edits: the body after b3Body::CreateParticle / CreateFixture /
+ * CreateForce / Destroy* calls (body.cpp:62-354) between steps. The caller
+ * passes the edited description (surviving objects in their creation order,
+ * new ones after them, as the reference's lists hold them) and, per object of
+ * the new description, the creation index of the object it continues in the
+ * previous description, or -1 for a new object (any map may be NULL: nothing
+ * of that kind carries over). Carried: particle positions and velocities
+ * (from the device; new particles take the description's), tetrahedron
+ * warm-start quaternions (tetrahedron_element_force.h:154), and the contacts
+ * whose sphere and world fixture both carry over, with their manifold, normal
+ * force and friction flag, in their list order. Everything else is rebuilt as
+ * b3g_body_build_batch does; solver, reduction, graph and action-force
+ * settings are kept. A single body passes count = 1. A map entry at or past
+ * the previous description's count is rejected (B3G_INVALID_ARGUMENT) before
+ * anything changes.
+ */
+typedef struct b3g_carry {
+	const int64_t* particle_prev;   /* n_particles entries */
+	const int64_t* tet_prev;        /* n_tet entries */
+	const int64_t* sphere_prev;     /* n_spheres entries */
+	const int64_t* world_prev;      /* n_world entries */
+} b3g_carry;
+
+b3g_status b3g_body_rebuild(b3g_body body, const b3g_body_desc* descs, uint32_t count, const b3g_carry* carry);
+
+/*
+ * Action forces: m_f1.. of b3SpringForce (GetActionForce / GetReactionForce,
+ * spring_force.h:155-163), b3StretchForce and b3ShearForce (GetActionForce1..3)
+ * and b3MouseForce (GetActionForce1..4): the force each applied to its
+ * particles, summed over the Newton iterations of the last step. Off by
+ * default (it costs one pass over those forces per iteration). get writes,
+ * for the forces of `force_type` in creation order, 2 (spring), 3 (stretch,
+ * shear) or 4 (mouse) vectors of 3 floats each, at most `capacity` floats.
+ */
+b3g_status b3g_set_action_forces(b3g_body body, int enabled);
+b3g_status b3g_get_action_forces(b3g_body body, uint32_t force_type, float* out, uint64_t capacity);
+
+/*
  * Replaces b3Body::RayCastSingle(output, p1, p2) (dynamics/body.cpp:497-521,
  * declared body.h:156) for `count` rays p1[i] -> p2[i] (3 floats each) against
  * the triangle fixtures of instance `instance` (0 for a single body), at the

@@ -162,11 +162,21 @@ def _lib(which: str):
         lib[p + "build_sdf"].argtypes = [f32p, C.c_uint32, u32p, C.c_uint32, C.c_uint32, f32p, C.c_float, f32p, u32p,
                                          f32p, C.c_uint64]
         lib[p + "build_sdf"].restype = C.c_uint64
+        lib[p + "get_action_forces"].argtypes = [C.c_void_p, C.c_uint32, f32p]
+        lib[p + "get_action_forces"].restype = C.c_uint32
+        if which == "oracle":
+            lib.b3o_carry.argtypes = [C.c_void_p, C.c_void_p] + [C.POINTER(C.c_int64)] * 4
         if which == "oracle":
             lib.b3o_step.argtypes = [C.c_void_p, C.c_float, C.c_uint32, C.c_uint32, C.c_float, C.c_float]
             lib.b3o_get_state64.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
             lib.b3o_spmv.argtypes = [C.c_uint32, u32p, u32p, f32p, f32p, f32p]
         else:
+            lib.refh_destroy_force.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
+            lib.refh_create_particle.argtypes = [C.c_void_p, C.c_uint32, f32p, f32p, C.c_float]
+            lib.refh_create_particle.restype = C.c_uint32
+            lib.refh_create_sphere.argtypes = [C.c_void_p, C.c_uint32, C.c_float, C.c_float]
+            lib.refh_create_spring.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float]
+            lib.refh_create_spring.restype = C.c_uint32
             lib.refh_step.argtypes = [C.c_void_p, C.c_float, C.c_uint32, C.c_uint32]
             lib.refh_step_tol.argtypes = [C.c_void_p, C.c_float, C.c_uint32, C.c_uint32, C.c_float, C.c_float]
             lib.refh_reset_stats.argtypes = []
@@ -241,6 +251,12 @@ class _Body:
         self._f("get_tet_quats")(self.h, _ptr(q, f32p))
         return q[: 4 * self.n_tet].reshape(-1, 4)
 
+    def action_forces(self, ftype: int, per: int):
+        """Action forces of the forces of one type in creation order: array (count, per, 3)."""
+        out = np.zeros(3 * per * 65536, np.float32)
+        n = self._f("get_action_forces")(self.h, ftype, _ptr(out, f32p))
+        return out[: 3 * per * n].reshape(n, per, 3)
+
     def ray_cast(self, p1, p2):
         """``b3Body::RayCastSingle`` for each ray p1[i] -> p2[i]: (creation index of the triangle fixture hit or
         -1, fraction, normal)."""
@@ -269,6 +285,12 @@ class _Body:
 
 class OracleBody(_Body):
     """The CPU restatement (``precision`` 32 or 64)."""
+
+    def carry_from(self, old, carry: dict):
+        """Takes over ``old``'s state for the objects ``carry`` maps (bounce_softbody_amd.scene.edit_arrays)."""
+        i64 = C.POINTER(C.c_int64)
+        arrs = [np.ascontiguousarray(carry[k], np.int64) for k in ("particle_prev", "tet_prev", "sphere_prev", "world_prev")]
+        self.lib.b3o_carry(self.h, old.h, *[a.ctypes.data_as(i64) for a in arrs])
 
     def assemble64(self, dt):
         """Like :meth:`assemble` (list-order CSR, column-major blocks) without rounding to float."""
@@ -321,6 +343,24 @@ class RefBody(_Body):
 
     def step_timed(self, steps, dt, iters=1, sub_iters=50, tol=0.0, sub_tol=0.0):
         return self.lib.refh_step_timed(self.h, steps, C.c_float(dt), iters, sub_iters, C.c_float(tol), C.c_float(sub_tol))
+
+    def edit(self, edits):
+        """Applies topology edits to the live body through the reference's own API (see
+        bounce_softbody_amd.scene.edit_arrays for the edit records)."""
+        for e in edits:
+            if e[0] == "destroy_force":
+                self.lib.refh_destroy_force(self.h, e[1], e[2])
+            elif e[0] == "create_particle":
+                pos = np.ascontiguousarray(e[2], np.float32)
+                vel = np.ascontiguousarray(e[3], np.float32)
+                self.lib.refh_create_particle(self.h, e[1], _ptr(pos, f32p), _ptr(vel, f32p), C.c_float(e[4]))
+                self.n += 1
+            elif e[0] == "create_sphere":
+                self.lib.refh_create_sphere(self.h, e[1], C.c_float(e[2]), C.c_float(e[3]))
+            elif e[0] == "create_spring":
+                self.lib.refh_create_spring(self.h, e[1], e[2], C.c_float(e[3]), C.c_float(e[4]), C.c_float(e[5]))
+            else:
+                raise ValueError(e[0])
 
 
 def uv_initialize(which: str, A, B, C_):

@@ -55,23 +55,27 @@ template <class T> struct StretchForce {      // stretch_force.h
 	T alpha, du1, dv1, du2, dv2, inv_det;
 	V3<T> dwudx, dwvdx;
 	T ks_u, kd_u, b_u, ks_v, kd_v, b_v;
+	V3<T> act[3];                             // m_f1..m_f3
 };
 template <class T> struct ShearForce {        // shear_force.h
 	uint32_t p[3];
 	T alpha, du1, dv1, du2, dv2, inv_det;
 	V3<T> dwudx, dwvdx;
 	T ks, kd;
+	V3<T> act[3];                             // m_f1..m_f3: accumulated over a step's ApplyForces calls
 };
 template <class T> struct SpringForce {       // spring_force.h
 	uint32_t p[2];
 	T L0, ks, kd;
+	V3<T> act[2];                             // m_f1, m_f2
 };
 template <class T> struct MouseForce {        // mouse_force.h
 	uint32_t p[4];
 	T w2, w3, w4, ks, kd, L0;
+	V3<T> act[4];                             // m_f1..m_f4
 };
 
-template <class T> struct SphereFixture { uint32_t p; T radius, friction; };
+template <class T> struct SphereFixture { uint32_t p; T radius, friction; uint32_t index; };   // index: creation order
 template <class T> struct TriFixture { uint32_t p[3]; T area, density; uint32_t index; };   // index: creation order
 template <class T> struct TetFixture { uint32_t p[4]; T volume, density; };
 
@@ -353,7 +357,7 @@ template <class T> void Body<T>::Build(const b3s_scene& sc) {
 			uint32_t p = L(sc.s_particle[c]);
 			if (has[p]) continue;
 			has[p] = 1;
-			created.push_back({p, T(sc.s_radius[c]), T(sc.s_friction[c])});
+			created.push_back({p, T(sc.s_radius[c]), T(sc.s_friction[c]), c});
 		}
 		spheres.assign(created.rbegin(), created.rend());
 	}
@@ -465,6 +469,10 @@ template <class T> void Body<T>::Build(const b3s_scene& sc) {
 		default: break;
 		}
 	}
+	for (auto& e : stretches) for (auto& a : e.act) a.SetZero();
+	for (auto& e : shears) for (auto& a : e.act) a.SetZero();
+	for (auto& e : springs) for (auto& a : e.act) a.SetZero();
+	for (auto& e : mice) for (auto& a : e.act) a.SetZero();
 	// World fixtures: body.cpp:337-354 (head insertion), world_fixture.cpp:39-44
 	worlds.clear();
 	for (uint32_t c = sc.n_world; c-- > 0;) {
@@ -720,6 +728,7 @@ void Body<T>::ApplyStretch(StretchForce<T>& e, const DVec<T>& x, const DVec<T>& 
 			V3<T> fs[3];
 			for (int i = 0; i < 3; ++i) fs[i] = -pt.ks * C * dCdx[i];
 			for (int i = 0; i < 3; ++i) f[is[i]] += fs[i];
+			for (int i = 0; i < 3; ++i) e.act[i] += fs[i];
 			M33<T> K[3][3];
 			for (int i = 0; i < 3; ++i)
 				for (int j = 0; j < 3; ++j) {
@@ -738,6 +747,7 @@ void Body<T>::ApplyStretch(StretchForce<T>& e, const DVec<T>& x, const DVec<T>& 
 			V3<T> fs[3];
 			for (int i = 0; i < 3; ++i) fs[i] = -pt.kd * dCdt * dCdx[i];
 			for (int i = 0; i < 3; ++i) f[is[i]] += fs[i];
+			for (int i = 0; i < 3; ++i) e.act[i] += fs[i];
 			M33<T> K[3][3];
 			for (int i = 0; i < 3; ++i)
 				for (int j = 0; j < 3; ++j) K[i][j] = -pt.kd * Outer(dCdx[i], dCdx[j]);
@@ -764,6 +774,7 @@ void Body<T>::ApplyShear(ShearForce<T>& e, const DVec<T>& x, const DVec<T>& v, D
 		V3<T> fs[3];
 		for (int i = 0; i < 3; ++i) fs[i] = -e.ks * C * dCdx[i];
 		for (int i = 0; i < 3; ++i) f[is[i]] += fs[i];
+		for (int i = 0; i < 3; ++i) e.act[i] += fs[i];
 		M33<T> K[3][3];
 		for (int i = 0; i < 3; ++i)
 			for (int j = 0; j < 3; ++j) K[i][j] = -e.ks * Outer(dCdx[i], dCdx[j]);
@@ -775,6 +786,7 @@ void Body<T>::ApplyShear(ShearForce<T>& e, const DVec<T>& x, const DVec<T>& v, D
 		V3<T> fs[3];
 		for (int i = 0; i < 3; ++i) fs[i] = -e.kd * dCdt * dCdx[i];
 		for (int i = 0; i < 3; ++i) f[is[i]] += fs[i];
+		for (int i = 0; i < 3; ++i) e.act[i] += fs[i];
 		M33<T> K[3][3];
 		for (int i = 0; i < 3; ++i)
 			for (int j = 0; j < 3; ++j) K[i][j] = -e.kd * Outer(dCdx[i], dCdx[j]);
@@ -799,6 +811,8 @@ void Body<T>::ApplySpring(SpringForce<T>& e, const DVec<T>& x, const DVec<T>& v,
 			V3<T> f2 = -f1;
 			f[i1] += f1;
 			f[i2] += f2;
+			e.act[0] += f1;
+			e.act[1] += f2;
 			M33<T> K11 = -e.ks * (Outer(dx, dx) + (T(1) - e.L0 / L) * (I - Outer(dx, dx)));
 			M33<T> K12 = -K11;
 			M33<T> K21 = K12;
@@ -815,6 +829,8 @@ void Body<T>::ApplySpring(SpringForce<T>& e, const DVec<T>& x, const DVec<T>& v,
 		V3<T> f2 = -f1;
 		f[i1] += f1;
 		f[i2] += f2;
+		e.act[0] += f1;
+		e.act[1] += f2;
 		M33<T> K11 = -e.kd * Outer(n, n);
 		M33<T> K12 = -K11;
 		M33<T> K21 = K12;
@@ -851,6 +867,7 @@ void Body<T>::ApplyMouse(MouseForce<T>& e, const DVec<T>& x, const DVec<T>& v, D
 			V3<T> fs[4];
 			for (int i = 0; i < 4; ++i) fs[i] = -e.ks * C * dCdx[i];
 			for (int i = 0; i < 4; ++i) f[is[i]] += fs[i];
+			for (int i = 0; i < 4; ++i) e.act[i] += fs[i];
 			T L3 = L * L * L;
 			T inv_L3 = L3 > T(0) ? T(1) / L3 : T(0);
 			M33<T> d2[4][4];
@@ -882,6 +899,7 @@ void Body<T>::ApplyMouse(MouseForce<T>& e, const DVec<T>& x, const DVec<T>& v, D
 		V3<T> fs[4];
 		for (int i = 0; i < 4; ++i) fs[i] = -e.kd * dCdt * dCdx[i];
 		for (int i = 0; i < 4; ++i) f[is[i]] += fs[i];
+		for (int i = 0; i < 4; ++i) e.act[i] += fs[i];
 		for (int i = 0; i < 4; ++i)
 			for (int j = 0; j < 4; ++j) dfdv.at(is[i], is[j]) += -e.kd * Outer(dCdx[i], dCdx[j]);
 	}
@@ -1584,6 +1602,11 @@ template <class T> void Body<T>::Step(T dt, uint32_t iters, uint32_t subIters, T
 	if (!(tol > T(0))) tol = Eps<T>::value();
 	if (!(subTol > T(0))) subTol = Eps<T>::value();
 	UpdateContacts();
+	// b3Force::ClearForces (body.cpp:566-570): the action forces restart every step
+	for (auto& e : stretches) for (auto& a : e.act) a.SetZero();
+	for (auto& e : shears) for (auto& a : e.act) a.SetZero();
+	for (auto& e : springs) for (auto& a : e.act) a.SetZero();
+	for (auto& e : mice) for (auto& a : e.act) a.SetZero();
 	if (dt > T(0)) Solve(dt, iters, subIters, tol, subTol);
 	for (auto& p : particles) { p.force.SetZero(); p.translation.SetZero(); }
 	// Triangle-fixture proxy synchronisation (body.cpp:584-600) only moves

@@ -5,6 +5,7 @@
  */
 #include "cpu_ref.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <memory>
@@ -291,6 +292,80 @@ void b3o_spmv(uint32_t n, const uint32_t* rowptr, const uint32_t* cols, const fl
 			acc += a * MakeV3(x[3 * cols[e]], x[3 * cols[e] + 1], x[3 * cols[e] + 2]);
 		}
 		y[3 * i] = acc.x; y[3 * i + 1] = acc.y; y[3 * i + 2] = acc.z;
+	}
+}
+
+// The state a body keeps across topology edits, from `src` into `dst` (built
+// from the edited description): per object of dst its creation index in src
+// or -1 (new). Particle positions and velocities, tetrahedron warm-start
+// quaternions, and the contacts whose sphere and world fixture both carry
+// over, in src's list order (the contact list is edited only by removal).
+void b3o_carry(void* dst, void* src, const int64_t* particle_prev, const int64_t* tet_prev, const int64_t* sphere_prev,
+               const int64_t* world_prev) {
+	Body<float>& D = *((Handle*)dst)->f;
+	const Body<float>& S = *((Handle*)src)->f;
+	const size_t nD = D.particles.size(), nS = S.particles.size();
+	for (size_t c = 0; c < nD; ++c) {
+		const int64_t q = particle_prev[c];
+		if (q < 0) continue;
+		D.particles[nD - 1 - c].x = S.particles[nS - 1 - (size_t)q].x;
+		D.particles[nD - 1 - c].v = S.particles[nS - 1 - (size_t)q].v;
+	}
+	const size_t mD = D.tets.size(), mS = S.tets.size();
+	for (size_t c = 0; c < mD; ++c)
+		if (tet_prev[c] >= 0) D.tets[mD - 1 - c].q = S.tets[mS - 1 - (size_t)tet_prev[c]].q;
+	// creation index -> list position, and old -> new creation index
+	uint32_t maxIndex = 0;
+	for (const auto& sp : D.spheres) maxIndex = std::max(maxIndex, sp.index);
+	for (const auto& sp : S.spheres) maxIndex = std::max(maxIndex, sp.index);
+	std::vector<int64_t> sphereNew(maxIndex + 1, -1), sphereListD(maxIndex + 1, -1);
+	std::vector<int64_t> worldNew(S.worlds.size() + 1, -1);
+	for (size_t k = 0; k < D.spheres.size(); ++k) {
+		const uint32_t c = D.spheres[k].index;
+		sphereListD[c] = (int64_t)k;
+		if (sphere_prev[c] >= 0) sphereNew[sphere_prev[c]] = c;
+	}
+	const size_t wD = D.worlds.size(), wS = S.worlds.size();
+	for (size_t c = 0; c < wD; ++c)
+		if (world_prev[c] >= 0) worldNew[world_prev[c]] = (int64_t)c;
+	D.contacts.clear();
+	D.contactKeys.clear();
+	for (const Contact<float>& ct : S.contacts) {
+		const int64_t sc = sphereNew[S.spheres[ct.sphere].index];
+		const int64_t wc = worldNew[wS - 1 - ct.world];
+		if (sc < 0 || wc < 0) continue;
+		Contact<float> n = ct;
+		n.sphere = (uint32_t)sphereListD[sc];
+		n.world = (uint32_t)(wD - 1 - (size_t)wc);
+		D.contacts.push_back(n);
+		D.contactKeys.insert({n.sphere, n.world});
+	}
+}
+
+// Action forces (m_f1.. of spring / stretch / shear / mouse forces, e.g.
+// spring_force.h:155-163) of the forces of one type in creation order: 2, 3,
+// 3 or 4 vectors per force. Returns the force count.
+uint32_t b3o_get_action_forces(void* p, uint32_t type, float* out) {
+	Handle* h = (Handle*)p;
+	Body<float>& b = *h->f;
+	auto put = [&](const auto& table, int k) {
+		const uint32_t m = (uint32_t)table.size();
+		for (uint32_t c = 0; c < m; ++c) {   // tables are in list order: reverse creation within the type
+			const auto& e = table[m - 1 - c];
+			for (int i = 0; i < k; ++i) {
+				out[3 * ((size_t)k * c + i)] = e.act[i].x;
+				out[3 * ((size_t)k * c + i) + 1] = e.act[i].y;
+				out[3 * ((size_t)k * c + i) + 2] = e.act[i].z;
+			}
+		}
+		return m;
+	};
+	switch (type) {
+	case B3S_FORCE_SPRING: return put(b.springs, 2);
+	case B3S_FORCE_STRETCH: return put(b.stretches, 3);
+	case B3S_FORCE_SHEAR: return put(b.shears, 3);
+	case B3S_FORCE_MOUSE: return put(b.mice, 4);
+	default: return 0;
 	}
 }
 

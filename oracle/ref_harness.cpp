@@ -64,6 +64,7 @@ struct RefBody {
 	uint32_t nworld = 0;
 	std::vector<b3WorldFixture*> worlds;  // creation order
 	std::vector<b3TriangleFixture*> triangles;   // creation order (a duplicate maps to the fixture it repeats)
+	std::vector<b3Force*> byType[7];             // forces of each type, index = position in the type's table
 	// captured linear system (list order)
 	std::vector<uint32_t> capRowptr, capCols;
 	std::vector<float> capVals, capB, capF;
@@ -164,6 +165,10 @@ void* refh_create(const b3s_scene* sc, int fast) {
 		body.m_tetrahedronList = t; ++body.m_tetrahedronCount;
 	}
 	if (fast) body.ResetMass();
+	auto keep = [&](uint32_t type, uint32_t k, b3Force* f) {
+		if (rb->byType[type].size() <= k) rb->byType[type].resize(k + 1, nullptr);
+		rb->byType[type][k] = f;
+	};
 	for (uint32_t c = 0; c < sc->n_forces; ++c) {
 		uint32_t k = sc->f_index[c];
 		switch (sc->f_type[c]) {
@@ -177,6 +182,7 @@ void* refh_create(const b3s_scene* sc, int fast) {
 			d.stiffnessDamping = sc->tet_param[3 * k + 2];
 			d.userIndex = c;
 			rb->tets.push_back(body.CreateForce(d));
+			keep(B3S_FORCE_TET_ELEMENT, k, rb->tets.back());
 			break;
 		}
 		case B3S_FORCE_TRI_ELEMENT: {
@@ -189,7 +195,7 @@ void* refh_create(const b3s_scene* sc, int fast) {
 			d.youngModulusX = q[0]; d.youngModulusY = q[1]; d.shearModulus = q[2];
 			d.poissonRationXY = q[3]; d.poissonRationYX = q[4]; d.stiffnessDamping = q[5];
 			d.userIndex = c;
-			body.CreateForce(d);
+			keep(B3S_FORCE_TRI_ELEMENT, k, body.CreateForce(d));
 			break;
 		}
 		case B3S_FORCE_STRETCH: {
@@ -201,7 +207,7 @@ void* refh_create(const b3s_scene* sc, int fast) {
 			d.stiffness_u = q[6]; d.damping_stiffness_u = q[7]; d.b_u = q[8];
 			d.stiffness_v = q[9]; d.damping_stiffness_v = q[10]; d.b_v = q[11];
 			d.userIndex = c;
-			body.CreateForce(d);
+			keep(B3S_FORCE_STRETCH, k, body.CreateForce(d));
 			break;
 		}
 		case B3S_FORCE_SHEAR: {
@@ -212,7 +218,7 @@ void* refh_create(const b3s_scene* sc, int fast) {
 			d.u1 = q[0]; d.v1 = q[1]; d.u2 = q[2]; d.v2 = q[3]; d.u3 = q[4]; d.v3 = q[5];
 			d.stiffness = q[6]; d.dampingStiffness = q[7];
 			d.userIndex = c;
-			body.CreateForce(d);
+			keep(B3S_FORCE_SHEAR, k, body.CreateForce(d));
 			break;
 		}
 		case B3S_FORCE_SPRING: {
@@ -221,7 +227,7 @@ void* refh_create(const b3s_scene* sc, int fast) {
 			const float* q = sc->sp_param + 3 * k;
 			d.length = q[0]; d.stiffness = q[1]; d.dampingStiffness = q[2];
 			d.userIndex = c;
-			body.CreateForce(d);
+			keep(B3S_FORCE_SPRING, k, body.CreateForce(d));
 			break;
 		}
 		case B3S_FORCE_MOUSE: {
@@ -231,7 +237,7 @@ void* refh_create(const b3s_scene* sc, int fast) {
 			const float* q = sc->mo_param + 6 * k;
 			d.w2 = q[0]; d.w3 = q[1]; d.w4 = q[2]; d.length = q[3]; d.stiffness = q[4]; d.dampingStiffness = q[5];
 			d.userIndex = c;
-			body.CreateForce(d);
+			keep(B3S_FORCE_MOUSE, k, body.CreateForce(d));
 			break;
 		}
 		default: break;
@@ -559,6 +565,88 @@ uint32_t refh_solve_cg(uint32_t n, const uint32_t* rowptr, const uint32_t* cols,
 	b3SparseSolveCG(&out, &in);
 	for (uint32_t i = 0; i < n; ++i) { x[3 * i] = xv[i].x; x[3 * i + 1] = xv[i].y; x[3 * i + 2] = xv[i].z; }
 	return out.iterations;
+}
+
+// Topology edits on the live reference body (body.cpp:62-354), creation
+// indices as the scene numbers them; new objects take the next index.
+void refh_destroy_force(void* p, uint32_t type, uint32_t k) {
+	RefBody* rb = (RefBody*)p;
+	rb->body.DestroyForce(rb->byType[type][k]);
+	rb->byType[type][k] = nullptr;
+}
+
+uint32_t refh_create_particle(void* p, uint32_t type, const float* pos, const float* vel, float damping) {
+	RefBody* rb = (RefBody*)p;
+	b3ParticleDef pd;
+	pd.type = (b3ParticleType)type;
+	pd.position = V(pos);
+	pd.velocity = V(vel);
+	pd.damping = damping;
+	pd.userIndex = (uint32)rb->particles.size();
+	rb->particles.push_back(rb->body.CreateParticle(pd));
+	return (uint32_t)rb->particles.size() - 1;
+}
+
+void refh_create_sphere(void* p, uint32_t particle, float radius, float friction) {
+	RefBody* rb = (RefBody*)p;
+	b3SphereFixtureDef sd;
+	sd.p = rb->particles[particle];
+	sd.radius = radius;
+	sd.friction = friction;
+	rb->body.CreateSphere(sd);
+}
+
+uint32_t refh_create_spring(void* p, uint32_t p1, uint32_t p2, float length, float ks, float kd) {
+	RefBody* rb = (RefBody*)p;
+	b3SpringForceDef d;
+	d.p1 = rb->particles[p1];
+	d.p2 = rb->particles[p2];
+	d.length = length;
+	d.stiffness = ks;
+	d.dampingStiffness = kd;
+	rb->byType[B3S_FORCE_SPRING].push_back(rb->body.CreateForce(d));
+	return (uint32_t)rb->byType[B3S_FORCE_SPRING].size() - 1;
+}
+
+// Action forces of the forces of one type, table order: spring
+// GetActionForce / GetReactionForce (spring_force.h:155-163), stretch and
+// shear GetActionForce1..3, mouse GetActionForce1..4. Returns the count.
+uint32_t refh_get_action_forces(void* p, uint32_t type, float* out) {
+	RefBody* rb = (RefBody*)p;
+	const std::vector<b3Force*>& fs = rb->byType[type];
+	auto put = [&](size_t slot, const b3Vec3& v) { out[3 * slot] = v.x; out[3 * slot + 1] = v.y; out[3 * slot + 2] = v.z; };
+	for (size_t c = 0; c < fs.size(); ++c) {
+		if (!fs[c]) {   // destroyed
+			const int per = type == B3S_FORCE_SPRING ? 2 : type == B3S_FORCE_MOUSE ? 4 : 3;
+			for (int i = 0; i < per; ++i) put(per * c + i, b3Vec3(0.0f, 0.0f, 0.0f));
+			continue;
+		}
+		switch (type) {
+		case B3S_FORCE_SPRING: {
+			b3SpringForce* f = (b3SpringForce*)fs[c];
+			put(2 * c, f->GetActionForce()); put(2 * c + 1, f->GetReactionForce());
+			break;
+		}
+		case B3S_FORCE_STRETCH: {
+			b3StretchForce* f = (b3StretchForce*)fs[c];
+			put(3 * c, f->GetActionForce1()); put(3 * c + 1, f->GetActionForce2()); put(3 * c + 2, f->GetActionForce3());
+			break;
+		}
+		case B3S_FORCE_SHEAR: {
+			b3ShearForce* f = (b3ShearForce*)fs[c];
+			put(3 * c, f->GetActionForce1()); put(3 * c + 1, f->GetActionForce2()); put(3 * c + 2, f->GetActionForce3());
+			break;
+		}
+		case B3S_FORCE_MOUSE: {
+			b3MouseForce* f = (b3MouseForce*)fs[c];
+			put(4 * c, f->GetActionForce1()); put(4 * c + 1, f->GetActionForce2());
+			put(4 * c + 2, f->GetActionForce3()); put(4 * c + 3, f->GetActionForce4());
+			break;
+		}
+		default: return 0;
+		}
+	}
+	return (uint32_t)fs.size();
 }
 
 // b3Body::RayCastSingle (body.cpp:497-521) for n rays: index = creation

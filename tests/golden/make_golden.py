@@ -14,11 +14,14 @@ it through ``oracle/ref_harness.cpp`` on the parity scenes of
 ``sdf_build.npz`` holds ``b3BuildSDF`` grids of the reference for three
 closed meshes (with the meshes, cell sizes and box extensions).
 
+``tearing.npz`` holds a topology-edit run of the reference (the spring cloth
+split at its most tensioned springs between steps, through Create / Destroy).
+
 ``ray_cast.npz`` holds ``b3Body::RayCastSingle`` results of the reference
 (creation index of the triangle fixture hit or -1, fraction, normal) for
 seeded rays at the state reached after 12 steps of four scenes.
 
-Usage: python tests/golden/make_golden.py [rays | sdf]   (needs oracle/_ref built; ``rays`` / ``sdf``: that fixture only)
+Usage: python tests/golden/make_golden.py [rays | sdf | tearing]   (needs oracle/_ref built; an argument: that fixture only)
 """
 
 import hashlib
@@ -144,6 +147,46 @@ def sdf_goldens():
     np.savez_compressed(os.path.join(OUT, "sdf_build.npz"), **rec)
 
 
+def tearing_golden(pre: int = 12, post: int = 15):
+    """Cloth tearing (cloth_tearing.h, simplified): the reference steps the spring cloth, splits it at its two most
+    tensioned springs through its own Create / Destroy API, and steps on. Records the edits (JSON), per-step
+    digests before and after, the spring action forces at the edit (creation order) and at the end (the edited
+    spring table: surviving springs in order, then the new ones), the final state and contacts."""
+    import json
+    arrays = configs.scene_tearing().arrays()
+    body = oracle.RefBody(arrays)
+    dig = []
+    for _ in range(pre):
+        body.step(configs.DT, 2, 20)
+        dig.append(digest(*body.state()))
+    x, v = body.state()
+    act0 = body.action_forces(3, 2)
+    edits = configs.tear_edits(arrays, x, v, act0, count=2)
+    body.edit(edits)
+    for _ in range(post):
+        body.step(configs.DT, 2, 20)
+        dig.append(digest(*body.state()))
+    x, v = body.state()
+    p, w, f = body.contacts()
+    act = body.action_forces(3, 2)
+    destroyed = {e[2] for e in edits if e[0] == "destroy_force"}
+    keep = [k for k in range(len(act)) if k not in destroyed]
+    enc = [[e[0]] + [np.asarray(a).tolist() if isinstance(a, np.ndarray) else a for a in e[1:]] for e in edits]
+    rec = {"edits": np.array(json.dumps(enc)), "pre": np.int64(pre), "post": np.int64(post), "digests": np.stack(dig),
+           "action_at_edit": act0, "action_final": act[keep], "x": x, "v": v, "contact_particle": p,
+           "contact_world": w, "contact_force": f}
+    np.savez_compressed(os.path.join(OUT, "tearing.npz"), **rec)
+    print(f"tearing: {len(edits)} edits, {len(p)} contacts at the end")
+
+
+def decode_edits(blob) -> list:
+    import json
+    out = []
+    for e in json.loads(str(blob)):
+        out.append(tuple(np.float32(a) if isinstance(a, list) else a for a in e))
+    return out
+
+
 def main():
     if not oracle.reference_available():
         sys.exit("oracle/_ref/libb3ref.so is missing: run `make -C oracle ref` where the reference tree exists")
@@ -153,8 +196,12 @@ def main():
     if sys.argv[1:] == ["sdf"]:
         sdf_goldens()
         return
+    if sys.argv[1:] == ["tearing"]:
+        tearing_golden()
+        return
     ray_goldens()
     sdf_goldens()
+    tearing_golden()
     for name, fn in configs.PARITY_SCENES.items():
         rec = run(name, fn().arrays(), schedule(name, 30))
         np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **rec)

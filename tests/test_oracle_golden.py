@@ -19,7 +19,7 @@ from tests.golden.make_golden import digest, external_loads
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 FIXTURES = sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
-                  if os.path.basename(p) not in ("ray_cast.npz", "sdf_build.npz"))
+                  if os.path.basename(p) not in ("ray_cast.npz", "sdf_build.npz", "tearing.npz"))
 
 
 def scene_arrays(name):
@@ -128,3 +128,34 @@ def test_sdf_build_approaches_the_sphere_distance():
     assert np.abs(np.abs(vox[out]) - (r[out] - 0.5)).max() < 0.01
     assert (vox[out] > 0).mean() > 0.95
     assert (vox[r < 0.3] < 0).mean() > 0.95
+
+
+def test_topology_edits_match_reference_golden():
+    """Cloth tearing: the reference edits its live body (Create / Destroy) between steps; the oracle rebuilt from
+    the edited description, taking over the previous body's state, continues bit for bit (states, contacts, spring
+    action forces)."""
+    from bounce_softbody_amd.scene import edit_arrays
+    from tests.golden.make_golden import decode_edits
+    g = np.load(os.path.join(GOLDEN, "tearing.npz"))
+    arrays = configs.scene_tearing().arrays()
+    o = oracle.OracleBody(arrays)
+    pre, post = int(g["pre"]), int(g["post"])
+    for k in range(pre):
+        o.step(configs.DT, 2, 20)
+        assert np.array_equal(digest(*o.state()), g["digests"][k]), k
+    assert np.array_equal(o.action_forces(3, 2), g["action_at_edit"])
+    x, v = o.state()
+    edits = decode_edits(g["edits"])
+    assert [e[0] for e in edits] == [e[0] for e in configs.tear_edits(arrays, x, v, g["action_at_edit"], count=2)]
+    arr2, carry = edit_arrays(arrays, edits)
+    o2 = oracle.OracleBody(arr2)
+    o2.carry_from(o, carry)
+    for k in range(post):
+        o2.step(configs.DT, 2, 20)
+        assert np.array_equal(digest(*o2.state()), g["digests"][pre + k]), k
+    x, v = o2.state()
+    assert np.array_equal(x, g["x"]) and np.array_equal(v, g["v"])
+    p, w, f = o2.contacts()
+    assert np.array_equal(p, g["contact_particle"]) and np.array_equal(w, g["contact_world"])
+    assert np.array_equal(f, g["contact_force"])
+    assert np.array_equal(o2.action_forces(3, 2), g["action_final"])
