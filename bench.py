@@ -441,7 +441,6 @@ def main():
     ap.add_argument("--workload", choices=["two_body", "precision", "batch", "contact", "sdf_build", "ray_cast"],
                     default="two_body")
     ap.add_argument("--sdf-subdivisions", type=int, default=4, help="sdf_build workload: icosphere subdivisions")
-    ap.add_argument("--sdf-res", type=int, default=128, help="sdf_build workload: voxels per axis")
     ap.add_argument("--rays", type=int, default=1000000, help="ray_cast workload: rays per query")
     ap.add_argument("--bodies", type=int, default=None, help="batch / contact workloads: number of bodies "
                                                                "(default 4096 / 256)")
@@ -451,7 +450,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="batch / contact workloads: uncaptured steps")
     ap.add_argument("--contact-cloth", type=int, default=127, help="contact workload: cloth cells per side")
     ap.add_argument("--contact-tet", type=int, default=16, help="contact workload: tet block cells per axis")
-    ap.add_argument("--sdf-res", type=int, default=128, help="contact workload: SDF voxels per axis")
+    ap.add_argument("--sdf-res", type=int, default=128, help="contact and sdf_build workloads: SDF voxels per axis")
     ap.add_argument("--pre-steps", type=int, default=None,
                     help="precision / contact workloads: untimed steps first (default 10 / 40)")
     ap.add_argument("--inner-iterations", type=int, default=50, help="precision workload: PCG iterations per sweep")
