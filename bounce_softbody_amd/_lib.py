@@ -19,7 +19,7 @@ f32p = C.POINTER(C.c_float)
 u8p = C.POINTER(C.c_uint8)
 
 B3G_OK = 0
-REDUCE_FP64, REDUCE_SEQUENTIAL, REDUCE_FP64_FUSED = 0, 1, 2
+REDUCE_FP64, REDUCE_SEQUENTIAL, REDUCE_FP64_FUSED, REDUCE_FP64_SEPARATE = 0, 1, 2, 3
 SOLVER_PCG, SOLVER_REFINED_FP32, SOLVER_REFINED_BF16 = 0, 1, 2
 STATUS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "out of device memory", 4: "degenerate element",
           5: "unsupported", 6: "body not built"}

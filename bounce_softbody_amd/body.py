@@ -232,7 +232,8 @@ class Body:
         check(self._L.b3g_set_graph(self._h, 1 if enabled else 0), "b3g_set_graph")
 
     def set_reduction(self, mode: int):
-        """_lib.REDUCE_FP64 (default) or _lib.REDUCE_SEQUENTIAL (the reference's FP32 accumulation order)."""
+        """_lib.REDUCE_FP64 (default; fused or separate by instance size), REDUCE_FP64_FUSED,
+        REDUCE_FP64_SEPARATE (identical results) or REDUCE_SEQUENTIAL (the reference's FP32 accumulation order)."""
         check(self._L.b3g_set_reduction(self._h, mode), "b3g_set_reduction")
 
     def save_state(self) -> bytes:

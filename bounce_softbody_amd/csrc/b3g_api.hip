@@ -568,7 +568,8 @@ b3g_status b3g_get_system(b3g_body h, uint32_t* rowptr, uint32_t* cols, float* v
 
 b3g_status b3g_set_reduction(b3g_body h, int mode) {
 	CHECK_BODY(h);
-	if (mode != B3G_REDUCE_FP64 && mode != B3G_REDUCE_SEQUENTIAL && mode != B3G_REDUCE_FP64_FUSED) {
+	if (mode != B3G_REDUCE_FP64 && mode != B3G_REDUCE_SEQUENTIAL && mode != B3G_REDUCE_FP64_FUSED &&
+	    mode != B3G_REDUCE_FP64_SEPARATE) {
 		set_error("unknown reduction mode");
 		return B3G_INVALID_ARGUMENT;
 	}

@@ -16,11 +16,14 @@ namespace b3g {
 enum : uint8_t { kStatic = 0, kKinematic = 1, kDynamic = 2 };
 
 // Contribution codes of the per-slot lists: a staged block index (< 2^30),
-// kTetCode | e for a tetrahedron block computed during assembly, or
+// kTetCode | (i + 4 j) << kTetBlockShift | e for block (i, j) of tetrahedron e
+// computed during assembly, or
 // kUVCode | variant << kUVShift | e for a stretch (variants 0..17: 9 part +
 // 3 i + j) or shear (18 + 3 i + j) block computed during assembly from the
 // element's record.
 constexpr uint32_t kTetCode = 0x80000000u;
+constexpr uint32_t kTetBlockShift = 27;
+constexpr uint32_t kTetMaxElements = 1u << kTetBlockShift;   // tetrahedra per body or batch
 constexpr uint32_t kUVCode = 0x40000000u;
 constexpr uint32_t kUVShift = 25;
 constexpr uint32_t kUVMaxElements = 1u << kUVShift;   // stretch / shear forces per body or batch
@@ -53,8 +56,7 @@ struct TetTable {
 	uint32_t m = 0;
 	DBuf p;          // 4 planes of uint32 (particle list index)
 	DBuf rest;       // 12 planes: rest positions x1..x4
-	DBuf invE;       // 9 planes
-	DBuf K0;         // 144 planes: K[k] (k = i + 4j) column-major 3x3
+	DBuf rec;        // 16 floats per element: gradients, volume, Lame parameters (b3g_tet.h)
 	DBuf q;          // float4 per element: warm-start quaternion (x y z s)
 	DBuf kd;         // 1 plane
 	DBuf param;      // 3 planes (E, nu, kd) for the set-up kernel
@@ -135,7 +137,7 @@ struct SolverState {
 	// running statistics (force_solver.cpp:30-34)
 	uint32_t statNewton, statMinSub, statMaxSub;
 	uint32_t contactCount;
-	// reduction mode of the dot products: 0 FP64 tree, 1 sequential FP32 in row order
+	// reduction mode of the dot products (B3G_REDUCE_*): FP64 tree (default / fused / separate) or sequential FP32 in row order
 	uint32_t seq;
 	// iterative refinement (precision form): FP64 residual norms of the
 	// Newton iteration's linear system, gate and counters of the sweeps
@@ -199,6 +201,7 @@ struct Body {
 	DBuf fPtr, fIdx;                              // per-row force contributions
 	DBuf tiles;                                   // uint32 row boundaries of SpMV / row-assembly tiles
 	uint32_t nTiles = 0;
+	uint32_t maxInstPartials = 0;   // largest tile or row-block count of one instance
 	DBuf tileInst;                                // uint32 instance per tile
 	DBuf rowBlk;                                  // uint32 boundaries of <= 256-row blocks (vector kernels)
 	DBuf blkInst;
@@ -206,8 +209,6 @@ struct Body {
 	DBuf slotTiles;                               // uint32 slot boundaries of slot-assembly tiles
 	DBuf slotTileInst;
 	uint32_t nSlotTiles = 0;
-	DBuf K0x, K0v;                                // tetrahedron K0 blocks in jx / jv contribution order (9 planes)
-	uint32_t nJx = 0, nJv = 0;                    // lengths of the jx / jv lists when K0x / K0v exist
 	bool anyJv = false;                           // the body has dfdv contributions beyond mass damping
 	std::vector<uint32_t> hostRowptr, hostCols;
 
@@ -312,7 +313,6 @@ b3g_status host_pattern(const b3g_body_desc& d, std::vector<uint32_t>& rowptr, s
 
 // Kernel launchers (b3g_kernels.hip)
 b3g_status launch_tet_setup(Body& b);
-b3g_status launch_tet_perm(Body& b, const std::vector<uint32_t>& src, DBuf& dst);
 b3g_status enqueue_step(Body& b, float dt, uint32_t iters, uint32_t subIters, float tol, float subTol,
                         bool assembleOnly);
 b3g_status prepare_step(Body& b, uint32_t subIters);

@@ -7,6 +7,7 @@
 // contributions are staged once per linearisation and gathered per block in
 // the reference's summation order (no atomics, deterministic).
 #include "b3g_collide.h"
+#include "b3g_tet.h"
 
 #include <algorithm>
 
@@ -14,6 +15,7 @@ namespace b3g {
 
 constexpr int kBlock = 256;        // threads per workgroup (4 wave64)
 constexpr int kTileCap = 1024;     // 3x3 blocks per SpMV tile chunk (12 KB of LDS products)
+constexpr uint32_t kFuseMaxPartials = 6144;   // default FP64 mode: fused reductions up to this many partials per instance
 constexpr int kTileRows = 256;     // rows per SpMV tile (one phase-2 thread each)
 constexpr int kRowPer = kTileCap / kBlock;   // chunk entries per thread
 
@@ -229,8 +231,8 @@ __global__ void __launch_bounds__(kFinBlock) k_finalize(Reduce rd, int sel, int 
 // ---------------------------------------------------------------------------
 // Set-up: tetrahedron rest data (tetrahedron_element_force.cpp:219-265)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_tet_setup(uint32_t m, const float* rest, const float* param, float* invE,
-                                                       float* K0, uint32_t* degenerate) {
+__global__ void __launch_bounds__(kBlock) k_tet_setup(uint32_t m, const float* rest, const float* param, float4* rec,
+                                                       uint32_t* degenerate) {
 	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= m) return;
 	V3 x[4];
@@ -244,51 +246,10 @@ __global__ void __launch_bounds__(kBlock) k_tet_setup(uint32_t m, const float* r
 	float V = fabs_(dt) / 6.0f;
 	dt = 1.0f / dt;
 	M33 iE = dt * adjugate(Em);
-	const float* pi = &iE.x.x;
-	for (int c = 0; c < 9; ++c) invE[(size_t)c * m + e] = pi[c];
-	// D (b3ComputeD, :37-57)
-	float lambda = (nu * E) / ((1.0f + nu) * (1.0f - 2.0f * nu));
-	float mu = E / (2.0f * (1.0f + nu));
-	float D[36];
-	for (int i = 0; i < 36; ++i) D[i] = 0.0f;
-	D[0] = lambda + 2.0f * mu; D[1] = lambda; D[2] = lambda;
-	D[6] = lambda; D[7] = lambda + 2.0f * mu; D[8] = lambda;
-	D[12] = lambda; D[13] = lambda; D[14] = lambda + 2.0f * mu;
-	D[21] = mu; D[28] = mu; D[35] = mu;
-	// B (b3ComputeB, :121-159), 6 x 12 column-major
-	float g[4][3];
-	g[1][0] = iE.x.x; g[2][0] = iE.x.y; g[3][0] = iE.x.z;
-	g[1][1] = iE.y.x; g[2][1] = iE.y.y; g[3][1] = iE.y.z;
-	g[1][2] = iE.z.x; g[2][2] = iE.z.y; g[3][2] = iE.z.z;
-	for (int a = 0; a < 3; ++a) g[0][a] = -g[1][a] - g[2][a] - g[3][a];
-	float B[72];
-	for (int v = 0; v < 4; ++v) {
-		float dx = g[v][0], dy = g[v][1], dz = g[v][2];
-		float* c0 = B + 6 * (3 * v);
-		float* c1 = B + 6 * (3 * v + 1);
-		float* c2 = B + 6 * (3 * v + 2);
-		c0[0] = dx; c0[1] = 0; c0[2] = 0; c0[3] = dy; c0[4] = 0; c0[5] = dz;
-		c1[0] = 0; c1[1] = dy; c1[2] = 0; c1[3] = dx; c1[4] = dz; c1[5] = 0;
-		c2[0] = 0; c2[1] = 0; c2[2] = dz; c2[3] = 0; c2[4] = dy; c2[5] = dx;
-	}
-	// BT_D = B^T D (12 x 6), then K = BT_D B (12 x 12), times V
-	float BTD[72];
-	for (int i = 0; i < 12; ++i)
-		for (int j = 0; j < 6; ++j) {
-			float acc = 0.0f;
-			for (int k = 0; k < 6; ++k) acc += B[k + 6 * i] * D[k + 6 * j];
-			BTD[i + 12 * j] = acc;
-		}
-	for (int i = 0; i < 12; ++i)
-		for (int j = 0; j < 12; ++j) {
-			float acc = 0.0f;
-			for (int k = 0; k < 6; ++k) acc += BTD[i + 12 * k] * B[k + 6 * j];
-			acc *= V;
-			// b3SetK: block (i/3 + 4 (j/3)), element (i%3 + 3 (j%3)) -> plane
-			int blk = (i / 3) + 4 * (j / 3);
-			int el = (i % 3) + 3 * (j % 3);
-			K0[(size_t)(9 * blk + el) * m + e] = acc;
-		}
+	// the K0 blocks are recomputed from this record where they are used
+	float r[kTetRecFloats];
+	tet_record(iE, E, nu, V, r);
+	for (int k = 0; k < 4; ++k) rec[4 * (size_t)e + k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -358,8 +319,7 @@ struct TetArgs {
 	uint32_t m;
 	const uint32_t* p;
 	const float* rest;
-	const float* invE;
-	const float* K0;
+	const float4* rec;       // b3g_tet.h record per element
 	float4* q;
 	const float* kd;
 	const V3* x;
@@ -396,9 +356,10 @@ __device__ Q4 extract_rotation(const M33& A, Q4 q) {
 	return q;
 }
 
-// tetrahedron_element_force.cpp:313-464. The element's Jacobian blocks
-// -R K0_ij R^T are not staged: the slot assembly recomputes them from the
-// stored rotation (same operations, same values).
+// tetrahedron_element_force.cpp:313-464. K0_ij comes from the element's
+// record (b3g_tet.h). The element's Jacobian blocks -R K0_ij R^T are not
+// staged: the slot assembly recomputes them from the record and the stored
+// rotation (same operations, same values).
 __global__ void __launch_bounds__(kBlock) k_tet_forces(TetArgs a) {
 	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= a.m) return;
@@ -411,8 +372,13 @@ __global__ void __launch_bounds__(kBlock) k_tet_forces(TetArgs a) {
 		p[k] = a.x[id[k]];
 		xr[k] = v3(a.rest[(size_t)(3 * k) * m + e], a.rest[(size_t)(3 * k + 1) * m + e], a.rest[(size_t)(3 * k + 2) * m + e]);
 	}
+	float rec[kTetRecFloats];
+	for (int k = 0; k < 4; ++k) {
+		const float4 r4 = a.rec[4 * (size_t)e + k];
+		rec[4 * k] = r4.x; rec[4 * k + 1] = r4.y; rec[4 * k + 2] = r4.z; rec[4 * k + 3] = r4.w;
+	}
 	M33 Em = m33(p[1] - p[0], p[2] - p[0], p[3] - p[0]);
-	M33 F = Em * ldM(a.invE, m, 0, e);
+	M33 F = Em * tet_inv_e(rec);
 	const float4 q4 = a.q[e];
 	Q4 q;
 	q.v = v3(q4.x, q4.y, q4.z);
@@ -432,7 +398,8 @@ __global__ void __launch_bounds__(kBlock) k_tet_forces(TetArgs a) {
 		V3 fds = zero3();
 		for (int j = 0; j < 4; ++j) {
 			const int blk = i + 4 * j;
-			M33 K0 = ldM(a.K0, m, 9 * blk, e);
+			(void)blk;
+			M33 K0 = tet_k0(rec, i, j);
 			fs = fs + K0 * us[j];
 			if (kd > 0.0f) {
 				M33 K = R * K0 * RT;
@@ -1083,10 +1050,9 @@ struct AsmArgs {
 	const uint32_t* diagSlot;
 	const M33* jstage;
 	const V3* fstage;
-	// tetrahedra: contribution-ordered K0 blocks (9 planes) and the element rotations
-	const float* K0x;
-	const float* K0v;
-	uint32_t nJx, nJv, tetM;
+	// tetrahedra: rest records (b3g_tet.h) and the element rotations
+	const float* tetRec;
+	uint32_t tetM;
 	const float4* tetQ;
 	const float* tetKd;
 	// stretch / shear records (UVTable::rec)
@@ -1124,15 +1090,13 @@ struct AsmArgs {
 // acc += the contributions [ptr[s], ptr[s + 1]) of slot s, in order, for the
 // tile's slots [s0, s1). Each chunk of kAsmCap contributions is produced in
 // three batched phases per thread (kAsmCap / kBlock contributions each):
-// codes, then the dependent loads (a tetrahedron's K0 block and rotation, or
-// a staged block), then the products into LDS, so a thread has all its loads
-// of a phase in flight at once.
+// codes, then the dependent loads (a tetrahedron's two gradients, Lame
+// parameters and rotation, or a record or staged block), then the products
+// into LDS, so a thread has all its loads of a phase in flight at once.
 constexpr int kAsmPer = kAsmCap / kBlock;
 
 struct Contrib {
 	const uint32_t* idx;    // codes
-	const float* K0;        // contribution-ordered K0 planes (tetrahedra) or null
-	uint32_t count;         // plane stride of K0
 	const float* kd;        // per-tetrahedron scale (dfdv) or null (dfdx)
 	bool dv;                // the dfdv list
 };
@@ -1177,8 +1141,9 @@ __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib&
 	for (uint32_t cb = c0; cb < c1; cb += kAsmCap) {
 		const uint32_t ce = min(cb + (uint32_t)kAsmCap, c1);
 		uint32_t code[kAsmPer];
-		// per contribution: a K0 or staged block in r[0..2] (+ the rotation in
-		// r[3]) or a stretch / shear record in r[0..3]
+		// per contribution: a tetrahedron's g_i | V, g_j | L2, (L, mu) in
+		// r[0..2] and its rotation in r[3], a staged block in r[0..2], or a
+		// stretch / shear record in r[0..3]
 		float4 r[kAsmPer][4];
 #pragma unroll
 		for (int u = 0; u < kAsmPer; ++u) {
@@ -1191,11 +1156,14 @@ __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib&
 			if (k >= ce) continue;
 			const uint32_t c = code[u];
 			if (c & kTetCode) {
-				const M33 m = ldA(cb_.K0, cb_.count, k);
-				r[u][0] = make_float4(m.x.x, m.x.y, m.x.z, m.y.x);
-				r[u][1] = make_float4(m.y.y, m.y.z, m.z.x, m.z.y);
-				r[u][2] = make_float4(m.z.z, 0.0f, 0.0f, 0.0f);
-				r[u][3] = a.tetQ[c & ~kTetCode];
+				const uint32_t e = c & (kTetMaxElements - 1), blk = (c & ~kTetCode) >> kTetBlockShift;
+				const float* rec = a.tetRec + (size_t)kTetRecFloats * e;
+				const V3 gi = tet_grad(rec, (int)(blk & 3u)), gj = tet_grad(rec, (int)(blk >> 2));
+				const float4 s4 = *reinterpret_cast<const float4*>(rec + kTetRecV);
+				r[u][0] = make_float4(gi.x, gi.y, gi.z, s4.x);
+				r[u][1] = make_float4(gj.x, gj.y, gj.z, s4.y);
+				r[u][2] = make_float4(s4.z, s4.w, 0.0f, 0.0f);
+				r[u][3] = a.tetQ[e];
 			} else if (c & kUVCode) {
 				const uint32_t variant = (c & ~kUVCode) >> kUVShift, e = c & (kUVMaxElements - 1);
 				if (variant < kShearVariant) {
@@ -1223,7 +1191,11 @@ __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib&
 			if (k >= ce) continue;
 			const uint32_t c = code[u];
 			M33 v;
-			if (c & kUVCode && !(c & kTetCode)) {
+			if (c & kTetCode) {
+				// K0_ij (b3g_tet.h)
+				v = tet_k0(v3(r[u][0].x, r[u][0].y, r[u][0].z), v3(r[u][1].x, r[u][1].y, r[u][1].z), r[u][0].w,
+				           r[u][1].w, r[u][2].x, r[u][2].y);
+			} else if (c & kUVCode) {
 				v = uv_block((c & ~kUVCode) >> kUVShift, r[u], cb_.dv);
 			} else {
 				v.x = v3(r[u][0].x, r[u][0].y, r[u][0].z);
@@ -1238,7 +1210,7 @@ __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib&
 				M33 R = qmatrix(q);
 				M33 RT = transpose(R);
 				M33 K = R * v * RT;
-				v = cb_.kd ? -(cb_.kd[c & ~kTetCode] * K) : -K;
+				v = cb_.kd ? -(cb_.kd[c & (kTetMaxElements - 1)] * K) : -K;
 			}
 			const float* pv = &v.x.x;
 #pragma unroll
@@ -1311,11 +1283,11 @@ __global__ void __launch_bounds__(kBlock) k_assemble_slots(AsmArgs a) {
 	const V3 w_j = own ? a.w[j] : zero3();
 	// dfdx(i, j): contributions in list order, then contacts (force_solver.cpp:70-86)
 	M33 dx = mzero();
-	tile_accumulate(a, Contrib{a.jxIdx, a.K0x, a.nJx, nullptr, false}, a.jxPtr, s0, s1, s, own, lds, dx);
+	tile_accumulate(a, Contrib{a.jxIdx, nullptr, false}, a.jxPtr, s0, s1, s, own, lds, dx);
 	// dfdv(i, j): particle mass damping first, then contributions
 	M33 dv = mzero();
 	if (own && i == j && damp_i > 0.0f) dv = dv + mdiag(-damp_i * mass_i);
-	if (a.anyJv) tile_accumulate(a, Contrib{a.jvIdx, a.K0v, a.nJv, a.tetKd, true}, a.jvPtr, s0, s1, s, own, lds, dv);
+	if (a.anyJv) tile_accumulate(a, Contrib{a.jvIdx, a.tetKd, true}, a.jvPtr, s0, s1, s, own, lds, dv);
 	if (!own) return;
 	const bool dyn_i = type_i == kDynamic;
 	if (i == j && sph_i >= 0 && dyn_i) {
@@ -1899,6 +1871,9 @@ b3g_status build_tiles(Body& b) {
 	for (uint32_t k = 0; k < K; ++k)
 		for (int slot = 0; slot < 6; slot += 2)
 			if (range[6 * k + slot] == 0xFFFFFFFFu) range[6 * k + slot] = range[6 * k + slot + 1] = 0;
+	b.maxInstPartials = 0;
+	for (uint32_t k = 0; k < K; ++k)
+		b.maxInstPartials = std::max({b.maxInstPartials, range[6 * k + 1] - range[6 * k], range[6 * k + 3] - range[6 * k + 2]});
 	b3g_status s;
 	if ((s = dupload(b, b.tiles, t.data(), t.size() * 4)) != B3G_OK) return s;
 	if ((s = dupload(b, b.tileInst, ti.data(), ti.size() * 4)) != B3G_OK) return s;
@@ -1946,41 +1921,13 @@ b3g_status ensure_cg_log(Body& b, uint32_t entries) {
 	return B3G_OK;
 }
 
-// K0 blocks in contribution order: dst plane c, position k = K0 block src[k]
-// (src = block * m + e) of the element-ordered planes.
-__global__ void __launch_bounds__(kBlock) k_tet_perm(uint32_t m, uint32_t count, const uint32_t* src, const float* K0,
-                                                      float* dst) {
-	uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-	if (k >= count) return;
-	const uint32_t sk = src[k];
-	if (sk == 0xFFFFFFFFu) return;
-	const uint32_t blk = sk / m, e = sk % m;
-#pragma unroll
-	for (int c = 0; c < 9; ++c) dst[(size_t)c * count + k] = K0[(size_t)(9 * blk + c) * m + e];
-}
-
-b3g_status launch_tet_perm(Body& b, const std::vector<uint32_t>& src, DBuf& dst) {
-	const uint32_t count = (uint32_t)src.size();
-	if (!count) return B3G_OK;
-	DBuf tmp;
-	b3g_status s = dupload(b, tmp, src.data(), src.size() * 4);
-	if (s != B3G_OK) return s;
-	s = dalloc(b, dst, (size_t)count * 9 * sizeof(float));
-	if (s != B3G_OK) return s;
-	LAUNCH(k_tet_perm, nblk(count), b.tet.m, count, dptr<uint32_t>(tmp), dptr<float>(b.tet.K0), dptr<float>(dst));
-	B3G_HIP(hipStreamSynchronize(b.stream));
-	(void)hipFree(tmp.p);
-	b.owned.erase(std::remove(b.owned.begin(), b.owned.end(), &tmp), b.owned.end());
-	return B3G_OK;
-}
-
 b3g_status launch_tet_setup(Body& b) {
 	DBuf flag;
 	b3g_status s = dalloc(b, flag, 4);
 	if (s != B3G_OK) return s;
 	B3G_HIP(hipMemsetAsync(flag.p, 0, 4, b.stream));
 	LAUNCH(k_tet_setup, nblk(b.tet.m), b.tet.m, dptr<const float>(b.tet.rest), dptr<const float>(b.tet.param),
-	       dptr<float>(b.tet.invE), dptr<float>(b.tet.K0), dptr<uint32_t>(flag));
+	       dptr<float4>(b.tet.rec), dptr<uint32_t>(flag));
 	uint32_t deg = 0;
 	B3G_HIP(hipMemcpyAsync(&deg, flag.p, 4, hipMemcpyDeviceToHost, b.stream));
 	B3G_HIP(hipStreamSynchronize(b.stream));
@@ -2078,7 +2025,11 @@ static CGArgs cg_args(Body& b) {
 	a.range = dptr<uint32_t>(b.instRange);
 	a.log = dptr<CGLog>(b.cgLog);
 	a.logStride = b.logStride;
-	a.fused = b.reduction == B3G_REDUCE_FP64_FUSED ? 1u : 0u;
+	// the default FP64 mode fuses while the consumers' re-reads of the
+	// partials stay cheaper than two extra launches per iteration (measured
+	// crossover between 3.5k and 11.5k partials per instance)
+	a.fused = b.reduction == B3G_REDUCE_FP64_FUSED ||
+	          (b.reduction == B3G_REDUCE_FP64 && b.maxInstPartials <= kFuseMaxPartials) ? 1u : 0u;
 	a.bf16 = b.solver == B3G_SOLVER_REFINED_BF16 && b.Ab.p ? 1u : 0u;
 	a.Ab = dptr<uint4>(b.Ab);
 	a.Ab8 = dptr<uint16_t>(b.Ab8);
@@ -2112,8 +2063,7 @@ static b3g_status enqueue_linearise(Body& b, float h) {
 	LAUNCH(k_newton_prep, nblk(b.n), sa);
 	const bool precise = b.solver != B3G_SOLVER_PCG && b.pb64.p;
 	if (b.tet.m) {
-		TetArgs t{b.tet.m, dptr<uint32_t>(b.tet.p), dptr<float>(b.tet.rest), dptr<float>(b.tet.invE), dptr<float>(b.tet.K0),
-		          dptr<float4>(b.tet.q), dptr<float>(b.tet.kd), dptr<V3>(b.x), dptr<V3>(b.v), fs, b.tet.f, b.tet.fd, st, ri};
+		TetArgs t{b.tet.m, dptr<uint32_t>(b.tet.p), dptr<float>(b.tet.rest), dptr<float4>(b.tet.rec), dptr<float4>(b.tet.q), dptr<float>(b.tet.kd), dptr<V3>(b.x), dptr<V3>(b.v), fs, b.tet.f, b.tet.fd, st, ri};
 		if (precise) LAUNCH(k_tet_forces64, nblk(b.tet.m), t, dptr<double>(b.tet64), dptr<D3>(b.fstage64));
 		else LAUNCH(k_tet_forces, nblk(b.tet.m), t);
 	}
@@ -2168,7 +2118,7 @@ static b3g_status enqueue_linearise(Body& b, float h) {
 	a.n = b.n; a.nnz = b.nnz; a.stride = b.aStride; a.hasContacts = b.nPairs ? 1u : 0u; a.h = h;
 	a.slotTiles = dptr<uint32_t>(b.slotTiles); a.slotTileInst = dptr<uint32_t>(b.slotTileInst);
 	a.rowTiles = dptr<uint32_t>(b.tiles); a.tileInst = dptr<uint32_t>(b.tileInst);
-	a.K0x = dptr<float>(b.K0x); a.K0v = dptr<float>(b.K0v); a.nJx = b.nJx; a.nJv = b.nJv;
+	a.tetRec = dptr<float>(b.tet.rec);
 	a.tetM = b.tet.m; a.tetQ = dptr<float4>(b.tet.q); a.tetKd = dptr<float>(b.tet.kd); a.anyJv = b.anyJv ? 1u : 0u;
 	a.stRec = dptr<float4>(b.stretch.rec); a.shRec = dptr<float4>(b.shear.rec);
 	a.slotRow = dptr<uint32_t>(b.slotRow); a.cols = dptr<uint32_t>(b.cols);

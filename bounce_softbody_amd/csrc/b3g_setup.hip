@@ -18,6 +18,7 @@
 // reference adds into it, in the reference's order.
 #include "b3g_bvh.h"
 #include "b3g_internal.h"
+#include "b3g_tet.h"
 
 #include <algorithm>
 #include <functional>
@@ -196,7 +197,7 @@ inline uint32_t uv_code(uint32_t variant, uint32_t e) { return kUVCode | (varian
 // tetrahedron in 31 bits, stretch / shear indices in kUVShift bits, staged
 // blocks (triangle elements, springs, mouse) below kUVCode.
 bool code_limits(const b3g_body_desc& d) {
-	if (d.n_tet >= (1u << 27)) {
+	if (d.n_tet >= kTetMaxElements) {
 		set_error("more than 2^27 - 1 tetrahedron elements");
 		return false;
 	}
@@ -984,23 +985,18 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 	b.aStride = plane_stride(b.nnz);
 	b.hostRowptr = P.rowptr;
 	b.hostCols = P.cols;
-	// tetrahedron entries: code kTetCode | e, and the source block of each
-	// position for the contribution-ordered copy of K0
-	std::vector<uint32_t> permX, permV;
-	auto tet_codes = [&](std::vector<uint32_t>& idx, std::vector<uint32_t>& perm) {
+	// tetrahedron entries: the pattern numbers block (i, j) of element e as
+	// (i + 4 j) m + e; the assembly decodes kTetCode | (i + 4 j) << 27 | e
+	auto tet_codes = [&](std::vector<uint32_t>& idx) {
 		if (!d.n_tet) return;
-		perm.assign(idx.size(), 0xFFFFFFFFu);
 		for (size_t k = 0; k < idx.size(); ++k)
 			if (idx[k] & kTetCode) {
 				const uint32_t src = idx[k] & ~kTetCode;
-				perm[k] = src;
-				idx[k] = kTetCode | (src % d.n_tet);
+				idx[k] = kTetCode | (src / d.n_tet) << kTetBlockShift | (src % d.n_tet);
 			}
 	};
-	tet_codes(P.jxIdx, permX);
-	if (b.tet.anyDamping) tet_codes(P.jvIdx, permV);
-	b.nJx = (uint32_t)permX.size();
-	b.nJv = (uint32_t)permV.size();
+	tet_codes(P.jxIdx);
+	if (b.tet.anyDamping) tet_codes(P.jvIdx);
 	b.anyJv = !P.jvIdx.empty();
 	// slot-assembly tiles: <= 256 slots, <= 1024 contributions of each list
 	// (one LDS chunk of the slot assembly)
@@ -1060,8 +1056,7 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 		std::vector<float> prm(d.tet_param, d.tet_param + 3 * (size_t)d.n_tet);
 		UP(b.tet.param, planes(prm, d.n_tet, 3));
 		UP(b.tet.kd, tetKd);
-		AL(b.tet.invE, (size_t)d.n_tet * 9 * 4);
-		AL(b.tet.K0, (size_t)d.n_tet * 144 * 4);
+		AL(b.tet.rec, (size_t)d.n_tet * kTetRecFloats * 4);
 		std::vector<float> q(4 * (size_t)d.n_tet, 0.0f);
 		for (uint32_t e = 0; e < d.n_tet; ++e) q[4 * (size_t)e + 3] = 1.0f;   // identity (x y z s per element)
 		UP(b.tet.q, q);
@@ -1179,10 +1174,6 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 	if (st != B3G_OK) return st;
 	if (d.n_tet) {
 		st = launch_tet_setup(b);
-		if (st != B3G_OK) return st;
-		st = launch_tet_perm(b, permX, b.K0x);
-		if (st != B3G_OK) return st;
-		st = launch_tet_perm(b, permV, b.K0v);
 		if (st != B3G_OK) return st;
 	}
 	st = ensure_actions(b);

@@ -329,18 +329,22 @@ b3g_status b3g_get_system(b3g_body body, uint32_t* rowptr, uint32_t* cols, float
 
 /*
  * Reduction of the PCG dot products and of the Newton error norm.
- *   B3G_REDUCE_FP64 (default): deterministic FP64 tree over the device; a
+ *   B3G_REDUCE_FP64_SEPARATE: deterministic FP64 tree over the device; a
  *   small kernel per reduction forms alpha / beta / convergence.
  *   B3G_REDUCE_FP64_FUSED: the same sums in the same order, formed instead at
- *   the start of the kernel that consumes them (two launches fewer per PCG
- *   iteration, a little more work per workgroup); results are identical to
- *   B3G_REDUCE_FP64 bit for bit.
+ *   the start of the kernel that consumes them: two launches fewer per PCG
+ *   iteration, but every consuming workgroup re-reads the instance's partials
+ *   (one per SpMV tile or row block), so the extra work grows with the square
+ *   of the instance's size.
+ *   B3G_REDUCE_FP64 (default): the fused form when every instance has at most
+ *   6144 partials per reduction (about 480k rows), the separate form
+ *   otherwise. The three FP64 modes give identical results bit for bit.
  *   B3G_REDUCE_SEQUENTIAL: FP32 terms added one by one in row order, as
  *   b3Dot / b3LengthSquared do (dense_vec3.h:151-175). With it a step is the
  *   reference's step operation for operation; it is O(rows) latency per
  *   reduction and meant for validation, not throughput.
  */
-enum { B3G_REDUCE_FP64 = 0, B3G_REDUCE_SEQUENTIAL = 1, B3G_REDUCE_FP64_FUSED = 2 };
+enum { B3G_REDUCE_FP64 = 0, B3G_REDUCE_SEQUENTIAL = 1, B3G_REDUCE_FP64_FUSED = 2, B3G_REDUCE_FP64_SEPARATE = 3 };
 b3g_status b3g_set_reduction(b3g_body body, int mode);
 
 /*

@@ -125,12 +125,15 @@ def test_batch_reports_the_failing_instance():
         Body([good, bad])
 
 
-def test_fused_reductions_equal_separate_finalize():
+@pytest.mark.parametrize("other", [_lib.REDUCE_FP64_FUSED, _lib.REDUCE_FP64], ids=["fused", "default"])
+def test_fused_reductions_equal_separate_finalize(other):
     """B3G_REDUCE_FP64_FUSED forms the same sums in the same order inside the
-    consuming kernels: trajectories equal the default mode bit for bit."""
+    consuming kernels, and the default mode picks one of the two forms:
+    trajectories equal the separate-kernel form bit for bit."""
     arrs = [configs.PARITY_SCENES[n]().arrays() for n in ("tet_grid", "cloth_sphere", "springs", "sdf")]
     a, b = Body(arrs), Body(arrs)
-    b.set_reduction(_lib.REDUCE_FP64_FUSED)
+    a.set_reduction(_lib.REDUCE_FP64_SEPARATE)
+    b.set_reduction(other)
     for _ in range(8):
         a.Step(configs.DT, 2, 30, 0.0, 1e-6)
         b.Step(configs.DT, 2, 30, 0.0, 1e-6)
