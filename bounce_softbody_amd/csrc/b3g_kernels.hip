@@ -144,6 +144,21 @@ struct Reduce {
 	uint32_t logStride;
 };
 
+// The workgroup's FP64 partial (each thread's `part` added in the block
+// tree's fixed order) into partial[slot].
+__device__ __forceinline__ void block_store(double part, double* partial, uint32_t slot) {
+	__shared__ double sh[kBlock / 64];
+	const double bs = block_sum(part, sh);
+	if (threadIdx.x == 0) partial[slot] = bs;
+}
+
+// Row blocks of the vector kernels: kRowBlkRows rows per workgroup, thread x
+// taking rows begin + x + u kBlock (u < kRowsPerThread), so a partial covers
+// four times as many rows as a workgroup has threads and the kernels that
+// re-read all of an instance's partials (fused reductions) read fewer.
+constexpr int kRowsPerThread = 4;
+constexpr uint32_t kRowBlkRows = kRowsPerThread * kBlock;
+
 // One partial per tile / row block `slot` of instance `inst`.
 __device__ __forceinline__ void block_partial(float term, bool has, uint32_t row, double* partial, float* terms,
                                               bool seq, uint32_t slot) {
@@ -156,21 +171,23 @@ __device__ __forceinline__ void block_partial(float term, bool has, uint32_t row
 constexpr int kFinBlock = 256;
 
 // Sum of partials [p0, p1) by a kFinBlock-thread workgroup in a fixed order
-// (thread t adds p0 + t, p0 + t + 256, ..., four loads in flight; then a
+// (thread t adds p0 + t, p0 + t + 256, ..., sixteen loads in flight; then a
 // wave tree and the four wave sums in order). Valid in every thread.
 __device__ double wg_sum(const double* partial, uint32_t p0, uint32_t p1) {
 	__shared__ double sh[kFinBlock / 64];
 	__shared__ double total;
+	// thread x adds partial[p0 + x + j kFinBlock] for j = 0, 1, ... in order;
+	// 16 loads per pass keep up to 16 kFinBlock partials one L2 round trip
 	double t = 0.0;
-	for (uint32_t base = p0; base < p1; base += 4 * kFinBlock) {
-		double p[4];
+	for (uint32_t base = p0; base < p1; base += 16 * kFinBlock) {
+		double p[16];
 #pragma unroll
-		for (int u = 0; u < 4; ++u) {
+		for (int u = 0; u < 16; ++u) {
 			const uint32_t k = base + threadIdx.x + u * kFinBlock;
 			p[u] = k < p1 ? partial[k] : 0.0;
 		}
 #pragma unroll
-		for (int u = 0; u < 4; ++u) t += p[u];
+		for (int u = 0; u < 16; ++u) t += p[u];
 	}
 	for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1470,8 +1487,6 @@ struct CGArgs {
 
 __device__ __forceinline__ V3 diag_apply(V3 im, V3 r) { return mdiag3(im.x, im.y, im.z) * r; }
 
-// One SpMV tile: products of the tile's blocks with the gathered vector go to
-// LDS, then each row thread adds its products in CSR (reference) order.
 __device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
 
@@ -1488,23 +1503,28 @@ __device__ __forceinline__ M33 ldAb(const CGArgs& a, uint32_t k) {
 	return m;
 }
 
-template <class Gather>
-__device__ __forceinline__ V3 tile_rows(const CGArgs& a, uint32_t t, uint32_t& row, Gather gather, float* lds) {
+// One SpMV tile into acc (products in LDS, then row sums in CSR order).
+// before() runs first, uniformly over the workgroup; false from it ends the
+// tile (the function returns false).
+template <class Gather, class Before>
+__device__ __forceinline__ bool tile_rows(const CGArgs& a, uint32_t t, uint32_t& row, Gather gather, float* lds,
+                                          Before before, V3& acc) {
 	const uint32_t r0 = a.tiles[t], r1 = a.tiles[t + 1];
 	const uint32_t b0 = a.rowptr[r0], b1 = a.rowptr[r1];
 	row = r0 + threadIdx.x;
 	const bool own = row < r1;
 	uint32_t rb = own ? a.rowptr[row] : 0, re = own ? a.rowptr[row + 1] : 0;
-	V3 acc = zero3();
+	auto ld = [&](uint32_t k) { return a.bf16 ? ldAb(a, k) : ldA(a.A, a.stride, k); };
+	auto put = [&](uint32_t i, const V3& p) {
+		lds[3 * i] = p.x;
+		lds[3 * i + 1] = p.y;
+		lds[3 * i + 2] = p.z;
+	};
+	acc = zero3();
+	if (!before()) return false;
 	for (uint32_t cb = b0; cb < b1; cb += kTileCap) {
 		const uint32_t ce = min(cb + (uint32_t)kTileCap, b1);
-		for (uint32_t k = cb + threadIdx.x; k < ce; k += kBlock) {
-			M33 blk = a.bf16 ? ldAb(a, k) : ldA(a.A, a.stride, k);
-			V3 p = blk * gather(a.cols[k]);
-			lds[3 * (k - cb)] = p.x;
-			lds[3 * (k - cb) + 1] = p.y;
-			lds[3 * (k - cb) + 2] = p.z;
-		}
+		for (uint32_t k = cb + threadIdx.x; k < ce; k += kBlock) put(k - cb, ld(k) * gather(a.cols[k]));
 		__syncthreads();
 		if (own) {
 			const uint32_t kb = max(rb, cb), ke = min(re, ce);
@@ -1513,8 +1533,10 @@ __device__ __forceinline__ V3 tile_rows(const CGArgs& a, uint32_t t, uint32_t& r
 		__syncthreads();
 	}
 	if (a.bf16 && own) acc = acc + ldM(a.diagF, a.n, 0, row) * gather(row);   // FP32 diagonal block (zero in the BF16 planes)
-	return acc;
+	return true;
 }
+
+__device__ __forceinline__ bool no_prologue() { return true; }
 
 // r = b - A x ; s = invM r ; delta = r.s   (mode kFinInit: CG start, kFinDelta: true-residual
 // refresh after iteration k, which ran only if log[k].active)
@@ -1526,7 +1548,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_residual(CGArgs a, int mode, uint
 	if (mode == kFinDelta && !a.log[(size_t)inst * a.logStride + k].active) return;
 	__shared__ float lds[3 * kTileCap];
 	uint32_t row;
-	V3 ax = tile_rows(a, t, row, [&](uint32_t c) { return a.px[c]; }, lds);
+	V3 ax;
+	tile_rows(a, t, row, [&](uint32_t c) { return a.px[c]; }, lds, no_prologue, ax);
 	float part = 0.0f;
 	const bool has = row < a.tiles[t + 1];
 	if (has) {
@@ -1555,30 +1578,34 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, uint32_t k, int
 	float beta = 0.0f;
 	if (k > 0) {
 		if (!force && !log[k - 1].active) return;
-		if (a.fused) {
-			const uint32_t* R = a.range + 6 * (size_t)inst;
-			const float v = (float)wg_sum(a.partial, R[sel], R[sel + 1]);
-			const float prev = log[k - 1].delta;
-			beta = v / prev;
-			const bool act = k != st.maxIt && !(v <= st.eps2_sub * st.delta0);
-			if (!force && t == R[0] && threadIdx.x == 0) {
-				log[k].delta = v;
-				log[k].beta = beta;
-				log[k].active = act ? 1u : 0u;
-				st.it = k;
-			}
-			if (!force && !act) return;
-		} else {
+		if (!a.fused) {
 			if (!force && !log[k].active) return;
 			beta = log[k].beta;
 		}
 	} else if (!force && !log[0].active) {
 		return;
 	}
+	// fused mode: delta_k / beta_k from the partials
+	auto prologue = [&]() -> bool {
+		if (k == 0 || !a.fused) return true;
+		const uint32_t* R = a.range + 6 * (size_t)inst;
+		const float v = (float)wg_sum(a.partial, R[sel], R[sel + 1]);
+		const float prev = log[k - 1].delta;
+		beta = v / prev;
+		const bool act = k != st.maxIt && !(v <= st.eps2_sub * st.delta0);
+		if (!force && t == R[0] && threadIdx.x == 0) {
+			log[k].delta = v;
+			log[k].beta = beta;
+			log[k].active = act ? 1u : 0u;
+			st.it = k;
+		}
+		return force || act;
+	};
 	const bool first = k == 0;
 	auto dval = [&](uint32_t c) { return first ? a.s[c] : a.s[c] + beta * a.dold[c]; };
 	uint32_t row;
-	V3 q = tile_rows(a, t, row, dval, lds);
+	V3 q;
+	if (!tile_rows(a, t, row, dval, lds, prologue, q)) return;
 	float part = 0.0f;
 	const bool has = row < a.tiles[t + 1];
 	if (has) {
@@ -1606,19 +1633,24 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k) {
 	const uint32_t blk = blockIdx.x, inst = a.blkInst[blk];
 	if (!a.st[inst].newtonActive || !a.log[(size_t)inst * a.logStride + k].active) return;
 	const float alpha = cg_alpha(a, inst, blk, k);
-	const uint32_t i = a.rowBlk[blk] + threadIdx.x;
-	float part = 0.0f;
-	const bool has = i < a.rowBlk[blk + 1];
-	if (has) {
+	const uint32_t r0 = a.rowBlk[blk], r1 = a.rowBlk[blk + 1];
+	const bool seq = a.st[inst].seq != 0;
+	double part = 0.0;
+#pragma unroll
+	for (int u = 0; u < kRowsPerThread; ++u) {
+		const uint32_t i = r0 + threadIdx.x + u * kBlock;
+		if (i >= r1) break;
 		V3 d = a.dnew[i];
 		a.px[i] = a.px[i] + alpha * d;
 		V3 r = a.r[i] - alpha * a.q[i];
 		V3 s = diag_apply(a.invM[i], r);
 		a.r[i] = r;
 		a.s[i] = s;
-		part = dot(r, s);
+		const float term = dot(r, s);
+		if (seq) a.terms[i] = term;
+		part += (double)term;
 	}
-	block_partial(part, has, i, a.partial, a.terms, a.st[inst].seq != 0, blk);
+	block_store(part, a.partial, blk);
 }
 
 // x += alpha d (iterations that refresh the true residual)
@@ -1626,26 +1658,35 @@ __global__ void __launch_bounds__(kBlock) k_cg_update_x(CGArgs a, uint32_t k) {
 	const uint32_t blk = blockIdx.x, inst = a.blkInst[blk];
 	if (!a.st[inst].newtonActive || !a.log[(size_t)inst * a.logStride + k].active) return;
 	const float alpha = cg_alpha(a, inst, blk, k);
-	const uint32_t i = a.rowBlk[blk] + threadIdx.x;
-	if (i >= a.rowBlk[blk + 1]) return;
-	a.px[i] = a.px[i] + alpha * a.dnew[i];
+	const uint32_t r0 = a.rowBlk[blk], r1 = a.rowBlk[blk + 1];
+#pragma unroll
+	for (int u = 0; u < kRowsPerThread; ++u) {
+		const uint32_t i = r0 + threadIdx.x + u * kBlock;
+		if (i >= r1) break;
+		a.px[i] = a.px[i] + alpha * a.dnew[i];
+	}
 }
 
 // v += dv ; x = x0 + h v + y ; error = |dv|^2 (sparse_force_solver.cpp:193-223)
 __global__ void __launch_bounds__(kBlock) k_newton_update(StepArgs s, double* partial, float* terms) {
 	const uint32_t blk = blockIdx.x, inst = s.blkInst[blk];
 	if (!s.st[inst].newtonActive) return;
-	const uint32_t i = s.rowBlk[blk] + threadIdx.x;
-	float part = 0.0f;
-	const bool has = i < s.rowBlk[blk + 1];
-	if (has) {
+	const uint32_t r0 = s.rowBlk[blk], r1 = s.rowBlk[blk + 1];
+	const bool seq = s.st[inst].seq != 0;
+	double part = 0.0;
+#pragma unroll
+	for (int u = 0; u < kRowsPerThread; ++u) {
+		const uint32_t i = r0 + threadIdx.x + u * kBlock;
+		if (i >= r1) break;
 		V3 dv = s.px[i] + zero3();
 		V3 v = s.v[i] + dv;
 		s.v[i] = v;
 		s.x[i] = (s.x0[i] + s.h * v) + s.y[i];
-		part = length_sq(dv);
+		const float term = length_sq(dv);
+		if (seq) terms[i] = term;
+		part += (double)term;
 	}
-	block_partial(part, has, i, partial, terms, s.st[inst].seq != 0, blk);
+	block_store(part, partial, blk);
 }
 
 __global__ void __launch_bounds__(kBlock) k_clear_external(uint32_t n, V3* force, V3* translation) {
@@ -1660,7 +1701,8 @@ __global__ void __launch_bounds__(kBlock) k_spmv(CGArgs a, const V3* xin, V3* yo
 	__shared__ float lds[3 * kTileCap];
 	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
 	uint32_t row;
-	V3 y = tile_rows(a, t, row, [&](uint32_t c) { return xin[c]; }, lds);
+	V3 y;
+	tile_rows(a, t, row, [&](uint32_t c) { return xin[c]; }, lds, no_prologue, y);
 	if (row < a.tiles[t + 1]) yout[row] = y;
 }
 
@@ -1824,7 +1866,7 @@ static inline uint32_t nblk(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
 // Row decompositions, both cut at instance boundaries so that every tile /
 // block belongs to one instance: SpMV tiles (<= kTileRows rows, <= kTileCap
-// blocks) and row blocks (<= kBlock rows) for the vector kernels; plus each
+// blocks) and row blocks (<= kRowBlkRows rows) for the vector kernels; plus each
 // instance's ranges of tiles, blocks and rows.
 b3g_status build_tiles(Body& b) {
 	const uint32_t K = std::max(1u, b.nInst);
@@ -1840,7 +1882,7 @@ b3g_status build_tiles(Body& b) {
 			rows = 0;
 			blocks = 0;
 		}
-		if (brows > 0 && (newInst || brows == (uint32_t)kBlock)) {
+		if (brows > 0 && (newInst || brows == kRowBlkRows)) {
 			rb.push_back(i);
 			bi.push_back(inst(i - 1));
 			brows = 0;
