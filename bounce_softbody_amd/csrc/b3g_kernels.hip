@@ -1514,7 +1514,6 @@ __device__ __forceinline__ bool tile_rows(const CGArgs& a, uint32_t t, uint32_t&
 	row = r0 + threadIdx.x;
 	const bool own = row < r1;
 	uint32_t rb = own ? a.rowptr[row] : 0, re = own ? a.rowptr[row + 1] : 0;
-	auto ld = [&](uint32_t k) { return a.bf16 ? ldAb(a, k) : ldA(a.A, a.stride, k); };
 	auto put = [&](uint32_t i, const V3& p) {
 		lds[3 * i] = p.x;
 		lds[3 * i + 1] = p.y;
@@ -1524,7 +1523,28 @@ __device__ __forceinline__ bool tile_rows(const CGArgs& a, uint32_t t, uint32_t&
 	if (!before()) return false;
 	for (uint32_t cb = b0; cb < b1; cb += kTileCap) {
 		const uint32_t ce = min(cb + (uint32_t)kTileCap, b1);
-		for (uint32_t k = cb + threadIdx.x; k < ce; k += kBlock) put(k - cb, ld(k) * gather(a.cols[k]));
+		// The thread's columns first, so each block's gathers issue together
+		// with its matrix loads; a lane past the chunk loads the chunk's first
+		// block (in bounds) and stores nothing, which keeps the loads free of
+		// branches.
+		uint32_t col[kRowPer];
+#pragma unroll
+		for (int u = 0; u < kRowPer; ++u) {
+			const uint32_t k = cb + threadIdx.x + u * kBlock;
+			col[u] = a.cols[k < ce ? k : cb];
+		}
+		auto products = [&](auto ld) {
+#pragma unroll
+			for (int u = 0; u < kRowPer; ++u) {
+				const uint32_t k = cb + threadIdx.x + u * kBlock;
+				const bool in = k < ce;
+				const V3 g = gather(col[u]);
+				const V3 p = ld(in ? k : cb) * g;
+				if (in) put(k - cb, p);
+			}
+		};
+		if (a.bf16) products([&](uint32_t k) { return ldAb(a, k); });
+		else products([&](uint32_t k) { return ldA(a.A, a.stride, k); });
 		__syncthreads();
 		if (own) {
 			const uint32_t kb = max(rb, cb), ke = min(re, ce);
@@ -1601,8 +1621,11 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, uint32_t k, int
 		}
 		return force || act;
 	};
-	const bool first = k == 0;
-	auto dval = [&](uint32_t c) { return first ? a.s[c] : a.s[c] + beta * a.dold[c]; };
+	// d = s + beta d_old, and d = s at k = 0: there dsrc is s and beta is +0,
+	// and s + (+0) s = s exactly for finite s (the zero product carries the
+	// sign of s), so every iteration gathers the same way, without a branch
+	const V3* dsrc = k == 0 ? a.s : a.dold;
+	auto dval = [&](uint32_t c) { return a.s[c] + beta * dsrc[c]; };
 	uint32_t row;
 	V3 q;
 	if (!tile_rows(a, t, row, dval, lds, prologue, q)) return;
