@@ -1658,18 +1658,30 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k) {
 	const float alpha = cg_alpha(a, inst, blk, k);
 	const uint32_t r0 = a.rowBlk[blk], r1 = a.rowBlk[blk + 1];
 	const bool seq = a.st[inst].seq != 0;
+	// every input of the thread's rows is loaded before the first store (the
+	// stores could alias them as far as the compiler knows); a row past the
+	// block loads row r0 and stores nothing
+	V3 d[kRowsPerThread], x[kRowsPerThread], r[kRowsPerThread], q[kRowsPerThread], im[kRowsPerThread];
+#pragma unroll
+	for (int u = 0; u < kRowsPerThread; ++u) {
+		const uint32_t i = r0 + threadIdx.x + u * kBlock, ii = i < r1 ? i : r0;
+		d[u] = a.dnew[ii];
+		x[u] = a.px[ii];
+		r[u] = a.r[ii];
+		q[u] = a.q[ii];
+		im[u] = a.invM[ii];
+	}
 	double part = 0.0;
 #pragma unroll
 	for (int u = 0; u < kRowsPerThread; ++u) {
 		const uint32_t i = r0 + threadIdx.x + u * kBlock;
 		if (i >= r1) break;
-		V3 d = a.dnew[i];
-		a.px[i] = a.px[i] + alpha * d;
-		V3 r = a.r[i] - alpha * a.q[i];
-		V3 s = diag_apply(a.invM[i], r);
-		a.r[i] = r;
-		a.s[i] = s;
-		const float term = dot(r, s);
+		const V3 rn = r[u] - alpha * q[u];
+		const V3 sn = diag_apply(im[u], rn);
+		a.px[i] = x[u] + alpha * d[u];
+		a.r[i] = rn;
+		a.s[i] = sn;
+		const float term = dot(rn, sn);
 		if (seq) a.terms[i] = term;
 		part += (double)term;
 	}
@@ -1682,11 +1694,18 @@ __global__ void __launch_bounds__(kBlock) k_cg_update_x(CGArgs a, uint32_t k) {
 	if (!a.st[inst].newtonActive || !a.log[(size_t)inst * a.logStride + k].active) return;
 	const float alpha = cg_alpha(a, inst, blk, k);
 	const uint32_t r0 = a.rowBlk[blk], r1 = a.rowBlk[blk + 1];
+	V3 d[kRowsPerThread], x[kRowsPerThread];
+#pragma unroll
+	for (int u = 0; u < kRowsPerThread; ++u) {
+		const uint32_t i = r0 + threadIdx.x + u * kBlock, ii = i < r1 ? i : r0;
+		d[u] = a.dnew[ii];
+		x[u] = a.px[ii];
+	}
 #pragma unroll
 	for (int u = 0; u < kRowsPerThread; ++u) {
 		const uint32_t i = r0 + threadIdx.x + u * kBlock;
 		if (i >= r1) break;
-		a.px[i] = a.px[i] + alpha * a.dnew[i];
+		a.px[i] = x[u] + alpha * d[u];
 	}
 }
 
