@@ -122,7 +122,7 @@ EXPORTS = [
     "b3g_body_build", "b3g_body_build_batch", "b3g_get_instance_count", "b3g_step", "b3g_get_state", "b3g_set_state", "b3g_apply_external", "b3g_get_masses",
     "b3g_get_stats", "b3g_get_energy", "b3g_get_contacts", "b3g_get_tet_quats", "b3g_assemble", "b3g_get_system",
     "b3g_synchronize", "b3g_time_spmv", "b3g_set_solver", "b3g_set_graph", "b3g_get_system_size", "b3g_pattern",
-    "b3g_solve_cg", "b3g_spmv", "b3g_set_reduction", "b3g_save_state", "b3g_load_state", "b3g_ray_cast",
+    "b3g_solve_cg", "b3g_spmv", "b3g_set_reduction", "b3g_save_state", "b3g_load_state", "b3g_ray_cast", "b3g_ray_cast_device",
     "b3g_sdf_grid", "b3g_build_sdf", "b3g_body_rebuild", "b3g_set_action_forces", "b3g_get_action_forces",
 ]
 
@@ -132,6 +132,63 @@ class Carry(C.Structure):
 
     _fields_ = [("particle_prev", C.POINTER(C.c_int64)), ("tet_prev", C.POINTER(C.c_int64)),
                 ("sphere_prev", C.POINTER(C.c_int64)), ("world_prev", C.POINTER(C.c_int64))]
+
+
+class DeviceArray:
+    """A device buffer from the HIP runtime libb3g.so is linked against (hipMalloc / hipMemcpy / hipFree), for
+    the entry points that take device pointers. Another HIP runtime in the same process (a framework's bundled
+    copy) keeps its own allocations, which this library's runtime does not recognise."""
+
+    _hip = None
+
+    @classmethod
+    def runtime(cls):
+        if cls._hip is None:
+            lib()   # libb3g.so first: the runtime it bound is the one returned for its soname
+            h = C.CDLL("libamdhip64.so.7")
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipFree.argtypes = [C.c_void_p]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, nbytes: int):
+        self.nbytes = max(1, int(nbytes))
+        self.ptr = C.c_void_p()
+        rc = self.runtime().hipMalloc(C.byref(self.ptr), self.nbytes)
+        if rc != 0:
+            raise B3gError(rc, "hipMalloc", f"hipMalloc of {self.nbytes} bytes failed")
+
+    @classmethod
+    def of(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        rc = self.runtime().hipMemcpy(self.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes, 1)   # host to device
+        if rc != 0:
+            raise B3gError(rc, "hipMemcpy", "host to device copy failed")
+
+    def download(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        rc = self.runtime().hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes, 2)   # device to host
+        if rc != 0:
+            raise B3gError(rc, "hipMemcpy", "device to host copy failed")
+        return out
+
+    @property
+    def address(self) -> int:
+        return self.ptr.value
+
+    def __del__(self):
+        if self.ptr and self._hip is not None:
+            self._hip.hipFree(self.ptr)
+            self.ptr = C.c_void_p()
 
 
 def lib():
@@ -153,6 +210,7 @@ def lib():
         "b3g_body_build_batch": (C.c_int, [vp, C.POINTER(BodyDesc), C.c_uint32]),
         "b3g_get_instance_count": (C.c_int, [vp, u32p]),
         "b3g_ray_cast": (C.c_int, [vp, C.c_uint32, C.c_uint32, f32p, f32p, C.POINTER(C.c_int64), f32p, f32p]),
+        "b3g_ray_cast_device": (C.c_int, [vp, C.c_uint32, C.c_uint32, vp, vp, vp, vp, vp]),
         "b3g_sdf_grid": (C.c_int, [f32p, C.c_uint32, f32p, C.c_float, f32p, u32p]),
         "b3g_body_rebuild": (C.c_int, [vp, C.c_void_p, C.c_uint32, C.POINTER(Carry)]),
         "b3g_set_action_forces": (C.c_int, [vp, C.c_int]),

@@ -262,6 +262,13 @@ class Body:
               "b3g_ray_cast")
         return idx[:n], frac[:n], nrm[:n]
 
+    def ray_cast_device(self, count: int, p1: int, p2: int, index: int, fraction: int, normal: int = 0,
+                        instance: int = 0):
+        """b3g_ray_cast_device: the same query on arrays already in device memory, given as addresses (p1, p2:
+        3 floats per ray; index: int64; fraction: float; normal: 3 floats per ray, or 0). Blocks until done."""
+        check(self._L.b3g_ray_cast_device(self._h, instance, count, C.c_void_p(p1), C.c_void_p(p2), C.c_void_p(index),
+                                          C.c_void_p(fraction), C.c_void_p(normal or None)), "b3g_ray_cast_device")
+
     def time_spmv(self, launches: int = 100):
         """Mean duration (ms) of the PCG's SpMV kernel over back-to-back launches, and its algorithmic bytes."""
         ms = C.c_double(0)

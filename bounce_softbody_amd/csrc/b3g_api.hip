@@ -319,6 +319,28 @@ b3g_status b3g_ray_cast(b3g_body h, uint32_t instance, uint32_t count, const flo
 	return ray_cast(h->body, instance, count, p1, p2, index, fraction, normal);
 }
 
+b3g_status b3g_ray_cast_device(b3g_body h, uint32_t instance, uint32_t count, const float* p1, const float* p2,
+                               int64_t* index, float* fraction, float* normal) {
+	CHECK_BUILT(h);
+	if (count && (!p1 || !p2 || !index || !fraction)) { set_error("null ray or result array"); return B3G_INVALID_ARGUMENT; }
+	if (instance >= h->body.nInst) { set_error("instance out of range"); return B3G_INVALID_ARGUMENT; }
+	SET_DEVICE(h);
+	if (count) {
+		// only device memory is accepted: a host pointer would fault the kernel
+		const void* ptrs[5] = {p1, p2, index, fraction, normal};
+		for (const void* p : ptrs) {
+			if (!p) continue;
+			hipPointerAttribute_t attr;
+			if (hipPointerGetAttributes(&attr, p) != hipSuccess || attr.type != hipMemoryTypeDevice) {
+				(void)hipGetLastError();
+				set_error("b3g_ray_cast_device: an array is not device memory");
+				return B3G_INVALID_ARGUMENT;
+			}
+		}
+	}
+	return ray_cast_device(h->body, instance, count, p1, p2, index, fraction, normal);
+}
+
 b3g_status b3g_step(b3g_body h, float dt, uint32_t iterations, uint32_t sub_iterations, float tolerance,
                     float sub_tolerance) {
 	CHECK_BUILT(h);

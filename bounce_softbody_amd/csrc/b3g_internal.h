@@ -274,7 +274,7 @@ struct Body {
 	uint64_t rtVersion = ~0ull;                   // xVersion the device tree was built at
 	size_t rtSortBytes = 0;
 	DBuf rtTri, rtInstBegin, rtBox, rtKey, rtKey2, rtVal, rtVal2, rtSortTemp, rtLeafTri, rtLeafBox, rtChild, rtNodeBox,
-	    rtChunk1, rtChunk2, rtChunk3, rtInstBox, rtRays, rtHits;
+	    rtChunk1, rtChunk2, rtChunk3, rtInstBox, rtRays, rtHits, rtFault;
 	size_t rtRayCap = 0;
 
 	// ---------------------------------------------------------------- action forces
@@ -301,6 +301,8 @@ b3g_status build_sdf(int device, const float* vertices, uint32_t nv, const uint3
                      uint32_t stride, const float cell[3], float extension, float* voxels, uint64_t capacity);
 // b3Body::RayCastSingle for count rays against instance inst (b3g_raycast.hip).
 b3g_status ray_cast(Body& b, uint32_t inst, uint32_t count, const float* p1, const float* p2, int64_t* index,
+                    float* fraction, float* normal);
+b3g_status ray_cast_device(Body& b, uint32_t inst, uint32_t count, const float* p1, const float* p2, int64_t* index,
                     float* fraction, float* normal);
 b3g_status dupload(Body& b, DBuf& buf, const void* host, size_t bytes);
 void dfree_all(Body& b);

@@ -151,21 +151,24 @@ __global__ void __launch_bounds__(kRtBlock) k_rt_node_box(uint32_t T, RtLevels L
 
 struct RtCast {
 	uint32_t count, s, e;       // rays; the instance's segment of sorted positions
-	const float* rays;          // 6 floats per ray: p1, p2
+	const float* p1;            // 3 floats per ray
+	const float* p2;
 	const uint2* child;
 	const RtBox* nodeBox;
 	const RtBox* leafBox;
 	const uint4* leafTri;       // list-order particles, creation index
 	const V3* x;
-	uint32_t* hits;             // 5 words per ray: creation index (or ~0u), fraction, normal
+	int64_t* index;             // per ray: creation index of the triangle fixture hit, or -1
+	float* fraction;
+	float* normal;              // 3 floats per ray, or null
 	uint32_t* fault;            // set when a stack or index bound would be exceeded
 };
 
 __global__ void __launch_bounds__(kRtBlock) k_rt_cast(RtCast a) {
 	const uint32_t r = blockIdx.x * kRtBlock + threadIdx.x;
 	if (r >= a.count) return;
-	const float* q = a.rays + 6 * (size_t)r;
-	const V3 p1 = v3(q[0], q[1], q[2]), p2 = v3(q[3], q[4], q[5]);
+	const V3 p1 = v3(a.p1[3 * (size_t)r], a.p1[3 * (size_t)r + 1], a.p1[3 * (size_t)r + 2]);
+	const V3 p2 = v3(a.p2[3 * (size_t)r], a.p2[3 * (size_t)r + 1], a.p2[3 * (size_t)r + 2]);
 	const V3 d = p2 - p1;
 	uint32_t bestIdx = 0xFFFFFFFFu;
 	float bestT = 1.0f;
@@ -200,12 +203,14 @@ __global__ void __launch_bounds__(kRtBlock) k_rt_cast(RtCast a) {
 			stack[sp++] = ch.x;
 		}
 	}
-	uint32_t* o = a.hits + 5 * (size_t)r;
-	o[0] = bestIdx;
-	o[1] = __float_as_uint(bestIdx != 0xFFFFFFFFu ? bestT : 0.0f);
-	o[2] = __float_as_uint(bestN.x);
-	o[3] = __float_as_uint(bestN.y);
-	o[4] = __float_as_uint(bestN.z);
+	const bool hit = bestIdx != 0xFFFFFFFFu;
+	a.index[r] = hit ? (int64_t)bestIdx : -1;
+	a.fraction[r] = hit ? bestT : 0.0f;
+	if (a.normal) {
+		a.normal[3 * (size_t)r] = bestN.x;
+		a.normal[3 * (size_t)r + 1] = bestN.y;
+		a.normal[3 * (size_t)r + 2] = bestN.z;
+	}
 }
 
 b3g_status rt_alloc(Body& b, DBuf& buf, size_t bytes) {
@@ -288,59 +293,80 @@ b3g_status rt_build(Body& b) {
 
 } // namespace
 
-b3g_status ray_cast(Body& b, uint32_t inst, uint32_t count, const float* p1, const float* p2, int64_t* index,
-                    float* fraction, float* normal) {
-	if (count == 0) return B3G_OK;
+namespace {
+
+// The instance's tree (rebuilt when particles moved since the last query)
+// and the cast of `count` rays whose arrays are all in device memory; the
+// fault word is cleared first and checked after (the call blocks).
+b3g_status cast_rays(Body& b, uint32_t inst, uint32_t count, const float* p1, const float* p2, int64_t* index,
+                     float* fraction, float* normal, uint32_t* fault) {
 	const uint32_t s0 = b.hostRayInstBegin.empty() ? 0 : b.hostRayInstBegin[inst];
 	const uint32_t e0 = b.hostRayInstBegin.empty() ? 0 : b.hostRayInstBegin[inst + 1];
 	b3g_status s;
 	if (e0 > s0 && b.rtVersion != b.xVersion && (s = rt_build(b)) != B3G_OK) return s;
-	if (b.rtRayCap < count) {
-		if ((s = dalloc(b, b.rtRays, (size_t)count * 6 * 4)) != B3G_OK) return s;
-		if ((s = dalloc(b, b.rtHits, (size_t)count * 5 * 4 + 4)) != B3G_OK) return s;
-		b.rtRayCap = count;
-	}
-	std::vector<float> rays((size_t)count * 6);
-	for (size_t r = 0; r < count; ++r)
-		for (int k = 0; k < 3; ++k) {
-			rays[6 * r + k] = p1[3 * r + k];
-			rays[6 * r + 3 + k] = p2[3 * r + k];
-		}
-	uint32_t* hits = dptr<uint32_t>(b.rtHits);
-	uint32_t* fault = hits + (size_t)count * 5;
-	B3G_HIP(hipMemcpyAsync(b.rtRays.p, rays.data(), rays.size() * 4, hipMemcpyHostToDevice, b.stream));
 	B3G_HIP(hipMemsetAsync(fault, 0, 4, b.stream));
 	RtCast a;
 	a.count = count;
 	a.s = s0;
 	a.e = e0;
-	a.rays = dptr<float>(b.rtRays);
+	a.p1 = p1;
+	a.p2 = p2;
 	a.child = dptr<uint2>(b.rtChild);
 	a.nodeBox = dptr<RtBox>(b.rtNodeBox);
 	a.leafBox = dptr<RtBox>(b.rtLeafBox);
 	a.leafTri = dptr<uint4>(b.rtLeafTri);
 	a.x = dptr<V3>(b.x);
-	a.hits = hits;
+	a.index = index;
+	a.fraction = fraction;
+	a.normal = normal;
 	a.fault = fault;
 	hipLaunchKernelGGL(k_rt_cast, dim3(rt_blocks(count)), dim3(kRtBlock), 0, b.stream, a);
 	B3G_HIP(hipGetLastError());
-	std::vector<uint32_t> out((size_t)count * 5 + 1);
-	B3G_HIP(hipMemcpyAsync(out.data(), hits, out.size() * 4, hipMemcpyDeviceToHost, b.stream));
+	uint32_t f = 0;
+	B3G_HIP(hipMemcpyAsync(&f, fault, 4, hipMemcpyDeviceToHost, b.stream));
 	B3G_HIP(hipStreamSynchronize(b.stream));
-	if (out[(size_t)count * 5]) {
+	if (f) {
 		set_error("ray cast: tree traversal exceeded its bounds");
 		return B3G_HIP_ERROR;
 	}
-	for (size_t r = 0; r < count; ++r) {
-		const uint32_t* o = &out[5 * r];
-		index[r] = o[0] == 0xFFFFFFFFu ? -1 : (int64_t)o[0];
-		float f, n[3];
-		std::memcpy(&f, &o[1], 4);
-		std::memcpy(n, &o[2], 12);
-		fraction[r] = f;
-		if (normal) for (int k = 0; k < 3; ++k) normal[3 * r + k] = n[k];
-	}
 	return B3G_OK;
+}
+
+} // namespace
+
+b3g_status ray_cast(Body& b, uint32_t inst, uint32_t count, const float* p1, const float* p2, int64_t* index,
+                    float* fraction, float* normal) {
+	if (count == 0) return B3G_OK;
+	b3g_status s;
+	// staging: p1, p2 (3 floats per ray each); index (int64), fraction,
+	// normal (3 floats), then the fault word
+	if (b.rtRayCap < count) {
+		if ((s = dalloc(b, b.rtRays, (size_t)count * 6 * 4)) != B3G_OK) return s;
+		if ((s = dalloc(b, b.rtHits, (size_t)count * 24 + 4)) != B3G_OK) return s;
+		b.rtRayCap = count;
+	}
+	float* dp1 = dptr<float>(b.rtRays);
+	float* dp2 = dp1 + 3 * (size_t)count;
+	int64_t* di = dptr<int64_t>(b.rtHits);
+	float* df = reinterpret_cast<float*>(di + count);
+	float* dn = df + count;
+	uint32_t* fault = reinterpret_cast<uint32_t*>(dn + 3 * (size_t)count);
+	B3G_HIP(hipMemcpyAsync(dp1, p1, (size_t)count * 12, hipMemcpyHostToDevice, b.stream));
+	B3G_HIP(hipMemcpyAsync(dp2, p2, (size_t)count * 12, hipMemcpyHostToDevice, b.stream));
+	if ((s = cast_rays(b, inst, count, dp1, dp2, di, df, normal ? dn : nullptr, fault)) != B3G_OK) return s;
+	B3G_HIP(hipMemcpyAsync(index, di, (size_t)count * 8, hipMemcpyDeviceToHost, b.stream));
+	B3G_HIP(hipMemcpyAsync(fraction, df, (size_t)count * 4, hipMemcpyDeviceToHost, b.stream));
+	if (normal) B3G_HIP(hipMemcpyAsync(normal, dn, (size_t)count * 12, hipMemcpyDeviceToHost, b.stream));
+	B3G_HIP(hipStreamSynchronize(b.stream));
+	return B3G_OK;
+}
+
+b3g_status ray_cast_device(Body& b, uint32_t inst, uint32_t count, const float* p1, const float* p2, int64_t* index,
+                           float* fraction, float* normal) {
+	if (count == 0) return B3G_OK;
+	b3g_status s;
+	if (!b.rtFault.p && (s = dalloc(b, b.rtFault, 4)) != B3G_OK) return s;
+	return cast_rays(b, inst, count, p1, p2, index, fraction, normal, dptr<uint32_t>(b.rtFault));
 }
 
 } // namespace b3g

@@ -255,6 +255,16 @@ b3g_status b3g_ray_cast(b3g_body body, uint32_t instance, uint32_t count, const 
                         int64_t* index, float* fraction, float* normal);
 
 /*
+ * b3g_ray_cast with every array in device memory (hipMalloc'd, on the body's
+ * device): same arguments, results and ties, no host copies; the call blocks
+ * until the results are written. An array that is not device memory is
+ * rejected (B3G_INVALID_ARGUMENT). No counterpart in the reference, whose
+ * rays live on the host; it serves callers that keep rays on the GPU.
+ */
+b3g_status b3g_ray_cast_device(b3g_body body, uint32_t instance, uint32_t count, const float* p1, const float* p2,
+                               int64_t* index, float* fraction, float* normal);
+
+/*
  * The voxel grid b3BuildSDF(sdf, mesh, cellSize, aabbVolumeExtension) creates
  * (collision/geometry/sdf_builder.cpp:155-174, declared sdf.h:53): the mesh's
  * box grown by `aabb_extension`, box[0..2] lower and box[3..5] upper bound,

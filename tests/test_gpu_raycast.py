@@ -103,3 +103,42 @@ def test_ray_cast_validates():
         body.RayCastSingle(np.zeros((1, 3)), np.ones((1, 3)), instance=1)
     idx, frac, nrm = body.RayCastSingle(np.zeros((0, 3)), np.zeros((0, 3)))
     assert len(idx) == 0
+
+
+def _device_cast(body, p1, p2, instance=0):
+    n = len(p1)
+    d1 = _lib.DeviceArray.of(np.asarray(p1, np.float32))
+    d2 = _lib.DeviceArray.of(np.asarray(p2, np.float32))
+    di, df, dn = _lib.DeviceArray(8 * n), _lib.DeviceArray(4 * n), _lib.DeviceArray(12 * n)
+    body.ray_cast_device(n, d1.address, d2.address, di.address, df.address, dn.address, instance)
+    return di.download(np.int64, n), df.download(np.float32, n), dn.download(np.float32, (n, 3))
+
+
+def test_ray_cast_device_arrays_equal_host_arrays():
+    """b3g_ray_cast_device (rays and results in device memory) answers exactly as b3g_ray_cast, in a single body
+    and in a batch instance."""
+    arr = configs.cloth_on_sphere(64, size=1.6, height=0.5).arrays()
+    body = Body(arr)
+    for _ in range(3):
+        body.Step(configs.DT, 1, 30, 0.0, 1e-5)
+    x, _ = body.state()
+    p1, p2 = rays_for(x, seed=901, n=3000)
+    assert same(_device_cast(body, p1, p2), body.RayCastSingle(p1, p2))
+    arrs = [configs.PARITY_SCENES[n]().arrays() for n in ("springs", "cloth_sphere")]
+    batch = Body(arrs)
+    batch.Step(configs.DT, 1, 20, 0.0, 1e-5)
+    x, _ = batch.state()
+    p1, p2 = rays_for(x, seed=902, n=500)
+    assert same(_device_cast(batch, p1, p2, instance=1), batch.RayCastSingle(p1, p2, instance=1))
+
+
+def test_ray_cast_device_rejects_host_arrays():
+    """Host arrays passed to the device entry point are refused before anything runs on the GPU."""
+    body = Body(configs.PARITY_SCENES["cloth_sphere"]().arrays())
+    p = np.zeros((4, 3), np.float32)
+    idx = np.zeros(4, np.int64)
+    fr = np.zeros(4, np.float32)
+    with pytest.raises(_lib.B3gError, match="not device memory"):
+        body.ray_cast_device(4, p.ctypes.data, p.ctypes.data, idx.ctypes.data, fr.ctypes.data)
+    idx2, frac2, _ = body.RayCastSingle(p, p + 1.0)   # the body still answers
+    assert len(idx2) == 4
