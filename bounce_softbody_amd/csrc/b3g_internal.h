@@ -274,7 +274,7 @@ struct Body {
 	uint64_t rtVersion = ~0ull;                   // xVersion the device tree was built at
 	size_t rtSortBytes = 0;
 	DBuf rtTri, rtInstBegin, rtBox, rtKey, rtKey2, rtVal, rtVal2, rtSortTemp, rtLeafTri, rtLeafBox, rtChild, rtNodeBox,
-	    rtChunk1, rtChunk2, rtChunk3, rtInstBox, rtRays, rtHits, rtFault;
+	    rtChunk1, rtChunk2, rtChunk3, rtInstBox, rtInstPart, rtRays, rtHits, rtFault;
 	size_t rtRayCap = 0;
 
 	// ---------------------------------------------------------------- action forces

@@ -57,12 +57,16 @@ __global__ void __launch_bounds__(kRtBlock) k_rt_leaf(uint32_t T, const uint4* t
 	box[t] = o;
 }
 
-// One workgroup per instance: bounds of the leaf-box centres.
-__global__ void __launch_bounds__(kRtBlock) k_rt_inst_box(const uint32_t* begin, const RtBox* box, RtBox* out) {
-	const uint32_t k = blockIdx.x;
+// Bounds of the leaf-box centres of each instance: `parts` workgroups per
+// instance each cover a strided share of its leaves (partial[k parts + part]),
+// then k_rt_inst_join unites an instance's partials (min / max are exact in
+// any order).
+__global__ void __launch_bounds__(kRtBlock) k_rt_inst_box(const uint32_t* begin, uint32_t parts, const RtBox* box,
+                                                          RtBox* partial) {
+	const uint32_t k = blockIdx.x / parts, part = blockIdx.x % parts;
 	const uint32_t s = begin[k], e = begin[k + 1];
 	RtBox r = rt_empty();
-	for (uint32_t t = s + threadIdx.x; t < e; t += kRtBlock) {
+	for (uint32_t t = s + part * kRtBlock + threadIdx.x; t < e; t += parts * kRtBlock) {
 		const RtBox b = box[t];
 		const float4 c = make_float4(0.5f * (b.lo.x + b.hi.x), 0.5f * (b.lo.y + b.hi.y), 0.5f * (b.lo.z + b.hi.z), 0.0f);
 		RtBox p;
@@ -77,7 +81,16 @@ __global__ void __launch_bounds__(kRtBlock) k_rt_inst_box(const uint32_t* begin,
 		if (threadIdx.x < w) rt_unite(lds[threadIdx.x], lds[threadIdx.x + w]);
 		__syncthreads();
 	}
-	if (threadIdx.x == 0) out[k] = lds[0];
+	if (threadIdx.x == 0) partial[blockIdx.x] = lds[0];
+}
+
+__global__ void __launch_bounds__(kRtBlock) k_rt_inst_join(uint32_t nInst, uint32_t parts, const RtBox* partial,
+                                                           RtBox* out) {
+	const uint32_t k = blockIdx.x * kRtBlock + threadIdx.x;
+	if (k >= nInst) return;
+	RtBox r = rt_empty();
+	for (uint32_t p = 0; p < parts; ++p) rt_unite(r, partial[(size_t)k * parts + p]);
+	out[k] = r;
 }
 
 __global__ void __launch_bounds__(kRtBlock) k_rt_morton(uint32_t T, uint32_t nInst, const uint32_t* begin,
@@ -240,6 +253,14 @@ b3g_status rt_build(Body& b) {
 	if ((s = rt_alloc(b, b.rtChunk2, (size_t)std::max<uint64_t>(1, n2) * sizeof(RtBox))) != B3G_OK) return s;
 	if ((s = rt_alloc(b, b.rtChunk3, (size_t)std::max<uint64_t>(1, n3) * sizeof(RtBox))) != B3G_OK) return s;
 	if ((s = rt_alloc(b, b.rtInstBox, (size_t)nInst * sizeof(RtBox))) != B3G_OK) return s;
+	// workgroups per instance for its bounds: about 32 leaves per thread for
+	// the largest instance, at most 65536 workgroups in all
+	uint32_t maxLeaves = 0;
+	for (uint32_t k = 0; k < nInst; ++k)
+		maxLeaves = std::max(maxLeaves, b.hostRayInstBegin[k + 1] - b.hostRayInstBegin[k]);
+	uint32_t parts = std::max(1u, (maxLeaves + 32 * kRtBlock - 1) / (32 * kRtBlock));
+	parts = std::max(1u, std::min(parts, 65536u / std::max(1u, nInst)));
+	if ((s = rt_alloc(b, b.rtInstPart, (size_t)nInst * parts * sizeof(RtBox))) != B3G_OK) return s;
 	int instBits = 0;
 	while (instBits < 34 && (1ull << instBits) < nInst) ++instBits;
 	const int endBit = 30 + instBits;
@@ -258,7 +279,11 @@ b3g_status rt_build(Body& b) {
 	hipLaunchKernelGGL(k_rt_leaf, dim3(rt_blocks(T)), dim3(kRtBlock), 0, b.stream, T, dptr<uint4>(b.rtTri), dptr<V3>(b.x),
 	                   box);
 	B3G_HIP(hipGetLastError());
-	hipLaunchKernelGGL(k_rt_inst_box, dim3(nInst), dim3(kRtBlock), 0, b.stream, begin, box, dptr<RtBox>(b.rtInstBox));
+	hipLaunchKernelGGL(k_rt_inst_box, dim3(nInst * parts), dim3(kRtBlock), 0, b.stream, begin, parts, box,
+	                   dptr<RtBox>(b.rtInstPart));
+	B3G_HIP(hipGetLastError());
+	hipLaunchKernelGGL(k_rt_inst_join, dim3(rt_blocks(nInst)), dim3(kRtBlock), 0, b.stream, nInst, parts,
+	                   dptr<RtBox>(b.rtInstPart), dptr<RtBox>(b.rtInstBox));
 	B3G_HIP(hipGetLastError());
 	hipLaunchKernelGGL(k_rt_morton, dim3(rt_blocks(T)), dim3(kRtBlock), 0, b.stream, T, nInst, begin, box,
 	                   dptr<RtBox>(b.rtInstBox), key, val);
