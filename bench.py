@@ -277,10 +277,18 @@ def run_contact(args):
     batch.set_graph(not args.no_graph)
     rows, blocks = batch.system_size()
     log(f"batch built on the device in {time.time() - t0:.1f} s: {rows} rows, {blocks} blocks")
+    from bounce_softbody_amd import _lib
+    solver = args.contact_solver
+    if solver == "pcg":
+        sub, tol = args.sub_iterations, args.sub_tolerance
+    else:
+        mode = _lib.SOLVER_REFINED_FP32 if solver == "refined_fp32" else _lib.SOLVER_REFINED_BF16
+        batch.set_solver(mode, args.inner_iterations, args.inner_tolerance)
+        sub, tol = args.max_sweeps * args.inner_iterations, args.refine_tolerance
 
     def steps(k):
         for _ in range(k):
-            batch.Step(dt, 1, args.sub_iterations, 0.0, args.sub_tolerance)
+            batch.Step(dt, 1, sub, 0.0, tol)
         batch.synchronize()
 
     steps(args.pre_steps)
@@ -327,8 +335,11 @@ def run_contact(args):
         "config": {"workload": f"contact: {args.bodies} x [cloth {args.contact_cloth + 1}^2 vertices + tet "
                                f"{args.contact_tet}^3 cells] over one {args.sdf_res}^3 SDF, {nv} particles "
                                f"(all sphere fixtures), {nt} tets, {ntri} cloth triangles, {blocks} blocks",
-                   "dt": "1/60", "newton_iterations": 1, "pcg_max_iterations": args.sub_iterations,
-                   "pcg_tolerance": args.sub_tolerance, "pre_steps": args.pre_steps,
+                   "dt": "1/60", "newton_iterations": 1, "pcg_max_iterations": sub,
+                   "pcg_tolerance": tol, "pre_steps": args.pre_steps,
+                   "solver": solver if solver == "pcg" else
+                   f"{solver}: {args.inner_iterations} inner iterations at {args.inner_tolerance} per sweep, "
+                   f"<= {args.max_sweeps} sweeps to {args.refine_tolerance}",
                    "form": "one batch" + ("" if args.no_graph else ", captured steps"),
                    "body_steps_per_s": round(value * args.bodies, 1),
                    "contacts_before_after": [contacts_start, contacts_end],
@@ -471,6 +482,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="batch / contact workloads: uncaptured steps")
     ap.add_argument("--contact-cloth", type=int, default=127, help="contact workload: cloth cells per side")
     ap.add_argument("--contact-tet", type=int, default=16, help="contact workload: tet block cells per axis")
+    ap.add_argument("--contact-solver", choices=["pcg", "refined_fp32", "refined_bf16"], default="pcg",
+                    help="contact workload: the reference's PCG, or FP64 refinement around an FP32 / BF16 inner PCG")
     ap.add_argument("--sdf-res", type=int, default=128, help="contact and sdf_build workloads: SDF voxels per axis")
     ap.add_argument("--pre-steps", type=int, default=None,
                     help="precision / contact workloads: untimed steps first (default 10 / 40)")
