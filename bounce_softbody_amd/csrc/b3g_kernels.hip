@@ -1235,14 +1235,22 @@ __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib&
 		}
 		__syncthreads();
 		if (own) {
+			// the slot's contributions in list order; two LDS reads ahead of their adds
 			const uint32_t kb = max(rb, cb), ke = min(re, ce);
-			for (uint32_t k = kb; k < ke; ++k) {
+			auto at = [&](uint32_t k) {
 				M33 v;
 				float* pv = &v.x.x;
 #pragma unroll
 				for (int c = 0; c < 9; ++c) pv[c] = lds[c * kAsmCap + (k - cb)];
-				acc = acc + v;
+				return v;
+			};
+			uint32_t k = kb;
+			for (; k + 2 <= ke; k += 2) {
+				const M33 v0 = at(k), v1 = at(k + 1);
+				acc = acc + v0;
+				acc = acc + v1;
 			}
+			for (; k < ke; ++k) acc = acc + at(k);
 		}
 		__syncthreads();
 	}
@@ -1547,8 +1555,18 @@ __device__ __forceinline__ bool tile_rows(const CGArgs& a, uint32_t t, uint32_t&
 		else products([&](uint32_t k) { return ldA(a.A, a.stride, k); });
 		__syncthreads();
 		if (own) {
+			// the row's products in CSR order; four LDS reads ahead of their adds
 			const uint32_t kb = max(rb, cb), ke = min(re, ce);
-			for (uint32_t k = kb; k < ke; ++k) acc = acc + v3(lds[3 * (k - cb)], lds[3 * (k - cb) + 1], lds[3 * (k - cb) + 2]);
+			auto at = [&](uint32_t k) { return v3(lds[3 * (k - cb)], lds[3 * (k - cb) + 1], lds[3 * (k - cb) + 2]); };
+			uint32_t k = kb;
+			for (; k + 4 <= ke; k += 4) {
+				const V3 v0 = at(k), v1 = at(k + 1), v2 = at(k + 2), v3_ = at(k + 3);
+				acc = acc + v0;
+				acc = acc + v1;
+				acc = acc + v2;
+				acc = acc + v3_;
+			}
+			for (; k < ke; ++k) acc = acc + at(k);
 		}
 		__syncthreads();
 	}
