@@ -160,6 +160,93 @@ def run_precision(args):
     print(json.dumps(line), flush=True)
 
 
+def run_batch_precision(args):
+    """Precision form on the many-body batch (args.bodies tet boxes, as in the batch workload): every solver form
+    steps the whole batch, captured, from the same state (after args.pre_steps steps); its error is the largest,
+    over the first args.cpu_sample_bodies bodies, relative max-norm of one step's velocities against the FP64
+    oracle (the reference restated in double, PCG tolerance 1e-10) from that body's state. value = steps/s of the
+    fastest refined form within 1e-5; config has every form and the ratio to the fastest plain FP32 form that
+    meets 1e-5 (or the tightest plain form when none does)."""
+    import oracle
+    from bounce_softbody_amd import _lib, configs
+    from bounce_softbody_amd.body import Body
+
+    dt = configs.DT
+    t0 = time.time()
+    arrs = configs.batch_boxes(args.bodies)
+    batch = Body(arrs)
+    batch.set_graph(not args.no_graph)
+    log(f"{args.bodies} bodies built in {time.time() - t0:.1f} s; system {batch.system_size()}")
+    for _ in range(args.pre_steps):
+        batch.Step(dt, 1, 200, 0.0, 1e-6)
+    batch.synchronize()
+    start = batch.save_state()
+    x0, v0 = batch.state()
+    xs, vs = batch.split(x0), batch.split(v0)
+    sample = list(range(min(args.cpu_sample_bodies, len(arrs))))
+    t0 = time.time()
+    v64 = {}
+    for k in sample:
+        o = oracle.OracleBody(arrs[k], precision=64)
+        o.set_state(xs[k], vs[k])
+        o.step(dt, 1, 20000, 0.0, 1e-10)
+        v64[k] = o.state64()[1]
+        del o
+    log(f"FP64 oracle steps of {len(sample)} bodies in {time.time() - t0:.1f} s")
+    forms = [
+        ("plain_fp32", _lib.SOLVER_PCG, (), args.sub_iterations, args.sub_tolerance),
+        ("plain_fp32_200", _lib.SOLVER_PCG, (), 200, 1e-6),
+        ("plain_fp32_tight", _lib.SOLVER_PCG, (), 1000, 1e-7),
+        ("refined_fp32", _lib.SOLVER_REFINED_FP32, (args.inner_iterations, args.inner_tolerance),
+         args.max_sweeps * args.inner_iterations, args.refine_tolerance),
+        ("refined_bf16", _lib.SOLVER_REFINED_BF16, (args.inner_iterations, args.inner_tolerance),
+         args.max_sweeps * args.inner_iterations, args.refine_tolerance),
+    ]
+    out = {}
+    for name, mode, inner, sub, tol in forms:
+        batch.set_solver(mode, *inner)
+        batch.load_state(start)
+        batch.Step(dt, 1, sub, 0.0, tol)
+        _, v1 = batch.state()
+        v1s = batch.split(v1)
+        err = max(float(np.abs(v1s[k] - v64[k]).max() / np.abs(v64[k]).max()) for k in sample)
+        abs_err = max(float(np.abs(v1s[k] - v64[k]).max()) for k in sample)
+        batch.load_state(start)
+        for _ in range(args.warmup):
+            batch.Step(dt, 1, sub, 0.0, tol)
+        batch.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            batch.Step(dt, 1, sub, 0.0, tol)
+        batch.synchronize()
+        el = time.perf_counter() - t0
+        out[name] = {"steps_per_s": round(args.steps / el, 3), "max_rel_err_vs_fp64": err,
+                     "max_abs_err_m_per_s": abs_err,
+                     "pcg": {"max_iterations": sub, "tolerance": tol, "inner": list(inner)}}
+        log(f"{name}: {out[name]}")
+    refined = [k for k in ("refined_fp32", "refined_bf16") if out[k]["max_rel_err_vs_fp64"] <= 1e-5]
+    best_name = max(refined or ["refined_fp32", "refined_bf16"], key=lambda k: out[k]["steps_per_s"])
+    plain_ok = [k for k in ("plain_fp32", "plain_fp32_200", "plain_fp32_tight") if out[k]["max_rel_err_vs_fp64"] <= 1e-5]
+    plain_name = max(plain_ok, key=lambda k: out[k]["steps_per_s"]) if plain_ok else "plain_fp32_tight"
+    best = out[best_name]
+    line = {
+        "metric": "sim steps/s", "value": best["steps_per_s"], "unit": "steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 / best["steps_per_s"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 inner / f64 residual" if best_name == "refined_fp32" else "bf16 off-diagonal inner / f64 residual",
+        "data": "synthetic",
+        "config": {"workload": f"precision form on the batch of {args.bodies} tet boxes (a, b, c ~ U{{3..24}} cells)",
+                   "dt": "1/60", "newton_iterations": 1, "form": f"{best_name} (value), one batch"
+                   + ("" if args.no_graph else ", captured steps"),
+                   "error_sample_bodies": len(sample), "forms": out,
+                   "plain_reference_form": plain_name,
+                   "speedup_vs_plain_at_1e-5": round(best["steps_per_s"] / out[plain_name]["steps_per_s"], 3),
+                   "plain_meets_1e-5": bool(plain_ok)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def run_batch(args):
     """Many-body batch: args.bodies uneven tet boxes in one handle (one launch sequence per step, captured as a
     HIP graph). Also times the same first args.handles_subset bodies as separate handles, and the reference
@@ -458,8 +545,8 @@ def run_ray_cast(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 100; batch_precision: 5 per form)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default 5; batch_precision: 1)")
     ap.add_argument("--tet", type=int, default=64, help="cells per axis of the tet grid")
     ap.add_argument("--cloth", type=int, default=511, help="cells per side of the cloth (vertices = cells + 1)")
     ap.add_argument("--sub-iterations", type=int, default=50)
@@ -470,7 +557,8 @@ def main():
                     help="step the two bodies as one batch of two instances instead of two handles")
     ap.add_argument("--graph", action="store_true", help="replay each body's step as a captured HIP graph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["two_body", "precision", "batch", "contact", "sdf_build", "ray_cast"],
+    ap.add_argument("--workload", choices=["two_body", "precision", "batch", "batch_precision", "contact", "sdf_build",
+                                           "ray_cast"],
                     default="two_body")
     ap.add_argument("--sdf-subdivisions", type=int, default=4, help="sdf_build workload: icosphere subdivisions")
     ap.add_argument("--rays", type=int, default=1000000, help="ray_cast workload: rays per query")
@@ -502,8 +590,13 @@ def main():
     contact = args.workload == "contact"
     if args.bodies is None:
         args.bodies = 256 if contact else 4096
+    bp = args.workload == "batch_precision"
+    if args.steps is None:
+        args.steps = 5 if bp else 100
+    if args.warmup is None:
+        args.warmup = 1 if bp else 5
     if args.cpu_sample_bodies is None:
-        args.cpu_sample_bodies = 2 if contact else 32
+        args.cpu_sample_bodies = 2 if contact else (8 if bp else 32)
     if args.pre_steps is None:
         args.pre_steps = 40 if contact else 10
     if args.workload == "precision":
@@ -520,6 +613,9 @@ def main():
         return
     if args.workload == "batch":
         run_batch(args)
+        return
+    if args.workload == "batch_precision":
+        run_batch_precision(args)
         return
 
     from bounce_softbody_amd import configs

@@ -7,7 +7,7 @@ import sys
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-WORKLOADS = ["two_body", "precision", "batch", "contact", "sdf_build", "ray_cast"]
+WORKLOADS = ["two_body", "precision", "batch", "batch_precision", "contact", "sdf_build", "ray_cast"]
 
 
 def run(*args):
