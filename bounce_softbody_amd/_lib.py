@@ -185,10 +185,16 @@ class DeviceArray:
     def address(self) -> int:
         return self.ptr.value
 
-    def __del__(self):
+    def free(self):
         if self.ptr and self._hip is not None:
             self._hip.hipFree(self.ptr)
-            self.ptr = C.c_void_p()
+        self.ptr = C.c_void_p()
+
+    def __del__(self):
+        # at interpreter exit the runtime may already be torn down: leave the memory to the process's end
+        import sys
+        if not sys.is_finalizing():
+            self.free()
 
 
 def lib():

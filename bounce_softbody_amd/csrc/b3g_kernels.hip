@@ -414,9 +414,7 @@ __global__ void __launch_bounds__(kBlock) k_tet_forces(TetArgs a) {
 		V3 fs = zero3();
 		V3 fds = zero3();
 		for (int j = 0; j < 4; ++j) {
-			const int blk = i + 4 * j;
-			(void)blk;
-			M33 K0 = tet_k0(rec, i, j);
+			M33 K0 = tet_k0(rec, i, j);   // m_K[i + 4 j]
 			fs = fs + K0 * us[j];
 			if (kd > 0.0f) {
 				M33 K = R * K0 * RT;
