@@ -108,3 +108,28 @@ def test_rebuild_validates_maps():
     x2, v2 = b.state()   # a rejected call changes nothing
     assert np.array_equal(x, x2) and np.array_equal(v, v2)
     b.Step(configs.DT, 1, 20, 0.0, 1e-5)
+
+
+def test_batch_rebuild_with_identity_maps_changes_nothing():
+    """b3g_body_rebuild on a batch: the maps index the instances' concatenated creation order. Identity maps over
+    an uneven batch with contacts leave every later step equal to those of an untouched batch."""
+    arrs = [configs.PARITY_SCENES[n]().arrays() for n in ("drop_on_box", "cloth_shapes", "springs", "tet_grid")]
+    a, b = Body(arrs), Body(arrs)
+    for body in (a, b):
+        for _ in range(8):
+            body.Step(configs.DT, 1, 20, 0.0, 1e-5)
+    assert len(a.contacts()[0]) > 0
+    total = lambda key, w=1: sum(len(x[key]) // w for x in arrs)
+    carry = {"particle_prev": np.arange(total("p_type"), dtype=np.int64),
+             "tet_prev": np.arange(total("tet_param", 3), dtype=np.int64),
+             "sphere_prev": np.arange(total("s_particle"), dtype=np.int64),
+             "world_prev": np.arange(total("w_type"), dtype=np.int64)}
+    b.rebuild(arrs, carry)
+    assert np.array_equal(a.tet_quats(), b.tet_quats())
+    assert all(np.array_equal(u, w) for u, w in zip(a.contacts(), b.contacts()))
+    for _ in range(5):
+        for body in (a, b):
+            body.Step(configs.DT, 1, 20, 0.0, 1e-5)
+        xa, va = a.state()
+        xb, vb = b.state()
+        assert np.array_equal(xa, xb) and np.array_equal(va, vb)
