@@ -180,18 +180,19 @@ struct MeshTrees {
 	std::vector<uint2> child;
 };
 
-inline void build_mesh_tree(const float* verts, const uint32_t* tris, uint32_t nt, MeshTrees& out) {
+inline void build_mesh_tree(const float* verts, const uint32_t* tris, uint32_t nt, MeshTrees& out, uint32_t stride = 6,
+                            float pad = kLinearSlop) {
 	const uint32_t base = (uint32_t)out.leaf.size();
 	std::vector<RtBox> box(nt);
 	RtBox cb = rt_empty();
 	for (uint32_t t = 0; t < nt; ++t) {
 		auto vert = [&](int j) {
-			const float* p = verts + 3 * (size_t)tris[6 * (size_t)t + j];
+			const float* p = verts + 3 * (size_t)tris[stride * (size_t)t + j];
 			return v3(p[0], p[1], p[2]);
 		};
 		const V3 v1 = vert(0), v2 = vert(1), v3_ = vert(2);
-		const V3 lo = vmin(v1, vmin(v2, v3_)) - v3(kLinearSlop, kLinearSlop, kLinearSlop);
-		const V3 hi = vmax(v1, vmax(v2, v3_)) + v3(kLinearSlop, kLinearSlop, kLinearSlop);
+		const V3 lo = vmin(v1, vmin(v2, v3_)) - v3(pad, pad, pad);
+		const V3 hi = vmax(v1, vmax(v2, v3_)) + v3(pad, pad, pad);
 		box[t].lo = make_float4(lo.x, lo.y, lo.z, 0.0f);
 		box[t].hi = make_float4(hi.x, hi.y, hi.z, 0.0f);
 		RtBox c;

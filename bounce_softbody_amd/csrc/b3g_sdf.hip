@@ -4,15 +4,18 @@
 // each voxel the closest point on every triangle (O(voxels x triangles)) and
 // five parity rays to fixed far points, each against every triangle. Each
 // voxel's value is a minimum and five hit counts, so the order in which
-// triangles are visited does not change it. Here one thread owns one voxel,
-// and the workgroup stages the triangles through LDS in tiles of 256 (every
-// thread reads the same record: a broadcast). Per triangle the closest-point
-// test and all five rays share one pass. Quantities that do not depend on the
-// voxel (the edge cross product and the unit normal of b3TriangleShape::RayCast)
+// triangles are visited does not change it. Here one thread owns one voxel.
+// k_sdf_tree walks a static tree of the triangles and tests only those whose
+// padded box is near enough or crossed by a ray (the default); k_sdf visits
+// every triangle, staged through LDS in tiles of 256 (the fallback when the
+// tree is too deep for the walk's stack). Per triangle the closest-point test
+// and the rays share one pass. Quantities that do not depend on the voxel
+// (the edge cross product and the unit normal of b3TriangleShape::RayCast)
 // are computed once per triangle with the same operations, and the ray terms
 // that do not depend on the triangle once per voxel, so every value is the
 // reference's bit for bit.
 
+#include "b3g_bvh.h"
 #include "b3g_collide.h"
 #include "b3g_internal.h"
 
@@ -73,18 +76,21 @@ B3G_HD void sdf_begin(const SdfArgs& a, uint32_t x, uint32_t y, uint32_t z, SdfV
 	s.best = 3.40282347e+38f;   // B3_MAX_SCALAR
 }
 
-// One triangle record (4 float4: A, B, C, cross(B - A, C - A), unit normal).
-B3G_HD void sdf_triangle(const float4* rec, SdfVoxel& s) {
+// One triangle record (4 float4: A, B, C, cross(B - A, C - A), unit normal):
+// its distance when `dist`, and the parity rays of `rays` (bit r: ray r).
+B3G_HD void sdf_triangle(const float4* rec, SdfVoxel& s, bool dist = true, uint32_t rays = 0x1Fu) {
 	const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
 	const V3 A = v3(r0.x, r0.y, r0.z), B = v3(r0.w, r1.x, r1.y), C = v3(r1.z, r1.w, r2.x);
 	const V3 ABxAC = v3(r2.y, r2.z, r2.w), nrm = v3(r3.x, r3.y, r3.z);
 	// distance (sdf_builder.cpp:82-104)
-	const float dd = length_sq(closest_point(A, B, C, s.P) - s.P);
-	if (dd < s.best) s.best = dd;
+	if (dist) {
+		const float dd = length_sq(closest_point(A, B, C, s.P) - s.P);
+		if (dd < s.best) s.best = dd;
+	}
 	// parity rays (sdf_builder.cpp:26-78, triangle_shape.cpp:314-382)
 	const float num = dot(nrm, A - s.P);
 	for (int r = 0; r < 5; ++r) {
-		if (!s.live[r]) continue;
+		if (!s.live[r] || !(rays >> r & 1u)) continue;
 		const float den = dot(nrm, s.d[r]);
 		if (den == 0.0f) continue;
 		const float tt = num / den;
@@ -124,6 +130,102 @@ __global__ void __launch_bounds__(kSdfBlock) k_sdf(SdfArgs a) {
 		__syncthreads();
 	}
 	if (own) a.out[x + (size_t)y * a.W + (size_t)z * a.W * a.H] = sdf_end(s);
+}
+
+// The same voxels over a static tree of the triangles. A node is entered when
+// the voxel's squared distance to its box is at most the best so far, or
+// when a live parity ray's segment passes its box; a leaf then runs only
+// those tests. The leaf boxes are padded by far more than the rounding of a
+// computed distance or ray hit point (the padding grows with the far points'
+// distance), so a skipped triangle can neither lower the minimum nor add a
+// counted crossing: every value equals the brute-force kernel's.
+constexpr int kSdfStack = 64;
+
+struct SdfTree {
+	const uint2* child;       // internal nodes: children (kRtLeaf marks a leaf position)
+	const RtBox* nodeBox;
+	const RtBox* leafBox;     // per leaf position
+	const uint32_t* leaf;     // per leaf position: triangle index
+};
+
+// rt_slab's conservative test with the ray's reciprocal direction computed
+// once per voxel (the 1e-6 relative margin on t covers its rounding).
+B3G_HD bool sdf_slab(const RtBox& b, V3 p1, V3 d, V3 inv) {
+	float t0 = 0.0f, t1 = 1.0f;
+	const float o[3] = {p1.x, p1.y, p1.z}, dd[3] = {d.x, d.y, d.z}, iv[3] = {inv.x, inv.y, inv.z};
+	const float lo[3] = {b.lo.x, b.lo.y, b.lo.z}, hi[3] = {b.hi.x, b.hi.y, b.hi.z};
+	for (int k = 0; k < 3; ++k) {
+		const float m = (fabsf(lo[k]) + fabsf(hi[k]) + fabsf(o[k])) * 1.0e-5f + 1.0e-30f;
+		const float l = lo[k] - m, h = hi[k] + m;
+		if (dd[k] == 0.0f) {
+			if (!(o[k] >= l && o[k] <= h)) return false;
+			continue;
+		}
+		float a = (l - o[k]) * iv[k], c = (h - o[k]) * iv[k];
+		if (a > c) { const float tmp = a; a = c; c = tmp; }
+		t0 = fmaxf(t0, a - 1.0e-6f * fabsf(a));
+		t1 = fminf(t1, c + 1.0e-6f * fabsf(c));
+		if (!(t0 <= t1)) return false;
+	}
+	return true;
+}
+
+B3G_HD float box_dist2(const RtBox& b, V3 P) {
+	const float dx = fmaxf(fmaxf(b.lo.x - P.x, 0.0f), P.x - b.hi.x);
+	const float dy = fmaxf(fmaxf(b.lo.y - P.y, 0.0f), P.y - b.hi.y);
+	const float dz = fmaxf(fmaxf(b.lo.z - P.z, 0.0f), P.z - b.hi.z);
+	return dx * dx + dy * dy + dz * dz;
+}
+
+__global__ void __launch_bounds__(kSdfBlock) k_sdf_tree(SdfArgs a, SdfTree tr) {
+	const uint64_t v = (uint64_t)blockIdx.x * kSdfBlock + threadIdx.x;
+	const uint64_t count = (uint64_t)a.W * a.H * a.D;
+	if (v >= count) return;
+	const uint32_t x = (uint32_t)(v % a.W), y = (uint32_t)((v / a.W) % a.H), z = (uint32_t)(v / ((uint64_t)a.W * a.H));
+	SdfVoxel s;
+	sdf_begin(a, x, y, z, s);
+	V3 inv[5];
+	for (int r = 0; r < 5; ++r)
+		inv[r] = v3(s.d[r].x != 0.0f ? 1.0f / s.d[r].x : 0.0f, s.d[r].y != 0.0f ? 1.0f / s.d[r].y : 0.0f,
+		            s.d[r].z != 0.0f ? 1.0f / s.d[r].z : 0.0f);
+	uint32_t stack[kSdfStack];
+	int sp = 0;
+	stack[sp++] = a.nt == 1 ? kRtLeaf : 0u;   // the host checked that the tree's depth fits the stack
+	while (sp > 0) {
+		const uint32_t c = stack[--sp];
+		const bool leaf = (c & kRtLeaf) != 0;
+		const RtBox bx = leaf ? tr.leafBox[c & ~kRtLeaf] : tr.nodeBox[c];
+		const bool dist = box_dist2(bx, s.P) <= s.best;
+		uint32_t rays = 0;
+		for (int r = 0; r < 5; ++r)
+			if (s.live[r] && sdf_slab(bx, s.P, s.d[r], inv[r])) rays |= 1u << r;
+		if (!dist && !rays) continue;
+		if (leaf) {
+			sdf_triangle(a.tri + 4 * (size_t)tr.leaf[c & ~kRtLeaf], s, dist, rays);
+		} else {
+			const uint2 ch = tr.child[c];
+			stack[sp++] = ch.y;
+			stack[sp++] = ch.x;
+		}
+	}
+	a.out[x + (size_t)y * a.W + (size_t)z * a.W * a.H] = sdf_end(s);
+}
+
+// Depth of a tree (root 0, kRtLeaf leaves): the stack of k_sdf_tree holds at
+// most depth + 1 entries.
+inline uint32_t tree_depth(const std::vector<uint2>& child, uint32_t nt) {
+	if (nt < 2) return 0;
+	uint32_t deepest = 0;
+	std::vector<std::pair<uint32_t, uint32_t>> todo{{0u, 0u}};
+	while (!todo.empty()) {
+		const auto [c, d] = todo.back();
+		todo.pop_back();
+		deepest = std::max(deepest, d);
+		if (c & kRtLeaf) continue;
+		todo.push_back({child[c].x, d + 1});
+		todo.push_back({child[c].y, d + 1});
+	}
+	return deepest;
 }
 
 // per triangle: the vertices, cross(B - A, C - A) and its unit vector as
@@ -200,11 +302,19 @@ b3g_status build_sdf(int device, const float* vertices, uint32_t nv, const uint3
 	}
 	if (count > (1ull << 32)) { set_error("more than 2^32 voxels"); return B3G_UNSUPPORTED; }
 	std::vector<float4> rec = sdf_records(vertices, triangles, nt, stride);
+	// the triangle tree, padded beyond the rounding of distances and ray hits
+	SdfArgs args = sdf_args(box, dims, nt);
+	const V3 ext = box.hi - box.lo;
+	const float pad = kLinearSlop + 1.0e-5f * (fabsf(args.far) + ext.x + ext.y + ext.z);
+	MeshTrees tree;
+	build_mesh_tree(vertices, triangles, nt, tree, stride, pad);
+	const bool useTree = tree_depth(tree.child, nt) + 2 <= (uint32_t)kSdfStack;
 	B3G_HIP(hipSetDevice(device));
 	hipStream_t stream;
 	B3G_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
 	float4* dTri = nullptr;
 	float* dOut = nullptr;
+	void* dTree = nullptr;
 	b3g_status st = B3G_OK;
 	do {
 		if (hipMalloc(&dTri, rec.size() * sizeof(float4)) != hipSuccess || hipMalloc(&dOut, count * 4) != hipSuccess) {
@@ -216,10 +326,39 @@ b3g_status build_sdf(int device, const float* vertices, uint32_t nv, const uint3
 			st = B3G_HIP_ERROR;
 			break;
 		}
-		SdfArgs a = sdf_args(box, dims, nt);
+		SdfArgs a = args;
 		a.tri = dTri;
 		a.out = dOut;
-		hipLaunchKernelGGL(k_sdf, dim3((unsigned)((count + kSdfBlock - 1) / kSdfBlock)), dim3(kSdfBlock), 0, stream, a);
+		const unsigned grid = (unsigned)((count + kSdfBlock - 1) / kSdfBlock);
+		if (useTree) {
+			// one allocation: child, node boxes, leaf boxes, leaf triangles
+			const size_t nc = std::max<size_t>(1, tree.child.size());
+			const size_t bChild = nc * sizeof(uint2), bNode = nc * sizeof(RtBox), bLeaf = tree.leafBox.size() * sizeof(RtBox),
+			             bIdx = tree.leaf.size() * 4;
+			if (hipMalloc(&dTree, bChild + bNode + bLeaf + bIdx) != hipSuccess) {
+				set_error("out of device memory for the triangle tree");
+				st = B3G_OUT_OF_MEMORY;
+				break;
+			}
+			char* p = static_cast<char*>(dTree);
+			SdfTree tr;
+			tr.child = reinterpret_cast<const uint2*>(p);
+			tr.nodeBox = reinterpret_cast<const RtBox*>(p + bChild);
+			tr.leafBox = reinterpret_cast<const RtBox*>(p + bChild + bNode);
+			tr.leaf = reinterpret_cast<const uint32_t*>(p + bChild + bNode + bLeaf);
+			if ((!tree.child.empty() &&
+			     (hipMemcpyAsync(p, tree.child.data(), tree.child.size() * sizeof(uint2), hipMemcpyHostToDevice, stream) != hipSuccess ||
+			      hipMemcpyAsync(p + bChild, tree.nodeBox.data(), tree.nodeBox.size() * sizeof(RtBox), hipMemcpyHostToDevice,
+			                     stream) != hipSuccess)) ||
+			    hipMemcpyAsync(p + bChild + bNode, tree.leafBox.data(), bLeaf, hipMemcpyHostToDevice, stream) != hipSuccess ||
+			    hipMemcpyAsync(p + bChild + bNode + bLeaf, tree.leaf.data(), bIdx, hipMemcpyHostToDevice, stream) != hipSuccess) {
+				st = B3G_HIP_ERROR;
+				break;
+			}
+			hipLaunchKernelGGL(k_sdf_tree, dim3(grid), dim3(kSdfBlock), 0, stream, a, tr);
+		} else {
+			hipLaunchKernelGGL(k_sdf, dim3(grid), dim3(kSdfBlock), 0, stream, a);
+		}
 		if (hipGetLastError() != hipSuccess) { st = B3G_HIP_ERROR; break; }
 		if (hipMemcpyAsync(voxels, dOut, count * 4, hipMemcpyDeviceToHost, stream) != hipSuccess ||
 		    hipStreamSynchronize(stream) != hipSuccess)
@@ -229,6 +368,7 @@ b3g_status build_sdf(int device, const float* vertices, uint32_t nv, const uint3
 	(void)hipStreamSynchronize(stream);
 	if (dTri) (void)hipFree(dTri);
 	if (dOut) (void)hipFree(dOut);
+	if (dTree) (void)hipFree(dTree);
 	(void)hipStreamDestroy(stream);
 	return st;
 }
