@@ -361,12 +361,20 @@ __device__ Q4 extract_rotation(const M33& A, Q4 q) {
 		if (w < kTol) break;
 		V3 axis = omega / w;
 		float theta = 0.5f * w;
-		// the reference narrows the double-precision sin/cos (quat.h:83-88)
-		double sd, cd;
-		sincos((double)theta, &sd, &cd);
+		// the reference narrows the double-precision sin/cos (quat.h:83-88).
+		// Below 2e-4 the narrowed values are exactly theta and 1 (theta^3 / 6
+		// and theta^2 / 2 are under half a float ulp; every float there is
+		// checked against the host's libm), so the double sincos is skipped
+		float sf = theta, cf = 1.0f;
+		if (!(theta < 2.0e-4f)) {
+			double sd, cd;
+			sincos((double)theta, &sd, &cd);
+			sf = (float)sd;
+			cf = (float)cd;
+		}
 		Q4 qw;
-		qw.v = (float)sd * axis;
-		qw.s = (float)cd;
+		qw.v = sf * axis;
+		qw.s = cf;
 		q = qmul(qw, q);
 		qnormalize(q);
 	}
