@@ -32,21 +32,32 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 PROFILE_PMC = os.path.join(ROOT, "profiles", "k_cg_spmv_dq_tet64_pmc.txt")
+PROFILE_PMC_BATCH = os.path.join(ROOT, "profiles", "k_cg_spmv_dq_batch4096_pmc.txt")
+PROFILE_PMC_CONTACT = os.path.join(ROOT, "profiles", "k_cg_spmv_dq_contact256_pmc.txt")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def traffic_from_profile():
-    """HBM bytes per SpMV launch measured by rocprofv3 --pmc (profiles/k_cg_spmv_dq_tet64_pmc.txt), if present."""
+def traffic_from_profile(path=PROFILE_PMC, workload=None):
+    """HBM bytes per SpMV launch measured by rocprofv3 --pmc, from a profile summary under profiles/, if present.
+    With `workload`, the summary's `workload:` line must equal it: a figure applies only to the system it was
+    measured on."""
+    fields = {}
     try:
-        for line in open(PROFILE_PMC):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":", 1)[1])
+        for line in open(path):
+            key, sep, rest = line.partition(":")
+            if sep and key in ("workload", "traffic_bytes_per_launch"):
+                fields[key] = rest.strip()
     except OSError:
-        pass
-    return None
+        return None
+    if workload is not None and fields.get("workload") != workload:
+        return None
+    try:
+        return float(fields["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(tet_arr, cloth_arr, dt, sub, sub_tol, sample_steps):
@@ -314,6 +325,8 @@ def run_batch(args):
                     "sample": f"one step of each of the first {len(sample)} bodies ({st} tets, {total:.1f} s of "
                               f"CPU time), scaled to the batch's {nt} tets; single thread"}
         log(f"CPU baseline: {baseline}")
+    workload = (f"batch of {args.bodies} tet boxes (a, b, c ~ U{{3..24}} cells), {nv} particles, {nt} tets, "
+                f"{blocks} blocks")
     line = {
         "metric": "sim steps/s",
         "value": round(value, 3),
@@ -327,8 +340,7 @@ def run_batch(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"batch of {args.bodies} tet boxes (a, b, c ~ U{{3..24}} cells), {nv} particles, "
-                               f"{nt} tets, {blocks} blocks",
+        "config": {"workload": workload,
                    "dt": "1/60", "newton_iterations": 1, "pcg_max_iterations": args.sub_iterations,
                    "pcg_tolerance": args.sub_tolerance, "form": "one batch" + ("" if args.no_graph else ", captured steps"),
                    "body_steps_per_s": round(value * args.bodies, 1),
@@ -337,7 +349,8 @@ def run_batch(args):
                                           "batch": round(1e3 * t_small, 3)}},
         "roofline": {"kernel": "k_cg_spmv_dq (all instances, back-to-back pass)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": by,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profile(PROFILE_PMC_BATCH, workload), "bytes_per_launch": by,
                      "mean_launch_us": round(mean_ms * 1e3, 2)},
         "cpu_baseline": baseline,
     }
@@ -406,6 +419,9 @@ def run_contact(args):
                     "sample": f"the first step of each of the first {len(sample)} bodies ({total:.1f} s of CPU "
                               f"time), scaled to the batch's {len(arrs)} equal-sized bodies; single thread"}
         log(f"CPU baseline: {baseline}")
+    workload = (f"contact: {args.bodies} x [cloth {args.contact_cloth + 1}^2 vertices + tet {args.contact_tet}^3 cells] "
+                f"over one {args.sdf_res}^3 SDF, {nv} particles (all sphere fixtures), {nt} tets, {ntri} cloth "
+                f"triangles, {blocks} blocks")
     line = {
         "metric": "sim steps/s",
         "value": round(value, 3),
@@ -419,9 +435,7 @@ def run_contact(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"contact: {args.bodies} x [cloth {args.contact_cloth + 1}^2 vertices + tet "
-                               f"{args.contact_tet}^3 cells] over one {args.sdf_res}^3 SDF, {nv} particles "
-                               f"(all sphere fixtures), {nt} tets, {ntri} cloth triangles, {blocks} blocks",
+        "config": {"workload": workload,
                    "dt": "1/60", "newton_iterations": 1, "pcg_max_iterations": sub,
                    "pcg_tolerance": tol, "pre_steps": args.pre_steps,
                    "solver": solver if solver == "pcg" else
@@ -433,7 +447,10 @@ def run_contact(args):
                    "pcg_iterations_min_max": [min(its), max(its)]},
         "roofline": {"kernel": "k_cg_spmv_dq (all instances, back-to-back pass)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": by,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     # the summary was measured on the FP32 blocks; the BF16 operator moves other bytes
+                     "traffic": None if solver == "refined_bf16" else traffic_from_profile(PROFILE_PMC_CONTACT, workload),
+                     "bytes_per_launch": by,
                      "mean_launch_us": round(mean_ms * 1e3, 2)},
         "cpu_baseline": baseline,
     }
