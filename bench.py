@@ -60,6 +60,26 @@ def traffic_from_profile(path=PROFILE_PMC, workload=None):
         return None
 
 
+def host_cpu():
+    """The host's CPU model and the number of CPUs this process may run on."""
+    model = "unknown CPU"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return f"{model}, {len(os.sched_getaffinity(0))} CPUs available to the process"
+
+
+def emit(line):
+    """Prints the bench line; a CPU baseline also names the host it ran on."""
+    if isinstance(line.get("cpu_baseline"), dict):
+        line["cpu_baseline"]["host"] = host_cpu()
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline(tet_arr, cloth_arr, dt, sub, sub_tol, sample_steps):
     """Times `sample_steps` steps of both bodies with the reference build (or the oracle)."""
     import oracle
@@ -168,7 +188,7 @@ def run_precision(args):
                      "bytes_per_launch": best["spmv"]["bytes_per_launch"], "mean_launch_us": best["spmv"]["mean_us"]},
         "cpu_baseline": None,
     }
-    print(json.dumps(line), flush=True)
+    emit(line)
 
 
 def run_batch_precision(args):
@@ -255,7 +275,7 @@ def run_batch_precision(args):
                    "plain_meets_1e-5": bool(plain_ok)},
         "roofline": None, "cpu_baseline": None,
     }
-    print(json.dumps(line), flush=True)
+    emit(line)
 
 
 def run_batch(args):
@@ -354,7 +374,7 @@ def run_batch(args):
                      "mean_launch_us": round(mean_ms * 1e3, 2)},
         "cpu_baseline": baseline,
     }
-    print(json.dumps(line), flush=True)
+    emit(line)
 
 
 def run_contact(args):
@@ -454,7 +474,7 @@ def run_contact(args):
                      "mean_launch_us": round(mean_ms * 1e3, 2)},
         "cpu_baseline": baseline,
     }
-    print(json.dumps(line), flush=True)
+    emit(line)
 
 
 def run_sdf_build(args):
@@ -492,7 +512,7 @@ def run_sdf_build(args):
             "config": {"workload": f"b3BuildSDF of an icosphere ({len(t)} triangles) on a {dims[0]}x{dims[1]}x{dims[2]} grid",
                        "voxel_triangle_pairs_per_s": round(pairs / el, 1)},
             "roofline": None, "cpu_baseline": baseline}
-    print(json.dumps(line), flush=True)
+    emit(line)
 
 
 def run_ray_cast(args):
@@ -556,7 +576,7 @@ def run_ray_cast(args):
                        "ms_tree_reused": round(reuse * 1e3, 3),
                        "host_arrays_rays_per_s": round(args.rays / host, 1)},
             "roofline": None, "cpu_baseline": None}
-    print(json.dumps(line), flush=True)
+    emit(line)
 
 
 def main():
@@ -737,7 +757,7 @@ def main():
         },
         "cpu_baseline": baseline,
     }
-    print(json.dumps(out), flush=True)
+    emit(out)
 
 
 if __name__ == "__main__":
