@@ -193,6 +193,24 @@ struct SymbolicInput {
 // A stretch / shear block rebuilt in the assembly (variant: see kUVCode).
 inline uint32_t uv_code(uint32_t variant, uint32_t e) { return kUVCode | (variant << kUVShift) | e; }
 
+// A batch is merged into one description with 32-bit counts: every count,
+// summed over the instances, must fit.
+bool batch_limits(const b3g_body_desc* ds, uint32_t count) {
+	uint64_t t[12] = {};
+	for (uint32_t k = 0; k < count; ++k) {
+		const b3g_body_desc& d = ds[k];
+		const uint32_t c[12] = {d.n_particles, d.n_spheres, d.n_trifix, d.n_tetfix, d.n_forces, d.n_tet,
+		                        d.n_tri, d.n_stretch, d.n_shear, d.n_spring, d.n_mouse, d.n_world};
+		for (int i = 0; i < 12; ++i) t[i] += c[i];
+	}
+	for (int i = 0; i < 12; ++i)
+		if (t[i] > 0xFFFFFFFFull) {
+			set_error("a count summed over the batch's instances exceeds 2^32 - 1");
+			return false;
+		}
+	return true;
+}
+
 // Element counts the 32-bit contribution codes can address: 16 blocks per
 // tetrahedron in 31 bits, stretch / shear indices in kUVShift bits, staged
 // blocks (triangle elements, springs, mouse) below kUVCode.
@@ -213,7 +231,7 @@ bool code_limits(const b3g_body_desc& d) {
 	return true;
 }
 
-void build_pattern(const SymbolicInput& in, Pattern& P) {
+bool build_pattern(const SymbolicInput& in, Pattern& P) {
 	const uint32_t n = in.n;
 	RowLists D(n), V(n);
 	std::vector<std::pair<uint32_t, uint32_t>> fTouch;   // (row, stage) in order
@@ -343,7 +361,13 @@ void build_pattern(const SymbolicInput& in, Pattern& P) {
 		// reverse(V_i \ {i}) with V_i head-to-tail = first-touch order of dfdv
 		for (uint32_t c : vft)
 			if (c != i) rc.push_back(c);
-		P.rowptr[i + 1] = P.rowptr[i] + (uint32_t)rc.size();
+		// block indices are 32-bit and the planes of A are padded to a multiple of 4
+		const uint64_t end = (uint64_t)P.rowptr[i] + rc.size();
+		if (end > 0xFFFFFFFCull) {
+			set_error("more than 2^32 - 4 blocks in pA");
+			return false;
+		}
+		P.rowptr[i + 1] = (uint32_t)end;
 	}
 	const uint32_t nnz = P.rowptr[n];
 	P.cols.resize(nnz);
@@ -401,6 +425,7 @@ void build_pattern(const SymbolicInput& in, Pattern& P) {
 	P.fIdx.resize(fTouch.size());
 	std::vector<uint32_t> fill(P.fPtr.begin(), P.fPtr.end() - 1);
 	for (auto& ft : fTouch) P.fIdx[fill[ft.first]++] = ft.second;
+	return true;
 }
 
 // ComputeAABB of each world shape (shapes/*.cpp), static over the body's life.
@@ -644,6 +669,7 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 			return st;
 		}
 	}
+	if (!batch_limits(descs, count)) return B3G_UNSUPPORTED;
 	Merged M;
 	merge(descs, count, M);
 	const b3g_body_desc& d = M.d;
@@ -980,7 +1006,7 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 	si.spJx = b.spring.jx; si.spJv = b.spring.jv; si.spF = b.spring.f;
 	si.moJx = b.mouse.jx; si.moJv = b.mouse.jv; si.moF = b.mouse.f;
 	Pattern P;
-	build_pattern(si, P);
+	if (!build_pattern(si, P)) return B3G_UNSUPPORTED;
 	b.nnz = P.rowptr[n];
 	b.aStride = plane_stride(b.nnz);
 	b.hostRowptr = P.rowptr;
@@ -1226,7 +1252,7 @@ b3g_status host_pattern(const b3g_body_desc& d, std::vector<uint32_t>& rowptr, s
 	si.tetF = si.tetFd = si.triJx = si.triJv = si.triF = si.triFd = 0;
 	si.stF = si.shF = si.spJx = si.spJv = si.spF = si.moJx = si.moJv = si.moF = 0;
 	Pattern P;
-	build_pattern(si, P);
+	if (!build_pattern(si, P)) return B3G_UNSUPPORTED;
 	rowptr = P.rowptr;
 	cols = P.cols;
 	return B3G_OK;
