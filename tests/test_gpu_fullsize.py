@@ -81,3 +81,32 @@ def test_full_size_forms_agree(bodies):
         assert np.array_equal(x[static], x0[k][static]) and not np.any(v[static])
         assert np.all(np.isfinite(x)) and np.all(np.isfinite(v))
         assert not np.array_equal(x, x0[k])   # the dynamic vertices did move
+
+
+def test_full_size_precision_forms_reach_fp64():
+    """The precision workload of bench.py at full size: a 64^3 tet grid whose Young's moduli are log-uniform in
+    [1e4, 1e7] Pa, one step from a deformed, moving state. Both refined forms (FP64 refinement around 50 inner
+    iterations at 1e-3, at most 4 sweeps, to an FP64 residual of 1e-4) come within 1e-5 of the FP64 oracle's
+    velocities (relative max-norm); the plain FP32 PCG (50 iterations at 1e-5) does not."""
+    arr = configs.tet_grid(64, young_per_element=configs.log_uniform_young(64, seed=7)).arrays()
+    g = Body(arr)
+    for _ in range(10):
+        g.Step(configs.DT, 1, 200, 0.0, 1e-6)
+    start = g.save_state()
+    x, v = g.state()
+    o = oracle.OracleBody(arr, precision=64)
+    o.set_state(x, v)
+    o.step(configs.DT, 1, 20000, 0.0, 1e-10)
+    v64 = o.state64()[1].reshape(-1, 3)
+    scale = float(np.abs(v64).max())
+    errs = {}
+    for name, mode, inner, sub, tol in (("plain", _lib.SOLVER_PCG, (), SUB, SUB_TOL),
+                                        ("refined_fp32", _lib.SOLVER_REFINED_FP32, (50, 1e-3), 200, 1e-4),
+                                        ("refined_bf16", _lib.SOLVER_REFINED_BF16, (50, 1e-3), 200, 1e-4)):
+        g.set_solver(mode, *inner)
+        g.load_state(start)
+        g.Step(configs.DT, 1, sub, 0.0, tol)
+        errs[name] = float(np.abs(g.state()[1] - v64).max()) / scale
+    print(errs)
+    assert errs["refined_fp32"] <= 1e-5 and errs["refined_bf16"] <= 1e-5, errs
+    assert errs["plain"] > 1e-5, errs
