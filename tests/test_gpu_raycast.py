@@ -59,9 +59,11 @@ def test_ray_cast_follows_the_steps():
         body.Step(configs.DT, 1, 30, 0.0, 1e-5)
 
 
-def test_ray_cast_large_cloth():
-    """A 256^2-vertex cloth (130,050 triangle fixtures) after a few steps: 800 rays equal the oracle's brute force."""
-    arr = configs.cloth_on_sphere(255).arrays()
+@pytest.mark.parametrize("cells", [255, 511], ids=["cloth256", "cloth512"])
+def test_ray_cast_large_cloth(cells):
+    """A 256^2-vertex cloth (130,050 triangle fixtures) and the benchmark's 512^2 cloth (522,242) after a few
+    steps: 800 rays equal the oracle's brute force."""
+    arr = configs.cloth_on_sphere(cells).arrays()
     body = Body(arr)
     for _ in range(5):
         body.Step(configs.DT, 1, 50, 0.0, 1e-5)
