@@ -2481,11 +2481,12 @@ b3g_status enqueue_step(Body& b, float dt, uint32_t iters, uint32_t subIters, fl
 	SolverState* st = dptr<SolverState>(b.state);
 	b3g_status s = enqueue_update_contacts(b);
 	if (s != B3G_OK) return s;
+	// b3Force::ClearForces runs on every step, before the dt > 0 test (body.cpp:563-574)
+	if (b.trackActions && b.act.p && !assembleOnly)
+		LAUNCH(k_clear_v3, (uint32_t)(((size_t)b.actCount + kBlock - 1) / kBlock), (size_t)b.actCount, dptr<V3>(b.act));
 	if (dt > 0.0f) {
 		StepArgs sa = step_args(b, dt);
 		LAUNCH(k_step_begin, nblk(b.n), sa);
-		if (b.trackActions && b.act.p && !assembleOnly)   // b3Force::ClearForces (body.cpp:566-570)
-			LAUNCH(k_clear_v3, (uint32_t)(((size_t)b.actCount + kBlock - 1) / kBlock), (size_t)b.actCount, dptr<V3>(b.act));
 		const bool refined = b.solver != B3G_SOLVER_PCG && !assembleOnly;
 		const uint32_t sweeps = refined ? std::max(1u, (subIters + b.innerIt - 1) / std::max(1u, b.innerIt)) : 0u;
 		if (refined) {

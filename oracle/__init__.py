@@ -168,6 +168,10 @@ def _lib(which: str):
             lib.b3o_carry.argtypes = [C.c_void_p, C.c_void_p] + [C.POINTER(C.c_int64)] * 4
         if which == "oracle":
             lib.b3o_step.argtypes = [C.c_void_p, C.c_float, C.c_uint32, C.c_uint32, C.c_float, C.c_float]
+            lib.b3o_clone.argtypes = [C.c_void_p]
+            lib.b3o_clone.restype = C.c_void_p
+            lib.b3o_set_wide_dots.argtypes = [C.c_void_p, C.c_int]
+            lib.b3o_reset_stats.argtypes = [C.c_void_p]
             lib.b3o_get_state64.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
             lib.b3o_spmv.argtypes = [C.c_uint32, u32p, u32p, f32p, f32p, f32p]
         else:
@@ -291,6 +295,24 @@ class OracleBody(_Body):
         i64 = C.POINTER(C.c_int64)
         arrs = [np.ascontiguousarray(carry[k], np.int64) for k in ("particle_prev", "tet_prev", "sphere_prev", "world_prev")]
         self.lib.b3o_carry(self.h, old.h, *[a.ctypes.data_as(i64) for a in arrs])
+
+    def clone(self):
+        """An independent copy of this body with its whole dynamic state."""
+        c = object.__new__(OracleBody)
+        c.lib, c.n, c.n_tet, c.precision = self.lib, self.n, self.n_tet, self.precision
+        c._scene, c._keep = self._scene, self._keep
+        c.h = self.lib.b3o_clone(self.h)
+        return c
+
+    def set_wide_dots(self, wide: bool):
+        """Accumulates the PCG dot products and the Newton error in double (each term still formed in the
+        body's precision), as the device's default FP64 reduction mode does; False restores the reference's
+        accumulation (b3Dot / b3LengthSquared, dense_vec3.h:151-175)."""
+        self.lib.b3o_set_wide_dots(self.h, 1 if wide else 0)
+
+    def reset_stats(self):
+        """Restarts the running Newton / PCG statistics, so that the next step's counts read alone."""
+        self.lib.b3o_reset_stats(self.h)
 
     def assemble64(self, dt):
         """Like :meth:`assemble` (list-order CSR, column-major blocks) without rounding to float."""

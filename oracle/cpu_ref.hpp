@@ -138,6 +138,9 @@ public:
 	V3<T> gravity;
 	StepStats stats;
 	Capture<T> capture;
+	// false: b3Dot / b3LengthSquared as the reference accumulates them (in T);
+	// true: accumulated in double (VDotWide), the device's FP64 reduction mode
+	bool wideDots = false;
 
 	// Build from a creation-ordered scene (mimics the b3Body::Create* calls).
 	void Build(const b3s_scene& sc);
@@ -1353,7 +1356,9 @@ void Body<T>::ApplyForces(T h, const DVec<T>& x, const DVec<T>& v, DVec<T>& f, S
 // ======================================================================
 
 // sparse_solver.cpp:28-109
-template <class T> uint32_t SolveCGFree(const SpMat<T>& A, const DVec<T>& b, uint32_t maxIt, T eps, DVec<T>& x) {
+template <class T>
+uint32_t SolveCGFree(const SpMat<T>& A, const DVec<T>& b, uint32_t maxIt, T eps, DVec<T>& x, bool wide = false) {
+	auto dot = [wide](const DVec<T>& u, const DVec<T>& w) { return wide ? VDotWide(u, w) : VDot(u, w); };
 	const uint32_t n = A.n();
 	DMat<T> invM(n);
 	for (uint32_t i = 0; i < n; ++i) {
@@ -1362,20 +1367,20 @@ template <class T> uint32_t SolveCGFree(const SpMat<T>& A, const DVec<T>& b, uin
 	}
 	DVec<T> r = VSub(b, SpMV(A, x));
 	DVec<T> d = DiagMul(invM, r);
-	T delta_new = VDot(r, d);
+	T delta_new = dot(r, d);
 	T delta_0 = delta_new;
 	uint32_t it = 0;
 	for (;;) {
 		if (it == maxIt) break;
 		if (delta_new <= eps * eps * delta_0) break;
 		DVec<T> q = SpMV(A, d);
-		T alpha = delta_new / VDot(d, q);
+		T alpha = delta_new / dot(d, q);
 		x = VAdd(x, VScale(alpha, d));
 		if (it % 50 == 0) r = VSub(b, SpMV(A, x));
 		else r = VSub(r, VScale(alpha, q));
 		DVec<T> s = DiagMul(invM, r);
 		T delta_old = delta_new;
-		delta_new = VDot(r, s);
+		delta_new = dot(r, s);
 		T beta = delta_new / delta_old;
 		d = VAdd(s, VScale(beta, d));
 		++it;
@@ -1384,7 +1389,7 @@ template <class T> uint32_t SolveCGFree(const SpMat<T>& A, const DVec<T>& b, uin
 }
 
 template <class T> uint32_t Body<T>::SolveCG(const SpMat<T>& A, const DVec<T>& b, uint32_t maxIt, T eps, DVec<T>& x) {
-	return SolveCGFree(A, b, maxIt, eps, x);
+	return SolveCGFree(A, b, maxIt, eps, x, wideDots);
 }
 
 // sparse_force_solver.cpp:104-232
@@ -1424,7 +1429,7 @@ void Body<T>::SolveBE(T h, uint32_t maxIt, T eps, uint32_t maxSub, T subEps, con
 		stats.maxSub = Max(stats.maxSub, subIt);
 		v = VAdd(v, dv);
 		x = VAdd(VAdd(x0, VScale(h, v)), y);
-		error = VLengthSquared(dv);
+		error = wideDots ? VLengthSquaredWide(dv) : VLengthSquared(dv);
 		if (it == 0) {
 			error0 = error;
 			if (error <= eps * eps) break;

@@ -165,6 +165,19 @@ template <class T> T VLengthSquared(const DVec<T>& v) {
 	for (size_t i = 0; i < v.size(); ++i) r += LengthSquared(v[i]);
 	return r;
 }
+// The same two reductions with the accumulator widened to double (each term
+// still formed in T, the sum rounded to T once): the dot products of the
+// device's FP64 reduction mode, restated. Not a reference operation.
+template <class T> T VDotWide(const DVec<T>& a, const DVec<T>& b) {
+	double r = 0.0;
+	for (size_t i = 0; i < a.size(); ++i) r += double(Dot(a[i], b[i]));
+	return T(r);
+}
+template <class T> T VLengthSquaredWide(const DVec<T>& v) {
+	double r = 0.0;
+	for (size_t i = 0; i < v.size(); ++i) r += double(LengthSquared(v[i]));
+	return T(r);
+}
 
 } // namespace b3o
 

@@ -55,6 +55,24 @@ void* b3o_create(const b3s_scene* sc, int precision) {
 
 void b3o_destroy(void* p) { delete (Handle*)p; }
 
+/* A new handle holding a copy of the body and its whole dynamic state. */
+void* b3o_clone(void* p) {
+	const Handle* h = (const Handle*)p;
+	Handle* c = new Handle();
+	c->precision = h->precision;
+	if (h->f) c->f.reset(new Body<float>(*h->f));
+	if (h->d) c->d.reset(new Body<double>(*h->d));
+	c->tetCreationToList = h->tetCreationToList;
+	return c;
+}
+
+/* 1: the solve's dot products and Newton error accumulated in double (VDotWide); 0: as the reference. */
+void b3o_set_wide_dots(void* p, int wide) {
+	Handle* h = (Handle*)p;
+	if (h->f) h->f->wideDots = wide != 0;
+	if (h->d) h->d->wideDots = wide != 0;
+}
+
 void b3o_step(void* p, float dt, uint32_t it, uint32_t sub, float tol, float subTol) {
 	Handle* h = (Handle*)p;
 	if (h->precision == 64) h->d->Step(double(dt), it, sub, double(tol), double(subTol));
@@ -130,6 +148,13 @@ void b3o_get_stats(void* p, uint32_t* out3) {
 	out3[0] = s.newtonIterations;
 	out3[1] = s.minSub;
 	out3[2] = s.maxSub;
+}
+
+/* Restarts the running statistics (b3_forceSolver* globals) as a fresh process would hold them. */
+void b3o_reset_stats(void* p) {
+	Handle* h = (Handle*)p;
+	if (h->f) h->f->stats = StepStats();
+	if (h->d) h->d->stats = StepStats();
 }
 
 double b3o_get_energy(void* p) {

@@ -14,6 +14,12 @@
  *   - refh_step_tol: the b3Body::Step sequence (body.cpp:552-604) with the
  *     Newton/PCG tolerances exposed, since b3ForceSolver::Solve keeps the
  *     B3_EPSILON defaults (force_solver.cpp:142-155).
+ *
+ * The same file builds both the reference as shipped (scalar = float,
+ * _ref/libb3ref.so) and the reference with scalar = double
+ * (_ref/libb3ref64.so, settings_f64.h forced in): arrays crossing this
+ * boundary are float either way, converted to and from scalar, and
+ * refh_get_state64 returns the state unrounded.
  */
 #include <cmath>
 #include <cstdint>
@@ -358,7 +364,7 @@ void refh_step_tol(void* p, float dt, uint32_t iters, uint32_t subIters, float t
 		b3SolveBEInput in;
 		in.forceModel = &model;
 		in.h = dt;
-		in.inv_h = 1.0f / dt;
+		in.inv_h = scalar(1) / scalar(dt);
 		in.dofCount = n;
 		in.x0 = &x0; in.v0 = &v0; in.fe = &fe; in.M = &M; in.y = &y; in.S = &S; in.z = &z;
 		in.maxIterations = iters;
@@ -375,7 +381,7 @@ void refh_step_tol(void* p, float dt, uint32_t iters, uint32_t subIters, float t
 		b3_forceSolverMaxSubIterations = out.maxSubIterations;
 		for (uint32 i = 0; i < n; ++i) { l.ps[i]->m_position = x[i]; l.ps[i]->m_velocity = v[i]; }
 		b3TimeStep step;
-		step.dt = dt; step.inv_dt = 1.0f / dt; step.forceIterations = iters; step.forceSubIterations = subIters;
+		step.dt = dt; step.inv_dt = scalar(1) / scalar(dt); step.forceIterations = iters; step.forceSubIterations = subIters;
 		for (b3SphereAndShapeContact* c : l.cs) c->ApplyFriction(step, gravity);
 	}
 	for (b3Particle* q = body.m_particleList; q; q = q->m_next) { q->m_force.SetZero(); q->m_translation.SetZero(); }
@@ -397,6 +403,16 @@ double refh_step_timed(void* p, uint32_t steps, float dt, uint32_t it, uint32_t 
 }
 
 void refh_get_state(void* p, float* x, float* v) {
+	RefBody* rb = (RefBody*)p;
+	for (size_t c = 0; c < rb->particles.size(); ++c) {
+		b3Vec3 a = rb->particles[c]->GetPosition(), b = rb->particles[c]->GetVelocity();
+		if (x) { x[3 * c] = a.x; x[3 * c + 1] = a.y; x[3 * c + 2] = a.z; }
+		if (v) { v[3 * c] = b.x; v[3 * c + 1] = b.y; v[3 * c + 2] = b.z; }
+	}
+}
+
+/* The same in double: the FP64 build's state without rounding. */
+void refh_get_state64(void* p, double* x, double* v) {
 	RefBody* rb = (RefBody*)p;
 	for (size_t c = 0; c < rb->particles.size(); ++c) {
 		b3Vec3 a = rb->particles[c]->GetPosition(), b = rb->particles[c]->GetVelocity();
@@ -506,7 +522,7 @@ uint32_t refh_assemble(void* p, float dt) {
 	b3SparseMat33 dfdx(n);
 	b3SparseMat33 dfdv(n);
 	b3SparseForceSolverData sd;
-	sd.h = h; sd.inv_h = 1.0f / h; sd.x = &x; sd.v = &v; sd.f = &fi; sd.dfdx = &dfdx; sd.dfdv = &dfdv;
+	sd.h = h; sd.inv_h = scalar(1) / h; sd.x = &x; sd.v = &v; sd.f = &fi; sd.dfdx = &dfdx; sd.dfdv = &dfdv;
 	model.ApplyForces(&sd);
 	b3SparseMat33 A = M - h * dfdv - (h * h) * dfdx;
 	b3DenseVec3 b = M * (v0 - v) + h * (fe + fi) + h * dfdx * (x0 - x + h * v + y);
@@ -553,7 +569,8 @@ uint32_t refh_solve_cg(uint32_t n, const uint32_t* rowptr, const uint32_t* cols,
 		for (uint32_t e = rowptr[i + 1]; e-- > rowptr[i];) {   // insert tail first so the head ends first
 			b3RowEntry* en = (b3RowEntry*)b3Alloc(sizeof(b3RowEntry));
 			en->column = cols[e];
-			std::memcpy(&en->value.x.x, vals + 9 * e, 9 * sizeof(float));
+			scalar* m = &en->value.x.x;
+			for (int t = 0; t < 9; ++t) m[t] = vals[9 * e + t];
 			A.rows[i].Insert(en);
 		}
 	b3DenseVec3 bv(n), xv(n);

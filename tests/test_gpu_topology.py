@@ -133,3 +133,22 @@ def test_batch_rebuild_with_identity_maps_changes_nothing():
         xa, va = a.state()
         xb, vb = b.state()
         assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+
+
+def test_action_forces_cleared_by_zero_dt_step():
+    """b3Body::Step clears the action forces before its dt > 0 test (body.cpp:563-574): after a step with dt = 0
+    they read zero, as the oracle's do, and the next full step accumulates afresh."""
+    arr = configs.PARITY_SCENES["springs"]().arrays()
+    b = Body(arr)
+    b.set_reduction(_lib.REDUCE_SEQUENTIAL)
+    b.set_action_forces(True)
+    o = oracle.OracleBody(arr)
+    for dt in (configs.DT, configs.DT, 0.0, configs.DT):
+        b.Step(dt, 2, 20)
+        o.step(dt, 2, 20)
+        got = b.action_forces(FORCE_SPRING)
+        assert np.array_equal(got, o.action_forces(FORCE_SPRING, 2)), dt
+        if dt == 0.0:
+            assert not np.any(got)
+        else:
+            assert np.any(got)
