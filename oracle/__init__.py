@@ -21,6 +21,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libb3ref.so")
+REF64_SO = os.path.join(HERE, "_ref", "libb3ref64.so")   # the reference built with scalar = double
 
 u32p = C.POINTER(C.c_uint32)
 f32p = C.POINTER(C.c_float)
@@ -107,7 +108,7 @@ def build(reference: bool | None = None) -> None:
     """Builds liboracle.so (and _ref/ when the reference tree exists)."""
     targets = ["all"]
     if reference or (reference is None and os.path.isdir("/root/reference/src")):
-        targets.append("ref")
+        targets += ["ref", "ref64"]
     subprocess.check_call(["make", "-s", "-C", HERE, "-j8"] + targets)
 
 
@@ -122,7 +123,7 @@ _libs = {}
 
 def _lib(which: str):
     if which not in _libs:
-        raw = _load(ORACLE_SO if which == "oracle" else REF_SO)
+        raw = _load({"oracle": ORACLE_SO, "ref": REF_SO, "ref64": REF64_SO}[which])
 
         class _Named:   # CDLL.__getitem__ builds a new function object per call; attribute access caches
             def __getitem__(self, name):
@@ -184,12 +185,13 @@ def _lib(which: str):
             lib.refh_step.argtypes = [C.c_void_p, C.c_float, C.c_uint32, C.c_uint32]
             lib.refh_step_tol.argtypes = [C.c_void_p, C.c_float, C.c_uint32, C.c_uint32, C.c_float, C.c_float]
             lib.refh_reset_stats.argtypes = []
+            lib.refh_get_state64.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _libs[which] = lib
     return _libs[which]
 
 
-def reference_available() -> bool:
-    return os.path.exists(REF_SO)
+def reference_available(precision: int = 32) -> bool:
+    return os.path.exists(REF64_SO if precision == 64 else REF_SO)
 
 
 class _Body:
@@ -348,10 +350,19 @@ class OracleBody(_Body):
 
 
 class RefBody(_Body):
-    """The reference library itself (``fast`` = O(N) fixture set-up)."""
+    """The reference library itself (``fast`` = O(N) fixture set-up); ``RefBody64`` is the same harness over the
+    reference built with ``scalar = double``."""
 
     prefix = "refh_"
     which = "ref"
+
+    def state64(self):
+        """Positions and velocities in double (unrounded in the FP64 build)."""
+        x = np.zeros(3 * self.n, np.float64)
+        v = np.zeros(3 * self.n, np.float64)
+        self.lib.refh_get_state64(self.h, x.ctypes.data_as(C.POINTER(C.c_double)),
+                                  v.ctypes.data_as(C.POINTER(C.c_double)))
+        return x.reshape(-1, 3), v.reshape(-1, 3)
 
     def _create(self, fast=False):
         self.lib.refh_reset_stats()
@@ -383,6 +394,12 @@ class RefBody(_Body):
                 self.lib.refh_create_spring(self.h, e[1], e[2], C.c_float(e[3]), C.c_float(e[4]), C.c_float(e[5]))
             else:
                 raise ValueError(e[0])
+
+
+class RefBody64(RefBody):
+    """The reference library compiled with its scalar type set to double (oracle/Makefile ref64)."""
+
+    which = "ref64"
 
 
 def uv_initialize(which: str, A, B, C_):

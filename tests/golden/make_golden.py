@@ -17,11 +17,19 @@ closed meshes (with the meshes, cell sizes and box extensions).
 ``tearing.npz`` holds a topology-edit run of the reference (the spring cloth
 split at its most tensioned springs between steps, through Create / Destroy).
 
+``fp64.npz`` holds runs of the reference built with ``scalar = double``
+(``make -C oracle ref64``, ``oracle/_ref/libb3ref64.so``): per scene the
+schedule, per-step SHA-256 digests of the float64 positions and velocities,
+and the final float64 state and statistics; plus single steps of the
+stiffness-contrast tet grid from a moving state with a PCG cap of 20000 at
+tolerance 1e-10 (the FP64 steps the precision-form errors are stated
+against).
+
 ``ray_cast.npz`` holds ``b3Body::RayCastSingle`` results of the reference
 (creation index of the triangle fixture hit or -1, fraction, normal) for
 seeded rays at the state reached after 12 steps of four scenes.
 
-Usage: python tests/golden/make_golden.py [rays | sdf | tearing]   (needs oracle/_ref built; an argument: that fixture only)
+Usage: python tests/golden/make_golden.py [rays | sdf | tearing | fp64]   (needs oracle/_ref built; an argument: that fixture only)
 """
 
 import hashlib
@@ -179,6 +187,67 @@ def tearing_golden(pre: int = 12, post: int = 15):
     print(f"tearing: {len(edits)} edits, {len(p)} contacts at the end")
 
 
+def digest64(x, v) -> np.ndarray:
+    h = hashlib.sha256(np.ascontiguousarray(x, np.float64).tobytes() + np.ascontiguousarray(v, np.float64).tobytes())
+    return np.frombuffer(h.digest(), np.uint8)
+
+
+FP64_SCENES = ("tet_grid", "node_types", "drop_on_box", "cloth_sphere", "cloth_shapes", "cloth_element", "springs",
+               "mouse", "sdf", "sdf_inverted", "external", "grid_contact", "mesh_terrain")
+
+
+def fp64_schedule(name: str, steps: int = 12):
+    """Public b3Body::Step (B3_EPSILON = DBL_EPSILON: every PCG iteration runs) alternating with explicit
+    tolerances."""
+    out = []
+    for s in range(steps):
+        row = (configs.DT, 2, 20, 0.0, 0.0) if s % 2 else (configs.DT, 1, 40, 1e-8, 1e-10)
+        out.append(row + ((name == "external" and s % 3 == 0),))
+    return out
+
+
+def precision_scene(N: int = 8):
+    return configs.tet_grid(N, young_per_element=configs.log_uniform_young(N, seed=7)).arrays()
+
+
+def fp64_goldens():
+    """The reference with scalar = double on the parity scenes, and FP64 steps of the stiffness-contrast grid."""
+    rec = {}
+    for name in FP64_SCENES:
+        arrays = configs.PARITY_SCENES[name]().arrays()
+        body = oracle.RefBody64(arrays, fast=True)
+        n = len(arrays["p_type"])
+        sched = fp64_schedule(name)
+        dig = []
+        for dt, it, sub, tol, stol, ext in sched:
+            if ext:
+                body.apply_external(*external_loads(n))
+            body.step(dt, it, sub, tol, stol)
+            dig.append(digest64(*body.state64()))
+        x, v = body.state64()
+        rec.update({f"{name}/schedule": np.array([r[:5] for r in sched], np.float64),
+                    f"{name}/external": np.array([r[5] for r in sched], np.bool_), f"{name}/digests": np.stack(dig),
+                    f"{name}/x": x, f"{name}/v": v, f"{name}/stats": body.stats()})
+        print(f"fp64 {name}: stats {body.stats()}")
+    # the precision workload's FP64 step: from the rest state and from a moving state reached by the FP32
+    # reference (15 steps of 200 PCG iterations at 1e-6), one step with a PCG cap of 20000 at 1e-10
+    arrays = precision_scene()
+    f32 = oracle.RefBody(arrays, fast=True)
+    starts = [f32.state()]
+    for _ in range(15):
+        f32.step(configs.DT, 1, 200, 0.0, 1e-6)
+    starts.append(f32.state())
+    for k, (x0, v0) in enumerate(starts):
+        body = oracle.RefBody64(arrays, fast=True)
+        body.set_state(x0, v0)
+        body.step(configs.DT, 1, 20000, 0.0, 1e-10)
+        x, v = body.state64()
+        rec.update({f"precision{k}/x0": x0, f"precision{k}/v0": v0, f"precision{k}/x": x, f"precision{k}/v": v,
+                    f"precision{k}/stats": body.stats()})
+        print(f"fp64 precision step {k}: stats {body.stats()}")
+    np.savez_compressed(os.path.join(OUT, "fp64.npz"), **rec)
+
+
 def decode_edits(blob) -> list:
     import json
     out = []
@@ -188,8 +257,8 @@ def decode_edits(blob) -> list:
 
 
 def main():
-    if not oracle.reference_available():
-        sys.exit("oracle/_ref/libb3ref.so is missing: run `make -C oracle ref` where the reference tree exists")
+    if not oracle.reference_available() or not oracle.reference_available(64):
+        sys.exit("oracle/_ref/libb3ref*.so are missing: run `make -C oracle ref ref64` where the reference tree exists")
     if sys.argv[1:] == ["rays"]:
         ray_goldens()
         return
@@ -199,9 +268,13 @@ def main():
     if sys.argv[1:] == ["tearing"]:
         tearing_golden()
         return
+    if sys.argv[1:] == ["fp64"]:
+        fp64_goldens()
+        return
     ray_goldens()
     sdf_goldens()
     tearing_golden()
+    fp64_goldens()
     for name, fn in configs.PARITY_SCENES.items():
         rec = run(name, fn().arrays(), schedule(name, 30))
         np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **rec)

@@ -1,8 +1,10 @@
 """Precision form: FP64 iterative refinement around an FP32 / BF16 inner PCG.
 
 Errors are stated against the FP64 oracle (the reference restated with its
-scalar type set to double) stepping from the identical state with a tight
-PCG tolerance: relative max-norm error of the step's velocities. The scene
+scalar type set to double, pinned bit for bit to the reference compiled with
+scalar = double by tests/test_oracle_golden.py, including this file's FP64
+step of the stiffness-contrast grid) stepping from the identical state with a
+tight PCG tolerance: relative max-norm error of the step's velocities. The scene
 is the stiffness-contrast workload at small size: a tet grid whose elements
 have Young's moduli log-uniform in [1e4, 1e7] Pa.
 """
