@@ -49,7 +49,7 @@ def test_first_linearisation_bit_exact(name):
     assert np.array_equal(gpb, opb), f"pb max diff {np.abs(gpb - opb).max()}"
 
 
-FIXTURES = sorted(os.path.splitext(f)[0] for f in os.listdir(GOLDEN) if f.endswith(".npz") and f not in ("ray_cast.npz", "sdf_build.npz", "tearing.npz"))
+FIXTURES = sorted(os.path.splitext(f)[0] for f in os.listdir(GOLDEN) if f.endswith(".npz") and f not in ("ray_cast.npz", "sdf_build.npz", "tearing.npz", "fp64.npz"))
 
 
 def scene_arrays(name):
