@@ -124,6 +124,7 @@ EXPORTS = [
     "b3g_synchronize", "b3g_time_spmv", "b3g_set_solver", "b3g_set_graph", "b3g_get_system_size", "b3g_pattern",
     "b3g_solve_cg", "b3g_spmv", "b3g_set_reduction", "b3g_save_state", "b3g_load_state", "b3g_ray_cast", "b3g_ray_cast_device",
     "b3g_sdf_grid", "b3g_build_sdf", "b3g_body_rebuild", "b3g_set_action_forces", "b3g_get_action_forces",
+    "b3g_solve_cg_static",
 ]
 
 
@@ -242,6 +243,8 @@ def lib():
         "b3g_pattern": (C.c_int, [C.POINTER(BodyDesc), u32p, u32p, u32p, C.c_uint32]),
         "b3g_solve_cg": (C.c_int, [C.c_int, C.c_uint32, u32p, u32p, f32p, f32p, f32p, C.c_uint32, C.c_float, u32p]),
         "b3g_spmv": (C.c_int, [C.c_int, C.c_uint32, u32p, u32p, f32p, f32p, f32p]),
+        "b3g_solve_cg_static": (C.c_int, [C.c_int, vp, C.c_uint32, C.c_uint32, vp, vp, vp, vp, vp, C.c_uint32, C.c_float,
+                                          vp]),
         "b3g_set_reduction": (C.c_int, [vp, C.c_int]),
         "b3g_save_state": (C.c_int, [vp, C.c_void_p, C.POINTER(C.c_uint64)]),
         "b3g_load_state": (C.c_int, [vp, C.c_void_p, C.c_uint64]),
@@ -289,6 +292,21 @@ def solve_cg(rowptr, cols, vals, b, x0, max_iterations, tolerance, device=0):
     check(lib().b3g_solve_cg(device, n, _ptr(rowptr, u32p), _ptr(cols, u32p), _ptr(vals, f32p), _ptr(b, f32p),
                              _ptr(x, f32p), max_iterations, C.c_float(tolerance), C.byref(it)), "b3g_solve_cg")
     return x.reshape(-1, 3), it.value
+
+
+CG_STATIC_MAX_ROWS = 65536
+CG_STATIC_MAX_ITERATIONS = 4095
+CG_STATIC_INVALID = 0xFFFFFFFF
+
+
+def solve_cg_static(n, nnz, rowptr, cols, planes, b, x, max_iterations, tolerance, iterations=None, device=0,
+                    stream=None):
+    """b3g_solve_cg_static on device arrays (DeviceArray or raw addresses): the PCG with no allocation, its
+    scratch a fixed device array of the library. ``planes``: the blocks as nine planes (entry c of block k at
+    c * nnz + k). Asynchronous on ``stream`` (None: the null stream)."""
+    addr = lambda a: None if a is None else (a.address if isinstance(a, DeviceArray) else int(a))
+    check(lib().b3g_solve_cg_static(device, stream, n, nnz, addr(rowptr), addr(cols), addr(planes), addr(b), addr(x),
+                                    max_iterations, C.c_float(tolerance), addr(iterations)), "b3g_solve_cg_static")
 
 
 def spmv(rowptr, cols, vals, x, device=0):

@@ -428,6 +428,38 @@ b3g_status b3g_get_system_size(b3g_body body, uint32_t* rows, uint32_t* blocks);
 b3g_status b3g_solve_cg(int device, uint32_t n, const uint32_t* rowptr, const uint32_t* cols, const float* vals,
                         const float* b, float* x, uint32_t max_iterations, float tolerance, uint32_t* iterations);
 
+/*
+ * b3g_solve_cg with every array in device memory and no allocation: no
+ * counterpart in the reference (whose b3SparseSolveCG allocates its
+ * temporaries per call, sparse_solver.cpp:38-56). It is for callers that
+ * cannot allocate device memory -- code being captured into a HIP graph,
+ * tight serving loops, harnesses that forbid it. The solve is the same PCG
+ * on the same kernels (Jacobi preconditioner, FP64-reduced dot products,
+ * true residual every 50th iteration); its scratch is a fixed device array
+ * of the library (about 5.05 MB per GPU, held until the process ends), which
+ * limits it to B3G_CG_STATIC_MAX_ROWS rows and B3G_CG_STATIC_MAX_ITERATIONS
+ * iterations (larger requests: B3G_INVALID_ARGUMENT).
+ *   rowptr (n + 1) / cols (nnz): list-order CSR as for b3g_solve_cg;
+ *   vals: the blocks as nine planes, entry c (column-major) of block k at
+ *         vals[c * nnz + k] -- the layout the library keeps pA in;
+ *   b, x: 3 n floats; x holds the initial guess on entry, the solution after;
+ *   iterations (optional, device uint32): the iteration count, or
+ *         B3G_CG_STATIC_INVALID when rowptr / cols are malformed (not
+ *         non-decreasing from 0 to nnz, or a column >= n), in which case no
+ *         PCG kernel reads the matrix and x is left unchanged.
+ * Asynchronous on `stream` (a hipStream_t; NULL: the device's null stream);
+ * nothing is synchronised, allocated or copied to the host. Every call on a
+ * device uses that device's one scratch array, which keeps the last call's
+ * contents: calls that overlap on one device (other streams or threads)
+ * race and must be ordered by the caller.
+ */
+#define B3G_CG_STATIC_MAX_ROWS 65536u
+#define B3G_CG_STATIC_MAX_ITERATIONS 4095u
+#define B3G_CG_STATIC_INVALID 0xFFFFFFFFu
+b3g_status b3g_solve_cg_static(int device, void* stream, uint32_t n, uint32_t nnz, const uint32_t* rowptr,
+                               const uint32_t* cols, const float* vals, const float* b, float* x,
+                               uint32_t max_iterations, float tolerance, uint32_t* iterations);
+
 /* y = A x for a list-order CSR matrix (b3Mul, sparse/sparse_mat33.h:494-512), host arrays. */
 b3g_status b3g_spmv(int device, uint32_t n, const uint32_t* rowptr, const uint32_t* cols, const float* vals,
                     const float* x, float* y);
