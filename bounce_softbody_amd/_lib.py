@@ -19,10 +19,12 @@ f32p = C.POINTER(C.c_float)
 u8p = C.POINTER(C.c_uint8)
 
 B3G_OK = 0
+B3G_NON_POSITIVE_DIAGONAL, B3G_NON_FINITE = 7, 8
+FAULT_DIAGONAL, FAULT_NON_FINITE = 1, 2
 REDUCE_FP64, REDUCE_SEQUENTIAL, REDUCE_FP64_FUSED, REDUCE_FP64_SEPARATE = 0, 1, 2, 3
 SOLVER_PCG, SOLVER_REFINED_FP32, SOLVER_REFINED_BF16 = 0, 1, 2
 STATUS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "out of device memory", 4: "degenerate element",
-          5: "unsupported", 6: "body not built"}
+          5: "unsupported", 6: "body not built", 7: "non-positive Jacobi diagonal", 8: "non-finite state"}
 
 
 class B3gError(RuntimeError):
@@ -58,7 +60,7 @@ class BodyDesc(C.Structure):
 class StepStats(C.Structure):
     _fields_ = [("newton_iterations", C.c_uint32), ("min_sub_iterations", C.c_uint32),
                 ("max_sub_iterations", C.c_uint32), ("contact_count", C.c_uint32),
-                ("refinement_sweeps", C.c_uint32), ("relative_residual", C.c_float)]
+                ("refinement_sweeps", C.c_uint32), ("relative_residual", C.c_float), ("faults", C.c_uint32)]
 
 
 def _ptr(a, t):
@@ -124,7 +126,7 @@ EXPORTS = [
     "b3g_synchronize", "b3g_time_spmv", "b3g_set_solver", "b3g_set_graph", "b3g_get_system_size", "b3g_pattern",
     "b3g_solve_cg", "b3g_spmv", "b3g_set_reduction", "b3g_save_state", "b3g_load_state", "b3g_ray_cast", "b3g_ray_cast_device",
     "b3g_sdf_grid", "b3g_build_sdf", "b3g_body_rebuild", "b3g_set_action_forces", "b3g_get_action_forces",
-    "b3g_solve_cg_static",
+    "b3g_solve_cg_static", "b3g_time_spmv_cold", "b3g_set_early_exit",
 ]
 
 
@@ -237,6 +239,8 @@ def lib():
         "b3g_get_system": (C.c_int, [vp, u32p, u32p, f32p, f32p, f32p]),
         "b3g_synchronize": (C.c_int, [vp]),
         "b3g_time_spmv": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "b3g_time_spmv_cold": (C.c_int, [vp, C.c_uint32, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "b3g_set_early_exit": (C.c_int, [vp, C.c_int]),
         "b3g_set_solver": (C.c_int, [vp, C.c_int, C.c_uint32, C.c_float]),
         "b3g_set_graph": (C.c_int, [vp, C.c_int]),
         "b3g_get_system_size": (C.c_int, [vp, u32p, u32p]),

@@ -134,7 +134,15 @@ class Body:
     step = Step
 
     def synchronize(self):
+        """Waits for the body's work; raises B3gError (B3G_NON_FINITE / B3G_NON_POSITIVE_DIAGONAL) once for the
+        steps since the previous call that raised a fault."""
         check(self._L.b3g_synchronize(self._h), "b3g_synchronize")
+
+    def faults(self):
+        """B3G_FAULT_* bits of every instance (sticky until a rebuild or a loaded state)."""
+        s = (StepStats * self.n_inst)()
+        check(self._L.b3g_get_stats(self._h, s), "b3g_get_stats")
+        return [x.faults for x in s]
 
     # ------------------------------------------------------------ state (creation order)
     def state(self):
@@ -275,6 +283,19 @@ class Body:
         by = C.c_double(0)
         check(self._L.b3g_time_spmv(self._h, launches, C.byref(ms), C.byref(by)), "b3g_time_spmv")
         return ms.value, by.value
+
+    def time_spmv_cold(self, launches: int = 50, evict_bytes: int = 1 << 30):
+        """Like :meth:`time_spmv` with the caches evicted before every launch (a streaming read over
+        ``evict_bytes``), so the launch reads from HBM: mean ms per launch (including one launch gap), bytes."""
+        ms = C.c_double(0)
+        by = C.c_double(0)
+        check(self._L.b3g_time_spmv_cold(self._h, launches, evict_bytes, C.byref(ms), C.byref(by)), "b3g_time_spmv_cold")
+        return ms.value, by.value
+
+    def set_early_exit(self, enabled: bool):
+        """Stop enqueuing the PCG's iterations once every instance has converged (the default; identical
+        results). Off: the reference's fixed sequence of iterations is enqueued, converged ones return at once."""
+        check(self._L.b3g_set_early_exit(self._h, 1 if enabled else 0), "b3g_set_early_exit")
 
 
 __all__ = ["Body", "B3gError"]

@@ -41,6 +41,24 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+CONSUMER_SRC = os.path.join(ROOT, "tests", "native", "b3g_consumer.cpp")
+CONSUMER = os.path.join(ROOT, "tests", "native", "_build", "b3g_consumer")
+
+
+def build_consumer(verbose: bool = False) -> str:
+    """The C++ test program that calls libb3g through include/b3g.h only (host compiler, linked against the
+    library built above)."""
+    os.makedirs(os.path.dirname(CONSUMER), exist_ok=True)
+    cmd = [shutil.which("g++") or "g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), CONSUMER_SRC,
+           "-L", HERE, "-lb3g", "-Wl,-rpath,$ORIGIN/../../../bounce_softbody_amd", "-o", CONSUMER + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(CONSUMER + ".tmp", CONSUMER)
+    return CONSUMER
+
+
 if __name__ == "__main__":
     build(verbose="-v" in sys.argv)
+    build_consumer(verbose="-v" in sys.argv)
     print(LIB)

@@ -2,10 +2,16 @@
 #include "b3g_internal.h"
 
 #include <algorithm>
+#include <condition_variable>
+#include <deque>
+#include <functional>
 #include <map>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
+#include <shared_mutex>
+#include <thread>
 #include <tuple>
 
 struct b3g_body_s {
@@ -17,6 +23,95 @@ namespace b3g {
 static thread_local std::string g_error;
 void set_error(const std::string& msg) { g_error = msg; }
 
+// The thread that enqueues a body's eager steps. With the PCG's early exit a
+// step's launch sequence depends on the device (the host polls a flag between
+// chunks of iterations), so b3g_step queues the whole sequence for this
+// thread and returns at once, as an asynchronous step does. The thread takes
+// the steps in order, so a step's first kernels are enqueued right behind
+// the previous step's last ones and the stream does not run dry; up to
+// kMaxQueued steps wait. Every other entry point first waits until the
+// queued steps are fully enqueued (drain), so stream order is program order,
+// and reports the first failure among them.
+// Stream capture and the step threads exclude each other: a HIP call of a
+// step thread while another thread captures a stream can invalidate that
+// capture. Step threads hold this lock shared while they enqueue a step; a
+// capture holds it exclusively.
+static std::shared_mutex g_captureLock;
+
+struct StepWorker {
+	static constexpr size_t kMaxQueued = 4;
+	std::thread thread;
+	std::mutex m;
+	std::condition_variable cv;
+	std::deque<std::function<b3g_status()>> jobs;
+	bool running = false, stop = false;
+	b3g_status status = B3G_OK;
+	std::string error;
+
+	explicit StepWorker(int device) {
+		thread = std::thread([this, device]() {
+			(void)hipSetDevice(device);
+			std::unique_lock<std::mutex> lk(m);
+			for (;;) {
+				cv.wait(lk, [&] { return !jobs.empty() || stop; });
+				if (jobs.empty()) return;
+				std::function<b3g_status()> fn = std::move(jobs.front());
+				jobs.pop_front();
+				running = true;
+				const bool skip = status != B3G_OK;   // after a failure the queued steps are dropped
+				lk.unlock();
+				cv.notify_all();
+				b3g_status s = B3G_OK;
+				if (!skip) {
+					std::shared_lock<std::shared_mutex> cap(g_captureLock);
+					s = fn();
+				}
+				const std::string err = s != B3G_OK ? g_error : std::string();
+				lk.lock();
+				if (s != B3G_OK && status == B3G_OK) { status = s; error = err; }
+				running = false;
+				cv.notify_all();
+			}
+		});
+	}
+	// Waits until every queued step is enqueued; returns (and clears) the first failure since the last drain.
+	b3g_status drain() {
+		std::unique_lock<std::mutex> lk(m);
+		cv.wait(lk, [&] { return jobs.empty() && !running; });
+		const b3g_status s = status;
+		if (s != B3G_OK) set_error(error);
+		status = B3G_OK;
+		error.clear();
+		return s;
+	}
+	b3g_status post(std::function<b3g_status()> fn) {
+		std::unique_lock<std::mutex> lk(m);
+		if (status != B3G_OK) {   // a queued step failed: report it now
+			const b3g_status s = status;
+			cv.wait(lk, [&] { return jobs.empty() && !running; });
+			set_error(error);
+			status = B3G_OK;
+			error.clear();
+			return s;
+		}
+		cv.wait(lk, [&] { return jobs.size() < kMaxQueued; });
+		jobs.push_back(std::move(fn));
+		cv.notify_all();
+		return B3G_OK;
+	}
+	~StepWorker() {
+		{
+			std::unique_lock<std::mutex> lk(m);
+			cv.wait(lk, [&] { return jobs.empty() && !running; });
+			stop = true;
+			cv.notify_all();
+		}
+		thread.join();
+	}
+};
+
+static b3g_status drain(Body& b) { return b.worker ? b.worker->drain() : B3G_OK; }
+
 } // namespace b3g
 
 using namespace b3g;
@@ -24,6 +119,8 @@ using namespace b3g;
 #define CHECK_BODY(bd)                                                  \
 	do {                                                                \
 		if (!(bd)) { set_error("null body handle"); return B3G_INVALID_ARGUMENT; } \
+		b3g_status ds_ = drain((bd)->body);                             \
+		if (ds_ != B3G_OK) return ds_;                                  \
 	} while (0)
 #define CHECK_BUILT(bd)                                                 \
 	do {                                                                \
@@ -76,6 +173,8 @@ const char* b3g_status_string(b3g_status s) {
 	case B3G_DEGENERATE_ELEMENT: return "degenerate element";
 	case B3G_UNSUPPORTED: return "unsupported";
 	case B3G_NOT_COMMITTED: return "body not built";
+	case B3G_NON_POSITIVE_DIAGONAL: return "non-positive Jacobi diagonal";
+	case B3G_NON_FINITE: return "non-finite state";
 	}
 	return "unknown status";
 }
@@ -101,16 +200,20 @@ b3g_status b3g_body_create(b3g_body* out, int device) {
 	h->body.device = device;
 	hipError_t e = hipStreamCreateWithFlags(&h->body.stream, hipStreamNonBlocking);
 	if (e != hipSuccess) { set_error(hipGetErrorString(e)); delete h; return B3G_HIP_ERROR; }
+	h->body.earlyExit = true;
 	*out = h;
 	return B3G_OK;
 }
 
 b3g_status b3g_body_destroy(b3g_body h) {
 	if (!h) return B3G_OK;
+	delete h->body.worker;   // waits for the step in hand to be enqueued
+	h->body.worker = nullptr;
 	(void)hipSetDevice(h->body.device);
 	(void)hipStreamSynchronize(h->body.stream);
 	drop_graph(h->body);
 	dfree_all(h->body);
+	free_exit_flags(h->body);
 	(void)hipStreamDestroy(h->body.stream);
 	delete h;
 	return B3G_OK;
@@ -343,14 +446,32 @@ b3g_status b3g_ray_cast_device(b3g_body h, uint32_t instance, uint32_t count, co
 
 b3g_status b3g_step(b3g_body h, float dt, uint32_t iterations, uint32_t sub_iterations, float tolerance,
                     float sub_tolerance) {
-	CHECK_BUILT(h);
+	if (!h) { set_error("null body handle"); return B3G_INVALID_ARGUMENT; }
 	if (!std::isfinite(dt)) { set_error("dt must be finite"); return B3G_INVALID_ARGUMENT; }
 	float tol = tolerance > 0.0f ? tolerance : kEps;
 	float stol = sub_tolerance > 0.0f ? sub_tolerance : kEps;
 	if (!(stol < 1.0f)) { set_error("sub_tolerance must be in (0, 1)"); return B3G_INVALID_ARGUMENT; }   // sparse_solver.cpp:36
-	SET_DEVICE(h);
 	Body& b = h->body;
-	b3g_status s = prepare_step(b, sub_iterations);
+	b3g_status s;
+	if (b.earlyExit && !b.useGraph && b.built) {
+		// queued for the body's thread, which also makes the step's one-time preparations; steps already
+		// queued keep running meanwhile (no drain here)
+		B3G_HIP(hipSetDevice(b.device));
+		if (!b.worker) {
+			if ((s = ensure_exit_flags(b)) != B3G_OK) return s;
+			b.worker = new (std::nothrow) StepWorker(b.device);
+			if (!b.worker) return B3G_OUT_OF_MEMORY;
+		}
+		++b.xVersion;
+		return b.worker->post([&b, dt, iterations, sub_iterations, tol, stol]() {
+			b3g_status st = prepare_step(b, sub_iterations);
+			if (st != B3G_OK) return st;
+			return enqueue_step(b, dt, iterations, sub_iterations, tol, stol, false);
+		});
+	}
+	CHECK_BUILT(h);
+	SET_DEVICE(h);
+	s = prepare_step(b, sub_iterations);
 	if (s != B3G_OK) return s;
 	++b.xVersion;
 	if (!b.useGraph) return enqueue_step(b, dt, iterations, sub_iterations, tol, stol, false);
@@ -361,8 +482,11 @@ b3g_status b3g_step(b3g_body h, float dt, uint32_t iterations, uint32_t sub_iter
 	if (!b.graphExec || key != b.graphKey) {
 		drop_graph(b);
 		hipGraph_t graph = nullptr;
+		std::unique_lock<std::shared_mutex> cap(g_captureLock);   // no step thread runs during the capture
 		B3G_HIP(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
+		b.capturing = true;   // the captured sequence is the fixed one
 		s = enqueue_step(b, dt, iterations, sub_iterations, tol, stol, false);
+		b.capturing = false;
 		hipError_t e = hipStreamEndCapture(b.stream, &graph);
 		if (s != B3G_OK) { if (graph) (void)hipGraphDestroy(graph); return s; }
 		if (e != hipSuccess) { set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e)); return B3G_HIP_ERROR; }
@@ -463,6 +587,7 @@ b3g_status b3g_get_stats(b3g_body h, b3g_step_stats* out) {
 		out[k].contact_count = count[k];
 		out[k].refinement_sweeps = st[k].refineOn ? st[k].statSweeps : 0u;
 		out[k].relative_residual = st[k].refineOn && st[k].res0 > 0.0 ? (float)std::sqrt(st[k].res / st[k].res0) : 0.0f;
+		out[k].faults = st[k].faults;
 	}
 	return B3G_OK;
 }
@@ -606,7 +731,7 @@ struct StateHeader {
 	uint32_t magic, version, n, nTet, pairs, stampCounter, nInst, reserved;
 };
 constexpr uint32_t kStateMagic = 0x53473342u;   // "B3GS"
-constexpr uint32_t kStateVersion = 3;            // 3: batches (per-instance solver statistics)
+constexpr uint32_t kStateVersion = 4;            // 3: batches (per-instance solver statistics); 4: fault bits
 
 struct StatePart {
 	DBuf* buf;
@@ -687,8 +812,25 @@ b3g_status b3g_load_state(b3g_body h, const void* blob, uint64_t bytes) {
 b3g_status b3g_synchronize(b3g_body h) {
 	CHECK_BODY(h);
 	SET_DEVICE(h);
-	B3G_HIP(hipStreamSynchronize(h->body.stream));
-	return B3G_OK;
+	Body& b = h->body;
+	B3G_HIP(hipStreamSynchronize(b.stream));
+	if (!b.built || !b.faultWord.p) return B3G_OK;
+	uint32_t word = 0;
+	B3G_HIP(hipMemcpy(&word, b.faultWord.p, 4, hipMemcpyDeviceToHost));
+	if (!word) return B3G_OK;
+	const uint32_t zero = 0;
+	B3G_HIP(hipMemcpy(b.faultWord.p, &zero, 4, hipMemcpyHostToDevice));   // reported once
+	std::vector<SolverState> st(b.nInst);
+	B3G_HIP(hipMemcpy(st.data(), b.state.p, st.size() * sizeof(SolverState), hipMemcpyDeviceToHost));
+	const uint32_t bit = (word & B3G_FAULT_NON_FINITE) ? (uint32_t)B3G_FAULT_NON_FINITE : (uint32_t)B3G_FAULT_DIAGONAL;
+	uint32_t inst = 0;   // instances in creation order, as b3g_get_stats numbers them
+	while (inst + 1 < b.nInst && !(st[inst].faults & bit)) ++inst;
+	if (bit == B3G_FAULT_NON_FINITE) {
+		set_error("a step produced a non-finite position or velocity (instance " + std::to_string(inst) + ")");
+		return B3G_NON_FINITE;
+	}
+	set_error("a Jacobi diagonal entry of pA was not positive (instance " + std::to_string(inst) + ")");
+	return B3G_NON_POSITIVE_DIAGONAL;
 }
 
 b3g_status b3g_time_spmv(b3g_body h, uint32_t launches, double* mean_ms, double* bytes_per_launch) {
@@ -702,6 +844,27 @@ b3g_status b3g_time_spmv(b3g_body h, uint32_t launches, double* mean_ms, double*
 	const bool bf16 = b.solver == B3G_SOLVER_REFINED_BF16 && b.Ab.p;
 	if (bytes_per_launch)
 		*bytes_per_launch = (bf16 ? 22.0 * b.nnz + 36.0 * b.n : 40.0 * b.nnz) + 4.0 * (b.n + 1) + 24.0 * b.n;
+	return B3G_OK;
+}
+
+b3g_status b3g_time_spmv_cold(b3g_body h, uint32_t launches, uint64_t evict_bytes, double* mean_ms,
+                              double* bytes_per_launch) {
+	CHECK_BUILT(h);
+	if (!mean_ms || launches == 0) return B3G_INVALID_ARGUMENT;
+	if (evict_bytes < (512ull << 20)) { set_error("evict_bytes must be at least 512 MiB"); return B3G_INVALID_ARGUMENT; }
+	SET_DEVICE(h);
+	Body& b = h->body;
+	b3g_status s = time_pcg_spmv_cold(b, launches, (size_t)evict_bytes, mean_ms);
+	if (s != B3G_OK) return s;
+	const bool bf16 = b.solver == B3G_SOLVER_REFINED_BF16 && b.Ab.p;
+	if (bytes_per_launch)
+		*bytes_per_launch = (bf16 ? 22.0 * b.nnz + 36.0 * b.n : 40.0 * b.nnz) + 4.0 * (b.n + 1) + 24.0 * b.n;
+	return B3G_OK;
+}
+
+b3g_status b3g_set_early_exit(b3g_body h, int enabled) {
+	CHECK_BODY(h);
+	h->body.earlyExit = enabled != 0;
 	return B3G_OK;
 }
 

@@ -144,6 +144,8 @@ struct SolverState {
 	double res0, res;
 	float refineTol2;
 	uint32_t refineOn, refineActive, sweeps, maxSweeps, itSum, statSweeps;
+	// B3G_FAULT_* bits (sticky): a non-positive Jacobi diagonal, a non-finite state
+	uint32_t faults;
 };
 
 
@@ -241,6 +243,7 @@ struct Body {
 	DBuf terms;      // float per row: per-row dot terms (sequential reduction mode)
 	int reduction = 0;
 	DBuf state;      // SolverState per instance
+	DBuf faultWord;  // uint32: B3G_FAULT_* bits raised by any instance since the last b3g_synchronize
 
 	// ---------------------------------------------------------------- contacts (sphere x world pairs of one instance)
 	DBuf worlds;     // WorldShape[nWorld]
@@ -285,6 +288,21 @@ struct Body {
 	DBuf act;
 	uint32_t actSpring = 0, actStretch = 0, actShear = 0, actMouse = 0, actCount = 0;
 
+	// ---------------------------------------------------------------- early exit of the PCG
+	// With earlyExit the PCG's iterations are enqueued in chunks; after each
+	// chunk a one-workgroup kernel writes into pinned host memory whether any
+	// instance's last iteration ran, and the host stops enqueuing once none
+	// did (the reference's loop exit, sparse_solver.cpp:63-73). The host waits
+	// on the chunk before the last one enqueued, so the device never idles.
+	// Off while a step is captured into a graph (a fixed launch sequence).
+	bool earlyExit = false;
+	bool capturing = false;
+	uint32_t* hostFlag = nullptr;         // hipHostMalloc'd: per slot, (sequence << 1) | any active
+	hipEvent_t flagEvent[2] = {nullptr, nullptr};
+	uint32_t flagSeq = 0;
+	uint32_t lastConverged = 0;           // largest iteration count of the previous solve (first-chunk size)
+	struct StepWorker* worker = nullptr;  // the thread that enqueues eager steps (b3g_api.hip)
+
 	// ---------------------------------------------------------------- diagnostics
 	bool captureFi = false;
 
@@ -323,9 +341,13 @@ b3g_status enqueue_update_contacts(Body& b);
 b3g_status enqueue_find_contacts(Body& b);
 b3g_status enqueue_spmv(Body& b, const DBuf& xin, DBuf& yout);
 b3g_status time_pcg_spmv(Body& b, uint32_t launches, double* mean_ms);
+b3g_status time_pcg_spmv_cold(Body& b, uint32_t launches, size_t evictBytes, double* mean_ms);
 b3g_status enqueue_solve_cg(Body& b, uint32_t maxIt, float eps);
 b3g_status build_tiles(Body& b);
 b3g_status ensure_partials(Body& b, uint32_t blocks);
 b3g_status ensure_cg_log(Body& b, uint32_t iterations);
+// Pinned flag memory and events of the PCG's early exit (allocated once, before any capture).
+b3g_status ensure_exit_flags(Body& b);
+void free_exit_flags(Body& b);
 
 } // namespace b3g

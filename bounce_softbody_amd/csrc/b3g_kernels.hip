@@ -10,6 +10,7 @@
 #include "b3g_tet.h"
 
 #include <algorithm>
+#include <cstring>
 
 namespace b3g {
 
@@ -285,6 +286,7 @@ struct StepArgs {
 	V3 *x, *v, *force, *translation, *x0, *v0, *fe, *y, *w, *px;
 	SolverState* st;            // per instance
 	const uint32_t* range;
+	uint32_t* faultWord;        // fault bits of any instance
 };
 
 // force_solver.cpp:100-130 + sparse_force_solver.cpp:135-139
@@ -994,32 +996,35 @@ __global__ void __launch_bounds__(kBlock) k_contact_forces(ContactArgs a) {
 
 // Contacts of one sphere in the reference's list order (newest first), as
 // positions within the sphere's pairs.
-__device__ int contact_order(const ContactArgs& a, uint32_t s, uint32_t* order) {
+// A sphere's contacts in list order are its active pairs by descending stamp
+// (stamps of one sphere are distinct: the call number times the sphere's pair
+// count plus the world position). next_contact returns the active pair with
+// the largest stamp below `below`, or kNoContact: walking from kNoContact
+// visits the list head to tail with no per-thread array, for any number of
+// world fixtures.
+constexpr uint32_t kNoContact = 0xFFFFFFFFu;
+__device__ __forceinline__ uint32_t next_contact(const ContactArgs& a, uint32_t s, uint32_t below) {
 	const uint32_t p0 = a.pairStart[s], p1 = a.pairStart[s + 1];
-	int cnt = 0;
-	for (uint32_t k = p0; k < p1; ++k)
-		if (a.active[k]) order[cnt++] = k - p0;
-	for (int i = 1; i < cnt; ++i) {   // insertion sort by stamp, descending
-		uint32_t key = order[i];
-		int j = i - 1;
-		while (j >= 0 && a.stamp[p0 + order[j]] < a.stamp[p0 + key]) { order[j + 1] = order[j]; --j; }
-		order[j + 1] = key;
+	uint32_t best = kNoContact, bestStamp = 0u;
+	for (uint32_t k = p0; k < p1; ++k) {
+		if (!a.active[k]) continue;
+		const uint32_t st = a.stamp[k];
+		if (st < below && (best == kNoContact || st > bestStamp)) { best = k; bestStamp = st; }
 	}
-	return cnt;
+	return best;
 }
+#define FOR_CONTACTS(a, s, k) \
+	for (uint32_t k = next_contact((a), (s), kNoContact); k != kNoContact; k = next_contact((a), (s), (a).stamp[k]))
 
 // sphere_shape_contact.cpp:145-189, contacts of a particle in list order
 __global__ void __launch_bounds__(kBlock) k_friction(ContactArgs a) {
 	uint32_t s = blockIdx.x * kBlock + threadIdx.x;
 	if (s >= a.nSpheres) return;
-	uint32_t order[16];
-	int cnt = contact_order(a, s, order);
-	if (cnt == 0) return;
+	if (next_contact(a, s, kNoContact) == kNoContact) return;
 	uint32_t p = a.sphereParticle[s];
 	V3 v1 = a.v[p];
 	float im1 = a.invMass[p];
-	for (int c = 0; c < cnt; ++c) {
-		uint32_t k = a.pairStart[s] + order[c];
+	FOR_CONTACTS(a, s, k) {
 		if (!a.applyFriction[k]) continue;
 		V3 normal = a.manifold[2 * k + 1];
 		float normalForce = a.normalForce[k];
@@ -1108,6 +1113,8 @@ struct AsmArgs {
 	const V3* force;
 	const V3* gravity;
 	const uint32_t* rowInst;
+	SolverState* stw;             // the same states, written: fault bits
+	uint32_t* faultWord;          // fault bits of any instance
 };
 
 // acc += the contributions [ptr[s], ptr[s + 1]) of slot s, in order, for the
@@ -1322,11 +1329,8 @@ __global__ void __launch_bounds__(kBlock) k_assemble_slots(AsmArgs a) {
 	if (!own) return;
 	const bool dyn_i = type_i == kDynamic;
 	if (i == j && sph_i >= 0 && dyn_i) {
-		uint32_t order[16];
-		uint32_t sph = (uint32_t)sph_i;
-		int cnt = contact_order(a.c, sph, order);
-		for (int c = 0; c < cnt; ++c) {
-			uint32_t k = a.c.pairStart[sph] + order[c];
+		const uint32_t sph = (uint32_t)sph_i;
+		FOR_CONTACTS(a.c, sph, k) {
 			if (a.c.hit[k]) dx = dx + a.c.cK[k];
 		}
 	}
@@ -1400,11 +1404,8 @@ __global__ void __launch_bounds__(kBlock) k_assemble_rows64(AsmArgs a) {
 	if (!own) return;
 	const bool dyn = a.type[i] == kDynamic;
 	if (a.hasContacts && a.sphereOf[i] >= 0 && dyn) {
-		uint32_t order[16];
-		uint32_t sph = (uint32_t)a.sphereOf[i];
-		int cnt = contact_order(a.c, sph, order);
-		for (int c = 0; c < cnt; ++c) {
-			uint32_t k = a.c.pairStart[sph] + order[c];
+		const uint32_t sph = (uint32_t)a.sphereOf[i];
+		FOR_CONTACTS(a.c, sph, k) {
 			if (a.c.hit[k]) fi = fi + d3(a.c.cF[k]);
 		}
 	}
@@ -1440,11 +1441,8 @@ __global__ void __launch_bounds__(kBlock) k_assemble_rows(AsmArgs a) {
 	if (!own) return;
 	const bool dyn = a.type[i] == kDynamic;
 	if (a.hasContacts && a.sphereOf[i] >= 0 && dyn) {
-		uint32_t order[16];
-		uint32_t sph = (uint32_t)a.sphereOf[i];
-		int cnt = contact_order(a.c, sph, order);
-		for (int c = 0; c < cnt; ++c) {
-			uint32_t k = a.c.pairStart[sph] + order[c];
+		const uint32_t sph = (uint32_t)a.sphereOf[i];
+		FOR_CONTACTS(a.c, sph, k) {
 			if (a.c.hit[k]) fi = fi + a.c.cF[k];
 		}
 	}
@@ -1457,10 +1455,15 @@ __global__ void __launch_bounds__(kBlock) k_assemble_rows(AsmArgs a) {
 	// pb = S (b - A z), z = 0
 	M33 S = dyn ? mdiag(1.0f) : mzero();
 	a.pb[i] = S * b;
-	// Jacobi preconditioner from pA(i, i) (sparse_solver.cpp:40-55)
+	// Jacobi preconditioner from pA(i, i) (sparse_solver.cpp:40-55); the
+	// reference asserts each diagonal entry > 0, here the fault is recorded
 	uint32_t ds = a.diagSlot[i];
 	float axx = a.A[ds], ayy = a.A[(size_t)4 * a.stride + ds], azz = a.A[(size_t)8 * a.stride + ds];
 	a.invM[i] = v3(1.0f / axx, 1.0f / ayy, 1.0f / azz);
+	if (!(axx > 0.0f) || !(ayy > 0.0f) || !(azz > 0.0f)) {
+		atomicOr(&a.stw[a.rowInst[i]].faults, (uint32_t)B3G_FAULT_DIAGONAL);
+		atomicOr(a.faultWord, (uint32_t)B3G_FAULT_DIAGONAL);
+	}
 }
 
 // ---------------------------------------------------------------------------
@@ -1679,12 +1682,12 @@ __device__ __forceinline__ float cg_alpha(const CGArgs& a, uint32_t inst, uint32
 __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k) {
 	const uint32_t blk = blockIdx.x, inst = a.blkInst[blk];
 	if (!a.st[inst].newtonActive || !a.log[(size_t)inst * a.logStride + k].active) return;
-	const float alpha = cg_alpha(a, inst, blk, k);
 	const uint32_t r0 = a.rowBlk[blk], r1 = a.rowBlk[blk + 1];
 	const bool seq = a.st[inst].seq != 0;
 	// every input of the thread's rows is loaded before the first store (the
-	// stores could alias them as far as the compiler knows); a row past the
-	// block loads row r0 and stores nothing
+	// stores could alias them as far as the compiler knows), and before the
+	// alpha prologue, whose partial sums then overlap these loads; a row past
+	// the block loads row r0 and stores nothing
 	V3 d[kRowsPerThread], x[kRowsPerThread], r[kRowsPerThread], q[kRowsPerThread], im[kRowsPerThread];
 #pragma unroll
 	for (int u = 0; u < kRowsPerThread; ++u) {
@@ -1695,6 +1698,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k) {
 		q[u] = a.q[ii];
 		im[u] = a.invM[ii];
 	}
+	const float alpha = cg_alpha(a, inst, blk, k);
 	double part = 0.0;
 #pragma unroll
 	for (int u = 0; u < kRowsPerThread; ++u) {
@@ -1716,7 +1720,6 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k) {
 __global__ void __launch_bounds__(kBlock) k_cg_update_x(CGArgs a, uint32_t k) {
 	const uint32_t blk = blockIdx.x, inst = a.blkInst[blk];
 	if (!a.st[inst].newtonActive || !a.log[(size_t)inst * a.logStride + k].active) return;
-	const float alpha = cg_alpha(a, inst, blk, k);
 	const uint32_t r0 = a.rowBlk[blk], r1 = a.rowBlk[blk + 1];
 	V3 d[kRowsPerThread], x[kRowsPerThread];
 #pragma unroll
@@ -1725,6 +1728,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update_x(CGArgs a, uint32_t k) {
 		d[u] = a.dnew[ii];
 		x[u] = a.px[ii];
 	}
+	const float alpha = cg_alpha(a, inst, blk, k);
 #pragma unroll
 	for (int u = 0; u < kRowsPerThread; ++u) {
 		const uint32_t i = r0 + threadIdx.x + u * kBlock;
@@ -1746,8 +1750,13 @@ __global__ void __launch_bounds__(kBlock) k_newton_update(StepArgs s, double* pa
 		if (i >= r1) break;
 		V3 dv = s.px[i] + zero3();
 		V3 v = s.v[i] + dv;
+		const V3 x = (s.x0[i] + s.h * v) + s.y[i];
 		s.v[i] = v;
-		s.x[i] = (s.x0[i] + s.h * v) + s.y[i];
+		s.x[i] = x;
+		if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) && isfinite(v.y) && isfinite(v.z))) {
+			atomicOr(&s.st[inst].faults, (uint32_t)B3G_FAULT_NON_FINITE);
+			atomicOr(s.faultWord, (uint32_t)B3G_FAULT_NON_FINITE);
+		}
 		const float term = length_sq(dv);
 		if (seq) terms[i] = term;
 		part += (double)term;
@@ -2064,6 +2073,7 @@ static StepArgs step_args(Body& b, float h) {
 	a.px = dptr<V3>(b.px);
 	a.st = dptr<SolverState>(b.state);
 	a.range = dptr<uint32_t>(b.instRange);
+	a.faultWord = dptr<uint32_t>(b.faultWord);
 	return a;
 }
 
@@ -2242,6 +2252,8 @@ static b3g_status enqueue_linearise(Body& b, float h) {
 	a.A = dptr<float>(b.A); a.tB = dptr<V3>(b.tB); a.pb = dptr<V3>(b.pb); a.invM = dptr<V3>(b.invM); a.fi = dptr<V3>(b.fi);
 	a.c = c;
 	a.st = st;
+	a.stw = dptr<SolverState>(b.state);
+	a.faultWord = dptr<uint32_t>(b.faultWord);
 	a.y = dptr<V3>(b.y);
 	a.tB64 = precise ? dptr<double>(b.tB64) : nullptr;
 	a.fstage64 = dptr<D3>(b.fstage64);
@@ -2277,31 +2289,146 @@ static b3g_status enqueue_finalize(Body& b, const DBuf& partial, int sel, int ki
 // blocks) otherwise.
 static inline int delta_range(uint32_t k) { return k % 50 == 0 ? kRangeTiles : kRangeBlocks; }
 
+// Early exit: whether any instance ran PCG iteration k (its log entry is
+// active and its Newton loop is), and the largest iteration count so far,
+// written with the chunk's sequence number into pinned host memory.
+__global__ void __launch_bounds__(kBlock) k_pcg_flag(const SolverState* st, const CGLog* log, uint32_t stride,
+                                                     uint32_t nInst, uint32_t k, uint32_t seq, uint32_t* out) {
+	uint32_t any = 0u, its = 0u;
+	for (uint32_t i = threadIdx.x; i < nInst; i += kBlock) {
+		if (st[i].newtonActive && log[(size_t)i * stride + k].active) any = 1u;
+		its = max(its, st[i].it);
+	}
+	__shared__ uint32_t shAny[kBlock], shIts[kBlock];
+	shAny[threadIdx.x] = any;
+	shIts[threadIdx.x] = its;
+	__syncthreads();
+	for (int o = kBlock / 2; o > 0; o >>= 1) {
+		if ((int)threadIdx.x < o) {
+			shAny[threadIdx.x] |= shAny[threadIdx.x + o];
+			shIts[threadIdx.x] = max(shIts[threadIdx.x], shIts[threadIdx.x + o]);
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		out[1] = shIts[0];
+		__threadfence_system();
+		out[0] = (seq << 1) | shAny[0];
+		__threadfence_system();
+	}
+}
+
+b3g_status ensure_exit_flags(Body& b) {
+	if (b.hostFlag) return B3G_OK;
+	void* p = nullptr;
+	B3G_HIP(hipHostMalloc(&p, 512, hipHostMallocDefault));   // two slots 256 bytes apart
+	std::memset(p, 0, 512);
+	b.hostFlag = static_cast<uint32_t*>(p);
+	for (hipEvent_t& e : b.flagEvent) B3G_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+	return B3G_OK;
+}
+
+void free_exit_flags(Body& b) {
+	for (hipEvent_t& e : b.flagEvent) {
+		if (e) (void)hipEventDestroy(e);
+		e = nullptr;
+	}
+	if (b.hostFlag) (void)hipHostFree(b.hostFlag);
+	b.hostFlag = nullptr;
+}
+
+// Waits for the flag of chunk `seq` in slot `slot`: spins on the pinned word
+// (its sequence number says the kernel has written it) and checks the event
+// every few thousand reads, so that a failed stream ends the wait.
+static b3g_status wait_flag(Body& b, uint32_t slot, uint32_t seq, bool* any, uint32_t* its) {
+	volatile uint32_t* f = b.hostFlag + 64 * slot;
+	for (uint32_t spin = 0;; ++spin) {
+		const uint32_t v = f[0];
+		if ((v >> 1) == (seq & 0x7FFFFFFFu)) {
+			*any = (v & 1u) != 0u;
+			*its = f[1];
+			return B3G_OK;
+		}
+		if ((spin & 4095u) == 4095u) {
+			const hipError_t e = hipEventQuery(b.flagEvent[slot]);
+			if (e == hipSuccess) {   // complete: the word is written
+				const uint32_t w = f[0];
+				if ((w >> 1) != (seq & 0x7FFFFFFFu)) {
+					set_error("PCG early-exit flag was not written");
+					return B3G_HIP_ERROR;
+				}
+				*any = (w & 1u) != 0u;
+				*its = f[1];
+				return B3G_OK;
+			}
+			if (e != hipErrorNotReady) {
+				set_error(std::string("PCG early-exit wait: ") + hipGetErrorString(e));
+				return B3G_HIP_ERROR;
+			}
+		}
+	}
+}
+
+constexpr uint32_t kExitChunk = 4;   // PCG iterations per chunk after the first
+
 // Iterations 0 .. maxIt - 1 of the PCG (sparse_solver.cpp:75-103) after its
 // Init reduction, then the reduction that ends the last one: the reference's
 // fixed launch sequence; iterations after convergence return at once on their
-// log entry.
+// log entry. With early exit, the sequence is enqueued in chunks and stops
+// once a chunk's last iteration ran in no instance: everything after it
+// would have returned at once, so the results are the same.
 static b3g_status enqueue_pcg_iterations(Body& b, const CGArgs& a, uint32_t maxIt) {
 	const bool fused = a.fused != 0;
 	const uint32_t rb = b.nRowBlk;
-	for (uint32_t k = 0; k < maxIt; ++k) {
-		CGArgs ak = a;
-		V3* d0 = dptr<V3>(b.d0);
-		V3* d1 = dptr<V3>(b.d1);
-		ak.dold = (k & 1) ? d1 : d0;
-		ak.dnew = (k & 1) ? d0 : d1;
-		const int sel = k > 0 ? delta_range(k - 1) : kRangeTiles;
-		if (!fused && k > 0) FINALIZE(b.partial, sel, kFinDelta, k, kGateIter, k - 1);
-		LAUNCH(k_cg_spmv_dq, b.nTiles, ak, k, sel, 0);
-		if (!fused) FINALIZE(b.partialQ, kRangeTiles, kFinAlpha, k, kGateIter, k);
-		if (k % 50 == 0) {
-			LAUNCH(k_cg_update_x, rb, ak, k);
-			LAUNCH(k_cg_residual, b.nTiles, ak, (int)kFinDelta, k);
-		} else {
-			LAUNCH(k_cg_update, rb, ak, k);
+	const bool poll = b.earlyExit && !b.capturing && b.hostFlag;
+	struct Pending { uint32_t slot, seq; };
+	Pending pend[2];
+	uint32_t npend = 0;
+	// first chunk: a little short of the previous solve's count, so its check lands near convergence
+	uint32_t chunk = poll ? std::max(kExitChunk, b.lastConverged > 2 ? b.lastConverged - 2 : 0u) : maxIt;
+	uint32_t k = 0, its = 0;
+	bool stopped = false;
+	while (k < maxIt && !stopped) {
+		const uint32_t e = std::min(maxIt, k + chunk);
+		for (; k < e; ++k) {
+			CGArgs ak = a;
+			V3* d0 = dptr<V3>(b.d0);
+			V3* d1 = dptr<V3>(b.d1);
+			ak.dold = (k & 1) ? d1 : d0;
+			ak.dnew = (k & 1) ? d0 : d1;
+			const int sel = k > 0 ? delta_range(k - 1) : kRangeTiles;
+			if (!fused && k > 0) FINALIZE(b.partial, sel, kFinDelta, k, kGateIter, k - 1);
+			LAUNCH(k_cg_spmv_dq, b.nTiles, ak, k, sel, 0);
+			if (!fused) FINALIZE(b.partialQ, kRangeTiles, kFinAlpha, k, kGateIter, k);
+			if (k % 50 == 0) {
+				LAUNCH(k_cg_update_x, rb, ak, k);
+				LAUNCH(k_cg_residual, b.nTiles, ak, (int)kFinDelta, k);
+			} else {
+				LAUNCH(k_cg_update, rb, ak, k);
+			}
+		}
+		chunk = kExitChunk;
+		if (!poll || k == maxIt) continue;
+		b.flagSeq = (b.flagSeq + 1u) & 0x7FFFFFFFu;
+		if (b.flagSeq == 0u) b.flagSeq = 1u;   // the zeroed words never match a chunk
+		const uint32_t seq = b.flagSeq, slot = seq & 1u;
+		hipLaunchKernelGGL(k_pcg_flag, dim3(1), dim3(kBlock), 0, b.stream, a.st, a.log, a.logStride, std::max(1u, b.nInst),
+		                   k - 1, seq, b.hostFlag + 64 * slot);
+		B3G_HIP(hipGetLastError());
+		B3G_HIP(hipEventRecord(b.flagEvent[slot], b.stream));
+		pend[npend++] = {slot, seq};
+		if (npend == 2) {   // the older chunk's flag, while the newer one runs
+			bool any = true;
+			b3g_status s = wait_flag(b, pend[0].slot, pend[0].seq, &any, &its);
+			if (s != B3G_OK) return s;
+			pend[0] = pend[1];
+			npend = 1;
+			stopped = !any;
 		}
 	}
-	if (maxIt > 0) FINALIZE(b.partial, delta_range(maxIt - 1), kFinDelta, maxIt, kGateIter, maxIt - 1);
+	// the next solve's first chunk: this one's count, or the cap when the sequence ran to the end
+	if (poll) b.lastConverged = stopped ? its : maxIt;
+	if (maxIt > 0 && !stopped) FINALIZE(b.partial, delta_range(maxIt - 1), kFinDelta, maxIt, kGateIter, maxIt - 1);
 	return B3G_OK;
 }
 
@@ -2441,6 +2568,80 @@ b3g_status time_pcg_spmv(Body& b, uint32_t launches, double* mean_ms) {
 		return st;
 	}
 	*mean_ms = launches ? (double)ms / launches : 0.0;
+	return B3G_OK;
+}
+
+// Streaming reads over a buffer larger than the L2s and the MALL together:
+// afterwards a kernel's loads of anything else come from HBM. Reads, not
+// writes: a write stream would leave the memory-side cache full of dirty
+// lines, whose write-back would then run during the measured kernel.
+__global__ void __launch_bounds__(kBlock) k_evict(const uint4* p, size_t count, uint32_t v, uint32_t* sink) {
+	uint32_t acc = 0u;
+	for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += (size_t)gridDim.x * kBlock) {
+		const uint4 w = p[i];
+		acc ^= w.x ^ w.y ^ w.z ^ w.w;
+	}
+	if (acc == v + 0x9E3779B9u) sink[threadIdx.x] = acc;   // keeps the loads; the buffer is zero, so never taken
+}
+
+// The SpMV pass from HBM: `launches` x (eviction, SpMV) between one event
+// pair, then `launches` evictions alone between another; the difference per
+// launch is the SpMV with one launch gap.
+b3g_status time_pcg_spmv_cold(Body& b, uint32_t launches, size_t evictBytes, double* mean_ms) {
+	b3g_status s0 = ensure_cg_log(b, 2);
+	if (s0 != B3G_OK) return s0;
+	void* buf = nullptr;
+	if (hipMalloc(&buf, evictBytes) != hipSuccess) {
+		(void)hipGetLastError();
+		set_error("no device memory for the eviction buffer");
+		return B3G_OUT_OF_MEMORY;
+	}
+	const size_t count = (evictBytes - 4096) / sizeof(uint4);
+	uint32_t* sink = reinterpret_cast<uint32_t*>(static_cast<char*>(buf) + count * sizeof(uint4));
+	if (hipMemsetAsync(buf, 0, evictBytes, b.stream) != hipSuccess) {
+		(void)hipGetLastError();
+		(void)hipFree(buf);
+		set_error("clearing the eviction buffer failed");
+		return B3G_HIP_ERROR;
+	}
+	CGArgs a = cg_args(b);
+	hipEvent_t e[4];
+	for (hipEvent_t& x : e) x = nullptr;
+	b3g_status st = B3G_OK;
+	float both = 0.0f, alone = 0.0f;
+	do {
+		bool ok = true;
+		for (hipEvent_t& x : e) ok = ok && hipEventCreate(&x) == hipSuccess;
+		if (!ok) { st = B3G_HIP_ERROR; break; }
+		// the first eviction kernel and SpMV launch warm up the code objects
+		hipLaunchKernelGGL(k_evict, dim3(4096), dim3(kBlock), 0, b.stream, (const uint4*)buf, count, 0u, sink);
+		ok = hipEventRecord(e[0], b.stream) == hipSuccess;
+		for (uint32_t k = 0; k < launches && ok; ++k) {
+			hipLaunchKernelGGL(k_evict, dim3(4096), dim3(kBlock), 0, b.stream, (const uint4*)buf, count, k, sink);
+			CGArgs ak = a;
+			ak.dold = (k & 1) ? dptr<V3>(b.d1) : dptr<V3>(b.d0);
+			ak.dnew = (k & 1) ? dptr<V3>(b.d0) : dptr<V3>(b.d1);
+			hipLaunchKernelGGL(k_cg_spmv_dq, dim3(b.nTiles), dim3(kBlock), 0, b.stream, ak, 1u, (int)kRangeBlocks, 1);
+			ok = hipGetLastError() == hipSuccess;
+		}
+		ok = ok && hipEventRecord(e[1], b.stream) == hipSuccess && hipEventRecord(e[2], b.stream) == hipSuccess;
+		for (uint32_t k = 0; k < launches && ok; ++k) {
+			hipLaunchKernelGGL(k_evict, dim3(4096), dim3(kBlock), 0, b.stream, (const uint4*)buf, count, k, sink);
+			ok = hipGetLastError() == hipSuccess;
+		}
+		ok = ok && hipEventRecord(e[3], b.stream) == hipSuccess && hipEventSynchronize(e[3]) == hipSuccess &&
+		     hipEventElapsedTime(&both, e[0], e[1]) == hipSuccess && hipEventElapsedTime(&alone, e[2], e[3]) == hipSuccess;
+		if (!ok) st = B3G_HIP_ERROR;
+	} while (0);
+	for (hipEvent_t& x : e)
+		if (x) (void)hipEventDestroy(x);
+	(void)hipStreamSynchronize(b.stream);
+	(void)hipFree(buf);
+	if (st != B3G_OK) {
+		set_error("cold SpMV timing pass failed");
+		return st;
+	}
+	*mean_ms = ((double)both - (double)alone) / launches;
 	return B3G_OK;
 }
 

@@ -528,7 +528,14 @@ b3g_status validate(const b3g_body_desc& d) {
 			if (q[7] < 2.0f || q[8] < 2.0f || q[9] < 2.0f) return bad("sdf voxel grid needs at least 2 voxels per axis");
 		}
 	}
-	if (d.n_world > 16) { set_error("at most 16 world fixtures are supported"); return B3G_UNSUPPORTED; }
+	// A contact's list position is its stamp, FindNewContacts call x world fixtures + world (32 bits): at 4096
+	// world fixtures it stays exact for 2^20 steps. Every sphere pairs with every world fixture
+	// (contact_manager.cpp:65-82), so the pair table holds spheres x world fixtures entries.
+	if (d.n_world > 4096) { set_error("at most 4096 world fixtures per body are supported"); return B3G_UNSUPPORTED; }
+	if ((uint64_t)d.n_spheres * d.n_world >= (1ull << 31)) {
+		set_error("sphere fixtures x world fixtures exceeds 2^31 contact pairs");
+		return B3G_UNSUPPORTED;
+	}
 	return B3G_OK;
 }
 
@@ -987,6 +994,10 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 				b.hostPairWorld.push_back(w);
 			}
 			if (w1 > w0) contactRow[row] = 1;
+			if (b.hostPairSphere.size() >= (1ull << 31)) {
+				set_error("sphere fixtures x world fixtures exceeds 2^31 contact pairs");
+				return B3G_UNSUPPORTED;
+			}
 			b.hostPairStart.push_back((uint32_t)b.hostPairSphere.size());
 		}
 		b.nPairs = (uint32_t)b.hostPairSphere.size();
@@ -1162,6 +1173,8 @@ b3g_status build_body(Body& b, const b3g_body_desc* descs, uint32_t count) {
 		s0.statMinSub = 0xFFFFFFFFu;
 		std::vector<SolverState> states(count, s0);
 		UP(b.state, states);
+		const std::vector<uint32_t> noFault(1, 0u);
+		UP(b.faultWord, noFault);
 	}
 	UP(b.gravity, b.hostGravity);
 	UP(b.rowInst, b.hostRowInst);

@@ -37,8 +37,13 @@ typedef enum b3g_status {
 	B3G_OUT_OF_MEMORY = 3,
 	B3G_DEGENERATE_ELEMENT = 4,   /* zero-volume tetrahedron / zero-area triangle (replaces B3_ASSERT(det != 0)) */
 	B3G_UNSUPPORTED = 5,
-	B3G_NOT_COMMITTED = 6
+	B3G_NOT_COMMITTED = 6,
+	B3G_NON_POSITIVE_DIAGONAL = 7, /* a Jacobi diagonal entry of pA was not > 0 (replaces B3_ASSERT, sparse_solver.cpp:45-52) */
+	B3G_NON_FINITE = 8             /* a step produced a non-finite position or velocity */
 } b3g_status;
+
+/* Per-instance fault bits (b3g_step_stats::faults), sticky until the body is rebuilt or a state is loaded. */
+enum { B3G_FAULT_DIAGONAL = 1, B3G_FAULT_NON_FINITE = 2 };
 
 typedef struct b3g_body_s* b3g_body;
 
@@ -46,7 +51,9 @@ typedef struct b3g_body_s* b3g_body;
 enum { B3G_FORCE_STRETCH = 1, B3G_FORCE_SHEAR = 2, B3G_FORCE_SPRING = 3, B3G_FORCE_MOUSE = 4,
        B3G_FORCE_TRI_ELEMENT = 5, B3G_FORCE_TET_ELEMENT = 6 };
 
-/* World shape codes (b3Shape::Type, collision/shapes/shape.h:42-51). Mesh shapes are not supported. */
+/* World shape codes (b3Shape::Type, collision/shapes/shape.h:42-51). Every shape type is supported; a body
+ * (each instance of a batch) takes up to 4096 world fixtures, and sphere fixtures x world fixtures must stay
+ * below 2^31 contact pairs (every sphere pairs with every world fixture, contact_manager.cpp:65-82). */
 enum { B3G_SHAPE_SPHERE = 0, B3G_SHAPE_CAPSULE = 1, B3G_SHAPE_TRIANGLE = 2, B3G_SHAPE_BOX = 3,
        B3G_SHAPE_MESH = 4, B3G_SHAPE_SDF = 5 };
 
@@ -157,6 +164,7 @@ typedef struct b3g_step_stats {
 	uint32_t contact_count;       /* live sphere-shape contacts (b3ContactManager::m_contactCount) */
 	uint32_t refinement_sweeps;   /* precision form: FP64 residual sweeps of the last Newton iteration */
 	float relative_residual;      /* precision form: last FP64 ||pb - pA x|| / ||pb - pA x0|| */
+	uint32_t faults;              /* B3G_FAULT_* bits this instance has raised */
 } b3g_step_stats;
 
 const char* b3g_status_string(b3g_status s);
@@ -393,7 +401,15 @@ b3g_status b3g_set_solver(b3g_body body, int mode, uint32_t inner_iterations, fl
  */
 b3g_status b3g_set_graph(b3g_body body, int enabled);
 
-/* Waits for all work queued on the body. */
+/*
+ * Waits for all work queued on the body. The reference asserts on a
+ * non-positive Jacobi diagonal (sparse_solver.cpp:45-52) and has no check for
+ * a diverging state; here the steps go on as the reference's release build
+ * does, the device records the fault per instance (b3g_step_stats::faults),
+ * and b3g_synchronize returns B3G_NON_FINITE or B3G_NON_POSITIVE_DIAGONAL
+ * once for the steps since the previous b3g_synchronize that raised one
+ * (b3g_last_error() names the first instance).
+ */
 b3g_status b3g_synchronize(b3g_body body);
 
 /*
@@ -408,6 +424,33 @@ b3g_status b3g_synchronize(b3g_body body);
  * positions, velocities and the solver state unchanged.
  */
 b3g_status b3g_time_spmv(b3g_body body, uint32_t launches, double* mean_ms, double* bytes_per_launch);
+
+/*
+ * The roofline pass from HBM (no counterpart in the reference): as
+ * b3g_time_spmv, but before every SpMV launch a kernel streams reads over
+ * `evict_bytes` (at least 512 MiB) of device memory, so the matrix and the
+ * vectors are no longer in the L2 or the 256 MB memory-side cache (MALL)
+ * and the launch reads them from HBM (reads leave no dirty lines whose
+ * write-back would overlap the SpMV). The same number of eviction launches
+ * alone is timed too; *mean_ms = (both - evictions alone) / launches, which
+ * still includes one launch gap per SpMV. The eviction buffer is allocated
+ * for the call and freed after it.
+ */
+b3g_status b3g_time_spmv_cold(b3g_body body, uint32_t launches, uint64_t evict_bytes, double* mean_ms,
+                              double* bytes_per_launch);
+
+/*
+ * Early exit of the PCG (no counterpart in the reference's API; it is the
+ * reference's own loop exit at sparse_solver.cpp:63-73): with enabled != 0
+ * (the default) an uncaptured b3g_step enqueues the PCG iterations in chunks
+ * and stops enqueuing once no instance's last iteration ran. Iterations past
+ * convergence do nothing either way, so the results are identical; only
+ * their launches are saved. The step's launch sequence is then enqueued by
+ * a thread of the body, and b3g_step returns at once; every other function
+ * of the body first waits until that step is fully enqueued. Captured steps
+ * (b3g_set_graph) always record the fixed sequence.
+ */
+b3g_status b3g_set_early_exit(b3g_body body, int enabled);
 
 /*
  * Host-only diagnostics: the block pattern of pA for a description (list-order

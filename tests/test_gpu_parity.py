@@ -215,3 +215,41 @@ def test_spmv_timing_pass_before_first_step():
     xa, va = a.state()
     xb, vb = b.state()
     assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+
+
+def many_worlds_scene(count=40):
+    """A cloth falling onto a field of `count` world fixtures (spheres, boxes and capsules): every sphere fixture
+    pairs with all of them (contact_manager.cpp:65-82), more than a fixed per-particle table would hold."""
+    from bounce_softbody_amd.meshes import GridClothMesh, cloth_body
+    m = GridClothMesh(10, 10).scale((0.2, 1.0, 0.2)).translate((0.0, 0.25, 0.0))
+    b = cloth_body(m, density=0.2, radius=0.03, friction=0.4, stretching_stiffness=1000.0, shear_stiffness=100.0,
+                   damping=0.02)
+    rng = np.random.default_rng(99)
+    for k in range(count):
+        c = (float(rng.uniform(-1.0, 1.0)), float(rng.uniform(-0.2, 0.1)), float(rng.uniform(-1.0, 1.0)))
+        if k % 3 == 0:
+            b.add_world_sphere(c, float(rng.uniform(0.05, 0.2)), float(rng.uniform(0.1, 0.6)))
+        elif k % 3 == 1:
+            b.add_world_box((0.1, 0.05, 0.1), translation=c, friction=float(rng.uniform(0.1, 0.6)))
+        else:
+            b.add_world_capsule(c, (c[0] + 0.2, c[1], c[2]), 0.05, float(rng.uniform(0.1, 0.6)))
+    b.gravity[:] = (0.0, -9.8, 0.0)
+    return b.arrays()
+
+
+def test_more_than_sixteen_world_fixtures_match_oracle():
+    arr = many_worlds_scene(40)
+    g = Body(arr)
+    g.set_reduction(_lib.REDUCE_SEQUENTIAL)
+    o = oracle.OracleBody(arr)
+    touched = 0
+    for step in range(25):
+        g.Step(configs.DT, 2, 20, 0.0, 1e-5)
+        o.step(configs.DT, 2, 20, 0.0, 1e-5)
+        xg, vg = g.state()
+        xo, vo = o.state()
+        assert np.array_equal(xg, xo) and np.array_equal(vg, vo), f"step {step}"
+        gc, oc = g.contacts(), o.contacts()
+        assert all(np.array_equal(p, q) for p, q in zip(gc, oc)), f"contacts, step {step}"
+        touched = max(touched, len(set(gc[1].tolist())))
+    assert touched >= 3    # contacts with several different world fixtures

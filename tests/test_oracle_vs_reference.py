@@ -67,3 +67,19 @@ def test_ray_cast_live(name):
     assert (a[0] >= 0).sum() > 300
     for u, w in zip(a, b):
         assert np.array_equal(u, w)
+
+
+def test_many_world_fixtures_bit_exact():
+    """A cloth over 40 world fixtures (spheres, boxes, capsules): the oracle's contact lists, states and statistics
+    equal the reference's, the case the device's more-than-16-world-fixture test compares against."""
+    from tests.test_gpu_parity import many_worlds_scene
+    arr = many_worlds_scene(40)
+    ref, orc = oracle.RefBody(arr), oracle.OracleBody(arr)
+    for s in range(25):
+        ref.step(configs.DT, 2, 20, 0.0, 1e-5)
+        orc.step(configs.DT, 2, 20, 0.0, 1e-5)
+        xr, vr = ref.state()
+        xo, vo = orc.state()
+        assert np.array_equal(xr, xo) and np.array_equal(vr, vo), f"step {s}"
+        assert all(np.array_equal(a, b) for a, b in zip(ref.contacts(), orc.contacts())), f"contacts, step {s}"
+    assert len(set(orc.contacts()[1].tolist())) > 16
