@@ -699,8 +699,15 @@ b3g_status b3g_get_system(b3g_body h, uint32_t* rowptr, uint32_t* cols, float* v
 	if (vals) {
 		std::vector<float> A((size_t)b.aStride * 9);
 		B3G_HIP(hipMemcpy(A.data(), b.A.p, A.size() * 4, hipMemcpyDeviceToHost));
-		for (size_t k = 0; k < b.nnz; ++k)
-			for (int c = 0; c < 9; ++c) vals[9 * k + c] = A[(size_t)c * b.aStride + k];
+		std::vector<uint32_t> pos;
+		if (b.sortedA) {   // storage position of each CSR slot
+			pos.resize(b.nnz);
+			B3G_HIP(hipMemcpy(pos.data(), b.slotPos.p, pos.size() * 4, hipMemcpyDeviceToHost));
+		}
+		for (size_t k = 0; k < b.nnz; ++k) {
+			const size_t at = b.sortedA ? pos[k] : k;
+			for (int c = 0; c < 9; ++c) vals[9 * k + c] = A[(size_t)c * b.aStride + at];
+		}
 	}
 	auto rows = [&](const DBuf& buf, float* out) -> b3g_status {
 		std::vector<V3> t(b.n);
@@ -917,7 +924,7 @@ b3g_status matrix_body(Body& b, int device, uint32_t n, const uint32_t* rowptr, 
 	for (DBuf* buf : {&b.pb, &b.invM, &b.px, &b.r, &b.s, &b.q, &b.d0, &b.d1})
 		if ((s = dalloc(b, *buf, (size_t)n * sizeof(V3))) != B3G_OK) return s;
 	if ((s = dalloc(b, b.state, sizeof(SolverState))) != B3G_OK) return s;
-	return build_tiles(b);
+	return build_tiles(b, false);   // the caller's CSR storage
 }
 
 void matrix_body_free(Body& b) {

@@ -208,6 +208,15 @@ struct Body {
 	DBuf rowBlk;                                  // uint32 boundaries of <= 256-row blocks (vector kernels)
 	DBuf blkInst;
 	uint32_t nRowBlk = 0;
+	// Column-sorted storage of pA (sortedA): within each kTileCap-block chunk of
+	// every SpMV tile the blocks are stored in column order, so a wavefront's
+	// gathers of the vector at the columns touch neighbouring rows. code[k] =
+	// (column - tileBase[tile]) << 10 | (k's CSR position within the chunk);
+	// the SpMV writes each product to its CSR position, so every row still
+	// sums in CSR order. slotPos maps a CSR slot to its storage position,
+	// storeSlot the reverse. Without sortedA, storage position = CSR slot.
+	bool sortedA = false;
+	DBuf code, tileBase, slotPos, storeSlot;     // uint32
 	DBuf slotTiles;                               // uint32 slot boundaries of slot-assembly tiles
 	DBuf slotTileInst;
 	uint32_t nSlotTiles = 0;
@@ -343,7 +352,9 @@ b3g_status enqueue_spmv(Body& b, const DBuf& xin, DBuf& yout);
 b3g_status time_pcg_spmv(Body& b, uint32_t launches, double* mean_ms);
 b3g_status time_pcg_spmv_cold(Body& b, uint32_t launches, size_t evictBytes, double* mean_ms);
 b3g_status enqueue_solve_cg(Body& b, uint32_t maxIt, float eps);
-b3g_status build_tiles(Body& b);
+// sorted: also the column-sorted storage of pA (bodies assembled on the device;
+// operator-level matrices given in CSR storage keep it)
+b3g_status build_tiles(Body& b, bool sorted = true);
 b3g_status ensure_partials(Body& b, uint32_t blocks);
 b3g_status ensure_cg_log(Body& b, uint32_t iterations);
 // Pinned flag memory and events of the PCG's early exit (allocated once, before any capture).

@@ -1115,6 +1115,7 @@ struct AsmArgs {
 	const uint32_t* rowInst;
 	SolverState* stw;             // the same states, written: fault bits
 	uint32_t* faultWord;          // fault bits of any instance
+	const uint32_t* slotPos;      // storage position of each CSR slot (column-sorted storage), or null
 };
 
 // acc += the contributions [ptr[s], ptr[s + 1]) of slot s, in order, for the
@@ -1344,7 +1345,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble_slots(AsmArgs a) {
 	M33 Sj = dyn_j ? mdiag(1.0f) : mzero();
 	M33 P = (Si * Aij) * Sj;
 	if (i == j) P = (P + mdiag(1.0f)) - Si;
-	stA(a.A, a.stride, s, P);
+	stA(a.A, a.stride, a.slotPos ? a.slotPos[s] : s, P);
 	// (h dfdx_ij) w_j for b's product
 	a.tB[s] = (h * dx) * w_j;
 	if (a.tB64) {   // precision form: the same product in FP64 with w in FP64
@@ -1457,7 +1458,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble_rows(AsmArgs a) {
 	a.pb[i] = S * b;
 	// Jacobi preconditioner from pA(i, i) (sparse_solver.cpp:40-55); the
 	// reference asserts each diagonal entry > 0, here the fault is recorded
-	uint32_t ds = a.diagSlot[i];
+	uint32_t ds = a.slotPos ? a.slotPos[a.diagSlot[i]] : a.diagSlot[i];
 	float axx = a.A[ds], ayy = a.A[(size_t)4 * a.stride + ds], azz = a.A[(size_t)8 * a.stride + ds];
 	a.invM[i] = v3(1.0f / axx, 1.0f / ayy, 1.0f / azz);
 	if (!(axx > 0.0f) || !(ayy > 0.0f) || !(azz > 0.0f)) {
@@ -1500,6 +1501,9 @@ struct CGArgs {
 	const uint4* Ab;
 	const uint16_t* Ab8;
 	const float* diagF;
+	// column-sorted storage (Body::sortedA), or null: CSR storage with cols
+	const uint32_t* code;
+	const uint32_t* tileBase;
 };
 
 __device__ __forceinline__ V3 diag_apply(V3 im, V3 r) { return mdiag3(im.x, im.y, im.z) * r; }
@@ -1523,11 +1527,12 @@ __device__ __forceinline__ M33 ldAb(const CGArgs& a, uint32_t k) {
 // One SpMV tile into acc (products in LDS, then row sums in CSR order).
 // before() runs first, uniformly over the workgroup; false from it ends the
 // tile (the function returns false).
-template <class Gather, class Before>
-__device__ __forceinline__ bool tile_rows(const CGArgs& a, uint32_t t, uint32_t& row, Gather gather, float* lds,
-                                          Before before, V3& acc) {
+template <bool Sorted, class Gather, class Before>
+__device__ __forceinline__ bool tile_rows_(const CGArgs& a, uint32_t t, uint32_t& row, Gather gather, float* lds,
+                                           Before before, V3& acc) {
 	const uint32_t r0 = a.tiles[t], r1 = a.tiles[t + 1];
 	const uint32_t b0 = a.rowptr[r0], b1 = a.rowptr[r1];
+	const uint32_t base = Sorted ? a.tileBase[t] : 0u;
 	row = r0 + threadIdx.x;
 	const bool own = row < r1;
 	uint32_t rb = own ? a.rowptr[row] : 0, re = own ? a.rowptr[row + 1] : 0;
@@ -1543,12 +1548,21 @@ __device__ __forceinline__ bool tile_rows(const CGArgs& a, uint32_t t, uint32_t&
 		// The thread's columns first, so each block's gathers issue together
 		// with its matrix loads; a lane past the chunk loads the chunk's first
 		// block (in bounds) and stores nothing, which keeps the loads free of
-		// branches.
-		uint32_t col[kRowPer];
+		// branches. With the column-sorted storage the block at storage
+		// position k has its column and its CSR position (where its product
+		// goes) in code[k].
+		uint32_t col[kRowPer], dst[kRowPer];
 #pragma unroll
 		for (int u = 0; u < kRowPer; ++u) {
-			const uint32_t k = cb + threadIdx.x + u * kBlock;
-			col[u] = a.cols[k < ce ? k : cb];
+			const uint32_t k = cb + threadIdx.x + u * kBlock, kk = k < ce ? k : cb;
+			if (Sorted) {
+				const uint32_t c = a.code[kk];
+				col[u] = base + (c >> 10);
+				dst[u] = c & 1023u;
+			} else {
+				col[u] = a.cols[kk];
+				dst[u] = kk - cb;
+			}
 		}
 		auto products = [&](auto ld) {
 #pragma unroll
@@ -1557,7 +1571,7 @@ __device__ __forceinline__ bool tile_rows(const CGArgs& a, uint32_t t, uint32_t&
 				const bool in = k < ce;
 				const V3 g = gather(col[u]);
 				const V3 p = ld(in ? k : cb) * g;
-				if (in) put(k - cb, p);
+				if (in) put(dst[u], p);
 			}
 		};
 		if (a.bf16) products([&](uint32_t k) { return ldAb(a, k); });
@@ -1581,6 +1595,15 @@ __device__ __forceinline__ bool tile_rows(const CGArgs& a, uint32_t t, uint32_t&
 	}
 	if (a.bf16 && own) acc = acc + ldM(a.diagF, a.n, 0, row) * gather(row);   // FP32 diagonal block (zero in the BF16 planes)
 	return true;
+}
+
+// The storage is a property of the body, uniform over the launch: one
+// instantiation per storage keeps each loop free of the other's branches.
+template <class Gather, class Before>
+__device__ __forceinline__ bool tile_rows(const CGArgs& a, uint32_t t, uint32_t& row, Gather gather, float* lds,
+                                          Before before, V3& acc) {
+	return a.code ? tile_rows_<true>(a, t, row, gather, lds, before, acc)
+	              : tile_rows_<false>(a, t, row, gather, lds, before, acc);
 }
 
 __device__ __forceinline__ bool no_prologue() { return true; }
@@ -1809,6 +1832,10 @@ struct RefineArgs {
 	uint4* Ab;
 	uint16_t* Ab8;
 	float* diagF;
+	// column-sorted storage (Body::sortedA), or null
+	const uint32_t* code;
+	const uint32_t* tileBase;
+	const uint32_t* storeSlot;
 };
 
 // Round-to-nearest-even FP32 -> BF16 (NaN stays NaN).
@@ -1824,11 +1851,12 @@ __device__ __forceinline__ uint32_t bf16_bits(float f) {
 __global__ void __launch_bounds__(kBlock) k_split_bf16(RefineArgs a) {
 	uint32_t k = blockIdx.x * kBlock + threadIdx.x;
 	if (k >= a.nnz) return;
-	const uint32_t row = a.slotRow[k];
+	const uint32_t slot = a.storeSlot ? a.storeSlot[k] : k;   // the CSR slot stored at position k
+	const uint32_t row = a.slotRow[slot];
 	if (!a.st[a.rowInst[row]].newtonActive) return;
 	M33 m = ldA(a.A, a.stride, k);
 	const float* p = &m.x.x;
-	if (a.cols[k] == row) {
+	if (a.cols[slot] == row) {
 		for (int c = 0; c < 9; ++c) a.diagF[(size_t)c * a.n + row] = p[c];
 		a.Ab[k] = make_uint4(0u, 0u, 0u, 0u);
 		a.Ab8[k] = 0;
@@ -1850,23 +1878,33 @@ __global__ void __launch_bounds__(kBlock) k_resid64(RefineArgs a, int first) {
 	const uint32_t inst = a.tileInst[t];
 	const SolverState& st = a.st[inst];
 	if (!st.newtonActive || (!first && !st.refineActive)) return;
-	__shared__ double lds[3 * (kTileCap / 2)];
+	__shared__ double lds[3 * kTileCap];
 	const uint32_t r0 = a.tiles[t], r1 = a.tiles[t + 1];
 	const uint32_t b0 = a.rowptr[r0], b1 = a.rowptr[r1];
+	const uint32_t base = a.code ? a.tileBase[t] : 0u;
 	const uint32_t row = r0 + threadIdx.x;
 	const bool own = row < r1;
 	const uint32_t rb = own ? a.rowptr[row] : 0, re = own ? a.rowptr[row + 1] : 0;
 	double ax = 0.0, ay = 0.0, az = 0.0;
-	constexpr uint32_t cap = kTileCap / 2;
+	constexpr uint32_t cap = kTileCap;   // the chunks of the column-sorted storage
 	for (uint32_t cb = b0; cb < b1; cb += cap) {
 		const uint32_t ce = min(cb + cap, b1);
 		for (uint32_t k = cb + threadIdx.x; k < ce; k += kBlock) {
 			M33 m = ldA(a.A, a.stride, k);
-			const double* xv = a.x64 + 3 * (size_t)a.cols[k];
+			uint32_t col, dst;
+			if (a.code) {
+				const uint32_t c = a.code[k];
+				col = base + (c >> 10);
+				dst = c & 1023u;
+			} else {
+				col = a.cols[k];
+				dst = k - cb;
+			}
+			const double* xv = a.x64 + 3 * (size_t)col;
 			const double x0 = xv[0], x1 = xv[1], x2 = xv[2];
-			lds[3 * (k - cb)] = (double)m.x.x * x0 + (double)m.y.x * x1 + (double)m.z.x * x2;
-			lds[3 * (k - cb) + 1] = (double)m.x.y * x0 + (double)m.y.y * x1 + (double)m.z.y * x2;
-			lds[3 * (k - cb) + 2] = (double)m.x.z * x0 + (double)m.y.z * x1 + (double)m.z.z * x2;
+			lds[3 * dst] = (double)m.x.x * x0 + (double)m.y.x * x1 + (double)m.z.x * x2;
+			lds[3 * dst + 1] = (double)m.x.y * x0 + (double)m.y.y * x1 + (double)m.z.y * x2;
+			lds[3 * dst + 2] = (double)m.x.z * x0 + (double)m.y.z * x1 + (double)m.z.z * x2;
 		}
 		__syncthreads();
 		if (own) {
@@ -1943,7 +1981,7 @@ static inline uint32_t nblk(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 // block belongs to one instance: SpMV tiles (<= kTileRows rows, <= kTileCap
 // blocks) and row blocks (<= kRowBlkRows rows) for the vector kernels; plus each
 // instance's ranges of tiles, blocks and rows.
-b3g_status build_tiles(Body& b) {
+b3g_status build_tiles(Body& b, bool sorted) {
 	const uint32_t K = std::max(1u, b.nInst);
 	auto inst = [&](uint32_t i) { return b.hostRowInst.empty() ? 0u : b.hostRowInst[i]; };
 	std::vector<uint32_t> t{0}, ti, rb{0}, bi;
@@ -1992,6 +2030,40 @@ b3g_status build_tiles(Body& b) {
 	for (uint32_t k = 0; k < K; ++k)
 		b.maxInstPartials = std::max({b.maxInstPartials, range[6 * k + 1] - range[6 * k], range[6 * k + 3] - range[6 * k + 2]});
 	b3g_status s;
+	// column-sorted storage within each chunk (see Body::sortedA); a tile whose
+	// columns span 2^22 rows or more cannot be coded, and the CSR storage stays
+	b.sortedA = false;
+	if (sorted && b.nnz) {
+		std::vector<uint32_t> code(b.aStride, 0u), store(b.aStride, 0u), pos(b.nnz), base(b.nTiles, 0u);
+		bool ok = true;
+		std::vector<uint32_t> idx;
+		for (uint32_t x = 0; x < b.nTiles && ok; ++x) {
+			const uint32_t b0 = b.hostRowptr[t[x]], b1 = b.hostRowptr[t[x + 1]];
+			uint32_t lo = 0xFFFFFFFFu;
+			for (uint32_t k = b0; k < b1; ++k) lo = std::min(lo, b.hostCols[k]);
+			base[x] = b1 > b0 ? lo : 0u;
+			for (uint32_t cb = b0; cb < b1 && ok; cb += kTileCap) {
+				const uint32_t ce = std::min(cb + (uint32_t)kTileCap, b1);
+				idx.resize(ce - cb);
+				for (uint32_t k = cb; k < ce; ++k) idx[k - cb] = k;
+				std::stable_sort(idx.begin(), idx.end(), [&](uint32_t u, uint32_t v) { return b.hostCols[u] < b.hostCols[v]; });
+				for (uint32_t j = 0; j < (uint32_t)idx.size(); ++j) {
+					const uint32_t src = idx[j], dst = cb + j, off = b.hostCols[src] - base[x];
+					if (off >= (1u << 22)) { ok = false; break; }
+					code[dst] = (off << 10) | (src - cb);
+					pos[src] = dst;
+					store[dst] = src;
+				}
+			}
+		}
+		if (ok) {
+			if ((s = dupload(b, b.code, code.data(), code.size() * 4)) != B3G_OK) return s;
+			if ((s = dupload(b, b.storeSlot, store.data(), store.size() * 4)) != B3G_OK) return s;
+			if ((s = dupload(b, b.slotPos, pos.data(), pos.size() * 4)) != B3G_OK) return s;
+			if ((s = dupload(b, b.tileBase, base.data(), base.size() * 4)) != B3G_OK) return s;
+			b.sortedA = true;
+		}
+	}
 	if ((s = dupload(b, b.tiles, t.data(), t.size() * 4)) != B3G_OK) return s;
 	if ((s = dupload(b, b.tileInst, ti.data(), ti.size() * 4)) != B3G_OK) return s;
 	if ((s = dupload(b, b.rowBlk, rb.data(), rb.size() * 4)) != B3G_OK) return s;
@@ -2152,6 +2224,8 @@ static CGArgs cg_args(Body& b) {
 	a.Ab = dptr<uint4>(b.Ab);
 	a.Ab8 = dptr<uint16_t>(b.Ab8);
 	a.diagF = dptr<float>(b.diagF);
+	a.code = b.sortedA ? dptr<uint32_t>(b.code) : nullptr;
+	a.tileBase = b.sortedA ? dptr<uint32_t>(b.tileBase) : nullptr;
 	return a;
 }
 
@@ -2254,6 +2328,7 @@ static b3g_status enqueue_linearise(Body& b, float h) {
 	a.st = st;
 	a.stw = dptr<SolverState>(b.state);
 	a.faultWord = dptr<uint32_t>(b.faultWord);
+	a.slotPos = b.sortedA ? dptr<uint32_t>(b.slotPos) : nullptr;
 	a.y = dptr<V3>(b.y);
 	a.tB64 = precise ? dptr<double>(b.tB64) : nullptr;
 	a.fstage64 = dptr<D3>(b.fstage64);
@@ -2470,6 +2545,9 @@ static RefineArgs refine_args(Body& b) {
 	r.Ab = dptr<uint4>(b.Ab);
 	r.Ab8 = dptr<uint16_t>(b.Ab8);
 	r.diagF = dptr<float>(b.diagF);
+	r.code = b.sortedA ? dptr<uint32_t>(b.code) : nullptr;
+	r.tileBase = b.sortedA ? dptr<uint32_t>(b.tileBase) : nullptr;
+	r.storeSlot = b.sortedA ? dptr<uint32_t>(b.storeSlot) : nullptr;
 	return r;
 }
 
