@@ -130,6 +130,24 @@ def cpu_body_parallel(sample, dt, sub, tol):
     return max(per_worker.values()), workers
 
 
+def autotune(bodies, cache):
+    """b3g_autotune on each body after its timed steps: the chosen schedule and the search's ms per step (whole
+    steps from the body's state, which is restored) with it and with the default."""
+    from bounce_softbody_amd import _lib
+
+    out = []
+    for b in bodies:
+        t0 = time.perf_counter()
+        chosen, tuned_ms, default_ms = b.autotune(cache)
+        out.append({"chosen": dict(zip(("tile_blocks", "tile_rows", "row_block_rows", "fused"), chosen)),
+                    "default": dict(zip(("tile_blocks", "tile_rows", "row_block_rows", "fused"),
+                                        _lib.DEFAULT_SCHEDULE)),
+                    "search_ms_per_step": {"chosen": round(tuned_ms, 4), "default": round(default_ms, 4)},
+                    "search_s": round(time.perf_counter() - t0, 2)})
+        log(f"autotune: {out[-1]}")
+    return out
+
+
 def run_precision(args):
     """Precision form on the stiffness-contrast workload: a tet grid (args.tet cells per axis) whose elements
     have Young's moduli log-uniform in [1e4, 1e7] Pa. Every form steps from the same deformed, moving state
@@ -342,10 +360,24 @@ def run_batch(args):
             b.synchronize()
 
     steps([batch], args.warmup)
+    start = batch.save_state() if args.autotune else None
     t0 = time.perf_counter()
     steps([batch], args.steps)
     el = time.perf_counter() - t0
     value = args.steps / el
+    schedule = None
+    if args.autotune:
+        # the same steps again from the same state under the chosen schedule (one untimed step first)
+        tuned = autotune([batch], args.schedule_cache)[0]
+        batch.load_state(start)
+        steps([batch], 1)
+        batch.load_state(start)
+        t0 = time.perf_counter()
+        steps([batch], args.steps)
+        tuned_value = args.steps / (time.perf_counter() - t0)
+        log(f"batch with the chosen schedule: {tuned_value:.2f} steps/s (default schedule {value:.2f})")
+        schedule = dict(tuned, steps_per_s={"chosen": round(tuned_value, 3), "default": round(value, 3)})
+        value = tuned_value
     its = [int(s[0][2]) for s in batch.instance_stats()]
     mean_ms, by = batch.time_spmv(args.spmv_launches)
     achieved = by / (mean_ms * 1e-3) / 1e9
@@ -407,7 +439,8 @@ def run_batch(args):
                    "body_steps_per_s": round(value * args.bodies, 1),
                    "pcg_iterations_min_max": [min(its), max(its)],
                    "subset_ms_per_step": {"bodies": len(sub), "handles": round(1e3 * t_handles, 3),
-                                          "batch": round(1e3 * t_small, 3)}},
+                                          "batch": round(1e3 * t_small, 3)},
+                   "schedule": schedule or "default"},
         "roofline": {"kernel": "k_cg_spmv_dq (all instances, back-to-back pass)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -659,6 +692,10 @@ def main():
     ap.add_argument("--cpu-sample-bodies", type=int, default=None,
                     help="batch / contact workloads: bodies in the single-thread CPU sample (default 256 / 16)")
     ap.add_argument("--no-graph", action="store_true", help="batch / contact workloads: uncaptured steps")
+    ap.add_argument("--autotune", action="store_true",
+                    help="two_body / batch workloads: after the timed steps, autotune each body's PCG schedule and "
+                         "time the steps again with it (reported as value; the default schedule's rate beside it)")
+    ap.add_argument("--schedule-cache", default=None, help="autotune: schedule cache file (looked up, appended)")
     ap.add_argument("--contact-cloth", type=int, default=127, help="contact workload: cloth cells per side")
     ap.add_argument("--contact-tet", type=int, default=16, help="contact workload: tet block cells per axis")
     ap.add_argument("--contact-solver", choices=["pcg", "refined_fp32", "refined_bf16"], default="pcg",
@@ -744,12 +781,32 @@ def main():
         step()
     sync()
     log(f"warmup done ({args.warmup} steps)")
+    start = [b.save_state() for b in bodies] if args.autotune else None
 
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     sync()
     elapsed = time.perf_counter() - t0
+    schedule = None
+    if args.autotune:
+        # the same steps again from the same states under the chosen schedules (one untimed step first)
+        tuned = autotune(bodies, args.schedule_cache)
+        for b, blob in zip(bodies, start):
+            b.load_state(blob)
+        step()
+        sync()
+        for b, blob in zip(bodies, start):
+            b.load_state(blob)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        sync()
+        tuned_elapsed = time.perf_counter() - t0
+        log(f"chosen schedules: {args.steps / tuned_elapsed:.2f} steps/s (default schedule {args.steps / elapsed:.2f})")
+        schedule = {"bodies": tuned, "steps_per_s": {"chosen": round(args.steps / tuned_elapsed, 3),
+                                                     "default": round(args.steps / elapsed, 3)}}
+        elapsed = tuned_elapsed
     # roofline pass: the PCG SpMV kernel back-to-back on the system of the last step (its 147 MB matrix stays
     # in the 256 MB MALL between launches), then with the caches evicted before every launch (from HBM)
     mean_ms, bytes_per_launch = roof.time_spmv(args.spmv_launches)
@@ -798,6 +855,7 @@ def main():
             "parallelism": "single GPU",
             "form": ("two handles" if not args.batch else "one batch of two instances")
                     + (", captured steps" if args.graph else ""),
+            "schedule": schedule or "default",
         },
         "roofline": {
             "kernel": "k_cg_spmv_dq (" + ("tet body" if not args.batch else "both bodies, one batched launch")

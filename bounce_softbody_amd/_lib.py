@@ -19,6 +19,8 @@ f32p = C.POINTER(C.c_float)
 u8p = C.POINTER(C.c_uint8)
 
 B3G_OK = 0
+B3G_INVALID_ARGUMENT, B3G_HIP_ERROR, B3G_OUT_OF_MEMORY, B3G_DEGENERATE_ELEMENT, B3G_UNSUPPORTED, B3G_NOT_COMMITTED = (
+    1, 2, 3, 4, 5, 6)
 B3G_NON_POSITIVE_DIAGONAL, B3G_NON_FINITE = 7, 8
 FAULT_DIAGONAL, FAULT_NON_FINITE = 1, 2
 REDUCE_FP64, REDUCE_SEQUENTIAL, REDUCE_FP64_FUSED, REDUCE_FP64_SEPARATE = 0, 1, 2, 3
@@ -126,8 +128,25 @@ EXPORTS = [
     "b3g_synchronize", "b3g_time_spmv", "b3g_set_solver", "b3g_set_graph", "b3g_get_system_size", "b3g_pattern",
     "b3g_solve_cg", "b3g_spmv", "b3g_set_reduction", "b3g_save_state", "b3g_load_state", "b3g_ray_cast", "b3g_ray_cast_device",
     "b3g_sdf_grid", "b3g_build_sdf", "b3g_body_rebuild", "b3g_set_action_forces", "b3g_get_action_forces",
-    "b3g_solve_cg_static", "b3g_time_spmv_cold", "b3g_set_early_exit",
+    "b3g_solve_cg_static", "b3g_time_spmv_cold", "b3g_set_early_exit", "b3g_get_schedule", "b3g_set_schedule",
+    "b3g_autotune",
 ]
+
+
+class Schedule(C.Structure):
+    """``b3g_schedule`` (include/b3g.h): the PCG's SpMV tile shape, vector row-block size and FP64 sum place."""
+
+    _fields_ = [("tile_blocks", C.c_uint32), ("tile_rows", C.c_uint32), ("row_block_rows", C.c_uint32),
+                ("fused", C.c_int32)]
+
+    def as_tuple(self):
+        return (self.tile_blocks, self.tile_rows, self.row_block_rows, self.fused)
+
+    def __repr__(self):
+        return "Schedule(tile_blocks=%d, tile_rows=%d, row_block_rows=%d, fused=%d)" % self.as_tuple()
+
+
+DEFAULT_SCHEDULE = (1024, 256, 1024, -1)
 
 
 class Carry(C.Structure):
@@ -241,6 +260,9 @@ def lib():
         "b3g_time_spmv": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "b3g_time_spmv_cold": (C.c_int, [vp, C.c_uint32, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "b3g_set_early_exit": (C.c_int, [vp, C.c_int]),
+        "b3g_get_schedule": (C.c_int, [vp, C.POINTER(Schedule)]),
+        "b3g_set_schedule": (C.c_int, [vp, C.POINTER(Schedule)]),
+        "b3g_autotune": (C.c_int, [vp, C.c_char_p, C.POINTER(Schedule), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "b3g_set_solver": (C.c_int, [vp, C.c_int, C.c_uint32, C.c_float]),
         "b3g_set_graph": (C.c_int, [vp, C.c_int]),
         "b3g_get_system_size": (C.c_int, [vp, u32p, u32p]),

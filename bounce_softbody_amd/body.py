@@ -209,7 +209,11 @@ class Body:
         """pA (list-order CSR), pb and fi of the first linearisation of a step at the current state."""
         nnz = C.c_uint32(0)
         check(self._L.b3g_assemble(self._h, C.c_float(dt), C.byref(nnz)), "b3g_assemble")
-        k = nnz.value
+        return self.system()
+
+    def system(self):
+        """pA (list-order CSR), pb and fi as the device holds them now (after a step: its last linearisation)."""
+        k = self.system_size()[1]
         rowptr = np.zeros(self.n + 1, np.uint32)
         cols = np.zeros(max(1, k), np.uint32)
         vals = np.zeros(9 * max(1, k), f32)
@@ -296,6 +300,28 @@ class Body:
         """Stop enqueuing the PCG's iterations once every instance has converged (the default; identical
         results). Off: the reference's fixed sequence of iterations is enqueued, converged ones return at once."""
         check(self._L.b3g_set_early_exit(self._h, 1 if enabled else 0), "b3g_set_early_exit")
+
+    def schedule(self):
+        """b3g_get_schedule: (tile_blocks, tile_rows, row_block_rows, fused) of the PCG's launches."""
+        sc = _lib.Schedule()
+        check(self._L.b3g_get_schedule(self._h, C.byref(sc)), "b3g_get_schedule")
+        return sc.as_tuple()
+
+    def set_schedule(self, tile_blocks: int, tile_rows: int, row_block_rows: int, fused: int = -1):
+        """b3g_set_schedule: set the PCG's launch schedule (needs a built body)."""
+        sc = _lib.Schedule(tile_blocks, tile_rows, row_block_rows, fused)
+        check(self._L.b3g_set_schedule(self._h, C.byref(sc)), "b3g_set_schedule")
+
+    def autotune(self, cache_path: str | None = None):
+        """b3g_autotune: time candidate schedules on this body's system and keep the fastest, through an optional
+        on-disk cache keyed by the system's shape. Returns (schedule tuple, tuned ms, default ms) per PCG
+        iteration; the times are negative after a cache hit that recorded none."""
+        sc = _lib.Schedule()
+        tm = C.c_double(0)
+        dm = C.c_double(0)
+        path = cache_path.encode() if cache_path else None
+        check(self._L.b3g_autotune(self._h, path, C.byref(sc), C.byref(tm), C.byref(dm)), "b3g_autotune")
+        return sc.as_tuple(), tm.value, dm.value
 
 
 __all__ = ["Body", "B3gError"]

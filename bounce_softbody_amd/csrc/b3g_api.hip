@@ -2,11 +2,13 @@
 #include "b3g_internal.h"
 
 #include <algorithm>
+#include <chrono>
 #include <condition_variable>
 #include <deque>
 #include <functional>
 #include <map>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -452,6 +454,12 @@ b3g_status b3g_step(b3g_body h, float dt, uint32_t iterations, uint32_t sub_iter
 	float stol = sub_tolerance > 0.0f ? sub_tolerance : kEps;
 	if (!(stol < 1.0f)) { set_error("sub_tolerance must be in (0, 1)"); return B3G_INVALID_ARGUMENT; }   // sparse_solver.cpp:36
 	Body& b = h->body;
+	b.lastStep.valid = true;
+	b.lastStep.dt = dt;
+	b.lastStep.it = iterations;
+	b.lastStep.sub = sub_iterations;
+	b.lastStep.tol = tolerance;
+	b.lastStep.stol = sub_tolerance;
 	b3g_status s;
 	if (b.earlyExit && !b.useGraph && b.built) {
 		// queued for the body's thread, which also makes the step's one-time preparations; steps already
@@ -866,6 +874,192 @@ b3g_status b3g_time_spmv_cold(b3g_body h, uint32_t launches, uint64_t evict_byte
 	const bool bf16 = b.solver == B3G_SOLVER_REFINED_BF16 && b.Ab.p;
 	if (bytes_per_launch)
 		*bytes_per_launch = (bf16 ? 22.0 * b.nnz + 36.0 * b.n : 40.0 * b.nnz) + 4.0 * (b.n + 1) + 24.0 * b.n;
+	return B3G_OK;
+}
+
+b3g_status b3g_get_schedule(b3g_body h, b3g_schedule* out) {
+	CHECK_BODY(h);
+	if (!out) return B3G_INVALID_ARGUMENT;
+	const Schedule& sc = h->body.sched;
+	*out = {sc.tileBlocks, sc.tileRows, sc.rowBlockRows, sc.fused};
+	return B3G_OK;
+}
+
+namespace {
+static b3g_status check_schedule(const b3g_schedule* sc) {
+	if (!sc) { set_error("schedule is NULL"); return B3G_INVALID_ARGUMENT; }
+	if (sc->tile_blocks < 32 || sc->tile_blocks > 1024 || sc->tile_rows < 16 || sc->tile_rows > 256 ||
+	    sc->row_block_rows < 64 || sc->row_block_rows > 1024 || sc->fused < -1 || sc->fused > 1) {
+		set_error("schedule out of range: tile_blocks 32..1024, tile_rows 16..256, row_block_rows 64..1024, fused -1..1");
+		return B3G_INVALID_ARGUMENT;
+	}
+	return B3G_OK;
+}
+
+// FNV-1a over the system's shape, the reduction mode and the device architecture.
+static uint64_t schedule_signature(const Body& b) {
+	uint64_t h = 1469598103934665603ull;
+	auto mix = [&](uint64_t v) {
+		for (int k = 0; k < 8; ++k) { h ^= (v >> (8 * k)) & 0xFFu; h *= 1099511628211ull; }
+	};
+	mix(b.n);
+	mix(b.nnz);
+	mix(b.nInst);
+	mix((uint64_t)(int64_t)b.reduction);
+	for (size_t k = 0; k < b.hostRowBegin.size() && k < b.hostRowEnd.size(); ++k) mix(b.hostRowEnd[k] - b.hostRowBegin[k]);
+	hipDeviceProp_t prop;
+	if (hipGetDeviceProperties(&prop, b.device) == hipSuccess)
+		for (const char* c = prop.gcnArchName; *c; ++c) mix((uint8_t)*c);
+	return h;
+}
+} // namespace
+
+// A change of schedule re-orders pA's storage (the column sort is per tile)
+// and the autotuner's timings run the body's own kernels: the assembled pA
+// (in CSR slot order), pb, fi, the action forces and the per-instance solver
+// states are kept across `change` and put back, pA in the new storage order.
+static b3g_status reschedule(Body& b, const std::function<b3g_status()>& change) {
+	B3G_HIP(hipStreamSynchronize(b.stream));
+	std::vector<SolverState> st(b.nInst);
+	if (!st.empty()) B3G_HIP(hipMemcpy(st.data(), b.state.p, st.size() * sizeof(SolverState), hipMemcpyDeviceToHost));
+	std::vector<std::pair<DBuf*, std::vector<char>>> kept;
+	for (DBuf* d : {&b.pb, &b.fi, &b.act})
+		if (d->p && d->bytes) {
+			kept.emplace_back(d, std::vector<char>(d->bytes));
+			B3G_HIP(hipMemcpy(kept.back().second.data(), d->p, d->bytes, hipMemcpyDeviceToHost));
+		}
+	std::vector<float> A((size_t)b.aStride * 9), csr((size_t)b.nnz * 9);
+	std::vector<uint32_t> pos(b.nnz);
+	auto positions = [&]() -> b3g_status {
+		if (b.sortedA) B3G_HIP(hipMemcpy(pos.data(), b.slotPos.p, pos.size() * 4, hipMemcpyDeviceToHost));
+		else for (uint32_t k = 0; k < b.nnz; ++k) pos[k] = k;
+		return B3G_OK;
+	};
+	b3g_status s;
+	if (b.nnz) {
+		B3G_HIP(hipMemcpy(A.data(), b.A.p, A.size() * 4, hipMemcpyDeviceToHost));
+		if ((s = positions()) != B3G_OK) return s;
+		for (size_t k = 0; k < b.nnz; ++k)
+			for (int c = 0; c < 9; ++c) csr[(size_t)c * b.nnz + k] = A[(size_t)c * b.aStride + pos[k]];
+	}
+	if ((s = change()) != B3G_OK) return s;
+	B3G_HIP(hipStreamSynchronize(b.stream));
+	drop_graph(b);
+	if (b.nnz) {
+		if ((s = positions()) != B3G_OK) return s;
+		for (size_t k = 0; k < b.nnz; ++k)
+			for (int c = 0; c < 9; ++c) A[(size_t)c * b.aStride + pos[k]] = csr[(size_t)c * b.nnz + k];
+		B3G_HIP(hipMemcpy(b.A.p, A.data(), A.size() * 4, hipMemcpyHostToDevice));
+	}
+	if (!st.empty()) B3G_HIP(hipMemcpy(b.state.p, st.data(), st.size() * sizeof(SolverState), hipMemcpyHostToDevice));
+	for (auto& k : kept) B3G_HIP(hipMemcpy(k.first->p, k.second.data(), k.first->bytes, hipMemcpyHostToDevice));
+	return B3G_OK;
+}
+
+b3g_status b3g_set_schedule(b3g_body h, const b3g_schedule* sc) {
+	CHECK_BUILT(h);
+	b3g_status s = check_schedule(sc);
+	if (s != B3G_OK) return s;
+	SET_DEVICE(h);
+	Body& b = h->body;
+	return reschedule(b, [&]() -> b3g_status {
+		b.sched.tileBlocks = sc->tile_blocks;
+		b.sched.tileRows = sc->tile_rows;
+		b.sched.rowBlockRows = sc->row_block_rows;
+		b.sched.fused = sc->fused;
+		return build_tiles(b, true, true);
+	});
+}
+
+b3g_status b3g_autotune(b3g_body h, const char* cache_path, b3g_schedule* chosen, double* tuned_ms,
+                        double* default_ms) {
+	CHECK_BUILT(h);
+	SET_DEVICE(h);
+	Body& b = h->body;
+	B3G_HIP(hipStreamSynchronize(b.stream));
+	char sig[32];
+	std::snprintf(sig, sizeof(sig), "%016llx", (unsigned long long)schedule_signature(b));
+	double tuned = -1.0, dflt = -1.0;
+	bool hit = false;
+	Schedule found;
+	if (cache_path) {
+		if (FILE* f = std::fopen(cache_path, "r")) {
+			char line[512];
+			while (std::fgets(line, sizeof(line), f)) {
+				char key[32];
+				b3g_schedule sc;
+				double tm = -1.0, dm = -1.0;
+				const int got = std::sscanf(line, "%31s %u %u %u %d %lf %lf", key, &sc.tile_blocks, &sc.tile_rows,
+				                            &sc.row_block_rows, &sc.fused, &tm, &dm);
+				if (got >= 5 && std::strcmp(key, sig) == 0 && check_schedule(&sc) == B3G_OK) {
+					found.tileBlocks = sc.tile_blocks;
+					found.tileRows = sc.tile_rows;
+					found.rowBlockRows = sc.row_block_rows;
+					found.fused = sc.fused;
+					tuned = got >= 6 ? tm : -1.0;
+					dflt = got >= 7 ? dm : -1.0;
+					hit = true;
+				}
+			}
+			std::fclose(f);
+		}
+	}
+	b3g_status s;
+	if (hit) {
+		if ((s = reschedule(b, [&]() -> b3g_status { b.sched = found; return build_tiles(b, true, true); })) != B3G_OK) return s;
+	} else {
+		// after a step: whole steps with the last step's parameters from the
+		// current state, which is saved first and loaded before every timing
+		// and at the end; before any step: PCG iterations on the system
+		std::function<b3g_status(double*)> timer;
+		std::vector<char> blob;
+		const uint32_t lastConverged = b.lastConverged;
+		if (b.lastStep.valid) {
+			uint64_t bytes = 0;
+			if ((s = b3g_save_state(h, nullptr, &bytes)) != B3G_OK) return s;
+			blob.resize(bytes);
+			if ((s = b3g_save_state(h, blob.data(), &bytes)) != B3G_OK) return s;
+			const auto ls = b.lastStep;
+			timer = [&, ls](double* ms) -> b3g_status {
+				auto run = [&](uint32_t steps, double* el) -> b3g_status {
+					b3g_status r = b3g_load_state(h, blob.data(), blob.size());
+					if (r != B3G_OK) return r;
+					b.lastConverged = lastConverged;
+					const auto t0 = std::chrono::steady_clock::now();
+					for (uint32_t k = 0; k < steps; ++k)
+						if ((r = b3g_step(h, ls.dt, ls.it, ls.sub, ls.tol, ls.stol)) != B3G_OK) return r;
+					if ((r = b3g_synchronize(h)) != B3G_OK) return r;
+					*el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+					return B3G_OK;
+				};
+				double probe = 0.0, el = 0.0;
+				b3g_status r = run(1, &probe);   // warm-up (and the capture of a captured step)
+				if (r != B3G_OK) return r;
+				const uint32_t steps = (uint32_t)std::min(20.0, std::max(2.0, 40.0 / std::max(probe, 1e-3)));
+				if ((r = run(steps, &el)) != B3G_OK) return r;
+				*ms = el / steps;
+				return B3G_OK;
+			};
+		}
+		s = reschedule(b, [&]() -> b3g_status { return autotune(b, timer, &tuned, &dflt); });
+		if (!blob.empty()) {
+			b3g_status r = b3g_load_state(h, blob.data(), blob.size());
+			b.lastConverged = lastConverged;
+			if (s == B3G_OK) s = r;
+		}
+		if (s != B3G_OK) return s;
+		if (cache_path) {
+			FILE* f = std::fopen(cache_path, "a");
+			if (!f) { set_error(std::string("cannot append to the schedule cache ") + cache_path); return B3G_INVALID_ARGUMENT; }
+			std::fprintf(f, "%s %u %u %u %d %.6f %.6f # ms per %s; rows %u blocks %u instances %u\n", sig,
+			             b.sched.tileBlocks, b.sched.tileRows, b.sched.rowBlockRows, b.sched.fused, tuned, dflt,
+			             blob.empty() ? "PCG iteration" : "step", b.n, b.nnz, b.nInst);
+			std::fclose(f);
+		}
+	}
+	if (chosen) *chosen = {b.sched.tileBlocks, b.sched.tileRows, b.sched.rowBlockRows, b.sched.fused};
+	if (tuned_ms) *tuned_ms = tuned;
+	if (default_ms) *default_ms = dflt;
 	return B3G_OK;
 }
 

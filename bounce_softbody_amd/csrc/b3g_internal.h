@@ -8,6 +8,7 @@
 
 #include <array>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -149,6 +150,17 @@ struct SolverState {
 };
 
 
+// The launch schedule of the PCG (b3g_schedule): SpMV tile shape, vector
+// row-block size and where the FP64 dot products are summed. Any schedule
+// gives the reference's per-row and per-block arithmetic; the tiles and row
+// blocks only group the FP64 partial sums of the dot products.
+struct Schedule {
+	uint32_t tileBlocks = 1024;    // blocks per SpMV tile (at most 1024)
+	uint32_t tileRows = 256;       // rows per SpMV tile (at most 256)
+	uint32_t rowBlockRows = 1024;  // rows per vector-kernel block (at most 1024)
+	int fused = -1;                // FP64 mode: -1 by instance size, 0 separate, 1 fused
+};
+
 // A body holds a batch of one or more independent instances (bodies of the
 // reference). Their particles, elements, blocks and contact pairs are
 // concatenated; every instance keeps its own list orders, its own Newton /
@@ -201,6 +213,7 @@ struct Body {
 	DBuf jxPtr, jxIdx, jvPtr, jvIdx;              // per-slot contribution lists (stage indices)
 	DBuf bPtr, bSlot;                             // per-row dfdx slots in first-touch order (b's SpMV)
 	DBuf fPtr, fIdx;                              // per-row force contributions
+	std::vector<uint32_t> hostTiles, hostTileInst;   // tiles and their instances, host copies
 	DBuf tiles;                                   // uint32 row boundaries of SpMV / row-assembly tiles
 	uint32_t nTiles = 0;
 	uint32_t maxInstPartials = 0;   // largest tile or row-block count of one instance
@@ -310,6 +323,17 @@ struct Body {
 	hipEvent_t flagEvent[2] = {nullptr, nullptr};
 	uint32_t flagSeq = 0;
 	uint32_t lastConverged = 0;           // largest iteration count of the previous solve (first-chunk size)
+
+	// ---------------------------------------------------------------- launch schedule
+	// sched: the PCG's tile shape, row-block size and FP64 sum place
+	// (b3g_set_schedule / b3g_autotune), which build_tiles lays out; lastStep:
+	// the parameters of the last b3g_step, which the autotuner's timings repeat
+	Schedule sched;
+	struct {
+		bool valid = false;
+		float dt = 0.0f, tol = 0.0f, stol = 0.0f;
+		uint32_t it = 0, sub = 0;
+	} lastStep;
 	struct StepWorker* worker = nullptr;  // the thread that enqueues eager steps (b3g_api.hip)
 
 	// ---------------------------------------------------------------- diagnostics
@@ -354,7 +378,16 @@ b3g_status time_pcg_spmv_cold(Body& b, uint32_t launches, size_t evictBytes, dou
 b3g_status enqueue_solve_cg(Body& b, uint32_t maxIt, float eps);
 // sorted: also the column-sorted storage of pA (bodies assembled on the device;
 // operator-level matrices given in CSR storage keep it)
-b3g_status build_tiles(Body& b, bool sorted = true);
+b3g_status build_tiles(Body& b, bool sorted = true, bool tiles = true);
+// Times candidate schedules and keeps the fastest (b3g_autotune): `timer`
+// measures the body under its current schedule (ms); without one, a PCG
+// iteration on the body's system (time_pcg_iteration). *tunedMs /
+// *defaultMs: the measure of the kept schedule and of the default.
+b3g_status autotune(Body& b, const std::function<b3g_status(double*)>& timer, double* tunedMs, double* defaultMs);
+// One PCG iteration's kernels (SpMV, then the vector update, and the finalize
+// launches of the separate reduction) `reps` times, forced on the solver's
+// scratch vectors whatever its state; mean ms per iteration.
+b3g_status time_pcg_iteration(Body& b, uint32_t reps, double* ms);
 b3g_status ensure_partials(Body& b, uint32_t blocks);
 b3g_status ensure_cg_log(Body& b, uint32_t iterations);
 // Pinned flag memory and events of the PCG's early exit (allocated once, before any capture).

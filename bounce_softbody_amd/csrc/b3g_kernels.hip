@@ -1702,9 +1702,9 @@ __device__ __forceinline__ float cg_alpha(const CGArgs& a, uint32_t inst, uint32
 }
 
 // x += alpha d ; r -= alpha q ; s = invM r ; partials of r.s (iteration k)
-__global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k) {
+__global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k, int force) {
 	const uint32_t blk = blockIdx.x, inst = a.blkInst[blk];
-	if (!a.st[inst].newtonActive || !a.log[(size_t)inst * a.logStride + k].active) return;
+	if (!force && (!a.st[inst].newtonActive || !a.log[(size_t)inst * a.logStride + k].active)) return;
 	const uint32_t r0 = a.rowBlk[blk], r1 = a.rowBlk[blk + 1];
 	const bool seq = a.st[inst].seq != 0;
 	// every input of the thread's rows is loaded before the first store (the
@@ -1978,24 +1978,37 @@ static inline uint32_t nblk(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 	} while (0)
 
 // Row decompositions, both cut at instance boundaries so that every tile /
-// block belongs to one instance: SpMV tiles (<= kTileRows rows, <= kTileCap
-// blocks) and row blocks (<= kRowBlkRows rows) for the vector kernels; plus each
-// instance's ranges of tiles, blocks and rows.
-b3g_status build_tiles(Body& b, bool sorted) {
+// block belongs to one instance: SpMV tiles (<= sched.tileRows rows,
+// <= sched.tileBlocks blocks unless one row has more) and row blocks
+// (<= sched.rowBlockRows rows) for the vector kernels; plus each instance's
+// ranges of tiles, blocks and rows. With tiles = false the SpMV tiles (and
+// the column-sorted storage built on them) are kept and only the row blocks
+// are cut again.
+b3g_status build_tiles(Body& b, bool sorted, bool tiles) {
 	const uint32_t K = std::max(1u, b.nInst);
 	auto inst = [&](uint32_t i) { return b.hostRowInst.empty() ? 0u : b.hostRowInst[i]; };
-	std::vector<uint32_t> t{0}, ti, rb{0}, bi;
+	const uint32_t tileRows = std::min(b.sched.tileRows, (uint32_t)kTileRows);
+	const uint32_t tileCap = std::min(b.sched.tileBlocks, (uint32_t)kTileCap);
+	const uint32_t blkRows = std::min(b.sched.rowBlockRows, kRowBlkRows);
+	std::vector<uint32_t>& t = b.hostTiles;
+	std::vector<uint32_t>& ti = b.hostTileInst;
+	std::vector<uint32_t> rb{0}, bi;
+	if (tiles || t.empty()) {
+		t.assign(1, 0u);
+		ti.clear();
+		tiles = true;
+	}
 	uint32_t rows = 0, blocks = 0, brows = 0;
 	for (uint32_t i = 0; i < b.n; ++i) {
 		const uint32_t len = b.hostRowptr[i + 1] - b.hostRowptr[i];
 		const bool newInst = i > 0 && inst(i) != inst(i - 1);
-		if (rows > 0 && (newInst || rows == (uint32_t)kTileRows || blocks + len > (uint32_t)kTileCap)) {
+		if (tiles && rows > 0 && (newInst || rows == tileRows || blocks + len > tileCap)) {
 			t.push_back(i);
 			ti.push_back(inst(i - 1));
 			rows = 0;
 			blocks = 0;
 		}
-		if (brows > 0 && (newInst || brows == kRowBlkRows)) {
+		if (brows > 0 && (newInst || brows == blkRows)) {
 			rb.push_back(i);
 			bi.push_back(inst(i - 1));
 			brows = 0;
@@ -2004,8 +2017,10 @@ b3g_status build_tiles(Body& b, bool sorted) {
 		blocks += len;
 		brows += 1;
 	}
-	t.push_back(b.n);
-	ti.push_back(b.n ? inst(b.n - 1) : 0u);
+	if (tiles) {
+		t.push_back(b.n);
+		ti.push_back(b.n ? inst(b.n - 1) : 0u);
+	}
 	rb.push_back(b.n);
 	bi.push_back(b.n ? inst(b.n - 1) : 0u);
 	b.nTiles = (uint32_t)t.size() - 1;
@@ -2030,42 +2045,44 @@ b3g_status build_tiles(Body& b, bool sorted) {
 	for (uint32_t k = 0; k < K; ++k)
 		b.maxInstPartials = std::max({b.maxInstPartials, range[6 * k + 1] - range[6 * k], range[6 * k + 3] - range[6 * k + 2]});
 	b3g_status s;
-	// column-sorted storage within each chunk (see Body::sortedA); a tile whose
-	// columns span 2^22 rows or more cannot be coded, and the CSR storage stays
-	b.sortedA = false;
-	if (sorted && b.nnz) {
-		std::vector<uint32_t> code(b.aStride, 0u), store(b.aStride, 0u), pos(b.nnz), base(b.nTiles, 0u);
-		bool ok = true;
-		std::vector<uint32_t> idx;
-		for (uint32_t x = 0; x < b.nTiles && ok; ++x) {
-			const uint32_t b0 = b.hostRowptr[t[x]], b1 = b.hostRowptr[t[x + 1]];
-			uint32_t lo = 0xFFFFFFFFu;
-			for (uint32_t k = b0; k < b1; ++k) lo = std::min(lo, b.hostCols[k]);
-			base[x] = b1 > b0 ? lo : 0u;
-			for (uint32_t cb = b0; cb < b1 && ok; cb += kTileCap) {
-				const uint32_t ce = std::min(cb + (uint32_t)kTileCap, b1);
-				idx.resize(ce - cb);
-				for (uint32_t k = cb; k < ce; ++k) idx[k - cb] = k;
-				std::stable_sort(idx.begin(), idx.end(), [&](uint32_t u, uint32_t v) { return b.hostCols[u] < b.hostCols[v]; });
-				for (uint32_t j = 0; j < (uint32_t)idx.size(); ++j) {
-					const uint32_t src = idx[j], dst = cb + j, off = b.hostCols[src] - base[x];
-					if (off >= (1u << 22)) { ok = false; break; }
-					code[dst] = (off << 10) | (src - cb);
-					pos[src] = dst;
-					store[dst] = src;
+	if (tiles) {
+		// column-sorted storage within each chunk (see Body::sortedA); a tile whose
+		// columns span 2^22 rows or more cannot be coded, and the CSR storage stays
+		b.sortedA = false;
+		if (sorted && b.nnz) {
+			std::vector<uint32_t> code(b.aStride, 0u), store(b.aStride, 0u), pos(b.nnz), base(b.nTiles, 0u);
+			bool ok = true;
+			std::vector<uint32_t> idx;
+			for (uint32_t x = 0; x < b.nTiles && ok; ++x) {
+				const uint32_t b0 = b.hostRowptr[t[x]], b1 = b.hostRowptr[t[x + 1]];
+				uint32_t lo = 0xFFFFFFFFu;
+				for (uint32_t k = b0; k < b1; ++k) lo = std::min(lo, b.hostCols[k]);
+				base[x] = b1 > b0 ? lo : 0u;
+				for (uint32_t cb = b0; cb < b1 && ok; cb += kTileCap) {
+					const uint32_t ce = std::min(cb + (uint32_t)kTileCap, b1);
+					idx.resize(ce - cb);
+					for (uint32_t k = cb; k < ce; ++k) idx[k - cb] = k;
+					std::stable_sort(idx.begin(), idx.end(), [&](uint32_t u, uint32_t v) { return b.hostCols[u] < b.hostCols[v]; });
+					for (uint32_t j = 0; j < (uint32_t)idx.size(); ++j) {
+						const uint32_t src = idx[j], dst = cb + j, off = b.hostCols[src] - base[x];
+						if (off >= (1u << 22)) { ok = false; break; }
+						code[dst] = (off << 10) | (src - cb);
+						pos[src] = dst;
+						store[dst] = src;
+					}
 				}
 			}
+			if (ok) {
+				if ((s = dupload(b, b.code, code.data(), code.size() * 4)) != B3G_OK) return s;
+				if ((s = dupload(b, b.storeSlot, store.data(), store.size() * 4)) != B3G_OK) return s;
+				if ((s = dupload(b, b.slotPos, pos.data(), pos.size() * 4)) != B3G_OK) return s;
+				if ((s = dupload(b, b.tileBase, base.data(), base.size() * 4)) != B3G_OK) return s;
+				b.sortedA = true;
+			}
 		}
-		if (ok) {
-			if ((s = dupload(b, b.code, code.data(), code.size() * 4)) != B3G_OK) return s;
-			if ((s = dupload(b, b.storeSlot, store.data(), store.size() * 4)) != B3G_OK) return s;
-			if ((s = dupload(b, b.slotPos, pos.data(), pos.size() * 4)) != B3G_OK) return s;
-			if ((s = dupload(b, b.tileBase, base.data(), base.size() * 4)) != B3G_OK) return s;
-			b.sortedA = true;
-		}
+		if ((s = dupload(b, b.tiles, t.data(), t.size() * 4)) != B3G_OK) return s;
+		if ((s = dupload(b, b.tileInst, ti.data(), ti.size() * 4)) != B3G_OK) return s;
 	}
-	if ((s = dupload(b, b.tiles, t.data(), t.size() * 4)) != B3G_OK) return s;
-	if ((s = dupload(b, b.tileInst, ti.data(), ti.size() * 4)) != B3G_OK) return s;
 	if ((s = dupload(b, b.rowBlk, rb.data(), rb.size() * 4)) != B3G_OK) return s;
 	if ((s = dupload(b, b.blkInst, bi.data(), bi.size() * 4)) != B3G_OK) return s;
 	if ((s = dupload(b, b.instRange, range.data(), range.size() * 4)) != B3G_OK) return s;
@@ -2218,8 +2235,10 @@ static CGArgs cg_args(Body& b) {
 	// the default FP64 mode fuses while the consumers' re-reads of the
 	// partials stay cheaper than two extra launches per iteration (measured
 	// crossover between 3.5k and 11.5k partials per instance)
-	a.fused = b.reduction == B3G_REDUCE_FP64_FUSED ||
-	          (b.reduction == B3G_REDUCE_FP64 && b.maxInstPartials <= kFuseMaxPartials) ? 1u : 0u;
+	bool fused = b.reduction == B3G_REDUCE_FP64_FUSED;
+	if (b.reduction == B3G_REDUCE_FP64)
+		fused = b.sched.fused >= 0 ? b.sched.fused != 0 : b.maxInstPartials <= kFuseMaxPartials;
+	a.fused = fused ? 1u : 0u;
 	a.bf16 = b.solver == B3G_SOLVER_REFINED_BF16 && b.Ab.p ? 1u : 0u;
 	a.Ab = dptr<uint4>(b.Ab);
 	a.Ab8 = dptr<uint16_t>(b.Ab8);
@@ -2479,7 +2498,7 @@ static b3g_status enqueue_pcg_iterations(Body& b, const CGArgs& a, uint32_t maxI
 				LAUNCH(k_cg_update_x, rb, ak, k);
 				LAUNCH(k_cg_residual, b.nTiles, ak, (int)kFinDelta, k);
 			} else {
-				LAUNCH(k_cg_update, rb, ak, k);
+				LAUNCH(k_cg_update, rb, ak, k, 0);
 			}
 		}
 		chunk = kExitChunk;
@@ -2720,6 +2739,108 @@ b3g_status time_pcg_spmv_cold(Body& b, uint32_t launches, size_t evictBytes, dou
 		return st;
 	}
 	*mean_ms = ((double)both - (double)alone) / launches;
+	return B3G_OK;
+}
+
+b3g_status time_pcg_iteration(Body& b, uint32_t reps, double* ms) {
+	b3g_status s0 = ensure_cg_log(b, 3);
+	if (s0 != B3G_OK) return s0;
+	CGArgs a = cg_args(b);
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	B3G_HIP(hipEventCreate(&e0));
+	B3G_HIP(hipEventCreate(&e1));
+	b3g_status st = B3G_OK;
+	float el = 0.0f;
+	auto body = [&]() -> b3g_status {
+		B3G_HIP(hipEventRecord(e0, b.stream));
+		for (uint32_t r = 0; r < reps; ++r) {
+			CGArgs ak = a;
+			ak.dold = (r & 1) ? dptr<V3>(b.d1) : dptr<V3>(b.d0);
+			ak.dnew = (r & 1) ? dptr<V3>(b.d0) : dptr<V3>(b.d1);
+			LAUNCH(k_cg_spmv_dq, b.nTiles, ak, 1u, (int)kRangeBlocks, 1);
+			if (!a.fused) FINALIZE(b.partialQ, kRangeTiles, kFinAlpha, 1, kGateIter, 1);
+			LAUNCH(k_cg_update, b.nRowBlk, ak, 1u, 1);
+			if (!a.fused) FINALIZE(b.partial, kRangeBlocks, kFinDelta, 2, kGateIter, 1);
+		}
+		B3G_HIP(hipEventRecord(e1, b.stream));
+		B3G_HIP(hipEventSynchronize(e1));
+		B3G_HIP(hipEventElapsedTime(&el, e0, e1));
+		return B3G_OK;
+	};
+	st = body();
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	if (st != B3G_OK) return st;
+	*ms = reps ? (double)el / reps : 0.0;
+	return B3G_OK;
+}
+
+// Candidate schedules: SpMV tile shapes (blocks, rows), vector row blocks and
+// the place of the FP64 sums, each timed by `timer` (or as PCG iterations on
+// the body's own system). The default is timed first; a candidate faster
+// than it is timed again against it, alternately, and kept only if it is
+// faster by more than 1% both times. Any captured step is dropped with each change of tiles.
+b3g_status autotune(Body& b, const std::function<b3g_status(double*)>& timer, double* tunedMs, double* defaultMs) {
+	const uint32_t shapes[][2] = {{1024, 256}, {512, 256}, {1024, 128}, {512, 128}, {256, 64}};
+	const uint32_t blocks[] = {1024, 512, 256};
+	auto timeIt = [&](double* ms) -> b3g_status {
+		drop_graph(b);
+		if (timer) return timer(ms);
+		double probe = 0.0;
+		b3g_status s = time_pcg_iteration(b, 2, &probe);   // warm-up and size estimate
+		if (s != B3G_OK) return s;
+		const uint32_t reps = (uint32_t)std::min(200.0, std::max(3.0, 20.0 / std::max(probe, 1e-3)));
+		return time_pcg_iteration(b, reps, ms);
+	};
+	auto apply = [&](const Schedule& sc) -> b3g_status {
+		b.sched = sc;
+		return build_tiles(b, true, true);
+	};
+	b3g_status s;
+	const Schedule dflt;
+	if ((s = apply(dflt)) != B3G_OK) return s;
+	double dfltMs = 0.0;
+	if ((s = timeIt(&dfltMs)) != B3G_OK) return s;
+	Schedule best = dflt;
+	double bestMs = dfltMs;
+	for (const auto& sh : shapes) {
+		b.sched.tileBlocks = sh[0];
+		b.sched.tileRows = sh[1];
+		b.sched.rowBlockRows = 1024;
+		if ((s = build_tiles(b, true, true)) != B3G_OK) return s;
+		for (uint32_t rb : blocks) {
+			b.sched.rowBlockRows = rb;
+			if ((s = build_tiles(b, true, false)) != B3G_OK) return s;
+			for (int fused = 0; fused < 2; ++fused) {
+				b.sched.fused = fused;
+				double ms = 0.0;
+				if ((s = timeIt(&ms)) != B3G_OK) return s;
+				if (ms < bestMs) { bestMs = ms; best = b.sched; }
+			}
+		}
+	}
+	// the confirmation's means are reported (a search minimum is biased low)
+	bool keep = bestMs < dfltMs;
+	if (keep) {
+		double dsum = 0.0, csum = 0.0;
+		for (int round = 0; round < 2 && keep; ++round) {
+			double d = 0.0, c = 0.0;
+			if ((s = apply(dflt)) != B3G_OK || (s = timeIt(&d)) != B3G_OK) return s;
+			if ((s = apply(best)) != B3G_OK || (s = timeIt(&c)) != B3G_OK) return s;
+			keep = c < 0.99 * d;   // by more than the timings' noise
+			dsum += d;
+			csum += c;
+		}
+		if (keep) {
+			bestMs = csum / 2;
+			dfltMs = dsum / 2;
+		}
+	}
+	if (!keep) bestMs = dfltMs;
+	if ((s = apply(keep ? best : dflt)) != B3G_OK) return s;
+	drop_graph(b);
+	*tunedMs = bestMs;
+	*defaultMs = dfltMs;
 	return B3G_OK;
 }
 

@@ -453,6 +453,49 @@ b3g_status b3g_time_spmv_cold(b3g_body body, uint32_t launches, uint64_t evict_b
 b3g_status b3g_set_early_exit(b3g_body body, int enabled);
 
 /*
+ * The PCG's launch schedule (no counterpart in the reference, which runs one
+ * thread): the SpMV's tiles (at most tile_blocks 3x3 blocks and tile_rows
+ * rows each, at most 1024 and 256), the rows per workgroup of the vector
+ * kernels (row_block_rows, at most 1024) and where the FP64 dot products are
+ * summed (fused: 1 in the consuming kernels, 0 in a kernel of their own, -1
+ * by instance size as B3G_REDUCE_FP64 does). Every schedule computes each row
+ * and block as the reference does; tiles and row blocks only group the FP64
+ * partial sums of the dot products, whose rounding to FP32 then agrees in
+ * all but rare ties. Default: 1024, 256, 1024, -1.
+ */
+typedef struct b3g_schedule {
+	uint32_t tile_blocks;
+	uint32_t tile_rows;
+	uint32_t row_block_rows;
+	int32_t fused;
+} b3g_schedule;
+
+b3g_status b3g_get_schedule(b3g_body body, b3g_schedule* schedule);
+b3g_status b3g_set_schedule(b3g_body body, const b3g_schedule* schedule);
+
+/*
+ * Autotuner (no counterpart in the reference): times candidate schedules --
+ * five SpMV tile shapes, three row-block sizes, both places of the FP64 sums
+ * -- and keeps the fastest; a candidate that beats the default is timed
+ * against it twice more and kept only if it is faster by more than 1% both
+ * times. After a
+ * b3g_step each timing runs whole steps with that step's parameters from the
+ * body's current state, which is saved first and loaded again at the end;
+ * before any step, PCG iterations on the body's system. The assembled
+ * system, the step statistics and the action forces of the last step are
+ * kept. With cache_path != NULL the choice is looked up first, and stored
+ * after a search, in that text file, keyed by a signature of the system's
+ * shape (rows, blocks, rows per instance), the reduction mode and the
+ * device; a hit sets the cached schedule without timing. *chosen receives
+ * the schedule kept; *tuned_ms and *default_ms (either may be NULL) the
+ * measured milliseconds per step (per PCG iteration before any step) with it
+ * and with the default (those recorded in the cache after a hit). A step
+ * that faults during a timing ends the search with its status.
+ */
+b3g_status b3g_autotune(b3g_body body, const char* cache_path, b3g_schedule* chosen, double* tuned_ms,
+                        double* default_ms);
+
+/*
  * Host-only diagnostics: the block pattern of pA for a description (list-order
  * CSR, columns of each row in the reference's traversal order). Needs no
  * device. *nnz receives the block count; cols is written when capacity >= nnz.
