@@ -305,6 +305,49 @@ def run_batch_precision(args):
         log(f"{name}: {out[name]}")
     refined = [k for k in ("refined_fp32", "refined_bf16") if out[k]["max_rel_err_vs_fp64"] <= 1e-5]
     best_name = max(refined or ["refined_fp32", "refined_bf16"], key=lambda k: out[k]["steps_per_s"])
+    # the same first bodies stepped by plain FP32 as one handle each and as one batch, against the chosen
+    # refined form as one batch (all from the same saved states, steps as above)
+    bmode, binner, bsub, btol = next((m, i, s_, t) for n_, m, i, s_, t in forms if n_ == best_name)
+    sub_arrs = arrs[: args.handles_subset]
+    handles = [Body(a) for a in sub_arrs]
+    small = Body(sub_arrs)
+    small.set_graph(not args.no_graph)
+    for k, h in enumerate(handles):
+        h.set_state(xs[k], vs[k])
+    sx = np.concatenate([xs[k] for k in range(len(sub_arrs))])
+    sv = np.concatenate([vs[k] for k in range(len(sub_arrs))])
+
+    def timed(bodies, setup, sub, tol):
+        for b in bodies:
+            setup(b)
+        for b in bodies:
+            b.Step(dt, 1, sub, 0.0, tol)
+        for b in bodies:
+            b.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            for b in bodies:
+                b.Step(dt, 1, sub, 0.0, tol)
+        for b in bodies:
+            b.synchronize()
+        return args.steps / (time.perf_counter() - t0)
+
+    plain = (args.sub_iterations, args.sub_tolerance)
+    small.set_state(sx, sv)
+    hs = [h.save_state() for h in handles]
+    ss = small.save_state()
+    plain_handles = timed(handles, lambda b: None, *plain)
+    for h, blob in zip(handles, hs):
+        h.load_state(blob)
+    plain_batch = timed([small], lambda b: b.load_state(ss), *plain)
+    refined_batch = timed([small], lambda b: (b.load_state(ss), b.set_solver(bmode, *binner)), bsub, btol)
+    del handles, small
+    subset = {"bodies": len(sub_arrs), "steps_per_s": {"plain_fp32_handles": round(plain_handles, 3),
+                                                      "plain_fp32_batch": round(plain_batch, 3),
+                                                      best_name + "_batch": round(refined_batch, 3)},
+              "speedup_vs_plain_fp32_handles": round(refined_batch / plain_handles, 3),
+              "speedup_vs_plain_fp32_batch": round(refined_batch / plain_batch, 3)}
+    log(f"first {len(sub_arrs)} bodies: {subset}")
     plain_ok = [k for k in ("plain_fp32", "plain_fp32_200", "plain_fp32_tight") if out[k]["max_rel_err_vs_fp64"] <= 1e-5]
     plain_name = max(plain_ok, key=lambda k: out[k]["steps_per_s"]) if plain_ok else "plain_fp32_tight"
     best = out[best_name]
@@ -318,6 +361,9 @@ def run_batch_precision(args):
                    "dt": "1/60", "newton_iterations": 1, "form": f"{best_name} (value), one batch"
                    + ("" if args.no_graph else ", captured steps"),
                    "error_sample_bodies": len(sample), "forms": out,
+                   "vs_plain_fp32": {"whole_batch_steps_per_s": {"plain_fp32_batch": out["plain_fp32"]["steps_per_s"],
+                                                                 best_name + "_batch": best["steps_per_s"]},
+                                     "subset": subset},
                    "plain_reference_form": plain_name,
                    "speedup_vs_plain_at_1e-5": round(best["steps_per_s"] / out[plain_name]["steps_per_s"], 3),
                    "plain_meets_1e-5": bool(plain_ok)},
@@ -705,9 +751,11 @@ def main():
                     help="precision / contact workloads: untimed steps first (default 10 / 40)")
     ap.add_argument("--inner-iterations", type=int, default=50, help="precision workload: PCG iterations per sweep")
     ap.add_argument("--inner-tolerance", type=float, default=1e-3, help="precision workload: PCG tolerance per sweep")
-    ap.add_argument("--max-sweeps", type=int, default=4, help="precision workload: refinement sweeps at most")
-    ap.add_argument("--refine-tolerance", type=float, default=1e-4,
-                    help="precision workload: FP64 residual target relative to the first sweep's")
+    ap.add_argument("--max-sweeps", type=int, default=None,
+                    help="precision workloads: refinement sweeps at most (default 4; batch_precision: 8)")
+    ap.add_argument("--refine-tolerance", type=float, default=None,
+                    help="precision workloads: FP64 residual target relative to the first sweep's (default 1e-4; "
+                         "batch_precision: 1e-5)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -725,6 +773,10 @@ def main():
         args.warmup = 1 if bp else 5
     if args.cpu_sample_bodies is None:
         args.cpu_sample_bodies = 16 if contact else (8 if bp else 256)
+    if args.max_sweeps is None:
+        args.max_sweeps = 8 if bp else 4
+    if args.refine_tolerance is None:
+        args.refine_tolerance = 1e-5 if bp else 1e-4
     if args.pre_steps is None:
         args.pre_steps = 40 if contact else 10
     if args.workload == "precision":

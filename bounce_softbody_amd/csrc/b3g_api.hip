@@ -855,8 +855,9 @@ b3g_status b3g_time_spmv(b3g_body h, uint32_t launches, double* mean_ms, double*
 	Body& b = h->body;
 	b3g_status s = time_pcg_spmv(b, launches, mean_ms);
 	if (s != B3G_OK) return s;
-	// FP32 blocks: 36 + 4 (column) per block; BF16 operator: 18 + 4 per block and a 36-byte FP32 diagonal per row
-	const bool bf16 = b.solver == B3G_SOLVER_REFINED_BF16 && b.Ab.p;
+	// FP32 blocks: 36 + 4 (column) per block; BF16 operator (pair planes): 18 + 4 per block and a 36-byte FP32
+	// diagonal per row
+	const bool bf16 = b.solver == B3G_SOLVER_REFINED_BF16 && b.AbP.p;
 	if (bytes_per_launch)
 		*bytes_per_launch = (bf16 ? 22.0 * b.nnz + 36.0 * b.n : 40.0 * b.nnz) + 4.0 * (b.n + 1) + 24.0 * b.n;
 	return B3G_OK;
@@ -871,7 +872,7 @@ b3g_status b3g_time_spmv_cold(b3g_body h, uint32_t launches, uint64_t evict_byte
 	Body& b = h->body;
 	b3g_status s = time_pcg_spmv_cold(b, launches, (size_t)evict_bytes, mean_ms);
 	if (s != B3G_OK) return s;
-	const bool bf16 = b.solver == B3G_SOLVER_REFINED_BF16 && b.Ab.p;
+	const bool bf16 = b.solver == B3G_SOLVER_REFINED_BF16 && b.AbP.p;
 	if (bytes_per_launch)
 		*bytes_per_launch = (bf16 ? 22.0 * b.nnz + 36.0 * b.n : 40.0 * b.nnz) + 4.0 * (b.n + 1) + 24.0 * b.n;
 	return B3G_OK;

@@ -254,7 +254,12 @@ struct Body {
 	DBuf x64;        // double3 per row: refined solution
 	DBuf dx, rhsF;   // V3 per row: inner correction, inner right-hand side
 	DBuf partialR;   // double per tile: FP64 residual norm
-	DBuf Ab, Ab8;    // BF16 blocks: entries 0..7 as uint4, entry 8 as uint16 (diagonal slots zero)
+	// BF16 blocks in pair planes: word m of plane c holds entry c of the
+	// blocks at storage positions 2m and 2m + 1 of a tile (low, high half);
+	// each tile starts a new pair (pairBase: its first pair), so every chunk
+	// of kTileCap blocks starts on a word; diagonal slots zero
+	DBuf AbP, pairBase;
+	uint32_t pairStride = 0;   // words per plane (pairs, rounded up to 64)
 	DBuf diagF;      // 9 planes x rows: FP32 diagonal blocks of the BF16 operator
 	DBuf tet64;      // 12 planes x tets: FP64 invE (9), volume, lambda, mu
 	DBuf mass64;     // double per row: lumped masses in FP64

@@ -1498,8 +1498,9 @@ struct CGArgs {
 	uint32_t fused;             // FP64 mode: alpha / delta formed in the consuming kernel
 	// BF16 operator (precision form): off-diagonal blocks in BF16, diagonal blocks FP32 per row
 	uint32_t bf16;
-	const uint4* Ab;
-	const uint16_t* Ab8;
+	const uint32_t* AbP;        // nine pair planes (Body::AbP)
+	const uint32_t* pairBase;   // first pair of each tile
+	uint32_t pairStride;
 	const float* diagF;
 	// column-sorted storage (Body::sortedA), or null: CSR storage with cols
 	const uint32_t* code;
@@ -1511,23 +1512,13 @@ __device__ __forceinline__ V3 diag_apply(V3 im, V3 r) { return mdiag3(im.x, im.y
 __device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
 
-// Block k of the BF16 operator (entries in the column-major order of M33).
-__device__ __forceinline__ M33 ldAb(const CGArgs& a, uint32_t k) {
-	const uint4 w = a.Ab[k];
-	M33 m;
-	float* p = &m.x.x;
-	p[0] = bf16_lo(w.x); p[1] = bf16_hi(w.x);
-	p[2] = bf16_lo(w.y); p[3] = bf16_hi(w.y);
-	p[4] = bf16_lo(w.z); p[5] = bf16_hi(w.z);
-	p[6] = bf16_lo(w.w); p[7] = bf16_hi(w.w);
-	p[8] = __uint_as_float((uint32_t)a.Ab8[k] << 16);
-	return m;
-}
 
 // One SpMV tile into acc (products in LDS, then row sums in CSR order).
 // before() runs first, uniformly over the workgroup; false from it ends the
 // tile (the function returns false).
-template <bool Sorted, class Gather, class Before>
+// Pairs: the BF16 operator, whose pair planes give each thread the blocks
+// 2l and 2l + 1 of the chunk (l = thread, thread + 256) from nine words.
+template <bool Sorted, bool Pairs, class Gather, class Before>
 __device__ __forceinline__ bool tile_rows_(const CGArgs& a, uint32_t t, uint32_t& row, Gather gather, float* lds,
                                            Before before, V3& acc) {
 	const uint32_t r0 = a.tiles[t], r1 = a.tiles[t + 1];
@@ -1552,9 +1543,12 @@ __device__ __forceinline__ bool tile_rows_(const CGArgs& a, uint32_t t, uint32_t
 		// position k has its column and its CSR position (where its product
 		// goes) in code[k].
 		uint32_t col[kRowPer], dst[kRowPer];
+		auto local = [](int u) -> uint32_t {
+			return Pairs ? 2 * (threadIdx.x + (u >> 1) * kBlock) + (u & 1) : threadIdx.x + u * kBlock;
+		};
 #pragma unroll
 		for (int u = 0; u < kRowPer; ++u) {
-			const uint32_t k = cb + threadIdx.x + u * kBlock, kk = k < ce ? k : cb;
+			const uint32_t k = cb + local(u), kk = k < ce ? k : cb;
 			if (Sorted) {
 				const uint32_t c = a.code[kk];
 				col[u] = base + (c >> 10);
@@ -1574,8 +1568,32 @@ __device__ __forceinline__ bool tile_rows_(const CGArgs& a, uint32_t t, uint32_t
 				if (in) put(dst[u], p);
 			}
 		};
-		if (a.bf16) products([&](uint32_t k) { return ldAb(a, k); });
-		else products([&](uint32_t k) { return ldA(a.A, a.stride, k); });
+		if (Pairs) {
+			const uint32_t p0 = a.pairBase[t] + ((cb - b0) >> 1);   // cb - b0 is a multiple of kTileCap
+#pragma unroll
+			for (int j = 0; j < kRowPer / 2; ++j) {
+				const uint32_t l = threadIdx.x + j * kBlock, k = cb + 2 * l;
+				const bool in0 = k < ce, in1 = k + 1 < ce;
+				const uint32_t pp = p0 + (in0 ? l : 0u);
+				uint32_t w[9];
+#pragma unroll
+				for (int c = 0; c < 9; ++c) w[c] = a.AbP[(size_t)c * a.pairStride + pp];
+				M33 m0, m1;
+				float* q0 = &m0.x.x;
+				float* q1 = &m1.x.x;
+#pragma unroll
+				for (int c = 0; c < 9; ++c) {
+					q0[c] = bf16_lo(w[c]);
+					q1[c] = bf16_hi(w[c]);
+				}
+				const V3 g0 = gather(col[2 * j]), g1 = gather(col[2 * j + 1]);
+				const V3 e0 = m0 * g0, e1 = m1 * g1;
+				if (in0) put(dst[2 * j], e0);
+				if (in1) put(dst[2 * j + 1], e1);
+			}
+		} else {
+			products([&](uint32_t k) { return ldA(a.A, a.stride, k); });
+		}
 		__syncthreads();
 		if (own) {
 			// the row's products in CSR order; four LDS reads ahead of their adds
@@ -1602,8 +1620,11 @@ __device__ __forceinline__ bool tile_rows_(const CGArgs& a, uint32_t t, uint32_t
 template <class Gather, class Before>
 __device__ __forceinline__ bool tile_rows(const CGArgs& a, uint32_t t, uint32_t& row, Gather gather, float* lds,
                                           Before before, V3& acc) {
-	return a.code ? tile_rows_<true>(a, t, row, gather, lds, before, acc)
-	              : tile_rows_<false>(a, t, row, gather, lds, before, acc);
+	if (a.bf16)
+		return a.code ? tile_rows_<true, true>(a, t, row, gather, lds, before, acc)
+		              : tile_rows_<false, true>(a, t, row, gather, lds, before, acc);
+	return a.code ? tile_rows_<true, false>(a, t, row, gather, lds, before, acc)
+	              : tile_rows_<false, false>(a, t, row, gather, lds, before, acc);
 }
 
 __device__ __forceinline__ bool no_prologue() { return true; }
@@ -1829,8 +1850,9 @@ struct RefineArgs {
 	double* partialR;
 	double* partial;
 	SolverState* st;
-	uint4* Ab;
-	uint16_t* Ab8;
+	uint16_t* AbP16;   // the pair planes as 16-bit entries: plane c, entry 2 pairBase[t] + (k - b0)
+	const uint32_t* pairBase;
+	uint32_t pairStride;
 	float* diagF;
 	// column-sorted storage (Body::sortedA), or null
 	const uint32_t* code;
@@ -1846,26 +1868,29 @@ __device__ __forceinline__ uint32_t bf16_bits(float f) {
 	return u >> 16;
 }
 
-// BF16 copy of pA's off-diagonal blocks (diagonal slots zero) and the FP32
-// diagonal blocks per row, once per linearisation.
+// BF16 copy of pA's off-diagonal blocks (diagonal slots zero) in pair
+// planes and the FP32 diagonal blocks per row, once per linearisation; one
+// workgroup per SpMV tile. A tile of odd length gets a zero high half in its
+// last pair.
 __global__ void __launch_bounds__(kBlock) k_split_bf16(RefineArgs a) {
-	uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-	if (k >= a.nnz) return;
-	const uint32_t slot = a.storeSlot ? a.storeSlot[k] : k;   // the CSR slot stored at position k
-	const uint32_t row = a.slotRow[slot];
-	if (!a.st[a.rowInst[row]].newtonActive) return;
-	M33 m = ldA(a.A, a.stride, k);
-	const float* p = &m.x.x;
-	if (a.cols[slot] == row) {
-		for (int c = 0; c < 9; ++c) a.diagF[(size_t)c * a.n + row] = p[c];
-		a.Ab[k] = make_uint4(0u, 0u, 0u, 0u);
-		a.Ab8[k] = 0;
-		return;
+	const uint32_t t = blockIdx.x;
+	if (!a.st[a.tileInst[t]].newtonActive) return;
+	const uint32_t b0 = a.rowptr[a.tiles[t]], b1 = a.rowptr[a.tiles[t + 1]];
+	const size_t plane = 2 * (size_t)a.pairStride;
+	const uint32_t e0 = 2 * a.pairBase[t];
+	for (uint32_t k = b0 + threadIdx.x; k < b1; k += kBlock) {
+		const uint32_t slot = a.storeSlot ? a.storeSlot[k] : k;   // the CSR slot stored at position k
+		const uint32_t row = a.slotRow[slot];
+		const uint32_t e = e0 + (k - b0);
+		M33 m = ldA(a.A, a.stride, k);
+		const float* p = &m.x.x;
+		const bool diag = a.cols[slot] == row;
+		if (diag)
+			for (int c = 0; c < 9; ++c) a.diagF[(size_t)c * a.n + row] = p[c];
+		for (int c = 0; c < 9; ++c) a.AbP16[c * plane + e] = diag ? (uint16_t)0 : (uint16_t)bf16_bits(p[c]);
+		if (k + 1 == b1 && ((k - b0) & 1u) == 0)
+			for (int c = 0; c < 9; ++c) a.AbP16[c * plane + e + 1] = 0;
 	}
-	uint32_t h[9];
-	for (int c = 0; c < 9; ++c) h[c] = bf16_bits(p[c]);
-	a.Ab[k] = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
-	a.Ab8[k] = (uint16_t)h[8];
 }
 
 // r = pb - pA x in FP64 (FP32 blocks promoted, FP64 x and sums, the
@@ -2082,6 +2107,17 @@ b3g_status build_tiles(Body& b, bool sorted, bool tiles) {
 		}
 		if ((s = dupload(b, b.tiles, t.data(), t.size() * 4)) != B3G_OK) return s;
 		if ((s = dupload(b, b.tileInst, ti.data(), ti.size() * 4)) != B3G_OK) return s;
+		// the BF16 pair planes: each tile's first pair
+		std::vector<uint32_t> pairs(std::max(1u, b.nTiles), 0u);
+		uint64_t total = 0;
+		for (uint32_t x = 0; x < b.nTiles; ++x) {
+			pairs[x] = (uint32_t)total;
+			total += (b.hostRowptr[t[x + 1]] - b.hostRowptr[t[x]] + 1u) / 2u;
+		}
+		if (total > 0x7FFFFFC0ull) { set_error("too many blocks for the BF16 pair planes"); return B3G_UNSUPPORTED; }
+		b.pairStride = (uint32_t)((total + 63u) & ~63ull);
+		if (b.pairStride == 0) b.pairStride = 64;
+		if ((s = dupload(b, b.pairBase, pairs.data(), pairs.size() * 4)) != B3G_OK) return s;
 	}
 	if ((s = dupload(b, b.rowBlk, rb.data(), rb.size() * 4)) != B3G_OK) return s;
 	if ((s = dupload(b, b.blkInst, bi.data(), bi.size() * 4)) != B3G_OK) return s;
@@ -2239,9 +2275,10 @@ static CGArgs cg_args(Body& b) {
 	if (b.reduction == B3G_REDUCE_FP64)
 		fused = b.sched.fused >= 0 ? b.sched.fused != 0 : b.maxInstPartials <= kFuseMaxPartials;
 	a.fused = fused ? 1u : 0u;
-	a.bf16 = b.solver == B3G_SOLVER_REFINED_BF16 && b.Ab.p ? 1u : 0u;
-	a.Ab = dptr<uint4>(b.Ab);
-	a.Ab8 = dptr<uint16_t>(b.Ab8);
+	a.bf16 = b.solver == B3G_SOLVER_REFINED_BF16 && b.AbP.p ? 1u : 0u;
+	a.AbP = dptr<uint32_t>(b.AbP);
+	a.pairBase = dptr<uint32_t>(b.pairBase);
+	a.pairStride = b.pairStride;
 	a.diagF = dptr<float>(b.diagF);
 	a.code = b.sortedA ? dptr<uint32_t>(b.code) : nullptr;
 	a.tileBase = b.sortedA ? dptr<uint32_t>(b.tileBase) : nullptr;
@@ -2561,8 +2598,9 @@ static RefineArgs refine_args(Body& b) {
 	r.partialR = dptr<double>(b.partialR);
 	r.partial = dptr<double>(b.partial);
 	r.st = dptr<SolverState>(b.state);
-	r.Ab = dptr<uint4>(b.Ab);
-	r.Ab8 = dptr<uint16_t>(b.Ab8);
+	r.AbP16 = dptr<uint16_t>(b.AbP);
+	r.pairBase = dptr<uint32_t>(b.pairBase);
+	r.pairStride = b.pairStride;
 	r.diagF = dptr<float>(b.diagF);
 	r.code = b.sortedA ? dptr<uint32_t>(b.code) : nullptr;
 	r.tileBase = b.sortedA ? dptr<uint32_t>(b.tileBase) : nullptr;
@@ -2580,8 +2618,8 @@ static b3g_status ensure_refine(Body& b) {
 	if (need(b.partialR, (size_t)b.partialCount * 8) && (s = dalloc(b, b.partialR, (size_t)b.partialCount * 8)) != B3G_OK)
 		return s;
 	if (b.solver == B3G_SOLVER_REFINED_BF16) {
-		if (need(b.Ab, (size_t)b.nnz * 16) && (s = dalloc(b, b.Ab, (size_t)b.nnz * 16)) != B3G_OK) return s;
-		if (need(b.Ab8, (size_t)b.nnz * 2) && (s = dalloc(b, b.Ab8, (size_t)b.nnz * 2)) != B3G_OK) return s;
+		const size_t pairBytes = (size_t)9 * b.pairStride * 4;
+		if (need(b.AbP, pairBytes) && (s = dalloc(b, b.AbP, pairBytes)) != B3G_OK) return s;
 		if (need(b.diagF, (size_t)b.n * 36) && (s = dalloc(b, b.diagF, (size_t)b.n * 36)) != B3G_OK) return s;
 	}
 	// FP64 right-hand side: tetrahedron forces and (h dfdx) w in double
@@ -2611,7 +2649,7 @@ static b3g_status enqueue_solve_refined(Body& b, uint32_t sweeps) {
 	CGArgs a = cg_args(b);
 	a.pb = dptr<V3>(b.rhsF);
 	a.px = dptr<V3>(b.dx);
-	if (b.solver == B3G_SOLVER_REFINED_BF16) LAUNCH(k_split_bf16, nblk(b.nnz), ra);
+	if (b.solver == B3G_SOLVER_REFINED_BF16) LAUNCH(k_split_bf16, b.nTiles, ra);
 	LAUNCH(k_x64_init, nblk(b.n), ra);
 	for (uint32_t sw = 0; sw < sweeps; ++sw) {
 		LAUNCH(k_resid64, b.nTiles, ra, sw == 0 ? 1 : 0);
