@@ -139,9 +139,8 @@ def autotune(bodies, cache):
     for b in bodies:
         t0 = time.perf_counter()
         chosen, tuned_ms, default_ms = b.autotune(cache)
-        out.append({"chosen": dict(zip(("tile_blocks", "tile_rows", "row_block_rows", "fused"), chosen)),
-                    "default": dict(zip(("tile_blocks", "tile_rows", "row_block_rows", "fused"),
-                                        _lib.DEFAULT_SCHEDULE)),
+        out.append({"chosen": dict(zip(_lib.SCHEDULE_FIELDS, chosen)),
+                    "default": dict(zip(_lib.SCHEDULE_FIELDS, _lib.DEFAULT_SCHEDULE)),
                     "search_ms_per_step": {"chosen": round(tuned_ms, 4), "default": round(default_ms, 4)},
                     "search_s": round(time.perf_counter() - t0, 2)})
         log(f"autotune: {out[-1]}")

@@ -137,16 +137,17 @@ class Schedule(C.Structure):
     """``b3g_schedule`` (include/b3g.h): the PCG's SpMV tile shape, vector row-block size and FP64 sum place."""
 
     _fields_ = [("tile_blocks", C.c_uint32), ("tile_rows", C.c_uint32), ("row_block_rows", C.c_uint32),
-                ("fused", C.c_int32)]
+                ("fused", C.c_int32), ("max_groups", C.c_uint32)]
 
     def as_tuple(self):
-        return (self.tile_blocks, self.tile_rows, self.row_block_rows, self.fused)
+        return (self.tile_blocks, self.tile_rows, self.row_block_rows, self.fused, self.max_groups)
 
     def __repr__(self):
-        return "Schedule(tile_blocks=%d, tile_rows=%d, row_block_rows=%d, fused=%d)" % self.as_tuple()
+        return "Schedule(tile_blocks=%d, tile_rows=%d, row_block_rows=%d, fused=%d, max_groups=%d)" % self.as_tuple()
 
 
-DEFAULT_SCHEDULE = (1024, 256, 1024, -1)
+SCHEDULE_FIELDS = ("tile_blocks", "tile_rows", "row_block_rows", "fused", "max_groups")
+DEFAULT_SCHEDULE = (1024, 256, 1024, -1, 0)
 
 
 class Carry(C.Structure):

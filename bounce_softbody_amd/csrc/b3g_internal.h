@@ -159,6 +159,7 @@ struct Schedule {
 	uint32_t tileRows = 256;       // rows per SpMV tile (at most 256)
 	uint32_t rowBlockRows = 1024;  // rows per vector-kernel block (at most 1024)
 	int fused = -1;                // FP64 mode: -1 by instance size, 0 separate, 1 fused
+	uint32_t maxGroups = 0;        // workgroups per PCG launch at most (0: one per tile or row block)
 };
 
 // A body holds a batch of one or more independent instances (bodies of the

@@ -31,6 +31,45 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t G) {
 	return start + b / 8;
 }
 
+// Work items (tiles or row blocks) [0, count) over the launch's workgroups.
+// Multi = false: one workgroup per item, item = xcd_tile(block) (Xcd) or the
+// block index, checked by every thread (the kernels of a single body).
+// Multi = true: workgroup v = xcd_tile(block) takes items v, v + G, v + 2G, ...
+// (G workgroups), so at any time the workgroups of one XCD work on one run
+// of adjacent items, whose vector gathers share that XCD's L2, as with one
+// workgroup per item. The items whose instance runs -- ok(instance of the
+// item) -- are found 64 at a time with one ballot, so a batch whose instances
+// have mostly converged costs a few loads per workgroup rather than a
+// workgroup per item. ok() and the item's instance are uniform; body(item)
+// may synchronise the workgroup.
+template <bool Multi, bool Xcd, class Ok, class Body>
+__device__ __forceinline__ void for_items(uint32_t count, const uint32_t* itemInst, Ok ok, Body body) {
+	if (!Multi) {
+		const uint32_t t = Xcd ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+		if (ok(itemInst[t])) body(t);
+		return;
+	}
+	const uint32_t G = gridDim.x, v = xcd_tile(blockIdx.x, G);
+	__shared__ uint64_t mask;
+	for (uint32_t p = v; p < count; p += 64 * G) {
+		if (threadIdx.x < 64) {
+			const uint32_t t = p + threadIdx.x * G;
+			const uint64_t m = __ballot(t < count && ok(itemInst[t]));
+			if (threadIdx.x == 0) mask = m;
+		}
+		__syncthreads();
+		const uint64_t w = mask;   // uniform: kept in scalar registers
+		uint64_t m = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(w >> 32)) << 32) |
+		             (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)w);
+		__syncthreads();   // the next pass rewrites the mask
+		while (m) {
+			const uint32_t b = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+			m &= m - 1;
+			body(p + b * G);
+		}
+	}
+}
+
 __device__ __forceinline__ M33 ldA(const float* A, uint32_t stride, uint32_t k) {
 	M33 a;
 	float* p = &a.x.x;
@@ -1471,7 +1510,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble_rows(AsmArgs a) {
 // PCG (sparse_solver.cpp:28-109)
 // ---------------------------------------------------------------------------
 struct CGArgs {
-	uint32_t n, nnz, nTiles;
+	uint32_t n, nnz, nTiles, nRowBlk;
 	uint32_t stride;            // plane stride of A and length of cols (nnz rounded up to 4)
 	const uint32_t* tiles;
 	const uint32_t* tileInst;
@@ -1631,26 +1670,28 @@ __device__ __forceinline__ bool no_prologue() { return true; }
 
 // r = b - A x ; s = invM r ; delta = r.s   (mode kFinInit: CG start, kFinDelta: true-residual
 // refresh after iteration k, which ran only if log[k].active)
+template <bool Multi>
 __global__ void __launch_bounds__(kBlock) k_cg_residual(CGArgs a, int mode, uint32_t k) {
-	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
-	const uint32_t inst = a.tileInst[t];
-	const SolverState& st = a.st[inst];
-	if (!st.newtonActive) return;
-	if (mode == kFinDelta && !a.log[(size_t)inst * a.logStride + k].active) return;
 	__shared__ float lds[3 * kTileCap];
-	uint32_t row;
-	V3 ax;
-	tile_rows(a, t, row, [&](uint32_t c) { return a.px[c]; }, lds, no_prologue, ax);
-	float part = 0.0f;
-	const bool has = row < a.tiles[t + 1];
-	if (has) {
-		V3 r = a.pb[row] - ax;
-		V3 s = diag_apply(a.invM[row], r);
-		a.r[row] = r;
-		a.s[row] = s;
-		part = dot(r, s);
-	}
-	block_partial(part, has, row, a.partial, a.terms, st.seq != 0, t);
+	auto ok = [&](uint32_t inst) {
+		return a.st[inst].newtonActive && (mode != kFinDelta || a.log[(size_t)inst * a.logStride + k].active);
+	};
+	for_items<Multi, true>(a.nTiles, a.tileInst, ok, [&](uint32_t t) {
+		const SolverState& st = a.st[a.tileInst[t]];
+		uint32_t row;
+		V3 ax;
+		tile_rows(a, t, row, [&](uint32_t c) { return a.px[c]; }, lds, no_prologue, ax);
+		float part = 0.0f;
+		const bool has = row < a.tiles[t + 1];
+		if (has) {
+			V3 r = a.pb[row] - ax;
+			V3 s = diag_apply(a.invM[row], r);
+			a.r[row] = r;
+			a.s[row] = s;
+			part = dot(r, s);
+		}
+		block_partial(part, has, row, a.partial, a.terms, st.seq != 0, t);
+	});
 }
 
 // PCG iteration k: q = A d with d = s + beta_k d_old formed on the fly
@@ -1659,56 +1700,56 @@ __global__ void __launch_bounds__(kBlock) k_cg_residual(CGArgs a, int mode, uint
 // partials of the reduction that ended iteration k - 1 (row blocks, or tiles
 // after a residual refresh: sel); the instance's first tile records them.
 // force = 1 runs it whatever the PCG state (timing pass on PCG scratch).
+template <bool Multi>
 __global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, uint32_t k, int sel, int force) {
-	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
-	const uint32_t inst = a.tileInst[t];
-	SolverState& st = a.st[inst];
-	CGLog* log = a.log + (size_t)inst * a.logStride;
 	__shared__ float lds[3 * kTileCap];
-	if (!force && !st.newtonActive) return;
-	float beta = 0.0f;
-	if (k > 0) {
-		if (!force && !log[k - 1].active) return;
-		if (!a.fused) {
-			if (!force && !log[k].active) return;
-			beta = log[k].beta;
-		}
-	} else if (!force && !log[0].active) {
-		return;
-	}
-	// fused mode: delta_k / beta_k from the partials
-	auto prologue = [&]() -> bool {
-		if (k == 0 || !a.fused) return true;
-		const uint32_t* R = a.range + 6 * (size_t)inst;
-		const float v = (float)wg_sum(a.partial, R[sel], R[sel + 1]);
-		const float prev = log[k - 1].delta;
-		beta = v / prev;
-		const bool act = k != st.maxIt && !(v <= st.eps2_sub * st.delta0);
-		if (!force && t == R[0] && threadIdx.x == 0) {
-			log[k].delta = v;
-			log[k].beta = beta;
-			log[k].active = act ? 1u : 0u;
-			st.it = k;
-		}
-		return force || act;
+	auto ok = [&](uint32_t inst) -> bool {
+		if (force) return true;
+		const CGLog* log = a.log + (size_t)inst * a.logStride;
+		if (!a.st[inst].newtonActive) return false;
+		if (k > 0) return log[k - 1].active && (a.fused || log[k].active);
+		return log[0].active != 0;
 	};
-	// d = s + beta d_old, and d = s at k = 0: there dsrc is s and beta is +0,
-	// and s + (+0) s = s exactly for finite s (the zero product carries the
-	// sign of s), so every iteration gathers the same way, without a branch
-	const V3* dsrc = k == 0 ? a.s : a.dold;
-	auto dval = [&](uint32_t c) { return a.s[c] + beta * dsrc[c]; };
-	uint32_t row;
-	V3 q;
-	if (!tile_rows(a, t, row, dval, lds, prologue, q)) return;
-	float part = 0.0f;
-	const bool has = row < a.tiles[t + 1];
-	if (has) {
-		V3 d = dval(row);
-		a.dnew[row] = d;
-		a.q[row] = q;
-		part = dot(d, q);
-	}
-	block_partial(part, has, row, a.partialQ, a.terms, st.seq != 0, t);
+	for_items<Multi, true>(a.nTiles, a.tileInst, ok, [&](uint32_t t) {
+		const uint32_t inst = a.tileInst[t];
+		SolverState& st = a.st[inst];
+		CGLog* log = a.log + (size_t)inst * a.logStride;
+		float beta = 0.0f;
+		if (k > 0 && !a.fused) beta = log[k].beta;
+		// fused mode: delta_k / beta_k from the partials
+		auto prologue = [&]() -> bool {
+			if (k == 0 || !a.fused) return true;
+			const uint32_t* R = a.range + 6 * (size_t)inst;
+			const float v = (float)wg_sum(a.partial, R[sel], R[sel + 1]);
+			const float prev = log[k - 1].delta;
+			beta = v / prev;
+			const bool act = k != st.maxIt && !(v <= st.eps2_sub * st.delta0);
+			if (!force && t == R[0] && threadIdx.x == 0) {
+				log[k].delta = v;
+				log[k].beta = beta;
+				log[k].active = act ? 1u : 0u;
+				st.it = k;
+			}
+			return force || act;
+		};
+		// d = s + beta d_old, and d = s at k = 0: there dsrc is s and beta is +0,
+		// and s + (+0) s = s exactly for finite s (the zero product carries the
+		// sign of s), so every iteration gathers the same way, without a branch
+		const V3* dsrc = k == 0 ? a.s : a.dold;
+		auto dval = [&](uint32_t c) { return a.s[c] + beta * dsrc[c]; };
+		uint32_t row;
+		V3 q;
+		if (!tile_rows(a, t, row, dval, lds, prologue, q)) return;
+		float part = 0.0f;
+		const bool has = row < a.tiles[t + 1];
+		if (has) {
+			V3 d = dval(row);
+			a.dnew[row] = d;
+			a.q[row] = q;
+			part = dot(d, q);
+		}
+		block_partial(part, has, row, a.partialQ, a.terms, st.seq != 0, t);
+	});
 }
 
 // alpha_k of a vector kernel of iteration k: fused mode sums the SpMV's d.q
@@ -1723,62 +1764,70 @@ __device__ __forceinline__ float cg_alpha(const CGArgs& a, uint32_t inst, uint32
 }
 
 // x += alpha d ; r -= alpha q ; s = invM r ; partials of r.s (iteration k)
+template <bool Multi>
 __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k, int force) {
-	const uint32_t blk = blockIdx.x, inst = a.blkInst[blk];
-	if (!force && (!a.st[inst].newtonActive || !a.log[(size_t)inst * a.logStride + k].active)) return;
-	const uint32_t r0 = a.rowBlk[blk], r1 = a.rowBlk[blk + 1];
-	const bool seq = a.st[inst].seq != 0;
-	// every input of the thread's rows is loaded before the first store (the
-	// stores could alias them as far as the compiler knows), and before the
-	// alpha prologue, whose partial sums then overlap these loads; a row past
-	// the block loads row r0 and stores nothing
-	V3 d[kRowsPerThread], x[kRowsPerThread], r[kRowsPerThread], q[kRowsPerThread], im[kRowsPerThread];
+	auto ok = [&](uint32_t inst) {
+		return force || (a.st[inst].newtonActive && a.log[(size_t)inst * a.logStride + k].active);
+	};
+	for_items<Multi, false>(a.nRowBlk, a.blkInst, ok, [&](uint32_t blk) {
+		const uint32_t inst = a.blkInst[blk];
+		const uint32_t r0 = a.rowBlk[blk], r1 = a.rowBlk[blk + 1];
+		const bool seq = a.st[inst].seq != 0;
+		// every input of the thread's rows is loaded before the first store (the
+		// stores could alias them as far as the compiler knows), and before the
+		// alpha prologue, whose partial sums then overlap these loads; a row past
+		// the block loads row r0 and stores nothing
+		V3 d[kRowsPerThread], x[kRowsPerThread], r[kRowsPerThread], q[kRowsPerThread], im[kRowsPerThread];
 #pragma unroll
-	for (int u = 0; u < kRowsPerThread; ++u) {
-		const uint32_t i = r0 + threadIdx.x + u * kBlock, ii = i < r1 ? i : r0;
-		d[u] = a.dnew[ii];
-		x[u] = a.px[ii];
-		r[u] = a.r[ii];
-		q[u] = a.q[ii];
-		im[u] = a.invM[ii];
-	}
-	const float alpha = cg_alpha(a, inst, blk, k);
-	double part = 0.0;
+		for (int u = 0; u < kRowsPerThread; ++u) {
+			const uint32_t i = r0 + threadIdx.x + u * kBlock, ii = i < r1 ? i : r0;
+			d[u] = a.dnew[ii];
+			x[u] = a.px[ii];
+			r[u] = a.r[ii];
+			q[u] = a.q[ii];
+			im[u] = a.invM[ii];
+		}
+		const float alpha = cg_alpha(a, inst, blk, k);
+		double part = 0.0;
 #pragma unroll
-	for (int u = 0; u < kRowsPerThread; ++u) {
-		const uint32_t i = r0 + threadIdx.x + u * kBlock;
-		if (i >= r1) break;
-		const V3 rn = r[u] - alpha * q[u];
-		const V3 sn = diag_apply(im[u], rn);
-		a.px[i] = x[u] + alpha * d[u];
-		a.r[i] = rn;
-		a.s[i] = sn;
-		const float term = dot(rn, sn);
-		if (seq) a.terms[i] = term;
-		part += (double)term;
-	}
-	block_store(part, a.partial, blk);
+		for (int u = 0; u < kRowsPerThread; ++u) {
+			const uint32_t i = r0 + threadIdx.x + u * kBlock;
+			if (i >= r1) break;
+			const V3 rn = r[u] - alpha * q[u];
+			const V3 sn = diag_apply(im[u], rn);
+			a.px[i] = x[u] + alpha * d[u];
+			a.r[i] = rn;
+			a.s[i] = sn;
+			const float term = dot(rn, sn);
+			if (seq) a.terms[i] = term;
+			part += (double)term;
+		}
+		block_store(part, a.partial, blk);
+	});
 }
 
 // x += alpha d (iterations that refresh the true residual)
+template <bool Multi>
 __global__ void __launch_bounds__(kBlock) k_cg_update_x(CGArgs a, uint32_t k) {
-	const uint32_t blk = blockIdx.x, inst = a.blkInst[blk];
-	if (!a.st[inst].newtonActive || !a.log[(size_t)inst * a.logStride + k].active) return;
-	const uint32_t r0 = a.rowBlk[blk], r1 = a.rowBlk[blk + 1];
-	V3 d[kRowsPerThread], x[kRowsPerThread];
+	auto ok = [&](uint32_t inst) { return a.st[inst].newtonActive && a.log[(size_t)inst * a.logStride + k].active; };
+	for_items<Multi, false>(a.nRowBlk, a.blkInst, ok, [&](uint32_t blk) {
+		const uint32_t inst = a.blkInst[blk];
+		const uint32_t r0 = a.rowBlk[blk], r1 = a.rowBlk[blk + 1];
+		V3 d[kRowsPerThread], x[kRowsPerThread];
 #pragma unroll
-	for (int u = 0; u < kRowsPerThread; ++u) {
-		const uint32_t i = r0 + threadIdx.x + u * kBlock, ii = i < r1 ? i : r0;
-		d[u] = a.dnew[ii];
-		x[u] = a.px[ii];
-	}
-	const float alpha = cg_alpha(a, inst, blk, k);
+		for (int u = 0; u < kRowsPerThread; ++u) {
+			const uint32_t i = r0 + threadIdx.x + u * kBlock, ii = i < r1 ? i : r0;
+			d[u] = a.dnew[ii];
+			x[u] = a.px[ii];
+		}
+		const float alpha = cg_alpha(a, inst, blk, k);
 #pragma unroll
-	for (int u = 0; u < kRowsPerThread; ++u) {
-		const uint32_t i = r0 + threadIdx.x + u * kBlock;
-		if (i >= r1) break;
-		a.px[i] = x[u] + alpha * d[u];
-	}
+		for (int u = 0; u < kRowsPerThread; ++u) {
+			const uint32_t i = r0 + threadIdx.x + u * kBlock;
+			if (i >= r1) break;
+			a.px[i] = x[u] + alpha * d[u];
+		}
+	});
 }
 
 // v += dv ; x = x0 + h v + y ; error = |dv|^2 (sparse_force_solver.cpp:193-223)
@@ -1996,10 +2045,31 @@ __global__ void __launch_bounds__(kBlock) k_x64_finish(RefineArgs a) {
 // ---------------------------------------------------------------------------
 static inline uint32_t nblk(uint32_t n) { return (n + kBlock - 1) / kBlock; }
 
+// Workgroups of a PCG launch over `items` tiles or row blocks (for_items):
+// one per item, or the schedule's maxGroups when that is at most half the
+// items. One per item is the default: on the 4096-body batch, 2048 ranged
+// workgroups took 84 ms per step against 69 ms (a workgroup that starts
+// late runs its whole range late, and the range loop holds more registers),
+// more than the converged tiles' early returns cost.
+static inline uint32_t groups(const Body& b, uint32_t items) {
+	const uint32_t cap = b.sched.maxGroups ? b.sched.maxGroups : items;
+	if (items / 2 < cap) return std::max(1u, items);
+	return cap;
+}
+
 #define LAUNCH(kernel, grid, ...)                                                                    \
 	do {                                                                                             \
 		if ((grid) > 0) hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, b.stream, __VA_ARGS__); \
 		B3G_HIP(hipGetLastError());                                                                  \
+	} while (0)
+
+// A PCG kernel over `items` tiles or row blocks: one workgroup per item, or
+// groups(b, items) workgroups that each take a range (for_items).
+#define LAUNCH_ITEMS(kernel, items, ...)                                     \
+	do {                                                                     \
+		const uint32_t n_ = (items), g_ = groups(b, n_);                     \
+		if (g_ < n_) LAUNCH(kernel<true>, g_, __VA_ARGS__);                  \
+		else LAUNCH(kernel<false>, n_, __VA_ARGS__);                         \
 	} while (0)
 
 // Row decompositions, both cut at instance boundaries so that every tile /
@@ -2246,6 +2316,7 @@ static CGArgs cg_args(Body& b) {
 	a.nnz = b.nnz;
 	a.stride = b.aStride;
 	a.nTiles = b.nTiles;
+	a.nRowBlk = b.nRowBlk;
 	a.tiles = dptr<uint32_t>(b.tiles);
 	a.tileInst = dptr<uint32_t>(b.tileInst);
 	a.rowBlk = dptr<uint32_t>(b.rowBlk);
@@ -2529,13 +2600,13 @@ static b3g_status enqueue_pcg_iterations(Body& b, const CGArgs& a, uint32_t maxI
 			ak.dnew = (k & 1) ? d0 : d1;
 			const int sel = k > 0 ? delta_range(k - 1) : kRangeTiles;
 			if (!fused && k > 0) FINALIZE(b.partial, sel, kFinDelta, k, kGateIter, k - 1);
-			LAUNCH(k_cg_spmv_dq, b.nTiles, ak, k, sel, 0);
+			LAUNCH_ITEMS(k_cg_spmv_dq, b.nTiles, ak, k, sel, 0);
 			if (!fused) FINALIZE(b.partialQ, kRangeTiles, kFinAlpha, k, kGateIter, k);
 			if (k % 50 == 0) {
-				LAUNCH(k_cg_update_x, rb, ak, k);
-				LAUNCH(k_cg_residual, b.nTiles, ak, (int)kFinDelta, k);
+				LAUNCH_ITEMS(k_cg_update_x, rb, ak, k);
+				LAUNCH_ITEMS(k_cg_residual, b.nTiles, ak, (int)kFinDelta, k);
 			} else {
-				LAUNCH(k_cg_update, rb, ak, k, 0);
+				LAUNCH_ITEMS(k_cg_update, rb, ak, k, 0);
 			}
 		}
 		chunk = kExitChunk;
@@ -2569,7 +2640,7 @@ b3g_status enqueue_solve_cg(Body& b, uint32_t maxIt, float eps) {
 	b3g_status s = ensure_cg_log(b, maxIt + 1);
 	if (s != B3G_OK) return s;
 	CGArgs a = cg_args(b);
-	LAUNCH(k_cg_residual, b.nTiles, a, (int)kFinInit, 0u);
+	LAUNCH_ITEMS(k_cg_residual, b.nTiles, a, (int)kFinInit, 0u);
 	FINALIZE(b.partial, kRangeTiles, kFinInit, 0, kGateNewton, 0);
 	return enqueue_pcg_iterations(b, a, maxIt);
 }
@@ -2688,7 +2759,13 @@ b3g_status time_pcg_spmv(Body& b, uint32_t launches, double* mean_ms) {
 			CGArgs ak = a;
 			ak.dold = (k & 1) ? dptr<V3>(b.d1) : dptr<V3>(b.d0);
 			ak.dnew = (k & 1) ? dptr<V3>(b.d0) : dptr<V3>(b.d1);
-			hipLaunchKernelGGL(k_cg_spmv_dq, dim3(b.nTiles), dim3(kBlock), 0, b.stream, ak, 1u, (int)kRangeBlocks, 1);
+			{
+				const uint32_t g = groups(b, b.nTiles);
+				if (g < b.nTiles)
+					hipLaunchKernelGGL(k_cg_spmv_dq<true>, dim3(g), dim3(kBlock), 0, b.stream, ak, 1u, (int)kRangeBlocks, 1);
+				else
+					hipLaunchKernelGGL(k_cg_spmv_dq<false>, dim3(g), dim3(kBlock), 0, b.stream, ak, 1u, (int)kRangeBlocks, 1);
+			}
 			if (hipGetLastError() != hipSuccess) st = B3G_HIP_ERROR;
 		}
 		if (st != B3G_OK) break;
@@ -2756,7 +2833,13 @@ b3g_status time_pcg_spmv_cold(Body& b, uint32_t launches, size_t evictBytes, dou
 			CGArgs ak = a;
 			ak.dold = (k & 1) ? dptr<V3>(b.d1) : dptr<V3>(b.d0);
 			ak.dnew = (k & 1) ? dptr<V3>(b.d0) : dptr<V3>(b.d1);
-			hipLaunchKernelGGL(k_cg_spmv_dq, dim3(b.nTiles), dim3(kBlock), 0, b.stream, ak, 1u, (int)kRangeBlocks, 1);
+			{
+				const uint32_t g = groups(b, b.nTiles);
+				if (g < b.nTiles)
+					hipLaunchKernelGGL(k_cg_spmv_dq<true>, dim3(g), dim3(kBlock), 0, b.stream, ak, 1u, (int)kRangeBlocks, 1);
+				else
+					hipLaunchKernelGGL(k_cg_spmv_dq<false>, dim3(g), dim3(kBlock), 0, b.stream, ak, 1u, (int)kRangeBlocks, 1);
+			}
 			ok = hipGetLastError() == hipSuccess;
 		}
 		ok = ok && hipEventRecord(e[1], b.stream) == hipSuccess && hipEventRecord(e[2], b.stream) == hipSuccess;
@@ -2795,9 +2878,9 @@ b3g_status time_pcg_iteration(Body& b, uint32_t reps, double* ms) {
 			CGArgs ak = a;
 			ak.dold = (r & 1) ? dptr<V3>(b.d1) : dptr<V3>(b.d0);
 			ak.dnew = (r & 1) ? dptr<V3>(b.d0) : dptr<V3>(b.d1);
-			LAUNCH(k_cg_spmv_dq, b.nTiles, ak, 1u, (int)kRangeBlocks, 1);
+			LAUNCH_ITEMS(k_cg_spmv_dq, b.nTiles, ak, 1u, (int)kRangeBlocks, 1);
 			if (!a.fused) FINALIZE(b.partialQ, kRangeTiles, kFinAlpha, 1, kGateIter, 1);
-			LAUNCH(k_cg_update, b.nRowBlk, ak, 1u, 1);
+			LAUNCH_ITEMS(k_cg_update, b.nRowBlk, ak, 1u, 1);
 			if (!a.fused) FINALIZE(b.partial, kRangeBlocks, kFinDelta, 2, kGateIter, 1);
 		}
 		B3G_HIP(hipEventRecord(e1, b.stream));
@@ -2814,7 +2897,7 @@ b3g_status time_pcg_iteration(Body& b, uint32_t reps, double* ms) {
 }
 
 // Candidate schedules: SpMV tile shapes (blocks, rows), vector row blocks and
-// the place of the FP64 sums, each timed by `timer` (or as PCG iterations on
+// the place of the FP64 sums, then workgroup caps, each timed by `timer` (or as PCG iterations on
 // the body's own system). The default is timed first; a candidate faster
 // than it is timed again against it, alternately, and kept only if it is
 // faster by more than 1% both times. Any captured step is dropped with each change of tiles.
@@ -2856,6 +2939,16 @@ b3g_status autotune(Body& b, const std::function<b3g_status(double*)>& timer, do
 				if (ms < bestMs) { bestMs = ms; best = b.sched; }
 			}
 		}
+	}
+	// the workgroup cap for the best shape (one per item, or 1024 / 2048 / 4096)
+	for (uint32_t cap : {0xFFFFFFFFu, 1024u, 2048u, 4096u}) {
+		Schedule sc = best;
+		sc.maxGroups = cap == 0xFFFFFFFFu ? std::max(1u, std::max(b.nTiles, b.nRowBlk)) : cap;
+		if (sc.maxGroups < 256) continue;
+		if ((s = apply(sc)) != B3G_OK) return s;
+		double ms = 0.0;
+		if ((s = timeIt(&ms)) != B3G_OK) return s;
+		if (ms < bestMs) { bestMs = ms; best = sc; }
 	}
 	// the confirmation's means are reported (a search minimum is biased low)
 	bool keep = bestMs < dfltMs;

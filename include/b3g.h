@@ -461,13 +461,17 @@ b3g_status b3g_set_early_exit(b3g_body body, int enabled);
  * by instance size as B3G_REDUCE_FP64 does). Every schedule computes each row
  * and block as the reference does; tiles and row blocks only group the FP64
  * partial sums of the dot products, whose rounding to FP32 then agrees in
- * all but rare ties. Default: 1024, 256, 1024, -1.
+ * all but rare ties. max_groups caps the workgroups of a PCG launch (0: one
+ * per tile or row block); under a cap of at most half the tiles (row blocks)
+ * each workgroup takes every G-th one and skips, 64 at a time, those of
+ * converged instances. Default: 1024, 256, 1024, -1, 0.
  */
 typedef struct b3g_schedule {
 	uint32_t tile_blocks;
 	uint32_t tile_rows;
 	uint32_t row_block_rows;
 	int32_t fused;
+	uint32_t max_groups;
 } b3g_schedule;
 
 b3g_status b3g_get_schedule(b3g_body body, b3g_schedule* schedule);
@@ -475,8 +479,8 @@ b3g_status b3g_set_schedule(b3g_body body, const b3g_schedule* schedule);
 
 /*
  * Autotuner (no counterpart in the reference): times candidate schedules --
- * five SpMV tile shapes, three row-block sizes, both places of the FP64 sums
- * -- and keeps the fastest; a candidate that beats the default is timed
+ * five SpMV tile shapes, three row-block sizes, both places of the FP64 sums,
+ * then four workgroup caps for the best of those -- and keeps the fastest; a candidate that beats the default is timed
  * against it twice more and kept only if it is faster by more than 1% both
  * times. After a
  * b3g_step each timing runs whole steps with that step's parameters from the

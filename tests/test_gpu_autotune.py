@@ -14,7 +14,8 @@ from bounce_softbody_amd.body import Body
 
 pytestmark = pytest.mark.gpu
 
-SCHEDULES = [(1024, 256, 1024, 0), (512, 128, 256, 1), (32, 16, 64, 0), (256, 64, 512, 1), (1024, 256, 64, -1)]
+SCHEDULES = [(1024, 256, 1024, 0, 0), (512, 128, 256, 1, 0), (32, 16, 64, 0, 256), (256, 64, 512, 1, 0),
+             (1024, 256, 64, -1, 300)]
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -38,13 +39,13 @@ def test_default_schedule_and_validation():
     b = Body(configs.tet_grid(3).arrays())
     assert b.schedule() == _lib.DEFAULT_SCHEDULE
     for bad in [(16, 256, 1024, -1), (1025, 256, 1024, -1), (1024, 8, 1024, -1), (1024, 257, 1024, -1),
-                (1024, 256, 32, -1), (1024, 256, 2048, -1), (1024, 256, 1024, 2)]:
+                (1024, 256, 32, -1), (1024, 256, 2048, -1), (1024, 256, 1024, 2), (1024, 256, 1024, -1, 100)]:
         with pytest.raises(_lib.B3gError) as e:
             b.set_schedule(*bad)
         assert e.value.status == _lib.B3G_INVALID_ARGUMENT
     assert b.schedule() == _lib.DEFAULT_SCHEDULE
-    b.set_schedule(512, 128, 256, 1)
-    assert b.schedule() == (512, 128, 256, 1)
+    b.set_schedule(512, 128, 256, 1, 4096)
+    assert b.schedule() == (512, 128, 256, 1, 4096)
 
 
 @pytest.mark.parametrize("sched", SCHEDULES)
@@ -56,6 +57,27 @@ def test_steps_under_a_schedule_match_the_default(sched):
         ref.Step(configs.DT, 1 + k % 2, 40, 0.0, 1e-5)
         b.Step(configs.DT, 1 + k % 2, 40, 0.0, 1e-5)
         assert close(ref, b), f"{sched}: step {k}"
+
+
+@pytest.mark.parametrize("fused", [0, 1])
+def test_ranges_per_workgroup_are_bit_identical(fused):
+    """max_groups below the item count gives each PCG workgroup a range of tiles, skipping converged instances 64
+    at a time: the same tiles and partials as one workgroup per tile, so the same bits."""
+    # more than 512 tiles of 16 rows and row blocks of 64 rows, so both kernel kinds take ranges
+    arrs = [configs.tet_grid(k).arrays() for k in (10, 3, 32, 6, 10, 4, 8, 7)]
+    arrs += [configs.PARITY_SCENES[n]().arrays() for n in ("drop_on_box", "cloth_shapes")]
+    assert sum(len(a["p_type"]) for a in arrs) // 64 >= 512
+    one, ranged = Body(arrs), Body(arrs)
+    one.set_schedule(1024, 16, 64, fused, 0)
+    ranged.set_schedule(1024, 16, 64, fused, 256)
+    for k in range(6):
+        it, stol = (1 + k % 2, 1e-5 if k % 3 else 1e-9)
+        one.Step(configs.DT, it, 40, 0.0, stol)
+        ranged.Step(configs.DT, it, 40, 0.0, stol)
+        xa, va = one.state()
+        xb, vb = ranged.state()
+        assert np.array_equal(xa, xb) and np.array_equal(va, vb), k
+        assert all(np.array_equal(p[0], q[0]) for p, q in zip(one.instance_stats(), ranged.instance_stats())), k
 
 
 def test_set_schedule_keeps_the_system_and_stats():
@@ -107,8 +129,8 @@ def test_autotune_keeps_state_and_caches_its_choice(tmp_path):
     chosen, tuned_ms, default_ms = b.autotune(cache)   # times whole steps from this state, then restores it
     assert 0 < tuned_ms <= default_ms
     assert b.schedule() == chosen
-    tb, tr, rb, fu = chosen
-    assert 32 <= tb <= 1024 and 16 <= tr <= 256 and 64 <= rb <= 1024 and fu in (-1, 0, 1)
+    tb, tr, rb, fu, mg = chosen
+    assert 32 <= tb <= 1024 and 16 <= tr <= 256 and 64 <= rb <= 1024 and fu in (-1, 0, 1) and (mg == 0 or mg >= 256)
     assert all(np.array_equal(p, q) for p, q in zip(before, b.system()))
     x1, v1 = b.state()
     assert np.array_equal(x0, x1) and np.array_equal(v0, v1)
