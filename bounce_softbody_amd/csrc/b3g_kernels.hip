@@ -1094,7 +1094,10 @@ __global__ void __launch_bounds__(kBlock) k_friction(ContactArgs a) {
 // per slot / row adds its own contributions in the reference's order. The
 // sums are the sequential ones of the reference; only the loads run wide.
 // ---------------------------------------------------------------------------
-constexpr int kAsmCap = 1024;      // contributions per LDS chunk of the slot assembly (36 KB)
+// Contributions per LDS chunk of the slot assembly (18 KB). Alone the kernel
+// runs as fast with 1024 (36 KB); with two bodies stepping on two streams,
+// 512 left room for the other stream's kernels and ran 14-24 % faster.
+constexpr int kAsmCap = 512;
 
 struct AsmArgs {
 	uint32_t n, nnz, hasContacts;
@@ -1654,23 +1657,21 @@ __device__ __forceinline__ bool tile_rows_(const CGArgs& a, uint32_t t, uint32_t
 	return true;
 }
 
-// The storage is a property of the body, uniform over the launch: one
-// instantiation per storage keeps each loop free of the other's branches.
-template <class Gather, class Before>
+// The storage of pA (St: bit 0 column-sorted chunks, bit 1 BF16 pair planes)
+// is a property of the body, uniform over a launch, and each kernel that
+// reads pA is instantiated per storage (tile_storage picks one at launch):
+// one kernel with every storage inside ran the FP32 step 3.5 % slower.
+template <int St, class Gather, class Before>
 __device__ __forceinline__ bool tile_rows(const CGArgs& a, uint32_t t, uint32_t& row, Gather gather, float* lds,
                                           Before before, V3& acc) {
-	if (a.bf16)
-		return a.code ? tile_rows_<true, true>(a, t, row, gather, lds, before, acc)
-		              : tile_rows_<false, true>(a, t, row, gather, lds, before, acc);
-	return a.code ? tile_rows_<true, false>(a, t, row, gather, lds, before, acc)
-	              : tile_rows_<false, false>(a, t, row, gather, lds, before, acc);
+	return tile_rows_<(St & 1) != 0, (St & 2) != 0>(a, t, row, gather, lds, before, acc);
 }
 
 __device__ __forceinline__ bool no_prologue() { return true; }
 
 // r = b - A x ; s = invM r ; delta = r.s   (mode kFinInit: CG start, kFinDelta: true-residual
 // refresh after iteration k, which ran only if log[k].active)
-template <bool Multi>
+template <bool Multi, int St>
 __global__ void __launch_bounds__(kBlock) k_cg_residual(CGArgs a, int mode, uint32_t k) {
 	__shared__ float lds[3 * kTileCap];
 	auto ok = [&](uint32_t inst) {
@@ -1680,7 +1681,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_residual(CGArgs a, int mode, uint
 		const SolverState& st = a.st[a.tileInst[t]];
 		uint32_t row;
 		V3 ax;
-		tile_rows(a, t, row, [&](uint32_t c) { return a.px[c]; }, lds, no_prologue, ax);
+		tile_rows<St>(a, t, row, [&](uint32_t c) { return a.px[c]; }, lds, no_prologue, ax);
 		float part = 0.0f;
 		const bool has = row < a.tiles[t + 1];
 		if (has) {
@@ -1700,7 +1701,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_residual(CGArgs a, int mode, uint
 // partials of the reduction that ended iteration k - 1 (row blocks, or tiles
 // after a residual refresh: sel); the instance's first tile records them.
 // force = 1 runs it whatever the PCG state (timing pass on PCG scratch).
-template <bool Multi>
+template <bool Multi, int St>
 __global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, uint32_t k, int sel, int force) {
 	__shared__ float lds[3 * kTileCap];
 	auto ok = [&](uint32_t inst) -> bool {
@@ -1739,7 +1740,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, uint32_t k, int
 		auto dval = [&](uint32_t c) { return a.s[c] + beta * dsrc[c]; };
 		uint32_t row;
 		V3 q;
-		if (!tile_rows(a, t, row, dval, lds, prologue, q)) return;
+		if (!tile_rows<St>(a, t, row, dval, lds, prologue, q)) return;
 		float part = 0.0f;
 		const bool has = row < a.tiles[t + 1];
 		if (has) {
@@ -1763,8 +1764,11 @@ __device__ __forceinline__ float cg_alpha(const CGArgs& a, uint32_t inst, uint32
 	return alpha;
 }
 
-// x += alpha d ; r -= alpha q ; s = invM r ; partials of r.s (iteration k)
-template <bool Multi>
+// x += alpha d ; r -= alpha q ; s = invM r ; partials of r.s (iteration k).
+// RPT rows per thread: row blocks of at most RPT x 256 rows (a thread adds its
+// rows r0 + t, r0 + t + 256, ... in that order, whatever RPT, so the partials
+// depend only on the block size); fewer rows per thread hold fewer registers.
+template <bool Multi, int RPT>
 __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k, int force) {
 	auto ok = [&](uint32_t inst) {
 		return force || (a.st[inst].newtonActive && a.log[(size_t)inst * a.logStride + k].active);
@@ -1777,9 +1781,9 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k, int 
 		// stores could alias them as far as the compiler knows), and before the
 		// alpha prologue, whose partial sums then overlap these loads; a row past
 		// the block loads row r0 and stores nothing
-		V3 d[kRowsPerThread], x[kRowsPerThread], r[kRowsPerThread], q[kRowsPerThread], im[kRowsPerThread];
+		V3 d[RPT], x[RPT], r[RPT], q[RPT], im[RPT];
 #pragma unroll
-		for (int u = 0; u < kRowsPerThread; ++u) {
+		for (int u = 0; u < RPT; ++u) {
 			const uint32_t i = r0 + threadIdx.x + u * kBlock, ii = i < r1 ? i : r0;
 			d[u] = a.dnew[ii];
 			x[u] = a.px[ii];
@@ -1790,7 +1794,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k, int 
 		const float alpha = cg_alpha(a, inst, blk, k);
 		double part = 0.0;
 #pragma unroll
-		for (int u = 0; u < kRowsPerThread; ++u) {
+		for (int u = 0; u < RPT; ++u) {
 			const uint32_t i = r0 + threadIdx.x + u * kBlock;
 			if (i >= r1) break;
 			const V3 rn = r[u] - alpha * q[u];
@@ -1865,12 +1869,13 @@ __global__ void __launch_bounds__(kBlock) k_clear_external(uint32_t n, V3* force
 }
 
 // Plain SpMV y = A x (diagnostics / operator-level entry point), reference row order.
+template <int St>
 __global__ void __launch_bounds__(kBlock) k_spmv(CGArgs a, const V3* xin, V3* yout) {
 	__shared__ float lds[3 * kTileCap];
 	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
 	uint32_t row;
 	V3 y;
-	tile_rows(a, t, row, [&](uint32_t c) { return xin[c]; }, lds, no_prologue, y);
+	tile_rows<St>(a, t, row, [&](uint32_t c) { return xin[c]; }, lds, no_prologue, y);
 	if (row < a.tiles[t + 1]) yout[row] = y;
 }
 
@@ -2070,6 +2075,55 @@ static inline uint32_t groups(const Body& b, uint32_t items) {
 		const uint32_t n_ = (items), g_ = groups(b, n_);                     \
 		if (g_ < n_) LAUNCH(kernel<true>, g_, __VA_ARGS__);                  \
 		else LAUNCH(kernel<false>, n_, __VA_ARGS__);                         \
+	} while (0)
+
+// The vector update of iteration k with the fewest rows per thread that
+// cover the schedule's row blocks.
+template <int RPT>
+static b3g_status launch_update_rpt(Body& b, const CGArgs& a, uint32_t k, int force) {
+	const uint32_t n = b.nRowBlk, g = groups(b, n);
+	if (n == 0) return B3G_OK;
+	if (g < n) hipLaunchKernelGGL((k_cg_update<true, RPT>), dim3(g), dim3(kBlock), 0, b.stream, a, k, force);
+	else hipLaunchKernelGGL((k_cg_update<false, RPT>), dim3(n), dim3(kBlock), 0, b.stream, a, k, force);
+	B3G_HIP(hipGetLastError());
+	return B3G_OK;
+}
+
+static b3g_status launch_update(Body& b, const CGArgs& a, uint32_t k, int force) {
+	const uint32_t rows = std::min(b.sched.rowBlockRows, kRowBlkRows);
+	if (rows <= (uint32_t)kBlock) return launch_update_rpt<1>(b, a, k, force);
+	if (rows <= 2u * kBlock) return launch_update_rpt<2>(b, a, k, force);
+	return launch_update_rpt<kRowsPerThread>(b, a, k, force);
+}
+
+// The storage instantiation of a kernel that reads pA (tile_rows).
+static inline int tile_storage(const CGArgs& a) { return (a.code ? 1 : 0) | (a.bf16 ? 2 : 0); }
+
+// A tile kernel (templated <Multi, St>) over `items` tiles for the storage of
+// the CGArgs `cga`: one workgroup per tile, or groups(b, items) that each
+// take a range (for_items). Leaves the launch status in hipGetLastError().
+#define TILE_KERNEL(kernel, cga, items, ...)                                                             \
+	do {                                                                                                 \
+		const uint32_t n_ = (items), g_ = groups(b, n_);                                                 \
+		if (n_ > 0) {                                                                                    \
+			const bool m_ = g_ < n_;                                                                     \
+			const dim3 gr_(m_ ? g_ : n_);                                                                \
+			switch (tile_storage(cga) * 2 + (m_ ? 1 : 0)) {                                              \
+			case 0: hipLaunchKernelGGL((kernel<false, 0>), gr_, dim3(kBlock), 0, b.stream, __VA_ARGS__); break; \
+			case 1: hipLaunchKernelGGL((kernel<true, 0>), gr_, dim3(kBlock), 0, b.stream, __VA_ARGS__); break;  \
+			case 2: hipLaunchKernelGGL((kernel<false, 1>), gr_, dim3(kBlock), 0, b.stream, __VA_ARGS__); break; \
+			case 3: hipLaunchKernelGGL((kernel<true, 1>), gr_, dim3(kBlock), 0, b.stream, __VA_ARGS__); break;  \
+			case 4: hipLaunchKernelGGL((kernel<false, 2>), gr_, dim3(kBlock), 0, b.stream, __VA_ARGS__); break; \
+			case 5: hipLaunchKernelGGL((kernel<true, 2>), gr_, dim3(kBlock), 0, b.stream, __VA_ARGS__); break;  \
+			case 6: hipLaunchKernelGGL((kernel<false, 3>), gr_, dim3(kBlock), 0, b.stream, __VA_ARGS__); break; \
+			default: hipLaunchKernelGGL((kernel<true, 3>), gr_, dim3(kBlock), 0, b.stream, __VA_ARGS__); break; \
+			}                                                                                            \
+		}                                                                                                \
+	} while (0)
+#define LAUNCH_TILES(kernel, cga, items, ...)       \
+	do {                                            \
+		TILE_KERNEL(kernel, cga, items, __VA_ARGS__); \
+		B3G_HIP(hipGetLastError());                 \
 	} while (0)
 
 // Row decompositions, both cut at instance boundaries so that every tile /
@@ -2600,13 +2654,14 @@ static b3g_status enqueue_pcg_iterations(Body& b, const CGArgs& a, uint32_t maxI
 			ak.dnew = (k & 1) ? d0 : d1;
 			const int sel = k > 0 ? delta_range(k - 1) : kRangeTiles;
 			if (!fused && k > 0) FINALIZE(b.partial, sel, kFinDelta, k, kGateIter, k - 1);
-			LAUNCH_ITEMS(k_cg_spmv_dq, b.nTiles, ak, k, sel, 0);
+			LAUNCH_TILES(k_cg_spmv_dq, ak, b.nTiles, ak, k, sel, 0);
 			if (!fused) FINALIZE(b.partialQ, kRangeTiles, kFinAlpha, k, kGateIter, k);
 			if (k % 50 == 0) {
 				LAUNCH_ITEMS(k_cg_update_x, rb, ak, k);
-				LAUNCH_ITEMS(k_cg_residual, b.nTiles, ak, (int)kFinDelta, k);
+				LAUNCH_TILES(k_cg_residual, ak, b.nTiles, ak, (int)kFinDelta, k);
 			} else {
-				LAUNCH_ITEMS(k_cg_update, rb, ak, k, 0);
+				const b3g_status us = launch_update(b, ak, k, 0);
+				if (us != B3G_OK) return us;
 			}
 		}
 		chunk = kExitChunk;
@@ -2640,7 +2695,7 @@ b3g_status enqueue_solve_cg(Body& b, uint32_t maxIt, float eps) {
 	b3g_status s = ensure_cg_log(b, maxIt + 1);
 	if (s != B3G_OK) return s;
 	CGArgs a = cg_args(b);
-	LAUNCH_ITEMS(k_cg_residual, b.nTiles, a, (int)kFinInit, 0u);
+	LAUNCH_TILES(k_cg_residual, a, b.nTiles, a, (int)kFinInit, 0u);
 	FINALIZE(b.partial, kRangeTiles, kFinInit, 0, kGateNewton, 0);
 	return enqueue_pcg_iterations(b, a, maxIt);
 }
@@ -2738,7 +2793,16 @@ static b3g_status enqueue_solve_refined(Body& b, uint32_t sweeps) {
 
 b3g_status enqueue_spmv(Body& b, const DBuf& xin, DBuf& yout) {
 	CGArgs a = cg_args(b);
-	LAUNCH(k_spmv, b.nTiles, a, dptr<V3>(xin), dptr<V3>(yout));
+	if (b.nTiles) {
+		const dim3 gr(b.nTiles);
+		switch (tile_storage(a)) {
+		case 0: hipLaunchKernelGGL(k_spmv<0>, gr, dim3(kBlock), 0, b.stream, a, dptr<V3>(xin), dptr<V3>(yout)); break;
+		case 1: hipLaunchKernelGGL(k_spmv<1>, gr, dim3(kBlock), 0, b.stream, a, dptr<V3>(xin), dptr<V3>(yout)); break;
+		case 2: hipLaunchKernelGGL(k_spmv<2>, gr, dim3(kBlock), 0, b.stream, a, dptr<V3>(xin), dptr<V3>(yout)); break;
+		default: hipLaunchKernelGGL(k_spmv<3>, gr, dim3(kBlock), 0, b.stream, a, dptr<V3>(xin), dptr<V3>(yout)); break;
+		}
+	}
+	B3G_HIP(hipGetLastError());
 	return B3G_OK;
 }
 
@@ -2759,13 +2823,7 @@ b3g_status time_pcg_spmv(Body& b, uint32_t launches, double* mean_ms) {
 			CGArgs ak = a;
 			ak.dold = (k & 1) ? dptr<V3>(b.d1) : dptr<V3>(b.d0);
 			ak.dnew = (k & 1) ? dptr<V3>(b.d0) : dptr<V3>(b.d1);
-			{
-				const uint32_t g = groups(b, b.nTiles);
-				if (g < b.nTiles)
-					hipLaunchKernelGGL(k_cg_spmv_dq<true>, dim3(g), dim3(kBlock), 0, b.stream, ak, 1u, (int)kRangeBlocks, 1);
-				else
-					hipLaunchKernelGGL(k_cg_spmv_dq<false>, dim3(g), dim3(kBlock), 0, b.stream, ak, 1u, (int)kRangeBlocks, 1);
-			}
+			TILE_KERNEL(k_cg_spmv_dq, ak, b.nTiles, ak, 1u, (int)kRangeBlocks, 1);
 			if (hipGetLastError() != hipSuccess) st = B3G_HIP_ERROR;
 		}
 		if (st != B3G_OK) break;
@@ -2833,13 +2891,7 @@ b3g_status time_pcg_spmv_cold(Body& b, uint32_t launches, size_t evictBytes, dou
 			CGArgs ak = a;
 			ak.dold = (k & 1) ? dptr<V3>(b.d1) : dptr<V3>(b.d0);
 			ak.dnew = (k & 1) ? dptr<V3>(b.d0) : dptr<V3>(b.d1);
-			{
-				const uint32_t g = groups(b, b.nTiles);
-				if (g < b.nTiles)
-					hipLaunchKernelGGL(k_cg_spmv_dq<true>, dim3(g), dim3(kBlock), 0, b.stream, ak, 1u, (int)kRangeBlocks, 1);
-				else
-					hipLaunchKernelGGL(k_cg_spmv_dq<false>, dim3(g), dim3(kBlock), 0, b.stream, ak, 1u, (int)kRangeBlocks, 1);
-			}
+			TILE_KERNEL(k_cg_spmv_dq, ak, b.nTiles, ak, 1u, (int)kRangeBlocks, 1);
 			ok = hipGetLastError() == hipSuccess;
 		}
 		ok = ok && hipEventRecord(e[1], b.stream) == hipSuccess && hipEventRecord(e[2], b.stream) == hipSuccess;
@@ -2878,9 +2930,12 @@ b3g_status time_pcg_iteration(Body& b, uint32_t reps, double* ms) {
 			CGArgs ak = a;
 			ak.dold = (r & 1) ? dptr<V3>(b.d1) : dptr<V3>(b.d0);
 			ak.dnew = (r & 1) ? dptr<V3>(b.d0) : dptr<V3>(b.d1);
-			LAUNCH_ITEMS(k_cg_spmv_dq, b.nTiles, ak, 1u, (int)kRangeBlocks, 1);
+			LAUNCH_TILES(k_cg_spmv_dq, ak, b.nTiles, ak, 1u, (int)kRangeBlocks, 1);
 			if (!a.fused) FINALIZE(b.partialQ, kRangeTiles, kFinAlpha, 1, kGateIter, 1);
-			LAUNCH_ITEMS(k_cg_update, b.nRowBlk, ak, 1u, 1);
+			{
+				b3g_status us = launch_update(b, ak, 1u, 1);
+				if (us != B3G_OK) return us;
+			}
 			if (!a.fused) FINALIZE(b.partial, kRangeBlocks, kFinDelta, 2, kGateIter, 1);
 		}
 		B3G_HIP(hipEventRecord(e1, b.stream));
