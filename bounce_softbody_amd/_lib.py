@@ -129,7 +129,8 @@ EXPORTS = [
     "b3g_solve_cg", "b3g_spmv", "b3g_set_reduction", "b3g_save_state", "b3g_load_state", "b3g_ray_cast", "b3g_ray_cast_device",
     "b3g_sdf_grid", "b3g_build_sdf", "b3g_body_rebuild", "b3g_set_action_forces", "b3g_get_action_forces",
     "b3g_solve_cg_static", "b3g_time_spmv_cold", "b3g_set_early_exit", "b3g_get_schedule", "b3g_set_schedule",
-    "b3g_autotune",
+    "b3g_autotune", "b3g_matrix_add", "b3g_matrix_sub", "b3g_be_create", "b3g_be_destroy", "b3g_be_set_reduction",
+    "b3g_be_input_defaults", "b3g_solve_be",
 ]
 
 
@@ -148,6 +149,33 @@ class Schedule(C.Structure):
 
 SCHEDULE_FIELDS = ("tile_blocks", "tile_rows", "row_block_rows", "fused", "max_groups")
 DEFAULT_SCHEDULE = (1024, 256, 1024, -1, 0)
+
+
+class ForceData(C.Structure):
+    """``b3g_force_data`` (include/b3g.h): b3SparseForceSolverData."""
+
+    _fields_ = [("h", C.c_float), ("inv_h", C.c_float), ("n", C.c_uint32), ("x", f32p), ("v", f32p), ("f", f32p),
+                ("dfdx", C.c_void_p), ("dfdv", C.c_void_p)]
+
+
+# b3g_force_model: b3g_status (*)(void* user, const b3g_force_data* data)
+FORCE_MODEL = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(ForceData))
+
+
+class BEInput(C.Structure):
+    """``b3g_be_input`` (include/b3g.h): b3SolveBEInput."""
+
+    _fields_ = [("h", C.c_float), ("inv_h", C.c_float), ("force_model", FORCE_MODEL), ("user", C.c_void_p),
+                ("n", C.c_uint32), ("x0", f32p), ("v0", f32p), ("fe", f32p), ("M", f32p), ("y", f32p), ("S", f32p),
+                ("z", f32p), ("max_iterations", C.c_uint32), ("tolerance", C.c_float),
+                ("max_sub_iterations", C.c_uint32), ("sub_tolerance", C.c_float)]
+
+
+class BEOutput(C.Structure):
+    """``b3g_be_output`` (include/b3g.h): b3SolveBEOutput."""
+
+    _fields_ = [("x", f32p), ("v", f32p), ("iterations", C.c_uint32), ("error", C.c_float),
+                ("min_sub_iterations", C.c_uint32), ("max_sub_iterations", C.c_uint32)]
 
 
 class Carry(C.Structure):
@@ -273,6 +301,13 @@ def lib():
         "b3g_solve_cg_static": (C.c_int, [C.c_int, vp, C.c_uint32, C.c_uint32, vp, vp, vp, vp, vp, C.c_uint32, C.c_float,
                                           vp]),
         "b3g_set_reduction": (C.c_int, [vp, C.c_int]),
+        "b3g_matrix_add": (C.c_int, [vp, C.c_uint32, u32p, u32p, f32p]),
+        "b3g_matrix_sub": (C.c_int, [vp, C.c_uint32, u32p, u32p, f32p]),
+        "b3g_be_create": (C.c_int, [C.POINTER(vp), C.c_int]),
+        "b3g_be_destroy": (C.c_int, [vp]),
+        "b3g_be_set_reduction": (C.c_int, [vp, C.c_int]),
+        "b3g_be_input_defaults": (C.c_int, [C.POINTER(BEInput)]),
+        "b3g_solve_be": (C.c_int, [vp, C.POINTER(BEOutput), C.POINTER(BEInput)]),
         "b3g_save_state": (C.c_int, [vp, C.c_void_p, C.POINTER(C.c_uint64)]),
         "b3g_load_state": (C.c_int, [vp, C.c_void_p, C.c_uint64]),
     }

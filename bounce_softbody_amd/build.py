@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libb3g.so")
-SOURCES = ["b3g_api.hip", "b3g_setup.hip", "b3g_kernels.hip", "b3g_raycast.hip", "b3g_sdf.hip", "b3g_static.hip"]
+SOURCES = ["b3g_api.hip", "b3g_setup.hip", "b3g_kernels.hip", "b3g_raycast.hip", "b3g_sdf.hip", "b3g_static.hip", "b3g_be.hip"]
 
 # -ffp-contract=off and IEEE division/sqrt keep every operation rounded as the
 # reference rounds it; packed FP32 instructions are disabled.

@@ -400,4 +400,37 @@ b3g_status ensure_cg_log(Body& b, uint32_t iterations);
 b3g_status ensure_exit_flags(Body& b);
 void free_exit_flags(Body& b);
 
+// Operator-level backward Euler (b3g_be.hip): arguments of its kernels.
+// Matrices of the force model are in list order (rows head to tail), one
+// M33 per block; slots of pA carry their row, dfdx / dfdv entry (-1: none).
+struct BEArgs {
+	uint32_t n = 0, nnz = 0;
+	float h = 0.0f, hh = 0.0f;
+	uint32_t useV = 0;                // h != 0: the (h dfdx) w term of b exists
+	const uint32_t* rowptr = nullptr; // pA (CSR slots)
+	const uint32_t* cols = nullptr;
+	const uint32_t* slotRow = nullptr;
+	const uint32_t* slotPos = nullptr;  // storage position of a CSR slot (column-sorted storage), or null
+	const uint32_t* diagSlot = nullptr;
+	const int32_t* slotX = nullptr;
+	const int32_t* slotV = nullptr;
+	const uint32_t* rowptrX = nullptr; // dfdx
+	const uint32_t* colsX = nullptr;
+	const M33* valX = nullptr;
+	const M33* valV = nullptr;
+	const M33* M = nullptr;
+	const M33* S = nullptr;
+	const V3 *x0 = nullptr, *v0 = nullptr, *fe = nullptr, *fi = nullptr, *y = nullptr, *z = nullptr;
+	V3 *x = nullptr, *v = nullptr, *w = nullptr, *px = nullptr, *pb = nullptr, *invM = nullptr;
+	M33* Araw = nullptr;
+	float* A = nullptr;
+	uint32_t stride = 0;
+	SolverState* st = nullptr;
+};
+// Solver state, py = 0 (once per solve); one Newton iteration after the force
+// model's output is on the device: linearisation, PCG, update, error.
+b3g_status enqueue_be_begin(Body& b, const BEArgs& a, uint32_t maxIt, float tol, uint32_t maxSub, float subTol,
+                            uint32_t minSub, uint32_t maxSubStat);
+b3g_status enqueue_be_iteration(Body& b, const BEArgs& a, uint32_t maxSub, float subTol);
+
 } // namespace b3g

@@ -3101,4 +3101,128 @@ b3g_status enqueue_step(Body& b, float dt, uint32_t iters, uint32_t subIters, fl
 	return enqueue_find_contacts(b);
 }
 
+// ---------------------------------------------------------------------------
+// Operator-level backward Euler (b3SparseSolveBE with a host force model,
+// sparse_force_solver.cpp:104-232). The symbolic phase on the host gives every
+// slot of pA (the reference's row order of A) its dfdx / dfdv entry; the
+// kernels below form the values operation for operation.
+// ---------------------------------------------------------------------------
+
+// Solver state for a solve: Newton and PCG limits and tolerances, the running
+// min / max of the PCG counts the caller passed in.
+__global__ void k_be_state(SolverState* st, uint32_t maxIt, float eps2, uint32_t maxSub, float subEps2, uint32_t seq,
+                           uint32_t minSub, uint32_t maxSubStat) {
+	uint32_t* words = reinterpret_cast<uint32_t*>(st);
+	for (size_t k = 0; k < sizeof(SolverState) / 4; ++k) words[k] = 0;
+	st->maxIt = maxSub;
+	st->eps2_sub = subEps2;
+	st->eps2 = eps2;
+	st->newtonMax = maxIt;
+	st->newtonActive = maxIt > 0 ? 1u : 0u;
+	st->seq = seq;
+	st->statMinSub = minSub;
+	st->statMaxSub = maxSubStat;
+}
+
+// w = x0 - x + h v + y, the vector h dfdx multiplies in b
+// (sparse_force_solver.cpp:169: VAdd(VAdd(VSub(x0, x), h v), y)).
+__global__ void __launch_bounds__(kBlock) k_be_w(BEArgs a) {
+	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= a.n) return;
+	a.w[i] = ((a.x0[i] - a.x[i]) + a.h * a.v[i]) + a.y[i];
+}
+
+// One thread per slot of pA (row i, column j):
+//   A_ij = ((j == i ? M_i : 0) - h dfdv_ij) - (h h) dfdx_ij, each term only
+//   where the reference's lists hold it (b3Sub(Diag, Sparse) then
+//   b3Sub(Sparse, Sparse), sparse_mat33.h:458-492; b3Mul(s, B) leaves B's
+//   copy empty for s == 0, sparse_mat33.h:518-521);
+//   pA_ij = (S_i A_ij) S_j^T, plus I minus S_i on the diagonal
+//   (sparse_mat33.h:535-576, 464-474).
+// A is kept per slot for the product A z.
+__global__ void __launch_bounds__(kBlock) k_be_slots(BEArgs a) {
+	const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+	if (k >= a.nnz) return;
+	const uint32_t i = a.slotRow[k], j = a.cols[k];
+	M33 t = j == i ? a.M[i] : mzero();
+	const int32_t kv = a.slotV[k], kx = a.slotX[k];
+	if (kv >= 0) t = t - a.h * a.valV[kv];
+	if (kx >= 0) t = t - a.hh * a.valX[kx];
+	a.Araw[k] = t;
+	M33 p = (a.S[i] * t) * transpose(a.S[j]);
+	if (j == i) p = (p + mdiag(1.0f)) - a.S[i];
+	const uint32_t pos = a.slotPos ? a.slotPos[k] : k;
+	const float* pe = &p.x.x;
+#pragma unroll
+	for (int c = 0; c < 9; ++c) a.A[(size_t)c * a.stride + pos] = pe[c];
+}
+
+// One thread per row: b = M (v0 - v) + h (fe + fi) + (h dfdx) w with the
+// product over reverse(dfdx row) (the copy Scale makes), pb = S (b - A z)
+// (sparse_force_solver.cpp:168-178), and the Jacobi preconditioner from pA's
+// diagonal block (sparse_solver.cpp:41-54), whose entries must be > 0.
+__global__ void __launch_bounds__(kBlock) k_be_rows(BEArgs a) {
+	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= a.n) return;
+	const V3 t1 = a.M[i] * (a.v0[i] - a.v[i]);
+	const V3 t2 = a.h * (a.fe[i] + a.fi[i]);
+	V3 t3 = zero3();
+	if (a.useV)
+		for (uint32_t k = a.rowptrX[i + 1]; k-- > a.rowptrX[i];) t3 = t3 + (a.h * a.valX[k]) * a.w[a.colsX[k]];
+	const V3 bb = (t1 + t2) + t3;
+	V3 az = zero3();
+	for (uint32_t k = a.rowptr[i]; k < a.rowptr[i + 1]; ++k) az = az + a.Araw[k] * a.z[a.cols[k]];
+	a.pb[i] = a.S[i] * (bb - az);
+	const uint32_t d = a.slotPos ? a.slotPos[a.diagSlot[i]] : a.diagSlot[i];
+	const float axx = a.A[d], ayy = a.A[(size_t)4 * a.stride + d], azz = a.A[(size_t)8 * a.stride + d];
+	if (!(axx > 0.0f) || !(ayy > 0.0f) || !(azz > 0.0f)) atomicOr(&a.st->faults, (uint32_t)B3G_FAULT_DIAGONAL);
+	a.invM[i] = v3(1.0f / axx, 1.0f / ayy, 1.0f / azz);
+}
+
+// dv = py + z; v += dv; x = x0 + h v + y; error = |dv|^2
+// (sparse_force_solver.cpp:180-204), one partial per row block.
+__global__ void __launch_bounds__(kBlock) k_be_update(BEArgs a, const uint32_t* rowBlk, double* partial, float* terms) {
+	if (!a.st->newtonActive) return;
+	const uint32_t blk = blockIdx.x, r0 = rowBlk[blk], r1 = rowBlk[blk + 1];
+	const bool seq = a.st->seq != 0;
+	double part = 0.0;
+#pragma unroll
+	for (int u = 0; u < kRowsPerThread; ++u) {
+		const uint32_t i = r0 + threadIdx.x + u * kBlock;
+		if (i >= r1) break;
+		const V3 dv = a.px[i] + a.z[i];
+		const V3 v = a.v[i] + dv;
+		const V3 x = (a.x0[i] + a.h * v) + a.y[i];
+		a.v[i] = v;
+		a.x[i] = x;
+		if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) && isfinite(v.y) && isfinite(v.z)))
+			atomicOr(&a.st->faults, (uint32_t)B3G_FAULT_NON_FINITE);
+		const float term = length_sq(dv);
+		if (seq) terms[i] = term;
+		part += (double)term;
+	}
+	block_store(part, partial, blk);
+}
+
+b3g_status enqueue_be_begin(Body& b, const BEArgs& a, uint32_t maxIt, float tol, uint32_t maxSub, float subTol,
+                            uint32_t minSub, uint32_t maxSubStat) {
+	const float eps2 = tol * tol, subEps2 = subTol * subTol;
+	hipLaunchKernelGGL(k_be_state, dim3(1), dim3(1), 0, b.stream, a.st, maxIt, eps2, maxSub, subEps2,
+	                   b.reduction == B3G_REDUCE_SEQUENTIAL ? 1u : 0u, minSub, maxSubStat);
+	B3G_HIP(hipGetLastError());
+	LAUNCH(k_clear_v3, nblk(b.n), (size_t)b.n, dptr<V3>(b.px));   // py = 0 (sparse_force_solver.cpp:133-134)
+	return B3G_OK;
+}
+
+b3g_status enqueue_be_iteration(Body& b, const BEArgs& a, uint32_t maxSub, float subTol) {
+	LAUNCH(k_be_w, nblk(b.n), a);
+	LAUNCH(k_be_slots, nblk(b.nnz), a);
+	LAUNCH(k_be_rows, nblk(b.n), a);
+	b3g_status s = enqueue_solve_cg(b, maxSub, subTol);
+	if (s != B3G_OK) return s;
+	LAUNCH(k_be_update, b.nRowBlk, a, dptr<uint32_t>(b.rowBlk), dptr<double>(b.partial), dptr<float>(b.terms));
+	FINALIZE(b.partial, kRangeBlocks, kFinNewton, 0, kGateNewton, 0);
+	return B3G_OK;
+}
+
 } // namespace b3g

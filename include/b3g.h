@@ -554,6 +554,94 @@ b3g_status b3g_solve_cg_static(int device, void* stream, uint32_t n, uint32_t nn
 b3g_status b3g_spmv(int device, uint32_t n, const uint32_t* rowptr, const uint32_t* cols, const float* vals,
                     const float* x, float* y);
 
+/*
+ * Operator-level replacement of b3SparseSolveBE (sparse/sparse_force_solver.h:97,
+ * sparse_force_solver.cpp:104-232) behind its plugin, b3SparseForceModel
+ * (sparse_force_solver.h:44-49). The caller's force model runs on the host
+ * once per Newton iteration, as the reference calls it; everything after it
+ * runs on the device: A = M - h dfdv - h^2 dfdx and the right-hand side
+ * with the reference's list orders (every sparse copy reverses a row,
+ * sparse_mat33.h:210-230), the pre-filter pA = S A S^T + I - S,
+ * pb = S (b - A z) for any 3x3 diagonal blocks M and S, the Jacobi PCG
+ * (sparse_solver.cpp:28-109, early exit included) warm-started across Newton
+ * iterations, and the Newton update and error. With B3G_REDUCE_SEQUENTIAL the
+ * results are the reference's bit for bit; the default FP64 reduction
+ * differs from it as b3g_step's does.
+ */
+
+/* A b3SparseMat33 being filled by a force model (host memory, the reference's list semantics). */
+typedef struct b3g_matrix_s* b3g_matrix;
+
+/*
+ * (*m)(rows[k], cols[k]) += blocks[9 k .. 9 k + 8] for k = 0 .. count - 1 in
+ * order: b3SparseMat33::operator()(i, j) (search the row from its head,
+ * insert a zero block at the head when absent; sparse_mat33.h:268-288), then
+ * b3Mat33::operator+= on the column-major block. b3g_matrix_sub does -=.
+ * Indices must be below the system's row count (B3G_INVALID_ARGUMENT).
+ */
+b3g_status b3g_matrix_add(b3g_matrix m, uint32_t count, const uint32_t* rows, const uint32_t* cols,
+                          const float* blocks);
+b3g_status b3g_matrix_sub(b3g_matrix m, uint32_t count, const uint32_t* rows, const uint32_t* cols,
+                          const float* blocks);
+
+/* b3SparseForceSolverData (sparse_force_solver.h:31-39): what the force model reads and fills. */
+typedef struct b3g_force_data {
+	float h, inv_h;
+	uint32_t n;
+	const float* x;     /* 3 n: the Newton iterate's positions */
+	const float* v;     /* 3 n: velocities */
+	float* f;           /* 3 n, zero on entry: the forces */
+	b3g_matrix dfdx;    /* empty on entry */
+	b3g_matrix dfdv;    /* empty on entry */
+} b3g_force_data;
+
+/* b3SparseForceModel::ApplyForces. A status other than B3G_OK ends the solve with that status. */
+typedef b3g_status (*b3g_force_model)(void* user, const b3g_force_data* data);
+
+/* b3SolveBEInput (sparse_force_solver.h:52-81); b3g_be_input_defaults sets its constructor's values. */
+typedef struct b3g_be_input {
+	float h, inv_h;
+	b3g_force_model force_model;
+	void* user;
+	uint32_t n;                    /* dofCount */
+	const float* x0;               /* 3 n */
+	const float* v0;               /* 3 n */
+	const float* fe;               /* 3 n: external forces over the step */
+	const float* M;                /* 9 n: diagonal 3x3 blocks of the mass matrix, column-major */
+	const float* y;                /* 3 n: translation */
+	const float* S;                /* 9 n: diagonal 3x3 blocks of the constraint filter */
+	const float* z;                /* 3 n: desired sub-solution (NULL: zero) */
+	uint32_t max_iterations;       /* Newton */
+	float tolerance;
+	uint32_t max_sub_iterations;   /* PCG */
+	float sub_tolerance;
+} b3g_be_input;
+
+/* b3SolveBEOutput (sparse_force_solver.h:84-93). min/max_sub_iterations are in/out, as in the reference:
+ * the solve folds its PCG counts into the values they hold on entry. */
+typedef struct b3g_be_output {
+	float* x;                      /* 3 n */
+	float* v;                      /* 3 n */
+	uint32_t iterations;
+	float error;
+	uint32_t min_sub_iterations;
+	uint32_t max_sub_iterations;
+} b3g_be_output;
+
+/* A device-side solver context: buffers and the symbolic pattern, kept between calls while the force model's
+ * pattern does not change. One HIP stream; not to be used from two threads at once. */
+typedef struct b3g_be_s* b3g_be;
+
+b3g_status b3g_be_create(b3g_be* out, int device);
+b3g_status b3g_be_destroy(b3g_be be);
+/* B3G_REDUCE_* as b3g_set_reduction; B3G_REDUCE_SEQUENTIAL gives the reference's results bit for bit. */
+b3g_status b3g_be_set_reduction(b3g_be be, int mode);
+/* The b3SolveBEInput constructor (sparse_force_solver.h:56-62): 1 Newton iteration, 20 PCG iterations,
+ * both tolerances FLT_EPSILON; every other field zero. */
+b3g_status b3g_be_input_defaults(b3g_be_input* in);
+/* Replaces b3SparseSolveBE(output, input). Blocks until the solve ends. */
+b3g_status b3g_solve_be(b3g_be be, b3g_be_output* out, const b3g_be_input* in);
+
 #ifdef __cplusplus
 }
 #endif

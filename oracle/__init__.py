@@ -455,3 +455,57 @@ def build_sdf(vertices, triangles, cell, extension=1.0, which: str = "oracle"):
                          C.c_float(extension), _ptr(box, f32p), _ptr(dims, u32p), _ptr(vox, f32p), n)
     W, H, D = (int(x) for x in dims)
     return box[:3].copy(), box[3:].copy(), (W, H, D), vox.reshape(D, H, W)
+
+
+# b3SparseForceModel callback of b3o_solve_be / refh_solve_be:
+# (user, n, x, v, f, dfdx, dfdv)
+MODEL_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_uint32, f32p, f32p, f32p, C.c_void_p, C.c_void_p)
+
+
+def solve_be(which: str, model, h, x0, v0, fe, M, y, S, z, max_it, tol, max_sub, sub_tol,
+             sub_minmax=(0xFFFFFFFF, 0), wide=False):
+    """``b3SparseSolveBE`` (sparse_force_solver.cpp:104-232) by the oracle (``which="oracle"``) or the reference
+    build (``"ref"``) with a Python force model. ``model(x, v)`` gets the iterate as (n, 3) float32 arrays and
+    returns ``(f, ops)``: the forces (n, 3) and a list of ``(matrix, sign, rows, cols, blocks)`` with matrix
+    ``"x"`` (dfdx) or ``"v"`` (dfdv), sign +1 (``+=``) or -1 (``-=``) and blocks (k, 9) column-major, applied in
+    order. M and S are (n, 9) diagonal blocks; z may be None (zero). Returns (x, v, iterations, error,
+    (min_sub, max_sub)). ``wide`` (oracle only): the widened-accumulator dot products."""
+    lib = _lib(which)
+    p = "b3o_" if which == "oracle" else "refh_"
+    add = lib[p + "matrix_add"]
+    add.argtypes = [C.c_void_p, C.c_uint32, u32p, u32p, f32p, C.c_int]
+    n = len(x0)
+
+    def cb(user, n_, xp, vp, fp, dfdx, dfdv):
+        x = np.ctypeslib.as_array(xp, (n_, 3)).copy()
+        v = np.ctypeslib.as_array(vp, (n_, 3)).copy()
+        f, ops = model(x, v)
+        np.ctypeslib.as_array(fp, (n_, 3))[:] = np.asarray(f, np.float32)
+        for target, sign, rows, cols, blocks in ops:
+            r = np.ascontiguousarray(rows, np.uint32)
+            c = np.ascontiguousarray(cols, np.uint32)
+            b = np.ascontiguousarray(blocks, np.float32).reshape(-1)
+            add(dfdx if target == "x" else dfdv, len(r), _ptr(r, u32p), _ptr(c, u32p), _ptr(b, f32p), int(sign))
+
+    fn = MODEL_FN(cb)
+    f32 = lambda a: None if a is None else np.ascontiguousarray(a, np.float32).reshape(-1)   # noqa: E731
+    X0, V0, FE, MM, Y, SS, Z = (f32(a) for a in (x0, v0, fe, M, y, S, z))
+    x = np.zeros(3 * n, np.float32)
+    v = np.zeros(3 * n, np.float32)
+    err = C.c_float(0.0)
+    mm = np.array(sub_minmax, np.uint32)
+    argt = [C.c_uint32, C.c_float] + [f32p] * 7 + [C.c_uint32, C.c_float, C.c_uint32, C.c_float, MODEL_FN, C.c_void_p]
+    if which == "oracle":
+        argt += [C.c_int]
+    argt += [f32p, f32p, C.POINTER(C.c_float), u32p]
+    fnc = lib[p + "solve_be"]
+    fnc.argtypes = argt
+    fnc.restype = C.c_uint32
+    args = [n, C.c_float(h), _ptr(X0, f32p), _ptr(V0, f32p), _ptr(FE, f32p), _ptr(MM, f32p), _ptr(Y, f32p),
+            _ptr(SS, f32p), _ptr(Z, f32p) if Z is not None else C.cast(None, f32p), max_it, C.c_float(tol), max_sub,
+            C.c_float(sub_tol), fn, None]
+    if which == "oracle":
+        args.append(1 if wide else 0)
+    args += [_ptr(x, f32p), _ptr(v, f32p), C.byref(err), _ptr(mm, u32p)]
+    it = fnc(*args)
+    return x.reshape(-1, 3), v.reshape(-1, 3), int(it), float(err.value), (int(mm[0]), int(mm[1]))

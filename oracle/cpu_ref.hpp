@@ -1392,11 +1392,18 @@ template <class T> uint32_t Body<T>::SolveCG(const SpMat<T>& A, const DVec<T>& b
 	return SolveCGFree(A, b, maxIt, eps, x, wideDots);
 }
 
-// sparse_force_solver.cpp:104-232
-template <class T>
-void Body<T>::SolveBE(T h, uint32_t maxIt, T eps, uint32_t maxSub, T subEps, const DVec<T>& x0, const DVec<T>& v0,
-                      const DVec<T>& fe, const DMat<T>& M, const DVec<T>& y, const DMat<T>& S, const DVec<T>& z,
-                      DVec<T>& xo, DVec<T>& vo) {
+// sparse_force_solver.cpp:104-232, with the force model as a callable
+// model(x, v, f, dfdx, dfdv) (b3SparseForceModel::ApplyForces,
+// sparse_force_solver.h:44-49): f arrives zeroed and the matrices empty.
+// wide: b3Dot / b3LengthSquared accumulated in double (VDotWide). stats
+// keeps the caller's running min / max of the PCG counts
+// (sparse_force_solver.cpp:186-188); returns the Newton iteration count.
+// The final error goes to *errorOut when given.
+template <class T, class Model>
+uint32_t SolveBEGeneric(T h, uint32_t maxIt, T eps, uint32_t maxSub, T subEps, const DVec<T>& x0, const DVec<T>& v0,
+                        const DVec<T>& fe, const DMat<T>& M, const DVec<T>& y, const DMat<T>& S, const DVec<T>& z,
+                        DVec<T>& xo, DVec<T>& vo, Model&& model, bool wide, StepStats& stats,
+                        Capture<T>* capture = nullptr, T* errorOut = nullptr) {
 	const uint32_t n = (uint32_t)x0.size();
 	DMat<T> ST(n), I(n);
 	for (uint32_t i = 0; i < n; ++i) { ST[i] = Transpose(S[i]); I[i] = M33<T>::Identity(); }
@@ -1409,7 +1416,7 @@ void Body<T>::SolveBE(T h, uint32_t maxIt, T eps, uint32_t maxSub, T subEps, con
 		DVec<T> fi(n);
 		for (auto& e : fi) e.SetZero();
 		SpMat<T> dfdx(n), dfdv(n);
-		ApplyForces(h, x, v, fi, dfdx, dfdv);
+		model(x, v, fi, dfdx, dfdv);
 		// A = M - h * dfdv - (h * h) * dfdx
 		SpMat<T> A = Minus(DiagMinus(M, Scale(h, dfdv)), Scale(h * h, dfdx));
 		// b = M * (v0 - v) + h * (fe + fi) + h * dfdx * (x0 - x + h * v + y)
@@ -1418,18 +1425,18 @@ void Body<T>::SolveBE(T h, uint32_t maxIt, T eps, uint32_t maxSub, T subEps, con
 		// pA = S * A * ST + I - S ; pb = S * (b - A * z)
 		SpMat<T> pA = MinusDiag(PlusDiag(TimesDiag(DiagTimes(S, A), ST), I), S);
 		DVec<T> pb = DiagMul(S, VSub(b, SpMV(A, z)));
-		if (capture.want && it == 0) {
-			if (capture.pA) *capture.pA = pA;
-			if (capture.pb) *capture.pb = pb;
-			if (capture.fi) *capture.fi = fi;
+		if (capture && capture->want && it == 0) {
+			if (capture->pA) *capture->pA = pA;
+			if (capture->pb) *capture->pb = pb;
+			if (capture->fi) *capture->fi = fi;
 		}
-		uint32_t subIt = SolveCG(pA, pb, maxSub, subEps, py);
+		uint32_t subIt = SolveCGFree(pA, pb, maxSub, subEps, py, wide);
 		DVec<T> dv = VAdd(py, z);
 		stats.minSub = Min(stats.minSub, subIt);
 		stats.maxSub = Max(stats.maxSub, subIt);
 		v = VAdd(v, dv);
 		x = VAdd(VAdd(x0, VScale(h, v)), y);
-		error = wideDots ? VLengthSquaredWide(dv) : VLengthSquared(dv);
+		error = wide ? VLengthSquaredWide(dv) : VLengthSquared(dv);
 		if (it == 0) {
 			error0 = error;
 			if (error <= eps * eps) break;
@@ -1440,7 +1447,19 @@ void Body<T>::SolveBE(T h, uint32_t maxIt, T eps, uint32_t maxSub, T subEps, con
 	}
 	xo = x;
 	vo = v;
-	stats.newtonIterations = it;
+	if (errorOut) *errorOut = error;
+	return it;
+}
+
+template <class T>
+void Body<T>::SolveBE(T h, uint32_t maxIt, T eps, uint32_t maxSub, T subEps, const DVec<T>& x0, const DVec<T>& v0,
+                      const DVec<T>& fe, const DMat<T>& M, const DVec<T>& y, const DMat<T>& S, const DVec<T>& z,
+                      DVec<T>& xo, DVec<T>& vo) {
+	auto model = [this, h](const DVec<T>& x, const DVec<T>& v, DVec<T>& f, SpMat<T>& dfdx, SpMat<T>& dfdv) {
+		ApplyForces(h, x, v, f, dfdx, dfdv);
+	};
+	stats.newtonIterations = SolveBEGeneric(h, maxIt, eps, maxSub, subEps, x0, v0, fe, M, y, S, z, xo, vo, model,
+	                                        wideDots, stats, &capture);
 }
 
 // force_solver.cpp:88-180 (+ body_solver.cpp:67-93 friction)

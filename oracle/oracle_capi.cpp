@@ -425,4 +425,70 @@ uint64_t b3o_build_sdf(const float* vertices, uint32_t nv, const uint32_t* trian
 	return count;
 }
 
+// b3SparseSolveBE (sparse_force_solver.cpp:104-232) with a caller's force
+// model. The model gets the iterate (x, v: 3 n floats), a zeroed f to fill
+// and the two matrices as opaque pointers for b3o_matrix_add. M and S are
+// 9 n floats (column-major diagonal blocks); z may be null (zero). stats:
+// in/out running min / max of the PCG counts. wide: the widened-accumulator
+// dot products. Returns the Newton iteration count.
+typedef void (*b3o_model_fn)(void* user, uint32_t n, const float* x, const float* v, float* f, void* dfdx, void* dfdv);
+
+// (*m)(rows[k], cols[k]) += blocks[9 k..] (sign > 0) or -= (sign < 0) for k in order, the reference's
+// b3SparseMat33::operator() (sparse_mat33.h:268-288) then b3Mat33::operator+= / -=.
+void b3o_matrix_add(void* m, uint32_t count, const uint32_t* rows, const uint32_t* cols, const float* blocks, int sign) {
+	SpMat<float>& A = *(SpMat<float>*)m;
+	for (uint32_t k = 0; k < count; ++k) {
+		M33<float> K;
+		float* d = &K.x.x;
+		for (int t = 0; t < 9; ++t) d[t] = blocks[9 * (size_t)k + t];
+		if (sign > 0) A.at(rows[k], cols[k]) += K;
+		else A.at(rows[k], cols[k]) -= K;
+	}
+}
+
+uint32_t b3o_solve_be(uint32_t n, float h, const float* x0, const float* v0, const float* fe, const float* M,
+                      const float* y, const float* S, const float* z, uint32_t maxIt, float tol, uint32_t maxSub,
+                      float subTol, b3o_model_fn fn, void* user, int wide, float* x, float* v, float* error,
+                      uint32_t* subMinMax) {
+	auto vec = [n](const float* p) {
+		DVec<float> o(n);
+		for (uint32_t i = 0; i < n; ++i) o[i] = p ? MakeV3(p[3 * i], p[3 * i + 1], p[3 * i + 2]) : MakeV3(0.0f, 0.0f, 0.0f);
+		return o;
+	};
+	auto mat = [n](const float* p) {
+		DMat<float> o(n);
+		for (uint32_t i = 0; i < n; ++i) {
+			float* d = &o[i].x.x;
+			for (int t = 0; t < 9; ++t) d[t] = p[9 * (size_t)i + t];
+		}
+		return o;
+	};
+	DVec<float> X0 = vec(x0), V0 = vec(v0), FE = vec(fe), Y = vec(y), Z = vec(z), XO, VO;
+	DMat<float> MM = mat(M), SS = mat(S);
+	std::vector<float> xb(3 * (size_t)n), vb(3 * (size_t)n), fb(3 * (size_t)n);
+	auto model = [&](const DVec<float>& xx, const DVec<float>& vv, DVec<float>& f, SpMat<float>& dfdx, SpMat<float>& dfdv) {
+		for (uint32_t i = 0; i < n; ++i) {
+			xb[3 * i] = xx[i].x; xb[3 * i + 1] = xx[i].y; xb[3 * i + 2] = xx[i].z;
+			vb[3 * i] = vv[i].x; vb[3 * i + 1] = vv[i].y; vb[3 * i + 2] = vv[i].z;
+		}
+		std::fill(fb.begin(), fb.end(), 0.0f);
+		fn(user, n, xb.data(), vb.data(), fb.data(), &dfdx, &dfdv);
+		for (uint32_t i = 0; i < n; ++i) f[i] = MakeV3(fb[3 * i], fb[3 * i + 1], fb[3 * i + 2]);
+	};
+	StepStats st;
+	st.minSub = subMinMax[0];
+	st.maxSub = subMinMax[1];
+	float err = 0.0f;
+	const uint32_t it = SolveBEGeneric(h, maxIt, tol, maxSub, subTol, X0, V0, FE, MM, Y, SS, Z, XO, VO, model, wide != 0,
+	                                   st, (Capture<float>*)nullptr, &err);
+	for (uint32_t i = 0; i < n; ++i) {
+		x[3 * i] = XO[i].x; x[3 * i + 1] = XO[i].y; x[3 * i + 2] = XO[i].z;
+		v[3 * i] = VO[i].x; v[3 * i + 1] = VO[i].y; v[3 * i + 2] = VO[i].z;
+	}
+	*error = err;
+	subMinMax[0] = st.minSub;
+	subMinMax[1] = st.maxSub;
+	return it;
+}
+
 } // extern "C"

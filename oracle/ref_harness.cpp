@@ -584,6 +584,84 @@ uint32_t refh_solve_cg(uint32_t n, const uint32_t* rowptr, const uint32_t* cols,
 	return out.iterations;
 }
 
+// b3SparseSolveBE (sparse_force_solver.cpp:104-232) itself, with a force
+// model that hands the iterate to a callback: the callback fills f (zeroed)
+// and adds blocks to the two b3SparseMat33 through refh_matrix_add. M and S
+// are 9 n floats (column-major diagonal blocks); z may be null (zero).
+// subMinMax: in/out, the output's minSubIterations / maxSubIterations.
+typedef void (*refh_model_fn)(void* user, uint32_t n, const float* x, const float* v, float* f, void* dfdx, void* dfdv);
+
+// (*m)(rows[k], cols[k]) += / -= blocks[9 k..] for k in order (sign > 0: +=).
+void refh_matrix_add(void* m, uint32_t count, const uint32_t* rows, const uint32_t* cols, const float* blocks, int sign) {
+	b3SparseMat33& A = *(b3SparseMat33*)m;
+	for (uint32_t k = 0; k < count; ++k) {
+		b3Mat33 K;
+		scalar* d = &K.x.x;
+		for (int t = 0; t < 9; ++t) d[t] = blocks[9 * (size_t)k + t];
+		if (sign > 0) A(rows[k], cols[k]) += K;
+		else A(rows[k], cols[k]) -= K;
+	}
+}
+
+namespace {
+struct CallbackModel : public b3SparseForceModel {
+	refh_model_fn fn;
+	void* user;
+	uint32_t n;
+	std::vector<float> x, v, f;
+	void ApplyForces(const b3SparseForceSolverData* data) override {
+		for (uint32_t i = 0; i < n; ++i) {
+			const b3Vec3 xi = (*data->x)[i], vi = (*data->v)[i];
+			x[3 * i] = (float)xi.x; x[3 * i + 1] = (float)xi.y; x[3 * i + 2] = (float)xi.z;
+			v[3 * i] = (float)vi.x; v[3 * i + 1] = (float)vi.y; v[3 * i + 2] = (float)vi.z;
+		}
+		std::fill(f.begin(), f.end(), 0.0f);
+		fn(user, n, x.data(), v.data(), f.data(), data->dfdx, data->dfdv);
+		for (uint32_t i = 0; i < n; ++i) (*data->f)[i] = V(f.data() + 3 * i);
+	}
+};
+} // namespace
+
+uint32_t refh_solve_be(uint32_t n, float h, const float* x0, const float* v0, const float* fe, const float* M,
+                       const float* y, const float* S, const float* z, uint32_t maxIt, float tol, uint32_t maxSub,
+                       float subTol, refh_model_fn fn, void* user, float* x, float* v, float* error,
+                       uint32_t* subMinMax) {
+	b3DenseVec3 X0(n), V0(n), FE(n), Y(n), Z(n), X(n), VV(n);
+	b3DiagMat33 MM(n), SS(n);
+	for (uint32_t i = 0; i < n; ++i) {
+		X0[i] = V(x0 + 3 * i); V0[i] = V(v0 + 3 * i); FE[i] = V(fe + 3 * i); Y[i] = V(y + 3 * i);
+		Z[i] = z ? V(z + 3 * i) : b3Vec3(scalar(0), scalar(0), scalar(0));
+		scalar* m = &MM[i].x.x;
+		scalar* s = &SS[i].x.x;
+		for (int t = 0; t < 9; ++t) { m[t] = M[9 * (size_t)i + t]; s[t] = S[9 * (size_t)i + t]; }
+	}
+	CallbackModel model;
+	model.fn = fn; model.user = user; model.n = n;
+	model.x.resize(3 * (size_t)n); model.v.resize(3 * (size_t)n); model.f.resize(3 * (size_t)n);
+	b3SolveBEInput in;
+	in.h = h;
+	in.inv_h = h != scalar(0) ? scalar(1) / scalar(h) : scalar(0);
+	in.forceModel = &model;
+	in.dofCount = n;
+	in.x0 = &X0; in.v0 = &V0; in.fe = &FE; in.M = &MM; in.y = &Y; in.S = &SS; in.z = &Z;
+	in.maxIterations = maxIt; in.tolerance = tol;
+	in.maxSubIterations = maxSub; in.subTolerance = subTol;
+	b3SolveBEOutput out;
+	out.x = &X; out.v = &VV;
+	out.iterations = 0; out.error = scalar(0);
+	out.minSubIterations = subMinMax[0];
+	out.maxSubIterations = subMinMax[1];
+	b3SparseSolveBE(&out, &in);
+	for (uint32_t i = 0; i < n; ++i) {
+		x[3 * i] = (float)X[i].x; x[3 * i + 1] = (float)X[i].y; x[3 * i + 2] = (float)X[i].z;
+		v[3 * i] = (float)VV[i].x; v[3 * i + 1] = (float)VV[i].y; v[3 * i + 2] = (float)VV[i].z;
+	}
+	*error = (float)out.error;
+	subMinMax[0] = out.minSubIterations;
+	subMinMax[1] = out.maxSubIterations;
+	return out.iterations;
+}
+
 // Topology edits on the live reference body (body.cpp:62-354), creation
 // indices as the scene numbers them; new objects take the next index.
 void refh_destroy_force(void* p, uint32_t type, uint32_t k) {
