@@ -129,7 +129,7 @@ EXPORTS = [
     "b3g_solve_cg", "b3g_spmv", "b3g_set_reduction", "b3g_save_state", "b3g_load_state", "b3g_ray_cast", "b3g_ray_cast_device",
     "b3g_sdf_grid", "b3g_build_sdf", "b3g_body_rebuild", "b3g_set_action_forces", "b3g_get_action_forces",
     "b3g_solve_cg_static", "b3g_time_spmv_cold", "b3g_set_early_exit", "b3g_get_schedule", "b3g_set_schedule",
-    "b3g_autotune", "b3g_matrix_add", "b3g_matrix_sub", "b3g_be_create", "b3g_be_destroy", "b3g_be_set_reduction",
+    "b3g_autotune", "b3g_matrix_add", "b3g_matrix_sub", "b3g_matrix_set_row", "b3g_be_create", "b3g_be_destroy", "b3g_be_set_reduction",
     "b3g_be_input_defaults", "b3g_solve_be",
 ]
 
@@ -303,6 +303,7 @@ def lib():
         "b3g_set_reduction": (C.c_int, [vp, C.c_int]),
         "b3g_matrix_add": (C.c_int, [vp, C.c_uint32, u32p, u32p, f32p]),
         "b3g_matrix_sub": (C.c_int, [vp, C.c_uint32, u32p, u32p, f32p]),
+        "b3g_matrix_set_row": (C.c_int, [vp, C.c_uint32, C.c_uint32, u32p, f32p]),
         "b3g_be_create": (C.c_int, [C.POINTER(vp), C.c_int]),
         "b3g_be_destroy": (C.c_int, [vp]),
         "b3g_be_set_reduction": (C.c_int, [vp, C.c_int]),

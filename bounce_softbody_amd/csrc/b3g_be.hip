@@ -391,6 +391,27 @@ b3g_status b3g_matrix_sub(b3g_matrix m, uint32_t count, const uint32_t* rows, co
 	return matrix_apply(m, count, rows, cols, blocks, true);
 }
 
+b3g_status b3g_matrix_set_row(b3g_matrix m, uint32_t row, uint32_t count, const uint32_t* cols, const float* blocks) {
+	if (!m) { set_error("null matrix"); return B3G_INVALID_ARGUMENT; }
+	if (row >= m->n) { set_error("row " + std::to_string(row) + " outside a " + std::to_string(m->n) + "-row matrix"); return B3G_INVALID_ARGUMENT; }
+	if (count && (!cols || !blocks)) { set_error("null argument"); return B3G_INVALID_ARGUMENT; }
+	std::vector<uint32_t> sorted(cols, cols + count);
+	std::sort(sorted.begin(), sorted.end());
+	for (uint32_t k = 0; k < count; ++k) {
+		if (sorted[k] >= m->n) { set_error("column " + std::to_string(sorted[k]) + " out of range"); return B3G_INVALID_ARGUMENT; }
+		if (k && sorted[k] == sorted[k - 1]) { set_error("column " + std::to_string(sorted[k]) + " repeated in a row"); return B3G_INVALID_ARGUMENT; }
+	}
+	auto& r = m->rows[row];
+	r.clear();
+	for (uint32_t k = count; k-- > 0;) {   // the head is the vector's back
+		b3g_matrix_s::Entry e;
+		e.col = cols[k];
+		std::memcpy(&e.val, blocks + 9 * (size_t)k, sizeof(M33));
+		r.push_back(e);
+	}
+	return B3G_OK;
+}
+
 b3g_status b3g_be_create(b3g_be* out, int device) {
 	if (!out) { set_error("null argument"); return B3G_INVALID_ARGUMENT; }
 	*out = nullptr;

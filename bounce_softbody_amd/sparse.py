@@ -43,6 +43,15 @@ class SparseMat33:
     def sub(self, rows, cols, blocks):
         self._apply(_lib.lib().b3g_matrix_sub, rows, cols, blocks)
 
+    def set_row(self, row, cols, blocks):
+        """Row ``row`` becomes ``cols`` / ``blocks`` head to tail, values as given (``b3g_matrix_set_row``)."""
+        c = np.ascontiguousarray(cols, np.uint32).reshape(-1)
+        b = np.ascontiguousarray(blocks, np.float32).reshape(-1)
+        if len(b) != 9 * len(c):
+            raise ValueError("cols and blocks disagree in length")
+        check(_lib.lib().b3g_matrix_set_row(self._h, int(row), len(c), _ptr(c, u32p), _ptr(b, f32p)),
+              "b3g_matrix_set_row")
+
 
 @dataclass
 class SparseForceSolverData:

@@ -584,6 +584,15 @@ b3g_status b3g_matrix_add(b3g_matrix m, uint32_t count, const uint32_t* rows, co
 b3g_status b3g_matrix_sub(b3g_matrix m, uint32_t count, const uint32_t* rows, const uint32_t* cols,
                           const float* blocks);
 
+/*
+ * Replaces row `row` of m with `count` blocks, head to tail (the order in which
+ * a b3SparseMat33 row lists them from its head, sparse_mat33.h:31-48), values
+ * taken as given; columns must be distinct and below the row count. For an
+ * integration that lets an unchanged force model fill the reference's own
+ * b3SparseMat33 and then hands its rows over.
+ */
+b3g_status b3g_matrix_set_row(b3g_matrix m, uint32_t row, uint32_t count, const uint32_t* cols, const float* blocks);
+
 /* b3SparseForceSolverData (sparse_force_solver.h:31-39): what the force model reads and fills. */
 typedef struct b3g_force_data {
 	float h, inv_h;

@@ -22,6 +22,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libb3ref.so")
 REF64_SO = os.path.join(HERE, "_ref", "libb3ref64.so")   # the reference built with scalar = double
+# the reference with b3SparseSolveBE replaced by the integration of b3g_solve_be (an integration test vehicle)
+REF_BE_SO = os.path.join(HERE, "_ref", "libb3ref_be.so")
 
 u32p = C.POINTER(C.c_uint32)
 f32p = C.POINTER(C.c_float)
@@ -108,7 +110,7 @@ def build(reference: bool | None = None) -> None:
     """Builds liboracle.so (and _ref/ when the reference tree exists)."""
     targets = ["all"]
     if reference or (reference is None and os.path.isdir("/root/reference/src")):
-        targets += ["ref", "ref64"]
+        targets += ["ref", "ref64", "ref_be"]
     subprocess.check_call(["make", "-s", "-C", HERE, "-j8"] + targets)
 
 
@@ -123,7 +125,7 @@ _libs = {}
 
 def _lib(which: str):
     if which not in _libs:
-        raw = _load({"oracle": ORACLE_SO, "ref": REF_SO, "ref64": REF64_SO}[which])
+        raw = _load({"oracle": ORACLE_SO, "ref": REF_SO, "ref64": REF64_SO, "ref_be": REF_BE_SO}[which])
 
         class _Named:   # CDLL.__getitem__ builds a new function object per call; attribute access caches
             def __getitem__(self, name):

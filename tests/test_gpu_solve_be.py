@@ -138,3 +138,50 @@ def test_large_network_fp64_mode():
     w = cpu(problem(**kw), 2, 1e-6, 50, 1e-5, wide=True)
     assert rel(g[0], w[0]) <= 1e-6 and rel(g[1], w[1]) <= 1e-6
     assert g[2] == w[2] and abs(g[4][1] - w[4][1]) <= 1
+
+
+class ListMatrix:
+    """The reference's b3SparseMat33 list semantics in Python (a row's head is the end of its list):
+    operator()(i, j) searches from the head and inserts a zero block there when absent."""
+
+    def __init__(self, n):
+        self.rows = [[] for _ in range(n)]
+
+    def apply(self, sign, rows, cols, blocks):
+        for i, j, K in zip(rows, cols, np.asarray(blocks, np.float32).reshape(-1, 9)):
+            r = self.rows[int(i)]
+            for e in reversed(r):
+                if e[0] == j:
+                    e[1] = (e[1] + K if sign > 0 else e[1] - K).astype(np.float32)
+                    break
+            else:
+                z = np.zeros(9, np.float32)
+                r.append([int(j), (z + K if sign > 0 else z - K).astype(np.float32)])
+
+
+def test_rows_handed_over_with_set_row_equal_incremental_adds():
+    """An integration that lets the reference's own b3SparseMat33 collect the force model's blocks hands each
+    row over head to tail with b3g_matrix_set_row: the solve equals the one whose model adds block by block."""
+    kw, it, tol, sub, subtol = CASES["newton"]
+    p = problem(**kw)
+    model = p["model"]
+
+    def by_rows(data):
+        f, ops = model(data.x, data.v)
+        data.f[:] = f
+        mats = {"x": ListMatrix(len(f)), "v": ListMatrix(len(f))}
+        for target, sign, rows, cols, blocks in ops:
+            mats[target].apply(sign, rows, cols, blocks)
+        for target, dst in (("x", data.dfdx), ("v", data.dfdv)):
+            for i, r in enumerate(mats[target].rows):
+                if r:
+                    head_to_tail = r[::-1]
+                    dst.set_row(i, [e[0] for e in head_to_tail], np.stack([e[1] for e in head_to_tail]))
+
+    s = BESolver(reduction=_lib.REDUCE_SEQUENTIAL)
+    inp = SolveBEInput(h=float(p["h"]), inv_h=float(np.float32(1) / p["h"]), forceModel=by_rows, x0=p["x0"],
+                       v0=p["v0"], fe=p["fe"], M=p["M"], y=p["y"], S=p["S"], z=p["z"], maxIterations=it,
+                       tolerance=tol, maxSubIterations=sub, subTolerance=subtol)
+    out = s.solve(SolveBEOutput(), inp)
+    o = cpu(problem(**kw), it, tol, sub, subtol)
+    assert np.array_equal(out.x, o[0]) and np.array_equal(out.v, o[1]) and out.iterations == o[2]
