@@ -138,17 +138,19 @@ class Schedule(C.Structure):
     """``b3g_schedule`` (include/b3g.h): the PCG's SpMV tile shape, vector row-block size and FP64 sum place."""
 
     _fields_ = [("tile_blocks", C.c_uint32), ("tile_rows", C.c_uint32), ("row_block_rows", C.c_uint32),
-                ("fused", C.c_int32), ("max_groups", C.c_uint32)]
+                ("fused", C.c_int32), ("max_groups", C.c_uint32), ("persist_tiles", C.c_uint32),
+                ("persist_groups", C.c_uint32)]
 
     def as_tuple(self):
-        return (self.tile_blocks, self.tile_rows, self.row_block_rows, self.fused, self.max_groups)
+        return tuple(getattr(self, f) for f in SCHEDULE_FIELDS)
 
     def __repr__(self):
-        return "Schedule(tile_blocks=%d, tile_rows=%d, row_block_rows=%d, fused=%d, max_groups=%d)" % self.as_tuple()
+        return "Schedule(%s)" % ", ".join("%s=%d" % (f, getattr(self, f)) for f in SCHEDULE_FIELDS)
 
 
-SCHEDULE_FIELDS = ("tile_blocks", "tile_rows", "row_block_rows", "fused", "max_groups")
-DEFAULT_SCHEDULE = (1024, 256, 1024, -1, 0)
+SCHEDULE_FIELDS = ("tile_blocks", "tile_rows", "row_block_rows", "fused", "max_groups", "persist_tiles",
+                   "persist_groups")
+DEFAULT_SCHEDULE = (1024, 256, 1024, -1, 0, 0, 0)
 
 
 class ForceData(C.Structure):

@@ -302,15 +302,16 @@ class Body:
         check(self._L.b3g_set_early_exit(self._h, 1 if enabled else 0), "b3g_set_early_exit")
 
     def schedule(self):
-        """b3g_get_schedule: (tile_blocks, tile_rows, row_block_rows, fused, max_groups) of the PCG's launches."""
+        """b3g_get_schedule: (tile_blocks, tile_rows, row_block_rows, fused, max_groups, persist_tiles,
+        persist_groups) of the PCG's launches."""
         sc = _lib.Schedule()
         check(self._L.b3g_get_schedule(self._h, C.byref(sc)), "b3g_get_schedule")
         return sc.as_tuple()
 
     def set_schedule(self, tile_blocks: int, tile_rows: int, row_block_rows: int, fused: int = -1,
-                     max_groups: int = 0):
+                     max_groups: int = 0, persist_tiles: int = 0, persist_groups: int = 0):
         """b3g_set_schedule: set the PCG's launch schedule (needs a built body)."""
-        sc = _lib.Schedule(tile_blocks, tile_rows, row_block_rows, fused, max_groups)
+        sc = _lib.Schedule(tile_blocks, tile_rows, row_block_rows, fused, max_groups, persist_tiles, persist_groups)
         check(self._L.b3g_set_schedule(self._h, C.byref(sc)), "b3g_set_schedule")
 
     def autotune(self, cache_path: str | None = None):

@@ -216,6 +216,7 @@ b3g_status b3g_body_destroy(b3g_body h) {
 	drop_graph(h->body);
 	dfree_all(h->body);
 	free_exit_flags(h->body);
+	free_persistent(h->body);
 	(void)hipStreamDestroy(h->body.stream);
 	delete h;
 	return B3G_OK;
@@ -882,7 +883,7 @@ b3g_status b3g_get_schedule(b3g_body h, b3g_schedule* out) {
 	CHECK_BODY(h);
 	if (!out) return B3G_INVALID_ARGUMENT;
 	const Schedule& sc = h->body.sched;
-	*out = {sc.tileBlocks, sc.tileRows, sc.rowBlockRows, sc.fused, sc.maxGroups};
+	*out = {sc.tileBlocks, sc.tileRows, sc.rowBlockRows, sc.fused, sc.maxGroups, sc.persistTiles, sc.persistGroups};
 	return B3G_OK;
 }
 
@@ -891,9 +892,9 @@ static b3g_status check_schedule(const b3g_schedule* sc) {
 	if (!sc) { set_error("schedule is NULL"); return B3G_INVALID_ARGUMENT; }
 	if (sc->tile_blocks < 32 || sc->tile_blocks > 1024 || sc->tile_rows < 16 || sc->tile_rows > 256 ||
 	    sc->row_block_rows < 64 || sc->row_block_rows > 1024 || sc->fused < -1 || sc->fused > 1 ||
-	    (sc->max_groups != 0 && sc->max_groups < 256)) {
+	    (sc->max_groups != 0 && sc->max_groups < 256) || sc->persist_tiles > (1u << 20)) {
 		set_error("schedule out of range: tile_blocks 32..1024, tile_rows 16..256, row_block_rows 64..1024, fused -1..1, "
-		          "max_groups 0 or at least 256");
+		          "max_groups 0 or at least 256, persist_tiles at most 2^20");
 		return B3G_INVALID_ARGUMENT;
 	}
 	return B3G_OK;
@@ -971,6 +972,8 @@ b3g_status b3g_set_schedule(b3g_body h, const b3g_schedule* sc) {
 		b.sched.rowBlockRows = sc->row_block_rows;
 		b.sched.fused = sc->fused;
 		b.sched.maxGroups = sc->max_groups;
+		b.sched.persistTiles = sc->persist_tiles;
+		b.sched.persistGroups = sc->persist_groups;
 		return build_tiles(b, true, true);
 	});
 }
@@ -993,16 +996,19 @@ b3g_status b3g_autotune(b3g_body h, const char* cache_path, b3g_schedule* chosen
 				char key[32];
 				b3g_schedule sc;
 				double tm = -1.0, dm = -1.0;
-				const int got = std::sscanf(line, "%31s %u %u %u %d %u %lf %lf", key, &sc.tile_blocks, &sc.tile_rows,
-				                            &sc.row_block_rows, &sc.fused, &sc.max_groups, &tm, &dm);
-				if (got >= 6 && std::strcmp(key, sig) == 0 && check_schedule(&sc) == B3G_OK) {
+				const int got = std::sscanf(line, "%31s %u %u %u %d %u %u %u %lf %lf", key, &sc.tile_blocks, &sc.tile_rows,
+				                            &sc.row_block_rows, &sc.fused, &sc.max_groups, &sc.persist_tiles,
+				                            &sc.persist_groups, &tm, &dm);
+				if (got >= 8 && std::strcmp(key, sig) == 0 && check_schedule(&sc) == B3G_OK) {
 					found.tileBlocks = sc.tile_blocks;
 					found.tileRows = sc.tile_rows;
 					found.rowBlockRows = sc.row_block_rows;
 					found.fused = sc.fused;
 					found.maxGroups = sc.max_groups;
-					tuned = got >= 7 ? tm : -1.0;
-					dflt = got >= 8 ? dm : -1.0;
+					found.persistTiles = sc.persist_tiles;
+					found.persistGroups = sc.persist_groups;
+					tuned = got >= 9 ? tm : -1.0;
+					dflt = got >= 10 ? dm : -1.0;
 					hit = true;
 				}
 			}
@@ -1056,14 +1062,16 @@ b3g_status b3g_autotune(b3g_body h, const char* cache_path, b3g_schedule* chosen
 		if (cache_path) {
 			FILE* f = std::fopen(cache_path, "a");
 			if (!f) { set_error(std::string("cannot append to the schedule cache ") + cache_path); return B3G_INVALID_ARGUMENT; }
-			std::fprintf(f, "%s %u %u %u %d %u %.6f %.6f # ms per %s; rows %u blocks %u instances %u\n", sig,
+			std::fprintf(f, "%s %u %u %u %d %u %u %u %.6f %.6f # ms per %s; rows %u blocks %u instances %u\n", sig,
 			             b.sched.tileBlocks, b.sched.tileRows, b.sched.rowBlockRows, b.sched.fused, b.sched.maxGroups,
-			             tuned, dflt,
+			             b.sched.persistTiles, b.sched.persistGroups, tuned, dflt,
 			             blob.empty() ? "PCG iteration" : "step", b.n, b.nnz, b.nInst);
 			std::fclose(f);
 		}
 	}
-	if (chosen) *chosen = {b.sched.tileBlocks, b.sched.tileRows, b.sched.rowBlockRows, b.sched.fused, b.sched.maxGroups};
+	if (chosen)
+		*chosen = {b.sched.tileBlocks, b.sched.tileRows, b.sched.rowBlockRows, b.sched.fused, b.sched.maxGroups,
+		           b.sched.persistTiles, b.sched.persistGroups};
 	if (tuned_ms) *tuned_ms = tuned;
 	if (default_ms) *default_ms = dflt;
 	return B3G_OK;

@@ -160,6 +160,11 @@ struct Schedule {
 	uint32_t rowBlockRows = 1024;  // rows per vector-kernel block (at most 1024)
 	int fused = -1;                // FP64 mode: -1 by instance size, 0 separate, 1 fused
 	uint32_t maxGroups = 0;        // workgroups per PCG launch at most (0: one per tile or row block)
+	// persistent PCG: instances of at most persistTiles SpMV tiles (0: none)
+	// run all their iterations in one launch, persistGroups workgroups
+	// (0: one per such instance), beside the launch sequence of the others
+	uint32_t persistTiles = 0;
+	uint32_t persistGroups = 0;
 };
 
 // A body holds a batch of one or more independent instances (bodies of the
@@ -342,6 +347,20 @@ struct Body {
 	} lastStep;
 	struct StepWorker* worker = nullptr;  // the thread that enqueues eager steps (b3g_api.hip)
 
+	// ---------------------------------------------------------------- persistent PCG (Schedule::persistTiles)
+	// persist: uint32 per instance, 1 for the instances the persistent kernel
+	// solves (the launch sequence skips them after the solve's start); plist:
+	// those instances, largest first; stream2 runs the persistent kernel,
+	// forked from and joined to the body's stream by events
+	DBuf persist, plist;
+	// the persistent kernel's r.s partials (tiles, then row blocks): its
+	// instances run out of step with one another and with the launch sequence,
+	// whose partial array holds tile and row-block partials by turns
+	DBuf partialP;
+	uint32_t nPersist = 0;
+	hipStream_t stream2 = nullptr;
+	hipEvent_t forkEvent = nullptr, joinEvent = nullptr;
+
 	// ---------------------------------------------------------------- diagnostics
 	bool captureFi = false;
 
@@ -399,6 +418,8 @@ b3g_status ensure_cg_log(Body& b, uint32_t iterations);
 // Pinned flag memory and events of the PCG's early exit (allocated once, before any capture).
 b3g_status ensure_exit_flags(Body& b);
 void free_exit_flags(Body& b);
+// The persistent PCG's second stream and its events.
+void free_persistent(Body& b);
 
 // Operator-level backward Euler (b3g_be.hip): arguments of its kernels.
 // Matrices of the force model are in list order (rows head to tail), one

@@ -70,6 +70,9 @@ __device__ __forceinline__ void for_items(uint32_t count, const uint32_t* itemIn
 	}
 }
 
+// Whether the persistent kernel runs instance inst's PCG iterations (persist: per-instance flags, or null).
+__device__ __forceinline__ bool persistent(const uint32_t* persist, uint32_t inst) { return persist && persist[inst]; }
+
 __device__ __forceinline__ M33 ldA(const float* A, uint32_t stride, uint32_t k) {
 	M33 a;
 	float* p = &a.x.x;
@@ -182,6 +185,7 @@ struct Reduce {
 	const uint32_t* range;   // per instance: tiles, row blocks, rows (begin, end)
 	CGLog* log;              // per instance, logStride entries
 	uint32_t logStride;
+	const uint32_t* persist; // instances whose iterations the persistent kernel reduces itself, or null
 };
 
 // The workgroup's FP64 partial (each thread's `part` added in the block
@@ -262,6 +266,7 @@ __global__ void __launch_bounds__(kFinBlock) k_finalize(Reduce rd, int sel, int 
 	SolverState* st = rd.st + inst;
 	CGLog* log = rd.log + (size_t)inst * rd.logStride;
 	if (!st->newtonActive) return;
+	if ((kind == kFinAlpha || kind == kFinDelta) && persistent(rd.persist, inst)) return;
 	if (gate == kGateIter && !log[gk].active) return;
 	if (kind == kFinInit) clear_log(log, rd.logStride);
 	const uint32_t* R = rd.range + 6 * (size_t)inst;
@@ -1547,7 +1552,11 @@ struct CGArgs {
 	// column-sorted storage (Body::sortedA), or null: CSR storage with cols
 	const uint32_t* code;
 	const uint32_t* tileBase;
+	// per instance, 1 where the persistent kernel runs the PCG iterations (the
+	// launch sequence then skips the instance after the solve's start), or null
+	const uint32_t* persist;
 };
+
 
 __device__ __forceinline__ V3 diag_apply(V3 im, V3 r) { return mdiag3(im.x, im.y, im.z) * r; }
 
@@ -1675,7 +1684,8 @@ template <bool Multi, int St>
 __global__ void __launch_bounds__(kBlock) k_cg_residual(CGArgs a, int mode, uint32_t k) {
 	__shared__ float lds[3 * kTileCap];
 	auto ok = [&](uint32_t inst) {
-		return a.st[inst].newtonActive && (mode != kFinDelta || a.log[(size_t)inst * a.logStride + k].active);
+		return a.st[inst].newtonActive &&
+		       (mode != kFinDelta || (a.log[(size_t)inst * a.logStride + k].active && !persistent(a.persist, inst)));
 	};
 	for_items<Multi, true>(a.nTiles, a.tileInst, ok, [&](uint32_t t) {
 		const SolverState& st = a.st[a.tileInst[t]];
@@ -1707,7 +1717,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, uint32_t k, int
 	auto ok = [&](uint32_t inst) -> bool {
 		if (force) return true;
 		const CGLog* log = a.log + (size_t)inst * a.logStride;
-		if (!a.st[inst].newtonActive) return false;
+		if (!a.st[inst].newtonActive || persistent(a.persist, inst)) return false;
 		if (k > 0) return log[k - 1].active && (a.fused || log[k].active);
 		return log[0].active != 0;
 	};
@@ -1771,7 +1781,8 @@ __device__ __forceinline__ float cg_alpha(const CGArgs& a, uint32_t inst, uint32
 template <bool Multi, int RPT>
 __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k, int force) {
 	auto ok = [&](uint32_t inst) {
-		return force || (a.st[inst].newtonActive && a.log[(size_t)inst * a.logStride + k].active);
+		return force || (a.st[inst].newtonActive && a.log[(size_t)inst * a.logStride + k].active &&
+		                 !persistent(a.persist, inst));
 	};
 	for_items<Multi, false>(a.nRowBlk, a.blkInst, ok, [&](uint32_t blk) {
 		const uint32_t inst = a.blkInst[blk];
@@ -1813,7 +1824,9 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k, int 
 // x += alpha d (iterations that refresh the true residual)
 template <bool Multi>
 __global__ void __launch_bounds__(kBlock) k_cg_update_x(CGArgs a, uint32_t k) {
-	auto ok = [&](uint32_t inst) { return a.st[inst].newtonActive && a.log[(size_t)inst * a.logStride + k].active; };
+	auto ok = [&](uint32_t inst) {
+		return a.st[inst].newtonActive && a.log[(size_t)inst * a.logStride + k].active && !persistent(a.persist, inst);
+	};
 	for_items<Multi, false>(a.nRowBlk, a.blkInst, ok, [&](uint32_t blk) {
 		const uint32_t inst = a.blkInst[blk];
 		const uint32_t r0 = a.rowBlk[blk], r1 = a.rowBlk[blk + 1];
@@ -1866,6 +1879,121 @@ __global__ void __launch_bounds__(kBlock) k_clear_external(uint32_t n, V3* force
 	if (i >= n) return;
 	force[i] = zero3();
 	translation[i] = zero3();
+}
+
+// Persistent PCG (Schedule::persistTiles): each workgroup runs, in turn, the
+// whole PCG solve of the listed instances (sparse_solver.cpp:62-103) from the
+// state the solve's start left (r, s, delta_0, log[0]), iteration after
+// iteration in one launch, until the instance's own convergence test stops it.
+// Its SpMV tiles, row blocks, FP64 partials and their fixed-order sums are
+// the launch sequence's, in the same order, with the fused reductions' scalar
+// steps, and it writes the same log entries; so every instance's results are
+// bit-identical to the launch sequence's. Only this workgroup touches the
+// instance's rows: __syncthreads orders its stores before its loads. Its r.s
+// partials go to arrays of its own, tiles to partialT and row blocks to
+// partialK: its instances run out of step with one another and with the launch
+// sequence, whose partial array holds tile and row-block partials by turns
+// (one instance's tile indices overlap another's row-block indices).
+template <int St>
+__global__ void __launch_bounds__(kBlock) k_pcg_persistent(CGArgs a, const uint32_t* plist, uint32_t count, V3* d0,
+                                                           V3* d1, double* partialT, double* partialK) {
+	__shared__ float lds[3 * kTileCap];
+	for (uint32_t j = blockIdx.x; j < count; j += gridDim.x) {
+		const uint32_t inst = plist[j];
+		SolverState& st = a.st[inst];
+		CGLog* log = a.log + (size_t)inst * a.logStride;
+		const uint32_t* R = a.range + 6 * (size_t)inst;
+		const uint32_t t0 = R[0], t1 = R[1], k0 = R[2], k1 = R[3];
+		if (!st.newtonActive || !log[0].active) continue;
+		const uint32_t maxIt = st.maxIt;
+		const float eps2 = st.eps2_sub, delta0 = st.delta0;
+		float beta = 0.0f;
+		for (uint32_t k = 0;; ++k) {
+			V3* dold = (k & 1) ? d1 : d0;
+			V3* dnew = (k & 1) ? d0 : d1;
+			if (k > 0) {   // Delta(k), from the r.s partials of iteration k - 1 (tiles after a refresh)
+				const bool refreshed = (k - 1) % 50 == 0;
+				const float v = refreshed ? (float)wg_sum(partialT, t0, t1) : (float)wg_sum(partialK, k0, k1);
+				beta = v / log[k - 1].delta;
+				const bool act = k != maxIt && !(v <= eps2 * delta0);
+				if (threadIdx.x == 0) {
+					log[k].delta = v;
+					log[k].beta = beta;
+					log[k].active = act ? 1u : 0u;
+					st.it = k;
+				}
+				if (!act) break;
+			}
+			// q = A d, d = s + beta d_old (d = s at k = 0), partials of d.q (k_cg_spmv_dq)
+			const V3* dsrc = k == 0 ? a.s : dold;
+			auto dval = [&](uint32_t c) { return a.s[c] + beta * dsrc[c]; };
+			for (uint32_t t = t0; t < t1; ++t) {
+				uint32_t row;
+				V3 q;
+				tile_rows<St>(a, t, row, dval, lds, no_prologue, q);
+				float part = 0.0f;
+				const bool has = row < a.tiles[t + 1];
+				if (has) {
+					const V3 d = dval(row);
+					dnew[row] = d;
+					a.q[row] = q;
+					part = dot(d, q);
+				}
+				block_partial(part, has, row, a.partialQ, a.terms, false, t);
+			}
+			__syncthreads();
+			const float alpha = log[k].delta / (float)wg_sum(a.partialQ, t0, t1);   // Alpha(k)
+			if (threadIdx.x == 0) log[k].alpha = alpha;
+			if (k % 50 == 0) {
+				// x += alpha d, then the true residual r = b - A x, s = invM r (k_cg_update_x, k_cg_residual)
+				for (uint32_t blk = k0; blk < k1; ++blk) {
+					const uint32_t r0 = a.rowBlk[blk], r1 = a.rowBlk[blk + 1];
+#pragma unroll
+					for (int u = 0; u < kRowsPerThread; ++u) {
+						const uint32_t i = r0 + threadIdx.x + u * kBlock;
+						if (i >= r1) break;
+						a.px[i] = a.px[i] + alpha * dnew[i];
+					}
+				}
+				__syncthreads();
+				for (uint32_t t = t0; t < t1; ++t) {
+					uint32_t row;
+					V3 ax;
+					tile_rows<St>(a, t, row, [&](uint32_t c) { return a.px[c]; }, lds, no_prologue, ax);
+					float part = 0.0f;
+					const bool has = row < a.tiles[t + 1];
+					if (has) {
+						const V3 r = a.pb[row] - ax;
+						const V3 s = diag_apply(a.invM[row], r);
+						a.r[row] = r;
+						a.s[row] = s;
+						part = dot(r, s);
+					}
+					block_partial(part, has, row, partialT, a.terms, false, t);
+				}
+			} else {
+				// x += alpha d; r -= alpha q; s = invM r; partials of r.s (k_cg_update)
+				for (uint32_t blk = k0; blk < k1; ++blk) {
+					const uint32_t r0 = a.rowBlk[blk], r1 = a.rowBlk[blk + 1];
+					double part = 0.0;
+#pragma unroll
+					for (int u = 0; u < kRowsPerThread; ++u) {
+						const uint32_t i = r0 + threadIdx.x + u * kBlock;
+						if (i >= r1) break;
+						const V3 rn = a.r[i] - alpha * a.q[i];
+						const V3 sn = diag_apply(a.invM[i], rn);
+						a.px[i] = a.px[i] + alpha * dnew[i];
+						a.r[i] = rn;
+						a.s[i] = sn;
+						part += (double)dot(rn, sn);
+					}
+					block_store(part, partialK, blk);
+				}
+			}
+			__syncthreads();
+		}
+		__syncthreads();   // the next instance reuses the shared memory
+	}
 }
 
 // Plain SpMV y = A x (diagnostics / operator-level entry point), reference row order.
@@ -2049,6 +2177,12 @@ __global__ void __launch_bounds__(kBlock) k_x64_finish(RefineArgs a) {
 // Host launchers
 // ---------------------------------------------------------------------------
 static inline uint32_t nblk(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+// The persistent PCG serves the plain solver in the FP64 reduction modes (its
+// sums are the fused reductions'; the sequential mode keeps the launches).
+static inline bool use_persistent(const Body& b) {
+	return b.nPersist > 0 && b.persist.p && b.solver == B3G_SOLVER_PCG && b.reduction != B3G_REDUCE_SEQUENTIAL;
+}
 
 // Workgroups of a PCG launch over `items` tiles or row blocks (for_items):
 // one per item, or the schedule's maxGroups when that is at most half the
@@ -2246,6 +2380,31 @@ b3g_status build_tiles(Body& b, bool sorted, bool tiles) {
 	if ((s = dupload(b, b.rowBlk, rb.data(), rb.size() * 4)) != B3G_OK) return s;
 	if ((s = dupload(b, b.blkInst, bi.data(), bi.size() * 4)) != B3G_OK) return s;
 	if ((s = dupload(b, b.instRange, range.data(), range.size() * 4)) != B3G_OK) return s;
+	// the persistent PCG's instances: at most sched.persistTiles tiles, largest first
+	b.nPersist = 0;
+	if (b.sched.persistTiles > 0) {
+		std::vector<uint32_t> flag(K, 0u), list;
+		for (uint32_t k = 0; k < K; ++k) {
+			const uint32_t nt = range[6 * k + 1] - range[6 * k];
+			if (nt > 0 && nt <= b.sched.persistTiles) {
+				flag[k] = 1u;
+				list.push_back(k);
+			}
+		}
+		std::stable_sort(list.begin(), list.end(), [&](uint32_t u, uint32_t v) {
+			return range[6 * u + 5] - range[6 * u + 4] > range[6 * v + 5] - range[6 * v + 4];
+		});
+		if (!list.empty()) {
+			if ((s = dupload(b, b.persist, flag.data(), flag.size() * 4)) != B3G_OK) return s;
+			if ((s = dupload(b, b.plist, list.data(), list.size() * 4)) != B3G_OK) return s;
+			const size_t pk = ((size_t)b.nTiles + b.nRowBlk + 1) * sizeof(double);   // tile, then row-block partials
+			if ((!b.partialP.p || b.partialP.bytes < pk) && (s = dalloc(b, b.partialP, pk)) != B3G_OK) return s;
+			if (!b.stream2) B3G_HIP(hipStreamCreateWithFlags(&b.stream2, hipStreamNonBlocking));
+			if (!b.forkEvent) B3G_HIP(hipEventCreateWithFlags(&b.forkEvent, hipEventDisableTiming));
+			if (!b.joinEvent) B3G_HIP(hipEventCreateWithFlags(&b.joinEvent, hipEventDisableTiming));
+			b.nPersist = (uint32_t)list.size();
+		}
+	}
 	return ensure_partials(b, std::max(b.nTiles, b.nRowBlk));
 }
 
@@ -2407,6 +2566,7 @@ static CGArgs cg_args(Body& b) {
 	a.diagF = dptr<float>(b.diagF);
 	a.code = b.sortedA ? dptr<uint32_t>(b.code) : nullptr;
 	a.tileBase = b.sortedA ? dptr<uint32_t>(b.tileBase) : nullptr;
+	a.persist = use_persistent(b) ? dptr<uint32_t>(b.persist) : nullptr;
 	return a;
 }
 
@@ -2528,7 +2688,7 @@ static b3g_status enqueue_linearise(Body& b, float h) {
 
 static b3g_status enqueue_finalize(Body& b, const DBuf& partial, int sel, int kind, uint32_t k, int gate, uint32_t gk) {
 	Reduce rd{dptr<double>(partial), dptr<float>(b.terms), dptr<SolverState>(b.state), dptr<uint32_t>(b.instRange),
-	          dptr<CGLog>(b.cgLog), b.logStride};
+	          dptr<CGLog>(b.cgLog), b.logStride, use_persistent(b) ? dptr<uint32_t>(b.persist) : nullptr};
 	hipLaunchKernelGGL(k_finalize, dim3(std::max(1u, b.nInst)), dim3(kFinBlock), 0, b.stream, rd, sel, kind, k, gate, gk);
 	B3G_HIP(hipGetLastError());
 	return B3G_OK;
@@ -2549,9 +2709,11 @@ static inline int delta_range(uint32_t k) { return k % 50 == 0 ? kRangeTiles : k
 // active and its Newton loop is), and the largest iteration count so far,
 // written with the chunk's sequence number into pinned host memory.
 __global__ void __launch_bounds__(kBlock) k_pcg_flag(const SolverState* st, const CGLog* log, uint32_t stride,
-                                                     uint32_t nInst, uint32_t k, uint32_t seq, uint32_t* out) {
+                                                     uint32_t nInst, uint32_t k, uint32_t seq, uint32_t* out,
+                                                     const uint32_t* persist) {
 	uint32_t any = 0u, its = 0u;
 	for (uint32_t i = threadIdx.x; i < nInst; i += kBlock) {
+		if (persistent(persist, i)) continue;   // the persistent kernel's instances need no launches
 		if (st[i].newtonActive && log[(size_t)i * stride + k].active) any = 1u;
 		its = max(its, st[i].it);
 	}
@@ -2582,6 +2744,15 @@ b3g_status ensure_exit_flags(Body& b) {
 	b.hostFlag = static_cast<uint32_t*>(p);
 	for (hipEvent_t& e : b.flagEvent) B3G_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 	return B3G_OK;
+}
+
+void free_persistent(Body& b) {
+	if (b.stream2) (void)hipStreamSynchronize(b.stream2);
+	if (b.forkEvent) (void)hipEventDestroy(b.forkEvent);
+	if (b.joinEvent) (void)hipEventDestroy(b.joinEvent);
+	if (b.stream2) (void)hipStreamDestroy(b.stream2);
+	b.forkEvent = b.joinEvent = nullptr;
+	b.stream2 = nullptr;
 }
 
 void free_exit_flags(Body& b) {
@@ -2670,7 +2841,7 @@ static b3g_status enqueue_pcg_iterations(Body& b, const CGArgs& a, uint32_t maxI
 		if (b.flagSeq == 0u) b.flagSeq = 1u;   // the zeroed words never match a chunk
 		const uint32_t seq = b.flagSeq, slot = seq & 1u;
 		hipLaunchKernelGGL(k_pcg_flag, dim3(1), dim3(kBlock), 0, b.stream, a.st, a.log, a.logStride, std::max(1u, b.nInst),
-		                   k - 1, seq, b.hostFlag + 64 * slot);
+		                   k - 1, seq, b.hostFlag + 64 * slot, a.persist);
 		B3G_HIP(hipGetLastError());
 		B3G_HIP(hipEventRecord(b.flagEvent[slot], b.stream));
 		pend[npend++] = {slot, seq};
@@ -2695,9 +2866,31 @@ b3g_status enqueue_solve_cg(Body& b, uint32_t maxIt, float eps) {
 	b3g_status s = ensure_cg_log(b, maxIt + 1);
 	if (s != B3G_OK) return s;
 	CGArgs a = cg_args(b);
-	LAUNCH_TILES(k_cg_residual, a, b.nTiles, a, (int)kFinInit, 0u);
+	// the solve's start for every instance, the persistent ones included
+	CGArgs start = a;
+	start.persist = nullptr;
+	LAUNCH_TILES(k_cg_residual, start, b.nTiles, start, (int)kFinInit, 0u);
 	FINALIZE(b.partial, kRangeTiles, kFinInit, 0, kGateNewton, 0);
-	return enqueue_pcg_iterations(b, a, maxIt);
+	if (!a.persist) return enqueue_pcg_iterations(b, a, maxIt);
+	// the persistent instances' iterations on the second stream, beside the launch sequence of the others
+	B3G_HIP(hipEventRecord(b.forkEvent, b.stream));
+	B3G_HIP(hipStreamWaitEvent(b.stream2, b.forkEvent, 0));
+	const uint32_t g = b.sched.persistGroups ? std::min(b.sched.persistGroups, b.nPersist) : b.nPersist;
+	const uint32_t* plist = dptr<uint32_t>(b.plist);
+	V3* d0 = dptr<V3>(b.d0);
+	V3* d1 = dptr<V3>(b.d1);
+	double* pt = dptr<double>(b.partialP);
+	double* pk = pt + b.nTiles;
+	if (tile_storage(a) == 1)   // the plain solver: FP32 blocks, column-sorted or CSR storage
+		hipLaunchKernelGGL(k_pcg_persistent<1>, dim3(g), dim3(kBlock), 0, b.stream2, a, plist, b.nPersist, d0, d1, pt, pk);
+	else
+		hipLaunchKernelGGL(k_pcg_persistent<0>, dim3(g), dim3(kBlock), 0, b.stream2, a, plist, b.nPersist, d0, d1, pt, pk);
+	B3G_HIP(hipGetLastError());
+	B3G_HIP(hipEventRecord(b.joinEvent, b.stream2));
+	// every instance persistent: the launch sequence would only skip them
+	const b3g_status s2 = b.nPersist < std::max(1u, b.nInst) ? enqueue_pcg_iterations(b, a, maxIt) : B3G_OK;
+	B3G_HIP(hipStreamWaitEvent(b.stream, b.joinEvent, 0));   // joined even when the launch sequence failed
+	return s2;
 }
 
 static RefineArgs refine_args(Body& b) {

@@ -464,7 +464,14 @@ b3g_status b3g_set_early_exit(b3g_body body, int enabled);
  * all but rare ties. max_groups caps the workgroups of a PCG launch (0: one
  * per tile or row block); under a cap of at most half the tiles (row blocks)
  * each workgroup takes every G-th one and skips, 64 at a time, those of
- * converged instances. Default: 1024, 256, 1024, -1, 0.
+ * converged instances. persist_tiles > 0 runs the PCG iterations of every
+ * instance of at most that many SpMV tiles in one persistent launch beside
+ * the launch sequence of the others: one workgroup works through a whole
+ * solve of an instance, largest instances first, on persist_groups
+ * workgroups (0: one per such instance); its tiles, row blocks and sums are
+ * the launch sequence's, so the results are bit-identical. It serves the
+ * plain solver in the FP64 reduction modes. Default: 1024, 256, 1024, -1, 0,
+ * 0, 0.
  */
 typedef struct b3g_schedule {
 	uint32_t tile_blocks;
@@ -472,6 +479,8 @@ typedef struct b3g_schedule {
 	uint32_t row_block_rows;
 	int32_t fused;
 	uint32_t max_groups;
+	uint32_t persist_tiles;
+	uint32_t persist_groups;
 } b3g_schedule;
 
 b3g_status b3g_get_schedule(b3g_body body, b3g_schedule* schedule);

@@ -45,7 +45,7 @@ def test_default_schedule_and_validation():
         assert e.value.status == _lib.B3G_INVALID_ARGUMENT
     assert b.schedule() == _lib.DEFAULT_SCHEDULE
     b.set_schedule(512, 128, 256, 1, 4096)
-    assert b.schedule() == (512, 128, 256, 1, 4096)
+    assert b.schedule() == (512, 128, 256, 1, 4096, 0, 0)
 
 
 @pytest.mark.parametrize("sched", SCHEDULES)
@@ -129,8 +129,9 @@ def test_autotune_keeps_state_and_caches_its_choice(tmp_path):
     chosen, tuned_ms, default_ms = b.autotune(cache)   # times whole steps from this state, then restores it
     assert 0 < tuned_ms <= default_ms
     assert b.schedule() == chosen
-    tb, tr, rb, fu, mg = chosen
+    tb, tr, rb, fu, mg, pt, pg = chosen
     assert 32 <= tb <= 1024 and 16 <= tr <= 256 and 64 <= rb <= 1024 and fu in (-1, 0, 1) and (mg == 0 or mg >= 256)
+    assert pt <= 1 << 20
     assert all(np.array_equal(p, q) for p, q in zip(before, b.system()))
     x1, v1 = b.state()
     assert np.array_equal(x0, x1) and np.array_equal(v0, v1)
