@@ -394,6 +394,9 @@ def run_batch(args):
     t0 = time.time()
     batch = Body(arrs)
     batch.set_graph(not args.no_graph)
+    if args.persist_tiles:
+        from bounce_softbody_amd import _lib
+        batch.set_schedule(*_lib.DEFAULT_SCHEDULE[:5], args.persist_tiles, args.persist_groups)
     rows, blocks = batch.system_size()
     log(f"batch built on the device in {time.time() - t0:.1f} s: {rows} rows, {blocks} blocks")
 
@@ -485,7 +488,8 @@ def run_batch(args):
                    "pcg_iterations_min_max": [min(its), max(its)],
                    "subset_ms_per_step": {"bodies": len(sub), "handles": round(1e3 * t_handles, 3),
                                           "batch": round(1e3 * t_small, 3)},
-                   "schedule": schedule or "default"},
+                   "schedule": schedule or ({"persist_tiles": args.persist_tiles, "persist_groups": args.persist_groups}
+                                            if args.persist_tiles else "default")},
         "roofline": {"kernel": "k_cg_spmv_dq (all instances, back-to-back pass)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -737,6 +741,10 @@ def main():
     ap.add_argument("--cpu-sample-bodies", type=int, default=None,
                     help="batch / contact workloads: bodies in the single-thread CPU sample (default 256 / 16)")
     ap.add_argument("--no-graph", action="store_true", help="batch / contact workloads: uncaptured steps")
+    ap.add_argument("--persist-tiles", type=int, default=0,
+                    help="batch workload: persistent PCG for the instances of at most this many SpMV tiles (0: off)")
+    ap.add_argument("--persist-groups", type=int, default=0,
+                    help="batch workload: workgroups of the persistent PCG (0: one per persistent instance)")
     ap.add_argument("--autotune", action="store_true",
                     help="two_body / batch workloads: after the timed steps, autotune each body's PCG schedule and "
                          "time the steps again with it (reported as value; the default schedule's rate beside it)")

@@ -3198,6 +3198,19 @@ b3g_status autotune(Body& b, const std::function<b3g_status(double*)>& timer, do
 		if ((s = timeIt(&ms)) != B3G_OK) return s;
 		if (ms < bestMs) { bestMs = ms; best = sc; }
 	}
+	// the persistent PCG for a batch's smaller instances; whole steps only (the
+	// PCG-iteration timing before any step does not run it)
+	if (timer && b.nInst > 1 && b.solver == B3G_SOLVER_PCG && b.reduction != B3G_REDUCE_SEQUENTIAL) {
+		for (uint32_t pt : {4u, 16u, 64u}) {
+			Schedule sc = best;
+			sc.persistTiles = pt;
+			sc.persistGroups = 0;
+			if ((s = apply(sc)) != B3G_OK) return s;
+			double ms = 0.0;
+			if ((s = timeIt(&ms)) != B3G_OK) return s;
+			if (ms < bestMs) { bestMs = ms; best = sc; }
+		}
+	}
 	// the confirmation's means are reported (a search minimum is biased low)
 	bool keep = bestMs < dfltMs;
 	if (keep) {

@@ -489,7 +489,9 @@ b3g_status b3g_set_schedule(b3g_body body, const b3g_schedule* schedule);
 /*
  * Autotuner (no counterpart in the reference): times candidate schedules --
  * five SpMV tile shapes, three row-block sizes, both places of the FP64 sums,
- * then four workgroup caps for the best of those -- and keeps the fastest; a candidate that beats the default is timed
+ * then four workgroup caps for the best of those, then (a batch, after a
+ * step) the persistent PCG for instances of at most 4, 16 or 64 tiles -- and
+ * keeps the fastest; a candidate that beats the default is timed
  * against it twice more and kept only if it is faster by more than 1% both
  * times. After a
  * b3g_step each timing runs whole steps with that step's parameters from the
