@@ -19,7 +19,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_SO = os.path.join(HERE, "liboracle.so")
+# B3O_ORACLE_SO points the oracle at another build of the same sources (the sanitizer build of
+# tests/test_host_sanitizers.py)
+ORACLE_SO = os.environ.get("B3O_ORACLE_SO") or os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libb3ref.so")
 REF64_SO = os.path.join(HERE, "_ref", "libb3ref64.so")   # the reference built with scalar = double
 # the reference with b3SparseSolveBE replaced by the integration of b3g_solve_be (an integration test vehicle)
