@@ -756,10 +756,12 @@ def main():
     ap.add_argument("--sdf-res", type=int, default=128, help="contact and sdf_build workloads: SDF voxels per axis")
     ap.add_argument("--pre-steps", type=int, default=None,
                     help="precision / contact workloads: untimed steps first (default 10 / 40)")
-    ap.add_argument("--inner-iterations", type=int, default=50, help="precision workload: PCG iterations per sweep")
-    ap.add_argument("--inner-tolerance", type=float, default=1e-3, help="precision workload: PCG tolerance per sweep")
+    ap.add_argument("--inner-iterations", type=int, default=None,
+                    help="precision workloads: PCG iterations per sweep (default 30; contact: 50)")
+    ap.add_argument("--inner-tolerance", type=float, default=None,
+                    help="precision workloads: PCG tolerance per sweep (default 1e-2; contact: 1e-3)")
     ap.add_argument("--max-sweeps", type=int, default=None,
-                    help="precision workloads: refinement sweeps at most (default 4; batch_precision: 8)")
+                    help="precision workloads: refinement sweeps at most (default 4; batch_precision: 10)")
     ap.add_argument("--refine-tolerance", type=float, default=None,
                     help="precision workloads: FP64 residual target relative to the first sweep's (default 1e-4; "
                          "batch_precision: 1e-5)")
@@ -780,8 +782,14 @@ def main():
         args.warmup = 1 if bp else 5
     if args.cpu_sample_bodies is None:
         args.cpu_sample_bodies = 16 if contact else (8 if bp else 256)
+    # precision workloads: the fastest refined settings within 1e-5 of the FP64 oracle in sweeps of inner
+    # iterations / tolerances and sweep limits on the 64^3 grid and on the 4096-body batch
+    if args.inner_iterations is None:
+        args.inner_iterations = 50 if contact else 30
+    if args.inner_tolerance is None:
+        args.inner_tolerance = 1e-3 if contact else 1e-2
     if args.max_sweeps is None:
-        args.max_sweeps = 8 if bp else 4
+        args.max_sweeps = 10 if bp else 4
     if args.refine_tolerance is None:
         args.refine_tolerance = 1e-5 if bp else 1e-4
     if args.pre_steps is None:
