@@ -71,3 +71,37 @@ def test_reference_step_with_the_fp64_device_solve_stays_within_1e4(name):
             dev.set_state(rx, rv)   # the next step starts from the reference's state
     finally:
         set_mode(_lib.REDUCE_SEQUENTIAL)
+
+
+def test_reference_step_with_the_device_solve_at_full_size():
+    """The 64^3 tet grid (274,625 particles, 1.31 M tetrahedra): one step of the reference's own b3Body::Step with
+    b3SparseSolveBE on the GPU, PCG <= 50 at 1e-5, against the unmodified reference from the same set-up --
+    bit for bit in sequential mode, within 1e-4 in FP64 mode."""
+    import time
+    from bounce_softbody_amd import configs
+    arr = configs.tet_grid(64).arrays()
+    t0 = time.perf_counter()
+    ref = oracle.RefBody(arr, fast=True)
+    ref.step(configs.DT, 1, 50, 0.0, 1e-5)
+    t_ref = time.perf_counter() - t0
+    rx, rv = ref.state()
+    for mode in (_lib.REDUCE_SEQUENTIAL, _lib.REDUCE_FP64):
+        set_mode(mode)
+        try:
+            t0 = time.perf_counter()
+            dev = RefBodyBE(arr, fast=True)
+            dev.step(configs.DT, 1, 50, 0.0, 1e-5)
+            t_dev = time.perf_counter() - t0
+            assert last_status() == _lib.B3G_OK
+            dx, dv = dev.state()
+            print(f"64^3 mode {mode}: reference set-up + step {t_ref:.1f} s, with the device solve {t_dev:.1f} s",
+                  flush=True)
+            if mode == _lib.REDUCE_SEQUENTIAL:
+                assert np.array_equal(dx, rx) and np.array_equal(dv, rv)
+                assert np.array_equal(dev.stats(), ref.stats())
+            else:
+                assert np.abs(dx - rx).max() <= 1e-4 * np.abs(rx).max()
+                assert np.abs(dv - rv).max() <= 1e-4 * np.abs(rv).max()
+            del dev
+        finally:
+            set_mode(_lib.REDUCE_SEQUENTIAL)
