@@ -1212,6 +1212,20 @@ __device__ __forceinline__ M33 uv_block(uint32_t variant, const float4* r, bool 
 	return (dv ? -r[2].w : -r[1].w) * outer(dCi, dCj);
 }
 
+// Contribution kinds a body's slot lists hold (k_assemble_slots is instantiated
+// per combination, so a body with one kind runs no test of the code's kind bits).
+enum : int { kKindTet = 1, kKindUV = 2, kKindStaged = 4, kKindAll = 7 };
+
+template <int Kinds>
+__device__ __forceinline__ bool is_tet(uint32_t c) {
+	return (Kinds & kKindTet) && (Kinds == kKindTet || (c & kTetCode));
+}
+template <int Kinds>
+__device__ __forceinline__ bool is_uv(uint32_t c) {
+	return (Kinds & kKindUV) && (Kinds == kKindUV || (Kinds == (kKindTet | kKindUV)) || (c & kUVCode));
+}
+
+template <int Kinds>
 __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib& cb_, const uint32_t* ptr, uint32_t s0,
                                                 uint32_t s1, uint32_t s, bool own, float* lds, M33& acc) {
 	const uint32_t c0 = ptr[s0], c1 = ptr[s1];
@@ -1233,7 +1247,7 @@ __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib&
 			const uint32_t k = cb + threadIdx.x + u * kBlock;
 			if (k >= ce) continue;
 			const uint32_t c = code[u];
-			if (c & kTetCode) {
+			if (is_tet<Kinds>(c)) {
 				const uint32_t e = c & (kTetMaxElements - 1), blk = (c & ~kTetCode) >> kTetBlockShift;
 				const float* rec = a.tetRec + (size_t)kTetRecFloats * e;
 				const V3 gi = tet_grad(rec, (int)(blk & 3u)), gj = tet_grad(rec, (int)(blk >> 2));
@@ -1242,7 +1256,7 @@ __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib&
 				r[u][1] = make_float4(gj.x, gj.y, gj.z, s4.y);
 				r[u][2] = make_float4(s4.z, s4.w, 0.0f, 0.0f);
 				r[u][3] = a.tetQ[e];
-			} else if (c & kUVCode) {
+			} else if (is_uv<Kinds>(c)) {
 				const uint32_t variant = (c & ~kUVCode) >> kUVShift, e = c & (kUVMaxElements - 1);
 				if (variant < kShearVariant) {
 					const float4* rec = a.stRec + (size_t)(2 * e + variant / 9) * 3;
@@ -1269,18 +1283,18 @@ __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib&
 			if (k >= ce) continue;
 			const uint32_t c = code[u];
 			M33 v;
-			if (c & kTetCode) {
+			if (is_tet<Kinds>(c)) {
 				// K0_ij (b3g_tet.h)
 				v = tet_k0(v3(r[u][0].x, r[u][0].y, r[u][0].z), v3(r[u][1].x, r[u][1].y, r[u][1].z), r[u][0].w,
 				           r[u][1].w, r[u][2].x, r[u][2].y);
-			} else if (c & kUVCode) {
+			} else if (is_uv<Kinds>(c)) {
 				v = uv_block((c & ~kUVCode) >> kUVShift, r[u], cb_.dv);
 			} else {
 				v.x = v3(r[u][0].x, r[u][0].y, r[u][0].z);
 				v.y = v3(r[u][0].w, r[u][1].x, r[u][1].y);
 				v.z = v3(r[u][1].z, r[u][1].w, r[u][2].x);
 			}
-			if (c & kTetCode) {
+			if (is_tet<Kinds>(c)) {
 				// R K0_ij R^T of the tetrahedron (tetrahedron_element_force.cpp:400-420)
 				Q4 q;
 				q.v = v3(r[u][3].x, r[u][3].y, r[u][3].z);
@@ -1354,6 +1368,7 @@ __device__ __forceinline__ void tile_accumulate3(const uint32_t* ptr, const uint
 	}
 }
 
+template <int Kinds>
 __global__ void __launch_bounds__(kBlock) k_assemble_slots(AsmArgs a) {
 	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
 	if (!a.st[a.slotTileInst[t]].newtonActive) return;
@@ -1369,11 +1384,11 @@ __global__ void __launch_bounds__(kBlock) k_assemble_slots(AsmArgs a) {
 	const V3 w_j = own ? a.w[j] : zero3();
 	// dfdx(i, j): contributions in list order, then contacts (force_solver.cpp:70-86)
 	M33 dx = mzero();
-	tile_accumulate(a, Contrib{a.jxIdx, nullptr, false}, a.jxPtr, s0, s1, s, own, lds, dx);
+	tile_accumulate<Kinds>(a, Contrib{a.jxIdx, nullptr, false}, a.jxPtr, s0, s1, s, own, lds, dx);
 	// dfdv(i, j): particle mass damping first, then contributions
 	M33 dv = mzero();
 	if (own && i == j && damp_i > 0.0f) dv = dv + mdiag(-damp_i * mass_i);
-	if (a.anyJv) tile_accumulate(a, Contrib{a.jvIdx, a.tetKd, true}, a.jvPtr, s0, s1, s, own, lds, dv);
+	if (a.anyJv) tile_accumulate<Kinds>(a, Contrib{a.jvIdx, a.tetKd, true}, a.jvPtr, s0, s1, s, own, lds, dv);
 	if (!own) return;
 	const bool dyn_i = type_i == kDynamic;
 	if (i == j && sph_i >= 0 && dyn_i) {
@@ -2680,7 +2695,14 @@ static b3g_status enqueue_linearise(Body& b, float h) {
 	a.force = dptr<V3>(b.force);
 	a.gravity = dptr<V3>(b.gravity);
 	a.rowInst = dptr<uint32_t>(b.rowInst);
-	LAUNCH(k_assemble_slots, b.nSlotTiles, a);
+	// the contribution kinds of the body's slot lists: tetrahedra, stretch / shear records, staged blocks
+	const int kinds = (b.tet.m ? kKindTet : 0) | (b.stretch.m + b.shear.m ? kKindUV : 0) | (b.nJStage ? kKindStaged : 0);
+	switch (kinds) {
+	case kKindTet: LAUNCH(k_assemble_slots<kKindTet>, b.nSlotTiles, a); break;
+	case kKindUV: LAUNCH(k_assemble_slots<kKindUV>, b.nSlotTiles, a); break;
+	case kKindTet | kKindUV: LAUNCH(k_assemble_slots<kKindTet | kKindUV>, b.nSlotTiles, a); break;
+	default: LAUNCH(k_assemble_slots<kKindAll>, b.nSlotTiles, a); break;
+	}
 	LAUNCH(k_assemble_rows, b.nTiles, a);
 	if (precise) LAUNCH(k_assemble_rows64, b.nTiles, a);
 	return B3G_OK;
