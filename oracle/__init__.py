@@ -480,16 +480,23 @@ def solve_be(which: str, model, h, x0, v0, fe, M, y, S, z, max_it, tol, max_sub,
     add.argtypes = [C.c_void_p, C.c_uint32, u32p, u32p, f32p, C.c_int]
     n = len(x0)
 
+    raised = []
+
     def cb(user, n_, xp, vp, fp, dfdx, dfdv):
-        x = np.ctypeslib.as_array(xp, (n_, 3)).copy()
-        v = np.ctypeslib.as_array(vp, (n_, 3)).copy()
-        f, ops = model(x, v)
-        np.ctypeslib.as_array(fp, (n_, 3))[:] = np.asarray(f, np.float32)
-        for target, sign, rows, cols, blocks in ops:
-            r = np.ascontiguousarray(rows, np.uint32)
-            c = np.ascontiguousarray(cols, np.uint32)
-            b = np.ascontiguousarray(blocks, np.float32).reshape(-1)
-            add(dfdx if target == "x" else dfdv, len(r), _ptr(r, u32p), _ptr(c, u32p), _ptr(b, f32p), int(sign))
+        if raised:   # a model that failed once is not called again; the error is raised after the solve
+            return
+        try:
+            x = np.ctypeslib.as_array(xp, (n_, 3)).copy()
+            v = np.ctypeslib.as_array(vp, (n_, 3)).copy()
+            f, ops = model(x, v)
+            np.ctypeslib.as_array(fp, (n_, 3))[:] = np.asarray(f, np.float32)
+            for target, sign, rows, cols, blocks in ops:
+                r = np.ascontiguousarray(rows, np.uint32)
+                c = np.ascontiguousarray(cols, np.uint32)
+                b = np.ascontiguousarray(blocks, np.float32).reshape(-1)
+                add(dfdx if target == "x" else dfdv, len(r), _ptr(r, u32p), _ptr(c, u32p), _ptr(b, f32p), int(sign))
+        except BaseException as exc:   # noqa: BLE001 -- ctypes would print and drop it
+            raised.append(exc)
 
     fn = MODEL_FN(cb)
     f32 = lambda a: None if a is None else np.ascontiguousarray(a, np.float32).reshape(-1)   # noqa: E731
@@ -512,4 +519,6 @@ def solve_be(which: str, model, h, x0, v0, fe, M, y, S, z, max_it, tol, max_sub,
         args.append(1 if wide else 0)
     args += [_ptr(x, f32p), _ptr(v, f32p), C.byref(err), _ptr(mm, u32p)]
     it = fnc(*args)
+    if raised:
+        raise raised[0]
     return x.reshape(-1, 3), v.reshape(-1, 3), int(it), float(err.value), (int(mm[0]), int(mm[1]))

@@ -21,7 +21,8 @@ class SpringNetwork:
     along the current direction (dfdv -c n n^T) and per-particle damping (dfdv -c I). Elements contribute in
     a seeded order; diagonal blocks go in with +=, off-diagonal ones with -=."""
 
-    def __init__(self, pairs, k, L, damp_pairs=None, damp_k=None, damp_idx=None, damp_c=None):
+    def __init__(self, pairs, k, L, damp_pairs=None, damp_k=None, damp_idx=None, damp_c=None, tension_only=False):
+        self.tension_only = tension_only   # Jacobian blocks only for stretched springs (the pattern follows x)
         self.pairs = np.asarray(pairs, np.uint32)
         self.k = np.asarray(k, f32)
         self.L = np.asarray(L, f32)
@@ -45,14 +46,21 @@ class SpringNetwork:
         f = np.zeros((n, 3), f32)
         ops = []
         i, j = self.pairs[:, 0], self.pairs[:, 1]
+        if len(i) and self.tension_only:
+            _, ln = self._dir(x, i, j)
+            keep = ln > self.L
+            i, j = i[keep], j[keep]
+            kk, LL = self.k[keep], self.L[keep]
+        else:
+            kk, LL = self.k, self.L
         if len(i):
             nrm, ln = self._dir(x, i, j)
-            fs = ((-self.k * (ln - self.L))[:, None] * nrm).astype(f32)
+            fs = ((-kk * (ln - LL))[:, None] * nrm).astype(f32)
             np.add.at(f, i, fs)
             np.add.at(f, j, -fs)
             nn = (nrm[:, :, None] * nrm[:, None, :]).astype(f32)
-            a = (f32(1) - self.L / ln).astype(f32)
-            K = (-self.k[:, None, None] * (a[:, None, None] * (EYE - nn) + nn)).astype(f32)
+            a = (f32(1) - LL / ln).astype(f32)
+            K = (-kk[:, None, None] * (a[:, None, None] * (EYE - nn) + nn)).astype(f32)
             Kc = np.ascontiguousarray(K.transpose(0, 2, 1)).reshape(-1, 9)   # column-major
             ops.append(("x", +1, np.stack([i, j], 1).reshape(-1), np.stack([i, j], 1).reshape(-1),
                         np.repeat(Kc, 2, axis=0)))
@@ -79,7 +87,7 @@ class SpringNetwork:
         return f, ops
 
 
-def problem(shape=(6, 5, 4), seed=0, h=1.0 / 60.0, general=True, damping=True):
+def problem(shape=(6, 5, 4), seed=0, h=1.0 / 60.0, general=True, damping=True, tension_only=False):
     """A jittered grid of particles joined by edge and diagonal springs, elements in a seeded order. general:
     mass blocks with off-diagonal terms, constraint filters of three kinds (I, 0, a plane projection), a
     translation y and a desired sub-solution z; otherwise M = m I, S in {0, I}, y = z = 0 as b3ForceSolver
@@ -107,7 +115,7 @@ def problem(shape=(6, 5, 4), seed=0, h=1.0 / 60.0, general=True, damping=True):
         kw = dict(damp_pairs=pairs[dsel], damp_k=rng.uniform(0.1, 2.0, len(dsel)).astype(f32),
                   damp_idx=rng.permutation(n)[: n // 4].astype(np.uint32),
                   damp_c=rng.uniform(0.05, 0.5, n // 4).astype(f32))
-    model = SpringNetwork(pairs, k, L, **kw)
+    model = SpringNetwork(pairs, k, L, tension_only=tension_only, **kw)
     m = rng.uniform(0.05, 0.2, n).astype(f32)
     x0 = (rest + rng.uniform(-0.02, 0.02, (n, 3))).astype(f32)
     v0 = rng.uniform(-0.5, 0.5, (n, 3)).astype(f32)

@@ -19,6 +19,8 @@ CASES = {
     "h_zero": (dict(shape=(4, 4, 4), seed=6, general=True, h=0.0), 2, 1e-4, 30, 1e-5),
     "hh_underflow": (dict(shape=(4, 4, 4), seed=7, general=True, h=1e-23), 1, 1e-4, 30, 1e-5),
     "no_newton": (dict(shape=(3, 3, 3), seed=8, general=True), 0, 1e-4, 30, 1e-5),
+    # springs add Jacobian blocks only while stretched: the pattern changes between Newton iterations
+    "changing_pattern": (dict(shape=(6, 6, 4), seed=9, general=True, tension_only=True), 5, 1e-7, 60, 1e-6),
 }
 
 
