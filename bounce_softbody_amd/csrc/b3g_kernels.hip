@@ -437,9 +437,32 @@ __global__ void __launch_bounds__(kBlock) k_tet_forces(TetArgs a) {
 	if (!a.st[a.rowInst[a.p[e]]].newtonActive) return;
 	const uint32_t m = a.m;
 	uint32_t id[4];
+	// Only F and the warm start live through the polar iterations: the
+	// positions, rest positions and the stiffness record are (re)loaded after
+	// them (L2-hot), which keeps the loop's registers and the occupancy up.
+	M33 F;
+	{
+		V3 p[4];
+		for (int k = 0; k < 4; ++k) {
+			id[k] = a.p[(size_t)k * m + e];
+			p[k] = a.x[id[k]];
+		}
+		float rec[kTetRecFloats];
+		for (int k = 0; k < 3; ++k) {
+			const float4 r4 = a.rec[4 * (size_t)e + k];
+			rec[4 * k] = r4.x; rec[4 * k + 1] = r4.y; rec[4 * k + 2] = r4.z; rec[4 * k + 3] = r4.w;
+		}
+		const M33 Em = m33(p[1] - p[0], p[2] - p[0], p[3] - p[0]);
+		F = Em * tet_inv_e(rec);
+	}
+	const float4 q4 = a.q[e];
+	Q4 q;
+	q.v = v3(q4.x, q4.y, q4.z);
+	q.s = q4.w;
+	q = extract_rotation(F, q);
+	a.q[e] = make_float4(q.v.x, q.v.y, q.v.z, q.s);
 	V3 p[4], xr[4];
 	for (int k = 0; k < 4; ++k) {
-		id[k] = a.p[(size_t)k * m + e];
 		p[k] = a.x[id[k]];
 		xr[k] = v3(a.rest[(size_t)(3 * k) * m + e], a.rest[(size_t)(3 * k + 1) * m + e], a.rest[(size_t)(3 * k + 2) * m + e]);
 	}
@@ -448,36 +471,30 @@ __global__ void __launch_bounds__(kBlock) k_tet_forces(TetArgs a) {
 		const float4 r4 = a.rec[4 * (size_t)e + k];
 		rec[4 * k] = r4.x; rec[4 * k + 1] = r4.y; rec[4 * k + 2] = r4.z; rec[4 * k + 3] = r4.w;
 	}
-	M33 Em = m33(p[1] - p[0], p[2] - p[0], p[3] - p[0]);
-	M33 F = Em * tet_inv_e(rec);
-	const float4 q4 = a.q[e];
-	Q4 q;
-	q.v = v3(q4.x, q4.y, q4.z);
-	q.s = q4.w;
-	q = extract_rotation(F, q);
-	a.q[e] = make_float4(q.v.x, q.v.y, q.v.z, q.s);
 	M33 R = qmatrix(q);
 	M33 RT = transpose(R);
 	V3 us[4];
 	for (int k = 0; k < 4; ++k) us[k] = RT * p[k] - xr[k];
-	const float kd = a.kd[e];
-	V3 vs[4];
-	if (kd > 0.0f)
-		for (int k = 0; k < 4; ++k) vs[k] = a.v[id[k]];
 	for (int i = 0; i < 4; ++i) {
 		V3 fs = zero3();
-		V3 fds = zero3();
-		for (int j = 0; j < 4; ++j) {
-			M33 K0 = tet_k0(rec, i, j);   // m_K[i + 4 j]
-			fs = fs + K0 * us[j];
-			if (kd > 0.0f) {
-				M33 K = R * K0 * RT;
+		for (int j = 0; j < 4; ++j) fs = fs + tet_k0(rec, i, j) * us[j];   // m_K[i + 4 j]
+		a.fstage[a.f + (size_t)i * m + e] = -(R * fs);      // f[vi] -= R fs
+	}
+	// Damping in a pass of its own (same sums per vertex), so that the
+	// velocities and the displacements are never live together.
+	const float kd = a.kd[e];
+	if (kd > 0.0f) {
+		V3 vs[4];
+		for (int k = 0; k < 4; ++k) vs[k] = a.v[id[k]];
+		for (int i = 0; i < 4; ++i) {
+			V3 fds = zero3();
+			for (int j = 0; j < 4; ++j) {
+				M33 K = R * tet_k0(rec, i, j) * RT;
 				M33 kK = kd * K;
 				fds = fds + kK * vs[j];
 			}
+			a.fstage[a.fd + (size_t)i * m + e] = -fds;
 		}
-		a.fstage[a.f + (size_t)i * m + e] = -(R * fs);      // f[vi] -= R fs
-		if (kd > 0.0f) a.fstage[a.fd + (size_t)i * m + e] = -fds;
 	}
 }
 
