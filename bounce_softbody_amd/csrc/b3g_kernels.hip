@@ -1554,8 +1554,10 @@ struct CGArgs {
 	uint32_t stride;            // plane stride of A and length of cols (nnz rounded up to 4)
 	const uint32_t* tiles;
 	const uint32_t* tileInst;
+	const uint32_t* tileBlk;    // block boundaries of the tiles (rowptr at tiles)
 	const uint32_t* rowBlk;
 	const uint32_t* blkInst;
+	uint32_t single;            // one instance: every tile and row block is instance 0
 	const uint32_t* rowptr;
 	const uint32_t* cols;
 	const float* A;
@@ -1601,12 +1603,37 @@ __device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w 
 // tile (the function returns false).
 // Pairs: the BF16 operator, whose pair planes give each thread the blocks
 // 2l and 2l + 1 of the chunk (l = thread, thread + 256) from nine words.
+// A tile's rows [r0, r1), blocks [b0, b1), first column (column-sorted
+// storage) and first BF16 pair: independent loads, issued together.
+struct TileRange {
+	uint32_t r0, r1, b0, b1, base, pair;
+};
+template <bool Sorted, bool Pairs>
+__device__ __forceinline__ TileRange tile_range_(const CGArgs& a, uint32_t t) {
+	TileRange tr;
+	tr.r0 = a.tiles[t];
+	tr.r1 = a.tiles[t + 1];
+	tr.b0 = a.tileBlk[t];
+	tr.b1 = a.tileBlk[t + 1];
+	tr.base = Sorted ? a.tileBase[t] : 0u;
+	tr.pair = Pairs ? a.pairBase[t] : 0u;
+	return tr;
+}
+template <int St>
+__device__ __forceinline__ TileRange tile_range(const CGArgs& a, uint32_t t) {
+	return tile_range_<(St & 1) != 0, (St & 2) != 0>(a, t);
+}
+
+// Holds the uniform values v... in scalar registers at this point, so their
+// loads (independent of one another) are all issued ahead of it and waited
+// for once; without it the compiler moves each load down to its first use,
+// behind the branches before it, and the loads run one round trip each.
+#define B3G_SETTLE(...) asm volatile("" ::__VA_ARGS__)
+
 template <bool Sorted, bool Pairs, class Gather, class Before>
-__device__ __forceinline__ bool tile_rows_(const CGArgs& a, uint32_t t, uint32_t& row, Gather gather, float* lds,
-                                           Before before, V3& acc) {
-	const uint32_t r0 = a.tiles[t], r1 = a.tiles[t + 1];
-	const uint32_t b0 = a.rowptr[r0], b1 = a.rowptr[r1];
-	const uint32_t base = Sorted ? a.tileBase[t] : 0u;
+__device__ __forceinline__ bool tile_rows_(const CGArgs& a, const TileRange& tr, uint32_t& row, Gather gather,
+                                           float* lds, Before before, V3& acc) {
+	const uint32_t r0 = tr.r0, r1 = tr.r1, b0 = tr.b0, b1 = tr.b1, base = tr.base;
 	row = r0 + threadIdx.x;
 	const bool own = row < r1;
 	uint32_t rb = own ? a.rowptr[row] : 0, re = own ? a.rowptr[row + 1] : 0;
@@ -1615,29 +1642,39 @@ __device__ __forceinline__ bool tile_rows_(const CGArgs& a, uint32_t t, uint32_t
 		lds[3 * i + 1] = p.y;
 		lds[3 * i + 2] = p.z;
 	};
+	// The thread's columns first, so each block's gathers issue together with
+	// its matrix loads; a lane past the chunk loads the chunk's first block
+	// (in bounds) and stores nothing, which keeps the loads free of branches.
+	// With the column-sorted storage the block at storage position k has its
+	// column and its CSR position (where its product goes) in code[k].
+	auto local = [](int u) -> uint32_t {
+		return Pairs ? 2 * (threadIdx.x + (u >> 1) * kBlock) + (u & 1) : threadIdx.x + u * kBlock;
+	};
+	uint32_t raw[kRowPer];
+	auto load_codes = [&](uint32_t cb, uint32_t ce) {
+#pragma unroll
+		for (int u = 0; u < kRowPer; ++u) {
+			const uint32_t k = cb + local(u), kk = k < ce ? k : cb;
+			raw[u] = Sorted ? a.code[kk] : a.cols[kk];
+		}
+	};
+	// the first chunk's codes are requested before the prologue, whose loads
+	// then share their round trip
+	if (b0 < b1) load_codes(b0, min(b0 + (uint32_t)kTileCap, b1));
 	acc = zero3();
 	if (!before()) return false;
 	for (uint32_t cb = b0; cb < b1; cb += kTileCap) {
 		const uint32_t ce = min(cb + (uint32_t)kTileCap, b1);
-		// The thread's columns first, so each block's gathers issue together
-		// with its matrix loads; a lane past the chunk loads the chunk's first
-		// block (in bounds) and stores nothing, which keeps the loads free of
-		// branches. With the column-sorted storage the block at storage
-		// position k has its column and its CSR position (where its product
-		// goes) in code[k].
+		if (cb != b0) load_codes(cb, ce);
 		uint32_t col[kRowPer], dst[kRowPer];
-		auto local = [](int u) -> uint32_t {
-			return Pairs ? 2 * (threadIdx.x + (u >> 1) * kBlock) + (u & 1) : threadIdx.x + u * kBlock;
-		};
 #pragma unroll
 		for (int u = 0; u < kRowPer; ++u) {
 			const uint32_t k = cb + local(u), kk = k < ce ? k : cb;
 			if (Sorted) {
-				const uint32_t c = a.code[kk];
-				col[u] = base + (c >> 10);
-				dst[u] = c & 1023u;
+				col[u] = base + (raw[u] >> 10);
+				dst[u] = raw[u] & 1023u;
 			} else {
-				col[u] = a.cols[kk];
+				col[u] = raw[u];
 				dst[u] = kk - cb;
 			}
 		}
@@ -1652,7 +1689,7 @@ __device__ __forceinline__ bool tile_rows_(const CGArgs& a, uint32_t t, uint32_t
 			}
 		};
 		if (Pairs) {
-			const uint32_t p0 = a.pairBase[t] + ((cb - b0) >> 1);   // cb - b0 is a multiple of kTileCap
+			const uint32_t p0 = tr.pair + ((cb - b0) >> 1);   // cb - b0 is a multiple of kTileCap
 #pragma unroll
 			for (int j = 0; j < kRowPer / 2; ++j) {
 				const uint32_t l = threadIdx.x + j * kBlock, k = cb + 2 * l;
@@ -1703,9 +1740,9 @@ __device__ __forceinline__ bool tile_rows_(const CGArgs& a, uint32_t t, uint32_t
 // reads pA is instantiated per storage (tile_storage picks one at launch):
 // one kernel with every storage inside ran the FP32 step 3.5 % slower.
 template <int St, class Gather, class Before>
-__device__ __forceinline__ bool tile_rows(const CGArgs& a, uint32_t t, uint32_t& row, Gather gather, float* lds,
-                                          Before before, V3& acc) {
-	return tile_rows_<(St & 1) != 0, (St & 2) != 0>(a, t, row, gather, lds, before, acc);
+__device__ __forceinline__ bool tile_rows(const CGArgs& a, const TileRange& tr, uint32_t& row, Gather gather,
+                                          float* lds, Before before, V3& acc) {
+	return tile_rows_<(St & 1) != 0, (St & 2) != 0>(a, tr, row, gather, lds, before, acc);
 }
 
 __device__ __forceinline__ bool no_prologue() { return true; }
@@ -1715,17 +1752,23 @@ __device__ __forceinline__ bool no_prologue() { return true; }
 template <bool Multi, int St>
 __global__ void __launch_bounds__(kBlock) k_cg_residual(CGArgs a, int mode, uint32_t k) {
 	__shared__ float lds[3 * kTileCap];
-	auto ok = [&](uint32_t inst) {
-		return a.st[inst].newtonActive &&
-		       (mode != kFinDelta || (a.log[(size_t)inst * a.logStride + k].active && !persistent(a.persist, inst)));
-	};
-	for_items<Multi, true>(a.nTiles, a.tileInst, ok, [&](uint32_t t) {
-		const SolverState& st = a.st[a.tileInst[t]];
+	// the tile's bounds and the instance's gate are read together, before
+	// anything depends on them (every address is valid whether or not the
+	// tile runs)
+	auto tile = [&](uint32_t t, uint32_t inst) {
+		const TileRange tr = tile_range<St>(a, t);
+		const SolverState& st = a.st[inst];
+		const uint32_t active = st.newtonActive, seq = st.seq;
+		const uint32_t logActive = a.log[(size_t)inst * a.logStride + k].active;
+		const uint32_t persist = a.persist ? a.persist[inst] : 0u;
+		B3G_SETTLE("s"(tr.r0), "s"(tr.r1), "s"(tr.b0), "s"(tr.b1), "s"(tr.base), "s"(tr.pair), "s"(active), "s"(seq),
+		           "s"(logActive), "s"(persist));
+		if (!active || (mode == kFinDelta && (!logActive || persist))) return;
 		uint32_t row;
 		V3 ax;
-		tile_rows<St>(a, t, row, [&](uint32_t c) { return a.px[c]; }, lds, no_prologue, ax);
+		tile_rows<St>(a, tr, row, [&](uint32_t c) { return a.px[c]; }, lds, no_prologue, ax);
 		float part = 0.0f;
-		const bool has = row < a.tiles[t + 1];
+		const bool has = row < tr.r1;
 		if (has) {
 			V3 r = a.pb[row] - ax;
 			V3 s = diag_apply(a.invM[row], r);
@@ -1733,8 +1776,19 @@ __global__ void __launch_bounds__(kBlock) k_cg_residual(CGArgs a, int mode, uint
 			a.s[row] = s;
 			part = dot(r, s);
 		}
-		block_partial(part, has, row, a.partial, a.terms, st.seq != 0, t);
-	});
+		block_partial(part, has, row, a.partial, a.terms, seq != 0, t);
+	};
+	if (!Multi) {   // instance 0 as a constant: its reads need no load of the tile's instance first
+		const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+		if (a.single) tile(t, 0u);
+		else tile(t, a.tileInst[t]);
+		return;
+	}
+	auto ok = [&](uint32_t inst) {
+		return a.st[inst].newtonActive &&
+		       (mode != kFinDelta || (a.log[(size_t)inst * a.logStride + k].active && !persistent(a.persist, inst)));
+	};
+	for_items<Multi, true>(a.nTiles, a.tileInst, ok, [&](uint32_t t) { tile(t, a.tileInst[t]); });
 }
 
 // PCG iteration k: q = A d with d = s + beta_k d_old formed on the fly
@@ -1746,28 +1800,36 @@ __global__ void __launch_bounds__(kBlock) k_cg_residual(CGArgs a, int mode, uint
 template <bool Multi, int St>
 __global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, uint32_t k, int sel, int force) {
 	__shared__ float lds[3 * kTileCap];
-	auto ok = [&](uint32_t inst) -> bool {
-		if (force) return true;
-		const CGLog* log = a.log + (size_t)inst * a.logStride;
-		if (!a.st[inst].newtonActive || persistent(a.persist, inst)) return false;
-		if (k > 0) return log[k - 1].active && (a.fused || log[k].active);
-		return log[0].active != 0;
-	};
-	for_items<Multi, true>(a.nTiles, a.tileInst, ok, [&](uint32_t t) {
-		const uint32_t inst = a.tileInst[t];
+	// The tile's bounds, the instance's gate and the scalars of the fused
+	// prologue are read together, before anything depends on them (every
+	// address is valid whether or not the tile runs): one round trip ahead of
+	// the tile's column codes instead of a chain of dependent ones.
+	auto tile = [&](uint32_t t, uint32_t inst) {
+		const TileRange tr = tile_range<St>(a, t);
 		SolverState& st = a.st[inst];
 		CGLog* log = a.log + (size_t)inst * a.logStride;
+		const uint32_t* R = a.range + 6 * (size_t)inst;
+		const uint32_t km = k > 0 ? k - 1 : 0u;
+		const uint32_t active = st.newtonActive, seq = st.seq, maxIt = st.maxIt;
+		const float eps2 = st.eps2_sub, delta0 = st.delta0;
+		const uint32_t prevActive = log[km].active, curActive = log[k].active;
+		const float prevDelta = log[km].delta, logBeta = log[k].beta;
+		const uint32_t p0 = R[sel], p1 = R[sel + 1], firstTile = R[0];
+		const uint32_t persist = a.persist ? a.persist[inst] : 0u;
+		B3G_SETTLE("s"(tr.r0), "s"(tr.r1), "s"(tr.b0), "s"(tr.b1), "s"(tr.base), "s"(tr.pair), "s"(active), "s"(seq),
+		           "s"(maxIt), "s"(eps2), "s"(delta0), "s"(prevActive), "s"(curActive), "s"(prevDelta), "s"(logBeta),
+		           "s"(p0), "s"(p1), "s"(firstTile), "s"(persist));
+		// log[k].active is this launch's own output in fused mode, and read only otherwise
+		if (!force && !(active && !persist && prevActive && (k == 0 || a.fused || curActive))) return;
 		float beta = 0.0f;
-		if (k > 0 && !a.fused) beta = log[k].beta;
+		if (k > 0 && !a.fused) beta = logBeta;
 		// fused mode: delta_k / beta_k from the partials
 		auto prologue = [&]() -> bool {
 			if (k == 0 || !a.fused) return true;
-			const uint32_t* R = a.range + 6 * (size_t)inst;
-			const float v = (float)wg_sum(a.partial, R[sel], R[sel + 1]);
-			const float prev = log[k - 1].delta;
-			beta = v / prev;
-			const bool act = k != st.maxIt && !(v <= st.eps2_sub * st.delta0);
-			if (!force && t == R[0] && threadIdx.x == 0) {
+			const float v = (float)wg_sum(a.partial, p0, p1);
+			beta = v / prevDelta;
+			const bool act = k != maxIt && !(v <= eps2 * delta0);
+			if (!force && t == firstTile && threadIdx.x == 0) {
 				log[k].delta = v;
 				log[k].beta = beta;
 				log[k].active = act ? 1u : 0u;
@@ -1782,17 +1844,31 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, uint32_t k, int
 		auto dval = [&](uint32_t c) { return a.s[c] + beta * dsrc[c]; };
 		uint32_t row;
 		V3 q;
-		if (!tile_rows<St>(a, t, row, dval, lds, prologue, q)) return;
+		if (!tile_rows<St>(a, tr, row, dval, lds, prologue, q)) return;
 		float part = 0.0f;
-		const bool has = row < a.tiles[t + 1];
+		const bool has = row < tr.r1;
 		if (has) {
 			V3 d = dval(row);
 			a.dnew[row] = d;
 			a.q[row] = q;
 			part = dot(d, q);
 		}
-		block_partial(part, has, row, a.partialQ, a.terms, st.seq != 0, t);
-	});
+		block_partial(part, has, row, a.partialQ, a.terms, seq != 0, t);
+	};
+	if (!Multi) {   // instance 0 as a constant: its reads need no load of the tile's instance first
+		const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+		if (a.single) tile(t, 0u);
+		else tile(t, a.tileInst[t]);
+		return;
+	}
+	auto ok = [&](uint32_t inst) -> bool {
+		if (force) return true;
+		const CGLog* log = a.log + (size_t)inst * a.logStride;
+		if (!a.st[inst].newtonActive || persistent(a.persist, inst)) return false;
+		if (k > 0) return log[k - 1].active && (a.fused || log[k].active);
+		return log[0].active != 0;
+	};
+	for_items<Multi, true>(a.nTiles, a.tileInst, ok, [&](uint32_t t) { tile(t, a.tileInst[t]); });
 }
 
 // alpha_k of a vector kernel of iteration k: fused mode sums the SpMV's d.q
@@ -1812,29 +1888,48 @@ __device__ __forceinline__ float cg_alpha(const CGArgs& a, uint32_t inst, uint32
 // depend only on the block size); fewer rows per thread hold fewer registers.
 template <bool Multi, int RPT>
 __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k, int force) {
-	auto ok = [&](uint32_t inst) {
-		return force || (a.st[inst].newtonActive && a.log[(size_t)inst * a.logStride + k].active &&
-		                 !persistent(a.persist, inst));
-	};
-	for_items<Multi, false>(a.nRowBlk, a.blkInst, ok, [&](uint32_t blk) {
-		const uint32_t inst = a.blkInst[blk];
+	// The block's bounds, the instance's gate and the scalars of alpha are
+	// read together before anything depends on them (every address is valid
+	// whether or not the block runs). With one instance the rows are loaded
+	// before the gate as well (in bounds whatever it says); in a batch only
+	// after it, so converged instances cost no row traffic.
+	auto block = [&](uint32_t blk, uint32_t inst) {
 		const uint32_t r0 = a.rowBlk[blk], r1 = a.rowBlk[blk + 1];
-		const bool seq = a.st[inst].seq != 0;
+		const SolverState& st = a.st[inst];
+		CGLog* log = a.log + (size_t)inst * a.logStride;
+		const uint32_t* R = a.range + 6 * (size_t)inst;
+		const uint32_t active = st.newtonActive, seq = st.seq, logActive = log[k].active;
+		const float delta = log[k].delta, logAlpha = log[k].alpha;
+		const uint32_t q0 = R[0], q1 = R[1], firstBlk = R[2];
+		const uint32_t persist = a.persist ? a.persist[inst] : 0u;
+		B3G_SETTLE("s"(r0), "s"(r1), "s"(active), "s"(seq), "s"(logActive), "s"(delta), "s"(logAlpha), "s"(q0), "s"(q1),
+		           "s"(firstBlk), "s"(persist));
 		// every input of the thread's rows is loaded before the first store (the
 		// stores could alias them as far as the compiler knows), and before the
 		// alpha prologue, whose partial sums then overlap these loads; a row past
 		// the block loads row r0 and stores nothing
 		V3 d[RPT], x[RPT], r[RPT], q[RPT], im[RPT];
+		auto load = [&]() {
 #pragma unroll
-		for (int u = 0; u < RPT; ++u) {
-			const uint32_t i = r0 + threadIdx.x + u * kBlock, ii = i < r1 ? i : r0;
-			d[u] = a.dnew[ii];
-			x[u] = a.px[ii];
-			r[u] = a.r[ii];
-			q[u] = a.q[ii];
-			im[u] = a.invM[ii];
+			for (int u = 0; u < RPT; ++u) {
+				const uint32_t i = r0 + threadIdx.x + u * kBlock, ii = i < r1 ? i : r0;
+				d[u] = a.dnew[ii];
+				x[u] = a.px[ii];
+				r[u] = a.r[ii];
+				q[u] = a.q[ii];
+				im[u] = a.invM[ii];
+			}
+		};
+		if (a.single) load();
+		if (!force && !(active && logActive && !persist)) return;
+		if (!a.single) load();
+		// alpha_k (Alpha(k)): fused mode sums the SpMV's d.q partials of the
+		// instance, and the instance's first block records it
+		float alpha = logAlpha;
+		if (a.fused) {
+			alpha = delta / (float)wg_sum(a.partialQ, q0, q1);
+			if (blk == firstBlk && threadIdx.x == 0) log[k].alpha = alpha;
 		}
-		const float alpha = cg_alpha(a, inst, blk, k);
 		double part = 0.0;
 #pragma unroll
 		for (int u = 0; u < RPT; ++u) {
@@ -1850,18 +1945,29 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k, int 
 			part += (double)term;
 		}
 		block_store(part, a.partial, blk);
-	});
+	};
+	if (!Multi) {   // instance 0 as a constant: its reads need no load of the block's instance first
+		const uint32_t blk = blockIdx.x;
+		if (a.single) block(blk, 0u);
+		else block(blk, a.blkInst[blk]);
+		return;
+	}
+	auto ok = [&](uint32_t inst) {
+		return force || (a.st[inst].newtonActive && a.log[(size_t)inst * a.logStride + k].active &&
+		                 !persistent(a.persist, inst));
+	};
+	for_items<Multi, false>(a.nRowBlk, a.blkInst, ok, [&](uint32_t blk) { block(blk, a.blkInst[blk]); });
 }
 
 // x += alpha d (iterations that refresh the true residual)
 template <bool Multi>
 __global__ void __launch_bounds__(kBlock) k_cg_update_x(CGArgs a, uint32_t k) {
-	auto ok = [&](uint32_t inst) {
-		return a.st[inst].newtonActive && a.log[(size_t)inst * a.logStride + k].active && !persistent(a.persist, inst);
-	};
-	for_items<Multi, false>(a.nRowBlk, a.blkInst, ok, [&](uint32_t blk) {
-		const uint32_t inst = a.blkInst[blk];
+	auto block = [&](uint32_t blk, uint32_t inst) {
 		const uint32_t r0 = a.rowBlk[blk], r1 = a.rowBlk[blk + 1];
+		const uint32_t active = a.st[inst].newtonActive, logActive = a.log[(size_t)inst * a.logStride + k].active;
+		const uint32_t persist = a.persist ? a.persist[inst] : 0u;
+		B3G_SETTLE("s"(r0), "s"(r1), "s"(active), "s"(logActive), "s"(persist));
+		if (!(active && logActive && !persist)) return;
 		V3 d[kRowsPerThread], x[kRowsPerThread];
 #pragma unroll
 		for (int u = 0; u < kRowsPerThread; ++u) {
@@ -1876,7 +1982,17 @@ __global__ void __launch_bounds__(kBlock) k_cg_update_x(CGArgs a, uint32_t k) {
 			if (i >= r1) break;
 			a.px[i] = x[u] + alpha * d[u];
 		}
-	});
+	};
+	if (!Multi) {   // instance 0 as a constant: its reads need no load of the block's instance first
+		const uint32_t blk = blockIdx.x;
+		if (a.single) block(blk, 0u);
+		else block(blk, a.blkInst[blk]);
+		return;
+	}
+	auto ok = [&](uint32_t inst) {
+		return a.st[inst].newtonActive && a.log[(size_t)inst * a.logStride + k].active && !persistent(a.persist, inst);
+	};
+	for_items<Multi, false>(a.nRowBlk, a.blkInst, ok, [&](uint32_t blk) { block(blk, a.blkInst[blk]); });
 }
 
 // v += dv ; x = x0 + h v + y ; error = |dv|^2 (sparse_force_solver.cpp:193-223)
@@ -1960,11 +2076,12 @@ __global__ void __launch_bounds__(kBlock) k_pcg_persistent(CGArgs a, const uint3
 			const V3* dsrc = k == 0 ? a.s : dold;
 			auto dval = [&](uint32_t c) { return a.s[c] + beta * dsrc[c]; };
 			for (uint32_t t = t0; t < t1; ++t) {
+				const TileRange tr = tile_range<St>(a, t);
 				uint32_t row;
 				V3 q;
-				tile_rows<St>(a, t, row, dval, lds, no_prologue, q);
+				tile_rows<St>(a, tr, row, dval, lds, no_prologue, q);
 				float part = 0.0f;
-				const bool has = row < a.tiles[t + 1];
+				const bool has = row < tr.r1;
 				if (has) {
 					const V3 d = dval(row);
 					dnew[row] = d;
@@ -1989,11 +2106,12 @@ __global__ void __launch_bounds__(kBlock) k_pcg_persistent(CGArgs a, const uint3
 				}
 				__syncthreads();
 				for (uint32_t t = t0; t < t1; ++t) {
+					const TileRange tr = tile_range<St>(a, t);
 					uint32_t row;
 					V3 ax;
-					tile_rows<St>(a, t, row, [&](uint32_t c) { return a.px[c]; }, lds, no_prologue, ax);
+					tile_rows<St>(a, tr, row, [&](uint32_t c) { return a.px[c]; }, lds, no_prologue, ax);
 					float part = 0.0f;
-					const bool has = row < a.tiles[t + 1];
+					const bool has = row < tr.r1;
 					if (has) {
 						const V3 r = a.pb[row] - ax;
 						const V3 s = diag_apply(a.invM[row], r);
@@ -2033,10 +2151,11 @@ template <int St>
 __global__ void __launch_bounds__(kBlock) k_spmv(CGArgs a, const V3* xin, V3* yout) {
 	__shared__ float lds[3 * kTileCap];
 	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+	const TileRange tr = tile_range<St>(a, t);
 	uint32_t row;
 	V3 y;
-	tile_rows<St>(a, t, row, [&](uint32_t c) { return xin[c]; }, lds, no_prologue, y);
-	if (row < a.tiles[t + 1]) yout[row] = y;
+	tile_rows<St>(a, tr, row, [&](uint32_t c) { return xin[c]; }, lds, no_prologue, y);
+	if (row < tr.r1) yout[row] = y;
 }
 
 // ---------------------------------------------------------------------------
@@ -2397,6 +2516,9 @@ b3g_status build_tiles(Body& b, bool sorted, bool tiles) {
 		}
 		if ((s = dupload(b, b.tiles, t.data(), t.size() * 4)) != B3G_OK) return s;
 		if ((s = dupload(b, b.tileInst, ti.data(), ti.size() * 4)) != B3G_OK) return s;
+		std::vector<uint32_t> tb(t.size());
+		for (size_t x = 0; x < t.size(); ++x) tb[x] = b.hostRowptr[t[x]];
+		if ((s = dupload(b, b.tileBlk, tb.data(), tb.size() * 4)) != B3G_OK) return s;
 		// the BF16 pair planes: each tile's first pair
 		std::vector<uint32_t> pairs(std::max(1u, b.nTiles), 0u);
 		uint64_t total = 0;
@@ -2564,8 +2686,10 @@ static CGArgs cg_args(Body& b) {
 	a.nRowBlk = b.nRowBlk;
 	a.tiles = dptr<uint32_t>(b.tiles);
 	a.tileInst = dptr<uint32_t>(b.tileInst);
+	a.tileBlk = dptr<uint32_t>(b.tileBlk);
 	a.rowBlk = dptr<uint32_t>(b.rowBlk);
 	a.blkInst = dptr<uint32_t>(b.blkInst);
+	a.single = b.nInst <= 1 ? 1u : 0u;
 	a.rowptr = dptr<uint32_t>(b.rowptr);
 	a.cols = dptr<uint32_t>(b.cols);
 	a.A = dptr<float>(b.A);

@@ -44,7 +44,8 @@ constexpr size_t kOffLog = kOffPartialQ + align256(kPartials * 8);
 constexpr size_t kOffState = kOffLog + align256((size_t)kLogEntries * sizeof(CGLog));
 constexpr size_t kOffTiles = kOffState + align256(sizeof(SolverState));
 constexpr size_t kOffTileInst = kOffTiles + align256((kMaxTiles + 1) * 4);
-constexpr size_t kOffRowBlk = kOffTileInst + align256(kMaxTiles * 4);
+constexpr size_t kOffTileBlk = kOffTileInst + align256(kMaxTiles * 4);
+constexpr size_t kOffRowBlk = kOffTileBlk + align256((kMaxTiles + 1) * 4);
 constexpr size_t kOffBlkInst = kOffRowBlk + align256((kMaxRowBlks + 1) * 4);
 constexpr size_t kOffRange = kOffBlkInst + align256(kMaxRowBlks * 4);
 constexpr size_t kOffFlag = kOffRange + align256(6 * 4);
@@ -108,6 +109,7 @@ struct StaticSetup {
 	const uint32_t* bad;
 	uint32_t* tiles;
 	uint32_t* tileInst;
+	uint32_t* tileBlk;
 	uint32_t* rowBlk;
 	uint32_t* blkInst;
 	uint32_t* range;
@@ -118,7 +120,10 @@ struct StaticSetup {
 __global__ void __launch_bounds__(256) k_static_setup(StaticSetup s) {
 	const uint32_t i = blockIdx.x * 256 + threadIdx.x;
 	const bool bad = *s.bad != 0u;
-	if (i <= s.nTiles) s.tiles[i] = min(i * kTileRowsStatic, s.n);
+	if (i <= s.nTiles) {
+		s.tiles[i] = min(i * kTileRowsStatic, s.n);
+		s.tileBlk[i] = s.rowptr[min(i * kTileRowsStatic, s.n)];   // read only after the structure check passed
+	}
 	if (i < s.nTiles) s.tileInst[i] = 0u;
 	if (i <= s.nRowBlk) s.rowBlk[i] = min(i * kRowBlkRowsStatic, s.n);
 	if (i < s.nRowBlk) s.blkInst[i] = 0u;
@@ -212,6 +217,7 @@ extern "C" b3g_status b3g_solve_cg_static(int device, void* stream, uint32_t n, 
 	at(t.state, kOffState, sizeof(SolverState));
 	at(t.tiles, kOffTiles, (kMaxTiles + 1) * 4);
 	at(t.tileInst, kOffTileInst, kMaxTiles * 4);
+	at(t.tileBlk, kOffTileBlk, (kMaxTiles + 1) * 4);
 	at(t.rowBlk, kOffRowBlk, (kMaxRowBlks + 1) * 4);
 	at(t.blkInst, kOffBlkInst, kMaxRowBlks * 4);
 	at(t.instRange, kOffRange, 6 * 4);
@@ -230,7 +236,7 @@ extern "C" b3g_status b3g_solve_cg_static(int device, void* stream, uint32_t n, 
 	su.n = n; su.nnz = nnz; su.nTiles = t.nTiles; su.nRowBlk = t.nRowBlk; su.maxIt = max_iterations;
 	su.eps2 = tolerance * tolerance;
 	su.rowptr = rowptr; su.cols = cols; su.A = vals; su.bad = bad;
-	su.tiles = dptr<uint32_t>(t.tiles); su.tileInst = dptr<uint32_t>(t.tileInst);
+	su.tiles = dptr<uint32_t>(t.tiles); su.tileInst = dptr<uint32_t>(t.tileInst); su.tileBlk = dptr<uint32_t>(t.tileBlk);
 	su.rowBlk = dptr<uint32_t>(t.rowBlk); su.blkInst = dptr<uint32_t>(t.blkInst);
 	su.range = dptr<uint32_t>(t.instRange); su.st = dptr<SolverState>(t.state); su.invM = dptr<V3>(t.invM);
 	const uint32_t setupThreads = std::max(n, t.nTiles + 1);
