@@ -31,6 +31,12 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t G) {
 	return start + b / 8;
 }
 
+// Holds the uniform values v... in scalar registers at this point, so their
+// loads (independent of one another) are all issued ahead of it and waited
+// for once; without it the compiler moves each load down to its first use,
+// behind the branches before it, and the loads run one round trip each.
+#define B3G_SETTLE(...) asm volatile("" ::__VA_ARGS__)
+
 // Work items (tiles or row blocks) [0, count) over the launch's workgroups.
 // Multi = false: one workgroup per item, item = xcd_tile(block) (Xcd) or the
 // block index, checked by every thread (the kernels of a single body).
@@ -1151,6 +1157,7 @@ struct AsmArgs {
 	const float4* stRec;
 	const float4* shRec;
 	uint32_t anyJv;
+	uint32_t single;             // one instance: every tile is instance 0
 	const uint8_t* type;
 	const float* mass;
 	const float* damping;
@@ -1246,7 +1253,7 @@ template <int Kinds>
 __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib& cb_, const uint32_t* ptr, uint32_t s0,
                                                 uint32_t s1, uint32_t s, bool own, float* lds, M33& acc) {
 	const uint32_t c0 = ptr[s0], c1 = ptr[s1];
-	const uint32_t rb = own ? ptr[s] : 0, re = own ? ptr[s + 1] : 0;
+	const uint32_t rb = ptr[own ? s : s0], re = ptr[own ? s + 1 : s0];   // used by owners only
 	for (uint32_t cb = c0; cb < c1; cb += kAsmCap) {
 		const uint32_t ce = min(cb + (uint32_t)kAsmCap, c1);
 		uint32_t code[kAsmPer];
@@ -1254,15 +1261,16 @@ __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib&
 		// r[0..2] and its rotation in r[3], a staged block in r[0..2], or a
 		// stretch / shear record in r[0..3]
 		float4 r[kAsmPer][4];
+		// A lane past the chunk takes the chunk's first contribution (in
+		// bounds) and stores nothing: with no branch around them, the loads of
+		// all of a thread's contributions issue together.
 #pragma unroll
 		for (int u = 0; u < kAsmPer; ++u) {
 			const uint32_t k = cb + threadIdx.x + u * kBlock;
-			code[u] = k < ce ? cb_.idx[k] : 0u;
+			code[u] = cb_.idx[k < ce ? k : cb];
 		}
 #pragma unroll
 		for (int u = 0; u < kAsmPer; ++u) {
-			const uint32_t k = cb + threadIdx.x + u * kBlock;
-			if (k >= ce) continue;
 			const uint32_t c = code[u];
 			if (is_tet<Kinds>(c)) {
 				const uint32_t e = c & (kTetMaxElements - 1), blk = (c & ~kTetCode) >> kTetBlockShift;
@@ -1271,22 +1279,18 @@ __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib&
 				const float4 s4 = *reinterpret_cast<const float4*>(rec + kTetRecV);
 				r[u][0] = make_float4(gi.x, gi.y, gi.z, s4.x);
 				r[u][1] = make_float4(gj.x, gj.y, gj.z, s4.y);
-				r[u][2] = make_float4(s4.z, s4.w, 0.0f, 0.0f);
+				r[u][2] = make_float4(s4.z, s4.w, cb_.kd ? cb_.kd[e] : 0.0f, 0.0f);   // z: the dfdv scale
 				r[u][3] = a.tetQ[e];
 			} else if (is_uv<Kinds>(c)) {
+				// a stretch record is three float4, a shear record four: one
+				// address per load, no branch (the stretch's fourth is unused)
 				const uint32_t variant = (c & ~kUVCode) >> kUVShift, e = c & (kUVMaxElements - 1);
-				if (variant < kShearVariant) {
-					const float4* rec = a.stRec + (size_t)(2 * e + variant / 9) * 3;
-					r[u][0] = rec[0];
-					r[u][1] = rec[1];
-					r[u][2] = rec[2];
-				} else {
-					const float4* rec = a.shRec + 4 * (size_t)e;
-					r[u][0] = rec[0];
-					r[u][1] = rec[1];
-					r[u][2] = rec[2];
-					r[u][3] = rec[3];
-				}
+				const bool shear = variant >= kShearVariant;
+				const float4* rec = shear ? a.shRec + 4 * (size_t)e : a.stRec + (size_t)(2 * e + variant / 9) * 3;
+				r[u][0] = rec[0];
+				r[u][1] = rec[1];
+				r[u][2] = rec[2];
+				r[u][3] = rec[shear ? 3 : 2];
 			} else {
 				const M33 m = a.jstage[c];
 				r[u][0] = make_float4(m.x.x, m.x.y, m.x.z, m.y.x);
@@ -1319,7 +1323,7 @@ __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib&
 				M33 R = qmatrix(q);
 				M33 RT = transpose(R);
 				M33 K = R * v * RT;
-				v = cb_.kd ? -(cb_.kd[c & (kTetMaxElements - 1)] * K) : -K;
+				v = cb_.kd ? -(r[u][2].z * K) : -K;
 			}
 			const float* pv = &v.x.x;
 #pragma unroll
@@ -1348,25 +1352,31 @@ __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib&
 	}
 }
 
-// Same over V3 contributions of rows (kTileCap per chunk): val[idx[k]].
+// Same over V3 contributions of rows (kTileCap per chunk): val[idx[k]]. The
+// next chunk's indices are requested together with this chunk's values, so
+// a chunk costs one memory round trip rather than two.
 __device__ __forceinline__ void tile_accumulate3(const uint32_t* ptr, const uint32_t* idx, const V3* val, uint32_t r0,
                                                  uint32_t r1, uint32_t r, bool own, float* lds, V3& acc) {
 	const uint32_t c0 = ptr[r0], c1 = ptr[r1];
-	const uint32_t rb = own ? ptr[r] : 0, re = own ? ptr[r + 1] : 0;
+	const uint32_t rb = ptr[own ? r : r0], re = ptr[own ? r + 1 : r0];   // used by owners only
+	// a lane past the chunk reads the chunk's first entry (in bounds) and
+	// stores nothing: no branch around the loads, which then issue together
+	uint32_t id[kRowPer];
+	auto load_idx = [&](uint32_t cb) {
+		const uint32_t ce = min(cb + (uint32_t)kTileCap, c1);
+#pragma unroll
+		for (int u = 0; u < kRowPer; ++u) {
+			const uint32_t k = cb + threadIdx.x + u * kBlock;
+			id[u] = idx[k < ce ? k : cb];
+		}
+	};
+	if (c0 < c1) load_idx(c0);
 	for (uint32_t cb = c0; cb < c1; cb += kTileCap) {
 		const uint32_t ce = min(cb + (uint32_t)kTileCap, c1);
-		uint32_t id[kRowPer];
 		V3 v[kRowPer];
 #pragma unroll
-		for (int u = 0; u < kRowPer; ++u) {
-			const uint32_t k = cb + threadIdx.x + u * kBlock;
-			id[u] = k < ce ? idx[k] : 0u;
-		}
-#pragma unroll
-		for (int u = 0; u < kRowPer; ++u) {
-			const uint32_t k = cb + threadIdx.x + u * kBlock;
-			if (k < ce) v[u] = val[id[u]];
-		}
+		for (int u = 0; u < kRowPer; ++u) v[u] = val[id[u]];
+		if (cb + kTileCap < c1) load_idx(cb + kTileCap);
 #pragma unroll
 		for (int u = 0; u < kRowPer; ++u) {
 			const uint32_t k = cb + threadIdx.x + u * kBlock;
@@ -1388,17 +1398,23 @@ __device__ __forceinline__ void tile_accumulate3(const uint32_t* ptr, const uint
 template <int Kinds>
 __global__ void __launch_bounds__(kBlock) k_assemble_slots(AsmArgs a) {
 	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
-	if (!a.st[a.slotTileInst[t]].newtonActive) return;
 	__shared__ float lds[9 * kAsmCap];
 	const uint32_t s0 = a.slotTiles[t], s1 = a.slotTiles[t + 1];
+	// the tile's slots and its instance's gate in one round of loads
+	const uint32_t active = a.single ? a.st[0].newtonActive : a.st[a.slotTileInst[t]].newtonActive;
+	B3G_SETTLE("s"(s0), "s"(s1), "s"(active));
+	if (!active) return;
 	const uint32_t s = s0 + threadIdx.x;
 	const bool own = s < s1;
-	const uint32_t i = own ? a.slotRow[s] : 0, j = own ? a.cols[s] : 0;
-	// per-slot data of the epilogue, loaded ahead of the contribution phase
-	const uint8_t type_i = own ? a.type[i] : 0, type_j = own ? a.type[j] : 0;
-	const float mass_i = own ? a.mass[i] : 0.0f, damp_i = own ? a.damping[i] : 0.0f;
-	const int32_t sph_i = own && a.hasContacts ? a.sphereOf[i] : -1;
-	const V3 w_j = own ? a.w[j] : zero3();
+	// per-slot data of the epilogue, loaded ahead of the contribution phase;
+	// a thread past the tile reads slot s0's (in bounds, no branch around the
+	// loads) and keeps none of it
+	const uint32_t sc = own ? s : s0;
+	const uint32_t i = a.slotRow[sc], j = a.cols[sc];
+	const uint8_t type_i = a.type[i], type_j = a.type[j];
+	const float mass_i = a.mass[i], damp_i = a.damping[i];
+	const int32_t sph_i = a.hasContacts ? a.sphereOf[i] : -1;
+	const V3 w_j = a.w[j];
 	// dfdx(i, j): contributions in list order, then contacts (force_solver.cpp:70-86)
 	M33 dx = mzero();
 	tile_accumulate<Kinds>(a, Contrib{a.jxIdx, nullptr, false}, a.jxPtr, s0, s1, s, own, lds, dx);
@@ -1504,16 +1520,20 @@ __global__ void __launch_bounds__(kBlock) k_assemble_rows64(AsmArgs a) {
 
 __global__ void __launch_bounds__(kBlock) k_assemble_rows(AsmArgs a) {
 	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
-	if (!a.st[a.tileInst[t]].newtonActive) return;
 	__shared__ float lds[3 * kTileCap];
 	const uint32_t r0 = a.rowTiles[t], r1 = a.rowTiles[t + 1];
+	// the tile's rows and its instance's gate in one round of loads
+	const uint32_t active = a.single ? a.st[0].newtonActive : a.st[a.tileInst[t]].newtonActive;
+	B3G_SETTLE("s"(r0), "s"(r1), "s"(active));
+	if (!active) return;
 	const uint32_t i = r0 + threadIdx.x;
 	const bool own = i < r1;
-	const float mass = own ? a.mass[i] : 0.0f;
-	const V3 vi = own ? a.v[i] : zero3();
+	const uint32_t ic = own ? i : r0;   // a thread past the tile reads row r0 (in bounds) and keeps nothing
+	const float mass = a.mass[ic], damping = a.damping[ic];
+	const V3 vi = a.v[ic];
 	// fi: particle damping, forces in list order, contacts (force_solver.cpp:70-86)
 	V3 fi = zero3();
-	if (own && a.damping[i] > 0.0f) fi = fi + (-a.damping[i] * mass) * vi;
+	if (own && damping > 0.0f) fi = fi + (-damping * mass) * vi;
 	tile_accumulate3(a.fPtr, a.fIdx, a.fstage, r0, r1, i, own, lds, fi);
 	// h dfdx (x0 - x + h v + y) in the first-touch order of dfdx
 	V3 hdw = zero3();
@@ -1623,12 +1643,6 @@ template <int St>
 __device__ __forceinline__ TileRange tile_range(const CGArgs& a, uint32_t t) {
 	return tile_range_<(St & 1) != 0, (St & 2) != 0>(a, t);
 }
-
-// Holds the uniform values v... in scalar registers at this point, so their
-// loads (independent of one another) are all issued ahead of it and waited
-// for once; without it the compiler moves each load down to its first use,
-// behind the branches before it, and the loads run one round trip each.
-#define B3G_SETTLE(...) asm volatile("" ::__VA_ARGS__)
 
 template <bool Sorted, bool Pairs, class Gather, class Before>
 __device__ __forceinline__ bool tile_rows_(const CGArgs& a, const TileRange& tr, uint32_t& row, Gather gather,
@@ -2836,6 +2850,7 @@ static b3g_status enqueue_linearise(Body& b, float h) {
 	a.force = dptr<V3>(b.force);
 	a.gravity = dptr<V3>(b.gravity);
 	a.rowInst = dptr<uint32_t>(b.rowInst);
+	a.single = b.nInst <= 1 ? 1u : 0u;
 	// the contribution kinds of the body's slot lists: tetrahedra, stretch / shear records, staged blocks
 	const int kinds = (b.tet.m ? kKindTet : 0) | (b.stretch.m + b.shear.m ? kKindUV : 0) | (b.nJStage ? kKindStaged : 0);
 	switch (kinds) {
