@@ -37,6 +37,14 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t G) {
 // behind the branches before it, and the loads run one round trip each.
 #define B3G_SETTLE(...) asm volatile("" ::__VA_ARGS__)
 
+// The Newton gate of an element kernel: the iteration of the instance of the
+// element's first particle is active. st null: the step's first Newton
+// iteration, which every instance runs, so nothing is read.
+__device__ __forceinline__ bool element_gate(const SolverState* st, const uint32_t* rowInst, const uint32_t* p,
+                                             uint32_t e) {
+	return !st || st[rowInst[p[e]]].newtonActive != 0;
+}
+
 // Work items (tiles or row blocks) [0, count) over the launch's workgroups.
 // Multi = false: one workgroup per item, item = xcd_tile(block) (Xcd) or the
 // block index, checked by every thread (the kernels of a single body).
@@ -440,7 +448,7 @@ __device__ Q4 extract_rotation(const M33& A, Q4 q) {
 __global__ void __launch_bounds__(kBlock) k_tet_forces(TetArgs a) {
 	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= a.m) return;
-	if (!a.st[a.rowInst[a.p[e]]].newtonActive) return;
+	if (!element_gate(a.st, a.rowInst, a.p, e)) return;
 	const uint32_t m = a.m;
 	uint32_t id[4];
 	// Only F and the warm start live through the polar iterations: the
@@ -579,7 +587,7 @@ __device__ DQ4 extract_rotation64(const DM33& A, DQ4 q) {
 __global__ void __launch_bounds__(kBlock) k_tet_forces64(TetArgs a, const double* tet64, D3* fstage64) {
 	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= a.m) return;
-	if (!a.st[a.rowInst[a.p[e]]].newtonActive) return;
+	if (!element_gate(a.st, a.rowInst, a.p, e)) return;
 	const uint32_t m = a.m;
 	uint32_t id[4];
 	D3 p[4], xr[4];
@@ -660,7 +668,7 @@ __device__ M22 extract_rotation2(const M22& M) {   // triangle_element_force.cpp
 __global__ void __launch_bounds__(kBlock) k_tri_forces(TriArgs a) {
 	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= a.m) return;
-	if (!a.st[a.rowInst[a.p[e]]].newtonActive) return;
+	if (!element_gate(a.st, a.rowInst, a.p, e)) return;
 	const uint32_t m = a.m;
 	uint32_t id[3];
 	V3 p[3];
@@ -742,7 +750,7 @@ __device__ __forceinline__ float coef(const UVArgs& a, int k, uint32_t e) { retu
 __global__ void __launch_bounds__(kBlock) k_stretch_forces(UVArgs a) {
 	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= a.m) return;
-	if (!a.st[a.rowInst[a.p[e]]].newtonActive) return;
+	if (!element_gate(a.st, a.rowInst, a.p, e)) return;
 	const uint32_t m = a.m;
 	float alpha = coef(a, 0, e), du1 = coef(a, 1, e), dv1 = coef(a, 2, e), du2 = coef(a, 3, e), dv2 = coef(a, 4, e);
 	float inv_det = coef(a, 5, e);
@@ -785,7 +793,7 @@ __global__ void __launch_bounds__(kBlock) k_stretch_forces(UVArgs a) {
 __global__ void __launch_bounds__(kBlock) k_shear_forces(UVArgs a) {
 	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= a.m) return;
-	if (!a.st[a.rowInst[a.p[e]]].newtonActive) return;
+	if (!element_gate(a.st, a.rowInst, a.p, e)) return;
 	const uint32_t m = a.m;
 	float alpha = coef(a, 0, e), du1 = coef(a, 1, e), dv1 = coef(a, 2, e), du2 = coef(a, 3, e), dv2 = coef(a, 4, e);
 	float inv_det = coef(a, 5, e);
@@ -836,7 +844,7 @@ struct ActArgs {
 __global__ void __launch_bounds__(kBlock) k_action_forces(ActArgs a) {
 	const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= a.m) return;
-	if (!a.st[a.rowInst[a.p[e]]].newtonActive) return;
+	if (!element_gate(a.st, a.rowInst, a.p, e)) return;
 	const uint32_t m = a.m;
 	for (uint32_t i = 0; i < a.np; ++i) {
 		V3 s = a.act[a.base + (size_t)i * m + e];
@@ -868,7 +876,7 @@ struct PairArgs {
 __global__ void __launch_bounds__(kBlock) k_spring_forces(PairArgs a) {
 	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= a.m) return;
-	if (!a.st[a.rowInst[a.p[e]]].newtonActive) return;
+	if (!element_gate(a.st, a.rowInst, a.p, e)) return;
 	const uint32_t m = a.m;
 	uint32_t i1 = a.p[e], i2 = a.p[(size_t)m + e];
 	float L0 = a.c[e], ks = a.c[(size_t)m + e], kd = a.c[(size_t)2 * m + e];
@@ -919,7 +927,7 @@ __global__ void __launch_bounds__(kBlock) k_spring_forces(PairArgs a) {
 __global__ void __launch_bounds__(kBlock) k_mouse_forces(PairArgs a) {
 	uint32_t e = blockIdx.x * kBlock + threadIdx.x;
 	if (e >= a.m) return;
-	if (!a.st[a.rowInst[a.p[e]]].newtonActive) return;
+	if (!element_gate(a.st, a.rowInst, a.p, e)) return;
 	const uint32_t m = a.m;
 	uint32_t id[4];
 	V3 xs[4], vs[4];
@@ -1401,7 +1409,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble_slots(AsmArgs a) {
 	__shared__ float lds[9 * kAsmCap];
 	const uint32_t s0 = a.slotTiles[t], s1 = a.slotTiles[t + 1];
 	// the tile's slots and its instance's gate in one round of loads
-	const uint32_t active = a.single ? a.st[0].newtonActive : a.st[a.slotTileInst[t]].newtonActive;
+	const uint32_t active = !a.st ? 1u : a.single ? a.st[0].newtonActive : a.st[a.slotTileInst[t]].newtonActive;
 	B3G_SETTLE("s"(s0), "s"(s1), "s"(active));
 	if (!active) return;
 	const uint32_t s = s0 + threadIdx.x;
@@ -1457,7 +1465,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble_slots(AsmArgs a) {
 // forces, in the reference's order per row.
 __global__ void __launch_bounds__(kBlock) k_assemble_rows64(AsmArgs a) {
 	const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
-	if (!a.st[a.tileInst[t]].newtonActive) return;
+	if (a.st && !a.st[a.tileInst[t]].newtonActive) return;
 	__shared__ double lds[3 * (kTileCap / 2)];
 	constexpr uint32_t cap = kTileCap / 2;
 	const uint32_t r0 = a.rowTiles[t], r1 = a.rowTiles[t + 1];
@@ -1523,7 +1531,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble_rows(AsmArgs a) {
 	__shared__ float lds[3 * kTileCap];
 	const uint32_t r0 = a.rowTiles[t], r1 = a.rowTiles[t + 1];
 	// the tile's rows and its instance's gate in one round of loads
-	const uint32_t active = a.single ? a.st[0].newtonActive : a.st[a.tileInst[t]].newtonActive;
+	const uint32_t active = !a.st ? 1u : a.single ? a.st[0].newtonActive : a.st[a.tileInst[t]].newtonActive;
 	B3G_SETTLE("s"(r0), "s"(r1), "s"(active));
 	if (!active) return;
 	const uint32_t i = r0 + threadIdx.x;
@@ -2757,8 +2765,10 @@ b3g_status enqueue_find_contacts(Body& b) {
 }
 
 // One linearisation: element kernels, contacts, pA / pb assembly.
-static b3g_status enqueue_linearise(Body& b, float h) {
-	const SolverState* st = dptr<SolverState>(b.state);
+// first: the step's first Newton iteration, which every instance runs (its
+// element and assembly kernels then read no gate).
+static b3g_status enqueue_linearise(Body& b, float h, bool first) {
+	const SolverState* st = first ? nullptr : dptr<SolverState>(b.state);
 	const uint32_t* ri = dptr<uint32_t>(b.rowInst);
 	M33* js = dptr<M33>(b.jstage);
 	V3* fs = dptr<V3>(b.fstage);
@@ -3468,7 +3478,7 @@ b3g_status enqueue_step(Body& b, float dt, uint32_t iters, uint32_t subIters, fl
 		       refined ? b.innerTol : subTol, b.reduction == B3G_REDUCE_SEQUENTIAL ? 1u : 0u, refined ? 1u : 0u, sweeps,
 		       subTol);
 		for (uint32_t it = 0; it < (assembleOnly ? 1u : iters); ++it) {
-			s = enqueue_linearise(b, dt);
+			s = enqueue_linearise(b, dt, it == 0);
 			if (s != B3G_OK) return s;
 			if (assembleOnly) return B3G_OK;
 			s = refined ? enqueue_solve_refined(b, sweeps) : enqueue_solve_cg(b, subIters, subTol);
