@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 namespace b3g {
 
@@ -345,20 +346,35 @@ struct StepArgs {
 	SolverState* st;            // per instance
 	const uint32_t* range;
 	uint32_t* faultWord;        // fault bits of any instance
+	uint32_t single;            // one instance: every row is instance 0
 };
 
-// force_solver.cpp:100-130 + sparse_force_solver.cpp:135-139
+// force_solver.cpp:100-130 + sparse_force_solver.cpp:135-139. Every input is
+// read (unconditionally, in bounds) before the first store, so the loads
+// issue together.
 __global__ void __launch_bounds__(kBlock) k_step_begin(StepArgs a) {
 	uint32_t i = blockIdx.x * kBlock + threadIdx.x;
 	if (i >= a.n) return;
-	V3 x = a.x[i], v = a.v[i];
-	a.x0[i] = x;
-	a.v0[i] = v;
-	V3 fe = a.force[i];
-	if (a.type[i] == kDynamic) fe = fe + a.mass[i] * a.gravity[a.rowInst[i]];
-	a.fe[i] = fe;
-	a.y[i] = a.translation[i];
-	a.px[i] = zero3();
+	// one instance: its gravity without the row's instance (no dependent load)
+	auto run = [&](auto single) {
+		const V3 x = a.x[i], v = a.v[i], force = a.force[i], translation = a.translation[i];
+		const uint32_t type = a.type[i];
+		const float mass = a.mass[i];
+		const V3 g = a.gravity[decltype(single)::value ? 0u : a.rowInst[i]];
+		// all of them in registers here: one wait, not a load sunk into the branch below
+		B3G_SETTLE("v"(x.x), "v"(x.y), "v"(x.z), "v"(v.x), "v"(v.y), "v"(v.z), "v"(force.x), "v"(force.y), "v"(force.z),
+		           "v"(translation.x), "v"(translation.y), "v"(translation.z), "v"(type), "v"(mass), "v"(g.x), "v"(g.y),
+		           "v"(g.z));
+		V3 fe = force;
+		if (type == kDynamic) fe = fe + mass * g;
+		a.x0[i] = x;
+		a.v0[i] = v;
+		a.fe[i] = fe;
+		a.y[i] = translation;
+		a.px[i] = zero3();
+	};
+	if (a.single) run(std::true_type{});
+	else run(std::false_type{});
 }
 
 __global__ void __launch_bounds__(kBlock) k_state_begin(SolverState* sts, uint32_t count, uint32_t iters, uint32_t subIters,
@@ -385,8 +401,9 @@ __global__ void __launch_bounds__(kBlock) k_state_begin(SolverState* sts, uint32
 __global__ void __launch_bounds__(kBlock) k_newton_prep(StepArgs a) {
 	uint32_t i = blockIdx.x * kBlock + threadIdx.x;
 	if (i >= a.n) return;
-	if (!a.st[a.rowInst[i]].newtonActive) return;
-	a.w[i] = ((a.x0[i] - a.x[i]) + a.h * a.v[i]) + a.y[i];
+	const V3 x0 = a.x0[i], x = a.x[i], v = a.v[i], y = a.y[i];   // in bounds whatever the gate says
+	if (!a.st[a.single ? 0u : a.rowInst[i]].newtonActive) return;
+	a.w[i] = ((x0 - x) + a.h * v) + y;
 }
 
 // ---------------------------------------------------------------------------
@@ -752,13 +769,23 @@ __global__ void __launch_bounds__(kBlock) k_stretch_forces(UVArgs a) {
 	if (e >= a.m) return;
 	if (!element_gate(a.st, a.rowInst, a.p, e)) return;
 	const uint32_t m = a.m;
-	float alpha = coef(a, 0, e), du1 = coef(a, 1, e), dv1 = coef(a, 2, e), du2 = coef(a, 3, e), dv2 = coef(a, 4, e);
-	float inv_det = coef(a, 5, e);
-	V3 dwudx = v3(coef(a, 6, e), coef(a, 7, e), coef(a, 8, e));
-	V3 dwvdx = v3(coef(a, 9, e), coef(a, 10, e), coef(a, 11, e));
+	// every input in registers before any use (coefficients, positions,
+	// velocities): the loads issue in two rounds (indices, then the rest)
+	// instead of each one sinking to its first use behind a branch
+	float cf[18];
+#pragma unroll
+	for (int k = 0; k < 18; ++k) cf[k] = coef(a, k, e);
 	uint32_t id[3];
 	V3 xs[3], vs[3];
 	for (int k = 0; k < 3; ++k) { id[k] = a.p[(size_t)k * m + e]; xs[k] = a.x[id[k]]; vs[k] = a.v[id[k]]; }
+	B3G_SETTLE("v"(cf[0]), "v"(cf[1]), "v"(cf[2]), "v"(cf[3]), "v"(cf[4]), "v"(cf[5]), "v"(cf[6]), "v"(cf[7]),
+	           "v"(cf[8]), "v"(cf[9]), "v"(cf[10]), "v"(cf[11]), "v"(cf[12]), "v"(cf[13]), "v"(cf[14]), "v"(cf[15]),
+	           "v"(cf[16]), "v"(cf[17]));
+	B3G_SETTLE("v"(xs[0].x), "v"(xs[0].y), "v"(xs[0].z), "v"(xs[1].x), "v"(xs[1].y), "v"(xs[1].z), "v"(xs[2].x),
+	           "v"(xs[2].y), "v"(xs[2].z), "v"(vs[0].x), "v"(vs[0].y), "v"(vs[0].z), "v"(vs[1].x), "v"(vs[1].y),
+	           "v"(vs[1].z), "v"(vs[2].x), "v"(vs[2].y), "v"(vs[2].z));
+	const float alpha = cf[0], du1 = cf[1], dv1 = cf[2], du2 = cf[3], dv2 = cf[4], inv_det = cf[5];
+	const V3 dwudx = v3(cf[6], cf[7], cf[8]), dwvdx = v3(cf[9], cf[10], cf[11]);
 	V3 dx1 = xs[1] - xs[0], dx2 = xs[2] - xs[0];
 	V3 wu = inv_det * (dv2 * dx1 - dv1 * dx2);
 	float len_wu = length(wu);
@@ -768,7 +795,7 @@ __global__ void __launch_bounds__(kBlock) k_stretch_forces(UVArgs a) {
 		V3 wvec = part == 0 ? wu : wv;
 		float len = part == 0 ? len_wu : len_wv;
 		V3 dwdx = part == 0 ? dwudx : dwvdx;
-		float ks = coef(a, 12 + 3 * part, e), kd = coef(a, 13 + 3 * part, e), bb = coef(a, 14 + 3 * part, e);
+		float ks = cf[12 + 3 * part], kd = cf[13 + 3 * part], bb = cf[14 + 3 * part];
 		bool live = len > 0.0f;
 		float inv_len = live ? 1.0f / len : 0.0f;
 		V3 nw = inv_len * wvec;
@@ -795,14 +822,21 @@ __global__ void __launch_bounds__(kBlock) k_shear_forces(UVArgs a) {
 	if (e >= a.m) return;
 	if (!element_gate(a.st, a.rowInst, a.p, e)) return;
 	const uint32_t m = a.m;
-	float alpha = coef(a, 0, e), du1 = coef(a, 1, e), dv1 = coef(a, 2, e), du2 = coef(a, 3, e), dv2 = coef(a, 4, e);
-	float inv_det = coef(a, 5, e);
-	V3 dwudx = v3(coef(a, 6, e), coef(a, 7, e), coef(a, 8, e));
-	V3 dwvdx = v3(coef(a, 9, e), coef(a, 10, e), coef(a, 11, e));
-	float ks = coef(a, 12, e), kd = coef(a, 13, e);
+	// every input in registers before any use (see k_stretch_forces)
+	float cf[14];
+#pragma unroll
+	for (int k = 0; k < 14; ++k) cf[k] = coef(a, k, e);
 	uint32_t id[3];
 	V3 xs[3], vs[3];
 	for (int k = 0; k < 3; ++k) { id[k] = a.p[(size_t)k * m + e]; xs[k] = a.x[id[k]]; vs[k] = a.v[id[k]]; }
+	B3G_SETTLE("v"(cf[0]), "v"(cf[1]), "v"(cf[2]), "v"(cf[3]), "v"(cf[4]), "v"(cf[5]), "v"(cf[6]), "v"(cf[7]),
+	           "v"(cf[8]), "v"(cf[9]), "v"(cf[10]), "v"(cf[11]), "v"(cf[12]), "v"(cf[13]));
+	B3G_SETTLE("v"(xs[0].x), "v"(xs[0].y), "v"(xs[0].z), "v"(xs[1].x), "v"(xs[1].y), "v"(xs[1].z), "v"(xs[2].x),
+	           "v"(xs[2].y), "v"(xs[2].z), "v"(vs[0].x), "v"(vs[0].y), "v"(vs[0].z), "v"(vs[1].x), "v"(vs[1].y),
+	           "v"(vs[1].z), "v"(vs[2].x), "v"(vs[2].y), "v"(vs[2].z));
+	const float alpha = cf[0], du1 = cf[1], dv1 = cf[2], du2 = cf[3], dv2 = cf[4], inv_det = cf[5];
+	const V3 dwudx = v3(cf[6], cf[7], cf[8]), dwvdx = v3(cf[9], cf[10], cf[11]);
+	const float ks = cf[12], kd = cf[13];
 	V3 dx1 = xs[1] - xs[0], dx2 = xs[2] - xs[0];
 	V3 wu = inv_det * (dv2 * dx1 - dv1 * dx2);
 	V3 wv = inv_det * (-du2 * dx1 + du1 * dx2);
@@ -2019,18 +2053,30 @@ __global__ void __launch_bounds__(kBlock) k_cg_update_x(CGArgs a, uint32_t k) {
 
 // v += dv ; x = x0 + h v + y ; error = |dv|^2 (sparse_force_solver.cpp:193-223)
 __global__ void __launch_bounds__(kBlock) k_newton_update(StepArgs s, double* partial, float* terms) {
-	const uint32_t blk = blockIdx.x, inst = s.blkInst[blk];
-	if (!s.st[inst].newtonActive) return;
+	const uint32_t blk = blockIdx.x, inst = s.single ? 0u : s.blkInst[blk];
 	const uint32_t r0 = s.rowBlk[blk], r1 = s.rowBlk[blk + 1];
-	const bool seq = s.st[inst].seq != 0;
+	const uint32_t active = s.st[inst].newtonActive, seqw = s.st[inst].seq;
+	B3G_SETTLE("s"(r0), "s"(r1), "s"(active), "s"(seqw));
+	if (!active) return;
+	const bool seq = seqw != 0;
+	// every row's inputs before the first store; a row past the block reads row r0 and stores nothing
+	V3 pxv[kRowsPerThread], vv[kRowsPerThread], x0v[kRowsPerThread], yv[kRowsPerThread];
+#pragma unroll
+	for (int u = 0; u < kRowsPerThread; ++u) {
+		const uint32_t i = r0 + threadIdx.x + u * kBlock, ii = i < r1 ? i : r0;
+		pxv[u] = s.px[ii];
+		vv[u] = s.v[ii];
+		x0v[u] = s.x0[ii];
+		yv[u] = s.y[ii];
+	}
 	double part = 0.0;
 #pragma unroll
 	for (int u = 0; u < kRowsPerThread; ++u) {
 		const uint32_t i = r0 + threadIdx.x + u * kBlock;
 		if (i >= r1) break;
-		V3 dv = s.px[i] + zero3();
-		V3 v = s.v[i] + dv;
-		const V3 x = (s.x0[i] + s.h * v) + s.y[i];
+		V3 dv = pxv[u] + zero3();
+		V3 v = vv[u] + dv;
+		const V3 x = (x0v[u] + s.h * v) + yv[u];
 		s.v[i] = v;
 		s.x[i] = x;
 		if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) && isfinite(v.y) && isfinite(v.z))) {
@@ -2658,6 +2704,7 @@ static StepArgs step_args(Body& b, float h) {
 	a.st = dptr<SolverState>(b.state);
 	a.range = dptr<uint32_t>(b.instRange);
 	a.faultWord = dptr<uint32_t>(b.faultWord);
+	a.single = b.nInst <= 1 ? 1u : 0u;
 	return a;
 }
 
