@@ -1734,14 +1734,28 @@ __device__ __forceinline__ bool tile_rows_(const CGArgs& a, const TileRange& tr,
 				dst[u] = kk - cb;
 			}
 		}
+		// two blocks' loads in flight per thread at a time
 		auto products = [&](auto ld) {
 #pragma unroll
-			for (int u = 0; u < kRowPer; ++u) {
-				const uint32_t k = cb + threadIdx.x + u * kBlock;
-				const bool in = k < ce;
-				const V3 g = gather(col[u]);
-				const V3 p = ld(in ? k : cb) * g;
-				if (in) put(dst[u], p);
+			for (int u0 = 0; u0 < kRowPer; u0 += 2) {
+				M33 m[2];
+				V3 g[2];
+#pragma unroll
+				for (int w = 0; w < 2; ++w) {
+					const uint32_t k = cb + threadIdx.x + (u0 + w) * kBlock;
+					g[w] = gather(col[u0 + w]);
+					m[w] = ld(k < ce ? k : cb);
+				}
+				B3G_SETTLE("v"(g[0].x), "v"(g[0].y), "v"(g[0].z), "v"(g[1].x), "v"(g[1].y), "v"(g[1].z), "v"(m[0].x.x),
+				           "v"(m[0].x.y), "v"(m[0].x.z), "v"(m[0].y.x), "v"(m[0].y.y), "v"(m[0].y.z), "v"(m[0].z.x),
+				           "v"(m[0].z.y), "v"(m[0].z.z), "v"(m[1].x.x), "v"(m[1].x.y), "v"(m[1].x.z), "v"(m[1].y.x),
+				           "v"(m[1].y.y), "v"(m[1].y.z), "v"(m[1].z.x), "v"(m[1].z.y), "v"(m[1].z.z));
+#pragma unroll
+				for (int w = 0; w < 2; ++w) {
+					const uint32_t k = cb + threadIdx.x + (u0 + w) * kBlock;
+					const V3 p = m[w] * g[w];
+					if (k < ce) put(dst[u0 + w], p);
+				}
 			}
 		};
 		if (Pairs) {
