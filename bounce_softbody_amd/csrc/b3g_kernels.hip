@@ -515,6 +515,12 @@ __global__ void __launch_bounds__(kBlock) k_tet_forces(TetArgs a) {
 	// velocities and the displacements are never live together.
 	const float kd = a.kd[e];
 	if (kd > 0.0f) {
+		// the record read again after the force stores: the blocks are
+		// recomputed here, not kept live from the loop above
+		for (int k = 0; k < 4; ++k) {
+			const float4 r4 = a.rec[4 * (size_t)e + k];
+			rec[4 * k] = r4.x; rec[4 * k + 1] = r4.y; rec[4 * k + 2] = r4.z; rec[4 * k + 3] = r4.w;
+		}
 		V3 vs[4];
 		for (int k = 0; k < 4; ++k) vs[k] = a.v[id[k]];
 		for (int i = 0; i < 4; ++i) {
@@ -1294,23 +1300,33 @@ __device__ __forceinline__ bool is_uv(uint32_t c) {
 template <int Kinds>
 __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib& cb_, const uint32_t* ptr, uint32_t s0,
                                                 uint32_t s1, uint32_t s, bool own, float* lds, M33& acc) {
+	constexpr bool kTetOnly = Kinds == kKindTet;
 	const uint32_t c0 = ptr[s0], c1 = ptr[s1];
 	const uint32_t rb = ptr[own ? s : s0], re = ptr[own ? s + 1 : s0];   // used by owners only
+	// A lane past the chunk takes the chunk's first contribution (in bounds)
+	// and stores nothing: with no branch around them, the loads of all of a
+	// thread's contributions issue together. The next chunk's codes are
+	// requested with this chunk's records (the last chunk requests its own
+	// again), so a chunk costs one memory round trip rather than two.
+	uint32_t next[kAsmPer];
+	auto load_codes = [&](uint32_t cb) {
+		const uint32_t ce = min(cb + (uint32_t)kAsmCap, c1);
+#pragma unroll
+		for (int u = 0; u < kAsmPer; ++u) {
+			const uint32_t k = cb + threadIdx.x + u * kBlock;
+			next[u] = cb_.idx[k < ce ? k : cb];
+		}
+	};
+	if (c0 < c1) load_codes(c0);
 	for (uint32_t cb = c0; cb < c1; cb += kAsmCap) {
 		const uint32_t ce = min(cb + (uint32_t)kAsmCap, c1);
 		uint32_t code[kAsmPer];
+#pragma unroll
+		for (int u = 0; u < kAsmPer; ++u) code[u] = next[u];
 		// per contribution: a tetrahedron's g_i | V, g_j | L2, (L, mu) in
 		// r[0..2] and its rotation in r[3], a staged block in r[0..2], or a
 		// stretch / shear record in r[0..3]
 		float4 r[kAsmPer][4];
-		// A lane past the chunk takes the chunk's first contribution (in
-		// bounds) and stores nothing: with no branch around them, the loads of
-		// all of a thread's contributions issue together.
-#pragma unroll
-		for (int u = 0; u < kAsmPer; ++u) {
-			const uint32_t k = cb + threadIdx.x + u * kBlock;
-			code[u] = cb_.idx[k < ce ? k : cb];
-		}
 #pragma unroll
 		for (int u = 0; u < kAsmPer; ++u) {
 			const uint32_t c = code[u];
@@ -1340,6 +1356,7 @@ __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib&
 				r[u][2] = make_float4(m.z.z, 0.0f, 0.0f, 0.0f);
 			}
 		}
+		load_codes(cb + kAsmCap < c1 ? cb + kAsmCap : cb);
 #pragma unroll
 		for (int u = 0; u < kAsmPer; ++u) {
 			const uint32_t k = cb + threadIdx.x + u * kBlock;
@@ -1365,7 +1382,10 @@ __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib&
 				M33 R = qmatrix(q);
 				M33 RT = transpose(R);
 				M33 K = R * v * RT;
-				v = cb_.kd ? -(r[u][2].z * K) : -K;
+				// -K (or -(kd K)) goes in; a tetrahedron-only body stores K and
+				// subtracts it in the sums below (a + (-b) is a - b, exactly)
+				if (kTetOnly) v = cb_.kd ? r[u][2].z * K : K;
+				else v = cb_.kd ? -(r[u][2].z * K) : -K;
 			}
 			const float* pv = &v.x.x;
 #pragma unroll
@@ -1385,10 +1405,10 @@ __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib&
 			uint32_t k = kb;
 			for (; k + 2 <= ke; k += 2) {
 				const M33 v0 = at(k), v1 = at(k + 1);
-				acc = acc + v0;
-				acc = acc + v1;
+				acc = kTetOnly ? acc - v0 : acc + v0;
+				acc = kTetOnly ? acc - v1 : acc + v1;
 			}
-			for (; k < ke; ++k) acc = acc + at(k);
+			for (; k < ke; ++k) acc = kTetOnly ? acc - at(k) : acc + at(k);
 		}
 		__syncthreads();
 	}

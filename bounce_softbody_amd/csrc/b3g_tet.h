@@ -57,12 +57,18 @@ B3G_HD M33 tet_inv_e(const float* rec) {
 //   a = 1: [py L, py L2, py L, px mu, pz mu, 0]
 //   a = 2: [pz L, pz L, pz L2, 0, py mu, px mu]
 // B columns of vertex j: [qx 0 0 qy 0 qz], [0 qy 0 qx qz 0], [0 0 qz 0 qy qx].
+//
+// Each BT_D entry is its product added to +0, which changes the product only
+// when it is -0. Every BT_D entry is then only multiplied by a component of q
+// and added into a K sum that starts from +0 and so is never -0; adding a
+// zero of either sign to such a sum gives the same result, so the sign of a
+// zero BT_D entry never reaches K and its +0 addition is left out.
 B3G_HD M33 tet_k0(V3 p, V3 q, float V, float L2, float L, float mu) {
 	const float z = 0.0f;
-	// BT_D entries
-	const float a0 = z + p.x * L2, a1 = z + p.x * L, a3 = z + p.y * mu, a5 = z + p.z * mu;
-	const float b0 = z + p.y * L, b1 = z + p.y * L2, b3 = z + p.x * mu, b4 = z + p.z * mu;
-	const float c0 = z + p.z * L, c2 = z + p.z * L2, c4 = z + p.y * mu, c5 = z + p.x * mu;
+	// BT_D entries (see above: the +0 additions drop out)
+	const float a0 = p.x * L2, a1 = p.x * L, a3 = p.y * mu, a5 = p.z * mu;
+	const float b0 = p.y * L, b1 = p.y * L2, b3 = p.x * mu, b4 = p.z * mu;
+	const float c0 = p.z * L, c2 = p.z * L2, c4 = p.y * mu, c5 = p.x * mu;
 	// K(a, b), summed in k order
 	const float k00 = ((z + a0 * q.x) + a3 * q.y) + a5 * q.z;
 	const float k01 = (z + a1 * q.y) + a3 * q.x;
