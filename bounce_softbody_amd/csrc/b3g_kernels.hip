@@ -229,14 +229,59 @@ __device__ __forceinline__ void block_partial(float term, bool has, uint32_t row
 
 constexpr int kFinBlock = 256;
 
-// Sum of partials [p0, p1) by a kFinBlock-thread workgroup in a fixed order
-// (thread t adds p0 + t, p0 + t + 256, ..., sixteen loads in flight; then a
-// wave tree and the four wave sums in order). Valid in every thread.
-__device__ double wg_sum(const double* partial, uint32_t p0, uint32_t p1) {
+// The first pass of wg_sum's loads (partials p0 + x + j kFinBlock, j < 16),
+// issued on their own so that a kernel can put other loads in flight before
+// it waits for them. A lane past the range loads nothing (a whole wave past
+// it skips the load), so a short range costs few loads.
+__device__ __forceinline__ void wg_sum_load(const double* partial, uint32_t p0, uint32_t p1, double (&p)[16]) {
+#pragma unroll
+	for (int u = 0; u < 16; ++u) {
+		const uint32_t k = p0 + threadIdx.x + u * kFinBlock;
+		p[u] = k < p1 ? partial[k] : 0.0;
+	}
+}
+
+// Finishes wg_sum from its first pass's values (the same additions in the
+// same order as wg_sum, so the same result).
+__device__ double wg_sum_finish(const double* partial, uint32_t p0, uint32_t p1, const double (&first)[16]) {
 	__shared__ double sh[kFinBlock / 64];
 	__shared__ double total;
 	// thread x adds partial[p0 + x + j kFinBlock] for j = 0, 1, ... in order;
 	// 16 loads per pass keep up to 16 kFinBlock partials one L2 round trip
+	double t = 0.0;
+#pragma unroll
+	for (int u = 0; u < 16; ++u) t += first[u];
+	for (uint32_t base = p0 + 16 * kFinBlock; base < p1; base += 16 * kFinBlock) {
+		double p[16];
+#pragma unroll
+		for (int u = 0; u < 16; ++u) {
+			const uint32_t k = base + threadIdx.x + u * kFinBlock;
+			p[u] = k < p1 ? partial[k] : 0.0;
+		}
+#pragma unroll
+		for (int u = 0; u < 16; ++u) t += p[u];
+	}
+	for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	__syncthreads();
+	if (lane == 0) sh[wave] = t;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double tot = 0.0;
+		for (int k = 0; k < kFinBlock / 64; ++k) tot += sh[k];
+		total = tot;
+	}
+	__syncthreads();
+	return total;
+}
+
+// Sum of partials [p0, p1) by a kFinBlock-thread workgroup in a fixed order
+// (thread t adds p0 + t, p0 + t + 256, ..., sixteen loads in flight; then a
+// wave tree and the four wave sums in order). Valid in every thread. An
+// empty range sums to +0.
+__device__ double wg_sum(const double* partial, uint32_t p0, uint32_t p1) {
+	__shared__ double sh[kFinBlock / 64];
+	__shared__ double total;
 	double t = 0.0;
 	for (uint32_t base = p0; base < p1; base += 16 * kFinBlock) {
 		double p[16];
@@ -1998,6 +2043,10 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k, int 
 		// stores could alias them as far as the compiler knows), and before the
 		// alpha prologue, whose partial sums then overlap these loads; a row past
 		// the block loads row r0 and stores nothing
+		// the SpMV's d.q partials for alpha, requested first: their sum then
+		// runs while the rows are still in flight
+		double pq[16];
+		wg_sum_load(a.partialQ, q0, q1, pq);
 		V3 d[RPT], x[RPT], r[RPT], q[RPT], im[RPT];
 		auto load = [&]() {
 #pragma unroll
@@ -2017,7 +2066,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k, int 
 		// instance, and the instance's first block records it
 		float alpha = logAlpha;
 		if (a.fused) {
-			alpha = delta / (float)wg_sum(a.partialQ, q0, q1);
+			alpha = delta / (float)wg_sum_finish(a.partialQ, q0, q1, pq);
 			if (blk == firstBlk && threadIdx.x == 0) log[k].alpha = alpha;
 		}
 		double part = 0.0;
