@@ -1887,18 +1887,30 @@ __device__ __forceinline__ bool no_prologue() { return true; }
 template <bool Multi, int St>
 __global__ void __launch_bounds__(kBlock) k_cg_residual(CGArgs a, int mode, uint32_t k) {
 	__shared__ float lds[3 * kTileCap];
-	// the tile's bounds and the instance's gate are read together, before
-	// anything depends on them (every address is valid whether or not the
-	// tile runs)
-	auto tile = [&](uint32_t t, uint32_t inst) {
-		const TileRange tr = tile_range<St>(a, t);
+	// One body: the tile's bounds and the instance's gate are read together,
+	// before anything depends on them (every address is valid whether or not
+	// the tile runs). A batch reads the gate first, so a tile of a converged
+	// instance costs one round of three scalar loads.
+	auto tile = [&](uint32_t t, uint32_t inst, auto single) {
+		constexpr bool S = decltype(single)::value;
 		const SolverState& st = a.st[inst];
-		const uint32_t active = st.newtonActive, seq = st.seq;
+		const uint32_t active = st.newtonActive;
 		const uint32_t logActive = a.log[(size_t)inst * a.logStride + k].active;
 		const uint32_t persist = a.persist ? a.persist[inst] : 0u;
-		B3G_SETTLE("s"(tr.r0), "s"(tr.r1), "s"(tr.b0), "s"(tr.b1), "s"(tr.base), "s"(tr.pair), "s"(active), "s"(seq),
-		           "s"(logActive), "s"(persist));
-		if (!active || (mode == kFinDelta && (!logActive || persist))) return;
+		const bool run = active && (mode != kFinDelta || (logActive && !persist));
+		if constexpr (!S) {
+			B3G_SETTLE("s"(active), "s"(logActive), "s"(persist));
+			if (!run) return;
+		}
+		const TileRange tr = tile_range<St>(a, t);
+		const uint32_t seq = st.seq;
+		if constexpr (S) {
+			B3G_SETTLE("s"(tr.r0), "s"(tr.r1), "s"(tr.b0), "s"(tr.b1), "s"(tr.base), "s"(tr.pair), "s"(active),
+			           "s"(seq), "s"(logActive), "s"(persist));
+			if (!run) return;
+		} else {
+			B3G_SETTLE("s"(tr.r0), "s"(tr.r1), "s"(tr.b0), "s"(tr.b1), "s"(tr.base), "s"(tr.pair), "s"(seq));
+		}
 		uint32_t row;
 		V3 ax;
 		tile_rows<St>(a, tr, row, [&](uint32_t c) { return a.px[c]; }, lds, no_prologue, ax);
@@ -1915,15 +1927,15 @@ __global__ void __launch_bounds__(kBlock) k_cg_residual(CGArgs a, int mode, uint
 	};
 	if (!Multi) {   // instance 0 as a constant: its reads need no load of the tile's instance first
 		const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
-		if (a.single) tile(t, 0u);
-		else tile(t, a.tileInst[t]);
+		if (a.single) tile(t, 0u, std::true_type{});
+		else tile(t, a.tileInst[t], std::false_type{});
 		return;
 	}
 	auto ok = [&](uint32_t inst) {
 		return a.st[inst].newtonActive &&
 		       (mode != kFinDelta || (a.log[(size_t)inst * a.logStride + k].active && !persistent(a.persist, inst)));
 	};
-	for_items<Multi, true>(a.nTiles, a.tileInst, ok, [&](uint32_t t) { tile(t, a.tileInst[t]); });
+	for_items<Multi, true>(a.nTiles, a.tileInst, ok, [&](uint32_t t) { tile(t, a.tileInst[t], std::false_type{}); });
 }
 
 // PCG iteration k: q = A d with d = s + beta_k d_old formed on the fly
@@ -1935,27 +1947,41 @@ __global__ void __launch_bounds__(kBlock) k_cg_residual(CGArgs a, int mode, uint
 template <bool Multi, int St>
 __global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, uint32_t k, int sel, int force) {
 	__shared__ float lds[3 * kTileCap];
-	// The tile's bounds, the instance's gate and the scalars of the fused
-	// prologue are read together, before anything depends on them (every
-	// address is valid whether or not the tile runs): one round trip ahead of
-	// the tile's column codes instead of a chain of dependent ones.
-	auto tile = [&](uint32_t t, uint32_t inst) {
-		const TileRange tr = tile_range<St>(a, t);
+	// One body: the tile's bounds, the instance's gate and the scalars of the
+	// fused prologue are read together, before anything depends on them
+	// (every address is valid whether or not the tile runs), one round trip
+	// ahead of the tile's column codes instead of a chain of dependent ones.
+	// A batch reads the gate first, so a tile of a converged instance costs
+	// one round of four scalar loads; the rest is read after it.
+	auto tile = [&](uint32_t t, uint32_t inst, auto single) {
+		constexpr bool S = decltype(single)::value;
 		SolverState& st = a.st[inst];
 		CGLog* log = a.log + (size_t)inst * a.logStride;
-		const uint32_t* R = a.range + 6 * (size_t)inst;
 		const uint32_t km = k > 0 ? k - 1 : 0u;
-		const uint32_t active = st.newtonActive, seq = st.seq, maxIt = st.maxIt;
+		const uint32_t active = st.newtonActive, prevActive = log[km].active, curActive = log[k].active;
+		const uint32_t persist = a.persist ? a.persist[inst] : 0u;
+		// log[k].active is this launch's own output in fused mode, and read only otherwise
+		const bool run = force || (active && !persist && prevActive && (k == 0 || a.fused || curActive));
+		if constexpr (!S) {
+			B3G_SETTLE("s"(active), "s"(prevActive), "s"(curActive), "s"(persist));
+			if (!run) return;
+		}
+		const TileRange tr = tile_range<St>(a, t);
+		const uint32_t* R = a.range + 6 * (size_t)inst;
+		const uint32_t seq = st.seq, maxIt = st.maxIt;
 		const float eps2 = st.eps2_sub, delta0 = st.delta0;
-		const uint32_t prevActive = log[km].active, curActive = log[k].active;
 		const float prevDelta = log[km].delta, logBeta = log[k].beta;
 		const uint32_t p0 = R[sel], p1 = R[sel + 1], firstTile = R[0];
-		const uint32_t persist = a.persist ? a.persist[inst] : 0u;
-		B3G_SETTLE("s"(tr.r0), "s"(tr.r1), "s"(tr.b0), "s"(tr.b1), "s"(tr.base), "s"(tr.pair), "s"(active), "s"(seq),
-		           "s"(maxIt), "s"(eps2), "s"(delta0), "s"(prevActive), "s"(curActive), "s"(prevDelta), "s"(logBeta),
-		           "s"(p0), "s"(p1), "s"(firstTile), "s"(persist));
-		// log[k].active is this launch's own output in fused mode, and read only otherwise
-		if (!force && !(active && !persist && prevActive && (k == 0 || a.fused || curActive))) return;
+		if constexpr (S) {
+			B3G_SETTLE("s"(tr.r0), "s"(tr.r1), "s"(tr.b0), "s"(tr.b1), "s"(tr.base), "s"(tr.pair), "s"(active),
+			           "s"(seq), "s"(maxIt), "s"(eps2), "s"(delta0), "s"(prevActive), "s"(curActive), "s"(prevDelta),
+			           "s"(logBeta), "s"(p0), "s"(p1), "s"(firstTile), "s"(persist));
+			if (!run) return;
+		} else {
+			B3G_SETTLE("s"(tr.r0), "s"(tr.r1), "s"(tr.b0), "s"(tr.b1), "s"(tr.base), "s"(tr.pair), "s"(seq),
+			           "s"(maxIt), "s"(eps2), "s"(delta0), "s"(prevDelta), "s"(logBeta), "s"(p0), "s"(p1),
+			           "s"(firstTile));
+		}
 		float beta = 0.0f;
 		if (k > 0 && !a.fused) beta = logBeta;
 		// fused mode: delta_k / beta_k from the partials
@@ -1992,8 +2018,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, uint32_t k, int
 	};
 	if (!Multi) {   // instance 0 as a constant: its reads need no load of the tile's instance first
 		const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
-		if (a.single) tile(t, 0u);
-		else tile(t, a.tileInst[t]);
+		if (a.single) tile(t, 0u, std::true_type{});
+		else tile(t, a.tileInst[t], std::false_type{});
 		return;
 	}
 	auto ok = [&](uint32_t inst) -> bool {
@@ -2003,7 +2029,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, uint32_t k, int
 		if (k > 0) return log[k - 1].active && (a.fused || log[k].active);
 		return log[0].active != 0;
 	};
-	for_items<Multi, true>(a.nTiles, a.tileInst, ok, [&](uint32_t t) { tile(t, a.tileInst[t]); });
+	for_items<Multi, true>(a.nTiles, a.tileInst, ok, [&](uint32_t t) { tile(t, a.tileInst[t], std::false_type{}); });
 }
 
 // alpha_k of a vector kernel of iteration k: fused mode sums the SpMV's d.q
@@ -2023,45 +2049,52 @@ __device__ __forceinline__ float cg_alpha(const CGArgs& a, uint32_t inst, uint32
 // depend only on the block size); fewer rows per thread hold fewer registers.
 template <bool Multi, int RPT>
 __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k, int force) {
-	// The block's bounds, the instance's gate and the scalars of alpha are
-	// read together before anything depends on them (every address is valid
-	// whether or not the block runs). With one instance the rows are loaded
-	// before the gate as well (in bounds whatever it says); in a batch only
-	// after it, so converged instances cost no row traffic.
-	auto block = [&](uint32_t blk, uint32_t inst) {
-		const uint32_t r0 = a.rowBlk[blk], r1 = a.rowBlk[blk + 1];
+	// One body: the block's bounds, the instance's gate and the scalars of
+	// alpha are read together before anything depends on them (every address
+	// is valid whether or not the block runs), then alpha's partials and the
+	// rows, all before the gate (in bounds whatever it says). A batch reads
+	// the gate first, so a block of a converged instance costs one round of
+	// three scalar loads and no row traffic.
+	auto block = [&](uint32_t blk, uint32_t inst, auto single) {
+		constexpr bool S = decltype(single)::value;
 		const SolverState& st = a.st[inst];
 		CGLog* log = a.log + (size_t)inst * a.logStride;
+		const uint32_t active = st.newtonActive, logActive = log[k].active;
+		const uint32_t persist = a.persist ? a.persist[inst] : 0u;
+		const bool run = force || (active && logActive && !persist);
+		if constexpr (!S) {
+			B3G_SETTLE("s"(active), "s"(logActive), "s"(persist));
+			if (!run) return;
+		}
+		const uint32_t r0 = a.rowBlk[blk], r1 = a.rowBlk[blk + 1];
 		const uint32_t* R = a.range + 6 * (size_t)inst;
-		const uint32_t active = st.newtonActive, seq = st.seq, logActive = log[k].active;
+		const uint32_t seq = st.seq;
 		const float delta = log[k].delta, logAlpha = log[k].alpha;
 		const uint32_t q0 = R[0], q1 = R[1], firstBlk = R[2];
-		const uint32_t persist = a.persist ? a.persist[inst] : 0u;
-		B3G_SETTLE("s"(r0), "s"(r1), "s"(active), "s"(seq), "s"(logActive), "s"(delta), "s"(logAlpha), "s"(q0), "s"(q1),
-		           "s"(firstBlk), "s"(persist));
-		// every input of the thread's rows is loaded before the first store (the
-		// stores could alias them as far as the compiler knows), and before the
-		// alpha prologue, whose partial sums then overlap these loads; a row past
-		// the block loads row r0 and stores nothing
+		if constexpr (S)
+			B3G_SETTLE("s"(r0), "s"(r1), "s"(active), "s"(seq), "s"(logActive), "s"(delta), "s"(logAlpha), "s"(q0),
+			           "s"(q1), "s"(firstBlk), "s"(persist));
+		else
+			B3G_SETTLE("s"(r0), "s"(r1), "s"(seq), "s"(delta), "s"(logAlpha), "s"(q0), "s"(q1), "s"(firstBlk));
 		// the SpMV's d.q partials for alpha, requested first: their sum then
-		// runs while the rows are still in flight
+		// runs while the rows are still in flight. Every input of the thread's
+		// rows is loaded before the first store (the stores could alias them
+		// as far as the compiler knows); a row past the block loads row r0 and
+		// stores nothing.
 		double pq[16];
 		wg_sum_load(a.partialQ, q0, q1, pq);
 		V3 d[RPT], x[RPT], r[RPT], q[RPT], im[RPT];
-		auto load = [&]() {
 #pragma unroll
-			for (int u = 0; u < RPT; ++u) {
-				const uint32_t i = r0 + threadIdx.x + u * kBlock, ii = i < r1 ? i : r0;
-				d[u] = a.dnew[ii];
-				x[u] = a.px[ii];
-				r[u] = a.r[ii];
-				q[u] = a.q[ii];
-				im[u] = a.invM[ii];
-			}
-		};
-		if (a.single) load();
-		if (!force && !(active && logActive && !persist)) return;
-		if (!a.single) load();
+		for (int u = 0; u < RPT; ++u) {
+			const uint32_t i = r0 + threadIdx.x + u * kBlock, ii = i < r1 ? i : r0;
+			d[u] = a.dnew[ii];
+			x[u] = a.px[ii];
+			r[u] = a.r[ii];
+			q[u] = a.q[ii];
+			im[u] = a.invM[ii];
+		}
+		if constexpr (S)
+			if (!run) return;
 		// alpha_k (Alpha(k)): fused mode sums the SpMV's d.q partials of the
 		// instance, and the instance's first block records it
 		float alpha = logAlpha;
@@ -2087,15 +2120,16 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k, int 
 	};
 	if (!Multi) {   // instance 0 as a constant: its reads need no load of the block's instance first
 		const uint32_t blk = blockIdx.x;
-		if (a.single) block(blk, 0u);
-		else block(blk, a.blkInst[blk]);
+		if (a.single) block(blk, 0u, std::true_type{});
+		else block(blk, a.blkInst[blk], std::false_type{});
 		return;
 	}
 	auto ok = [&](uint32_t inst) {
 		return force || (a.st[inst].newtonActive && a.log[(size_t)inst * a.logStride + k].active &&
 		                 !persistent(a.persist, inst));
 	};
-	for_items<Multi, false>(a.nRowBlk, a.blkInst, ok, [&](uint32_t blk) { block(blk, a.blkInst[blk]); });
+	for_items<Multi, false>(a.nRowBlk, a.blkInst, ok,
+	                        [&](uint32_t blk) { block(blk, a.blkInst[blk], std::false_type{}); });
 }
 
 // x += alpha d (iterations that refresh the true residual)
