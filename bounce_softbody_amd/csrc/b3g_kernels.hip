@@ -1249,6 +1249,7 @@ struct AsmArgs {
 	// stretch / shear records (UVTable::rec)
 	const float4* stRec;
 	const float4* shRec;
+	const float4* uvFirst;       // a stretch or shear record that exists (stRec or shRec), for loads that are discarded
 	uint32_t anyJv;
 	uint32_t single;             // one instance: every tile is instance 0
 	const uint8_t* type;
@@ -1375,7 +1376,28 @@ __device__ __forceinline__ void tile_accumulate(const AsmArgs& a, const Contrib&
 #pragma unroll
 		for (int u = 0; u < kAsmPer; ++u) {
 			const uint32_t c = code[u];
-			if (is_tet<Kinds>(c)) {
+			if constexpr (Kinds == (kKindTet | kKindUV)) {
+				// both kinds' loads for every contribution, the other kind's from its
+				// first record (in bounds, cache-hot), and a select: no branch
+				// around the loads when tetrahedra and cloth records mix in a tile
+				const bool tet = (c & kTetCode) != 0;
+				const uint32_t et = tet ? c & (kTetMaxElements - 1) : 0u;
+				const uint32_t blk = tet ? (c & ~kTetCode) >> kTetBlockShift : 0u;
+				const float* trec = a.tetRec + (size_t)kTetRecFloats * et;
+				const V3 gi = tet_grad(trec, (int)(blk & 3u)), gj = tet_grad(trec, (int)(blk >> 2));
+				const float4 s4 = *reinterpret_cast<const float4*>(trec + kTetRecV);
+				const float kdv = cb_.kd ? cb_.kd[et] : 0.0f;
+				const float4 q4 = a.tetQ[et];
+				const uint32_t variant = (c & ~kUVCode) >> kUVShift, e = c & (kUVMaxElements - 1);
+				const bool shear = variant >= kShearVariant;
+				const float4* urec = tet ? a.uvFirst
+				                         : shear ? a.shRec + 4 * (size_t)e : a.stRec + (size_t)(2 * e + variant / 9) * 3;
+				const float4 u0 = urec[0], u1 = urec[1], u2 = urec[2], u3 = urec[shear && !tet ? 3 : 2];
+				r[u][0] = tet ? make_float4(gi.x, gi.y, gi.z, s4.x) : u0;
+				r[u][1] = tet ? make_float4(gj.x, gj.y, gj.z, s4.y) : u1;
+				r[u][2] = tet ? make_float4(s4.z, s4.w, kdv, 0.0f) : u2;
+				r[u][3] = tet ? q4 : u3;
+			} else if (is_tet<Kinds>(c)) {
 				const uint32_t e = c & (kTetMaxElements - 1), blk = (c & ~kTetCode) >> kTetBlockShift;
 				const float* rec = a.tetRec + (size_t)kTetRecFloats * e;
 				const V3 gi = tet_grad(rec, (int)(blk & 3u)), gj = tet_grad(rec, (int)(blk >> 2));
@@ -2998,6 +3020,7 @@ static b3g_status enqueue_linearise(Body& b, float h, bool first) {
 	a.tetRec = dptr<float>(b.tet.rec);
 	a.tetM = b.tet.m; a.tetQ = dptr<float4>(b.tet.q); a.tetKd = dptr<float>(b.tet.kd); a.anyJv = b.anyJv ? 1u : 0u;
 	a.stRec = dptr<float4>(b.stretch.rec); a.shRec = dptr<float4>(b.shear.rec);
+	a.uvFirst = b.stretch.m ? a.stRec : a.shRec;
 	a.slotRow = dptr<uint32_t>(b.slotRow); a.cols = dptr<uint32_t>(b.cols);
 	a.jxPtr = dptr<uint32_t>(b.jxPtr); a.jxIdx = dptr<uint32_t>(b.jxIdx);
 	a.jvPtr = dptr<uint32_t>(b.jvPtr); a.jvIdx = dptr<uint32_t>(b.jvIdx);
