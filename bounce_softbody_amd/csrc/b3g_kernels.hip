@@ -658,9 +658,29 @@ __global__ void __launch_bounds__(kBlock) k_tet_forces64(TetArgs a, const double
 	if (!element_gate(a.st, a.rowInst, a.p, e)) return;
 	const uint32_t m = a.m;
 	uint32_t id[4];
+	// Only F and the warm start live through the polar iterations; the
+	// positions, rest data and inverse rest shape are read again after them
+	// (as k_tet_forces does), which keeps the loop's registers down.
+	DM33 F;
+	{
+		D3 p[4];
+		for (int k = 0; k < 4; ++k) {
+			id[k] = a.p[(size_t)k * m + e];
+			p[k] = d3(a.x[id[k]]);
+		}
+		DM33 iE;
+		double* pi = &iE.x.x;
+		for (int c = 0; c < 9; ++c) pi[c] = tet64[(size_t)c * m + e];
+		F = dm33(p[1] - p[0], p[2] - p[0], p[3] - p[0]) * iE;
+	}
+	const float4 q4 = a.q[e];
+	DQ4 q;
+	q.v = d3(q4.x, q4.y, q4.z);
+	q.s = q4.w;
+	q = extract_rotation64(F, q);
+	a.q[e] = make_float4((float)q.v.x, (float)q.v.y, (float)q.v.z, (float)q.s);
 	D3 p[4], xr[4];
 	for (int k = 0; k < 4; ++k) {
-		id[k] = a.p[(size_t)k * m + e];
 		p[k] = d3(a.x[id[k]]);
 		xr[k] = d3(a.rest[(size_t)(3 * k) * m + e], a.rest[(size_t)(3 * k + 1) * m + e], a.rest[(size_t)(3 * k + 2) * m + e]);
 	}
@@ -668,13 +688,6 @@ __global__ void __launch_bounds__(kBlock) k_tet_forces64(TetArgs a, const double
 	double* pi = &iE.x.x;
 	for (int c = 0; c < 9; ++c) pi[c] = tet64[(size_t)c * m + e];
 	const double V = tet64[(size_t)9 * m + e], lambda = tet64[(size_t)10 * m + e], mu = tet64[(size_t)11 * m + e];
-	DM33 F = dm33(p[1] - p[0], p[2] - p[0], p[3] - p[0]) * iE;
-	const float4 q4 = a.q[e];
-	DQ4 q;
-	q.v = d3(q4.x, q4.y, q4.z);
-	q.s = q4.w;
-	q = extract_rotation64(F, q);
-	a.q[e] = make_float4((float)q.v.x, (float)q.v.y, (float)q.v.z, (float)q.s);
 	DM33 R = qmatrix(q);
 	DM33 RT = transpose(R);
 	// shape-function gradients (rows of invE; vertex 0 = minus their sum)
