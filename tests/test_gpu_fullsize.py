@@ -110,3 +110,22 @@ def test_full_size_precision_forms_reach_fp64():
     print(errs)
     assert errs["refined_fp32"] <= 1e-5 and errs["refined_bf16"] <= 1e-5, errs
     assert errs["plain"] > 1e-5, errs
+
+
+def test_two_pass_partial_sums_equal_the_separate_form():
+    """A body with more than 4096 SpMV tiles (72^3 cells: 4.96 M blocks, at most 1024 per tile; still summed in
+    the consuming kernels by default): the vector update's fixed-order sum of the SpMV's partials takes a second
+    pass of 16 x 256 after the first one, which it requests before its rows. The fused form equals the
+    separate-kernel form bit for bit, step after step."""
+    arr = configs.tet_grid(72).arrays()
+    sep, fused = Body(arr), Body(arr)
+    sep.set_reduction(_lib.REDUCE_FP64_SEPARATE)
+    fused.set_reduction(_lib.REDUCE_FP64_FUSED)
+    for step in range(3):
+        sep.Step(configs.DT, 1, SUB, 0.0, SUB_TOL)
+        fused.Step(configs.DT, 1, SUB, 0.0, SUB_TOL)
+        xa, va = sep.state()
+        xb, vb = fused.state()
+        assert np.array_equal(xa, xb) and np.array_equal(va, vb), step
+    assert fused.system_size()[1] > 4096 * 1024
+    assert [s[0].tolist() for s in sep.instance_stats()] == [s[0].tolist() for s in fused.instance_stats()]
