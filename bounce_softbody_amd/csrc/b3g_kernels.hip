@@ -201,6 +201,7 @@ struct Reduce {
 	CGLog* log;              // per instance, logStride entries
 	uint32_t logStride;
 	const uint32_t* persist; // instances whose iterations the persistent kernel reduces itself, or null
+	uint32_t* tileGate;      // batches: per-tile "instance done with this solve" words, reset at a solve's start
 };
 
 // The workgroup's FP64 partial (each thread's `part` added in the block
@@ -328,8 +329,13 @@ __global__ void __launch_bounds__(kFinBlock) k_finalize(Reduce rd, int sel, int 
 	if (!st->newtonActive) return;
 	if ((kind == kFinAlpha || kind == kFinDelta) && persistent(rd.persist, inst)) return;
 	if (gate == kGateIter && !log[gk].active) return;
-	if (kind == kFinInit) clear_log(log, rd.logStride);
 	const uint32_t* R = rd.range + 6 * (size_t)inst;
+	if (kind == kFinInit) {
+		clear_log(log, rd.logStride);
+		// the instance's tiles run again until the SpMV finds it done
+		if (rd.tileGate)
+			for (uint32_t t = R[0] + threadIdx.x; t < R[1]; t += blockDim.x) rd.tileGate[t] = 1u;
+	}
 	__shared__ float chunk[4 * kFinBlock];
 	if (st->seq && kind != kFinRefine0 && kind != kFinRefine) {
 		const uint32_t r0 = R[4], r1 = R[5];
@@ -1720,6 +1726,7 @@ struct CGArgs {
 	const uint32_t* rowBlk;
 	const uint32_t* blkInst;
 	uint32_t single;            // one instance: every tile and row block is instance 0
+	uint32_t* tileGate;         // batches: 0 once the tile's instance is done with the current solve, or null
 	const uint32_t* rowptr;
 	const uint32_t* cols;
 	const float* A;
@@ -1997,9 +2004,17 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, uint32_t k, int
 		const uint32_t persist = a.persist ? a.persist[inst] : 0u;
 		// log[k].active is this launch's own output in fused mode, and read only otherwise
 		const bool run = force || (active && !persist && prevActive && (k == 0 || a.fused || curActive));
+		// a batch's tile of an instance that is done with this solve (converged,
+		// out of the Newton loop or persistent) says so for the later launches
+		auto done = [&]() {
+			if (!S && a.tileGate && !force && threadIdx.x == 0) a.tileGate[t] = 0u;
+		};
 		if constexpr (!S) {
 			B3G_SETTLE("s"(active), "s"(prevActive), "s"(curActive), "s"(persist));
-			if (!run) return;
+			if (!run) {
+				done();
+				return;
+			}
 		}
 		const TileRange tr = tile_range<St>(a, t);
 		const uint32_t* R = a.range + 6 * (size_t)inst;
@@ -2031,6 +2046,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, uint32_t k, int
 				log[k].active = act ? 1u : 0u;
 				st.it = k;
 			}
+			if (!act) done();
 			return force || act;
 		};
 		// d = s + beta d_old, and d = s at k = 0: there dsrc is s and beta is +0,
@@ -2053,8 +2069,16 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv_dq(CGArgs a, uint32_t k, int
 	};
 	if (!Multi) {   // instance 0 as a constant: its reads need no load of the tile's instance first
 		const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
-		if (a.single) tile(t, 0u, std::true_type{});
-		else tile(t, a.tileInst[t], std::false_type{});
+		if (a.single) {
+			tile(t, 0u, std::true_type{});
+		} else {
+			// the tile's gate word with its instance: a tile of an instance that
+			// is done costs one round of two loads
+			const uint32_t inst = a.tileInst[t], g = a.tileGate && !force ? a.tileGate[t] : 1u;
+			B3G_SETTLE("s"(inst), "s"(g));
+			if (!g) return;
+			tile(t, inst, std::false_type{});
+		}
 		return;
 	}
 	auto ok = [&](uint32_t inst) -> bool {
@@ -2739,6 +2763,8 @@ b3g_status build_tiles(Body& b, bool sorted, bool tiles) {
 		std::vector<uint32_t> tb(t.size());
 		for (size_t x = 0; x < t.size(); ++x) tb[x] = b.hostRowptr[t[x]];
 		if ((s = dupload(b, b.tileBlk, tb.data(), tb.size() * 4)) != B3G_OK) return s;
+		std::vector<uint32_t> gate(std::max(1u, b.nTiles), 1u);
+		if ((s = dupload(b, b.tileGate, gate.data(), gate.size() * 4)) != B3G_OK) return s;
 		// the BF16 pair planes: each tile's first pair
 		std::vector<uint32_t> pairs(std::max(1u, b.nTiles), 0u);
 		uint64_t total = 0;
@@ -2908,6 +2934,7 @@ static CGArgs cg_args(Body& b) {
 	a.tiles = dptr<uint32_t>(b.tiles);
 	a.tileInst = dptr<uint32_t>(b.tileInst);
 	a.tileBlk = dptr<uint32_t>(b.tileBlk);
+	a.tileGate = b.nInst > 1 ? dptr<uint32_t>(b.tileGate) : nullptr;
 	a.rowBlk = dptr<uint32_t>(b.rowBlk);
 	a.blkInst = dptr<uint32_t>(b.blkInst);
 	a.single = b.nInst <= 1 ? 1u : 0u;
@@ -3076,7 +3103,8 @@ static b3g_status enqueue_linearise(Body& b, float h, bool first) {
 
 static b3g_status enqueue_finalize(Body& b, const DBuf& partial, int sel, int kind, uint32_t k, int gate, uint32_t gk) {
 	Reduce rd{dptr<double>(partial), dptr<float>(b.terms), dptr<SolverState>(b.state), dptr<uint32_t>(b.instRange),
-	          dptr<CGLog>(b.cgLog), b.logStride, use_persistent(b) ? dptr<uint32_t>(b.persist) : nullptr};
+	          dptr<CGLog>(b.cgLog), b.logStride, use_persistent(b) ? dptr<uint32_t>(b.persist) : nullptr,
+	          b.nInst > 1 ? dptr<uint32_t>(b.tileGate) : nullptr};
 	hipLaunchKernelGGL(k_finalize, dim3(std::max(1u, b.nInst)), dim3(kFinBlock), 0, b.stream, rd, sel, kind, k, gate, gk);
 	B3G_HIP(hipGetLastError());
 	return B3G_OK;
