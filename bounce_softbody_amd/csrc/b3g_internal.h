@@ -226,6 +226,7 @@ struct Body {
 	DBuf tileInst;                                // uint32 instance per tile
 	DBuf tileBlk;                                 // uint32 block boundaries of the tiles (rowptr at tiles)
 	DBuf tileGate;                                // uint32 per tile: 0 once its instance is done with the current PCG solve
+	DBuf blkGate;                                 // uint32 per row block: the same for the vector update
 	DBuf rowBlk;                                  // uint32 boundaries of <= 256-row blocks (vector kernels)
 	DBuf blkInst;
 	uint32_t nRowBlk = 0;

@@ -202,6 +202,7 @@ struct Reduce {
 	uint32_t logStride;
 	const uint32_t* persist; // instances whose iterations the persistent kernel reduces itself, or null
 	uint32_t* tileGate;      // batches: per-tile "instance done with this solve" words, reset at a solve's start
+	uint32_t* blkGate;       // the same per row block of the vector update
 };
 
 // The workgroup's FP64 partial (each thread's `part` added in the block
@@ -335,6 +336,8 @@ __global__ void __launch_bounds__(kFinBlock) k_finalize(Reduce rd, int sel, int 
 		// the instance's tiles run again until the SpMV finds it done
 		if (rd.tileGate)
 			for (uint32_t t = R[0] + threadIdx.x; t < R[1]; t += blockDim.x) rd.tileGate[t] = 1u;
+		if (rd.blkGate)
+			for (uint32_t j = R[2] + threadIdx.x; j < R[3]; j += blockDim.x) rd.blkGate[j] = 1u;
 	}
 	__shared__ float chunk[4 * kFinBlock];
 	if (st->seq && kind != kFinRefine0 && kind != kFinRefine) {
@@ -1727,6 +1730,7 @@ struct CGArgs {
 	const uint32_t* blkInst;
 	uint32_t single;            // one instance: every tile and row block is instance 0
 	uint32_t* tileGate;         // batches: 0 once the tile's instance is done with the current solve, or null
+	uint32_t* blkGate;          // the same per row block
 	const uint32_t* rowptr;
 	const uint32_t* cols;
 	const float* A;
@@ -2123,7 +2127,11 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k, int 
 		const bool run = force || (active && logActive && !persist);
 		if constexpr (!S) {
 			B3G_SETTLE("s"(active), "s"(logActive), "s"(persist));
-			if (!run) return;
+			if (!run) {
+				// done with this solve (log[k] is final here): later launches skip the block
+				if (a.blkGate && threadIdx.x == 0) a.blkGate[blk] = 0u;
+				return;
+			}
 		}
 		const uint32_t r0 = a.rowBlk[blk], r1 = a.rowBlk[blk + 1];
 		const uint32_t* R = a.range + 6 * (size_t)inst;
@@ -2179,8 +2187,16 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(CGArgs a, uint32_t k, int 
 	};
 	if (!Multi) {   // instance 0 as a constant: its reads need no load of the block's instance first
 		const uint32_t blk = blockIdx.x;
-		if (a.single) block(blk, 0u, std::true_type{});
-		else block(blk, a.blkInst[blk], std::false_type{});
+		if (a.single) {
+			block(blk, 0u, std::true_type{});
+		} else {
+			// the block's gate word with its instance: a block of an instance
+			// that is done costs one round of two loads
+			const uint32_t inst = a.blkInst[blk], g = a.blkGate && !force ? a.blkGate[blk] : 1u;
+			B3G_SETTLE("s"(inst), "s"(g));
+			if (!g) return;
+			block(blk, inst, std::false_type{});
+		}
 		return;
 	}
 	auto ok = [&](uint32_t inst) {
@@ -2778,6 +2794,10 @@ b3g_status build_tiles(Body& b, bool sorted, bool tiles) {
 		if ((s = dupload(b, b.pairBase, pairs.data(), pairs.size() * 4)) != B3G_OK) return s;
 	}
 	if ((s = dupload(b, b.rowBlk, rb.data(), rb.size() * 4)) != B3G_OK) return s;
+	{
+		std::vector<uint32_t> gate(std::max<size_t>(1, rb.size()), 1u);
+		if ((s = dupload(b, b.blkGate, gate.data(), gate.size() * 4)) != B3G_OK) return s;
+	}
 	if ((s = dupload(b, b.blkInst, bi.data(), bi.size() * 4)) != B3G_OK) return s;
 	if ((s = dupload(b, b.instRange, range.data(), range.size() * 4)) != B3G_OK) return s;
 	// the persistent PCG's instances: at most sched.persistTiles tiles, largest first
@@ -2935,6 +2955,7 @@ static CGArgs cg_args(Body& b) {
 	a.tileInst = dptr<uint32_t>(b.tileInst);
 	a.tileBlk = dptr<uint32_t>(b.tileBlk);
 	a.tileGate = b.nInst > 1 ? dptr<uint32_t>(b.tileGate) : nullptr;
+	a.blkGate = b.nInst > 1 ? dptr<uint32_t>(b.blkGate) : nullptr;
 	a.rowBlk = dptr<uint32_t>(b.rowBlk);
 	a.blkInst = dptr<uint32_t>(b.blkInst);
 	a.single = b.nInst <= 1 ? 1u : 0u;
@@ -3104,7 +3125,7 @@ static b3g_status enqueue_linearise(Body& b, float h, bool first) {
 static b3g_status enqueue_finalize(Body& b, const DBuf& partial, int sel, int kind, uint32_t k, int gate, uint32_t gk) {
 	Reduce rd{dptr<double>(partial), dptr<float>(b.terms), dptr<SolverState>(b.state), dptr<uint32_t>(b.instRange),
 	          dptr<CGLog>(b.cgLog), b.logStride, use_persistent(b) ? dptr<uint32_t>(b.persist) : nullptr,
-	          b.nInst > 1 ? dptr<uint32_t>(b.tileGate) : nullptr};
+	          b.nInst > 1 ? dptr<uint32_t>(b.tileGate) : nullptr, b.nInst > 1 ? dptr<uint32_t>(b.blkGate) : nullptr};
 	hipLaunchKernelGGL(k_finalize, dim3(std::max(1u, b.nInst)), dim3(kFinBlock), 0, b.stream, rd, sel, kind, k, gate, gk);
 	B3G_HIP(hipGetLastError());
 	return B3G_OK;
